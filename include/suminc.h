@@ -1,0 +1,195 @@
+/*
+ * suminc.h -- C ABI of the MI355X (gfx950) implementation of SumInc's hot path:
+ * delta-accumulative PageRank and SSSP over a CSR fragment, optionally compressed
+ * into supernodes with entry/exit shortcuts, applied incrementally after an
+ * edge-update batch.
+ *
+ * The reference drives its GPU prototype through C++ functions in namespaces
+ * tjn / tjnsssp / tjnpr_precompute / tjnsssp_precompute (cudaCode/ headers (*.cuh)) that it
+ * calls from the workers after flattening host CSR arrays and cudaMalloc-ing
+ * buffers itself.  Each entry point below replaces one of those calls or one of
+ * the flattening/upload blocks; the comment names the reference call site.
+ *
+ * Conventions
+ *   - Every function returns int: SUMINC_OK (0) or a negative SUMINC_E_* code.
+ *     suminc_last_error() returns a message for the last failure on a context.
+ *     No entry point aborts the process.
+ *   - Host pointers are borrowed for the duration of the call only.  All device
+ *     memory is owned by the context and released by suminc_ctx_destroy().
+ *   - Vertex ids are dense uint32 in [0, n).  Offsets at the boundary are
+ *     uint64 (CSR row pointers, n+1 entries).
+ *   - SSSP distances are uint32 with SUMINC_INF_U32 (0xFFFFFFFF) for "unreached"
+ *     (the reference uses int32 with INT32_MAX, sssp/sssp_ingress.h:103-105).
+ *   - One context per host thread; calls on one context are serialised on the
+ *     context's own HIP stream.  Functions that return host data synchronise.
+ */
+#ifndef SUMINC_H_
+#define SUMINC_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SUMINC_OK 0
+#define SUMINC_E_INVALID (-1)      /* bad argument (null pointer, id out of range, ...) */
+#define SUMINC_E_HIP (-2)          /* a HIP runtime call failed */
+#define SUMINC_E_NOMEM (-3)        /* device allocation failed */
+#define SUMINC_E_STATE (-4)        /* call out of order (e.g. no graph uploaded) */
+#define SUMINC_E_TOO_LARGE (-5)    /* size beyond what the layout supports */
+#define SUMINC_E_NOT_CONVERGED (-6)/* round cap reached before the stop rule held */
+#define SUMINC_E_NUMERIC (-7)      /* NaN/Inf detected in the state */
+
+#define SUMINC_INF_U32 0xFFFFFFFFu
+
+/* precision of the PageRank state */
+#define SUMINC_PR_FP64 0        /* delta and rank in FP64 */
+#define SUMINC_PR_FP32_KAHAN 1  /* FP32 delta, FP32 rank with a Kahan compensation term */
+
+/* edge weight kinds carried by the graph */
+#define SUMINC_W_NONE 0
+#define SUMINC_W_U32 1
+#define SUMINC_W_F32 2
+
+/* update-batch op types: a line "a u v w" or "d u v" of the reference's update
+ * file (grape/fragment/inc_fragment_builder.h:38-72) */
+#define SUMINC_OP_ADD 0
+#define SUMINC_OP_DEL 1
+
+typedef struct suminc_update {
+  uint32_t type; /* SUMINC_OP_ADD / SUMINC_OP_DEL */
+  uint32_t u, v;
+  uint32_t w_u32; /* weight of an add when the graph has SUMINC_W_U32 weights */
+  float w_f32;    /* weight of an add when the graph has SUMINC_W_F32 weights */
+  uint32_t pad;
+} suminc_update;
+
+/* Run parameters; the reference's gflags (examples/analytical_apps/flags.cc)
+ * that reach the hot path. */
+typedef struct suminc_params {
+  double damping;             /* PageRankIngress::d; the reference uses 0.85f (pagerank_ingress.h:23) */
+  double termcheck_threshold; /* FLAGS_termcheck_threshold T: stop when sum|delta| < T, a vertex
+                                 is active when |delta| > T / N (sum_sync_iter_worker.h:2873-2925) */
+  int32_t max_rounds;         /* FLAGS_pr_mr cap on rounds per phase */
+  int32_t pr_precision;       /* SUMINC_PR_FP64 or SUMINC_PR_FP32_KAHAN */
+  uint32_t sssp_source;       /* FLAGS_sssp_source */
+  int32_t pull_threshold_div; /* push->pull switch: pull when active edges > m / div (0: never pull) */
+  int32_t reserved[8];
+} suminc_params;
+
+typedef struct suminc_stats {
+  double rounds;              /* rounds executed by the last run/step call sequence */
+  double edges_traversed;     /* sum over rounds of adjacency lengths processed (the
+                                 reference's f_send_delta_num, pagerank_ingress.h:59-61) */
+  double residual;            /* FP64 sum|delta| at stop (PageRank) */
+  double kernel_ms;           /* device time of the last run, from HIP events */
+  double push_rounds, pull_rounds;
+  double reserved[10];
+} suminc_stats;
+
+typedef struct suminc_ctx suminc_ctx;
+
+/* ---- context ---------------------------------------------------------------
+ * Replaces the worker's cudaSetDevice/cudaMalloc block and tjn::init's
+ * __device__-global aliasing (sum_sync_iter_worker.h:661-691, 778;
+ * cudaCode/my_worker.cu:84-122). */
+void suminc_default_params(suminc_params* p);
+int suminc_ctx_create(int device, const suminc_params* params, suminc_ctx** out);
+int suminc_ctx_destroy(suminc_ctx* ctx);
+int suminc_set_params(suminc_ctx* ctx, const suminc_params* params);
+const char* suminc_last_error(const suminc_ctx* ctx);
+int suminc_sync(suminc_ctx* ctx);
+/* hipStream_t the context launches on, as an opaque pointer (for event timing). */
+int suminc_get_stream(suminc_ctx* ctx, void** stream);
+
+/* ---- graph (CSR fragment in HBM) ---------------------------------------------
+ * Replaces the flattening of getOeoffset() into oeoffset_h/cur_oeoff_h/size_oe_h
+ * and their upload (sum_sync_iter_worker.h:575-597, 753-776); the device keeps
+ * out-CSR and in-CSR (CSC) sorted by (src, dst).  weights: NULL or m values of
+ * the kind given. */
+int suminc_upload_csr(suminc_ctx* ctx, uint32_t n, uint64_t m, const uint64_t* offsets,
+                      const uint32_t* cols, int weight_kind, const void* weights);
+/* Seeded R-MAT (a,b,c; d = 1-a-b-c) built directly on the device: edge_factor<<scale
+ * draws, self-loops removed, duplicates merged.  No reference counterpart (the
+ * reference loads edge files, grape/fragment/loader.h:30-61). */
+int suminc_generate_rmat(suminc_ctx* ctx, int scale, int edge_factor, double a, double b,
+                         double c, uint64_t seed, int weight_kind, uint32_t wmin, uint32_t wmax);
+int suminc_graph_info(suminc_ctx* ctx, uint32_t* n, uint64_t* m, int* weight_kind);
+int suminc_download_csr(suminc_ctx* ctx, uint64_t* offsets, uint32_t* cols, void* weights);
+
+/* ---- update batches ------------------------------------------------------------
+ * Seeded batch: n_del deletes of existing edges, then n_ins inserts with uniform
+ * endpoints and fresh weights (synthetic input; the reference generates update
+ * files offline with expr2/sh/gen_inc2.py). */
+int suminc_generate_batch(suminc_ctx* ctx, uint64_t seed, uint32_t batch_id, uint64_t n_del,
+                          uint64_t n_ins, uint32_t wmin, uint32_t wmax, suminc_update* ops_out);
+/* Apply a batch to the device CSR/CSC with IncFragmentBuilder semantics
+ * (inc_fragment_builder.h:29-133: first add of a pair wins, add beats delete, add of
+ * an existing pair replaces its weight).  Replaces reloadGraph()'s host rebuild and
+ * re-upload (sum_sync_iter_worker.h:214-345, 1527-1678).  Before the topology
+ * changes, the PageRank state (if initialised) is reseeded with AmendValue(-1) on
+ * the old out-edges and AmendValue(+1) on the new out-edges of every source named
+ * by an op (the active-vertex form the reference left commented out at :285-290,
+ * :338-342; equal in exact arithmetic to its full sweeps), and the SSSP state (if
+ * initialised) is repaired: every vertex whose parent edge was deleted or re-weighted
+ * is reset with its dependency subtree (sum_sync_traversal_worker.h:146-261), reset
+ * vertices re-pull from surviving in-neighbours and every insert is relaxed. */
+int suminc_apply_batch(suminc_ctx* ctx, const suminc_update* ops, uint64_t n_ops);
+
+/* ---- PageRank (plain Ingress path) --------------------------------------------
+ * init: delta = (1-d)/N, rank = 0 (PageRankIngress::init_c/init_v,
+ * pagerank_ingress.h:27-32; IterateKernel::Init, ingress_app_base.h:141-228). */
+int suminc_pr_init(suminc_ctx* ctx);
+/* Runs delta-PageRank rounds until sum|delta| < T or max_rounds: replaces
+ * tjn::g_function_pr + tjn::deltaSum (sum_sync_iter_worker.h:885-893, 1372;
+ * cudaCode/my_worker.cu:124-142, 304-328).  Returns SUMINC_E_NOT_CONVERGED if the
+ * cap is hit (state stays valid). */
+int suminc_pr_run(suminc_ctx* ctx, suminc_stats* stats);
+/* One round; residual_out = FP64 sum|delta| before the round. */
+int suminc_pr_step(suminc_ctx* ctx, double* residual_out);
+/* Copies rank (rank - compensation for FP32/Kahan) and delta to host as FP64. */
+int suminc_pr_download(suminc_ctx* ctx, double* rank, double* delta);
+/* Sets the PageRank state from host FP64 arrays (n each). */
+int suminc_pr_upload(suminc_ctx* ctx, const double* rank, const double* delta);
+
+/* ---- SSSP (plain Ingress path) ------------------------------------------------
+ * init: dist = inf, parent = self; source: dist 0 (SSSPIngress::GetInitValue /
+ * GetInitDelta, sssp/sssp_ingress.h:25-41). */
+int suminc_sssp_init(suminc_ctx* ctx, uint32_t source);
+/* Min-plus frontier rounds to a fixed point: replaces tjnsssp::g_function +
+ * setCurSize/getAllSize (sum_sync_traversal_worker.h:1090, 1136-1142;
+ * cudaCode/my_ssspworker.cu:117-495) with weighted relaxation (the reference GPU
+ * path uses unit weights, my_ssspworker.cu:390). */
+int suminc_sssp_run(suminc_ctx* ctx, suminc_stats* stats);
+int suminc_sssp_download(suminc_ctx* ctx, uint32_t* dist, uint32_t* parent);
+
+/* ---- stats ---------------------------------------------------------------------*/
+int suminc_get_stats(suminc_ctx* ctx, suminc_stats* stats);
+
+/* ---- kernel profile -------------------------------------------------------------
+ * Per-kernel device time from HIP events recorded on the context stream around
+ * every launch of the named kernel that did work, plus the algorithmic bytes
+ * those launches moved.  No reference counterpart (the reference times phases
+ * with host timers, examples/analytical_apps/timer.h:43-80). */
+#define SUMINC_KERNEL_PR_PUSH 0   /* PageRank scatter: 12 B/edge + 16 B/vertex */
+#define SUMINC_KERNEL_PR_GATHER 1 /* PageRank vertex pass: 4 B/vertex + 16 B/active vertex */
+#define SUMINC_KERNEL_SSSP_PUSH 2 /* SSSP relax: 16 B/edge + 16 B/vertex */
+#define SUMINC_KERNEL_COUNT 8
+typedef struct suminc_kernel_profile {
+  double launches;  /* timed launches */
+  double ms;        /* summed device time */
+  double alg_bytes; /* algorithmic bytes of those launches */
+  double edges;     /* edges processed by those launches */
+  double vertices;  /* vertices processed by those launches */
+  double reserved[3];
+} suminc_kernel_profile;
+/* on != 0 enables event timing and clears the accumulators. */
+int suminc_profile_enable(suminc_ctx* ctx, int on);
+int suminc_profile_get(suminc_ctx* ctx, int kernel, suminc_kernel_profile* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SUMINC_H_ */

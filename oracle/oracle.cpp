@@ -1,0 +1,490 @@
+// oracle.cpp -- CPU restatement of the reference's hot path.
+//
+// THIS IS TEST INFRASTRUCTURE, NOT PRODUCT CODE.  Only tests/, the smoke check
+// in __graft_entry__.py and the cpu_baseline leg of bench.py load this library,
+// and only as the checker / the timed CPU baseline.  The product library
+// (suminc_amd/csrc) never links, loads or calls anything in this directory.
+//
+// Every function restates an algorithm of tang0113/SumInc (paths relative to the
+// reference root) in plain C++17 + OpenMP with a C ABI for ctypes.  Arithmetic is
+// FP64 unless a function says otherwise.  Pinning (what each function is checked
+// against, beyond "the GPU agrees with it") is listed in the header comment of
+// each function and in tests/test_oracle.py:
+//   * orc_philox4x32_10  -- Random123 known-answer vectors.
+//   * orc_rmat_*         -- brute-force recount of a tiny graph; dedupe/self-loop
+//                           invariants; quadrant frequencies vs a,b,c,d.
+//   * orc_apply_batch    -- brute-force set semantics on tiny inputs.
+//   * orc_pr_ingress     -- scipy sparse direct solve of (I - d P^T) r = (1-d)/N
+//                           (closed form), mass invariant, error bound.
+//   * orc_pr_amend       -- mass invariant, reseeded fixed point == scratch solve.
+//   * orc_sssp_dijkstra  -- scipy.sparse.csgraph.dijkstra, edge invariants.
+//   * orc_pr_sum         -- equals the closed form (same fixed point), shortcut
+//                           rows equal a dense per-cluster solve.
+//   * orc_wpr_*          -- dense numpy solve + central finite differences.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <queue>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11).  Multipliers and Weyl key increments from that paper.
+// ---------------------------------------------------------------------------
+inline void philox(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+  uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+  uint32_t k0 = key_in[0], k1 = key_in[1];
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c1 ^ k0;
+    uint32_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Stream ids of the generator (the product's device generator uses the same
+// numbers; they are part of the synthetic-input definition, not of SumInc).
+constexpr uint32_t STREAM_RMAT = 0x524D4154u;   // "RMAT"
+constexpr uint32_t STREAM_WEIGHT = 0x57474854u; // "WGHT"
+constexpr uint32_t STREAM_DEL = 0x44454C45u;    // "DELE"
+constexpr uint32_t STREAM_INS = 0x494E5345u;    // "INSE"
+
+inline void seed_key(uint64_t seed, uint32_t key[2]) {
+  key[0] = (uint32_t)seed;
+  key[1] = (uint32_t)(seed >> 32);
+}
+
+inline uint32_t mulhi32(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
+
+struct Graph {
+  uint32_t n = 0;
+  uint64_t m = 0;
+  std::vector<uint64_t> off;   // n+1
+  std::vector<uint32_t> col;   // m
+  std::vector<uint32_t> wu;    // m or empty
+  std::vector<float> wf;       // m or empty
+  uint32_t deg(uint32_t u) const { return (uint32_t)(off[u + 1] - off[u]); }
+};
+
+// Weight of edge (u,v): a pure function of (seed, u, v) so that it does not
+// depend on duplicate order.  kind 1: u32 uniform in [wmin,wmax]; kind 2: f32 in
+// (0,1] with 24 random bits.
+inline void edge_weight(uint64_t seed, uint32_t u, uint32_t v, int kind, uint32_t wmin,
+                        uint32_t wmax, uint32_t* wu, float* wf) {
+  uint32_t key[2], ctr[4] = {u, v, 0u, STREAM_WEIGHT}, o[4];
+  seed_key(seed, key);
+  philox(ctr, key, o);
+  if (kind == 1) *wu = wmin + mulhi32(o[0], wmax - wmin + 1u);
+  if (kind == 2) *wf = (float)((o[0] >> 8) + 1u) * (1.0f / 16777216.0f);
+}
+
+void build_csr_from_sorted_keys(Graph& g, uint32_t n, const std::vector<uint64_t>& keys) {
+  g.n = n;
+  g.m = keys.size();
+  g.off.assign((size_t)n + 1, 0);
+  g.col.resize(g.m);
+  for (uint64_t i = 0; i < g.m; ++i) {
+    g.off[(keys[i] >> 32) + 1]++;
+    g.col[i] = (uint32_t)keys[i];
+  }
+  for (uint32_t i = 0; i < n; ++i) g.off[i + 1] += g.off[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+struct orc_update {  // one line of an update file: "a u v w" (type 0) or "d u v" (type 1)
+  uint32_t type, u, v, w_u32;
+  float w_f32;
+  uint32_t pad;
+};
+
+void orc_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  philox(ctr, key, out);
+}
+
+int orc_num_threads(void) { return omp_get_max_threads(); }
+void orc_set_threads(int t) { if (t > 0) omp_set_num_threads(t); }
+
+// ---------------------------------------------------------------------------
+// Graph handle
+// ---------------------------------------------------------------------------
+void* orc_graph_from_csr(uint32_t n, uint64_t m, const uint64_t* off, const uint32_t* col,
+                         const uint32_t* wu, const float* wf) {
+  Graph* g = new Graph();
+  g->n = n;
+  g->m = m;
+  g->off.assign(off, off + n + 1);
+  g->col.assign(col, col + m);
+  if (wu) g->wu.assign(wu, wu + m);
+  if (wf) g->wf.assign(wf, wf + m);
+  return g;
+}
+
+void orc_graph_info(void* h, uint32_t* n, uint64_t* m, int* has_wu, int* has_wf) {
+  Graph* g = (Graph*)h;
+  *n = g->n;
+  *m = g->m;
+  *has_wu = !g->wu.empty();
+  *has_wf = !g->wf.empty();
+}
+
+void orc_graph_copy(void* h, uint64_t* off, uint32_t* col, uint32_t* wu, float* wf) {
+  Graph* g = (Graph*)h;
+  if (off) std::copy(g->off.begin(), g->off.end(), off);
+  if (col) std::copy(g->col.begin(), g->col.end(), col);
+  if (wu && !g->wu.empty()) std::copy(g->wu.begin(), g->wu.end(), wu);
+  if (wf && !g->wf.empty()) std::copy(g->wf.begin(), g->wf.end(), wf);
+}
+
+void orc_graph_free(void* h) { delete (Graph*)h; }
+
+// ---------------------------------------------------------------------------
+// R-MAT (Chakrabarti, Zhan, Faloutsos, SDM'04) without noise or permutation:
+// edge i descends `scale` levels; at level l one Philox word (ctr = {i_lo, i_hi,
+// l/4, STREAM_RMAT}, word l%4) picks quadrant a|b|c|d by comparing against
+// floor(a*2^32), floor((a+b)*2^32), floor((a+b+c)*2^32).  Source bit first is the
+// MSB.  Self-loops dropped, duplicates merged, CSR sorted by (u, v).
+// ---------------------------------------------------------------------------
+void* orc_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+               int weight_kind, uint32_t wmin, uint32_t wmax) {
+  const uint32_t n = 1u << scale;
+  const uint64_t raw = (uint64_t)edge_factor << scale;
+  const uint64_t t1 = (uint64_t)(a * 4294967296.0);
+  const uint64_t t2 = (uint64_t)((a + b) * 4294967296.0);
+  const uint64_t t3 = (uint64_t)((a + b + c) * 4294967296.0);
+  uint32_t key[2];
+  seed_key(seed, key);
+  std::vector<uint64_t> keys(raw);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)raw; ++i) {
+    uint32_t u = 0, v = 0, o[4] = {0, 0, 0, 0};
+    for (int l = 0; l < scale; ++l) {
+      if ((l & 3) == 0) {
+        uint32_t ctr[4] = {(uint32_t)i, (uint32_t)((uint64_t)i >> 32), (uint32_t)(l >> 2), STREAM_RMAT};
+        philox(ctr, key, o);
+      }
+      uint64_t r = o[l & 3];
+      uint32_t bu = 0, bv = 0;
+      if (r < t1) { bu = 0; bv = 0; }
+      else if (r < t2) { bu = 0; bv = 1; }
+      else if (r < t3) { bu = 1; bv = 0; }
+      else { bu = 1; bv = 1; }
+      u |= bu << (scale - 1 - l);
+      v |= bv << (scale - 1 - l);
+    }
+    keys[i] = (u == v) ? ~0ull : (((uint64_t)u << 32) | v);
+  }
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  if (!keys.empty() && keys.back() == ~0ull) keys.pop_back();
+  Graph* g = new Graph();
+  build_csr_from_sorted_keys(*g, n, keys);
+  if (weight_kind == 1) g->wu.resize(g->m);
+  if (weight_kind == 2) g->wf.resize(g->m);
+  if (weight_kind) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)g->m; ++i) {
+      uint32_t u = (uint32_t)(keys[i] >> 32), v = (uint32_t)keys[i];
+      edge_weight(seed, u, v, weight_kind, wmin, wmax, weight_kind == 1 ? &g->wu[i] : nullptr,
+                  weight_kind == 2 ? &g->wf[i] : nullptr);
+    }
+  }
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// Update batch generator (synthetic input definition, mirrors the device one):
+// n_del deletes of existing edges picked by edge index (64-bit Philox draw scaled
+// to m), then n_ins inserts with uniform endpoints (v bumped by one when it equals
+// u) and a fresh weight.  Ops are emitted deletes first, in draw order.
+// ---------------------------------------------------------------------------
+void orc_gen_batch(void* h, uint64_t seed, uint32_t batch_id, uint64_t n_del, uint64_t n_ins,
+                   int weight_kind, uint32_t wmin, uint32_t wmax, orc_update* ops) {
+  Graph* g = (Graph*)h;
+  uint32_t key[2];
+  seed_key(seed, key);
+#pragma omp parallel for schedule(static)
+  for (int64_t j = 0; j < (int64_t)n_del; ++j) {
+    uint32_t ctr[4] = {(uint32_t)j, (uint32_t)((uint64_t)j >> 32), batch_id, STREAM_DEL}, o[4];
+    philox(ctr, key, o);
+    uint64_t r = ((uint64_t)o[0] << 32) | o[1];
+    uint64_t idx = (uint64_t)(((unsigned __int128)r * g->m) >> 64);
+    uint32_t u = (uint32_t)(std::upper_bound(g->off.begin(), g->off.end(), idx) - g->off.begin() - 1);
+    orc_update& op = ops[j];
+    op.type = 1; op.u = u; op.v = g->col[idx]; op.w_u32 = 0; op.w_f32 = 0.f; op.pad = 0;
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t j = 0; j < (int64_t)n_ins; ++j) {
+    uint32_t ctr[4] = {(uint32_t)j, (uint32_t)((uint64_t)j >> 32), batch_id, STREAM_INS}, o[4];
+    philox(ctr, key, o);
+    uint32_t u = mulhi32(o[0], g->n), v = mulhi32(o[1], g->n);
+    if (v == u) v = (v + 1u == g->n) ? 0u : v + 1u;
+    orc_update& op = ops[n_del + j];
+    op.type = 0; op.u = u; op.v = v; op.pad = 0;
+    op.w_u32 = (weight_kind == 1) ? wmin + mulhi32(o[2], wmax - wmin + 1u) : 0u;
+    op.w_f32 = (weight_kind == 2) ? (float)((o[2] >> 8) + 1u) * (1.0f / 16777216.0f) : 0.f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// IncFragmentBuilder::Init/Build (grape/fragment/inc_fragment_builder.h:29-133):
+// the first "a u v w" of a pair wins (unordered_map::emplace, :57); every added
+// pair is emitted (:83-93); an old edge survives iff it is neither deleted nor
+// re-added (:105-116).  So an add beats a delete of the same pair and an add of
+// an existing pair replaces its weight.  touched[u] = 1 for every u named as a
+// source by any op (sum_sync_iter_worker.h:247-283 marks is_update the same way).
+// ---------------------------------------------------------------------------
+void* orc_apply_batch(void* h, const orc_update* ops, uint64_t n_ops, uint8_t* touched) {
+  Graph* g = (Graph*)h;
+  std::unordered_map<uint64_t, std::pair<uint32_t, float>> added;
+  std::unordered_set<uint64_t> deleted;
+  added.reserve(n_ops * 2);
+  deleted.reserve(n_ops * 2);
+  for (uint64_t i = 0; i < n_ops; ++i) {
+    uint64_t k = ((uint64_t)ops[i].u << 32) | ops[i].v;
+    if (ops[i].type == 0) added.emplace(k, std::make_pair(ops[i].w_u32, ops[i].w_f32));
+    else deleted.insert(k);
+    if (touched) touched[ops[i].u] = 1;
+  }
+  const bool hu = !g->wu.empty(), hf = !g->wf.empty();
+  std::vector<std::pair<uint64_t, std::pair<uint32_t, float>>> edges;
+  edges.reserve(g->m + added.size());
+  for (auto& kv : added) edges.push_back(kv);
+  for (uint32_t u = 0; u < g->n; ++u) {
+    for (uint64_t e = g->off[u]; e < g->off[u + 1]; ++e) {
+      uint64_t k = ((uint64_t)u << 32) | g->col[e];
+      if (deleted.count(k) || added.count(k)) continue;
+      edges.push_back({k, {hu ? g->wu[e] : 0u, hf ? g->wf[e] : 0.f}});
+    }
+  }
+  std::sort(edges.begin(), edges.end(),
+            [](const auto& x, const auto& y) { return x.first < y.first; });
+  std::vector<uint64_t> keys(edges.size());
+  for (size_t i = 0; i < edges.size(); ++i) keys[i] = edges[i].first;
+  Graph* ng = new Graph();
+  build_csr_from_sorted_keys(*ng, g->n, keys);
+  if (hu) { ng->wu.resize(ng->m); for (size_t i = 0; i < edges.size(); ++i) ng->wu[i] = edges[i].second.first; }
+  if (hf) { ng->wf.resize(ng->m); for (size_t i = 0; i < edges.size(); ++i) ng->wf[i] = edges[i].second.second; }
+  return ng;
+}
+
+// ---------------------------------------------------------------------------
+// Plain delta-accumulative PageRank, the reference's CPU Ingress branch
+// (grape/worker/sum_sync_iter_worker.h:857-882) with
+//   init_c: delta = (1-d)/N, init_v: r = 0   (pagerank/pagerank_ingress.h:27-32)
+//   g_function: if outdeg>0, delta_w += delta*d/outdeg   (pagerank_ingress.h:50-72)
+//   isChange: |delta| > T/N                  (sum_sync_iter_worker.h:2914-2925)
+//   termCheck: sum_v |delta_v| < T (FP64)    (sum_sync_iter_worker.h:2873-2912)
+//   stop when termCheck holds or step > max_rounds (:1372, pr_mr default 100).
+// The vertex loop is an OpenMP parallel for with CAS-loop atomic adds like the
+// reference's cilk_for + atomic_add (grape/utils/atomic_ops.h:171-187), so the
+// rounding order is run-dependent, as in the reference.  If init != 0 the
+// caller's r/delta are used as the starting state (incremental use).
+// stats[0]=rounds, [1]=edges traversed (f_send_delta_num), [2]=final sum|delta|.
+// ---------------------------------------------------------------------------
+static inline void atomic_add_f64(double* p, double v) {
+  std::atomic_ref<double> a(*p);
+  double cur = a.load(std::memory_order_relaxed);
+  while (!a.compare_exchange_weak(cur, cur + v, std::memory_order_relaxed)) {}
+}
+
+int orc_pr_ingress(void* h, double d, double T, int max_rounds, int init, double* r,
+                   double* delta, double* stats) {
+  Graph* g = (Graph*)h;
+  const uint32_t n = g->n;
+  if (init) {
+    for (uint32_t v = 0; v < n; ++v) { r[v] = 0.0; delta[v] = (1.0 - d) / n; }
+  }
+  const double eps = T / n;
+  int step = 0;
+  double edges = 0, sum = 0;
+  while (true) {
+    ++step;
+    double e_round = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : e_round)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+      uint32_t u = (uint32_t)i;
+      std::atomic_ref<double> du(delta[u]);
+      if (std::fabs(du.load(std::memory_order_relaxed)) > eps) {
+        double dv = du.exchange(0.0, std::memory_order_relaxed);
+        uint32_t deg = g->deg(u);
+        if (deg > 0) {
+          double outv = dv * d / deg;
+          for (uint64_t e = g->off[u]; e < g->off[u + 1]; ++e) atomic_add_f64(&delta[g->col[e]], outv);
+          e_round += deg;
+        }
+        r[u] += dv;  // only this iteration touches r[u]
+      }
+    }
+    edges += e_round;
+    sum = 0;
+#pragma omp parallel for reduction(+ : sum)
+    for (int64_t i = 0; i < (int64_t)n; ++i) sum += std::fabs(delta[i]);
+    if (sum < T || step > max_rounds) break;
+  }
+  if (stats) { stats[0] = step; stats[1] = edges; stats[2] = sum; }
+  return step;
+}
+
+// Jacobi form of the same iteration with no per-vertex threshold: used to reach
+// a tight FP64 fixed point (the "FP64 run") quickly.  Stops when sum|delta| < tol.
+int orc_pr_solve(void* h, double d, double tol, int max_rounds, double* r, double* delta_io,
+                 int init) {
+  Graph* g = (Graph*)h;
+  const uint32_t n = g->n;
+  std::vector<double> cur(n), nxt(n);
+  if (init) { for (uint32_t v = 0; v < n; ++v) { r[v] = 0; cur[v] = (1.0 - d) / n; } }
+  else { for (uint32_t v = 0; v < n; ++v) cur[v] = delta_io[v]; }
+  int step = 0;
+  for (; step < max_rounds; ++step) {
+    std::fill(nxt.begin(), nxt.end(), 0.0);
+    double sum = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : sum)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+      uint32_t u = (uint32_t)i;
+      double dv = cur[u];
+      if (dv == 0.0) continue;
+      sum += std::fabs(dv);
+      r[u] += dv;
+      uint32_t deg = g->deg(u);
+      if (!deg) continue;
+      double outv = dv * d / deg;
+      for (uint64_t e = g->off[u]; e < g->off[u + 1]; ++e) atomic_add_f64(&nxt[g->col[e]], outv);
+    }
+    std::swap(cur, nxt);
+    if (sum < tol) { ++step; break; }
+  }
+  if (delta_io) for (uint32_t v = 0; v < n; ++v) delta_io[v] = cur[v];
+  return step;
+}
+
+// AmendValue(type) (sum_sync_iter_worker.h:139-180): for every source u (or only
+// those with touched[u] != 0 when touched is given -- the exact-arithmetic
+// equivalent the reference left commented out at :187-208), push
+// g_function(delta = type * r_u) along u's out-edges of graph h.
+void orc_pr_amend(void* h, double d, int type, const double* r, double* delta,
+                  const uint8_t* touched) {
+  Graph* g = (Graph*)h;
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t i = 0; i < (int64_t)g->n; ++i) {
+    uint32_t u = (uint32_t)i;
+    if (touched && !touched[u]) continue;
+    uint32_t deg = g->deg(u);
+    if (!deg || r[u] == 0.0) continue;
+    double outv = type * r[u] * d / deg;
+    for (uint64_t e = g->off[u]; e < g->off[u + 1]; ++e) atomic_add_f64(&delta[g->col[e]], outv);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SSSP.  Dijkstra with a binary heap: dist[v] (0xFFFFFFFF = unreachable, the
+// reference's INT32_MAX sentinel mapped to u32) and parent[v] = the smallest id u
+// with dist[u] + w(u,v) == dist[v] (source: itself; unreachable: itself, like
+// GetInitDelta's GenDelta(v, inf) in sssp/sssp_ingress.h:30-41).
+// ---------------------------------------------------------------------------
+void orc_sssp_dijkstra(void* h, uint32_t src, uint32_t* dist, uint32_t* parent) {
+  Graph* g = (Graph*)h;
+  const uint32_t INF = 0xFFFFFFFFu;
+  std::vector<uint64_t> best(g->n, ~0ull);
+  for (uint32_t v = 0; v < g->n; ++v) { dist[v] = INF; parent[v] = v; }
+  typedef std::pair<uint64_t, uint32_t> item;
+  std::priority_queue<item, std::vector<item>, std::greater<item>> pq;
+  best[src] = 0;
+  pq.push({0, src});
+  while (!pq.empty()) {
+    auto [du, u] = pq.top();
+    pq.pop();
+    if (du != best[u] || dist[u] != INF) continue;
+    dist[u] = (uint32_t)du;
+    for (uint64_t e = g->off[u]; e < g->off[u + 1]; ++e) {
+      uint32_t v = g->col[e];
+      uint64_t nd = du + (g->wu.empty() ? 1u : g->wu[e]);
+      if (nd < best[v]) { best[v] = nd; pq.push({nd, v}); }
+    }
+  }
+  // parents: min-id predecessor on a shortest path
+  for (uint32_t v = 0; v < g->n; ++v) if (v != src && dist[v] != INF) parent[v] = INF;
+  for (uint32_t u = 0; u < g->n; ++u) {
+    if (dist[u] == INF) continue;
+    for (uint64_t e = g->off[u]; e < g->off[u + 1]; ++e) {
+      uint32_t v = g->col[e];
+      if (v == src) continue;
+      uint64_t nd = (uint64_t)dist[u] + (g->wu.empty() ? 1u : g->wu[e]);
+      if (nd == dist[v] && u < parent[v]) parent[v] = u;
+    }
+  }
+}
+
+// The reference's weighted SSSP Ingress loop (the -cilk=false branch of
+// sssp/sssp_ingress.h:131-175 driven by sum_sync_traversal_worker.h:1146-1159):
+// value/delta pairs, CombineValueDelta(value, delta) then push delta.value + w
+// with SetIfLessThanAtomic (grape/utils/dependency_data.h:21-34, a CAS on the
+// {parent, value} pair) to every out-neighbour, activating it on success.
+// Frontier-synchronous like the reference's curr_modified_/next_modified_.
+// stats[0] = rounds, stats[1] = edges traversed.
+int orc_sssp_ingress(void* h, uint32_t src, uint32_t* dist, uint32_t* parent, double* stats) {
+  Graph* g = (Graph*)h;
+  const uint32_t n = g->n, INF = 0xFFFFFFFFu;
+  // packed {value:hi32, parent:lo32}; atomic min == SetIfLessThanAtomic with the
+  // smaller parent id winning ties (the reference keeps whichever CAS lands first).
+  std::vector<uint64_t> delta(n);
+  std::vector<uint32_t> value(n, INF);
+  for (uint32_t v = 0; v < n; ++v) delta[v] = ((uint64_t)INF << 32) | v;
+  delta[src] = ((uint64_t)0 << 32) | src;
+  std::vector<uint8_t> cur(n, 0), nxt(n, 0);
+  cur[src] = 1;
+  int rounds = 0;
+  double edges = 0;
+  bool any = true;
+  while (any) {
+    ++rounds;
+    double e_round = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : e_round)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+      uint32_t u = (uint32_t)i;
+      if (!cur[u]) continue;
+      cur[u] = 0;
+      uint64_t du = std::atomic_ref<uint64_t>(delta[u]).load(std::memory_order_relaxed);
+      uint32_t dv = (uint32_t)(du >> 32);
+      if (dv >= value[u]) continue;  // CombineValueDelta false
+      value[u] = dv;
+      for (uint64_t e = g->off[u]; e < g->off[u + 1]; ++e) {
+        uint32_t v = g->col[e];
+        uint64_t cand = ((uint64_t)(dv + (g->wu.empty() ? 1u : g->wu[e])) << 32) | u;
+        std::atomic_ref<uint64_t> a(delta[v]);
+        uint64_t old = a.load(std::memory_order_relaxed);
+        while (cand < old && !a.compare_exchange_weak(old, cand, std::memory_order_relaxed)) {}
+        if (cand < old) nxt[v] = 1;
+      }
+      e_round += (double)g->deg(u);
+    }
+    edges += e_round;
+    any = false;
+    for (uint32_t v = 0; v < n; ++v) if (nxt[v]) { any = true; break; }
+    std::swap(cur, nxt);
+  }
+  for (uint32_t v = 0; v < n; ++v) {
+    dist[v] = value[v];
+    parent[v] = (value[v] == INF) ? v : (uint32_t)delta[v];
+  }
+  if (stats) { stats[0] = rounds; stats[1] = edges; }
+  return rounds;
+}
+
+}  // extern "C"

@@ -1,0 +1,154 @@
+"""ctypes binding of libsuminc.so (include/suminc.h).
+
+The shared library is built in-tree by suminc_amd/build.py.  Loading fails
+loudly when it is missing: there is no CPU fallback for the device path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libsuminc.so"
+HEADER = Path(__file__).resolve().parent.parent / "include" / "suminc.h"
+
+OK = 0
+E_INVALID = -1
+E_HIP = -2
+E_NOMEM = -3
+E_STATE = -4
+E_TOO_LARGE = -5
+E_NOT_CONVERGED = -6
+E_NUMERIC = -7
+ERROR_NAMES = {
+    E_INVALID: "SUMINC_E_INVALID",
+    E_HIP: "SUMINC_E_HIP",
+    E_NOMEM: "SUMINC_E_NOMEM",
+    E_STATE: "SUMINC_E_STATE",
+    E_TOO_LARGE: "SUMINC_E_TOO_LARGE",
+    E_NOT_CONVERGED: "SUMINC_E_NOT_CONVERGED",
+    E_NUMERIC: "SUMINC_E_NUMERIC",
+}
+
+INF_U32 = 0xFFFFFFFF
+PR_FP64 = 0
+PR_FP32_KAHAN = 1
+W_NONE, W_U32, W_F32 = 0, 1, 2
+OP_ADD, OP_DEL = 0, 1
+
+UPDATE_DTYPE = np.dtype(
+    [("type", "<u4"), ("u", "<u4"), ("v", "<u4"), ("w_u32", "<u4"), ("w_f32", "<f4"), ("pad", "<u4")]
+)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("damping", C.c_double),
+        ("termcheck_threshold", C.c_double),
+        ("max_rounds", C.c_int32),
+        ("pr_precision", C.c_int32),
+        ("sssp_source", C.c_uint32),
+        ("pull_threshold_div", C.c_int32),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("rounds", C.c_double),
+        ("edges_traversed", C.c_double),
+        ("residual", C.c_double),
+        ("kernel_ms", C.c_double),
+        ("push_rounds", C.c_double),
+        ("pull_rounds", C.c_double),
+        ("reserved", C.c_double * 10),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        d["reserved"] = list(self.reserved)
+        return d
+
+
+class KernelProfile(C.Structure):
+    _fields_ = [
+        ("launches", C.c_double),
+        ("ms", C.c_double),
+        ("alg_bytes", C.c_double),
+        ("edges", C.c_double),
+        ("vertices", C.c_double),
+        ("reserved", C.c_double * 3),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class SumincError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {message}")
+        self.code = code
+
+
+_lib = None
+
+vp = C.c_void_p
+u32, u64, i32, f64 = C.c_uint32, C.c_uint64, C.c_int, C.c_double
+
+SIGNATURES = {
+    "suminc_default_params": (None, [C.POINTER(Params)]),
+    "suminc_ctx_create": (i32, [i32, C.POINTER(Params), C.POINTER(vp)]),
+    "suminc_ctx_destroy": (i32, [vp]),
+    "suminc_set_params": (i32, [vp, C.POINTER(Params)]),
+    "suminc_last_error": (C.c_char_p, [vp]),
+    "suminc_sync": (i32, [vp]),
+    "suminc_get_stream": (i32, [vp, C.POINTER(vp)]),
+    "suminc_upload_csr": (i32, [vp, u32, u64, vp, vp, i32, vp]),
+    "suminc_generate_rmat": (i32, [vp, i32, i32, f64, f64, f64, u64, i32, u32, u32]),
+    "suminc_graph_info": (i32, [vp, C.POINTER(u32), C.POINTER(u64), C.POINTER(i32)]),
+    "suminc_download_csr": (i32, [vp, vp, vp, vp]),
+    "suminc_generate_batch": (i32, [vp, u64, u32, u64, u64, u32, u32, vp]),
+    "suminc_apply_batch": (i32, [vp, vp, u64]),
+    "suminc_pr_init": (i32, [vp]),
+    "suminc_pr_run": (i32, [vp, C.POINTER(Stats)]),
+    "suminc_pr_step": (i32, [vp, C.POINTER(f64)]),
+    "suminc_pr_download": (i32, [vp, vp, vp]),
+    "suminc_pr_upload": (i32, [vp, vp, vp]),
+    "suminc_sssp_init": (i32, [vp, u32]),
+    "suminc_sssp_run": (i32, [vp, C.POINTER(Stats)]),
+    "suminc_sssp_download": (i32, [vp, vp, vp]),
+    "suminc_get_stats": (i32, [vp, C.POINTER(Stats)]),
+    "suminc_profile_enable": (i32, [vp, i32]),
+    "suminc_profile_get": (i32, [vp, i32, C.POINTER(KernelProfile)]),
+}
+
+KERNEL_PR_PUSH = 0
+KERNEL_PR_GATHER = 1
+KERNEL_SSSP_PUSH = 2
+
+
+def lib():
+    """Loads libsuminc.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                "or `python suminc_amd/build.py` (the device path has no CPU fallback)"
+            )
+        L = C.CDLL(str(LIB_PATH), mode=getattr(os, "RTLD_NOW", 2))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "arrays passed to libsuminc must be C-contiguous"
+    return a.ctypes.data_as(vp)

@@ -1,0 +1,173 @@
+"""Python handle over one libsuminc context (one device, one stream).
+
+This is plumbing for tests and the benchmark; the host-side worker that mirrors
+the reference's SumSyncIterWorker / SumSyncTraversalWorker phase loop lives in
+suminc_amd/worker.py and drives the same C ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Context:
+    def __init__(self, device: int = 0, **params):
+        self._lib = L.lib()
+        p = L.Params()
+        self._lib.suminc_default_params(C.byref(p))
+        for k, v in params.items():
+            if not hasattr(p, k):
+                raise TypeError(f"unknown parameter {k!r}")
+            setattr(p, k, v)
+        h = C.c_void_p()
+        rc = self._lib.suminc_ctx_create(device, C.byref(p), C.byref(h))
+        if rc != L.OK:
+            raise L.SumincError(rc, "suminc_ctx_create failed (is a GPU visible?)")
+        self._h = h
+        self.params = p
+
+    # ---- plumbing -------------------------------------------------------------
+    def _check(self, rc: int):
+        if rc != L.OK:
+            msg = self._lib.suminc_last_error(self._h)
+            raise L.SumincError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.suminc_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_params(self, **params):
+        for k, v in params.items():
+            if not hasattr(self.params, k):
+                raise TypeError(f"unknown parameter {k!r}")
+            setattr(self.params, k, v)
+        self._check(self._lib.suminc_set_params(self._h, C.byref(self.params)))
+
+    def sync(self):
+        self._check(self._lib.suminc_sync(self._h))
+
+    @property
+    def stream(self) -> int:
+        s = C.c_void_p()
+        self._check(self._lib.suminc_get_stream(self._h, C.byref(s)))
+        return s.value or 0
+
+    def stats(self) -> dict:
+        s = L.Stats()
+        self._check(self._lib.suminc_get_stats(self._h, C.byref(s)))
+        return s.as_dict()
+
+    def profile_enable(self, on: bool = True):
+        self._check(self._lib.suminc_profile_enable(self._h, int(on)))
+
+    def profile(self, kernel: int) -> dict:
+        p = L.KernelProfile()
+        self._check(self._lib.suminc_profile_get(self._h, kernel, C.byref(p)))
+        return p.as_dict()
+
+    # ---- graph ----------------------------------------------------------------
+    def generate_rmat(self, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, weights=None, wmin=1, wmax=64):
+        kind = {None: L.W_NONE, "u32": L.W_U32, "f32": L.W_F32}[weights]
+        self._check(self._lib.suminc_generate_rmat(self._h, scale, edge_factor, a, b, c, seed, kind, wmin, wmax))
+
+    def upload_csr(self, offsets, cols, weights=None):
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        col = np.ascontiguousarray(cols, dtype=np.uint32)
+        kind, w = L.W_NONE, None
+        if weights is not None:
+            w = np.ascontiguousarray(weights)
+            if w.dtype == np.float32:
+                kind = L.W_F32
+            else:
+                w = w.astype(np.uint32, copy=False)
+                kind = L.W_U32
+        n = len(off) - 1
+        self._check(self._lib.suminc_upload_csr(self._h, n, len(col), L.ptr(off), L.ptr(col), kind, L.ptr(w)))
+
+    def graph_info(self):
+        n, m, k = C.c_uint32(), C.c_uint64(), C.c_int()
+        self._check(self._lib.suminc_graph_info(self._h, C.byref(n), C.byref(m), C.byref(k)))
+        return n.value, m.value, k.value
+
+    def download_csr(self):
+        n, m, kind = self.graph_info()
+        off = np.zeros(n + 1, np.uint64)
+        col = np.zeros(m, np.uint32)
+        w = None
+        if kind == L.W_U32:
+            w = np.zeros(m, np.uint32)
+        elif kind == L.W_F32:
+            w = np.zeros(m, np.float32)
+        self._check(self._lib.suminc_download_csr(self._h, L.ptr(off), L.ptr(col), L.ptr(w)))
+        return off, col, w
+
+    # ---- batches --------------------------------------------------------------
+    def generate_batch(self, seed, batch_id, n_del, n_ins, wmin=1, wmax=64):
+        ops = np.zeros(n_del + n_ins, L.UPDATE_DTYPE)
+        self._check(self._lib.suminc_generate_batch(self._h, seed, batch_id, n_del, n_ins, wmin, wmax, L.ptr(ops)))
+        return ops
+
+    def apply_batch(self, ops):
+        ops = np.ascontiguousarray(ops, dtype=L.UPDATE_DTYPE)
+        self._check(self._lib.suminc_apply_batch(self._h, L.ptr(ops), len(ops)))
+        return self.stats()
+
+    # ---- PageRank -------------------------------------------------------------
+    def pr_init(self):
+        self._check(self._lib.suminc_pr_init(self._h))
+
+    def pr_run(self, allow_not_converged=False) -> dict:
+        s = L.Stats()
+        rc = self._lib.suminc_pr_run(self._h, C.byref(s))
+        if not (allow_not_converged and rc == L.E_NOT_CONVERGED):
+            self._check(rc)
+        return s.as_dict()
+
+    def pr_step(self) -> float:
+        r = C.c_double()
+        self._check(self._lib.suminc_pr_step(self._h, C.byref(r)))
+        return r.value
+
+    def pr_download(self):
+        n = self.graph_info()[0]
+        rank = np.zeros(n, np.float64)
+        delta = np.zeros(n, np.float64)
+        self._check(self._lib.suminc_pr_download(self._h, L.ptr(rank), L.ptr(delta)))
+        return rank, delta
+
+    def pr_upload(self, rank, delta):
+        r = np.ascontiguousarray(rank, np.float64)
+        d = np.ascontiguousarray(delta, np.float64)
+        self._check(self._lib.suminc_pr_upload(self._h, L.ptr(r), L.ptr(d)))
+
+    # ---- SSSP -----------------------------------------------------------------
+    def sssp_init(self, source=0):
+        self._check(self._lib.suminc_sssp_init(self._h, source))
+
+    def sssp_run(self) -> dict:
+        s = L.Stats()
+        self._check(self._lib.suminc_sssp_run(self._h, C.byref(s)))
+        return s.as_dict()
+
+    def sssp_download(self):
+        n = self.graph_info()[0]
+        dist = np.zeros(n, np.uint32)
+        parent = np.zeros(n, np.uint32)
+        self._check(self._lib.suminc_sssp_download(self._h, L.ptr(dist), L.ptr(parent)))
+        return dist, parent

@@ -1,0 +1,806 @@
+// batch.hip -- edge-update batches on the device: seeded batch generation, the
+// CSR/CSC delta-merge with IncFragmentBuilder semantics, the PageRank reseed
+// (AmendValue -1 on old / +1 on new out-edges of touched sources) and the SSSP
+// repair (reset of the dependency subtree of every deleted or re-weighted parent
+// edge, re-pull of reset vertices, relaxation of inserts).
+//
+// Reference: grape/fragment/inc_fragment_builder.h:29-133 (batch semantics),
+// grape/worker/sum_sync_iter_worker.h:139-345 (reloadGraph + AmendValue),
+// grape/worker/sum_sync_traversal_worker.h:113-352 (deltaCompute).
+//
+// Merge: ops are sorted by (u,v) (stable LSD radix sort, so the first "a" of a
+// pair in file order is the first one met), collapsed into one group per pair
+// {row, col, action, weight}; each group is probed in the old adjacency (binary
+// search, rows are sorted).  New row length = old + #adds - #existing-pairs.  Old
+// edges of untouched rows are copied to their shifted position; edges of touched
+// rows get their position from two prefix counts over the row's groups (existing
+// pairs before, adds before), adds are placed symmetrically.  The same kernels
+// run on the transposed groups for the CSC.  Cost O(m + G log).
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include <vector>
+
+#include "ctx.hpp"
+#include "device.hpp"
+#include "pr.hpp"
+
+namespace suminc {
+
+// ---------------------------------------------------------------------------
+// batch generation (synthetic input; same definition as the oracle's)
+// ---------------------------------------------------------------------------
+__global__ void k_gen_dels(const uint32_t* off, const uint32_t* col, uint32_t n, uint64_t m, uint32_t k0,
+                           uint32_t k1, uint32_t batch_id, uint64_t n_del, suminc_update* ops) {
+  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_del) return;
+  U4 o = philox4x32_10((uint32_t)j, (uint32_t)(j >> 32), batch_id, kStreamDel, k0, k1);
+  uint64_t r = ((uint64_t)o.x << 32) | o.y;
+  uint64_t idx = __umul64hi(r, m);
+  uint32_t lo = 0, hi = n - 1;  // last row with off[row] <= idx
+  while (lo < hi) {
+    uint32_t mid = lo + ((hi - lo + 1) >> 1);
+    if (off[mid] <= idx) lo = mid; else hi = mid - 1;
+  }
+  suminc_update op;
+  op.type = SUMINC_OP_DEL; op.u = lo; op.v = col[idx]; op.w_u32 = 0; op.w_f32 = 0.f; op.pad = 0;
+  ops[j] = op;
+}
+
+__global__ void k_gen_ins(uint32_t n, int wkind, uint32_t wmin, uint32_t wmax, uint32_t k0, uint32_t k1,
+                          uint32_t batch_id, uint64_t n_ins, suminc_update* ops) {
+  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_ins) return;
+  U4 o = philox4x32_10((uint32_t)j, (uint32_t)(j >> 32), batch_id, kStreamIns, k0, k1);
+  uint32_t u = mulhi32(o.x, n), v = mulhi32(o.y, n);
+  if (v == u) v = (v + 1u == n) ? 0u : v + 1u;
+  suminc_update op;
+  op.type = SUMINC_OP_ADD; op.u = u; op.v = v; op.pad = 0;
+  op.w_u32 = wkind == SUMINC_W_U32 ? wmin + mulhi32(o.z, wmax - wmin + 1u) : 0u;
+  op.w_f32 = wkind == SUMINC_W_F32 ? (float)((o.z >> 8) + 1u) * (1.0f / 16777216.0f) : 0.f;
+  ops[j] = op;
+}
+
+void generate_batch(suminc_ctx* c, uint64_t seed, uint32_t batch_id, uint64_t n_del, uint64_t n_ins, uint32_t wmin,
+                    uint32_t wmax, suminc_update* ops) {
+  SUMINC_REQUIRE(c->has_graph, SUMINC_E_STATE, "no graph");
+  SUMINC_REQUIRE(ops || n_del + n_ins == 0, SUMINC_E_INVALID, "null ops");
+  SUMINC_REQUIRE(n_del == 0 || c->m > 0, SUMINC_E_INVALID, "cannot delete from an empty graph");
+  SUMINC_REQUIRE(c->wkind != SUMINC_W_U32 || (wmin >= 1 && wmax >= wmin), SUMINC_E_INVALID, "bad weight range");
+  const uint64_t total = n_del + n_ins;
+  if (!total) return;
+  DevBuf d;
+  d.ensure(total * sizeof(suminc_update));
+  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  if (n_del)
+    k_gen_dels<<<grid_for(n_del), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(), c->n,
+                                                          c->m, k0, k1, batch_id, n_del, d.as<suminc_update>());
+  if (n_ins)
+    k_gen_ins<<<grid_for(n_ins), kBlock, 0, c->stream>>>(c->n, c->wkind, wmin, wmax, k0, k1, batch_id, n_ins,
+                                                         d.as<suminc_update>() + n_del);
+  SUMINC_HIP(hipGetLastError());
+  SUMINC_HIP(hipMemcpyAsync(ops, d.p, total * sizeof(suminc_update), hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+}
+
+// ---------------------------------------------------------------------------
+// grouping
+// ---------------------------------------------------------------------------
+__global__ void k_op_keys(const suminc_update* ops, uint64_t n_ops, uint64_t* keys, uint32_t* idx) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_ops) return;
+  keys[i] = ((uint64_t)ops[i].u << 32) | ops[i].v;
+  idx[i] = (uint32_t)i;
+}
+
+// Group heads: action 0 = add (first add of the pair in batch order, weight
+// bits of that add), 1 = delete.
+__global__ void k_group_heads(const uint64_t* skeys, const uint32_t* sidx, const suminc_update* ops, uint64_t n_ops,
+                              int wkind, uint4* rec, uint8_t* flag) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_ops) return;
+  uint64_t k = skeys[i];
+  bool head = (i == 0) || skeys[i - 1] != k;
+  flag[i] = head;
+  if (!head) return;
+  uint32_t act = 1, wbits = 0;
+  for (uint64_t j = i; j < n_ops && skeys[j] == k; ++j) {
+    const suminc_update& op = ops[sidx[j]];
+    if (op.type == SUMINC_OP_ADD) {
+      act = 0;
+      wbits = wkind == SUMINC_W_F32 ? __float_as_uint(op.w_f32) : op.w_u32;
+      break;
+    }
+  }
+  rec[i] = make_uint4((uint32_t)(k >> 32), (uint32_t)k, act, wbits);
+}
+
+// Group arrays in structure-of-arrays form, for the CSR (row=u, col=v) or the
+// transposed view for the CSC (row=v, col=u).
+struct Groups {
+  uint32_t count;
+  const uint32_t* row;
+  const uint32_t* col;
+  const uint8_t* act;   // 0 add, 1 delete
+  const uint32_t* w;    // weight bits of adds
+  const uint8_t* ex;    // pair exists in the old graph
+  const uint32_t* ex_pre;   // exclusive prefix of ex (count+1)
+  const uint32_t* add_pre;  // exclusive prefix of (act == 0) (count+1)
+  const uint32_t* tix;  // row -> index in touched list or INF32
+  const uint32_t* t_gbeg;   // touched row -> first group (T+1)
+};
+
+__global__ void k_split_rec(const uint4* rec, uint32_t G, uint32_t* row, uint32_t* col, uint8_t* act, uint32_t* w) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  uint4 r = rec[g];
+  row[g] = r.x; col[g] = r.y; act[g] = (uint8_t)r.z; w[g] = r.w;
+}
+
+// existence probe in the old out-CSR; also records the old weight bits
+__global__ void k_probe(const uint32_t* off, const uint32_t* colv, const uint32_t* wv, const uint32_t* grow,
+                        const uint32_t* gcol, uint32_t G, uint8_t* ex, uint32_t* eold_w) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  uint32_t u = grow[g], v = gcol[g];
+  uint32_t lo = off[u], hi = off[u + 1];
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (colv[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  bool found = lo < off[u + 1] && colv[lo] == v;
+  ex[g] = found;
+  eold_w[g] = (found && wv) ? wv[lo] : 0u;
+}
+
+__global__ void k_flags_u32(const uint8_t* ex, const uint8_t* act, uint32_t G, uint32_t* exu, uint32_t* addu) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g > G) return;
+  exu[g] = g < G ? ex[g] : 0u;
+  addu[g] = g < G ? (act[g] == 0) : 0u;
+}
+
+__global__ void k_row_heads(const uint32_t* row, uint32_t G, uint8_t* head) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  head[g] = (g == 0) || row[g - 1] != row[g];
+}
+
+__global__ void k_touched(const uint32_t* row, const uint32_t* t_gbeg, uint32_t T, uint32_t G, uint32_t* t_list,
+                          uint32_t* tix, uint32_t* t_gbeg_out) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > T) return;
+  if (t == T) { t_gbeg_out[T] = G; return; }
+  uint32_t g = t_gbeg[t];
+  t_gbeg_out[t] = g;
+  t_list[t] = row[g];
+  tix[row[g]] = t;
+}
+
+__global__ void k_new_deg(const uint32_t* off, uint32_t n, Groups gs, uint32_t* ndeg) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > n) return;
+  if (v == n) { ndeg[n] = 0; return; }
+  uint32_t d = off[v + 1] - off[v];
+  uint32_t t = gs.tix[v];
+  if (t != INF32) {
+    uint32_t b = gs.t_gbeg[t], e = gs.t_gbeg[t + 1];
+    d = d + (gs.add_pre[e] - gs.add_pre[b]) - (gs.ex_pre[e] - gs.ex_pre[b]);
+  }
+  ndeg[v] = d;
+}
+
+// Copy surviving old edges to their new positions.
+__global__ void __launch_bounds__(kBlock) k_copy_old(uint32_t n, const uint32_t* off, const uint32_t* colv,
+                                                     const uint32_t* wv, const uint32_t* noff, uint32_t* ncol,
+                                                     uint32_t* nw, Groups gs) {
+  __shared__ uint32_t s_scan[kBlock + 1];
+  __shared__ uint32_t s_start[kBlock];
+  __shared__ uint32_t s_warp[kBlock / 64 + 1];
+  __shared__ uint32_t s_row[kBlock];
+  const uint32_t tiles = (n + kBlock - 1) / kBlock;
+  for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    uint32_t v = t * kBlock + threadIdx.x;
+    uint32_t start = 0, len = 0;
+    if (v < n) {
+      start = off[v];
+      len = off[v + 1] - start;
+      s_row[threadIdx.x] = v;
+    }
+    block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t k, uint32_t e) {
+      uint32_t u = s_row[slot];
+      uint32_t x = colv[e];
+      uint32_t ti = gs.tix[u];
+      uint32_t dst;
+      if (ti == INF32) {
+        dst = noff[u] + k;
+      } else {
+        uint32_t b = gs.t_gbeg[ti], en = gs.t_gbeg[ti + 1];
+        uint32_t lo = b, hi = en;
+        while (lo < hi) {
+          uint32_t mid = (lo + hi) >> 1;
+          if (gs.col[mid] < x) lo = mid + 1; else hi = mid;
+        }
+        if (lo < en && gs.col[lo] == x) return;  // deleted or replaced by an add
+        dst = noff[u] + k - (gs.ex_pre[lo] - gs.ex_pre[b]) + (gs.add_pre[lo] - gs.add_pre[b]);
+      }
+      ncol[dst] = x;
+      if (nw) nw[dst] = wv[e];
+    });
+  }
+}
+
+__global__ void k_place_adds(const uint32_t* off, const uint32_t* colv, const uint32_t* noff, uint32_t* ncol,
+                             uint32_t* nw, Groups gs) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= gs.count || gs.act[g] != 0) return;
+  uint32_t u = gs.row[g], v = gs.col[g];
+  uint32_t ti = gs.tix[u];
+  uint32_t b = gs.t_gbeg[ti];
+  uint32_t lo = off[u], hi = off[u + 1];
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (colv[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  uint32_t old_before = lo - off[u];
+  uint32_t kept_before = old_before - (gs.ex_pre[g] - gs.ex_pre[b]);
+  uint32_t adds_before = gs.add_pre[g] - gs.add_pre[b];
+  uint32_t dst = noff[u] + kept_before + adds_before;
+  ncol[dst] = v;
+  if (nw) nw[dst] = gs.w[g];
+}
+
+__global__ void k_transpose_group_keys(const uint32_t* row, const uint32_t* col, uint32_t G, uint64_t* keys,
+                                       uint32_t* idx) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  keys[g] = ((uint64_t)col[g] << 32) | row[g];
+  idx[g] = g;
+}
+
+__global__ void k_gather_groups(const uint32_t* sidx, uint32_t G, const uint32_t* row, const uint32_t* col,
+                                const uint8_t* act, const uint32_t* w, const uint8_t* ex, uint32_t* trow,
+                                uint32_t* tcol, uint8_t* tact, uint32_t* tw, uint8_t* tex) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  uint32_t s = sidx[g];
+  trow[g] = col[s];
+  tcol[g] = row[s];
+  tact[g] = act[s];
+  tw[g] = w[s];
+  tex[g] = ex[s];
+}
+
+// ---------------------------------------------------------------------------
+// PageRank reseed: list (u, type*d*r_u/deg) for touched u over the current CSR
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_reseed_list(const uint32_t* t_list, uint32_t Tn, const uint32_t* off, const T* rank,
+                              const float* comp, T d, T type, uint32_t* wl_v, T* wl_c, uint4* heavy, Counters* cnt) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  bool light = false;
+  uint32_t u = 0;
+  T c = T(0);
+  if (t < Tn) {
+    u = t_list[t];
+    uint32_t deg = off[u + 1] - off[u];
+    T r = rank[u];
+    if constexpr (sizeof(T) == 4) r = (T)((double)r - (double)comp[u]);
+    if (deg > 0 && r != T(0)) {
+      c = type * d * r / (T)deg;
+      atomicAdd(&cnt->aux[5], (unsigned long long)deg);
+      if (deg <= kHeavyDeg) {
+        light = true;
+      } else {
+        uint32_t nch = (deg + kChunk - 1) / kChunk;
+        unsigned long long slot = atomicAdd(&cnt->heavy_count, (unsigned long long)nch);
+        for (uint32_t k = 0; k < nch; ++k) {
+          uint4 h;
+          h.x = u; h.y = k;
+          if constexpr (sizeof(T) == 4) { h.z = __float_as_uint((float)c); h.w = 0; }
+          else {
+            unsigned long long bits = __double_as_longlong((double)c);
+            h.z = (uint32_t)bits; h.w = (uint32_t)(bits >> 32);
+          }
+          heavy[slot + k] = h;
+        }
+      }
+    }
+  }
+  unsigned long long slot = wave_append(light, &cnt->wl_count[0]);
+  if (light) { wl_v[slot] = u; wl_c[slot] = c; }
+}
+
+// Same scatter as the PageRank push round, over the reseed list.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_reseed_push(const uint32_t* __restrict__ off, const uint32_t* __restrict__ col,
+                                                        T* __restrict__ delta, const uint32_t* __restrict__ wl_v,
+                                                        const T* __restrict__ wl_c, const uint4* __restrict__ heavy,
+                                                        const Counters* cnt) {
+  __shared__ uint32_t s_scan[kBlock + 1];
+  __shared__ uint32_t s_start[kBlock];
+  __shared__ uint32_t s_warp[kBlock / 64 + 1];
+  __shared__ T s_c[kBlock];
+  const unsigned long long nl = cnt->wl_count[0];
+  const unsigned long long nh = cnt->heavy_count;
+  const unsigned long long tiles = (nl + kBlock - 1) / kBlock;
+  for (unsigned long long t = blockIdx.x; t < tiles; t += gridDim.x) {
+    unsigned long long i = t * kBlock + threadIdx.x;
+    uint32_t start = 0, len = 0;
+    if (i < nl) {
+      uint32_t v = wl_v[i];
+      start = off[v];
+      len = off[v + 1] - start;
+      s_c[threadIdx.x] = wl_c[i];
+    }
+    block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp,
+                            [&](int slot, uint32_t, uint32_t e) { unsafeAtomicAdd(&delta[col[e]], s_c[slot]); });
+  }
+  for (unsigned long long h = blockIdx.x; h < nh; h += gridDim.x) {
+    uint4 hv = heavy[h];
+    T c;
+    if constexpr (sizeof(T) == 4) c = __uint_as_float(hv.z);
+    else c = __longlong_as_double((long long)(((unsigned long long)hv.w << 32) | hv.z));
+    uint32_t b = off[hv.x] + hv.y * kChunk;
+    uint32_t e = min(off[hv.x + 1], b + kChunk);
+    for (uint32_t k = b + threadIdx.x; k < e; k += kBlock) unsafeAtomicAdd(&delta[col[k]], c);
+  }
+}
+
+__global__ void k_reset_lists(Counters* cnt) {
+  if (threadIdx.x) return;
+  cnt->wl_count[0] = 0;
+  cnt->heavy_count = 0;
+}
+
+// ---------------------------------------------------------------------------
+// SSSP repair
+// ---------------------------------------------------------------------------
+// roots: deleted or re-weighted existing pair (u,v) with parent(v) == u
+__global__ void k_sssp_roots(const uint32_t* grow, const uint32_t* gcol, const uint8_t* gact, const uint32_t* gw,
+                             const uint8_t* ex, const uint32_t* eold_w, uint32_t G, const unsigned long long* dp,
+                             uint32_t src, uint32_t* rbits, uint32_t* rlist, Counters* cnt) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  bool add = false;
+  uint32_t v = 0;
+  if (g < G && ex[g]) {
+    uint32_t u = grow[g];
+    v = gcol[g];
+    bool changed = gact[g] == 1 || gw[g] != eold_w[g];
+    unsigned long long x = dp[v];
+    if (changed && v != src && (uint32_t)(x >> 32) != INF32 && (uint32_t)x == u) {
+      uint32_t bit = 1u << (v & 31);
+      add = !(atomicOr(&rbits[v >> 5], bit) & bit);
+    }
+  }
+  unsigned long long slot = wave_append(add, &cnt->aux[4]);
+  if (add) rlist[slot] = v;
+}
+
+// one round of the subtree walk over the OLD out-CSR: children x of u with
+// parent(x) == u join the reset set (sum_sync_traversal_worker.h:170-194)
+__global__ void __launch_bounds__(kBlock) k_reset_round(const uint32_t* off, const uint32_t* colv,
+                                                        const unsigned long long* dp, uint32_t* rbits,
+                                                        uint32_t* rlist, Counters* cnt) {
+  __shared__ uint32_t s_scan[kBlock + 1];
+  __shared__ uint32_t s_start[kBlock];
+  __shared__ uint32_t s_warp[kBlock / 64 + 1];
+  __shared__ uint32_t s_u[kBlock];
+  if (cnt->done) return;
+  const unsigned long long b = cnt->aux[2], e = cnt->aux[3];
+  const unsigned long long tiles = (e - b + kBlock - 1) / kBlock;
+  for (unsigned long long t = blockIdx.x; t < tiles; t += gridDim.x) {
+    unsigned long long i = b + t * kBlock + threadIdx.x;
+    uint32_t start = 0, len = 0;
+    if (i < e) {
+      uint32_t u = rlist[i];
+      s_u[threadIdx.x] = u;
+      start = off[u];
+      len = off[u + 1] - start;
+    }
+    block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t, uint32_t k) {
+      uint32_t u = s_u[slot];
+      uint32_t x = colv[k];
+      bool add = false;
+      if (x != u && (uint32_t)dp[x] == u && (uint32_t)(dp[x] >> 32) != INF32) {
+        uint32_t bit = 1u << (x & 31);
+        add = !(atomicOr(&rbits[x >> 5], bit) & bit);
+      }
+      unsigned long long slot2 = wave_append(add, &cnt->aux[4]);
+      if (add) rlist[slot2] = x;
+    });
+  }
+}
+
+__global__ void k_reset_end(Counters* cnt) {
+  if (threadIdx.x || cnt->done) return;
+  cnt->aux[2] = cnt->aux[3];
+  cnt->aux[3] = cnt->aux[4];
+  if (cnt->aux[3] == cnt->aux[2]) cnt->done = 1;
+}
+
+__global__ void k_reset_apply(const uint32_t* rlist, const Counters* cnt, unsigned long long* dp) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt->aux[4]) return;
+  uint32_t x = rlist[i];
+  dp[x] = ((unsigned long long)INF32 << 32) | x;
+}
+
+// reset vertices pull from surviving (non-reset) in-neighbours over the NEW CSC
+__global__ void __launch_bounds__(kBlock) k_reset_pull(const uint32_t* ioff, const uint32_t* icol,
+                                                       const uint32_t* iw, const uint32_t* rbits,
+                                                       const uint32_t* rlist, const Counters* cnt,
+                                                       unsigned long long* dp) {
+  __shared__ uint32_t s_scan[kBlock + 1];
+  __shared__ uint32_t s_start[kBlock];
+  __shared__ uint32_t s_warp[kBlock / 64 + 1];
+  __shared__ uint32_t s_x[kBlock];
+  const unsigned long long nr = cnt->aux[4];
+  const unsigned long long tiles = (nr + kBlock - 1) / kBlock;
+  for (unsigned long long t = blockIdx.x; t < tiles; t += gridDim.x) {
+    unsigned long long i = t * kBlock + threadIdx.x;
+    uint32_t start = 0, len = 0;
+    if (i < nr) {
+      uint32_t x = rlist[i];
+      s_x[threadIdx.x] = x;
+      start = ioff[x];
+      len = ioff[x + 1] - start;
+    }
+    block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t, uint32_t k) {
+      uint32_t u = icol[k];
+      if ((rbits[u >> 5] >> (u & 31)) & 1u) return;
+      uint32_t du = (uint32_t)(dp[u] >> 32);
+      if (du == INF32) return;
+      unsigned long long nd = (unsigned long long)du + (iw ? iw[k] : 1u);
+      if (nd >= INF32) return;
+      atomicMin(&dp[s_x[slot]], (nd << 32) | u);
+    });
+  }
+}
+
+// Seeds SSSP list 0 with every finite reset vertex.
+__global__ void k_reset_seed(const uint32_t* rlist, const Counters* cnt_ro, const unsigned long long* dp,
+                             uint32_t* wl0, uint32_t* bits0, Counters* cnt) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool add = false;
+  uint32_t x = 0;
+  if (i < cnt_ro->aux[4]) {
+    x = rlist[i];
+    if ((uint32_t)(dp[x] >> 32) != INF32) {
+      uint32_t bit = 1u << (x & 31);
+      add = !(atomicOr(&bits0[x >> 5], bit) & bit);
+    }
+  }
+  unsigned long long slot = wave_append(add, &cnt->wl_count[0]);
+  if (add) wl0[slot] = x;
+}
+
+// relax every add (new pair or new weight) from its current source distance
+__global__ void k_insert_relax(const uint32_t* grow, const uint32_t* gcol, const uint8_t* gact, const uint32_t* gw,
+                               uint32_t G, int unit, unsigned long long* dp, uint32_t* wl0, uint32_t* bits0,
+                               Counters* cnt) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  bool add = false;
+  uint32_t v = 0;
+  if (g < G && gact[g] == 0) {
+    uint32_t u = grow[g];
+    v = gcol[g];
+    uint32_t du = (uint32_t)(dp[u] >> 32);
+    unsigned long long nd = (unsigned long long)du + (unit ? 1u : gw[g]);
+    if (du != INF32 && nd < INF32) {
+      unsigned long long cand = (nd << 32) | u;
+      unsigned long long old = atomicMin(&dp[v], cand);
+      if ((old >> 32) > nd) {
+        uint32_t bit = 1u << (v & 31);
+        add = !(atomicOr(&bits0[v >> 5], bit) & bit);
+      }
+    }
+  }
+  unsigned long long slot = wave_append(add, &cnt->wl_count[0]);
+  if (add) wl0[slot] = v;
+}
+
+__global__ void k_sssp_repair_prepare(Counters* cnt) {
+  if (threadIdx.x) return;
+  cnt->wl_count[0] = 0;
+  cnt->wl_count[1] = 0;
+  cnt->heavy_count = 0;
+  cnt->aux[1] = 0;
+  cnt->aux[2] = 0;
+  cnt->aux[3] = 0;
+  cnt->aux[4] = 0;
+  cnt->done = 0;
+  cnt->rounds = 0;
+  cnt->edges = 0;
+}
+__global__ void k_reset_start(Counters* cnt) {
+  if (threadIdx.x) return;
+  cnt->aux[2] = 0;
+  cnt->aux[3] = cnt->aux[4];
+  cnt->done = cnt->aux[4] == 0;
+}
+__global__ void k_undone(Counters* cnt) {
+  if (threadIdx.x == 0) cnt->done = 0;
+}
+__global__ void k_batch_zero(Counters* cnt) {
+  if (threadIdx.x == 0) cnt->aux[5] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// host orchestration
+// ---------------------------------------------------------------------------
+template <class F>
+static void cub_call(suminc_ctx* c, DevBuf& tmp, F&& f) {
+  size_t bytes = 0;
+  SUMINC_HIP(f((void*)nullptr, bytes));
+  tmp.ensure(bytes);
+  SUMINC_HIP(f(tmp.p, bytes));
+}
+
+// rebuilds one adjacency direction (off/col/w) given its group view
+static void merge_direction(suminc_ctx* c, DevBuf& off, DevBuf& col, DevBuf& w, const Groups& gs_in, uint32_t Tn,
+                            uint64_t new_m) {
+  const uint32_t n = c->n;
+  Groups gs = gs_in;
+  DevBuf ndeg, noff, ncol, nw;
+  ndeg.ensure(((size_t)n + 1) * 4);
+  noff.ensure(((size_t)n + 1) * 4);
+  ncol.ensure(new_m * 4 + 4);
+  if (c->wkind) nw.ensure(new_m * 4 + 4);
+  k_new_deg<<<grid_for((uint64_t)n + 1), kBlock, 0, c->stream>>>(off.as<uint32_t>(), n, gs, ndeg.as<uint32_t>());
+  cub_call(c, c->tmp, [&](void* p, size_t& b) {
+    return rocprim::exclusive_scan(p, b, ndeg.as<uint32_t>(), noff.as<uint32_t>(), 0u, (size_t)n + 1,
+                                   rocprim::plus<uint32_t>(), c->stream);
+  });
+  k_copy_old<<<4096, kBlock, 0, c->stream>>>(n, off.as<uint32_t>(), col.as<uint32_t>(),
+                                            c->wkind ? w.as<uint32_t>() : nullptr, noff.as<uint32_t>(),
+                                            ncol.as<uint32_t>(), c->wkind ? nw.as<uint32_t>() : nullptr, gs);
+  if (gs.count)
+    k_place_adds<<<grid_for(gs.count), kBlock, 0, c->stream>>>(off.as<uint32_t>(), col.as<uint32_t>(),
+                                                               noff.as<uint32_t>(), ncol.as<uint32_t>(),
+                                                               c->wkind ? nw.as<uint32_t>() : nullptr, gs);
+  SUMINC_HIP(hipGetLastError());
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  off.swap(noff);
+  col.swap(ncol);
+  if (c->wkind) w.swap(nw);
+  (void)Tn;
+}
+
+template <class T>
+static void pr_reseed(suminc_ctx* c, const uint32_t* t_list, uint32_t Tn, double type) {
+  Counters* cnt = dev_counters(c);
+  k_reset_lists<<<1, 64, 0, c->stream>>>(cnt);
+  k_reseed_list<T><<<grid_for(Tn), kBlock, 0, c->stream>>>(t_list, Tn, c->out_off.as<uint32_t>(), c->pr_rank.as<T>(),
+                                                           c->pr_comp.as<float>(), (T)c->prm.damping, (T)type,
+                                                           c->wl0.as<uint32_t>(), c->wl1.as<T>(), c->heavy.as<uint4>(),
+                                                           cnt);
+  k_reseed_push<T><<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
+                                                          c->pr_delta.as<T>(), c->wl0.as<uint32_t>(), c->wl1.as<T>(),
+                                                          c->heavy.as<uint4>(), cnt);
+  k_reset_lists<<<1, 64, 0, c->stream>>>(cnt);
+  SUMINC_HIP(hipGetLastError());
+}
+
+void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops) {
+  SUMINC_REQUIRE(c->has_graph, SUMINC_E_STATE, "no graph");
+  SUMINC_REQUIRE(ops || n_ops == 0, SUMINC_E_INVALID, "null ops");
+  SUMINC_REQUIRE(n_ops < 0xFFFFFFFFull, SUMINC_E_TOO_LARGE, "batch too large");
+  for (uint64_t i = 0; i < n_ops; ++i) {
+    SUMINC_REQUIRE(ops[i].type == SUMINC_OP_ADD || ops[i].type == SUMINC_OP_DEL, SUMINC_E_INVALID,
+                   "op type must be SUMINC_OP_ADD or SUMINC_OP_DEL (line " + std::to_string(i) + ")");
+    SUMINC_REQUIRE(ops[i].u < c->n && ops[i].v < c->n, SUMINC_E_INVALID,
+                   "vertex id out of range (line " + std::to_string(i) + ")");
+  }
+  c->stats = suminc_stats{};
+  if (n_ops == 0) return;
+  const uint32_t n = c->n;
+  Counters* cnt = dev_counters(c);
+  k_batch_zero<<<1, 64, 0, c->stream>>>(cnt);
+  // An SSSP state that was initialised but never run is still the initial state:
+  // it is simply re-initialised on the new graph instead of repaired.
+  const bool sssp_repair = c->sssp_ready && !c->sssp_pending;
+  const bool sssp_reinit = c->sssp_ready && c->sssp_pending;
+
+  // 1. upload + sort + group
+  DevBuf d_ops, keys_a, keys_b, idx_a, idx_b, rec, flags, grec, gcount;
+  d_ops.ensure(n_ops * sizeof(suminc_update));
+  SUMINC_HIP(hipMemcpyAsync(d_ops.p, ops, n_ops * sizeof(suminc_update), hipMemcpyHostToDevice, c->stream));
+  keys_a.ensure(n_ops * 8); keys_b.ensure(n_ops * 8);
+  idx_a.ensure(n_ops * 4); idx_b.ensure(n_ops * 4);
+  rec.ensure(n_ops * 16); grec.ensure(n_ops * 16); flags.ensure(n_ops); gcount.ensure(8);
+  k_op_keys<<<grid_for(n_ops), kBlock, 0, c->stream>>>(d_ops.as<suminc_update>(), n_ops, keys_a.as<uint64_t>(),
+                                                       idx_a.as<uint32_t>());
+  cub_call(c, c->tmp, [&](void* p, size_t& b) {
+    return rocprim::radix_sort_pairs(p, b, keys_a.as<uint64_t>(), keys_b.as<uint64_t>(), idx_a.as<uint32_t>(),
+                                     idx_b.as<uint32_t>(), (size_t)n_ops, 0, 64, c->stream);
+  });
+  k_group_heads<<<grid_for(n_ops), kBlock, 0, c->stream>>>(keys_b.as<uint64_t>(), idx_b.as<uint32_t>(),
+                                                           d_ops.as<suminc_update>(), n_ops, c->wkind,
+                                                           rec.as<uint4>(), flags.as<uint8_t>());
+  cub_call(c, c->tmp, [&](void* p, size_t& b) {
+    return rocprim::select(p, b, rec.as<uint4>(), flags.as<uint8_t>(), grec.as<uint4>(),
+                           gcount.as<unsigned int>(), (size_t)n_ops, c->stream);
+  });
+  unsigned int G = 0;
+  SUMINC_HIP(hipMemcpyAsync(&G, gcount.p, 4, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+
+  // 2. group arrays (CSR view) + existence probe
+  DevBuf g_row, g_col, g_act, g_w, g_ex, g_eoldw, g_exu, g_addu, g_expre, g_addpre, head, t_gbeg_raw, t_count,
+      t_list, t_gbeg, tix;
+  g_row.ensure((size_t)G * 4 + 4); g_col.ensure((size_t)G * 4 + 4); g_act.ensure(G + 1); g_w.ensure((size_t)G * 4 + 4);
+  g_ex.ensure(G + 1); g_eoldw.ensure((size_t)G * 4 + 4);
+  g_exu.ensure(((size_t)G + 1) * 4); g_addu.ensure(((size_t)G + 1) * 4);
+  g_expre.ensure(((size_t)G + 1) * 4); g_addpre.ensure(((size_t)G + 1) * 4);
+  head.ensure(G + 1); t_gbeg_raw.ensure(((size_t)G + 1) * 4); t_count.ensure(8);
+  k_split_rec<<<grid_for(G), kBlock, 0, c->stream>>>(grec.as<uint4>(), G, g_row.as<uint32_t>(), g_col.as<uint32_t>(),
+                                                     g_act.as<uint8_t>(), g_w.as<uint32_t>());
+  k_probe<<<grid_for(G), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
+                                                 c->wkind ? c->out_w.as<uint32_t>() : nullptr, g_row.as<uint32_t>(),
+                                                 g_col.as<uint32_t>(), G, g_ex.as<uint8_t>(), g_eoldw.as<uint32_t>());
+  k_flags_u32<<<grid_for((uint64_t)G + 1), kBlock, 0, c->stream>>>(g_ex.as<uint8_t>(), g_act.as<uint8_t>(), G,
+                                                                   g_exu.as<uint32_t>(), g_addu.as<uint32_t>());
+  cub_call(c, c->tmp, [&](void* p, size_t& b) {
+    return rocprim::exclusive_scan(p, b, g_exu.as<uint32_t>(), g_expre.as<uint32_t>(), 0u, (size_t)G + 1,
+                                   rocprim::plus<uint32_t>(), c->stream);
+  });
+  cub_call(c, c->tmp, [&](void* p, size_t& b) {
+    return rocprim::exclusive_scan(p, b, g_addu.as<uint32_t>(), g_addpre.as<uint32_t>(), 0u, (size_t)G + 1,
+                                   rocprim::plus<uint32_t>(), c->stream);
+  });
+  k_row_heads<<<grid_for(G), kBlock, 0, c->stream>>>(g_row.as<uint32_t>(), G, head.as<uint8_t>());
+  cub_call(c, c->tmp, [&](void* p, size_t& b) {
+    return rocprim::select(p, b, rocprim::counting_iterator<uint32_t>(0), head.as<uint8_t>(),
+                           t_gbeg_raw.as<uint32_t>(), t_count.as<unsigned int>(), (size_t)G, c->stream);
+  });
+  unsigned int Tn = 0;
+  uint32_t ex_total = 0, add_total = 0;
+  SUMINC_HIP(hipMemcpyAsync(&Tn, t_count.p, 4, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipMemcpyAsync(&ex_total, g_expre.as<uint32_t>() + G, 4, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipMemcpyAsync(&add_total, g_addpre.as<uint32_t>() + G, 4, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  t_list.ensure((size_t)Tn * 4 + 4);
+  t_gbeg.ensure(((size_t)Tn + 1) * 4);
+  tix.ensure((size_t)n * 4);
+  SUMINC_HIP(hipMemsetAsync(tix.p, 0xFF, (size_t)n * 4, c->stream));
+  k_touched<<<grid_for((uint64_t)Tn + 1), kBlock, 0, c->stream>>>(g_row.as<uint32_t>(), t_gbeg_raw.as<uint32_t>(),
+                                                                  Tn, G, t_list.as<uint32_t>(), tix.as<uint32_t>(),
+                                                                  t_gbeg.as<uint32_t>());
+  SUMINC_HIP(hipGetLastError());
+  const uint64_t new_m = c->m + add_total - ex_total;
+  SUMINC_REQUIRE(new_m < 0xFFFFFFFFull, SUMINC_E_TOO_LARGE, "edge count would exceed 2^32-1");
+
+  // 3. PageRank reseed -1 on the old out-edges of touched sources
+  unsigned long long reseed_edges = 0;
+  if (c->pr_ready) {
+    if (c->pr_prec == SUMINC_PR_FP64) pr_reseed<double>(c, t_list.as<uint32_t>(), Tn, -1.0);
+    else pr_reseed<float>(c, t_list.as<uint32_t>(), Tn, -1.0);
+  }
+
+  // 4. SSSP: reset roots + subtree walk on the old graph
+  DevBuf rbits, rlist;
+  if (sssp_repair) {
+    const size_t words = ((size_t)n + 31) / 32 + 1;
+    rbits.ensure(words * 4);
+    rlist.ensure((size_t)n * 4 + 4);
+    SUMINC_HIP(hipMemsetAsync(rbits.p, 0, words * 4, c->stream));
+    k_sssp_repair_prepare<<<1, 64, 0, c->stream>>>(cnt);
+    k_sssp_roots<<<grid_for(G), kBlock, 0, c->stream>>>(g_row.as<uint32_t>(), g_col.as<uint32_t>(), g_act.as<uint8_t>(),
+                                                        g_w.as<uint32_t>(), g_ex.as<uint8_t>(), g_eoldw.as<uint32_t>(),
+                                                        G, c->sssp_dp.as<unsigned long long>(), c->sssp_src,
+                                                        rbits.as<uint32_t>(), rlist.as<uint32_t>(), cnt);
+    k_reset_start<<<1, 64, 0, c->stream>>>(cnt);
+    while (true) {
+      for (int i = 0; i < 8; ++i) {
+        k_reset_round<<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
+                                                             c->sssp_dp.as<unsigned long long>(), rbits.as<uint32_t>(),
+                                                             rlist.as<uint32_t>(), cnt);
+        k_reset_end<<<1, 64, 0, c->stream>>>(cnt);
+      }
+      SUMINC_HIP(hipGetLastError());
+      read_counters(c);
+      if (c->host_counters->done) break;
+    }
+    c->stats.reserved[0] = (double)c->host_counters->aux[4];  // reset vertex count
+  }
+
+  // 5. topology: CSR then CSC
+  Groups gs;
+  gs.count = G;
+  gs.row = g_row.as<uint32_t>(); gs.col = g_col.as<uint32_t>(); gs.act = g_act.as<uint8_t>();
+  gs.w = g_w.as<uint32_t>(); gs.ex = g_ex.as<uint8_t>();
+  gs.ex_pre = g_expre.as<uint32_t>(); gs.add_pre = g_addpre.as<uint32_t>();
+  gs.tix = tix.as<uint32_t>(); gs.t_gbeg = t_gbeg.as<uint32_t>();
+  merge_direction(c, c->out_off, c->out_col, c->out_w, gs, Tn, new_m);
+  {
+    DevBuf tk_a, tk_b, ti_a, ti_b, tg_row, tg_col, tg_act, tg_w, tg_ex, tg_exu, tg_addu, tg_expre, tg_addpre, thead,
+        tt_raw, tt_count, tt_list, tt_gbeg, ttix;
+    tk_a.ensure((size_t)G * 8 + 8); tk_b.ensure((size_t)G * 8 + 8); ti_a.ensure((size_t)G * 4 + 4);
+    ti_b.ensure((size_t)G * 4 + 4);
+    tg_row.ensure((size_t)G * 4 + 4); tg_col.ensure((size_t)G * 4 + 4); tg_act.ensure(G + 1);
+    tg_w.ensure((size_t)G * 4 + 4); tg_ex.ensure(G + 1);
+    tg_exu.ensure(((size_t)G + 1) * 4); tg_addu.ensure(((size_t)G + 1) * 4);
+    tg_expre.ensure(((size_t)G + 1) * 4); tg_addpre.ensure(((size_t)G + 1) * 4);
+    thead.ensure(G + 1); tt_raw.ensure(((size_t)G + 1) * 4); tt_count.ensure(8);
+    k_transpose_group_keys<<<grid_for(G), kBlock, 0, c->stream>>>(g_row.as<uint32_t>(), g_col.as<uint32_t>(), G,
+                                                                  tk_a.as<uint64_t>(), ti_a.as<uint32_t>());
+    cub_call(c, c->tmp, [&](void* p, size_t& b) {
+      return rocprim::radix_sort_pairs(p, b, tk_a.as<uint64_t>(), tk_b.as<uint64_t>(), ti_a.as<uint32_t>(),
+                                       ti_b.as<uint32_t>(), (size_t)G, 0, 64, c->stream);
+    });
+    k_gather_groups<<<grid_for(G), kBlock, 0, c->stream>>>(ti_b.as<uint32_t>(), G, g_row.as<uint32_t>(),
+                                                           g_col.as<uint32_t>(), g_act.as<uint8_t>(),
+                                                           g_w.as<uint32_t>(), g_ex.as<uint8_t>(),
+                                                           tg_row.as<uint32_t>(), tg_col.as<uint32_t>(),
+                                                           tg_act.as<uint8_t>(), tg_w.as<uint32_t>(),
+                                                           tg_ex.as<uint8_t>());
+    k_flags_u32<<<grid_for((uint64_t)G + 1), kBlock, 0, c->stream>>>(tg_ex.as<uint8_t>(), tg_act.as<uint8_t>(), G,
+                                                                     tg_exu.as<uint32_t>(), tg_addu.as<uint32_t>());
+    cub_call(c, c->tmp, [&](void* p, size_t& b) {
+      return rocprim::exclusive_scan(p, b, tg_exu.as<uint32_t>(), tg_expre.as<uint32_t>(), 0u, (size_t)G + 1,
+                                     rocprim::plus<uint32_t>(), c->stream);
+    });
+    cub_call(c, c->tmp, [&](void* p, size_t& b) {
+      return rocprim::exclusive_scan(p, b, tg_addu.as<uint32_t>(), tg_addpre.as<uint32_t>(), 0u, (size_t)G + 1,
+                                     rocprim::plus<uint32_t>(), c->stream);
+    });
+    k_row_heads<<<grid_for(G), kBlock, 0, c->stream>>>(tg_row.as<uint32_t>(), G, thead.as<uint8_t>());
+    cub_call(c, c->tmp, [&](void* p, size_t& b) {
+      return rocprim::select(p, b, rocprim::counting_iterator<uint32_t>(0), thead.as<uint8_t>(), tt_raw.as<uint32_t>(),
+                             tt_count.as<unsigned int>(), (size_t)G, c->stream);
+    });
+    unsigned int Tt = 0;
+    SUMINC_HIP(hipMemcpyAsync(&Tt, tt_count.p, 4, hipMemcpyDeviceToHost, c->stream));
+    SUMINC_HIP(hipStreamSynchronize(c->stream));
+    tt_list.ensure((size_t)Tt * 4 + 4);
+    tt_gbeg.ensure(((size_t)Tt + 1) * 4);
+    ttix.ensure((size_t)n * 4);
+    SUMINC_HIP(hipMemsetAsync(ttix.p, 0xFF, (size_t)n * 4, c->stream));
+    k_touched<<<grid_for((uint64_t)Tt + 1), kBlock, 0, c->stream>>>(tg_row.as<uint32_t>(), tt_raw.as<uint32_t>(), Tt,
+                                                                    G, tt_list.as<uint32_t>(), ttix.as<uint32_t>(),
+                                                                    tt_gbeg.as<uint32_t>());
+    Groups gt;
+    gt.count = G;
+    gt.row = tg_row.as<uint32_t>(); gt.col = tg_col.as<uint32_t>(); gt.act = tg_act.as<uint8_t>();
+    gt.w = tg_w.as<uint32_t>(); gt.ex = tg_ex.as<uint8_t>();
+    gt.ex_pre = tg_expre.as<uint32_t>(); gt.add_pre = tg_addpre.as<uint32_t>();
+    gt.tix = ttix.as<uint32_t>(); gt.t_gbeg = tt_gbeg.as<uint32_t>();
+    merge_direction(c, c->in_off, c->in_col, c->in_w, gt, Tt, new_m);
+  }
+  c->m = new_m;
+  // state buffers sized by m (heavy chunk list) may need to grow
+  if (c->pr_ready) c->heavy.ensure((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
+  if (c->sssp_ready) c->heavy.ensure((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
+
+  // 6. PageRank reseed +1 on the new out-edges
+  if (c->pr_ready) {
+    if (c->pr_prec == SUMINC_PR_FP64) pr_reseed<double>(c, t_list.as<uint32_t>(), Tn, +1.0);
+    else pr_reseed<float>(c, t_list.as<uint32_t>(), Tn, +1.0);
+  }
+
+  // 7. SSSP: reset, re-pull, relax inserts -> frontier list 0
+  if (sssp_reinit) sssp_init(c, c->sssp_src);
+  if (sssp_repair) {
+    k_reset_apply<<<grid_for(n), kBlock, 0, c->stream>>>(rlist.as<uint32_t>(), cnt, c->sssp_dp.as<unsigned long long>());
+    k_reset_pull<<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
+                                                        c->wkind == SUMINC_W_U32 ? c->in_w.as<uint32_t>() : nullptr,
+                                                        rbits.as<uint32_t>(), rlist.as<uint32_t>(), cnt,
+                                                        c->sssp_dp.as<unsigned long long>());
+    k_reset_seed<<<grid_for(n), kBlock, 0, c->stream>>>(rlist.as<uint32_t>(), cnt, c->sssp_dp.as<unsigned long long>(),
+                                                        c->wl0.as<uint32_t>(), c->sssp_flag0.as<uint32_t>(), cnt);
+    k_insert_relax<<<grid_for(G), kBlock, 0, c->stream>>>(g_row.as<uint32_t>(), g_col.as<uint32_t>(),
+                                                          g_act.as<uint8_t>(), g_w.as<uint32_t>(), G,
+                                                          c->wkind != SUMINC_W_U32, c->sssp_dp.as<unsigned long long>(),
+                                                          c->wl0.as<uint32_t>(), c->sssp_flag0.as<uint32_t>(), cnt);
+    k_undone<<<1, 64, 0, c->stream>>>(cnt);
+    SUMINC_HIP(hipGetLastError());
+  }
+  read_counters(c);
+  reseed_edges = c->host_counters->aux[5];
+  c->stats.reserved[1] = (double)G;
+  c->stats.reserved[2] = (double)Tn;
+  c->stats.reserved[3] = (double)reseed_edges;
+}
+
+}  // namespace suminc

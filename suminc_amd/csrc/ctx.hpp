@@ -1,0 +1,146 @@
+// ctx.hpp -- device context and helpers shared by the HIP translation units of
+// libsuminc.so.  Written for gfx950 (CDNA4, wave64) only.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/suminc.h"
+
+namespace suminc {
+
+// Internal failure carrying a SUMINC_E_* code; converted to a return code by
+// the C ABI wrappers in capi.hip (no entry point lets an exception escape).
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define SUMINC_HIP(expr)                                                              \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      throw ::suminc::Error(e_ == hipErrorOutOfMemory ? SUMINC_E_NOMEM : SUMINC_E_HIP, \
+                            std::string(#expr) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+#define SUMINC_REQUIRE(cond, code, msg) \
+  do {                                  \
+    if (!(cond)) throw ::suminc::Error((code), (msg)); \
+  } while (0)
+
+// Owning device allocation; ensure() grows without preserving contents.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  void ensure(size_t bytes) {
+    if (bytes <= cap && p) return;
+    release();
+    size_t b = bytes ? bytes : 256;
+    SUMINC_HIP(hipMalloc(&p, b));
+    cap = b;
+  }
+  template <class T>
+  T* as() const { return reinterpret_cast<T*>(p); }
+  void swap(DevBuf& o) {
+    std::swap(p, o.p);
+    std::swap(cap, o.cap);
+  }
+};
+
+// Device-side counters block (one allocation, read back in one copy).
+struct Counters {
+  unsigned long long wl_count[2];   // worklist lengths (double-buffered frontiers)
+  unsigned long long heavy_count;   // heavy-vertex chunk list length
+  unsigned long long edges;         // edges traversed (accumulated)
+  unsigned long long rounds;        // rounds executed on device
+  unsigned long long done;          // stop flag set by the device
+  unsigned long long aux[10];
+  double residual;                  // last FP64 sum|delta|
+  double residual_hist[2];
+  double dpad[5];
+};
+
+}  // namespace suminc
+
+struct suminc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  suminc_params prm{};
+  std::string err;
+
+  // graph: out-CSR and in-CSR (CSC), u32 row pointers, sorted by (src, dst)
+  bool has_graph = false;
+  uint32_t n = 0;
+  uint64_t m = 0;
+  int wkind = SUMINC_W_NONE;
+  suminc::DevBuf out_off, out_col, out_w, in_off, in_col, in_w;
+
+  // PageRank state
+  bool pr_ready = false;
+  int pr_prec = SUMINC_PR_FP32_KAHAN;
+  suminc::DevBuf pr_delta, pr_rank, pr_comp, pr_contrib;
+
+  // SSSP state: packed {dist:hi32, parent:lo32}
+  bool sssp_ready = false;
+  bool sssp_pending = false;  // initialised, not yet run
+  uint32_t sssp_src = 0;
+  suminc::DevBuf sssp_dp, sssp_flag0, sssp_flag1;
+
+  // work lists and scratch
+  suminc::DevBuf wl0, wl1, heavy, partial, tmp, tmp2, tmp3, tmp4, tmp5;
+  suminc::DevBuf counters;                  // suminc::Counters
+  suminc::Counters* host_counters = nullptr;  // pinned mirror
+
+  suminc_stats stats{};
+
+  // kernel profile (opt-in): event pairs around launches, accumulators per kernel
+  bool prof_on = false;
+  suminc_kernel_profile prof[SUMINC_KERNEL_COUNT]{};
+  std::vector<hipEvent_t> prof_events;  // pool, reused
+};
+
+namespace suminc {
+
+constexpr uint32_t INF32 = 0xFFFFFFFFu;
+constexpr int kBlock = 256;          // threads per block for edge-parallel kernels
+constexpr uint32_t kHeavyDeg = 4096; // adjacency longer than this goes to the chunk path
+constexpr uint32_t kChunk = 4096;    // edges per heavy chunk
+
+inline unsigned grid_for(uint64_t items, int block = kBlock, unsigned cap = 1u << 30) {
+  uint64_t g = (items + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+inline Counters* dev_counters(suminc_ctx* c) { return c->counters.as<Counters>(); }
+
+// Copies the device counters block to the pinned mirror and waits.
+void read_counters(suminc_ctx* c);
+
+// --- graph helpers (graph.hip) ---
+void build_csc(suminc_ctx* c);  // from out-CSR
+void ensure_state_sizes(suminc_ctx* c);
+
+// --- kernel profiling (capi.hip) ---
+// Event pair `slot` of the current run (grows the pool on demand).
+hipEvent_t prof_event(suminc_ctx* c, size_t slot);
+// Adds the elapsed time of the first `pairs` event pairs [2i, 2i+1] to kernel k.
+void prof_collect(suminc_ctx* c, int k, size_t pairs, size_t first_slot = 0, size_t stride = 2);
+
+}  // namespace suminc

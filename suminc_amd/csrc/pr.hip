@@ -1,0 +1,393 @@
+// pr.hip -- plain delta-accumulative PageRank on the device (the reference's
+// Ingress phase: sum_sync_iter_worker.h:857-882 on CPU; tjn::pr_Ingress +
+// tjn::deltaSum on GPU, cudaCode/my_worker.cu:124-142, 304-328).
+//
+// One round = three launches on the context stream:
+//   k_pr_gather  vertex pass: FP64 sum|delta| (deterministic block partials),
+//                activity |delta| > T/N, exchange delta -> 0, rank += delta
+//                (Kahan-compensated in FP32 mode), and for active vertices with
+//                out-degree > 0 append (v, d*delta/deg) to the light work list or
+//                (v, chunk, c) to the heavy chunk list.  Wave64 ballot + one
+//                atomic per wave for the append.
+//   k_pr_push    persistent blocks stride over light tiles (256 vertices, the
+//                block walks their adjacency cooperatively with coalesced column
+//                reads) then over heavy chunks (4096 edges each); scatter with
+//                hardware float/double atomic add.
+//   k_pr_end     single block: fixed-order sum of the partials -> residual,
+//                stop flag (residual < T or round cap), reset list counters.
+// Rounds are enqueued in groups without host synchronisation; a round whose
+// k_pr_gather sees the stop flag does nothing.
+#include "ctx.hpp"
+#include "device.hpp"
+#include "pr.hpp"
+
+namespace suminc {
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pr_gather(uint32_t n, const uint32_t* __restrict__ off, T* __restrict__ delta,
+                                                      T* __restrict__ rank, float* __restrict__ comp, double eps, T d,
+                                                      uint32_t* __restrict__ wl_v, T* __restrict__ wl_c,
+                                                      uint4* __restrict__ heavy, Counters* cnt,
+                                                      double* __restrict__ partial) {
+  __shared__ double s_sum[kBlock / 64];
+  __shared__ unsigned long long s_edges[kBlock / 64];
+  __shared__ unsigned long long s_act[kBlock / 64];
+  if (cnt->done) return;
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  double absd = 0.0;
+  uint32_t deg_active = 0;
+  bool light = false;
+  T c = T(0);
+  if (v < n) {
+    T dv = delta[v];
+    absd = (double)fabs_t(dv);
+    if (absd > eps) {
+      delta[v] = T(0);
+      if constexpr (sizeof(T) == 4) {
+        float r = rank[v], cc = comp[v];
+        float y = (float)dv - cc;
+        float t = r + y;
+        comp[v] = (t - r) - y;
+        rank[v] = t;
+      } else {
+        rank[v] += dv;
+      }
+      uint32_t b = off[v], e = off[v + 1];
+      uint32_t deg = e - b;
+      if (deg > 0) {
+        deg_active = deg;
+        c = d * dv / (T)deg;
+        if (deg <= kHeavyDeg) {
+          light = true;
+        } else {
+          uint32_t nch = (deg + kChunk - 1) / kChunk;
+          unsigned long long slot = atomicAdd(&cnt->heavy_count, (unsigned long long)nch);
+          for (uint32_t k = 0; k < nch; ++k) {
+            uint4 h;
+            h.x = v;
+            h.y = k;
+            if constexpr (sizeof(T) == 4) {
+              h.z = __float_as_uint((float)c);
+              h.w = 0;
+            } else {
+              unsigned long long bits = __double_as_longlong((double)c);
+              h.z = (uint32_t)bits;
+              h.w = (uint32_t)(bits >> 32);
+            }
+            heavy[slot + k] = h;
+          }
+        }
+      }
+    }
+  }
+  unsigned long long slot = wave_append(light, &cnt->wl_count[0]);
+  if (light) {
+    wl_v[slot] = v;
+    wl_c[slot] = c;
+  }
+  // block reductions: sum |delta| (FP64), active edges, active vertices
+  double s = absd;
+  unsigned long long ed = deg_active;
+  unsigned long long act = absd > eps ? 1ull : 0ull;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    s += __shfl_xor(s, o);
+    ed += __shfl_xor(ed, o);
+    act += __shfl_xor(act, o);
+  }
+  const int wid = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    s_sum[wid] = s;
+    s_edges[wid] = ed;
+    s_act[wid] = act;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    unsigned long long te = 0, ta = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      t += s_sum[w];
+      te += s_edges[w];
+      ta += s_act[w];
+    }
+    partial[blockIdx.x] = t;
+    if (te) atomicAdd(&cnt->edges, te);
+    if (ta) atomicAdd(&cnt->aux[8], ta);
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pr_push(const uint32_t* __restrict__ off, const uint32_t* __restrict__ col,
+                                                    T* __restrict__ delta, const uint32_t* __restrict__ wl_v,
+                                                    const T* __restrict__ wl_c, const uint4* __restrict__ heavy,
+                                                    const Counters* cnt) {
+  __shared__ uint32_t s_scan[kBlock + 1];
+  __shared__ uint32_t s_start[kBlock];
+  __shared__ uint32_t s_warp[kBlock / 64 + 1];
+  __shared__ T s_c[kBlock];
+  const unsigned long long nl = cnt->wl_count[0];
+  const unsigned long long nh = cnt->heavy_count;
+  const unsigned long long tiles = (nl + kBlock - 1) / kBlock;
+  for (unsigned long long t = blockIdx.x; t < tiles; t += gridDim.x) {
+    unsigned long long i = t * kBlock + threadIdx.x;
+    uint32_t start = 0, len = 0;
+    if (i < nl) {
+      uint32_t v = wl_v[i];
+      start = off[v];
+      len = off[v + 1] - start;
+      s_c[threadIdx.x] = wl_c[i];
+    }
+    block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t, uint32_t e) {
+      unsafeAtomicAdd(&delta[col[e]], s_c[slot]);
+    });
+  }
+  for (unsigned long long h = blockIdx.x; h < nh; h += gridDim.x) {
+    uint4 hv = heavy[h];
+    T c;
+    if constexpr (sizeof(T) == 4) c = __uint_as_float(hv.z);
+    else c = __longlong_as_double((long long)(((unsigned long long)hv.w << 32) | hv.z));
+    uint32_t b = off[hv.x] + hv.y * kChunk;
+    uint32_t e = min(off[hv.x + 1], b + kChunk);
+    for (uint32_t k = b + threadIdx.x; k < e; k += kBlock) unsafeAtomicAdd(&delta[col[k]], c);
+  }
+}
+
+// Single block: deterministic sum of partials, stop rule, counter reset.
+__global__ void __launch_bounds__(1024) k_pr_end(const double* __restrict__ partial, uint32_t nparts, Counters* cnt,
+                                                 double T, int max_rounds) {
+  __shared__ double s[1024];
+  if (cnt->done) return;
+  double acc = 0;
+  for (uint32_t i = threadIdx.x; i < nparts; i += 1024) acc += partial[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double r = s[0];
+    cnt->residual = r;
+    cnt->rounds += 1;
+    cnt->aux[6] += cnt->wl_count[0] + cnt->heavy_count;  // push entries processed
+    cnt->wl_count[0] = 0;
+    cnt->heavy_count = 0;
+    if (!(r >= T) || (long long)cnt->rounds >= (long long)max_rounds) cnt->done = 1;
+    if (r != r || r > 1e300) cnt->aux[0] = 1;  // numeric failure flag
+  }
+}
+
+template <class T>
+__global__ void k_pr_init(uint32_t n, T* delta, T* rank, float* comp, T init_delta) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  delta[v] = init_delta;
+  rank[v] = T(0);
+  if (comp) comp[v] = 0.f;
+}
+
+template <class T>
+__global__ void k_pr_to_f64(uint32_t n, const T* delta, const T* rank, const float* comp, double* d_out,
+                            double* r_out) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  d_out[v] = (double)delta[v];
+  double r = (double)rank[v];
+  if (comp) r -= (double)comp[v];
+  r_out[v] = r;
+}
+
+template <class T>
+__global__ void k_pr_from_f64(uint32_t n, T* delta, T* rank, float* comp, const double* d_in, const double* r_in) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  delta[v] = (T)d_in[v];
+  if (comp) {
+    float hi = (float)r_in[v];
+    rank[v] = (T)hi;
+    comp[v] = (float)((double)hi - r_in[v]);  // rank - comp == r_in to ~2^-48 relative
+  } else {
+    rank[v] = (T)r_in[v];
+  }
+}
+
+__global__ void k_zero_counters(Counters* cnt) {
+  if (threadIdx.x == 0) {
+    cnt->wl_count[0] = cnt->wl_count[1] = 0;
+    cnt->heavy_count = 0;
+    cnt->edges = 0;
+    cnt->rounds = 0;
+    cnt->done = 0;
+    for (int i = 0; i < 10; ++i) cnt->aux[i] = 0;
+    cnt->residual = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+unsigned push_grid() { return 2048; }
+
+void pr_alloc(suminc_ctx* c) {
+  const size_t n = c->n;
+  const size_t es = c->pr_prec == SUMINC_PR_FP64 ? 8 : 4;
+  c->pr_delta.ensure(n * es);
+  c->pr_rank.ensure(n * es);
+  if (c->pr_prec == SUMINC_PR_FP32_KAHAN) c->pr_comp.ensure(n * 4);
+  c->wl0.ensure(n * 4 + 4);
+  c->wl1.ensure(n * es + 8);
+  // heavy chunks: at most m / kChunk + (number of heavy vertices) entries
+  c->heavy.ensure((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
+  c->partial.ensure(((size_t)grid_for(n) + 1) * 8);
+}
+
+void pr_init(suminc_ctx* c) {
+  SUMINC_REQUIRE(c->has_graph, SUMINC_E_STATE, "no graph");
+  c->pr_prec = c->prm.pr_precision;
+  pr_alloc(c);
+  const double d = c->prm.damping;
+  const double init = (1.0 - d) / (double)c->n;
+  if (c->pr_prec == SUMINC_PR_FP64)
+    k_pr_init<double><<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->pr_delta.as<double>(),
+                                                               c->pr_rank.as<double>(), nullptr, init);
+  else
+    k_pr_init<float><<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->pr_delta.as<float>(), c->pr_rank.as<float>(),
+                                                              c->pr_comp.as<float>(), (float)init);
+  SUMINC_HIP(hipGetLastError());
+  c->pr_ready = true;
+}
+
+// Enqueues one round; with profiling on, events 3*slot, 3*slot+1, 3*slot+2
+// bracket the gather and push launches.
+template <class T>
+static void enqueue_round(suminc_ctx* c, double eps, double T_stop, int max_rounds, size_t slot) {
+  Counters* cnt = dev_counters(c);
+  const unsigned g = grid_for(c->n);
+  if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 3 * slot), c->stream));
+  k_pr_gather<T><<<g, kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), c->pr_delta.as<T>(),
+                                              c->pr_rank.as<T>(), c->pr_comp.as<float>(), eps,
+                                              (T)c->prm.damping, c->wl0.as<uint32_t>(), c->wl1.as<T>(),
+                                              c->heavy.as<uint4>(), cnt, c->partial.as<double>());
+  if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 3 * slot + 1), c->stream));
+  k_pr_push<T><<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
+                                                      c->pr_delta.as<T>(), c->wl0.as<uint32_t>(), c->wl1.as<T>(),
+                                                      c->heavy.as<uint4>(), cnt);
+  if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 3 * slot + 2), c->stream));
+  k_pr_end<<<1, 1024, 0, c->stream>>>(c->partial.as<double>(), g, cnt, T_stop, max_rounds);
+}
+
+// Adds the profile of the first `rounds` enqueued rounds (the ones that did work).
+static void pr_profile_rounds(suminc_ctx* c, size_t rounds) {
+  if (!c->prof_on) return;
+  prof_collect(c, SUMINC_KERNEL_PR_GATHER, rounds, 0, 3);
+  prof_collect(c, SUMINC_KERNEL_PR_PUSH, rounds, 1, 3);
+  const Counters* h = c->host_counters;
+  const double es = c->pr_prec == SUMINC_PR_FP64 ? 8.0 : 4.0;
+  suminc_kernel_profile& p = c->prof[SUMINC_KERNEL_PR_PUSH];
+  // push: column (4 B) + delta atomic read-modify-write (2 es) per edge; worklist
+  // entry (4 + es) and two row pointers (8 B) per entry
+  p.edges += (double)h->edges;
+  p.vertices += (double)h->aux[6];
+  p.alg_bytes += (double)h->edges * (4.0 + 2.0 * es) + (double)h->aux[6] * (12.0 + es);
+  suminc_kernel_profile& q = c->prof[SUMINC_KERNEL_PR_GATHER];
+  // gather: delta read for every vertex; per active vertex delta write, rank RMW,
+  // Kahan term RMW (FP32), row pointers, worklist entry
+  const double active = (double)h->aux[8];
+  const double kahan = c->pr_prec == SUMINC_PR_FP64 ? 0.0 : 8.0;
+  q.vertices += (double)rounds * c->n;
+  q.alg_bytes += (double)rounds * c->n * es + active * (es + 2 * es + kahan + 8.0 + 4.0 + es);
+}
+
+// Runs rounds until the device sets the stop flag; host polls every `group` rounds.
+void pr_run(suminc_ctx* c, suminc_stats* st) {
+  SUMINC_REQUIRE(c->pr_ready, SUMINC_E_STATE, "PageRank state not initialised");
+  const double T = c->prm.termcheck_threshold;
+  const double eps = T / (double)c->n;
+  const int max_rounds = c->prm.max_rounds > 0 ? c->prm.max_rounds : 1000000;
+  k_zero_counters<<<1, 64, 0, c->stream>>>(dev_counters(c));
+  SUMINC_HIP(hipEventRecord(c->ev_start, c->stream));
+  int group = 4;
+  size_t slot = 0;
+  while (true) {
+    for (int i = 0; i < group; ++i, ++slot) {
+      if (c->pr_prec == SUMINC_PR_FP64) enqueue_round<double>(c, eps, T, max_rounds, slot);
+      else enqueue_round<float>(c, eps, T, max_rounds, slot);
+    }
+    SUMINC_HIP(hipGetLastError());
+    read_counters(c);
+    if (c->host_counters->done) break;
+    if (group < 32) group *= 2;
+  }
+  SUMINC_HIP(hipEventRecord(c->ev_stop, c->stream));
+  SUMINC_HIP(hipEventSynchronize(c->ev_stop));
+  float ms = 0;
+  SUMINC_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+  Counters* h = c->host_counters;
+  pr_profile_rounds(c, (size_t)h->rounds);
+  c->stats = suminc_stats{};
+  c->stats.rounds = (double)h->rounds;
+  c->stats.edges_traversed = (double)h->edges;
+  c->stats.residual = h->residual;
+  c->stats.kernel_ms = ms;
+  c->stats.push_rounds = (double)h->rounds;
+  if (st) *st = c->stats;
+  SUMINC_REQUIRE(!h->aux[0], SUMINC_E_NUMERIC, "NaN/Inf in PageRank residual");
+  SUMINC_REQUIRE(!(h->residual >= T), SUMINC_E_NOT_CONVERGED, "round cap reached before sum|delta| < T");
+}
+
+void pr_step(suminc_ctx* c, double* residual) {
+  SUMINC_REQUIRE(c->pr_ready, SUMINC_E_STATE, "PageRank state not initialised");
+  const double T = c->prm.termcheck_threshold;
+  k_zero_counters<<<1, 64, 0, c->stream>>>(dev_counters(c));
+  // stop rule disabled for a single explicit step
+  if (c->pr_prec == SUMINC_PR_FP64) enqueue_round<double>(c, T / c->n, -1.0, 1 << 30, 0);
+  else enqueue_round<float>(c, T / c->n, -1.0, 1 << 30, 0);
+  SUMINC_HIP(hipGetLastError());
+  read_counters(c);
+  pr_profile_rounds(c, 1);
+  if (residual) *residual = c->host_counters->residual;
+}
+
+void pr_download(suminc_ctx* c, double* rank, double* delta) {
+  SUMINC_REQUIRE(c->pr_ready, SUMINC_E_STATE, "PageRank state not initialised");
+  DevBuf dd, dr;
+  dd.ensure((size_t)c->n * 8);
+  dr.ensure((size_t)c->n * 8);
+  if (c->pr_prec == SUMINC_PR_FP64)
+    k_pr_to_f64<double><<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->pr_delta.as<double>(),
+                                                                 c->pr_rank.as<double>(), nullptr,
+                                                                 dd.as<double>(), dr.as<double>());
+  else
+    k_pr_to_f64<float><<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->pr_delta.as<float>(),
+                                                                c->pr_rank.as<float>(), c->pr_comp.as<float>(),
+                                                                dd.as<double>(), dr.as<double>());
+  SUMINC_HIP(hipGetLastError());
+  if (rank) SUMINC_HIP(hipMemcpyAsync(rank, dr.p, (size_t)c->n * 8, hipMemcpyDeviceToHost, c->stream));
+  if (delta) SUMINC_HIP(hipMemcpyAsync(delta, dd.p, (size_t)c->n * 8, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+}
+
+void pr_upload(suminc_ctx* c, const double* rank, const double* delta) {
+  SUMINC_REQUIRE(c->has_graph, SUMINC_E_STATE, "no graph");
+  SUMINC_REQUIRE(rank && delta, SUMINC_E_INVALID, "null state array");
+  c->pr_prec = c->prm.pr_precision;
+  pr_alloc(c);
+  DevBuf dd, dr;
+  dd.ensure((size_t)c->n * 8);
+  dr.ensure((size_t)c->n * 8);
+  SUMINC_HIP(hipMemcpyAsync(dr.p, rank, (size_t)c->n * 8, hipMemcpyHostToDevice, c->stream));
+  SUMINC_HIP(hipMemcpyAsync(dd.p, delta, (size_t)c->n * 8, hipMemcpyHostToDevice, c->stream));
+  if (c->pr_prec == SUMINC_PR_FP64)
+    k_pr_from_f64<double><<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->pr_delta.as<double>(),
+                                                                   c->pr_rank.as<double>(), nullptr,
+                                                                   dd.as<double>(), dr.as<double>());
+  else
+    k_pr_from_f64<float><<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->pr_delta.as<float>(),
+                                                                  c->pr_rank.as<float>(), c->pr_comp.as<float>(),
+                                                                  dd.as<double>(), dr.as<double>());
+  SUMINC_HIP(hipGetLastError());
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  c->pr_ready = true;
+}
+
+}  // namespace suminc

@@ -1,0 +1,33 @@
+// pr.hpp -- host-side entry points of the device algorithms (called by capi.hip).
+#pragma once
+#include "ctx.hpp"
+
+namespace suminc {
+
+// graph.hip
+void generate_rmat(suminc_ctx* c, int scale, int edge_factor, double a, double b, double cc, uint64_t seed,
+                   int wkind, uint32_t wmin, uint32_t wmax);
+void upload_csr(suminc_ctx* c, uint32_t n, uint64_t m, const uint64_t* off, const uint32_t* col, int wkind,
+                const void* w);
+void download_csr(suminc_ctx* c, uint64_t* off, uint32_t* col, void* w);
+
+// pr.hip
+void pr_alloc(suminc_ctx* c);
+void pr_init(suminc_ctx* c);
+void pr_run(suminc_ctx* c, suminc_stats* st);
+void pr_step(suminc_ctx* c, double* residual);
+void pr_download(suminc_ctx* c, double* rank, double* delta);
+void pr_upload(suminc_ctx* c, const double* rank, const double* delta);
+unsigned push_grid();
+
+// sssp.hip
+void sssp_init(suminc_ctx* c, uint32_t source);
+void sssp_run(suminc_ctx* c, suminc_stats* st);
+void sssp_download(suminc_ctx* c, uint32_t* dist, uint32_t* parent);
+
+// batch.hip
+void generate_batch(suminc_ctx* c, uint64_t seed, uint32_t batch_id, uint64_t n_del, uint64_t n_ins, uint32_t wmin,
+                    uint32_t wmax, suminc_update* ops);
+void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops);
+
+}  // namespace suminc

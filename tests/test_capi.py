@@ -1,0 +1,56 @@
+"""The C ABI library loads and exports every function include/suminc.h declares;
+argument/state errors come back as codes.  CPU only (no compute calls)."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from suminc_amd import _lib as L
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions():
+    text = (ROOT / "include" / "suminc.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(suminc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", str(L.LIB_PATH)], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (suminc_[a-z0-9_]+)", out))
+    assert set(names) <= exported
+    # every declared function has a ctypes signature in the binding
+    assert set(names) <= set(L.SIGNATURES)
+
+
+def test_default_params_follow_reference_flags():
+    p = L.Params()
+    L.lib().suminc_default_params(C.byref(p))
+    assert p.damping == pytest.approx(0.8500000238418579, abs=0)  # (double)0.85f
+    assert p.pr_precision == L.PR_FP32_KAHAN
+    assert p.max_rounds > 0
+
+
+def test_null_arguments_return_codes():
+    lib = L.lib()
+    assert lib.suminc_ctx_create(0, None, None) == L.E_INVALID
+    assert lib.suminc_ctx_destroy(None) == L.E_INVALID
+    assert lib.suminc_pr_init(None) == L.E_INVALID
+    assert lib.suminc_last_error(None) == b"null context"
+
+
+def test_context_creation_without_gpu_fails_cleanly():
+    if Path("/dev/kfd").exists():
+        pytest.skip("a GPU is present")
+    h = C.c_void_p()
+    rc = L.lib().suminc_ctx_create(0, None, C.byref(h))
+    assert rc in (L.E_HIP, L.E_INVALID)
+    assert not h.value

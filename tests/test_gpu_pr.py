@@ -1,0 +1,128 @@
+"""Delta-PageRank on the device vs the oracle's closed form / FP64 runs.
+
+Tolerances (north star: L1 <= 1e-6 vs the FP64 oracle):
+  * FP64 path: ||r - r*||_1 <= residual / (1 - d) + 1e-12 (the stop-rule bound).
+  * FP32 + Kahan path: ||r - r*||_1 <= residual / (1 - d) + 1e-6, and the
+    FP64 mass invariant holds to 1e-6.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.conftest import DAMPING
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def mass_defect(off, r, delta, d):
+    nd = np.diff(off) > 0
+    return abs(r.sum() + delta.sum() - d * r[nd].sum() - (1 - d))
+
+
+def test_golden_test_base(gpu):
+    exp = json.loads((GOLD / "test_base_expected.json").read_text())
+    e = np.loadtxt(GOLD / "test_base.txt", dtype=np.int64, comments="#")
+    g = oracle.Graph.from_edges(32, e[:, 0], e[:, 1])
+    off, col, _ = g.csr()
+    for prec in (gpu.PR_FP64, gpu.PR_FP32_KAHAN):
+        with gpu.Context(pr_precision=prec, termcheck_threshold=1e-12) as ctx:
+            ctx.upload_csr(off, col)
+            ctx.pr_init()
+            st = ctx.pr_run()
+            r, d = ctx.pr_download()
+        err = np.abs(r - np.array(exp["pagerank"])).sum()
+        tol = st["residual"] / (1 - DAMPING) + (1e-13 if prec == gpu.PR_FP64 else 2e-8)
+        assert err <= tol, (prec, err, tol)
+
+
+@pytest.mark.parametrize("scale", [12, 16])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_pagerank_vs_fp64_oracle(gpu, scale, prec):
+    g = oracle.Graph.rmat(scale, 16, seed=1)
+    off, col, _ = g.csr()
+    rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
+    T = 1e-9 * g.n  # per-vertex threshold 1e-9 (the C1 setting)
+    p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
+    with gpu.Context(pr_precision=p, termcheck_threshold=T) as ctx:
+        ctx.upload_csr(off, col)
+        ctx.pr_init()
+        st = ctx.pr_run()
+        r, d = ctx.pr_download()
+    assert st["residual"] < T
+    err = np.abs(r - rs).sum()
+    extra = 1e-12 if prec == "fp64" else 1e-6
+    assert err <= st["residual"] / (1 - DAMPING) + extra, (err, st)
+    assert mass_defect(off, r, d, DAMPING) < (1e-12 if prec == "fp64" else 1e-6)
+    assert st["edges_traversed"] > 0 and st["rounds"] >= 2
+    # the reference-style CPU run with the same stop rule lands in the same band
+    ro, do, sto = g.pr_ingress(DAMPING, T, max_rounds=100000)
+    assert np.abs(ro - rs).sum() <= sto["residual"] / (1 - DAMPING) + 1e-12
+
+
+def test_l1_target_fp32_kahan(gpu):
+    """FP32 deltas + Kahan ranks reach L1 <= 1e-6 vs the FP64 fixed point."""
+    g = oracle.Graph.rmat(16, 16, seed=3)
+    off, col, _ = g.csr()
+    rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
+    with gpu.Context(pr_precision=gpu.PR_FP32_KAHAN, termcheck_threshold=1e-8) as ctx:
+        ctx.upload_csr(off, col)
+        ctx.pr_init()
+        ctx.pr_run()
+        r, _ = ctx.pr_download()
+    assert np.abs(r - rs).sum() <= 1e-6
+
+
+def test_round_cap_reports_not_converged(gpu):
+    g = oracle.Graph.rmat(10, 16, seed=1)
+    off, col, _ = g.csr()
+    with gpu.Context(termcheck_threshold=1e-12, max_rounds=3) as ctx:
+        ctx.upload_csr(off, col)
+        ctx.pr_init()
+        with pytest.raises(gpu.SumincError) as e:
+            ctx.pr_run()
+        assert e.value.code == -6
+        st = ctx.pr_run(allow_not_converged=True)  # the state is still valid and can continue
+        assert st["rounds"] == 3
+
+
+def test_step_and_upload(gpu):
+    g = oracle.Graph.rmat(11, 16, seed=1)
+    off, col, _ = g.csr()
+    r0 = np.zeros(g.n)
+    d0 = np.full(g.n, (1 - DAMPING) / g.n)
+    with gpu.Context(pr_precision=gpu.PR_FP64, termcheck_threshold=0.0) as ctx:
+        ctx.upload_csr(off, col)
+        ctx.pr_upload(r0, d0)
+        res = [ctx.pr_step() for _ in range(3)]
+        r, d = ctx.pr_download()
+    assert res[0] == pytest.approx(1 - DAMPING, rel=1e-12)
+    assert res[0] > res[1] > res[2]
+    # three synchronous rounds with no threshold == three oracle Jacobi rounds
+    ro, do, _ = g.pr_solve(DAMPING, tol=0.0, max_rounds=3)
+    assert np.abs(r - ro).max() < 1e-15 and np.abs(d - do).max() < 1e-15
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_incremental_reseed_matches_scratch(gpu, prec):
+    g = oracle.Graph.rmat(14, 16, seed=3)
+    off, col, _ = g.csr()
+    T = 1e-10 * g.n
+    p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
+    with gpu.Context(pr_precision=p, termcheck_threshold=T) as ctx:
+        ctx.upload_csr(off, col)
+        ctx.pr_init()
+        ctx.pr_run()
+        for b in range(3):
+            n_ops = int(0.01 * g.m)
+            ops = g.gen_batch(seed=3, batch_id=b, n_del=n_ops // 2, n_ins=n_ops - n_ops // 2)
+            ctx.apply_batch(ops)
+            g, _ = g.apply_batch(ops)
+            st = ctx.pr_run()
+            r, d = ctx.pr_download()
+            rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
+            err = np.abs(r - rs).sum()
+            assert err <= st["residual"] / (1 - DAMPING) + (1e-11 if prec == "fp64" else 2e-6), (b, err, st)
