@@ -176,6 +176,7 @@ int suminc_get_stats(suminc_ctx* ctx, suminc_stats* stats);
 #define SUMINC_KERNEL_PR_PUSH 0   /* PageRank scatter: 12 B/edge + 16 B/vertex */
 #define SUMINC_KERNEL_PR_GATHER 1 /* PageRank vertex pass: 4 B/vertex + 16 B/active vertex */
 #define SUMINC_KERNEL_SSSP_PUSH 2 /* SSSP relax: 16 B/edge + 16 B/vertex */
+#define SUMINC_KERNEL_PR_PULL 3   /* PageRank CSC gather: 8 B/edge + 16 B/vertex (FP32) */
 #define SUMINC_KERNEL_COUNT 8
 typedef struct suminc_kernel_profile {
   double launches;  /* timed launches */

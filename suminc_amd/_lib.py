@@ -127,6 +127,7 @@ SIGNATURES = {
 KERNEL_PR_PUSH = 0
 KERNEL_PR_GATHER = 1
 KERNEL_SSSP_PUSH = 2
+KERNEL_PR_PULL = 3
 
 
 def lib():

@@ -84,7 +84,7 @@ void generate_batch(suminc_ctx* c, uint64_t seed, uint32_t batch_id, uint64_t n_
 }
 
 // ---------------------------------------------------------------------------
-// grouping
+// grouping (all counts stay on the device; grids are sized by the op count)
 // ---------------------------------------------------------------------------
 __global__ void k_op_keys(const suminc_update* ops, uint64_t n_ops, uint64_t* keys, uint32_t* idx) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -116,32 +116,34 @@ __global__ void k_group_heads(const uint64_t* skeys, const uint32_t* sidx, const
 }
 
 // Group arrays in structure-of-arrays form, for the CSR (row=u, col=v) or the
-// transposed view for the CSC (row=v, col=u).
+// transposed view for the CSC (row=v, col=u).  Sizes live on the device.
 struct Groups {
-  uint32_t count;
+  const unsigned* countp;   // number of groups G
   const uint32_t* row;
   const uint32_t* col;
-  const uint8_t* act;   // 0 add, 1 delete
-  const uint32_t* w;    // weight bits of adds
-  const uint8_t* ex;    // pair exists in the old graph
-  const uint32_t* ex_pre;   // exclusive prefix of ex (count+1)
-  const uint32_t* add_pre;  // exclusive prefix of (act == 0) (count+1)
-  const uint32_t* tix;  // row -> index in touched list or INF32
+  const uint8_t* act;       // 0 add, 1 delete
+  const uint32_t* w;        // weight bits of adds
+  const uint8_t* ex;        // pair exists in the old graph
+  const uint32_t* ex_pre;   // exclusive prefix of ex (G+1 valid)
+  const uint32_t* add_pre;  // exclusive prefix of (act == 0) (G+1 valid)
+  const uint32_t* tix;      // row -> index in touched list or INF32
   const uint32_t* t_gbeg;   // touched row -> first group (T+1)
 };
 
-__global__ void k_split_rec(const uint4* rec, uint32_t G, uint32_t* row, uint32_t* col, uint8_t* act, uint32_t* w) {
+__global__ void k_split_rec(const uint4* rec, const unsigned* bc, uint32_t N, uint32_t* row, uint32_t* col,
+                            uint8_t* act, uint32_t* w) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= G) return;
+  if (g >= bc[0]) return;
   uint4 r = rec[g];
   row[g] = r.x; col[g] = r.y; act[g] = (uint8_t)r.z; w[g] = r.w;
+  (void)N;
 }
 
 // existence probe in the old out-CSR; also records the old weight bits
 __global__ void k_probe(const uint32_t* off, const uint32_t* colv, const uint32_t* wv, const uint32_t* grow,
-                        const uint32_t* gcol, uint32_t G, uint8_t* ex, uint32_t* eold_w) {
+                        const uint32_t* gcol, const unsigned* bc, uint8_t* ex, uint32_t* eold_w) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= G) return;
+  if (g >= bc[0]) return;
   uint32_t u = grow[g], v = gcol[g];
   uint32_t lo = off[u], hi = off[u + 1];
   while (lo < hi) {
@@ -153,28 +155,44 @@ __global__ void k_probe(const uint32_t* off, const uint32_t* colv, const uint32_
   eold_w[g] = (found && wv) ? wv[lo] : 0u;
 }
 
-__global__ void k_flags_u32(const uint8_t* ex, const uint8_t* act, uint32_t G, uint32_t* exu, uint32_t* addu) {
+__global__ void k_flags_u32(const uint8_t* ex, const uint8_t* act, const unsigned* bc, uint32_t N, uint32_t* exu,
+                            uint32_t* addu) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g > G) return;
+  if (g > N) return;
+  const uint32_t G = bc[0];
   exu[g] = g < G ? ex[g] : 0u;
   addu[g] = g < G ? (act[g] == 0) : 0u;
 }
 
-__global__ void k_row_heads(const uint32_t* row, uint32_t G, uint8_t* head) {
+__global__ void k_row_heads(const uint32_t* row, const unsigned* bc, uint32_t N, uint8_t* head) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= G) return;
-  head[g] = (g == 0) || row[g - 1] != row[g];
+  if (g >= N) return;
+  head[g] = g < bc[0] && ((g == 0) || row[g - 1] != row[g]);
 }
 
-__global__ void k_touched(const uint32_t* row, const uint32_t* t_gbeg, uint32_t T, uint32_t G, uint32_t* t_list,
-                          uint32_t* tix, uint32_t* t_gbeg_out) {
+// touched rows: list, first group, and row -> index map (reset after the batch)
+__global__ void k_touched(const uint32_t* row, const uint32_t* t_raw, const unsigned* bc, int which, uint32_t N,
+                          uint32_t* t_list, uint32_t* tix, uint32_t* t_gbeg) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t > T) return;
-  if (t == T) { t_gbeg_out[T] = G; return; }
-  uint32_t g = t_gbeg[t];
-  t_gbeg_out[t] = g;
+  const uint32_t T = bc[which];
+  if (t > T || t > N) return;
+  if (t == T) { t_gbeg[T] = bc[0]; return; }
+  uint32_t g = t_raw[t];
+  t_gbeg[t] = g;
   t_list[t] = row[g];
   tix[row[g]] = t;
+}
+
+__global__ void k_untouch(const uint32_t* t_list, const unsigned* bc, int which, uint32_t N, uint32_t* tix) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < N && t < bc[which]) tix[t_list[t]] = INF32;
+}
+
+__global__ void k_new_m(const unsigned* bc_in, const uint32_t* ex_pre, const uint32_t* add_pre, uint32_t m,
+                        unsigned* bc) {
+  if (threadIdx.x) return;
+  const uint32_t G = bc_in[0];
+  bc[3] = m + add_pre[G] - ex_pre[G];
 }
 
 __global__ void k_new_deg(const uint32_t* off, uint32_t n, Groups gs, uint32_t* ndeg) {
@@ -190,50 +208,43 @@ __global__ void k_new_deg(const uint32_t* off, uint32_t n, Groups gs, uint32_t* 
   ndeg[v] = d;
 }
 
-// Copy surviving old edges to their new positions.
-__global__ void __launch_bounds__(kBlock) k_copy_old(uint32_t n, const uint32_t* off, const uint32_t* colv,
-                                                     const uint32_t* wv, const uint32_t* noff, uint32_t* ncol,
-                                                     uint32_t* nw, Groups gs) {
-  __shared__ uint32_t s_scan[kBlock + 1];
-  __shared__ uint32_t s_start[kBlock];
-  __shared__ uint32_t s_warp[kBlock / 64 + 1];
-  __shared__ uint32_t s_row[kBlock];
-  const uint32_t tiles = (n + kBlock - 1) / kBlock;
-  for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-    uint32_t v = t * kBlock + threadIdx.x;
-    uint32_t start = 0, len = 0;
-    if (v < n) {
-      start = off[v];
-      len = off[v + 1] - start;
-      s_row[threadIdx.x] = v;
-    }
-    block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t k, uint32_t e) {
-      uint32_t u = s_row[slot];
-      uint32_t x = colv[e];
-      uint32_t ti = gs.tix[u];
-      uint32_t dst;
-      if (ti == INF32) {
-        dst = noff[u] + k;
-      } else {
-        uint32_t b = gs.t_gbeg[ti], en = gs.t_gbeg[ti + 1];
-        uint32_t lo = b, hi = en;
-        while (lo < hi) {
-          uint32_t mid = (lo + hi) >> 1;
-          if (gs.col[mid] < x) lo = mid + 1; else hi = mid;
-        }
-        if (lo < en && gs.col[lo] == x) return;  // deleted or replaced by an add
-        dst = noff[u] + k - (gs.ex_pre[lo] - gs.ex_pre[b]) + (gs.add_pre[lo] - gs.add_pre[b]);
-      }
-      ncol[dst] = x;
-      if (nw) nw[dst] = wv[e];
-    });
+// Copy surviving old edges to their new positions.  Edge-parallel (one thread
+// per old edge, row found by binary search over the row pointers) so that hub
+// rows do not serialise on one workgroup.
+__global__ void __launch_bounds__(kBlock) k_copy_old(uint32_t n, uint32_t m, const uint32_t* off,
+                                                     const uint32_t* colv, const uint32_t* wv, const uint32_t* noff,
+                                                     uint32_t* ncol, uint32_t* nw, Groups gs) {
+  const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= m) return;
+  uint32_t lo = 0, hi = n - 1;  // last row with off[row] <= e
+  while (lo < hi) {
+    uint32_t mid = lo + ((hi - lo + 1) >> 1);
+    if (off[mid] <= e) lo = mid; else hi = mid - 1;
   }
+  const uint32_t u = lo, k = e - off[u];
+  const uint32_t x = colv[e];
+  const uint32_t ti = gs.tix[u];
+  uint32_t dst;
+  if (ti == INF32) {
+    dst = noff[u] + k;
+  } else {
+    uint32_t b = gs.t_gbeg[ti], en = gs.t_gbeg[ti + 1];
+    uint32_t l = b, h = en;
+    while (l < h) {
+      uint32_t mid = (l + h) >> 1;
+      if (gs.col[mid] < x) l = mid + 1; else h = mid;
+    }
+    if (l < en && gs.col[l] == x) return;  // deleted or replaced by an add
+    dst = noff[u] + k - (gs.ex_pre[l] - gs.ex_pre[b]) + (gs.add_pre[l] - gs.add_pre[b]);
+  }
+  ncol[dst] = x;
+  if (nw) nw[dst] = wv[e];
 }
 
 __global__ void k_place_adds(const uint32_t* off, const uint32_t* colv, const uint32_t* noff, uint32_t* ncol,
                              uint32_t* nw, Groups gs) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= gs.count || gs.act[g] != 0) return;
+  if (g >= *gs.countp || gs.act[g] != 0) return;
   uint32_t u = gs.row[g], v = gs.col[g];
   uint32_t ti = gs.tix[u];
   uint32_t b = gs.t_gbeg[ti];
@@ -250,19 +261,19 @@ __global__ void k_place_adds(const uint32_t* off, const uint32_t* colv, const ui
   if (nw) nw[dst] = gs.w[g];
 }
 
-__global__ void k_transpose_group_keys(const uint32_t* row, const uint32_t* col, uint32_t G, uint64_t* keys,
-                                       uint32_t* idx) {
+__global__ void k_transpose_group_keys(const uint32_t* row, const uint32_t* col, const unsigned* bc, uint32_t N,
+                                       uint64_t* keys, uint32_t* idx) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= G) return;
-  keys[g] = ((uint64_t)col[g] << 32) | row[g];
+  if (g >= N) return;
+  keys[g] = g < bc[0] ? (((uint64_t)col[g] << 32) | row[g]) : ~0ull;
   idx[g] = g;
 }
 
-__global__ void k_gather_groups(const uint32_t* sidx, uint32_t G, const uint32_t* row, const uint32_t* col,
+__global__ void k_gather_groups(const uint32_t* sidx, const unsigned* bc, const uint32_t* row, const uint32_t* col,
                                 const uint8_t* act, const uint32_t* w, const uint8_t* ex, uint32_t* trow,
                                 uint32_t* tcol, uint8_t* tact, uint32_t* tw, uint8_t* tex) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= G) return;
+  if (g >= bc[0]) return;
   uint32_t s = sidx[g];
   trow[g] = col[s];
   tcol[g] = row[s];
@@ -275,13 +286,14 @@ __global__ void k_gather_groups(const uint32_t* sidx, uint32_t G, const uint32_t
 // PageRank reseed: list (u, type*d*r_u/deg) for touched u over the current CSR
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ void k_reseed_list(const uint32_t* t_list, uint32_t Tn, const uint32_t* off, const T* rank,
-                              const float* comp, T d, T type, uint32_t* wl_v, T* wl_c, uint4* heavy, Counters* cnt) {
+__global__ void k_reseed_list(const uint32_t* t_list, const unsigned* bc, uint32_t N, const uint32_t* off,
+                              const T* rank, const float* comp, T d, T type, uint32_t* wl_v, T* wl_c, uint4* heavy,
+                              Counters* cnt) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   bool light = false;
   uint32_t u = 0;
   T c = T(0);
-  if (t < Tn) {
+  if (t < N && t < bc[1]) {
     u = t_list[t];
     uint32_t deg = off[u + 1] - off[u];
     T r = rank[u];
@@ -358,12 +370,13 @@ __global__ void k_reset_lists(Counters* cnt) {
 // ---------------------------------------------------------------------------
 // roots: deleted or re-weighted existing pair (u,v) with parent(v) == u
 __global__ void k_sssp_roots(const uint32_t* grow, const uint32_t* gcol, const uint8_t* gact, const uint32_t* gw,
-                             const uint8_t* ex, const uint32_t* eold_w, uint32_t G, const unsigned long long* dp,
-                             uint32_t src, uint32_t* rbits, uint32_t* rlist, Counters* cnt) {
+                             const uint8_t* ex, const uint32_t* eold_w, const unsigned* bc, uint32_t N,
+                             const unsigned long long* dp, uint32_t src, uint32_t* rbits, uint32_t* rlist,
+                             Counters* cnt) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   bool add = false;
   uint32_t v = 0;
-  if (g < G && ex[g]) {
+  if (g < N && g < bc[0] && ex[g]) {
     uint32_t u = grow[g];
     v = gcol[g];
     bool changed = gact[g] == 1 || gw[g] != eold_w[g];
@@ -459,12 +472,12 @@ __global__ void __launch_bounds__(kBlock) k_reset_pull(const uint32_t* ioff, con
 }
 
 // Seeds SSSP list 0 with every finite reset vertex.
-__global__ void k_reset_seed(const uint32_t* rlist, const Counters* cnt_ro, const unsigned long long* dp,
-                             uint32_t* wl0, uint32_t* bits0, Counters* cnt) {
+__global__ void k_reset_seed(const uint32_t* rlist, const unsigned long long* dp, uint32_t* wl0, uint32_t* bits0,
+                             Counters* cnt) {
   unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   bool add = false;
   uint32_t x = 0;
-  if (i < cnt_ro->aux[4]) {
+  if (i < cnt->aux[4]) {
     x = rlist[i];
     if ((uint32_t)(dp[x] >> 32) != INF32) {
       uint32_t bit = 1u << (x & 31);
@@ -477,12 +490,12 @@ __global__ void k_reset_seed(const uint32_t* rlist, const Counters* cnt_ro, cons
 
 // relax every add (new pair or new weight) from its current source distance
 __global__ void k_insert_relax(const uint32_t* grow, const uint32_t* gcol, const uint8_t* gact, const uint32_t* gw,
-                               uint32_t G, int unit, unsigned long long* dp, uint32_t* wl0, uint32_t* bits0,
-                               Counters* cnt) {
+                               const unsigned* bc, uint32_t N, int unit, unsigned long long* dp, uint32_t* wl0,
+                               uint32_t* bits0, Counters* cnt) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   bool add = false;
   uint32_t v = 0;
-  if (g < G && gact[g] == 0) {
+  if (g < N && g < bc[0] && gact[g] == 0) {
     uint32_t u = grow[g];
     v = gcol[g];
     uint32_t du = (uint32_t)(dp[u] >> 32);
@@ -525,6 +538,10 @@ __global__ void k_undone(Counters* cnt) {
 __global__ void k_batch_zero(Counters* cnt) {
   if (threadIdx.x == 0) cnt->aux[5] = 0;
 }
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
 
 // ---------------------------------------------------------------------------
 // host orchestration
@@ -537,44 +554,44 @@ static void cub_call(suminc_ctx* c, DevBuf& tmp, F&& f) {
   SUMINC_HIP(f(tmp.p, bytes));
 }
 
-// rebuilds one adjacency direction (off/col/w) given its group view
-static void merge_direction(suminc_ctx* c, DevBuf& off, DevBuf& col, DevBuf& w, const Groups& gs_in, uint32_t Tn,
-                            uint64_t new_m) {
+// Rebuilds one adjacency direction into the workspace buffers, then swaps them
+// with the live arrays (the old arrays become next batch's workspace).
+static void merge_direction(suminc_ctx* c, DevBuf& off, DevBuf& col, DevBuf& w, DevBuf& noff, DevBuf& ncol,
+                            DevBuf& nw, const Groups& gs, uint32_t N, uint64_t cap_m) {
   const uint32_t n = c->n;
-  Groups gs = gs_in;
-  DevBuf ndeg, noff, ncol, nw;
-  ndeg.ensure(((size_t)n + 1) * 4);
+  BatchWs& b = c->bw;
   noff.ensure(((size_t)n + 1) * 4);
-  ncol.ensure(new_m * 4 + 4);
-  if (c->wkind) nw.ensure(new_m * 4 + 4);
-  k_new_deg<<<grid_for((uint64_t)n + 1), kBlock, 0, c->stream>>>(off.as<uint32_t>(), n, gs, ndeg.as<uint32_t>());
-  cub_call(c, c->tmp, [&](void* p, size_t& b) {
-    return rocprim::exclusive_scan(p, b, ndeg.as<uint32_t>(), noff.as<uint32_t>(), 0u, (size_t)n + 1,
+  ncol.ensure(cap_m * 4 + 4);
+  if (c->wkind) nw.ensure(cap_m * 4 + 4);
+  k_new_deg<<<grid_for((uint64_t)n + 1), kBlock, 0, c->stream>>>(off.as<uint32_t>(), n, gs, b.ndeg.as<uint32_t>());
+  cub_call(c, c->tmp, [&](void* p, size_t& bytes) {
+    return rocprim::exclusive_scan(p, bytes, b.ndeg.as<uint32_t>(), noff.as<uint32_t>(), 0u, (size_t)n + 1,
                                    rocprim::plus<uint32_t>(), c->stream);
   });
-  k_copy_old<<<4096, kBlock, 0, c->stream>>>(n, off.as<uint32_t>(), col.as<uint32_t>(),
-                                            c->wkind ? w.as<uint32_t>() : nullptr, noff.as<uint32_t>(),
-                                            ncol.as<uint32_t>(), c->wkind ? nw.as<uint32_t>() : nullptr, gs);
-  if (gs.count)
-    k_place_adds<<<grid_for(gs.count), kBlock, 0, c->stream>>>(off.as<uint32_t>(), col.as<uint32_t>(),
-                                                               noff.as<uint32_t>(), ncol.as<uint32_t>(),
-                                                               c->wkind ? nw.as<uint32_t>() : nullptr, gs);
+  if (c->m)
+    k_copy_old<<<grid_for(c->m), kBlock, 0, c->stream>>>(n, (uint32_t)c->m, off.as<uint32_t>(), col.as<uint32_t>(),
+                                                         c->wkind ? w.as<uint32_t>() : nullptr, noff.as<uint32_t>(),
+                                                         ncol.as<uint32_t>(), c->wkind ? nw.as<uint32_t>() : nullptr,
+                                                         gs);
+  k_place_adds<<<grid_for(N), kBlock, 0, c->stream>>>(off.as<uint32_t>(), col.as<uint32_t>(), noff.as<uint32_t>(),
+                                                      ncol.as<uint32_t>(), c->wkind ? nw.as<uint32_t>() : nullptr,
+                                                      gs);
   SUMINC_HIP(hipGetLastError());
-  SUMINC_HIP(hipStreamSynchronize(c->stream));
   off.swap(noff);
   col.swap(ncol);
   if (c->wkind) w.swap(nw);
-  (void)Tn;
 }
 
 template <class T>
-static void pr_reseed(suminc_ctx* c, const uint32_t* t_list, uint32_t Tn, double type) {
+static void pr_reseed(suminc_ctx* c, uint32_t N, double type) {
   Counters* cnt = dev_counters(c);
+  BatchWs& b = c->bw;
   k_reset_lists<<<1, 64, 0, c->stream>>>(cnt);
-  k_reseed_list<T><<<grid_for(Tn), kBlock, 0, c->stream>>>(t_list, Tn, c->out_off.as<uint32_t>(), c->pr_rank.as<T>(),
-                                                           c->pr_comp.as<float>(), (T)c->prm.damping, (T)type,
-                                                           c->wl0.as<uint32_t>(), c->wl1.as<T>(), c->heavy.as<uint4>(),
-                                                           cnt);
+  k_reseed_list<T><<<grid_for(N), kBlock, 0, c->stream>>>(b.t_list.as<uint32_t>(), b.bc.as<unsigned>(), N,
+                                                          c->out_off.as<uint32_t>(), c->pr_rank.as<T>(),
+                                                          c->pr_comp.as<float>(), (T)c->prm.damping, (T)type,
+                                                          c->wl0.as<uint32_t>(), c->wl1.as<T>(), c->heavy.as<uint4>(),
+                                                          cnt);
   k_reseed_push<T><<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
                                                           c->pr_delta.as<T>(), c->wl0.as<uint32_t>(), c->wl1.as<T>(),
                                                           c->heavy.as<uint4>(), cnt);
@@ -582,225 +599,215 @@ static void pr_reseed(suminc_ctx* c, const uint32_t* t_list, uint32_t Tn, double
   SUMINC_HIP(hipGetLastError());
 }
 
+// Builds the per-row group view (heads, prefix counts, touched rows) for one
+// direction; `which` selects the touched-row count slot in bc.
+static void build_view(suminc_ctx* c, uint32_t N, const uint32_t* row, const uint8_t* ex, const uint8_t* act,
+                       DevBuf& exu, DevBuf& addu, DevBuf& expre, DevBuf& addpre, DevBuf& head, DevBuf& t_raw,
+                       DevBuf& t_list, DevBuf& t_gbeg, DevBuf& tix, int which) {
+  BatchWs& b = c->bw;
+  const unsigned* bc = b.bc.as<unsigned>();
+  k_flags_u32<<<grid_for((uint64_t)N + 1), kBlock, 0, c->stream>>>(ex, act, bc, N, exu.as<uint32_t>(),
+                                                                   addu.as<uint32_t>());
+  cub_call(c, c->tmp, [&](void* p, size_t& bytes) {
+    return rocprim::exclusive_scan(p, bytes, exu.as<uint32_t>(), expre.as<uint32_t>(), 0u, (size_t)N + 1,
+                                   rocprim::plus<uint32_t>(), c->stream);
+  });
+  cub_call(c, c->tmp, [&](void* p, size_t& bytes) {
+    return rocprim::exclusive_scan(p, bytes, addu.as<uint32_t>(), addpre.as<uint32_t>(), 0u, (size_t)N + 1,
+                                   rocprim::plus<uint32_t>(), c->stream);
+  });
+  k_row_heads<<<grid_for(N), kBlock, 0, c->stream>>>(row, bc, N, head.as<uint8_t>());
+  cub_call(c, c->tmp, [&](void* p, size_t& bytes) {
+    return rocprim::select(p, bytes, rocprim::counting_iterator<uint32_t>(0), head.as<uint8_t>(),
+                           t_raw.as<uint32_t>(), b.bc.as<unsigned>() + which, (size_t)N, c->stream);
+  });
+  k_touched<<<grid_for((uint64_t)N + 1), kBlock, 0, c->stream>>>(row, t_raw.as<uint32_t>(), bc, which, N,
+                                                                 t_list.as<uint32_t>(), tix.as<uint32_t>(),
+                                                                 t_gbeg.as<uint32_t>());
+}
+
+static void ensure_workspace(suminc_ctx* c, uint64_t n_ops) {
+  BatchWs& b = c->bw;
+  const size_t N = n_ops;
+  b.ops.ensure(N * sizeof(suminc_update));
+  b.keys_a.ensure(N * 8); b.keys_b.ensure(N * 8); b.idx_a.ensure(N * 4); b.idx_b.ensure(N * 4);
+  b.rec.ensure(N * 16); b.grec.ensure(N * 16); b.flags.ensure(N);
+  for (DevBuf* d : {&b.g_row, &b.g_col, &b.g_w, &b.g_eoldw, &b.tg_row, &b.tg_col, &b.tg_w}) d->ensure(N * 4 + 4);
+  for (DevBuf* d : {&b.g_act, &b.g_ex, &b.head, &b.tg_act, &b.tg_ex, &b.thead}) d->ensure(N + 1);
+  for (DevBuf* d : {&b.g_exu, &b.g_addu, &b.g_expre, &b.g_addpre, &b.t_raw, &b.t_list, &b.t_gbeg, &b.tg_exu,
+                    &b.tg_addu, &b.tg_expre, &b.tg_addpre, &b.tt_raw, &b.tt_list, &b.tt_gbeg})
+    d->ensure((N + 1) * 4);
+  b.bc.ensure(64);
+  b.ndeg.ensure(((size_t)c->n + 1) * 4);
+  if (b.tix_n != c->n || !b.tix.p) {
+    b.tix.ensure((size_t)c->n * 4);
+    b.ttix.ensure((size_t)c->n * 4);
+    k_fill_u32<<<grid_for(c->n), kBlock, 0, c->stream>>>(b.tix.as<uint32_t>(), c->n, INF32);
+    k_fill_u32<<<grid_for(c->n), kBlock, 0, c->stream>>>(b.ttix.as<uint32_t>(), c->n, INF32);
+    b.tix_n = c->n;
+  }
+}
+
 void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops) {
   SUMINC_REQUIRE(c->has_graph, SUMINC_E_STATE, "no graph");
   SUMINC_REQUIRE(ops || n_ops == 0, SUMINC_E_INVALID, "null ops");
-  SUMINC_REQUIRE(n_ops < 0xFFFFFFFFull, SUMINC_E_TOO_LARGE, "batch too large");
+  SUMINC_REQUIRE(n_ops < 0x7FFFFFFFull, SUMINC_E_TOO_LARGE, "batch too large");
   for (uint64_t i = 0; i < n_ops; ++i) {
     SUMINC_REQUIRE(ops[i].type == SUMINC_OP_ADD || ops[i].type == SUMINC_OP_DEL, SUMINC_E_INVALID,
                    "op type must be SUMINC_OP_ADD or SUMINC_OP_DEL (line " + std::to_string(i) + ")");
     SUMINC_REQUIRE(ops[i].u < c->n && ops[i].v < c->n, SUMINC_E_INVALID,
                    "vertex id out of range (line " + std::to_string(i) + ")");
   }
+  const uint64_t cap_m = c->m + n_ops;
+  SUMINC_REQUIRE(cap_m < 0xFFFFFFFFull, SUMINC_E_TOO_LARGE, "edge count could exceed 2^32-1");
   c->stats = suminc_stats{};
   if (n_ops == 0) return;
   const uint32_t n = c->n;
+  const uint32_t N = (uint32_t)n_ops;
   Counters* cnt = dev_counters(c);
+  ensure_workspace(c, n_ops);
+  // heavy-chunk list sized for the largest graph this batch can produce
+  if (c->pr_ready || c->sssp_ready) c->heavy.ensure((cap_m / kChunk + cap_m / kHeavyDeg + 2) * sizeof(uint4));
+  BatchWs& b = c->bw;
+  unsigned* bc = b.bc.as<unsigned>();
   k_batch_zero<<<1, 64, 0, c->stream>>>(cnt);
   // An SSSP state that was initialised but never run is still the initial state:
   // it is simply re-initialised on the new graph instead of repaired.
   const bool sssp_repair = c->sssp_ready && !c->sssp_pending;
   const bool sssp_reinit = c->sssp_ready && c->sssp_pending;
 
-  // 1. upload + sort + group
-  DevBuf d_ops, keys_a, keys_b, idx_a, idx_b, rec, flags, grec, gcount;
-  d_ops.ensure(n_ops * sizeof(suminc_update));
-  SUMINC_HIP(hipMemcpyAsync(d_ops.p, ops, n_ops * sizeof(suminc_update), hipMemcpyHostToDevice, c->stream));
-  keys_a.ensure(n_ops * 8); keys_b.ensure(n_ops * 8);
-  idx_a.ensure(n_ops * 4); idx_b.ensure(n_ops * 4);
-  rec.ensure(n_ops * 16); grec.ensure(n_ops * 16); flags.ensure(n_ops); gcount.ensure(8);
-  k_op_keys<<<grid_for(n_ops), kBlock, 0, c->stream>>>(d_ops.as<suminc_update>(), n_ops, keys_a.as<uint64_t>(),
-                                                       idx_a.as<uint32_t>());
-  cub_call(c, c->tmp, [&](void* p, size_t& b) {
-    return rocprim::radix_sort_pairs(p, b, keys_a.as<uint64_t>(), keys_b.as<uint64_t>(), idx_a.as<uint32_t>(),
-                                     idx_b.as<uint32_t>(), (size_t)n_ops, 0, 64, c->stream);
+  // 1. upload + stable sort by (u,v) + one group per pair
+  SUMINC_HIP(hipMemcpyAsync(b.ops.p, ops, n_ops * sizeof(suminc_update), hipMemcpyHostToDevice, c->stream));
+  k_op_keys<<<grid_for(N), kBlock, 0, c->stream>>>(b.ops.as<suminc_update>(), N, b.keys_a.as<uint64_t>(),
+                                                   b.idx_a.as<uint32_t>());
+  cub_call(c, c->tmp, [&](void* p, size_t& bytes) {
+    return rocprim::radix_sort_pairs(p, bytes, b.keys_a.as<uint64_t>(), b.keys_b.as<uint64_t>(),
+                                     b.idx_a.as<uint32_t>(), b.idx_b.as<uint32_t>(), (size_t)N, 0, 64, c->stream);
   });
-  k_group_heads<<<grid_for(n_ops), kBlock, 0, c->stream>>>(keys_b.as<uint64_t>(), idx_b.as<uint32_t>(),
-                                                           d_ops.as<suminc_update>(), n_ops, c->wkind,
-                                                           rec.as<uint4>(), flags.as<uint8_t>());
-  cub_call(c, c->tmp, [&](void* p, size_t& b) {
-    return rocprim::select(p, b, rec.as<uint4>(), flags.as<uint8_t>(), grec.as<uint4>(),
-                           gcount.as<unsigned int>(), (size_t)n_ops, c->stream);
+  k_group_heads<<<grid_for(N), kBlock, 0, c->stream>>>(b.keys_b.as<uint64_t>(), b.idx_b.as<uint32_t>(),
+                                                       b.ops.as<suminc_update>(), N, c->wkind, b.rec.as<uint4>(),
+                                                       b.flags.as<uint8_t>());
+  cub_call(c, c->tmp, [&](void* p, size_t& bytes) {
+    return rocprim::select(p, bytes, b.rec.as<uint4>(), b.flags.as<uint8_t>(), b.grec.as<uint4>(), bc, (size_t)N,
+                           c->stream);
   });
-  unsigned int G = 0;
-  SUMINC_HIP(hipMemcpyAsync(&G, gcount.p, 4, hipMemcpyDeviceToHost, c->stream));
-  SUMINC_HIP(hipStreamSynchronize(c->stream));
 
-  // 2. group arrays (CSR view) + existence probe
-  DevBuf g_row, g_col, g_act, g_w, g_ex, g_eoldw, g_exu, g_addu, g_expre, g_addpre, head, t_gbeg_raw, t_count,
-      t_list, t_gbeg, tix;
-  g_row.ensure((size_t)G * 4 + 4); g_col.ensure((size_t)G * 4 + 4); g_act.ensure(G + 1); g_w.ensure((size_t)G * 4 + 4);
-  g_ex.ensure(G + 1); g_eoldw.ensure((size_t)G * 4 + 4);
-  g_exu.ensure(((size_t)G + 1) * 4); g_addu.ensure(((size_t)G + 1) * 4);
-  g_expre.ensure(((size_t)G + 1) * 4); g_addpre.ensure(((size_t)G + 1) * 4);
-  head.ensure(G + 1); t_gbeg_raw.ensure(((size_t)G + 1) * 4); t_count.ensure(8);
-  k_split_rec<<<grid_for(G), kBlock, 0, c->stream>>>(grec.as<uint4>(), G, g_row.as<uint32_t>(), g_col.as<uint32_t>(),
-                                                     g_act.as<uint8_t>(), g_w.as<uint32_t>());
-  k_probe<<<grid_for(G), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
-                                                 c->wkind ? c->out_w.as<uint32_t>() : nullptr, g_row.as<uint32_t>(),
-                                                 g_col.as<uint32_t>(), G, g_ex.as<uint8_t>(), g_eoldw.as<uint32_t>());
-  k_flags_u32<<<grid_for((uint64_t)G + 1), kBlock, 0, c->stream>>>(g_ex.as<uint8_t>(), g_act.as<uint8_t>(), G,
-                                                                   g_exu.as<uint32_t>(), g_addu.as<uint32_t>());
-  cub_call(c, c->tmp, [&](void* p, size_t& b) {
-    return rocprim::exclusive_scan(p, b, g_exu.as<uint32_t>(), g_expre.as<uint32_t>(), 0u, (size_t)G + 1,
-                                   rocprim::plus<uint32_t>(), c->stream);
-  });
-  cub_call(c, c->tmp, [&](void* p, size_t& b) {
-    return rocprim::exclusive_scan(p, b, g_addu.as<uint32_t>(), g_addpre.as<uint32_t>(), 0u, (size_t)G + 1,
-                                   rocprim::plus<uint32_t>(), c->stream);
-  });
-  k_row_heads<<<grid_for(G), kBlock, 0, c->stream>>>(g_row.as<uint32_t>(), G, head.as<uint8_t>());
-  cub_call(c, c->tmp, [&](void* p, size_t& b) {
-    return rocprim::select(p, b, rocprim::counting_iterator<uint32_t>(0), head.as<uint8_t>(),
-                           t_gbeg_raw.as<uint32_t>(), t_count.as<unsigned int>(), (size_t)G, c->stream);
-  });
-  unsigned int Tn = 0;
-  uint32_t ex_total = 0, add_total = 0;
-  SUMINC_HIP(hipMemcpyAsync(&Tn, t_count.p, 4, hipMemcpyDeviceToHost, c->stream));
-  SUMINC_HIP(hipMemcpyAsync(&ex_total, g_expre.as<uint32_t>() + G, 4, hipMemcpyDeviceToHost, c->stream));
-  SUMINC_HIP(hipMemcpyAsync(&add_total, g_addpre.as<uint32_t>() + G, 4, hipMemcpyDeviceToHost, c->stream));
-  SUMINC_HIP(hipStreamSynchronize(c->stream));
-  t_list.ensure((size_t)Tn * 4 + 4);
-  t_gbeg.ensure(((size_t)Tn + 1) * 4);
-  tix.ensure((size_t)n * 4);
-  SUMINC_HIP(hipMemsetAsync(tix.p, 0xFF, (size_t)n * 4, c->stream));
-  k_touched<<<grid_for((uint64_t)Tn + 1), kBlock, 0, c->stream>>>(g_row.as<uint32_t>(), t_gbeg_raw.as<uint32_t>(),
-                                                                  Tn, G, t_list.as<uint32_t>(), tix.as<uint32_t>(),
-                                                                  t_gbeg.as<uint32_t>());
+  // 2. CSR view: existence probe, prefix counts, touched rows
+  k_split_rec<<<grid_for(N), kBlock, 0, c->stream>>>(b.grec.as<uint4>(), bc, N, b.g_row.as<uint32_t>(),
+                                                     b.g_col.as<uint32_t>(), b.g_act.as<uint8_t>(),
+                                                     b.g_w.as<uint32_t>());
+  k_probe<<<grid_for(N), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
+                                                 c->wkind ? c->out_w.as<uint32_t>() : nullptr, b.g_row.as<uint32_t>(),
+                                                 b.g_col.as<uint32_t>(), bc, b.g_ex.as<uint8_t>(),
+                                                 b.g_eoldw.as<uint32_t>());
+  build_view(c, N, b.g_row.as<uint32_t>(), b.g_ex.as<uint8_t>(), b.g_act.as<uint8_t>(), b.g_exu, b.g_addu,
+             b.g_expre, b.g_addpre, b.head, b.t_raw, b.t_list, b.t_gbeg, b.tix, 1);
+  k_new_m<<<1, 64, 0, c->stream>>>(bc, b.g_expre.as<uint32_t>(), b.g_addpre.as<uint32_t>(), (uint32_t)c->m, bc);
   SUMINC_HIP(hipGetLastError());
-  const uint64_t new_m = c->m + add_total - ex_total;
-  SUMINC_REQUIRE(new_m < 0xFFFFFFFFull, SUMINC_E_TOO_LARGE, "edge count would exceed 2^32-1");
 
   // 3. PageRank reseed -1 on the old out-edges of touched sources
-  unsigned long long reseed_edges = 0;
   if (c->pr_ready) {
-    if (c->pr_prec == SUMINC_PR_FP64) pr_reseed<double>(c, t_list.as<uint32_t>(), Tn, -1.0);
-    else pr_reseed<float>(c, t_list.as<uint32_t>(), Tn, -1.0);
+    if (c->pr_prec == SUMINC_PR_FP64) pr_reseed<double>(c, N, -1.0);
+    else pr_reseed<float>(c, N, -1.0);
   }
 
   // 4. SSSP: reset roots + subtree walk on the old graph
-  DevBuf rbits, rlist;
   if (sssp_repair) {
     const size_t words = ((size_t)n + 31) / 32 + 1;
-    rbits.ensure(words * 4);
-    rlist.ensure((size_t)n * 4 + 4);
-    SUMINC_HIP(hipMemsetAsync(rbits.p, 0, words * 4, c->stream));
+    b.rbits.ensure(words * 4);
+    b.rlist.ensure((size_t)n * 4 + 4);
+    SUMINC_HIP(hipMemsetAsync(b.rbits.p, 0, words * 4, c->stream));
     k_sssp_repair_prepare<<<1, 64, 0, c->stream>>>(cnt);
-    k_sssp_roots<<<grid_for(G), kBlock, 0, c->stream>>>(g_row.as<uint32_t>(), g_col.as<uint32_t>(), g_act.as<uint8_t>(),
-                                                        g_w.as<uint32_t>(), g_ex.as<uint8_t>(), g_eoldw.as<uint32_t>(),
-                                                        G, c->sssp_dp.as<unsigned long long>(), c->sssp_src,
-                                                        rbits.as<uint32_t>(), rlist.as<uint32_t>(), cnt);
+    k_sssp_roots<<<grid_for(N), kBlock, 0, c->stream>>>(b.g_row.as<uint32_t>(), b.g_col.as<uint32_t>(),
+                                                        b.g_act.as<uint8_t>(), b.g_w.as<uint32_t>(),
+                                                        b.g_ex.as<uint8_t>(), b.g_eoldw.as<uint32_t>(), bc, N,
+                                                        c->sssp_dp.as<unsigned long long>(), c->sssp_src,
+                                                        b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(), cnt);
     k_reset_start<<<1, 64, 0, c->stream>>>(cnt);
+    int group = 4;
     while (true) {
-      for (int i = 0; i < 8; ++i) {
+      for (int i = 0; i < group; ++i) {
         k_reset_round<<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
-                                                             c->sssp_dp.as<unsigned long long>(), rbits.as<uint32_t>(),
-                                                             rlist.as<uint32_t>(), cnt);
+                                                             c->sssp_dp.as<unsigned long long>(),
+                                                             b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(), cnt);
         k_reset_end<<<1, 64, 0, c->stream>>>(cnt);
       }
       SUMINC_HIP(hipGetLastError());
       read_counters(c);
       if (c->host_counters->done) break;
+      group = 16;
     }
     c->stats.reserved[0] = (double)c->host_counters->aux[4];  // reset vertex count
   }
 
-  // 5. topology: CSR then CSC
+  // 5. topology: CSR, then the CSC from the transposed groups
   Groups gs;
-  gs.count = G;
-  gs.row = g_row.as<uint32_t>(); gs.col = g_col.as<uint32_t>(); gs.act = g_act.as<uint8_t>();
-  gs.w = g_w.as<uint32_t>(); gs.ex = g_ex.as<uint8_t>();
-  gs.ex_pre = g_expre.as<uint32_t>(); gs.add_pre = g_addpre.as<uint32_t>();
-  gs.tix = tix.as<uint32_t>(); gs.t_gbeg = t_gbeg.as<uint32_t>();
-  merge_direction(c, c->out_off, c->out_col, c->out_w, gs, Tn, new_m);
-  {
-    DevBuf tk_a, tk_b, ti_a, ti_b, tg_row, tg_col, tg_act, tg_w, tg_ex, tg_exu, tg_addu, tg_expre, tg_addpre, thead,
-        tt_raw, tt_count, tt_list, tt_gbeg, ttix;
-    tk_a.ensure((size_t)G * 8 + 8); tk_b.ensure((size_t)G * 8 + 8); ti_a.ensure((size_t)G * 4 + 4);
-    ti_b.ensure((size_t)G * 4 + 4);
-    tg_row.ensure((size_t)G * 4 + 4); tg_col.ensure((size_t)G * 4 + 4); tg_act.ensure(G + 1);
-    tg_w.ensure((size_t)G * 4 + 4); tg_ex.ensure(G + 1);
-    tg_exu.ensure(((size_t)G + 1) * 4); tg_addu.ensure(((size_t)G + 1) * 4);
-    tg_expre.ensure(((size_t)G + 1) * 4); tg_addpre.ensure(((size_t)G + 1) * 4);
-    thead.ensure(G + 1); tt_raw.ensure(((size_t)G + 1) * 4); tt_count.ensure(8);
-    k_transpose_group_keys<<<grid_for(G), kBlock, 0, c->stream>>>(g_row.as<uint32_t>(), g_col.as<uint32_t>(), G,
-                                                                  tk_a.as<uint64_t>(), ti_a.as<uint32_t>());
-    cub_call(c, c->tmp, [&](void* p, size_t& b) {
-      return rocprim::radix_sort_pairs(p, b, tk_a.as<uint64_t>(), tk_b.as<uint64_t>(), ti_a.as<uint32_t>(),
-                                       ti_b.as<uint32_t>(), (size_t)G, 0, 64, c->stream);
-    });
-    k_gather_groups<<<grid_for(G), kBlock, 0, c->stream>>>(ti_b.as<uint32_t>(), G, g_row.as<uint32_t>(),
-                                                           g_col.as<uint32_t>(), g_act.as<uint8_t>(),
-                                                           g_w.as<uint32_t>(), g_ex.as<uint8_t>(),
-                                                           tg_row.as<uint32_t>(), tg_col.as<uint32_t>(),
-                                                           tg_act.as<uint8_t>(), tg_w.as<uint32_t>(),
-                                                           tg_ex.as<uint8_t>());
-    k_flags_u32<<<grid_for((uint64_t)G + 1), kBlock, 0, c->stream>>>(tg_ex.as<uint8_t>(), tg_act.as<uint8_t>(), G,
-                                                                     tg_exu.as<uint32_t>(), tg_addu.as<uint32_t>());
-    cub_call(c, c->tmp, [&](void* p, size_t& b) {
-      return rocprim::exclusive_scan(p, b, tg_exu.as<uint32_t>(), tg_expre.as<uint32_t>(), 0u, (size_t)G + 1,
-                                     rocprim::plus<uint32_t>(), c->stream);
-    });
-    cub_call(c, c->tmp, [&](void* p, size_t& b) {
-      return rocprim::exclusive_scan(p, b, tg_addu.as<uint32_t>(), tg_addpre.as<uint32_t>(), 0u, (size_t)G + 1,
-                                     rocprim::plus<uint32_t>(), c->stream);
-    });
-    k_row_heads<<<grid_for(G), kBlock, 0, c->stream>>>(tg_row.as<uint32_t>(), G, thead.as<uint8_t>());
-    cub_call(c, c->tmp, [&](void* p, size_t& b) {
-      return rocprim::select(p, b, rocprim::counting_iterator<uint32_t>(0), thead.as<uint8_t>(), tt_raw.as<uint32_t>(),
-                             tt_count.as<unsigned int>(), (size_t)G, c->stream);
-    });
-    unsigned int Tt = 0;
-    SUMINC_HIP(hipMemcpyAsync(&Tt, tt_count.p, 4, hipMemcpyDeviceToHost, c->stream));
-    SUMINC_HIP(hipStreamSynchronize(c->stream));
-    tt_list.ensure((size_t)Tt * 4 + 4);
-    tt_gbeg.ensure(((size_t)Tt + 1) * 4);
-    ttix.ensure((size_t)n * 4);
-    SUMINC_HIP(hipMemsetAsync(ttix.p, 0xFF, (size_t)n * 4, c->stream));
-    k_touched<<<grid_for((uint64_t)Tt + 1), kBlock, 0, c->stream>>>(tg_row.as<uint32_t>(), tt_raw.as<uint32_t>(), Tt,
-                                                                    G, tt_list.as<uint32_t>(), ttix.as<uint32_t>(),
-                                                                    tt_gbeg.as<uint32_t>());
-    Groups gt;
-    gt.count = G;
-    gt.row = tg_row.as<uint32_t>(); gt.col = tg_col.as<uint32_t>(); gt.act = tg_act.as<uint8_t>();
-    gt.w = tg_w.as<uint32_t>(); gt.ex = tg_ex.as<uint8_t>();
-    gt.ex_pre = tg_expre.as<uint32_t>(); gt.add_pre = tg_addpre.as<uint32_t>();
-    gt.tix = ttix.as<uint32_t>(); gt.t_gbeg = tt_gbeg.as<uint32_t>();
-    merge_direction(c, c->in_off, c->in_col, c->in_w, gt, Tt, new_m);
-  }
-  c->m = new_m;
-  // state buffers sized by m (heavy chunk list) may need to grow
-  if (c->pr_ready) c->heavy.ensure((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
-  if (c->sssp_ready) c->heavy.ensure((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
+  gs.countp = bc;
+  gs.row = b.g_row.as<uint32_t>(); gs.col = b.g_col.as<uint32_t>(); gs.act = b.g_act.as<uint8_t>();
+  gs.w = b.g_w.as<uint32_t>(); gs.ex = b.g_ex.as<uint8_t>();
+  gs.ex_pre = b.g_expre.as<uint32_t>(); gs.add_pre = b.g_addpre.as<uint32_t>();
+  gs.tix = b.tix.as<uint32_t>(); gs.t_gbeg = b.t_gbeg.as<uint32_t>();
+  merge_direction(c, c->out_off, c->out_col, c->out_w, b.noff_out, b.ncol_out, b.nw_out, gs, N, cap_m);
+
+  k_transpose_group_keys<<<grid_for(N), kBlock, 0, c->stream>>>(b.g_row.as<uint32_t>(), b.g_col.as<uint32_t>(), bc,
+                                                                N, b.keys_a.as<uint64_t>(), b.idx_a.as<uint32_t>());
+  cub_call(c, c->tmp, [&](void* p, size_t& bytes) {
+    return rocprim::radix_sort_pairs(p, bytes, b.keys_a.as<uint64_t>(), b.keys_b.as<uint64_t>(),
+                                     b.idx_a.as<uint32_t>(), b.idx_b.as<uint32_t>(), (size_t)N, 0, 64, c->stream);
+  });
+  k_gather_groups<<<grid_for(N), kBlock, 0, c->stream>>>(b.idx_b.as<uint32_t>(), bc, b.g_row.as<uint32_t>(),
+                                                         b.g_col.as<uint32_t>(), b.g_act.as<uint8_t>(),
+                                                         b.g_w.as<uint32_t>(), b.g_ex.as<uint8_t>(),
+                                                         b.tg_row.as<uint32_t>(), b.tg_col.as<uint32_t>(),
+                                                         b.tg_act.as<uint8_t>(), b.tg_w.as<uint32_t>(),
+                                                         b.tg_ex.as<uint8_t>());
+  build_view(c, N, b.tg_row.as<uint32_t>(), b.tg_ex.as<uint8_t>(), b.tg_act.as<uint8_t>(), b.tg_exu, b.tg_addu,
+             b.tg_expre, b.tg_addpre, b.thead, b.tt_raw, b.tt_list, b.tt_gbeg, b.ttix, 2);
+  Groups gt;
+  gt.countp = bc;
+  gt.row = b.tg_row.as<uint32_t>(); gt.col = b.tg_col.as<uint32_t>(); gt.act = b.tg_act.as<uint8_t>();
+  gt.w = b.tg_w.as<uint32_t>(); gt.ex = b.tg_ex.as<uint8_t>();
+  gt.ex_pre = b.tg_expre.as<uint32_t>(); gt.add_pre = b.tg_addpre.as<uint32_t>();
+  gt.tix = b.ttix.as<uint32_t>(); gt.t_gbeg = b.tt_gbeg.as<uint32_t>();
+  merge_direction(c, c->in_off, c->in_col, c->in_w, b.noff_in, b.ncol_in, b.nw_in, gt, N, cap_m);
+  k_untouch<<<grid_for(N), kBlock, 0, c->stream>>>(b.tt_list.as<uint32_t>(), bc, 2, N, b.ttix.as<uint32_t>());
 
   // 6. PageRank reseed +1 on the new out-edges
   if (c->pr_ready) {
-    if (c->pr_prec == SUMINC_PR_FP64) pr_reseed<double>(c, t_list.as<uint32_t>(), Tn, +1.0);
-    else pr_reseed<float>(c, t_list.as<uint32_t>(), Tn, +1.0);
+    if (c->pr_prec == SUMINC_PR_FP64) pr_reseed<double>(c, N, +1.0);
+    else pr_reseed<float>(c, N, +1.0);
   }
+  k_untouch<<<grid_for(N), kBlock, 0, c->stream>>>(b.t_list.as<uint32_t>(), bc, 1, N, b.tix.as<uint32_t>());
 
   // 7. SSSP: reset, re-pull, relax inserts -> frontier list 0
   if (sssp_reinit) sssp_init(c, c->sssp_src);
   if (sssp_repair) {
-    k_reset_apply<<<grid_for(n), kBlock, 0, c->stream>>>(rlist.as<uint32_t>(), cnt, c->sssp_dp.as<unsigned long long>());
+    k_reset_apply<<<grid_for(n), kBlock, 0, c->stream>>>(b.rlist.as<uint32_t>(), cnt,
+                                                         c->sssp_dp.as<unsigned long long>());
     k_reset_pull<<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
                                                         c->wkind == SUMINC_W_U32 ? c->in_w.as<uint32_t>() : nullptr,
-                                                        rbits.as<uint32_t>(), rlist.as<uint32_t>(), cnt,
+                                                        b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(), cnt,
                                                         c->sssp_dp.as<unsigned long long>());
-    k_reset_seed<<<grid_for(n), kBlock, 0, c->stream>>>(rlist.as<uint32_t>(), cnt, c->sssp_dp.as<unsigned long long>(),
+    k_reset_seed<<<grid_for(n), kBlock, 0, c->stream>>>(b.rlist.as<uint32_t>(), c->sssp_dp.as<unsigned long long>(),
                                                         c->wl0.as<uint32_t>(), c->sssp_flag0.as<uint32_t>(), cnt);
-    k_insert_relax<<<grid_for(G), kBlock, 0, c->stream>>>(g_row.as<uint32_t>(), g_col.as<uint32_t>(),
-                                                          g_act.as<uint8_t>(), g_w.as<uint32_t>(), G,
-                                                          c->wkind != SUMINC_W_U32, c->sssp_dp.as<unsigned long long>(),
-                                                          c->wl0.as<uint32_t>(), c->sssp_flag0.as<uint32_t>(), cnt);
+    k_insert_relax<<<grid_for(N), kBlock, 0, c->stream>>>(b.g_row.as<uint32_t>(), b.g_col.as<uint32_t>(),
+                                                          b.g_act.as<uint8_t>(), b.g_w.as<uint32_t>(), bc, N,
+                                                          c->wkind != SUMINC_W_U32,
+                                                          c->sssp_dp.as<unsigned long long>(), c->wl0.as<uint32_t>(),
+                                                          c->sssp_flag0.as<uint32_t>(), cnt);
     k_undone<<<1, 64, 0, c->stream>>>(cnt);
-    SUMINC_HIP(hipGetLastError());
   }
-  read_counters(c);
-  reseed_edges = c->host_counters->aux[5];
-  c->stats.reserved[1] = (double)G;
-  c->stats.reserved[2] = (double)Tn;
-  c->stats.reserved[3] = (double)reseed_edges;
+  SUMINC_HIP(hipGetLastError());
+  unsigned hbc[4] = {0, 0, 0, 0};
+  SUMINC_HIP(hipMemcpyAsync(hbc, bc, sizeof(hbc), hipMemcpyDeviceToHost, c->stream));
+  read_counters(c);  // synchronises the stream
+  c->m = hbc[3];
+  c->graph_version++;
+  // per-round scratch sized by m may need to grow
+  if (c->pr_ready || c->sssp_ready) c->heavy.ensure((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
+  c->stats.reserved[1] = (double)hbc[0];
+  c->stats.reserved[2] = (double)hbc[1];
+  c->stats.reserved[3] = (double)c->host_counters->aux[5];  // reseed edges
 }
 
 }  // namespace suminc

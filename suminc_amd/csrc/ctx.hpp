@@ -69,9 +69,28 @@ struct Counters {
   unsigned long long rounds;        // rounds executed on device
   unsigned long long done;          // stop flag set by the device
   unsigned long long aux[10];
+  unsigned long long round_edges;   // active edges of the current round
+  unsigned long long mode;          // PageRank scatter of the current round: 0 push, 1 pull
+  unsigned long long pull_rounds;
+  unsigned long long push_edges;    // active edges of push rounds
+  unsigned long long push_entries;  // work-list entries of push rounds
+  unsigned long long active_vertices;
   double residual;                  // last FP64 sum|delta|
   double residual_hist[2];
   double dpad[5];
+};
+
+// Persistent scratch of the update-batch path (grows, never shrinks), so a batch
+// allocates nothing in steady state.
+struct BatchWs {
+  DevBuf ops, keys_a, keys_b, idx_a, idx_b, rec, flags, grec;
+  DevBuf g_row, g_col, g_act, g_w, g_ex, g_eoldw, g_exu, g_addu, g_expre, g_addpre, head, t_raw, t_list, t_gbeg;
+  DevBuf tg_row, tg_col, tg_act, tg_w, tg_ex, tg_exu, tg_addu, tg_expre, tg_addpre, thead, tt_raw, tt_list, tt_gbeg;
+  DevBuf tix, ttix;  // row -> touched index, all INF32 between batches
+  DevBuf ndeg, noff_out, ncol_out, nw_out, noff_in, ncol_in, nw_in;
+  DevBuf rbits, rlist;
+  DevBuf bc;  // device counts: [0]=G groups, [1]=touched rows (CSR), [2]=touched rows (CSC), [3]=new m
+  uint32_t tix_n = 0;
 };
 
 }  // namespace suminc
@@ -94,6 +113,10 @@ struct suminc_ctx {
   bool pr_ready = false;
   int pr_prec = SUMINC_PR_FP32_KAHAN;
   suminc::DevBuf pr_delta, pr_rank, pr_comp, pr_contrib;
+  // pull mode: static chunk list of high in-degree vertices, rebuilt per graph version
+  suminc::DevBuf pull_heavy;
+  uint64_t pull_heavy_count = 0, pull_heavy_version = ~0ull;
+  uint64_t graph_version = 0;
 
   // SSSP state: packed {dist:hi32, parent:lo32}
   bool sssp_ready = false;
@@ -103,6 +126,7 @@ struct suminc_ctx {
 
   // work lists and scratch
   suminc::DevBuf wl0, wl1, heavy, partial, tmp, tmp2, tmp3, tmp4, tmp5;
+  suminc::BatchWs bw;
   suminc::DevBuf counters;                  // suminc::Counters
   suminc::Counters* host_counters = nullptr;  // pinned mirror
 
@@ -118,8 +142,8 @@ namespace suminc {
 
 constexpr uint32_t INF32 = 0xFFFFFFFFu;
 constexpr int kBlock = 256;          // threads per block for edge-parallel kernels
-constexpr uint32_t kHeavyDeg = 4096; // adjacency longer than this goes to the chunk path
-constexpr uint32_t kChunk = 4096;    // edges per heavy chunk
+constexpr uint32_t kHeavyDeg = 256;  // adjacency longer than this goes to the chunk path
+constexpr uint32_t kChunk = 2048;    // edges per heavy chunk
 
 inline unsigned grid_for(uint64_t items, int block = kBlock, unsigned cap = 1u << 30) {
   uint64_t g = (items + block - 1) / block;

@@ -2,21 +2,27 @@
 // Ingress phase: sum_sync_iter_worker.h:857-882 on CPU; tjn::pr_Ingress +
 // tjn::deltaSum on GPU, cudaCode/my_worker.cu:124-142, 304-328).
 //
-// One round = three launches on the context stream:
-//   k_pr_gather  vertex pass: FP64 sum|delta| (deterministic block partials),
-//                activity |delta| > T/N, exchange delta -> 0, rank += delta
-//                (Kahan-compensated in FP32 mode), and for active vertices with
-//                out-degree > 0 append (v, d*delta/deg) to the light work list or
-//                (v, chunk, c) to the heavy chunk list.  Wave64 ballot + one
-//                atomic per wave for the append.
-//   k_pr_push    persistent blocks stride over light tiles (256 vertices, the
-//                block walks their adjacency cooperatively with coalesced column
-//                reads) then over heavy chunks (4096 edges each); scatter with
+// One round, all launches on the context stream, no host synchronisation:
+//   k_pr_count   vertex pass over delta only: FP64 sum|delta| (deterministic
+//                per-block partials) and the out-degree sum of active vertices
+//                (|delta| > T/N, the reference's isChange).
+//   k_pr_decide  one block: residual = fixed-order sum of the partials; stop if
+//                residual < T (termCheck) or the round cap is reached; else pick
+//                the scatter for this round: pull when the active edges exceed
+//                m / pull_threshold_div, push otherwise.
+//   k_pr_gather  exchange delta -> 0 for active vertices, rank += delta (Kahan
+//                compensated in FP32 mode); push: append (v, d*delta/deg) to the
+//                light list (wave64 ballot, one atomic per wave) or 2048-edge
+//                chunks of high-degree vertices to the heavy list; pull: write the
+//                contribution of every vertex (0 when inactive).
+//   k_pr_push    persistent blocks over light tiles (block-cooperative adjacency
+//                walk, coalesced column reads) then heavy chunks; scatter with the
 //                hardware float/double atomic add.
-//   k_pr_end     single block: fixed-order sum of the partials -> residual,
-//                stop flag (residual < T or round cap), reset list counters.
-// Rounds are enqueued in groups without host synchronisation; a round whose
-// k_pr_gather sees the stop flag does nothing.
+//   k_pr_pull_*  CSC gather: light tiles of 256 consecutive vertices accumulate
+//                in LDS (ds_add) and add to delta once; vertices with in-degree
+//                > 256 are summed per 2048-edge chunk and added with one atomic.
+//   k_pr_end     list counters reset, statistics.
+// Kernels not selected for the round return at entry.
 #include "ctx.hpp"
 #include "device.hpp"
 #include "pr.hpp"
@@ -24,24 +30,87 @@
 namespace suminc {
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_pr_gather(uint32_t n, const uint32_t* __restrict__ off, T* __restrict__ delta,
-                                                      T* __restrict__ rank, float* __restrict__ comp, double eps, T d,
-                                                      uint32_t* __restrict__ wl_v, T* __restrict__ wl_c,
-                                                      uint4* __restrict__ heavy, Counters* cnt,
-                                                      double* __restrict__ partial) {
+__global__ void __launch_bounds__(kBlock) k_pr_count(uint32_t n, const uint32_t* __restrict__ off,
+                                                     const T* __restrict__ delta, double eps, Counters* cnt,
+                                                     double* __restrict__ partial) {
   __shared__ double s_sum[kBlock / 64];
   __shared__ unsigned long long s_edges[kBlock / 64];
-  __shared__ unsigned long long s_act[kBlock / 64];
   if (cnt->done) return;
   const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
   double absd = 0.0;
-  uint32_t deg_active = 0;
-  bool light = false;
+  unsigned long long deg = 0;
+  if (v < n) {
+    absd = (double)fabs_t(delta[v]);
+    if (absd > eps) deg = off[v + 1] - off[v];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    absd += __shfl_xor(absd, o);
+    deg += __shfl_xor(deg, o);
+  }
+  const int wid = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    s_sum[wid] = absd;
+    s_edges[wid] = deg;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    unsigned long long te = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      t += s_sum[w];
+      te += s_edges[w];
+    }
+    partial[blockIdx.x] = t;
+    if (te) atomicAdd(&cnt->round_edges, te);
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_pr_decide(const double* __restrict__ partial, uint32_t nparts,
+                                                    Counters* cnt, double T, int max_rounds, uint64_t m, int div,
+                                                    uint8_t* mode_hist) {
+  __shared__ double s[1024];
+  if (cnt->done) return;
+  double acc = 0;
+  for (uint32_t i = threadIdx.x; i < nparts; i += 1024) acc += partial[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double r = s[0];
+    cnt->residual = r;
+    if (r != r || r > 1e300) cnt->aux[0] = 1;  // numeric failure flag
+    if (!(r >= T) || (long long)cnt->rounds >= (long long)max_rounds || cnt->aux[0]) {
+      cnt->done = 1;
+      cnt->round_edges = 0;
+      return;
+    }
+    const bool pull = div > 0 && cnt->round_edges * (unsigned long long)div > m;
+    cnt->mode = pull ? 1 : 0;
+    if (cnt->rounds < 65536) mode_hist[cnt->rounds] = pull ? 1 : 0;
+    cnt->rounds += 1;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pr_gather(uint32_t n, const uint32_t* __restrict__ off, T* __restrict__ delta,
+                                                      T* __restrict__ rank, float* __restrict__ comp, double eps, T d,
+                                                      uint32_t* __restrict__ wl_v, T* __restrict__ wl_c,
+                                                      uint4* __restrict__ heavy, T* __restrict__ contrib,
+                                                      Counters* cnt) {
+  __shared__ unsigned long long s_act[kBlock / 64];
+  if (cnt->done) return;
+  const bool pull = cnt->mode != 0;
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  bool light = false, active = false;
   T c = T(0);
   if (v < n) {
     T dv = delta[v];
-    absd = (double)fabs_t(dv);
-    if (absd > eps) {
+    if ((double)fabs_t(dv) > eps) {
+      active = true;
       delta[v] = T(0);
       if constexpr (sizeof(T) == 4) {
         float r = rank[v], cc = comp[v];
@@ -55,64 +124,49 @@ __global__ void __launch_bounds__(kBlock) k_pr_gather(uint32_t n, const uint32_t
       uint32_t b = off[v], e = off[v + 1];
       uint32_t deg = e - b;
       if (deg > 0) {
-        deg_active = deg;
         c = d * dv / (T)deg;
-        if (deg <= kHeavyDeg) {
-          light = true;
-        } else {
-          uint32_t nch = (deg + kChunk - 1) / kChunk;
-          unsigned long long slot = atomicAdd(&cnt->heavy_count, (unsigned long long)nch);
-          for (uint32_t k = 0; k < nch; ++k) {
-            uint4 h;
-            h.x = v;
-            h.y = k;
-            if constexpr (sizeof(T) == 4) {
-              h.z = __float_as_uint((float)c);
-              h.w = 0;
-            } else {
-              unsigned long long bits = __double_as_longlong((double)c);
-              h.z = (uint32_t)bits;
-              h.w = (uint32_t)(bits >> 32);
+        if (!pull) {
+          if (deg <= kHeavyDeg) {
+            light = true;
+          } else {
+            uint32_t nch = (deg + kChunk - 1) / kChunk;
+            unsigned long long slot = atomicAdd(&cnt->heavy_count, (unsigned long long)nch);
+            for (uint32_t k = 0; k < nch; ++k) {
+              uint4 h;
+              h.x = v;
+              h.y = k;
+              if constexpr (sizeof(T) == 4) {
+                h.z = __float_as_uint((float)c);
+                h.w = 0;
+              } else {
+                unsigned long long bits = __double_as_longlong((double)c);
+                h.z = (uint32_t)bits;
+                h.w = (uint32_t)(bits >> 32);
+              }
+              heavy[slot + k] = h;
             }
-            heavy[slot + k] = h;
           }
         }
       }
     }
+    if (pull) contrib[v] = c;
   }
-  unsigned long long slot = wave_append(light, &cnt->wl_count[0]);
-  if (light) {
-    wl_v[slot] = v;
-    wl_c[slot] = c;
+  if (!pull) {
+    unsigned long long slot = wave_append(light, &cnt->wl_count[0]);
+    if (light) {
+      wl_v[slot] = v;
+      wl_c[slot] = c;
+    }
   }
-  // block reductions: sum |delta| (FP64), active edges, active vertices
-  double s = absd;
-  unsigned long long ed = deg_active;
-  unsigned long long act = absd > eps ? 1ull : 0ull;
+  unsigned long long a = active ? 1ull : 0ull;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    s += __shfl_xor(s, o);
-    ed += __shfl_xor(ed, o);
-    act += __shfl_xor(act, o);
-  }
-  const int wid = threadIdx.x >> 6;
-  if (lane_id() == 0) {
-    s_sum[wid] = s;
-    s_edges[wid] = ed;
-    s_act[wid] = act;
-  }
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+  if (lane_id() == 0) s_act[threadIdx.x >> 6] = a;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double t = 0;
-    unsigned long long te = 0, ta = 0;
-    for (int w = 0; w < kBlock / 64; ++w) {
-      t += s_sum[w];
-      te += s_edges[w];
-      ta += s_act[w];
-    }
-    partial[blockIdx.x] = t;
-    if (te) atomicAdd(&cnt->edges, te);
-    if (ta) atomicAdd(&cnt->aux[8], ta);
+    unsigned long long t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += s_act[w];
+    if (t) atomicAdd(&cnt->active_vertices, t);
   }
 }
 
@@ -125,6 +179,7 @@ __global__ void __launch_bounds__(kBlock) k_pr_push(const uint32_t* __restrict__
   __shared__ uint32_t s_start[kBlock];
   __shared__ uint32_t s_warp[kBlock / 64 + 1];
   __shared__ T s_c[kBlock];
+  if (cnt->done || cnt->mode != 0) return;
   const unsigned long long nl = cnt->wl_count[0];
   const unsigned long long nh = cnt->heavy_count;
   const unsigned long long tiles = (nl + kBlock - 1) / kBlock;
@@ -152,29 +207,89 @@ __global__ void __launch_bounds__(kBlock) k_pr_push(const uint32_t* __restrict__
   }
 }
 
-// Single block: deterministic sum of partials, stop rule, counter reset.
-__global__ void __launch_bounds__(1024) k_pr_end(const double* __restrict__ partial, uint32_t nparts, Counters* cnt,
-                                                 double T, int max_rounds) {
-  __shared__ double s[1024];
-  if (cnt->done) return;
-  double acc = 0;
-  for (uint32_t i = threadIdx.x; i < nparts; i += 1024) acc += partial[i];
-  s[threadIdx.x] = acc;
-  __syncthreads();
-  for (int o = 512; o > 0; o >>= 1) {
-    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+// Pull, light part: tiles of 256 consecutive vertices with in-degree <= 256.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pr_pull_light(uint32_t n, const uint32_t* __restrict__ ioff,
+                                                          const uint32_t* __restrict__ icol,
+                                                          const T* __restrict__ contrib, T* __restrict__ delta,
+                                                          const Counters* cnt) {
+  __shared__ uint32_t s_scan[kBlock + 1];
+  __shared__ uint32_t s_start[kBlock];
+  __shared__ uint32_t s_warp[kBlock / 64 + 1];
+  __shared__ T s_acc[kBlock];
+  if (cnt->done || cnt->mode != 1) return;
+  const uint32_t tiles = (n + kBlock - 1) / kBlock;
+  for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const uint32_t v = t * kBlock + threadIdx.x;
+    uint32_t start = 0, len = 0;
+    if (v < n) {
+      start = ioff[v];
+      len = ioff[v + 1] - start;
+      if (len > kHeavyDeg) len = 0;
+    }
+    s_acc[threadIdx.x] = T(0);
+    block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t, uint32_t e) {
+      T x = contrib[icol[e]];
+      if (x != T(0)) atomicAdd(&s_acc[slot], x);
+    });
+    if (len) {
+      T a = s_acc[threadIdx.x];
+      if (a != T(0)) delta[v] += a;
+    }
+  }
+}
+
+// Pull, heavy part: 2048-edge chunks of vertices with in-degree > 256.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pr_pull_heavy(const uint32_t* __restrict__ ioff,
+                                                          const uint32_t* __restrict__ icol,
+                                                          const T* __restrict__ contrib, T* __restrict__ delta,
+                                                          const uint2* __restrict__ chunks, uint64_t nchunks,
+                                                          const Counters* cnt) {
+  __shared__ T s_red[kBlock / 64];
+  if (cnt->done || cnt->mode != 1) return;
+  for (uint64_t h = blockIdx.x; h < nchunks; h += gridDim.x) {
+    const uint2 ch = chunks[h];
+    const uint32_t b = ioff[ch.x] + ch.y * kChunk;
+    const uint32_t e = min(ioff[ch.x + 1], b + kChunk);
+    T acc = T(0);
+    for (uint32_t k = b + threadIdx.x; k < e; k += kBlock) acc += contrib[icol[k]];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane_id() == 0) s_red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      T s = T(0);
+      for (int w = 0; w < kBlock / 64; ++w) s += s_red[w];
+      if (s != T(0)) unsafeAtomicAdd(&delta[ch.x], s);
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    double r = s[0];
-    cnt->residual = r;
-    cnt->rounds += 1;
-    cnt->aux[6] += cnt->wl_count[0] + cnt->heavy_count;  // push entries processed
-    cnt->wl_count[0] = 0;
-    cnt->heavy_count = 0;
-    if (!(r >= T) || (long long)cnt->rounds >= (long long)max_rounds) cnt->done = 1;
-    if (r != r || r > 1e300) cnt->aux[0] = 1;  // numeric failure flag
+}
+
+__global__ void k_pr_end(Counters* cnt) {
+  if (threadIdx.x || cnt->done) return;
+  cnt->edges += cnt->round_edges;
+  if (cnt->mode) {
+    cnt->pull_rounds += 1;
+  } else {
+    cnt->push_edges += cnt->round_edges;
+    cnt->push_entries += cnt->wl_count[0] + cnt->heavy_count;
   }
+  cnt->round_edges = 0;
+  cnt->wl_count[0] = 0;
+  cnt->heavy_count = 0;
+}
+
+// static chunk list of vertices with in-degree > kHeavyDeg (pull mode)
+__global__ void k_pull_heavy_list(uint32_t n, const uint32_t* ioff, uint2* out, unsigned long long* count) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  uint32_t deg = ioff[v + 1] - ioff[v];
+  if (deg <= kHeavyDeg) return;
+  uint32_t nch = (deg + kChunk - 1) / kChunk;
+  unsigned long long slot = atomicAdd(count, (unsigned long long)nch);
+  for (uint32_t k = 0; k < nch; ++k) out[slot + k] = make_uint2(v, k);
 }
 
 template <class T>
@@ -213,13 +328,8 @@ __global__ void k_pr_from_f64(uint32_t n, T* delta, T* rank, float* comp, const 
 
 __global__ void k_zero_counters(Counters* cnt) {
   if (threadIdx.x == 0) {
-    cnt->wl_count[0] = cnt->wl_count[1] = 0;
-    cnt->heavy_count = 0;
-    cnt->edges = 0;
-    cnt->rounds = 0;
-    cnt->done = 0;
-    for (int i = 0; i < 10; ++i) cnt->aux[i] = 0;
-    cnt->residual = 0;
+    Counters z{};
+    *cnt = z;
   }
 }
 
@@ -234,11 +344,13 @@ void pr_alloc(suminc_ctx* c) {
   c->pr_delta.ensure(n * es);
   c->pr_rank.ensure(n * es);
   if (c->pr_prec == SUMINC_PR_FP32_KAHAN) c->pr_comp.ensure(n * 4);
+  c->pr_contrib.ensure(n * es + 8);
   c->wl0.ensure(n * 4 + 4);
   c->wl1.ensure(n * es + 8);
   // heavy chunks: at most m / kChunk + (number of heavy vertices) entries
   c->heavy.ensure((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
   c->partial.ensure(((size_t)grid_for(n) + 1) * 8);
+  c->tmp4.ensure(65536);  // per-round scatter mode history
 }
 
 void pr_init(suminc_ctx* c) {
@@ -257,45 +369,96 @@ void pr_init(suminc_ctx* c) {
   c->pr_ready = true;
 }
 
-// Enqueues one round; with profiling on, events 3*slot, 3*slot+1, 3*slot+2
-// bracket the gather and push launches.
+static void ensure_pull_heavy(suminc_ctx* c) {
+  if (c->pull_heavy_version == c->graph_version) return;
+  c->pull_heavy.ensure((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint2));
+  DevBuf cnt;
+  cnt.ensure(8);
+  SUMINC_HIP(hipMemsetAsync(cnt.p, 0, 8, c->stream));
+  k_pull_heavy_list<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->in_off.as<uint32_t>(), c->pull_heavy.as<uint2>(),
+                                                             cnt.as<unsigned long long>());
+  SUMINC_HIP(hipGetLastError());
+  unsigned long long h = 0;
+  SUMINC_HIP(hipMemcpyAsync(&h, cnt.p, 8, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  c->pull_heavy_count = h;
+  c->pull_heavy_version = c->graph_version;
+}
+
+// Enqueues one round; with profiling on, events 4*slot .. 4*slot+3 bracket the
+// count+decide, gather and scatter (push or pull) launches.
 template <class T>
-static void enqueue_round(suminc_ctx* c, double eps, double T_stop, int max_rounds, size_t slot) {
+static void enqueue_round(suminc_ctx* c, double eps, double T_stop, int max_rounds, int div, size_t slot) {
   Counters* cnt = dev_counters(c);
   const unsigned g = grid_for(c->n);
-  if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 3 * slot), c->stream));
+  if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 4 * slot), c->stream));
+  k_pr_count<T><<<g, kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), c->pr_delta.as<T>(), eps, cnt,
+                                             c->partial.as<double>());
+  k_pr_decide<<<1, 1024, 0, c->stream>>>(c->partial.as<double>(), g, cnt, T_stop, max_rounds, c->m, div,
+                                         c->tmp4.as<uint8_t>());
+  if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 4 * slot + 1), c->stream));
   k_pr_gather<T><<<g, kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), c->pr_delta.as<T>(),
-                                              c->pr_rank.as<T>(), c->pr_comp.as<float>(), eps,
-                                              (T)c->prm.damping, c->wl0.as<uint32_t>(), c->wl1.as<T>(),
-                                              c->heavy.as<uint4>(), cnt, c->partial.as<double>());
-  if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 3 * slot + 1), c->stream));
+                                              c->pr_rank.as<T>(), c->pr_comp.as<float>(), eps, (T)c->prm.damping,
+                                              c->wl0.as<uint32_t>(), c->wl1.as<T>(), c->heavy.as<uint4>(),
+                                              c->pr_contrib.as<T>(), cnt);
+  if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 4 * slot + 2), c->stream));
   k_pr_push<T><<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
                                                       c->pr_delta.as<T>(), c->wl0.as<uint32_t>(), c->wl1.as<T>(),
                                                       c->heavy.as<uint4>(), cnt);
-  if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 3 * slot + 2), c->stream));
-  k_pr_end<<<1, 1024, 0, c->stream>>>(c->partial.as<double>(), g, cnt, T_stop, max_rounds);
+  if (div > 0) {
+    k_pr_pull_light<T><<<push_grid(), kBlock, 0, c->stream>>>(c->n, c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
+                                                              c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt);
+    k_pr_pull_heavy<T><<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
+                                                              c->pr_contrib.as<T>(), c->pr_delta.as<T>(),
+                                                              c->pull_heavy.as<uint2>(), c->pull_heavy_count, cnt);
+  }
+  if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 4 * slot + 3), c->stream));
+  k_pr_end<<<1, 64, 0, c->stream>>>(cnt);
 }
 
 // Adds the profile of the first `rounds` enqueued rounds (the ones that did work).
 static void pr_profile_rounds(suminc_ctx* c, size_t rounds) {
-  if (!c->prof_on) return;
-  prof_collect(c, SUMINC_KERNEL_PR_GATHER, rounds, 0, 3);
-  prof_collect(c, SUMINC_KERNEL_PR_PUSH, rounds, 1, 3);
+  if (!c->prof_on || rounds == 0) return;
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  std::vector<uint8_t> mode(std::min<size_t>(rounds, 65536));
+  SUMINC_HIP(hipMemcpy(mode.data(), c->tmp4.p, mode.size(), hipMemcpyDeviceToHost));
+  double ms_push = 0, ms_pull = 0, ms_gather = 0;
+  size_t n_push = 0, n_pull = 0;
+  for (size_t i = 0; i < mode.size(); ++i) {
+    float t_g = 0, t_s = 0;
+    SUMINC_HIP(hipEventElapsedTime(&t_g, c->prof_events[4 * i + 1], c->prof_events[4 * i + 2]));
+    SUMINC_HIP(hipEventElapsedTime(&t_s, c->prof_events[4 * i + 2], c->prof_events[4 * i + 3]));
+    ms_gather += t_g;
+    if (mode[i]) { ms_pull += t_s; ++n_pull; } else { ms_push += t_s; ++n_push; }
+  }
   const Counters* h = c->host_counters;
   const double es = c->pr_prec == SUMINC_PR_FP64 ? 8.0 : 4.0;
   suminc_kernel_profile& p = c->prof[SUMINC_KERNEL_PR_PUSH];
+  p.launches += (double)n_push;
+  p.ms += ms_push;
   // push: column (4 B) + delta atomic read-modify-write (2 es) per edge; worklist
   // entry (4 + es) and two row pointers (8 B) per entry
-  p.edges += (double)h->edges;
-  p.vertices += (double)h->aux[6];
-  p.alg_bytes += (double)h->edges * (4.0 + 2.0 * es) + (double)h->aux[6] * (12.0 + es);
-  suminc_kernel_profile& q = c->prof[SUMINC_KERNEL_PR_GATHER];
-  // gather: delta read for every vertex; per active vertex delta write, rank RMW,
-  // Kahan term RMW (FP32), row pointers, worklist entry
-  const double active = (double)h->aux[8];
+  p.edges += (double)h->push_edges;
+  p.vertices += (double)h->push_entries;
+  p.alg_bytes += (double)h->push_edges * (4.0 + 2.0 * es) + (double)h->push_entries * (12.0 + es);
+  suminc_kernel_profile& q = c->prof[SUMINC_KERNEL_PR_PULL];
+  q.launches += (double)n_pull;
+  q.ms += ms_pull;
+  // pull: row index (4 B) + contribution gather (es) per in-edge of the graph;
+  // two row pointers (8 B) + delta read-modify-write (2 es) per vertex
+  q.edges += (double)n_pull * (double)c->m;
+  q.vertices += (double)n_pull * (double)c->n;
+  q.alg_bytes += (double)n_pull * ((double)c->m * (4.0 + es) + (double)c->n * (8.0 + 2.0 * es));
+  suminc_kernel_profile& g = c->prof[SUMINC_KERNEL_PR_GATHER];
+  g.launches += (double)mode.size();
+  g.ms += ms_gather;
+  // gather: delta read per vertex; per active vertex delta write, rank and Kahan
+  // term read-modify-write, row pointers; pull rounds write every contribution
+  const double active = (double)h->active_vertices;
   const double kahan = c->pr_prec == SUMINC_PR_FP64 ? 0.0 : 8.0;
-  q.vertices += (double)rounds * c->n;
-  q.alg_bytes += (double)rounds * c->n * es + active * (es + 2 * es + kahan + 8.0 + 4.0 + es);
+  g.vertices += (double)mode.size() * c->n;
+  g.alg_bytes += (double)mode.size() * c->n * es + active * (es + 2 * es + kahan + 8.0) +
+                 (double)n_pull * c->n * es + (double)h->push_entries * (4.0 + es);
 }
 
 // Runs rounds until the device sets the stop flag; host polls every `group` rounds.
@@ -304,14 +467,16 @@ void pr_run(suminc_ctx* c, suminc_stats* st) {
   const double T = c->prm.termcheck_threshold;
   const double eps = T / (double)c->n;
   const int max_rounds = c->prm.max_rounds > 0 ? c->prm.max_rounds : 1000000;
+  const int div = c->prm.pull_threshold_div;
+  if (div > 0) ensure_pull_heavy(c);
   k_zero_counters<<<1, 64, 0, c->stream>>>(dev_counters(c));
   SUMINC_HIP(hipEventRecord(c->ev_start, c->stream));
   int group = 4;
   size_t slot = 0;
   while (true) {
     for (int i = 0; i < group; ++i, ++slot) {
-      if (c->pr_prec == SUMINC_PR_FP64) enqueue_round<double>(c, eps, T, max_rounds, slot);
-      else enqueue_round<float>(c, eps, T, max_rounds, slot);
+      if (c->pr_prec == SUMINC_PR_FP64) enqueue_round<double>(c, eps, T, max_rounds, div, slot);
+      else enqueue_round<float>(c, eps, T, max_rounds, div, slot);
     }
     SUMINC_HIP(hipGetLastError());
     read_counters(c);
@@ -329,7 +494,8 @@ void pr_run(suminc_ctx* c, suminc_stats* st) {
   c->stats.edges_traversed = (double)h->edges;
   c->stats.residual = h->residual;
   c->stats.kernel_ms = ms;
-  c->stats.push_rounds = (double)h->rounds;
+  c->stats.pull_rounds = (double)h->pull_rounds;
+  c->stats.push_rounds = (double)h->rounds - (double)h->pull_rounds;
   if (st) *st = c->stats;
   SUMINC_REQUIRE(!h->aux[0], SUMINC_E_NUMERIC, "NaN/Inf in PageRank residual");
   SUMINC_REQUIRE(!(h->residual >= T), SUMINC_E_NOT_CONVERGED, "round cap reached before sum|delta| < T");
@@ -338,10 +504,12 @@ void pr_run(suminc_ctx* c, suminc_stats* st) {
 void pr_step(suminc_ctx* c, double* residual) {
   SUMINC_REQUIRE(c->pr_ready, SUMINC_E_STATE, "PageRank state not initialised");
   const double T = c->prm.termcheck_threshold;
+  const int div = c->prm.pull_threshold_div;
+  if (div > 0) ensure_pull_heavy(c);
   k_zero_counters<<<1, 64, 0, c->stream>>>(dev_counters(c));
   // stop rule disabled for a single explicit step
-  if (c->pr_prec == SUMINC_PR_FP64) enqueue_round<double>(c, T / c->n, -1.0, 1 << 30, 0);
-  else enqueue_round<float>(c, T / c->n, -1.0, 1 << 30, 0);
+  if (c->pr_prec == SUMINC_PR_FP64) enqueue_round<double>(c, T / c->n, -1.0, 1 << 30, div, 0);
+  else enqueue_round<float>(c, T / c->n, -1.0, 1 << 30, div, 0);
   SUMINC_HIP(hipGetLastError());
   read_counters(c);
   pr_profile_rounds(c, 1);

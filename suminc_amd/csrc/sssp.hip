@@ -209,7 +209,7 @@ void sssp_run(suminc_ctx* c, suminc_stats* st) {
     for (int i = 0; i < group; ++i, ++slot) {
       if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 2 * slot), c->stream));
       k_sssp_push<<<push_grid(), kBlock, 0, c->stream>>>(a);
-      k_sssp_heavy<<<512, kBlock, 0, c->stream>>>(a);
+      k_sssp_heavy<<<push_grid(), kBlock, 0, c->stream>>>(a);
       if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 2 * slot + 1), c->stream));
       k_sssp_end<<<1, 64, 0, c->stream>>>(a.cnt, max_rounds);
     }

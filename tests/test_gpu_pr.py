@@ -35,7 +35,8 @@ def test_golden_test_base(gpu):
             st = ctx.pr_run()
             r, d = ctx.pr_download()
         err = np.abs(r - np.array(exp["pagerank"])).sum()
-        tol = st["residual"] / (1 - DAMPING) + (1e-13 if prec == gpu.PR_FP64 else 2e-8)
+        # FP32 deltas round each push to 2^-24 relative: ~6e-8 x the pushed mass
+        tol = st["residual"] / (1 - DAMPING) + (1e-13 if prec == gpu.PR_FP64 else 1e-7)
         assert err <= tol, (prec, err, tol)
 
 
