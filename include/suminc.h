@@ -76,7 +76,9 @@ typedef struct suminc_params {
   int32_t pr_precision;       /* SUMINC_PR_FP64 or SUMINC_PR_FP32_KAHAN */
   uint32_t sssp_source;       /* FLAGS_sssp_source */
   int32_t pull_threshold_div; /* push->pull switch: pull when active edges > m / div (0: never pull) */
-  int32_t reserved[8];
+  int32_t shortcut_max_rounds;/* cap on the in-cluster rounds of a shortcut / precompute (the reference's 100,
+                                 iter_compressor.h:422) */
+  int32_t reserved[7];
 } suminc_params;
 
 typedef struct suminc_stats {
@@ -164,6 +166,49 @@ int suminc_sssp_init(suminc_ctx* ctx, uint32_t source);
  * path uses unit weights, my_ssspworker.cu:390). */
 int suminc_sssp_run(suminc_ctx* ctx, suminc_stats* stats);
 int suminc_sssp_download(suminc_ctx* ctx, uint32_t* dist, uint32_t* parent);
+
+/* ---- supernode sketch ------------------------------------------------------------
+ * Clustering input, acceptance, node types, shortcut precompute and the SumInc
+ * PageRank query (IterCompressor::init/run + SumSyncIterWorker::Query in the
+ * mirror-free form, mirror_k = 1e8: compressor_base.h:3160-3611, 4107-4182,
+ * 4636-4899; iter_compressor.h:65-196, 318-491, 534-785; sum_sync_iter_worker.h:
+ * 348-438, 949-1127, 1385-1491). */
+typedef struct suminc_sketch_info {
+  uint64_t clusters;            /* accepted clusters (supernode groups) */
+  uint64_t raw_clusters;        /* clusters in the input assignment */
+  uint64_t entries;             /* entry vertices = supernodes (Fc >= 0) */
+  uint64_t clustered_vertices;
+  uint64_t node_type_count[5];  /* Single, OnlyOut, OnlyIn, BothOutIn, Inner (grape/types.h:70-78) */
+  uint64_t sketch_edges;        /* ib_e: Singles' out-edges + exits' external edges */
+  uint64_t internal_edges;      /* edges inside accepted clusters */
+  uint64_t shortcut_slots;      /* sum over entries of their cluster size (dense rows) */
+  uint64_t max_cluster;
+  uint64_t reserved[8];
+} suminc_sketch_info;
+/* Built-in clustering: every vertex whose only neighbour (in or out) is h joins
+ * h's cluster (a pair anchors on its smaller id); clusters of fewer than
+ * min_node_num members are dropped, larger ones keep the first max_node_num - 1
+ * leaves by id.  Stands in for the reference's external Louvain file
+ * (sh/louvain_all.sh:24-49, read at compressor_base.h:3175-3236). */
+int suminc_cluster_leaves(suminc_ctx* ctx, uint32_t max_node_num, uint32_t min_node_num,
+                          uint32_t* cluster_of_out /* n, may be NULL */);
+/* Caller-supplied assignment: cluster_of[v] = cluster id (< n) or SUMINC_INF_U32. */
+int suminc_set_clusters(suminc_ctx* ctx, const uint32_t* cluster_of);
+/* Acceptance + node types + layout (the structure is rebuilt after every batch,
+ * acceptance is kept). */
+int suminc_sum_build(suminc_ctx* ctx, uint32_t min_node_num, uint32_t max_node_num, double compress_threshold);
+int suminc_sum_info(suminc_ctx* ctx, suminc_sketch_info* info);
+/* cluster_of: accepted cluster per vertex (SUMINC_INF_U32 for Singles); node_type: u8 per vertex. */
+int suminc_sum_download(suminc_ctx* ctx, uint32_t* cluster_of, uint8_t* node_type);
+/* PageRank shortcut of one entry: its cluster's members (exits first), reach mass
+ * (bound_delta + inner_value) and residual (inner_delta) per member. */
+int suminc_sum_pr_shortcut(suminc_ctx* ctx, uint32_t entry, uint32_t cap, uint32_t* members, double* reach,
+                           double* residual, uint32_t* count, uint32_t* nexit);
+/* The SumInc PageRank query from the current state: precompute of dirty clusters,
+ * sketch rounds, flush, correction, plain residual phase.  Replaces
+ * tjn::g_function_compr + OutMirrorSyncToMaster + deltaSum and the host correction
+ * (sum_sync_iter_worker.h:932-933, 1129-1146, 1385-1491). */
+int suminc_sum_pr_run(suminc_ctx* ctx, suminc_stats* stats);
 
 /* ---- stats ---------------------------------------------------------------------*/
 int suminc_get_stats(suminc_ctx* ctx, suminc_stats* stats);

@@ -51,6 +51,10 @@ def lib():
             "orc_pr_amend": (None, [vp, C.c_double, C.c_int, vp, vp, vp]),
             "orc_sssp_dijkstra": (None, [vp, C.c_uint32, vp, vp]),
             "orc_sssp_ingress": (C.c_int, [vp, C.c_uint32, vp, vp, vp]),
+            "orc_leaf_clusters": (None, [vp, C.c_uint32, C.c_uint32, vp]),
+            "orc_sum_structure": (None, [vp, vp, C.c_uint32, C.c_uint32, C.c_double, vp, vp]),
+            "orc_sum_shortcut": (C.c_int, [vp, vp, C.c_uint32, C.c_double, C.c_double, C.c_int, vp, vp]),
+            "orc_sum_pr": (None, [vp, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -176,3 +180,34 @@ class Graph:
         st = np.zeros(2)
         lib().orc_sssp_ingress(self.h, src, _p(dist), _p(par), _p(st))
         return dist, par, {"rounds": st[0], "edges": st[1]}
+
+    # ---- supernode compression ------------------------------------------------
+    def leaf_clusters(self, max_node=1024, min_node=2):
+        raw = np.zeros(self.n, np.uint32)
+        lib().orc_leaf_clusters(self.h, max_node, min_node, _p(raw))
+        return raw
+
+    def sum_structure(self, raw, min_node=2, max_node=1024, threshold=1.0):
+        raw = np.ascontiguousarray(raw, np.uint32)
+        cid = np.zeros(self.n, np.uint32)
+        typ = np.zeros(self.n, np.uint8)
+        lib().orc_sum_structure(self.h, _p(raw), min_node, max_node, threshold, _p(cid), _p(typ))
+        return cid, typ
+
+    def sum_shortcut(self, cid, entry, d, thr, maxit=100):
+        cid = np.ascontiguousarray(cid, np.uint32)
+        r = np.zeros(self.n)
+        dl = np.zeros(self.n)
+        steps = lib().orc_sum_shortcut(self.h, _p(cid), entry, d, thr, maxit, _p(r), _p(dl))
+        return r, dl, steps
+
+    def sum_pr(self, cid, typ, d, T, max_rounds=100000, r=None, delta=None, dirty=None):
+        init = r is None
+        cid = np.ascontiguousarray(cid, np.uint32)
+        typ = np.ascontiguousarray(typ, np.uint8)
+        r = np.zeros(self.n) if r is None else np.ascontiguousarray(r, np.float64).copy()
+        delta = np.zeros(self.n) if delta is None else np.ascontiguousarray(delta, np.float64).copy()
+        dv = None if dirty is None else np.ascontiguousarray(dirty, np.uint8)
+        st = np.zeros(4)
+        lib().orc_sum_pr(self.h, _p(cid), _p(typ), d, T, max_rounds, int(init), _p(dv), _p(r), _p(delta), _p(st))
+        return r, delta, {"sketch_rounds": st[0], "plain_rounds": st[1], "edges": st[2], "residual": st[3]}

@@ -487,4 +487,254 @@ int orc_sssp_ingress(void* h, uint32_t src, uint32_t* dist, uint32_t* parent, do
   return rounds;
 }
 
+// ---------------------------------------------------------------------------
+// Supernode compression (mirror-free form, mirror_k = 1e8)
+// ---------------------------------------------------------------------------
+
+// Leaf clustering (the build's stand-in for the reference's external Louvain
+// file): v joins h when h is v's only neighbour in or out; a leaf pair anchors on
+// its smaller id; clusters below min_node are dropped, larger ones keep the first
+// max_node - 1 leaves by id.  Cluster ids are dense in anchor order.
+void orc_leaf_clusters(void* h, uint32_t max_node, uint32_t min_node, uint32_t* raw) {
+  Graph* g = (Graph*)h;
+  const uint32_t n = g->n, INF = 0xFFFFFFFFu;
+  std::vector<std::vector<uint32_t>> in(n);
+  for (uint32_t u = 0; u < n; ++u)
+    for (uint64_t e = g->off[u]; e < g->off[u + 1]; ++e) in[g->col[e]].push_back(u);
+  std::vector<uint32_t> anchor(n, INF);
+  for (uint32_t v = 0; v < n; ++v) {
+    std::vector<uint32_t> nb;
+    for (uint64_t e = g->off[v]; e < g->off[v + 1]; ++e) nb.push_back(g->col[e]);
+    for (uint32_t u : in[v]) nb.push_back(u);
+    std::sort(nb.begin(), nb.end());
+    nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+    if (nb.size() == 1) anchor[v] = nb[0];
+  }
+  std::vector<std::vector<uint32_t>> leaves(n);
+  for (uint32_t v = 0; v < n; ++v) {
+    uint32_t a = anchor[v];
+    if (a == INF) continue;
+    if (anchor[a] == v && v < a) continue;  // v anchors the pair
+    leaves[a].push_back(v);                 // v ascending
+  }
+  for (uint32_t v = 0; v < n; ++v) raw[v] = INF;
+  uint32_t id = 0;
+  for (uint32_t a = 0; a < n; ++a) {
+    if (leaves[a].empty()) continue;
+    size_t take = std::min<size_t>(leaves[a].size(), max_node - 1);
+    if (take + 1 < min_node) continue;
+    raw[a] = id;
+    for (size_t i = 0; i < take; ++i) raw[leaves[a][i]] = id;
+    ++id;
+  }
+}
+
+// compress_by_cluster + get_init_supernode_by_clusters (mirror-free branch) +
+// judge_out_bound_node + get_nodetype_mirror (compressor_base.h:3160-3568,
+// 3924-3956, 4144-4182).  Clusters are read in id order; a cluster smaller than
+// min_node is skipped; with is_use_mirror == false all four benefits equal
+// benefit[0] = inner_edges - |old_entry| * |old_exit| and the cluster is kept iff
+// (float)benefit > compress_threshold.  out: accepted cluster id (dense, in raw
+// order) per vertex, node type per vertex.
+void orc_sum_structure(void* h, const uint32_t* raw, uint32_t min_node, uint32_t max_node, double threshold,
+                       uint32_t* cid, uint8_t* type) {
+  Graph* g = (Graph*)h;
+  const uint32_t n = g->n, INF = 0xFFFFFFFFu;
+  uint32_t K = 0;
+  for (uint32_t v = 0; v < n; ++v) if (raw[v] != INF) K = std::max(K, raw[v] + 1);
+  std::vector<std::vector<uint32_t>> cl(K);
+  for (uint32_t v = 0; v < n; ++v) if (raw[v] != INF) cl[raw[v]].push_back(v);
+  std::vector<std::vector<uint32_t>> in(n);
+  for (uint32_t u = 0; u < n; ++u)
+    for (uint64_t e = g->off[u]; e < g->off[u + 1]; ++e) in[g->col[e]].push_back(u);
+  std::vector<uint8_t> entry(n, 0), exit_(n, 0);
+  std::vector<uint32_t> newid(K, INF);
+  uint32_t acc = 0;
+  for (uint32_t j = 0; j < K; ++j) {
+    const auto& P = cl[j];
+    if (P.size() < min_node || P.size() > max_node) continue;
+    long long inner = 0;
+    std::vector<uint32_t> S, B;
+    for (uint32_t u : P) {
+      bool is_in = false, is_out = false;
+      for (uint32_t w : in[u]) { if (raw[w] != j) is_in = true; else ++inner; }
+      for (uint64_t e = g->off[u]; e < g->off[u + 1]; ++e) if (raw[g->col[e]] != j) is_out = true;
+      if (is_in) S.push_back(u);
+      if (is_out) B.push_back(u);
+    }
+    float benefit = (float)((double)inner - (double)S.size() * (double)B.size());
+    if (benefit > (float)threshold) {
+      newid[j] = acc++;
+      for (uint32_t u : S) entry[u] = 1;
+      for (uint32_t u : B) exit_[u] = 1;
+    }
+  }
+  for (uint32_t v = 0; v < n; ++v) {
+    uint32_t c = raw[v] == INF ? INF : newid[raw[v]];
+    cid[v] = c;
+    if (c == INF) type[v] = 0;                              // SingleNode
+    else if (entry[v] && exit_[v]) type[v] = 3;             // BothOutInNode
+    else if (entry[v]) type[v] = 2;                         // OnlyInNode
+    else if (exit_[v]) type[v] = 1;                         // OnlyOutNode
+    else type[v] = 4;                                       // InnerNode
+  }
+}
+
+// build_iter_index_mirror / inc_run_to_convergence_mirror (iter_compressor.h:
+// 318-431): delta = 1 at the entry, r = 0; sweeps over the cluster's members in
+// id order (in place, Gauss-Seidel like the reference's serial loop): to_send =
+// delta_v, delta_v = 0, push to_send*d/outdeg(v) along v's edges inside the
+// cluster, r_v += to_send; stop when the sweep's sum |dr| <= thr or after maxit
+// sweeps.  Outputs r (reach: bound_delta for exits, inner_value otherwise) and the
+// leftover delta (inner_delta) for every vertex (0 outside the cluster).
+int orc_sum_shortcut(void* h, const uint32_t* cid, uint32_t entry, double d, double thr, int maxit, double* r,
+                     double* delta) {
+  Graph* g = (Graph*)h;
+  const uint32_t n = g->n;
+  const uint32_t k = cid[entry];
+  std::vector<uint32_t> mem;
+  for (uint32_t v = 0; v < n; ++v) { r[v] = 0; delta[v] = 0; if (cid[v] == k) mem.push_back(v); }
+  delta[entry] = 1.0;
+  int step = 0;
+  while (true) {
+    ++step;
+    double diff = 0;
+    for (uint32_t v : mem) {
+      double to_send = delta[v];
+      delta[v] = 0;
+      if (to_send != 0) {
+        double outv = to_send * d / (double)g->deg(v);
+        for (uint64_t e = g->off[v]; e < g->off[v + 1]; ++e)
+          if (cid[g->col[e]] == k) delta[g->col[e]] += outv;
+        r[v] += to_send;
+      }
+      diff += std::fabs(to_send);
+    }
+    if (diff <= thr || step > maxit) break;
+  }
+  return step;
+}
+
+// The SumInc PageRank query restated (sum_sync_iter_worker.h:348-438 first_step,
+// 949-1127 the compressed rounds by node type, 1385-1491 correction, then the
+// plain Ingress rounds), FP64, with the reference's serial-in-place order inside
+// a round.  Shortcuts use orc_sum_shortcut with thr = T/N*|C|, 100 sweeps.
+// Deviations shared with the device path: the sketch phase stops on |delta| of
+// Single/entry vertices + |bound| of exits (Inner residuals are left to the plain
+// phase), pending exit inflow is flushed before the correction, and the
+// correction spreads every non-zero entry accumulator.
+// r/delta: in = state (init != 0: start from delta = (1-d)/N, r = 0), out = result.
+// stats: [0] sketch rounds, [1] plain rounds, [2] edges, [3] final residual.
+void orc_sum_pr(void* h, const uint32_t* cid, const uint8_t* type, double d, double T, int max_rounds, int init,
+                const uint8_t* dirty_vertex, double* r, double* delta, double* stats) {
+  Graph* g = (Graph*)h;
+  const uint32_t n = g->n, INF = 0xFFFFFFFFu;
+  if (init) for (uint32_t v = 0; v < n; ++v) { r[v] = 0; delta[v] = (1.0 - d) / n; }
+  uint32_t K = 0;
+  for (uint32_t v = 0; v < n; ++v) if (cid[v] != INF) K = std::max(K, cid[v] + 1);
+  std::vector<std::vector<uint32_t>> mem(K);
+  for (uint32_t v = 0; v < n; ++v) if (cid[v] != INF) mem[cid[v]].push_back(v);
+  const double eps = T / n;
+  double edges = 0;
+  // shortcuts of every entry
+  std::vector<std::vector<std::pair<uint32_t, double>>> reach(n), resid(n);
+  {
+    std::vector<double> rr(n), dd(n);
+    for (uint32_t v = 0; v < n; ++v) {
+      if (type[v] != 2 && type[v] != 3) continue;
+      orc_sum_shortcut(h, cid, v, d, eps * mem[cid[v]].size(), 100, rr.data(), dd.data());
+      for (uint32_t x : mem[cid[v]]) {
+        if (rr[x] != 0) reach[v].push_back({x, rr[x]});
+        if (dd[x] != 0) resid[v].push_back({x, dd[x]});
+      }
+    }
+  }
+  auto is_exit = [&](uint32_t v) { return type[v] == 1 || type[v] == 3; };
+  auto push_external = [&](uint32_t v, double amount) {  // d*amount/outdeg over edges leaving v's cluster
+    uint32_t deg = g->deg(v);
+    if (!deg) return;
+    double outv = amount * d / deg;
+    for (uint64_t e = g->off[v]; e < g->off[v + 1]; ++e)
+      if (cid[g->col[e]] != cid[v]) { delta[g->col[e]] += outv; edges += 1; }
+  };
+  // precompute (precompute_spnode_one + first_step exit sends + precompute_spnode_two)
+  std::vector<double> rloc(n, 0.0);
+  for (uint32_t k = 0; k < K; ++k) {
+    bool dirty = !dirty_vertex;
+    if (dirty_vertex) for (uint32_t v : mem[k]) if (dirty_vertex[v]) dirty = true;
+    if (!dirty) continue;
+    double thr = eps * mem[k].size();
+    int step = 0;
+    while (true) {
+      ++step;
+      double diff = 0;
+      for (uint32_t v : mem[k]) {
+        double to_send = delta[v];
+        if (to_send == 0) continue;
+        delta[v] = 0;
+        double outv = to_send * d / (double)g->deg(v);
+        for (uint64_t e = g->off[v]; e < g->off[v + 1]; ++e)
+          if (cid[g->col[e]] == k) { delta[g->col[e]] += outv; edges += 1; }
+        rloc[v] += to_send;
+        diff += std::fabs(to_send);
+      }
+      if (diff <= thr || step > 100) break;
+    }
+    for (uint32_t v : mem[k]) {
+      if (is_exit(v) && rloc[v] != 0) push_external(v, rloc[v]);
+      r[v] += rloc[v];
+      rloc[v] = 0;
+    }
+  }
+  // sketch rounds
+  std::vector<double> bound(n, 0.0), spnode(n, 0.0);
+  int sk_rounds = 0;
+  while (true) {
+    double res = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+      if (type[v] != 4 && type[v] != 1) res += std::fabs(delta[v]);
+      if (is_exit(v)) res += std::fabs(bound[v]);
+    }
+    if (res < T || sk_rounds >= max_rounds) break;
+    ++sk_rounds;
+    for (uint32_t v = 0; v < n; ++v) {
+      uint8_t t = type[v];
+      if (t == 0 && std::fabs(delta[v]) > eps) {
+        double dv = delta[v];
+        delta[v] = 0;
+        uint32_t deg = g->deg(v);
+        if (deg) for (uint64_t e = g->off[v]; e < g->off[v + 1]; ++e) delta[g->col[e]] += dv * d / deg;
+        edges += deg;
+        r[v] += dv;
+      }
+      if ((t == 2 || t == 3) && std::fabs(delta[v]) > eps) {
+        double dv = delta[v];
+        delta[v] = 0;
+        spnode[v] += dv;
+        for (auto& xw : reach[v]) if (is_exit(xw.first)) { bound[xw.first] += xw.second * dv; edges += 1; }
+      }
+      if (is_exit(v) && std::fabs(bound[v]) > eps) {
+        double b = bound[v];
+        bound[v] = 0;
+        r[v] += b;
+        push_external(v, b);
+      }
+    }
+  }
+  // flush + correction
+  for (uint32_t v = 0; v < n; ++v)
+    if (is_exit(v) && bound[v] != 0) { double b = bound[v]; bound[v] = 0; r[v] += b; push_external(v, b); }
+  for (uint32_t v = 0; v < n; ++v) {
+    if (spnode[v] == 0) continue;
+    double s = spnode[v];
+    for (auto& xw : reach[v]) if (!is_exit(xw.first)) r[xw.first] += s * xw.second;
+    for (auto& xw : resid[v]) delta[xw.first] += s * xw.second;
+    spnode[v] = 0;
+  }
+  // plain residual phase
+  double st[3];
+  orc_pr_ingress(h, d, T, max_rounds, 0, r, delta, st);
+  if (stats) { stats[0] = sk_rounds; stats[1] = st[0]; stats[2] = edges + st[1]; stats[3] = st[2]; }
+}
+
 }  // extern "C"

@@ -51,8 +51,29 @@ class Params(C.Structure):
         ("pr_precision", C.c_int32),
         ("sssp_source", C.c_uint32),
         ("pull_threshold_div", C.c_int32),
-        ("reserved", C.c_int32 * 8),
+        ("shortcut_max_rounds", C.c_int32),
+        ("reserved", C.c_int32 * 7),
     ]
+
+
+class SketchInfo(C.Structure):
+    _fields_ = [
+        ("clusters", C.c_uint64),
+        ("raw_clusters", C.c_uint64),
+        ("entries", C.c_uint64),
+        ("clustered_vertices", C.c_uint64),
+        ("node_type_count", C.c_uint64 * 5),
+        ("sketch_edges", C.c_uint64),
+        ("internal_edges", C.c_uint64),
+        ("shortcut_slots", C.c_uint64),
+        ("max_cluster", C.c_uint64),
+        ("reserved", C.c_uint64 * 8),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("reserved", "node_type_count")}
+        d["node_type_count"] = list(self.node_type_count)
+        return d
 
 
 class Stats(C.Structure):
@@ -120,6 +141,13 @@ SIGNATURES = {
     "suminc_sssp_run": (i32, [vp, C.POINTER(Stats)]),
     "suminc_sssp_download": (i32, [vp, vp, vp]),
     "suminc_get_stats": (i32, [vp, C.POINTER(Stats)]),
+    "suminc_cluster_leaves": (i32, [vp, u32, u32, vp]),
+    "suminc_set_clusters": (i32, [vp, vp]),
+    "suminc_sum_build": (i32, [vp, u32, u32, f64]),
+    "suminc_sum_info": (i32, [vp, C.POINTER(SketchInfo)]),
+    "suminc_sum_download": (i32, [vp, vp, vp]),
+    "suminc_sum_pr_shortcut": (i32, [vp, u32, u32, vp, vp, vp, C.POINTER(u32), C.POINTER(u32)]),
+    "suminc_sum_pr_run": (i32, [vp, C.POINTER(Stats)]),
     "suminc_profile_enable": (i32, [vp, i32]),
     "suminc_profile_get": (i32, [vp, i32, C.POINTER(KernelProfile)]),
 }

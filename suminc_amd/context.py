@@ -156,6 +156,48 @@ class Context:
         d = np.ascontiguousarray(delta, np.float64)
         self._check(self._lib.suminc_pr_upload(self._h, L.ptr(r), L.ptr(d)))
 
+    # ---- supernode sketch -----------------------------------------------------
+    def cluster_leaves(self, max_node_num=1024, min_node_num=2):
+        n = self.graph_info()[0]
+        out = np.zeros(n, np.uint32)
+        self._check(self._lib.suminc_cluster_leaves(self._h, max_node_num, min_node_num, L.ptr(out)))
+        return out
+
+    def set_clusters(self, cluster_of):
+        c = np.ascontiguousarray(cluster_of, np.uint32)
+        self._check(self._lib.suminc_set_clusters(self._h, L.ptr(c)))
+
+    def sum_build(self, min_node_num=2, max_node_num=1024, compress_threshold=1.0):
+        self._check(self._lib.suminc_sum_build(self._h, min_node_num, max_node_num, compress_threshold))
+        return self.sum_info()
+
+    def sum_info(self) -> dict:
+        i = L.SketchInfo()
+        self._check(self._lib.suminc_sum_info(self._h, C.byref(i)))
+        return i.as_dict()
+
+    def sum_download(self):
+        n = self.graph_info()[0]
+        cid = np.zeros(n, np.uint32)
+        typ = np.zeros(n, np.uint8)
+        self._check(self._lib.suminc_sum_download(self._h, L.ptr(cid), L.ptr(typ)))
+        return cid, typ
+
+    def sum_pr_shortcut(self, entry):
+        cnt, nex = C.c_uint32(), C.c_uint32()
+        self._check(self._lib.suminc_sum_pr_shortcut(self._h, entry, 0, None, None, None, C.byref(cnt), C.byref(nex)))
+        mem = np.zeros(cnt.value, np.uint32)
+        reach = np.zeros(cnt.value)
+        resid = np.zeros(cnt.value)
+        self._check(self._lib.suminc_sum_pr_shortcut(self._h, entry, cnt.value, L.ptr(mem), L.ptr(reach), L.ptr(resid),
+                                                     C.byref(cnt), C.byref(nex)))
+        return mem, reach, resid, nex.value
+
+    def sum_pr_run(self) -> dict:
+        s = L.Stats()
+        self._check(self._lib.suminc_sum_pr_run(self._h, C.byref(s)))
+        return s.as_dict()
+
     # ---- SSSP -----------------------------------------------------------------
     def sssp_init(self, source=0):
         self._check(self._lib.suminc_sssp_init(self._h, source))

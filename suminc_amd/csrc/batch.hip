@@ -24,6 +24,7 @@
 #include "ctx.hpp"
 #include "device.hpp"
 #include "pr.hpp"
+#include "sketch.hpp"
 
 namespace suminc {
 
@@ -704,6 +705,8 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops) {
              b.g_expre, b.g_addpre, b.head, b.t_raw, b.t_list, b.t_gbeg, b.tix, 1);
   k_new_m<<<1, 64, 0, c->stream>>>(bc, b.g_expre.as<uint32_t>(), b.g_addpre.as<uint32_t>(), (uint32_t)c->m, bc);
   SUMINC_HIP(hipGetLastError());
+  // supernode sketch: clusters of both endpoints need a new precompute
+  sketch_mark_dirty(c, b.g_row.as<uint32_t>(), b.g_col.as<uint32_t>(), bc, N);
 
   // 3. PageRank reseed -1 on the old out-edges of touched sources
   if (c->pr_ready) {

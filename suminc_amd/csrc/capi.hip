@@ -5,6 +5,7 @@
 
 #include "ctx.hpp"
 #include "pr.hpp"
+#include "sketch.hpp"
 
 namespace suminc {
 
@@ -209,6 +210,40 @@ int suminc_sssp_run(suminc_ctx* c, suminc_stats* stats) {
 
 int suminc_sssp_download(suminc_ctx* c, uint32_t* dist, uint32_t* parent) {
   return guarded(c, [&] { sssp_download(c, dist, parent); });
+}
+
+int suminc_cluster_leaves(suminc_ctx* c, uint32_t max_node_num, uint32_t min_node_num, uint32_t* cluster_of_out) {
+  return guarded(c, [&] { cluster_leaves(c, max_node_num, min_node_num, cluster_of_out); });
+}
+
+int suminc_set_clusters(suminc_ctx* c, const uint32_t* cluster_of) {
+  return guarded(c, [&] { set_clusters(c, cluster_of); });
+}
+
+int suminc_sum_build(suminc_ctx* c, uint32_t min_node_num, uint32_t max_node_num, double compress_threshold) {
+  return guarded(c, [&] {
+    SUMINC_REQUIRE(max_node_num >= 1 && min_node_num <= max_node_num, SUMINC_E_INVALID, "bad cluster size bounds");
+    c->sk.raw_acc.release();  // a fresh build decides acceptance again
+    c->sk.built = false;
+    sketch_build(c, min_node_num, max_node_num, compress_threshold);
+  });
+}
+
+int suminc_sum_info(suminc_ctx* c, suminc_sketch_info* info) {
+  return guarded(c, [&] { sketch_info(c, info); });
+}
+
+int suminc_sum_download(suminc_ctx* c, uint32_t* cluster_of, uint8_t* node_type) {
+  return guarded(c, [&] { sketch_download(c, cluster_of, node_type); });
+}
+
+int suminc_sum_pr_shortcut(suminc_ctx* c, uint32_t entry, uint32_t cap, uint32_t* members, double* reach,
+                           double* residual, uint32_t* count, uint32_t* nexit) {
+  return guarded(c, [&] { sketch_pr_shortcut(c, entry, cap, members, reach, residual, count, nexit); });
+}
+
+int suminc_sum_pr_run(suminc_ctx* c, suminc_stats* stats) {
+  return guarded(c, [&] { sum_pr_run(c, stats); });
 }
 
 int suminc_get_stats(suminc_ctx* c, suminc_stats* stats) {

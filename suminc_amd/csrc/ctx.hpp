@@ -93,6 +93,35 @@ struct BatchWs {
   uint32_t tix_n = 0;
 };
 
+// Supernode sketch of the graph (mirror-free form of the reference's compressor):
+// accepted clusters, node types, cluster member layout, entry (supernode) list,
+// internal in-edges, sketch out-edges and the dense shortcut rows.
+struct Sketch {
+  bool built = false;
+  uint32_t K = 0;            // accepted clusters
+  uint32_t E = 0;            // entries (one supernode each)
+  uint32_t M = 0;            // clustered vertices
+  uint32_t Kraw = 0;         // raw clusters in the input assignment
+  uint64_t sc_len = 0;       // total length of the dense shortcut rows
+  uint64_t ib_m = 0;         // sketch out-edges
+  uint64_t ie_m = 0;         // internal edges
+  uint32_t max_cluster = 0;  // largest accepted cluster
+  uint64_t version = ~0ull;  // graph version the structure was built for
+  int pr_rows_for = -1;      // precision the PR shortcut rows were computed in (-1: none)
+  uint64_t pr_rows_version = ~0ull;
+  uint64_t sssp_rows_version = ~0ull;
+  DevBuf raw_cid, raw_acc;   // input assignment (n), acceptance per raw cluster (Kraw)
+  DevBuf cid, ntype, lidx, is_entry, is_exit;
+  DevBuf cl_off, cl_mem, cl_nexit, cl_eoff, cl_ent, ent_of, sc_base;
+  DevBuf ie_off, ie_src, ie_w;
+  DevBuf ib_off, ib_col, ib_w;
+  DevBuf sc_R, sc_D;         // PageRank rows (reach mass / residual) per entry
+  DevBuf ss_dist, ss_par;    // SSSP rows (in-cluster distance / parent) per entry
+  DevBuf bound, spnode;      // PageRank sketch accumulators
+  DevBuf dirty;              // K: cluster touched since its last precompute
+  uint32_t counts[8] = {0};  // node type histogram
+};
+
 }  // namespace suminc
 
 struct suminc_ctx {
@@ -114,8 +143,8 @@ struct suminc_ctx {
   int pr_prec = SUMINC_PR_FP32_KAHAN;
   suminc::DevBuf pr_delta, pr_rank, pr_comp, pr_contrib;
   // pull mode: static chunk list of high in-degree vertices, rebuilt per graph version
-  suminc::DevBuf pull_heavy;
-  uint64_t pull_heavy_count = 0, pull_heavy_version = ~0ull;
+  suminc::DevBuf pull_heavy, pull_short, pull_wave;
+  uint64_t pull_heavy_count = 0, pull_short_count = 0, pull_wave_count = 0, pull_heavy_version = ~0ull;
   uint64_t graph_version = 0;
 
   // SSSP state: packed {dist:hi32, parent:lo32}
@@ -127,6 +156,7 @@ struct suminc_ctx {
   // work lists and scratch
   suminc::DevBuf wl0, wl1, heavy, partial, tmp, tmp2, tmp3, tmp4, tmp5;
   suminc::BatchWs bw;
+  suminc::Sketch sk;
   suminc::DevBuf counters;                  // suminc::Counters
   suminc::Counters* host_counters = nullptr;  // pinned mirror
 
@@ -144,6 +174,7 @@ constexpr uint32_t INF32 = 0xFFFFFFFFu;
 constexpr int kBlock = 256;          // threads per block for edge-parallel kernels
 constexpr uint32_t kHeavyDeg = 256;  // adjacency longer than this goes to the chunk path
 constexpr uint32_t kChunk = 2048;    // edges per heavy chunk
+constexpr uint32_t kPullThread = 32; // pull: in-degree up to this is summed by one thread
 
 inline unsigned grid_for(uint64_t items, int block = kBlock, unsigned cap = 1u << 30) {
   uint64_t g = (items + block - 1) / block;

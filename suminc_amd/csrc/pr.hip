@@ -26,6 +26,7 @@
 #include "ctx.hpp"
 #include "device.hpp"
 #include "pr.hpp"
+#include "push.hpp"
 
 namespace suminc {
 
@@ -62,7 +63,7 @@ __global__ void __launch_bounds__(kBlock) k_pr_count(uint32_t n, const uint32_t*
       te += s_edges[w];
     }
     partial[blockIdx.x] = t;
-    if (te) atomicAdd(&cnt->round_edges, te);
+    reinterpret_cast<unsigned long long*>(partial)[gridDim.x + blockIdx.x] = te;
   }
 }
 
@@ -70,17 +71,28 @@ __global__ void __launch_bounds__(1024) k_pr_decide(const double* __restrict__ p
                                                     Counters* cnt, double T, int max_rounds, uint64_t m, int div,
                                                     uint8_t* mode_hist) {
   __shared__ double s[1024];
+  __shared__ unsigned long long se[1024];
   if (cnt->done) return;
+  const unsigned long long* pe = reinterpret_cast<const unsigned long long*>(partial) + nparts;
   double acc = 0;
-  for (uint32_t i = threadIdx.x; i < nparts; i += 1024) acc += partial[i];
+  unsigned long long acc_e = 0;
+  for (uint32_t i = threadIdx.x; i < nparts; i += 1024) {
+    acc += partial[i];
+    acc_e += pe[i];
+  }
   s[threadIdx.x] = acc;
+  se[threadIdx.x] = acc_e;
   __syncthreads();
   for (int o = 512; o > 0; o >>= 1) {
-    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    if (threadIdx.x < o) {
+      s[threadIdx.x] += s[threadIdx.x + o];
+      se[threadIdx.x] += se[threadIdx.x + o];
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     double r = s[0];
+    cnt->round_edges = se[0];
     cnt->residual = r;
     if (r != r || r > 1e300) cnt->aux[0] = 1;  // numeric failure flag
     if (!(r >= T) || (long long)cnt->rounds >= (long long)max_rounds || cnt->aux[0]) {
@@ -100,7 +112,7 @@ __global__ void __launch_bounds__(kBlock) k_pr_gather(uint32_t n, const uint32_t
                                                       T* __restrict__ rank, float* __restrict__ comp, double eps, T d,
                                                       uint32_t* __restrict__ wl_v, T* __restrict__ wl_c,
                                                       uint4* __restrict__ heavy, T* __restrict__ contrib,
-                                                      Counters* cnt) {
+                                                      Counters* cnt, unsigned long long* __restrict__ act_partial) {
   __shared__ unsigned long long s_act[kBlock / 64];
   if (cnt->done) return;
   const bool pull = cnt->mode != 0;
@@ -166,76 +178,46 @@ __global__ void __launch_bounds__(kBlock) k_pr_gather(uint32_t n, const uint32_t
   if (threadIdx.x == 0) {
     unsigned long long t = 0;
     for (int w = 0; w < kBlock / 64; ++w) t += s_act[w];
-    if (t) atomicAdd(&cnt->active_vertices, t);
+    act_partial[blockIdx.x] = t;
   }
 }
 
+// Pull, short rows (in-degree 1..kPullThread): one thread per vertex sums its
+// in-edges in order; no atomics (each vertex has one owner).
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_pr_push(const uint32_t* __restrict__ off, const uint32_t* __restrict__ col,
-                                                    T* __restrict__ delta, const uint32_t* __restrict__ wl_v,
-                                                    const T* __restrict__ wl_c, const uint4* __restrict__ heavy,
-                                                    const Counters* cnt) {
-  __shared__ uint32_t s_scan[kBlock + 1];
-  __shared__ uint32_t s_start[kBlock];
-  __shared__ uint32_t s_warp[kBlock / 64 + 1];
-  __shared__ T s_c[kBlock];
-  if (cnt->done || cnt->mode != 0) return;
-  const unsigned long long nl = cnt->wl_count[0];
-  const unsigned long long nh = cnt->heavy_count;
-  const unsigned long long tiles = (nl + kBlock - 1) / kBlock;
-  for (unsigned long long t = blockIdx.x; t < tiles; t += gridDim.x) {
-    unsigned long long i = t * kBlock + threadIdx.x;
-    uint32_t start = 0, len = 0;
-    if (i < nl) {
-      uint32_t v = wl_v[i];
-      start = off[v];
-      len = off[v + 1] - start;
-      s_c[threadIdx.x] = wl_c[i];
-    }
-    block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t, uint32_t e) {
-      unsafeAtomicAdd(&delta[col[e]], s_c[slot]);
-    });
-  }
-  for (unsigned long long h = blockIdx.x; h < nh; h += gridDim.x) {
-    uint4 hv = heavy[h];
-    T c;
-    if constexpr (sizeof(T) == 4) c = __uint_as_float(hv.z);
-    else c = __longlong_as_double((long long)(((unsigned long long)hv.w << 32) | hv.z));
-    uint32_t b = off[hv.x] + hv.y * kChunk;
-    uint32_t e = min(off[hv.x + 1], b + kChunk);
-    for (uint32_t k = b + threadIdx.x; k < e; k += kBlock) unsafeAtomicAdd(&delta[col[k]], c);
-  }
-}
-
-// Pull, light part: tiles of 256 consecutive vertices with in-degree <= 256.
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_pr_pull_light(uint32_t n, const uint32_t* __restrict__ ioff,
+__global__ void __launch_bounds__(kBlock) k_pr_pull_short(const uint32_t* __restrict__ list, uint32_t count,
+                                                          const uint32_t* __restrict__ ioff,
                                                           const uint32_t* __restrict__ icol,
                                                           const T* __restrict__ contrib, T* __restrict__ delta,
                                                           const Counters* cnt) {
-  __shared__ uint32_t s_scan[kBlock + 1];
-  __shared__ uint32_t s_start[kBlock];
-  __shared__ uint32_t s_warp[kBlock / 64 + 1];
-  __shared__ T s_acc[kBlock];
   if (cnt->done || cnt->mode != 1) return;
-  const uint32_t tiles = (n + kBlock - 1) / kBlock;
-  for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-    const uint32_t v = t * kBlock + threadIdx.x;
-    uint32_t start = 0, len = 0;
-    if (v < n) {
-      start = ioff[v];
-      len = ioff[v + 1] - start;
-      if (len > kHeavyDeg) len = 0;
-    }
-    s_acc[threadIdx.x] = T(0);
-    block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t, uint32_t e) {
-      T x = contrib[icol[e]];
-      if (x != T(0)) atomicAdd(&s_acc[slot], x);
-    });
-    if (len) {
-      T a = s_acc[threadIdx.x];
-      if (a != T(0)) delta[v] += a;
-    }
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
+    const uint32_t v = list[i];
+    const uint32_t b = ioff[v], e = ioff[v + 1];
+    T acc = T(0);
+    for (uint32_t k = b; k < e; ++k) acc += contrib[icol[k]];
+    if (acc != T(0)) delta[v] += acc;
+  }
+}
+
+// Pull, medium rows (kPullThread < in-degree <= kHeavyDeg): one wave per vertex,
+// lanes stride the row (coalesced column reads), shuffle reduction.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pr_pull_wave(const uint32_t* __restrict__ list, uint32_t count,
+                                                         const uint32_t* __restrict__ ioff,
+                                                         const uint32_t* __restrict__ icol,
+                                                         const T* __restrict__ contrib, T* __restrict__ delta,
+                                                         const Counters* cnt) {
+  if (cnt->done || cnt->mode != 1) return;
+  const uint32_t waves = gridDim.x * (kBlock / 64);
+  for (uint32_t i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); i < count; i += waves) {
+    const uint32_t v = list[i];
+    const uint32_t b = ioff[v], e = ioff[v + 1];
+    T acc = T(0);
+    for (uint32_t k = b + lane_id(); k < e; k += 64) acc += contrib[icol[k]];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane_id() == 0 && acc != T(0)) delta[v] += acc;
   }
 }
 
@@ -267,8 +249,20 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_heavy(const uint32_t* __rest
   }
 }
 
-__global__ void k_pr_end(Counters* cnt) {
-  if (threadIdx.x || cnt->done) return;
+__global__ void __launch_bounds__(1024) k_pr_end(Counters* cnt, const unsigned long long* __restrict__ act_partial,
+                                                 uint32_t nparts) {
+  __shared__ unsigned long long s[1024];
+  if (cnt->done) return;
+  unsigned long long a = 0;
+  for (uint32_t i = threadIdx.x; i < nparts; i += 1024) a += act_partial[i];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x) return;
+  cnt->active_vertices += s[0];
   cnt->edges += cnt->round_edges;
   if (cnt->mode) {
     cnt->pull_rounds += 1;
@@ -281,15 +275,23 @@ __global__ void k_pr_end(Counters* cnt) {
   cnt->heavy_count = 0;
 }
 
-// static chunk list of vertices with in-degree > kHeavyDeg (pull mode)
-__global__ void k_pull_heavy_list(uint32_t n, const uint32_t* ioff, uint2* out, unsigned long long* count) {
+// Static pull work lists, rebuilt per graph version: vertices by in-degree
+// class (short: 1..kPullThread, wave: ..kHeavyDeg) and 2048-edge chunks above.
+__global__ void k_pull_lists(uint32_t n, const uint32_t* ioff, uint32_t* short_list, uint32_t* wave_list,
+                             uint2* heavy, unsigned long long* counts) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  uint32_t deg = ioff[v + 1] - ioff[v];
-  if (deg <= kHeavyDeg) return;
-  uint32_t nch = (deg + kChunk - 1) / kChunk;
-  unsigned long long slot = atomicAdd(count, (unsigned long long)nch);
-  for (uint32_t k = 0; k < nch; ++k) out[slot + k] = make_uint2(v, k);
+  uint32_t deg = v < n ? ioff[v + 1] - ioff[v] : 0;
+  const bool is_short = deg >= 1 && deg <= kPullThread;
+  const bool is_wave = deg > kPullThread && deg <= kHeavyDeg;
+  unsigned long long s1 = wave_append(is_short, &counts[0]);
+  if (is_short) short_list[s1] = v;
+  unsigned long long s2 = wave_append(is_wave, &counts[1]);
+  if (is_wave) wave_list[s2] = v;
+  if (deg > kHeavyDeg) {
+    uint32_t nch = (deg + kChunk - 1) / kChunk;
+    unsigned long long slot = atomicAdd(&counts[2], (unsigned long long)nch);
+    for (uint32_t k = 0; k < nch; ++k) heavy[slot + k] = make_uint2(v, k);
+  }
 }
 
 template <class T>
@@ -349,7 +351,7 @@ void pr_alloc(suminc_ctx* c) {
   c->wl1.ensure(n * es + 8);
   // heavy chunks: at most m / kChunk + (number of heavy vertices) entries
   c->heavy.ensure((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
-  c->partial.ensure(((size_t)grid_for(n) + 1) * 8);
+  c->partial.ensure(((size_t)grid_for(n) + 1) * 3 * 8);  // residual | edges | active vertices
   c->tmp4.ensure(65536);  // per-round scatter mode history
 }
 
@@ -372,16 +374,21 @@ void pr_init(suminc_ctx* c) {
 static void ensure_pull_heavy(suminc_ctx* c) {
   if (c->pull_heavy_version == c->graph_version) return;
   c->pull_heavy.ensure((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint2));
+  c->pull_short.ensure((size_t)c->n * 4 + 4);
+  c->pull_wave.ensure((size_t)c->n * 4 + 4);
   DevBuf cnt;
-  cnt.ensure(8);
-  SUMINC_HIP(hipMemsetAsync(cnt.p, 0, 8, c->stream));
-  k_pull_heavy_list<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->in_off.as<uint32_t>(), c->pull_heavy.as<uint2>(),
-                                                             cnt.as<unsigned long long>());
+  cnt.ensure(32);
+  SUMINC_HIP(hipMemsetAsync(cnt.p, 0, 32, c->stream));
+  k_pull_lists<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->in_off.as<uint32_t>(), c->pull_short.as<uint32_t>(),
+                                                        c->pull_wave.as<uint32_t>(), c->pull_heavy.as<uint2>(),
+                                                        cnt.as<unsigned long long>());
   SUMINC_HIP(hipGetLastError());
-  unsigned long long h = 0;
-  SUMINC_HIP(hipMemcpyAsync(&h, cnt.p, 8, hipMemcpyDeviceToHost, c->stream));
+  unsigned long long h[4] = {0, 0, 0, 0};
+  SUMINC_HIP(hipMemcpyAsync(h, cnt.p, 32, hipMemcpyDeviceToHost, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
-  c->pull_heavy_count = h;
+  c->pull_short_count = h[0];
+  c->pull_wave_count = h[1];
+  c->pull_heavy_count = h[2];
   c->pull_heavy_version = c->graph_version;
 }
 
@@ -400,20 +407,27 @@ static void enqueue_round(suminc_ctx* c, double eps, double T_stop, int max_roun
   k_pr_gather<T><<<g, kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), c->pr_delta.as<T>(),
                                               c->pr_rank.as<T>(), c->pr_comp.as<float>(), eps, (T)c->prm.damping,
                                               c->wl0.as<uint32_t>(), c->wl1.as<T>(), c->heavy.as<uint4>(),
-                                              c->pr_contrib.as<T>(), cnt);
+                                              c->pr_contrib.as<T>(), cnt,
+                                              reinterpret_cast<unsigned long long*>(c->partial.as<double>()) + 2 * g);
   if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 4 * slot + 2), c->stream));
   k_pr_push<T><<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
                                                       c->pr_delta.as<T>(), c->wl0.as<uint32_t>(), c->wl1.as<T>(),
                                                       c->heavy.as<uint4>(), cnt);
   if (div > 0) {
-    k_pr_pull_light<T><<<push_grid(), kBlock, 0, c->stream>>>(c->n, c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
-                                                              c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt);
+    if (c->pull_short_count)
+      k_pr_pull_short<T><<<grid_for(c->pull_short_count, kBlock, 8192), kBlock, 0, c->stream>>>(
+          c->pull_short.as<uint32_t>(), (uint32_t)c->pull_short_count, c->in_off.as<uint32_t>(),
+          c->in_col.as<uint32_t>(), c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt);
+    if (c->pull_wave_count)
+      k_pr_pull_wave<T><<<grid_for(c->pull_wave_count, kBlock / 64, 8192), kBlock, 0, c->stream>>>(
+          c->pull_wave.as<uint32_t>(), (uint32_t)c->pull_wave_count, c->in_off.as<uint32_t>(),
+          c->in_col.as<uint32_t>(), c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt);
     k_pr_pull_heavy<T><<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
                                                               c->pr_contrib.as<T>(), c->pr_delta.as<T>(),
                                                               c->pull_heavy.as<uint2>(), c->pull_heavy_count, cnt);
   }
   if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 4 * slot + 3), c->stream));
-  k_pr_end<<<1, 64, 0, c->stream>>>(cnt);
+  k_pr_end<<<1, 1024, 0, c->stream>>>(cnt, reinterpret_cast<unsigned long long*>(c->partial.as<double>()) + 2 * g, g);
 }
 
 // Adds the profile of the first `rounds` enqueued rounds (the ones that did work).

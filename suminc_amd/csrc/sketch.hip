@@ -1,0 +1,1185 @@
+// sketch.hip -- supernode compression on the device (the mirror-free form of
+// the reference's compressor and SumSyncIterWorker phases):
+//
+//   clustering   built-in leaf clustering (each hub with its private
+//                neighbours), or a caller-supplied assignment like the
+//                reference's "<efile>.c_<max_node_num>" cluster file
+//                (compressor_base.h:3160-3250).
+//   acceptance   a cluster becomes a supernode iff its members are at least
+//                min_node_num and inner_edges - |entries|*|exits| > compress_threshold
+//                (get_init_supernode_by_clusters, compressor_base.h:3253-3568, the
+//                benefit[0] branch that mirror_k = 1e8 leaves; the other three
+//                benefits collapse onto it without mirrors).
+//   node types   Single / OnlyOut (exit) / OnlyIn (entry) / BothOutIn / Inner
+//                (get_nodetype_mirror, compressor_base.h:4144-4182, types.h:70-78).
+//   layout       members of cluster k at cl_mem[cl_off[k]..], exits first; one
+//                dense shortcut row per entry with one slot per member (reach mass
+//                and residual for PageRank; distance and parent for SSSP); the
+//                sketch out-edges ib (a Single's whole out-list, an exit's edges
+//                that leave its cluster; sketch2csr_mirror, compressor_base.h:
+//                4636-4899); the internal in-edges of every member.
+//   shortcuts    one workgroup per cluster runs the unit injection of every
+//                entry inside the cluster in LDS (build_iter_index_mirror +
+//                inc_run_to_convergence_mirror, iter_compressor.h:318-431): Jacobi
+//                rounds r += delta, delta <- internal pull with weight d/outdeg,
+//                until the mass processed in a round is <= T/N*|C| or 100 rounds.
+//   PR query     precompute of dirty clusters (precompute_spnode_one/two +
+//                first_step exit sends, iter_compressor.h:534-785,
+//                sum_sync_iter_worker.h:348-438), sketch rounds by node type
+//                (sum_sync_iter_worker.h:949-1111), flush of the exits' pending
+//                inflow, correction through inner_value/inner_delta
+//                (:1414-1469), then the plain residual phase.
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "ctx.hpp"
+#include "device.hpp"
+#include "pr.hpp"
+#include "push.hpp"
+#include "sketch.hpp"
+
+namespace suminc {
+
+constexpr int kMaxCluster = 2048;  // LDS-resident cluster bound of the shortcut kernels
+
+__device__ __forceinline__ uint32_t row_of(const uint32_t* off, uint32_t n, uint32_t e) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = lo + ((hi - lo + 1) >> 1);
+    if (off[mid] <= e) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class F>
+static void cub(suminc_ctx* c, F&& f) {
+  size_t bytes = 0;
+  SUMINC_HIP(f((void*)nullptr, bytes));
+  c->tmp.ensure(bytes);
+  SUMINC_HIP(f(c->tmp.p, bytes));
+}
+
+__global__ void k_fill32(uint32_t* p, uint64_t n, uint32_t v) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// leaf clustering
+// ---------------------------------------------------------------------------
+// anchor[v] = the single distinct neighbour of v (in or out), else INF
+__global__ void k_leaf_anchor(uint32_t n, const uint32_t* ooff, const uint32_t* ocol, const uint32_t* ioff,
+                              const uint32_t* icol, uint32_t* anchor) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  uint32_t dout = ooff[v + 1] - ooff[v], din = ioff[v + 1] - ioff[v];
+  uint32_t h = INF32;
+  bool leaf = dout + din > 0 && dout <= 1 && din <= 1;
+  if (leaf && dout == 1) h = ocol[ooff[v]];
+  if (leaf && din == 1) {
+    uint32_t x = icol[ioff[v]];
+    if (h != INF32 && x != h) leaf = false;
+    h = x;
+  }
+  anchor[v] = leaf ? h : INF32;
+}
+
+// Leaves keyed by (anchor, leaf); a leaf pair anchors on its smaller id.
+__global__ void k_leaf_keys(uint32_t n, const uint32_t* anchor, uint64_t* keys) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  uint32_t h = anchor[v];
+  bool member = h != INF32 && !(anchor[h] == v && v < h);
+  keys[v] = member ? (((uint64_t)h << 32) | v) : ~0ull;
+}
+
+__global__ void k_seg_heads(const uint64_t* keys, uint32_t L, uint8_t* head) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L) return;
+  head[i] = (i == 0) || (keys[i] >> 32) != (keys[i - 1] >> 32);
+}
+
+__global__ void k_seg_ok(const uint32_t* seg_start, const unsigned* nseg_p, uint32_t L, uint32_t max_node,
+                         uint32_t min_node, uint32_t* ok) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t S = *nseg_p;
+  if (s > S) return;
+  if (s == S) { ok[S] = 0; return; }
+  uint32_t len = (s + 1 < S ? seg_start[s + 1] : L) - seg_start[s];
+  uint32_t take = min(len, max_node - 1);
+  ok[s] = (take + 1 >= min_node) ? 1u : 0u;
+}
+
+__global__ void k_leaf_assign(const uint64_t* keys, uint32_t L, const uint32_t* seg_start, const unsigned* nseg_p,
+                              const uint32_t* ok, const uint32_t* okpre, uint32_t max_node, uint32_t* raw) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L) return;
+  const uint32_t S = *nseg_p;
+  uint32_t lo = 0, hi = S - 1;  // segment of position i
+  while (lo < hi) {
+    uint32_t mid = lo + ((hi - lo + 1) >> 1);
+    if (seg_start[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  if (!ok[lo]) return;
+  uint32_t rank = i - seg_start[lo];
+  if (rank >= max_node - 1) return;
+  uint32_t id = okpre[lo];
+  raw[(uint32_t)keys[i]] = id;
+  if (rank == 0) raw[(uint32_t)(keys[i] >> 32)] = id;
+}
+
+void cluster_leaves(suminc_ctx* c, uint32_t max_node, uint32_t min_node, uint32_t* out) {
+  SUMINC_REQUIRE(c->has_graph, SUMINC_E_STATE, "no graph");
+  SUMINC_REQUIRE(max_node >= 2 && max_node <= (uint32_t)kMaxCluster, SUMINC_E_INVALID,
+                 "max_node_num must be in [2, 2048]");
+  const uint32_t n = c->n;
+  Sketch& s = c->sk;
+  DevBuf anchor, ka, kb, head, seg, nseg, ok, okpre;
+  anchor.ensure((size_t)n * 4); ka.ensure((size_t)n * 8); kb.ensure((size_t)n * 8); head.ensure(n + 1);
+  seg.ensure((size_t)n * 4 + 4); nseg.ensure(8); ok.ensure(((size_t)n + 1) * 4); okpre.ensure(((size_t)n + 1) * 4);
+  k_leaf_anchor<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
+                                                       c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
+                                                       anchor.as<uint32_t>());
+  k_leaf_keys<<<grid_for(n), kBlock, 0, c->stream>>>(n, anchor.as<uint32_t>(), ka.as<uint64_t>());
+  cub(c, [&](void* p, size_t& b) {
+    return rocprim::radix_sort_keys(p, b, ka.as<uint64_t>(), kb.as<uint64_t>(), (size_t)n, 0, 64, c->stream);
+  });
+  // leaves are the keys before the first sentinel
+  cub(c, [&](void* p, size_t& b) {
+    return rocprim::lower_bound(p, b, kb.as<uint64_t>(), rocprim::constant_iterator<uint64_t>(~0ull),
+                                okpre.as<uint32_t>(), (size_t)n, (size_t)1, rocprim::less<uint64_t>(), c->stream);
+  });
+  uint32_t L = 0;
+  SUMINC_HIP(hipMemcpyAsync(&L, okpre.p, 4, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  s.raw_cid.ensure((size_t)n * 4);
+  k_fill32<<<grid_for(n), kBlock, 0, c->stream>>>(s.raw_cid.as<uint32_t>(), n, INF32);
+  uint32_t K = 0;
+  if (L) {
+    k_seg_heads<<<grid_for(L), kBlock, 0, c->stream>>>(kb.as<uint64_t>(), L, head.as<uint8_t>());
+    cub(c, [&](void* p, size_t& b) {
+      return rocprim::select(p, b, rocprim::counting_iterator<uint32_t>(0), head.as<uint8_t>(), seg.as<uint32_t>(),
+                             nseg.as<unsigned>(), (size_t)L, c->stream);
+    });
+    k_seg_ok<<<grid_for((uint64_t)L + 1), kBlock, 0, c->stream>>>(seg.as<uint32_t>(), nseg.as<unsigned>(), L, max_node,
+                                                                  min_node, ok.as<uint32_t>());
+    cub(c, [&](void* p, size_t& b) {
+      return rocprim::exclusive_scan(p, b, ok.as<uint32_t>(), okpre.as<uint32_t>(), 0u, (size_t)L + 1,
+                                     rocprim::plus<uint32_t>(), c->stream);
+    });
+    k_leaf_assign<<<grid_for(L), kBlock, 0, c->stream>>>(kb.as<uint64_t>(), L, seg.as<uint32_t>(), nseg.as<unsigned>(),
+                                                         ok.as<uint32_t>(), okpre.as<uint32_t>(), max_node,
+                                                         s.raw_cid.as<uint32_t>());
+    unsigned S = 0;
+    SUMINC_HIP(hipMemcpyAsync(&S, nseg.p, 4, hipMemcpyDeviceToHost, c->stream));
+    SUMINC_HIP(hipStreamSynchronize(c->stream));
+    SUMINC_HIP(hipMemcpyAsync(&K, okpre.as<uint32_t>() + S, 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  SUMINC_HIP(hipGetLastError());
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  s.Kraw = K;
+  s.built = false;
+  s.raw_acc.release();
+  if (out) SUMINC_HIP(hipMemcpy(out, s.raw_cid.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+}
+
+void set_clusters(suminc_ctx* c, const uint32_t* cluster_of) {
+  SUMINC_REQUIRE(c->has_graph, SUMINC_E_STATE, "no graph");
+  SUMINC_REQUIRE(cluster_of, SUMINC_E_INVALID, "null cluster assignment");
+  const uint32_t n = c->n;
+  uint32_t K = 0;
+  for (uint32_t v = 0; v < n; ++v)
+    if (cluster_of[v] != INF32) {
+      SUMINC_REQUIRE(cluster_of[v] < n, SUMINC_E_INVALID, "cluster id must be < n or SUMINC_INF_U32");
+      K = std::max(K, cluster_of[v] + 1);
+    }
+  Sketch& s = c->sk;
+  s.raw_cid.ensure((size_t)n * 4);
+  SUMINC_HIP(hipMemcpy(s.raw_cid.p, cluster_of, (size_t)n * 4, hipMemcpyHostToDevice));
+  s.Kraw = K;
+  s.built = false;
+  s.raw_acc.release();
+}
+
+// ---------------------------------------------------------------------------
+// structure
+// ---------------------------------------------------------------------------
+__global__ void k_edge_class(uint32_t n, uint32_t m, const uint32_t* off, const uint32_t* col, const uint32_t* raw,
+                             uint32_t* inner, uint8_t* is_entry, uint8_t* is_exit) {
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  uint32_t u = row_of(off, n, e), v = col[e];
+  uint32_t cu = raw[u], cv = raw[v];
+  if (cu != INF32 && cu == cv) {
+    atomicAdd(&inner[cu], 1u);
+  } else {
+    if (cv != INF32) is_entry[v] = 1;
+    if (cu != INF32) is_exit[u] = 1;
+  }
+}
+
+__global__ void k_raw_counts(uint32_t n, const uint32_t* raw, const uint8_t* is_entry, const uint8_t* is_exit,
+                             uint32_t* size, uint32_t* ns, uint32_t* nb) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  uint32_t k = raw[v];
+  if (k == INF32) return;
+  atomicAdd(&size[k], 1u);
+  if (is_entry[v]) atomicAdd(&ns[k], 1u);
+  if (is_exit[v]) atomicAdd(&nb[k], 1u);
+}
+
+// benefit[0] of get_init_supernode_by_clusters, computed in double and compared
+// as float like the reference's std::vector<float> benefit (compressor_base.h:3463-3481)
+__global__ void k_accept(uint32_t K, const uint32_t* size, const uint32_t* inner, const uint32_t* ns,
+                         const uint32_t* nb, uint32_t min_node, uint32_t max_node, float threshold, uint8_t* acc) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  float benefit = (float)((double)inner[k] - (double)ns[k] * (double)nb[k]);
+  acc[k] = size[k] >= min_node && size[k] <= max_node && benefit > threshold;
+}
+
+__global__ void k_acc_u32(uint32_t K, const uint8_t* acc, uint32_t* out) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k <= K) out[k] = k < K ? acc[k] : 0u;
+}
+
+__global__ void k_assign(uint32_t n, const uint32_t* raw, const uint8_t* acc, const uint32_t* newid,
+                         const uint8_t* is_entry, const uint8_t* is_exit, uint32_t* cid, uint8_t* ntype,
+                         unsigned long long* keys, uint32_t* hist, int kbits) {
+  __shared__ uint32_t s_h[8];
+  if (threadIdx.x < 8) s_h[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) {
+    uint32_t k = raw[v];
+    uint32_t id = (k != INF32 && acc[k]) ? newid[k] : INF32;
+    cid[v] = id;
+    uint8_t t;
+    if (id == INF32) {
+      t = 0;  // SingleNode
+    } else {
+      bool en = is_entry[v], ex = is_exit[v];
+      t = en && ex ? 3 : en ? 2 : ex ? 1 : 4;  // BothOutIn / OnlyIn / OnlyOut / Inner
+    }
+    ntype[v] = t;
+    atomicAdd(&s_h[t], 1u);
+    keys[v] = id == INF32 ? ~0ull
+                          : (((unsigned long long)id << 33) | ((unsigned long long)(is_exit[v] ? 0 : 1) << 32) | v);
+  }
+  __syncthreads();
+  if (threadIdx.x < 8 && s_h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], s_h[threadIdx.x]);
+  (void)kbits;
+}
+
+__global__ void k_members(const unsigned long long* keys, uint32_t M, uint32_t* cl_mem, uint32_t* cl_key) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  cl_mem[i] = (uint32_t)keys[i];
+  cl_key[i] = (uint32_t)(keys[i] >> 33);
+}
+
+__global__ void k_row_ptr32(const uint32_t* rowkey, uint32_t M, uint32_t K, uint32_t* off) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > M) return;
+  int64_t row = (i == M) ? (int64_t)K : (int64_t)rowkey[i];
+  int64_t prev = (i == 0) ? -1 : (int64_t)rowkey[i - 1];
+  for (int64_t r = prev + 1; r <= row; ++r) off[r] = i;
+}
+
+__global__ void k_member_index(uint32_t M, const uint32_t* cl_mem, const uint32_t* cl_key, const uint32_t* cl_off,
+                               const uint8_t* is_exit, const uint8_t* is_entry, uint32_t* lidx, uint32_t* nexit,
+                               uint8_t* ent_flag) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  uint32_t v = cl_mem[i], k = cl_key[i];
+  lidx[v] = i - cl_off[k];
+  if (is_exit[v]) atomicAdd(&nexit[k], 1u);
+  ent_flag[i] = is_entry[v];
+}
+
+__global__ void k_entries(uint32_t E, const uint32_t* pos, const uint32_t* cl_mem, const uint32_t* cl_key,
+                          const uint32_t* cl_off, uint32_t* cl_ent, uint32_t* ent_key, uint32_t* ent_of,
+                          unsigned long long* rowlen) {
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e > E) return;
+  if (e == E) { rowlen[E] = 0; return; }
+  uint32_t i = pos[e];
+  uint32_t v = cl_mem[i], k = cl_key[i];
+  cl_ent[e] = v;
+  ent_key[e] = k;
+  ent_of[v] = e;
+  rowlen[e] = cl_off[k + 1] - cl_off[k];
+}
+
+// internal in-edges per member slot, and sketch out-edge counts per vertex
+__global__ void k_count_internal(uint32_t n, uint32_t m, const uint32_t* off, const uint32_t* col,
+                                 const uint32_t* cid, const uint32_t* lidx, const uint32_t* cl_off, uint32_t* ie_cnt,
+                                 uint32_t* ext_cnt) {
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  uint32_t u = row_of(off, n, e), v = col[e];
+  uint32_t cu = cid[u], cv = cid[v];
+  if (cu != INF32 && cu == cv) atomicAdd(&ie_cnt[cl_off[cu] + lidx[v]], 1u);
+  else if (cu != INF32) atomicAdd(&ext_cnt[u], 1u);
+}
+
+__global__ void k_ib_len(uint32_t n, const uint32_t* off, const uint32_t* cid, const uint8_t* ntype,
+                         const uint32_t* ext_cnt, uint32_t* len) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > n) return;
+  if (v == n) { len[n] = 0; return; }
+  uint8_t t = ntype[v];
+  len[v] = cid[v] == INF32 ? off[v + 1] - off[v] : ((t == 1 || t == 3) ? ext_cnt[v] : 0u);
+}
+
+__global__ void k_fill_internal(uint32_t n, uint32_t m, const uint32_t* off, const uint32_t* col, const uint32_t* w,
+                                const uint32_t* cid, const uint32_t* lidx, const uint32_t* cl_off,
+                                const uint32_t* ie_off, uint32_t* ie_cur, uint32_t* ie_src, uint32_t* ie_deg,
+                                uint32_t* ie_wt, const uint32_t* ib_off, uint32_t* ib_cur, uint32_t* ib_col,
+                                uint32_t* ib_w) {
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  uint32_t u = row_of(off, n, e), v = col[e];
+  uint32_t cu = cid[u], cv = cid[v];
+  if (cu != INF32 && cu == cv) {
+    uint32_t slot = cl_off[cu] + lidx[v];
+    uint32_t p = ie_off[slot] + atomicAdd(&ie_cur[slot], 1u);
+    ie_src[p] = lidx[u];
+    ie_deg[p] = off[u + 1] - off[u];
+    ie_wt[p] = w ? w[e] : 1u;
+  } else {
+    uint32_t p;
+    if (cu == INF32) p = ib_off[u] + (e - off[u]);       // Single: the whole row, in order
+    else p = ib_off[u] + atomicAdd(&ib_cur[u], 1u);      // exit: its external edges
+    ib_col[p] = v;
+    if (ib_w) ib_w[p] = w ? w[e] : 1u;
+  }
+}
+
+__global__ void k_max_cluster(uint32_t K, const uint32_t* cl_off, uint32_t* out) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < K) atomicMax(out, cl_off[k + 1] - cl_off[k]);
+}
+
+// Builds (or rebuilds after a batch) the sketch structure from the raw cluster
+// assignment.  Acceptance is decided on the first build and then kept, like the
+// reference keeps its supernodes across update batches.
+void sketch_build(suminc_ctx* c, uint32_t min_node, uint32_t max_node, double threshold) {
+  SUMINC_REQUIRE(c->has_graph, SUMINC_E_STATE, "no graph");
+  Sketch& s = c->sk;
+  SUMINC_REQUIRE(s.raw_cid.p, SUMINC_E_STATE, "no cluster assignment (call suminc_cluster_* first)");
+  const uint32_t n = c->n, m = (uint32_t)c->m;
+  const uint32_t Kr = s.Kraw;
+  DevBuf inner, size, ns, nb, acc32, newid, keys_a, keys_b, ext_cnt, ibl, hist, ckey, entf, epos, ecount, ekey,
+      rowlen, iecnt, iecur, ibcur, mx;
+  s.is_entry.ensure(n); s.is_exit.ensure(n);
+  SUMINC_HIP(hipMemsetAsync(s.is_entry.p, 0, n, c->stream));
+  SUMINC_HIP(hipMemsetAsync(s.is_exit.p, 0, n, c->stream));
+  inner.ensure((size_t)Kr * 4 + 4);
+  SUMINC_HIP(hipMemsetAsync(inner.p, 0, (size_t)Kr * 4 + 4, c->stream));
+  if (m)
+    k_edge_class<<<grid_for(m), kBlock, 0, c->stream>>>(n, m, c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
+                                                        s.raw_cid.as<uint32_t>(), inner.as<uint32_t>(),
+                                                        s.is_entry.as<uint8_t>(), s.is_exit.as<uint8_t>());
+  const bool decide = !s.raw_acc.p;
+  if (decide) {
+    size.ensure((size_t)Kr * 4 + 4); ns.ensure((size_t)Kr * 4 + 4); nb.ensure((size_t)Kr * 4 + 4);
+    for (DevBuf* b : {&size, &ns, &nb}) SUMINC_HIP(hipMemsetAsync(b->p, 0, (size_t)Kr * 4 + 4, c->stream));
+    k_raw_counts<<<grid_for(n), kBlock, 0, c->stream>>>(n, s.raw_cid.as<uint32_t>(), s.is_entry.as<uint8_t>(),
+                                                        s.is_exit.as<uint8_t>(), size.as<uint32_t>(),
+                                                        ns.as<uint32_t>(), nb.as<uint32_t>());
+    s.raw_acc.ensure(Kr + 1);
+    if (Kr)
+      k_accept<<<grid_for(Kr), kBlock, 0, c->stream>>>(Kr, size.as<uint32_t>(), inner.as<uint32_t>(),
+                                                       ns.as<uint32_t>(), nb.as<uint32_t>(), min_node, max_node,
+                                                       (float)threshold, s.raw_acc.as<uint8_t>());
+  }
+  acc32.ensure(((size_t)Kr + 1) * 4); newid.ensure(((size_t)Kr + 1) * 4);
+  k_acc_u32<<<grid_for((uint64_t)Kr + 1), kBlock, 0, c->stream>>>(Kr, s.raw_acc.as<uint8_t>(), acc32.as<uint32_t>());
+  cub(c, [&](void* p, size_t& b) {
+    return rocprim::exclusive_scan(p, b, acc32.as<uint32_t>(), newid.as<uint32_t>(), 0u, (size_t)Kr + 1,
+                                   rocprim::plus<uint32_t>(), c->stream);
+  });
+  uint32_t K = 0;
+  SUMINC_HIP(hipMemcpyAsync(&K, newid.as<uint32_t>() + Kr, 4, hipMemcpyDeviceToHost, c->stream));
+  // node types + member keys
+  s.cid.ensure((size_t)n * 4); s.ntype.ensure(n); s.lidx.ensure((size_t)n * 4);
+  keys_a.ensure((size_t)n * 8); keys_b.ensure((size_t)n * 8); hist.ensure(32);
+  SUMINC_HIP(hipMemsetAsync(hist.p, 0, 32, c->stream));
+  k_assign<<<grid_for(n), kBlock, 0, c->stream>>>(n, s.raw_cid.as<uint32_t>(), s.raw_acc.as<uint8_t>(),
+                                                  newid.as<uint32_t>(), s.is_entry.as<uint8_t>(),
+                                                  s.is_exit.as<uint8_t>(), s.cid.as<uint32_t>(), s.ntype.as<uint8_t>(),
+                                                  keys_a.as<unsigned long long>(), hist.as<uint32_t>(), 0);
+  cub(c, [&](void* p, size_t& b) {
+    return rocprim::radix_sort_keys(p, b, keys_a.as<uint64_t>(), keys_b.as<uint64_t>(), (size_t)n, 0, 64, c->stream);
+  });
+  SUMINC_HIP(hipMemcpyAsync(s.counts, hist.p, 32, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  const uint32_t M = n - s.counts[0];
+  s.K = K;
+  s.M = M;
+  // cluster member layout
+  s.cl_mem.ensure((size_t)M * 4 + 4); ckey.ensure((size_t)M * 4 + 4); s.cl_off.ensure(((size_t)K + 1) * 4);
+  s.cl_nexit.ensure((size_t)K * 4 + 4); entf.ensure(M + 1);
+  SUMINC_HIP(hipMemsetAsync(s.cl_nexit.p, 0, (size_t)K * 4 + 4, c->stream));
+  if (M) {
+    k_members<<<grid_for(M), kBlock, 0, c->stream>>>(keys_b.as<unsigned long long>(), M, s.cl_mem.as<uint32_t>(),
+                                                     ckey.as<uint32_t>());
+    k_row_ptr32<<<grid_for((uint64_t)M + 1), kBlock, 0, c->stream>>>(ckey.as<uint32_t>(), M, K,
+                                                                     s.cl_off.as<uint32_t>());
+    k_member_index<<<grid_for(M), kBlock, 0, c->stream>>>(M, s.cl_mem.as<uint32_t>(), ckey.as<uint32_t>(),
+                                                          s.cl_off.as<uint32_t>(), s.is_exit.as<uint8_t>(),
+                                                          s.is_entry.as<uint8_t>(), s.lidx.as<uint32_t>(),
+                                                          s.cl_nexit.as<uint32_t>(), entf.as<uint8_t>());
+  } else {
+    SUMINC_HIP(hipMemsetAsync(s.cl_off.p, 0, ((size_t)K + 1) * 4, c->stream));
+  }
+  // entries
+  epos.ensure((size_t)M * 4 + 4); ecount.ensure(8);
+  SUMINC_HIP(hipMemsetAsync(ecount.p, 0, 8, c->stream));
+  if (M)
+    cub(c, [&](void* p, size_t& b) {
+      return rocprim::select(p, b, rocprim::counting_iterator<uint32_t>(0), entf.as<uint8_t>(), epos.as<uint32_t>(),
+                             ecount.as<unsigned>(), (size_t)M, c->stream);
+    });
+  uint32_t E = 0;
+  SUMINC_HIP(hipMemcpyAsync(&E, ecount.p, 4, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  s.E = E;
+  s.cl_ent.ensure((size_t)E * 4 + 4); ekey.ensure((size_t)E * 4 + 4); s.ent_of.ensure((size_t)n * 4);
+  s.cl_eoff.ensure(((size_t)K + 1) * 4); rowlen.ensure(((size_t)E + 1) * 8); s.sc_base.ensure(((size_t)E + 1) * 8);
+  k_fill32<<<grid_for(n), kBlock, 0, c->stream>>>(s.ent_of.as<uint32_t>(), n, INF32);
+  k_entries<<<grid_for((uint64_t)E + 1), kBlock, 0, c->stream>>>(E, epos.as<uint32_t>(), s.cl_mem.as<uint32_t>(),
+                                                                 ckey.as<uint32_t>(), s.cl_off.as<uint32_t>(),
+                                                                 s.cl_ent.as<uint32_t>(), ekey.as<uint32_t>(),
+                                                                 s.ent_of.as<uint32_t>(),
+                                                                 rowlen.as<unsigned long long>());
+  k_row_ptr32<<<grid_for((uint64_t)E + 1), kBlock, 0, c->stream>>>(ekey.as<uint32_t>(), E, K,
+                                                                   s.cl_eoff.as<uint32_t>());
+  cub(c, [&](void* p, size_t& b) {
+    return rocprim::exclusive_scan(p, b, rowlen.as<unsigned long long>(), s.sc_base.as<unsigned long long>(), 0ull,
+                                   (size_t)E + 1, rocprim::plus<unsigned long long>(), c->stream);
+  });
+  unsigned long long sc_len = 0;
+  SUMINC_HIP(hipMemcpyAsync(&sc_len, s.sc_base.as<unsigned long long>() + E, 8, hipMemcpyDeviceToHost, c->stream));
+  // internal in-edges and sketch out-edges
+  iecnt.ensure(((size_t)M + 1) * 4); ext_cnt.ensure((size_t)n * 4); ibl.ensure(((size_t)n + 1) * 4);
+  SUMINC_HIP(hipMemsetAsync(iecnt.p, 0, ((size_t)M + 1) * 4, c->stream));
+  SUMINC_HIP(hipMemsetAsync(ext_cnt.p, 0, (size_t)n * 4, c->stream));
+  if (m)
+    k_count_internal<<<grid_for(m), kBlock, 0, c->stream>>>(n, m, c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
+                                                            s.cid.as<uint32_t>(), s.lidx.as<uint32_t>(),
+                                                            s.cl_off.as<uint32_t>(), iecnt.as<uint32_t>(),
+                                                            ext_cnt.as<uint32_t>());
+  k_ib_len<<<grid_for((uint64_t)n + 1), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), s.cid.as<uint32_t>(),
+                                                                s.ntype.as<uint8_t>(), ext_cnt.as<uint32_t>(),
+                                                                ibl.as<uint32_t>());
+  s.ie_off.ensure(((size_t)M + 1) * 4); s.ib_off.ensure(((size_t)n + 1) * 4);
+  cub(c, [&](void* p, size_t& b) {
+    return rocprim::exclusive_scan(p, b, iecnt.as<uint32_t>(), s.ie_off.as<uint32_t>(), 0u, (size_t)M + 1,
+                                   rocprim::plus<uint32_t>(), c->stream);
+  });
+  cub(c, [&](void* p, size_t& b) {
+    return rocprim::exclusive_scan(p, b, ibl.as<uint32_t>(), s.ib_off.as<uint32_t>(), 0u, (size_t)n + 1,
+                                   rocprim::plus<uint32_t>(), c->stream);
+  });
+  uint32_t ie_m = 0, ib_m = 0;
+  SUMINC_HIP(hipMemcpyAsync(&ie_m, s.ie_off.as<uint32_t>() + M, 4, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipMemcpyAsync(&ib_m, s.ib_off.as<uint32_t>() + n, 4, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  s.ie_m = ie_m;
+  s.ib_m = ib_m;
+  s.sc_len = sc_len;
+  s.ie_src.ensure((size_t)ie_m * 4 + 4); s.ie_w.ensure(((size_t)ie_m * 2 + 2) * 4);
+  s.ib_col.ensure((size_t)ib_m * 4 + 4);
+  if (c->wkind == SUMINC_W_U32) s.ib_w.ensure((size_t)ib_m * 4 + 4);
+  iecur.ensure(((size_t)M + 1) * 4); ibcur.ensure((size_t)n * 4);
+  SUMINC_HIP(hipMemsetAsync(iecur.p, 0, ((size_t)M + 1) * 4, c->stream));
+  SUMINC_HIP(hipMemsetAsync(ibcur.p, 0, (size_t)n * 4, c->stream));
+  if (m)
+    k_fill_internal<<<grid_for(m), kBlock, 0, c->stream>>>(
+        n, m, c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
+        c->wkind == SUMINC_W_U32 ? c->out_w.as<uint32_t>() : nullptr, s.cid.as<uint32_t>(), s.lidx.as<uint32_t>(),
+        s.cl_off.as<uint32_t>(), s.ie_off.as<uint32_t>(), iecur.as<uint32_t>(), s.ie_src.as<uint32_t>(),
+        s.ie_w.as<uint32_t>(), s.ie_w.as<uint32_t>() + ie_m + 1, s.ib_off.as<uint32_t>(), ibcur.as<uint32_t>(),
+        s.ib_col.as<uint32_t>(), c->wkind == SUMINC_W_U32 ? s.ib_w.as<uint32_t>() : nullptr);
+  mx.ensure(4);
+  SUMINC_HIP(hipMemsetAsync(mx.p, 0, 4, c->stream));
+  if (K) k_max_cluster<<<grid_for(K), kBlock, 0, c->stream>>>(K, s.cl_off.as<uint32_t>(), mx.as<uint32_t>());
+  uint32_t maxc = 0;
+  SUMINC_HIP(hipMemcpyAsync(&maxc, mx.p, 4, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipGetLastError());
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  SUMINC_REQUIRE(maxc <= (uint32_t)kMaxCluster, SUMINC_E_TOO_LARGE, "an accepted cluster has more than 2048 members");
+  s.max_cluster = maxc;
+  // every cluster needs a precompute after a (re)build of the PageRank side
+  s.dirty.ensure((size_t)K + 1);
+  if (!s.built || s.K != K) SUMINC_HIP(hipMemsetAsync(s.dirty.p, 1, (size_t)K + 1, c->stream));
+  s.version = c->graph_version;
+  s.built = true;
+  s.pr_rows_version = ~0ull;
+  s.sssp_rows_version = ~0ull;
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+}
+
+// A batch marks the clusters of both endpoints of every op dirty (the reference's
+// update_cluster_ids, compressor_base.h:5634-5852); the structure is rebuilt on
+// the next query.
+__global__ void k_mark_dirty(const uint32_t* rows, const uint32_t* cols, const unsigned* countp, uint32_t N,
+                             const uint32_t* cid, uint8_t* dirty) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= N || g >= *countp) return;
+  uint32_t a = cid[rows[g]], b = cid[cols[g]];
+  if (a != INF32) dirty[a] = 1;
+  if (b != INF32) dirty[b] = 1;
+}
+
+void sketch_mark_dirty(suminc_ctx* c, const uint32_t* rows, const uint32_t* cols, const unsigned* countp, uint32_t N) {
+  Sketch& s = c->sk;
+  if (!s.built || !N) return;
+  k_mark_dirty<<<grid_for(N), kBlock, 0, c->stream>>>(rows, cols, countp, N, s.cid.as<uint32_t>(),
+                                                      s.dirty.as<uint8_t>());
+  SUMINC_HIP(hipGetLastError());
+}
+
+void sketch_refresh(suminc_ctx* c) {
+  Sketch& s = c->sk;
+  SUMINC_REQUIRE(s.built, SUMINC_E_STATE, "no sketch (call suminc_sum_build first)");
+  if (s.version != c->graph_version) sketch_build(c, 0, 0, 0.0);  // acceptance kept
+}
+
+// ---------------------------------------------------------------------------
+// PageRank shortcuts: unit injection of every entry inside its cluster (LDS)
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T block_sum(T v, T* s_red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  __syncthreads();
+  if (lane_id() == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  T t = T(0);
+  for (int w = 0; w < kBlock / 64; ++w) t += s_red[w];
+  return t;
+}
+
+// Jacobi rounds inside one cluster held in LDS: r += delta; delta <- sum over
+// internal in-edges of delta[src] * d / outdeg(src); stops when the mass
+// processed in a round is <= thr or after maxit rounds.  Returns the rounds run
+// and the internal edges read.
+template <class T>
+__device__ uint32_t cluster_iterate(uint32_t size, uint32_t slot0, const uint32_t* __restrict__ ie_off,
+                                    const uint32_t* __restrict__ ie_src, const uint32_t* __restrict__ ie_deg, T d,
+                                    double thr, int maxit, T* s_r, T* s_d, T* s_n, T* s_red) {
+  uint32_t rounds = 0;
+  for (int it = 0; it < maxit; ++it) {
+    T processed = T(0);
+    for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+      T acc = T(0);
+      for (uint32_t p = ie_off[slot0 + j]; p < ie_off[slot0 + j + 1]; ++p) acc += s_d[ie_src[p]] * d / (T)ie_deg[p];
+      s_n[j] = acc;
+      processed += fabs_t(s_d[j]);
+    }
+    T tot = block_sum(processed, s_red);
+    for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+      s_r[j] += s_d[j];
+      s_d[j] = s_n[j];
+    }
+    __syncthreads();
+    ++rounds;
+    if ((double)tot <= thr) break;
+  }
+  return rounds;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pr_shortcuts(const uint32_t* __restrict__ cl_off,
+                                                         const uint32_t* __restrict__ cl_eoff,
+                                                         const uint32_t* __restrict__ cl_ent,
+                                                         const uint32_t* __restrict__ lidx,
+                                                         const unsigned long long* __restrict__ sc_base,
+                                                         const uint32_t* __restrict__ ie_off,
+                                                         const uint32_t* __restrict__ ie_src,
+                                                         const uint32_t* __restrict__ ie_deg, T d, double thr_per_vertex,
+                                                         int maxit, T* __restrict__ R, T* __restrict__ D,
+                                                         unsigned long long* work) {
+  __shared__ T s_r[kMaxCluster], s_d[kMaxCluster], s_n[kMaxCluster];
+  __shared__ T s_red[kBlock / 64];
+  const uint32_t k = blockIdx.x;
+  const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0;
+  const double thr = thr_per_vertex * size;
+  unsigned long long edges = 0;
+  for (uint32_t e = cl_eoff[k]; e < cl_eoff[k + 1]; ++e) {
+    const uint32_t src = lidx[cl_ent[e]];
+    for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+      s_r[j] = T(0);
+      s_d[j] = j == src ? T(1) : T(0);
+    }
+    __syncthreads();
+    uint32_t rounds = cluster_iterate<T>(size, slot0, ie_off, ie_src, ie_deg, d, thr, maxit, s_r, s_d, s_n, s_red);
+    edges += (unsigned long long)rounds * (ie_off[slot0 + size] - ie_off[slot0]);
+    const unsigned long long base = sc_base[e];
+    for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+      R[base + j] = s_r[j];
+      D[base + j] = s_d[j];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && edges) atomicAdd(work, edges);
+}
+
+template <class T>
+static void compute_pr_rows(suminc_ctx* c) {
+  Sketch& s = c->sk;
+  const size_t es = sizeof(T);
+  s.sc_R.ensure(s.sc_len * es + es);
+  s.sc_D.ensure(s.sc_len * es + es);
+  DevBuf work;
+  work.ensure(8);
+  SUMINC_HIP(hipMemsetAsync(work.p, 0, 8, c->stream));
+  const double thr_pv = c->prm.termcheck_threshold / (double)c->n;  // T / N_old * |C| per cluster
+  const int maxit = c->prm.shortcut_max_rounds > 0 ? c->prm.shortcut_max_rounds : 100;
+  if (s.K)
+    k_pr_shortcuts<T><<<s.K, kBlock, 0, c->stream>>>(
+        s.cl_off.as<uint32_t>(), s.cl_eoff.as<uint32_t>(), s.cl_ent.as<uint32_t>(), s.lidx.as<uint32_t>(),
+        s.sc_base.as<unsigned long long>(), s.ie_off.as<uint32_t>(), s.ie_src.as<uint32_t>(), s.ie_w.as<uint32_t>(),
+        (T)c->prm.damping, thr_pv, maxit, s.sc_R.as<T>(), s.sc_D.as<T>(), work.as<unsigned long long>());
+  SUMINC_HIP(hipGetLastError());
+  unsigned long long w = 0;
+  SUMINC_HIP(hipMemcpyAsync(&w, work.p, 8, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  c->stats.reserved[4] = (double)w;  // internal edges read by the shortcut build
+  s.pr_rows_for = c->pr_prec;
+  s.pr_rows_version = c->graph_version;
+}
+
+// ---------------------------------------------------------------------------
+// PageRank query on the sketch
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void kahan_add(T* rank, float* comp, uint32_t v, T x) {
+  if constexpr (sizeof(T) == 4) {
+    float r = rank[v], cc = comp[v];
+    float y = (float)x - cc;
+    float t = r + y;
+    comp[v] = (t - r) - y;
+    rank[v] = t;
+  } else {
+    rank[v] += x;
+  }
+}
+
+// Appends a push item (row v of the sketch out-edges, contribution cval).
+template <class T>
+__device__ __forceinline__ void push_item(bool want, uint32_t v, T cval, uint32_t rowlen, uint32_t* wl_v, T* wl_c,
+                                          uint4* heavy, Counters* cnt) {
+  bool light = want && rowlen > 0 && rowlen <= kHeavyDeg;
+  if (want && rowlen > kHeavyDeg) {
+    uint32_t nch = (rowlen + kChunk - 1) / kChunk;
+    unsigned long long slot = atomicAdd(&cnt->heavy_count, (unsigned long long)nch);
+    for (uint32_t k = 0; k < nch; ++k) {
+      uint4 h;
+      h.x = v;
+      h.y = k;
+      if constexpr (sizeof(T) == 4) {
+        h.z = __float_as_uint((float)cval);
+        h.w = 0;
+      } else {
+        unsigned long long bits = __double_as_longlong((double)cval);
+        h.z = (uint32_t)bits;
+        h.w = (uint32_t)(bits >> 32);
+      }
+      heavy[slot + k] = h;
+    }
+  }
+  unsigned long long slot = wave_append(light, &cnt->wl_count[0]);
+  if (light) {
+    wl_v[slot] = v;
+    wl_c[slot] = cval;
+  }
+}
+
+struct SkArgs {
+  uint32_t n;
+  const uint32_t* off;     // full out-CSR row pointers (degrees)
+  const uint32_t* ib_off;  // sketch out-edge rows
+  const uint8_t* ntype;
+  const uint32_t* ent_of;
+  uint32_t* wl_v;
+  uint4* heavy;
+  uint32_t* sc_e;          // shortcut items: entry index
+  Counters* cnt;
+  double* partial;
+  const uint32_t* cid;
+  const uint32_t* cl_nexit;
+};
+
+// residual of the sketch phase: |delta| of Single/entry vertices + |bound| of exits
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_sk_count(SkArgs a, const T* __restrict__ delta,
+                                                     const T* __restrict__ bound) {
+  __shared__ double s_sum[kBlock / 64];
+  if (a.cnt->done) return;
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  double x = 0;
+  if (v < a.n) {
+    uint8_t t = a.ntype[v];
+    if (t != 4 && t != 1) x += (double)fabs_t(delta[v]);
+    if (t == 1 || t == 3) x += (double)fabs_t(bound[v]);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  if (lane_id() == 0) s_sum[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += s_sum[w];
+    a.partial[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_sk_decide(const double* __restrict__ partial, uint32_t nparts, Counters* cnt,
+                                                    double T, int max_rounds) {
+  __shared__ double s[1024];
+  if (cnt->done) return;
+  double acc = 0;
+  for (uint32_t i = threadIdx.x; i < nparts; i += 1024) acc += partial[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double r = s[0];
+    cnt->residual = r;
+    if (r != r || r > 1e300) cnt->aux[0] = 1;
+    if (!(r >= T) || (long long)cnt->rounds >= (long long)max_rounds || cnt->aux[0]) {
+      cnt->done = 1;
+      return;
+    }
+    cnt->mode = 0;
+    cnt->rounds += 1;
+  }
+}
+
+// Node-type dispatch of one sketch round (sum_sync_iter_worker.h:949-1111).
+// flush != 0: only exits, every non-zero inflow (end of the sketch phase).
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_sk_gather(SkArgs a, T* __restrict__ delta, T* __restrict__ bound,
+                                                      T* __restrict__ spnode, T* __restrict__ rank,
+                                                      float* __restrict__ comp, T* __restrict__ wl_c,
+                                                      T* __restrict__ sc_x, double eps, T d, int flush) {
+  if (a.cnt->done) return;
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  bool want_push = false, want_sc = false;
+  T cval = T(0), sval = T(0);
+  uint32_t rowlen = 0;
+  if (v < a.n) {
+    const uint8_t t = a.ntype[v];
+    const uint32_t deg = a.off[v + 1] - a.off[v];
+    rowlen = a.ib_off[v + 1] - a.ib_off[v];
+    if (t == 0 && !flush) {  // SingleNode: plain push over its out-list
+      T dv = delta[v];
+      if ((double)fabs_t(dv) > eps) {
+        delta[v] = T(0);
+        kahan_add(rank, comp, v, dv);
+        if (deg) { want_push = true; cval = d * dv / (T)deg; }
+      }
+    }
+    if ((t == 2 || t == 3) && !flush) {  // entry: inject into the shortcut
+      T dv = delta[v];
+      if ((double)fabs_t(dv) > eps) {
+        delta[v] = T(0);
+        spnode[v] += dv;
+        want_sc = true;
+        sval = dv;
+      }
+    }
+    if (t == 1 || t == 3) {  // exit: inflow becomes rank, d/outdeg of it leaves the cluster
+      T b = bound[v];
+      if (flush ? b != T(0) : (double)fabs_t(b) > eps) {
+        bound[v] = T(0);
+        kahan_add(rank, comp, v, b);
+        if (deg && rowlen) { want_push = true; cval = d * b / (T)deg; }
+      }
+    }
+  }
+  push_item<T>(want_push, v, cval, rowlen, a.wl_v, wl_c, a.heavy, a.cnt);
+  unsigned long long slot = wave_append(want_sc, &a.cnt->wl_count[1]);
+  if (want_sc) {
+    a.sc_e[slot] = a.ent_of[v];
+    sc_x[slot] = sval;
+  }
+  // edges of this round: sketch out-edges pushed + shortcut slots used
+  __shared__ unsigned long long s_e[kBlock / 64];
+  unsigned long long ed = (want_push ? rowlen : 0u) + (want_sc ? a.cl_nexit[a.cid[v]] : 0u);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) ed += __shfl_xor(ed, o);
+  if (lane_id() == 0) s_e[threadIdx.x >> 6] = ed;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += s_e[w];
+    reinterpret_cast<unsigned long long*>(a.partial)[gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// Shortcut items: bound[x] += R[e][x] * delta for the exits x of the entry's
+// cluster (g_index_function, pagerank_ingress.h:102-125).
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_sk_shortcut(const uint32_t* __restrict__ sc_e, const T* __restrict__ sc_x,
+                                                        const Counters* cnt, const uint32_t* __restrict__ cl_ent,
+                                                        const uint32_t* __restrict__ cid,
+                                                        const uint32_t* __restrict__ cl_off,
+                                                        const uint32_t* __restrict__ cl_mem,
+                                                        const uint32_t* __restrict__ cl_nexit,
+                                                        const unsigned long long* __restrict__ sc_base,
+                                                        const T* __restrict__ R, T* __restrict__ bound) {
+  __shared__ uint32_t s_scan[kBlock + 1];
+  __shared__ uint32_t s_start[kBlock];
+  __shared__ uint32_t s_warp[kBlock / 64 + 1];
+  __shared__ uint32_t s_mbase[kBlock];
+  __shared__ unsigned long long s_rbase[kBlock];
+  __shared__ T s_x[kBlock];
+  if (cnt->done) return;
+  const unsigned long long ni = cnt->wl_count[1];
+  const unsigned long long tiles = (ni + kBlock - 1) / kBlock;
+  for (unsigned long long t = blockIdx.x; t < tiles; t += gridDim.x) {
+    unsigned long long i = t * kBlock + threadIdx.x;
+    uint32_t len = 0;
+    if (i < ni) {
+      uint32_t e = sc_e[i];
+      uint32_t k = cid[cl_ent[e]];
+      len = cl_nexit[k];
+      s_mbase[threadIdx.x] = cl_off[k];
+      s_rbase[threadIdx.x] = sc_base[e];
+      s_x[threadIdx.x] = sc_x[i];
+    }
+    block_adjacency<kBlock>(0, len, s_scan, s_start, s_warp, [&](int slot, uint32_t j, uint32_t) {
+      T w = R[s_rbase[slot] + j];
+      if (w != T(0)) unsafeAtomicAdd(&bound[cl_mem[s_mbase[slot] + j]], w * s_x[slot]);
+    });
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_sk_end(Counters* cnt, const unsigned long long* __restrict__ pe,
+                                                 uint32_t nparts) {
+  __shared__ unsigned long long s[1024];
+  if (cnt->done) return;
+  unsigned long long acc = 0;
+  for (uint32_t i = threadIdx.x; i < nparts; i += 1024) acc += pe[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x) return;
+  cnt->edges += s[0];
+  cnt->round_edges = 0;
+  cnt->wl_count[0] = 0;
+  cnt->wl_count[1] = 0;
+  cnt->heavy_count = 0;
+}
+
+// push list of a flush/precompute pass: run once, then clear
+__global__ void k_lists_clear(Counters* cnt) {
+  if (threadIdx.x) return;
+  cnt->wl_count[0] = 0;
+  cnt->wl_count[1] = 0;
+  cnt->heavy_count = 0;
+  cnt->done = 0;
+  cnt->mode = 0;
+}
+
+// Precompute of one dirty cluster per workgroup: the members' pending delta is
+// iterated inside the cluster (run_to_convergence_for_precpt), the mass that
+// arrived becomes rank, the residual stays in delta, and every exit sends d/outdeg
+// of its arrived mass over its external edges (first_step, :389-434).
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_sk_precompute(SkArgs a, const uint32_t* __restrict__ cl_off,
+                                                          const uint32_t* __restrict__ cl_mem,
+                                                          const uint32_t* __restrict__ cl_nexit,
+                                                          const uint32_t* __restrict__ ie_off,
+                                                          const uint32_t* __restrict__ ie_src,
+                                                          const uint32_t* __restrict__ ie_deg, uint8_t* dirty,
+                                                          T* __restrict__ delta, T* __restrict__ rank,
+                                                          float* __restrict__ comp, T* __restrict__ wl_c, T d,
+                                                          double thr_per_vertex, int maxit,
+                                                          unsigned long long* work) {
+  __shared__ T s_r[kMaxCluster], s_d[kMaxCluster], s_n[kMaxCluster];
+  __shared__ T s_red[kBlock / 64];
+  const uint32_t k = blockIdx.x;
+  if (!dirty[k]) return;
+  const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0, nexit = cl_nexit[k];
+  for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+    uint32_t v = cl_mem[slot0 + j];
+    s_r[j] = T(0);
+    s_d[j] = delta[v];
+  }
+  __syncthreads();
+  uint32_t rounds = cluster_iterate<T>(size, slot0, ie_off, ie_src, ie_deg, d, thr_per_vertex * size, maxit, s_r,
+                                       s_d, s_n, s_red);
+  for (uint32_t j0 = 0; j0 < size; j0 += kBlock) {  // uniform trip count: push_item uses wave ballots
+    const uint32_t j = j0 + threadIdx.x;
+    bool want = false;
+    uint32_t v = 0, rowlen = 0;
+    T cval = T(0);
+    if (j < size) {
+      v = cl_mem[slot0 + j];
+      delta[v] = s_d[j];
+      if (s_r[j] != T(0)) kahan_add(rank, comp, v, s_r[j]);
+      if (j < nexit && s_r[j] != T(0)) {
+        const uint32_t deg = a.off[v + 1] - a.off[v];
+        rowlen = a.ib_off[v + 1] - a.ib_off[v];
+        if (deg && rowlen) { want = true; cval = d * s_r[j] / (T)deg; }
+      }
+    }
+    push_item<T>(want, v, cval, rowlen, a.wl_v, wl_c, a.heavy, a.cnt);
+  }
+  if (threadIdx.x == 0) {
+    dirty[k] = 0;
+    atomicAdd(work, (unsigned long long)rounds * (ie_off[slot0 + size] - ie_off[slot0]));
+  }
+}
+
+// Correction (sum_sync_iter_worker.h:1414-1469): the delta accumulated at every
+// entry is spread with its shortcut row: rank of non-exits += s * R, delta of all
+// members += s * D (inner_value / inner_delta).  One workgroup per cluster sums
+// its entries in LDS so every member is updated once.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_sk_correct(const uint32_t* __restrict__ cl_off,
+                                                       const uint32_t* __restrict__ cl_mem,
+                                                       const uint32_t* __restrict__ cl_nexit,
+                                                       const uint32_t* __restrict__ cl_eoff,
+                                                       const uint32_t* __restrict__ cl_ent,
+                                                       const unsigned long long* __restrict__ sc_base,
+                                                       const T* __restrict__ R, const T* __restrict__ D,
+                                                       T* __restrict__ spnode, T* __restrict__ delta,
+                                                       T* __restrict__ rank, float* __restrict__ comp) {
+  __shared__ T s_ar[kMaxCluster], s_ad[kMaxCluster];
+  const uint32_t k = blockIdx.x;
+  const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0, nexit = cl_nexit[k];
+  for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+    s_ar[j] = T(0);
+    s_ad[j] = T(0);
+  }
+  __syncthreads();
+  for (uint32_t e = cl_eoff[k]; e < cl_eoff[k + 1]; ++e) {
+    const uint32_t s = cl_ent[e];
+    const T sp = spnode[s];
+    if (sp == T(0)) continue;  // uniform across the block
+    const unsigned long long base = sc_base[e];
+    for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+      if (j >= nexit) s_ar[j] += sp * R[base + j];
+      s_ad[j] += sp * D[base + j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) spnode[s] = T(0);
+  }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+    const uint32_t v = cl_mem[slot0 + j];
+    if (s_ar[j] != T(0)) kahan_add(rank, comp, v, s_ar[j]);
+    if (s_ad[j] != T(0)) delta[v] += s_ad[j];
+  }
+}
+
+template <class T>
+static void sk_push_pass(suminc_ctx* c) {
+  Sketch& s = c->sk;
+  k_pr_push<T><<<push_grid(), kBlock, 0, c->stream>>>(s.ib_off.as<uint32_t>(), s.ib_col.as<uint32_t>(),
+                                                      c->pr_delta.as<T>(), c->wl0.as<uint32_t>(), c->wl1.as<T>(),
+                                                      c->heavy.as<uint4>(), dev_counters(c));
+}
+
+template <class T>
+static void sum_pr_run_t(suminc_ctx* c, suminc_stats* st) {
+  Sketch& s = c->sk;
+  Counters* cnt = dev_counters(c);
+  const uint32_t n = c->n;
+  const double T_stop = c->prm.termcheck_threshold;
+  const double eps = T_stop / (double)n;
+  const int max_rounds = c->prm.max_rounds > 0 ? c->prm.max_rounds : 1000000;
+  const int maxit = c->prm.shortcut_max_rounds > 0 ? c->prm.shortcut_max_rounds : 100;
+  const T d = (T)c->prm.damping;
+  const size_t es = sizeof(T);
+  if (s.pr_rows_version != c->graph_version || s.pr_rows_for != c->pr_prec) compute_pr_rows<T>(c);
+  s.bound.ensure((size_t)n * es);
+  s.spnode.ensure((size_t)n * es);
+  SUMINC_HIP(hipMemsetAsync(s.bound.p, 0, (size_t)n * es, c->stream));
+  SUMINC_HIP(hipMemsetAsync(s.spnode.p, 0, (size_t)n * es, c->stream));
+  c->tmp2.ensure((size_t)n * 4 + 4);
+  c->tmp3.ensure((size_t)n * es + 8);
+  SkArgs a;
+  a.n = n;
+  a.off = c->out_off.as<uint32_t>();
+  a.ib_off = s.ib_off.as<uint32_t>();
+  a.ntype = s.ntype.as<uint8_t>();
+  a.ent_of = s.ent_of.as<uint32_t>();
+  a.wl_v = c->wl0.as<uint32_t>();
+  a.heavy = c->heavy.as<uint4>();
+  a.sc_e = c->tmp2.as<uint32_t>();
+  a.cnt = cnt;
+  a.partial = c->partial.as<double>();
+  a.cid = s.cid.as<uint32_t>();
+  a.cl_nexit = s.cl_nexit.as<uint32_t>();
+  DevBuf work;
+  work.ensure(8);
+  SUMINC_HIP(hipMemsetAsync(work.p, 0, 8, c->stream));
+  suminc_stats out{};
+  SUMINC_HIP(hipEventRecord(c->ev_start, c->stream));
+  // 1. precompute of dirty clusters + the exits' sends
+  k_lists_clear<<<1, 64, 0, c->stream>>>(cnt);
+  if (s.K)
+    k_sk_precompute<T><<<s.K, kBlock, 0, c->stream>>>(a, s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(),
+                                                      s.cl_nexit.as<uint32_t>(), s.ie_off.as<uint32_t>(),
+                                                      s.ie_src.as<uint32_t>(), s.ie_w.as<uint32_t>(),
+                                                      s.dirty.as<uint8_t>(), c->pr_delta.as<T>(), c->pr_rank.as<T>(),
+                                                      c->pr_comp.as<float>(), c->wl1.as<T>(), d, eps, maxit,
+                                                      work.as<unsigned long long>());
+  sk_push_pass<T>(c);
+  k_lists_clear<<<1, 64, 0, c->stream>>>(cnt);
+  SUMINC_HIP(hipGetLastError());
+  // 2. sketch rounds
+  const unsigned g = grid_for(n);
+  int group = 4;
+  while (true) {
+    for (int i = 0; i < group; ++i) {
+      k_sk_count<T><<<g, kBlock, 0, c->stream>>>(a, c->pr_delta.as<T>(), s.bound.as<T>());
+      k_sk_decide<<<1, 1024, 0, c->stream>>>(a.partial, g, cnt, T_stop, max_rounds);
+      k_sk_gather<T><<<g, kBlock, 0, c->stream>>>(a, c->pr_delta.as<T>(), s.bound.as<T>(), s.spnode.as<T>(),
+                                                  c->pr_rank.as<T>(), c->pr_comp.as<float>(), c->wl1.as<T>(),
+                                                  c->tmp3.as<T>(), eps, d, 0);
+      sk_push_pass<T>(c);
+      k_sk_shortcut<T><<<push_grid(), kBlock, 0, c->stream>>>(
+          c->tmp2.as<uint32_t>(), c->tmp3.as<T>(), cnt, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(),
+          s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.cl_nexit.as<uint32_t>(),
+          s.sc_base.as<unsigned long long>(), s.sc_R.as<T>(), s.bound.as<T>());
+      k_sk_end<<<1, 1024, 0, c->stream>>>(cnt, reinterpret_cast<unsigned long long*>(a.partial) + g, g);
+    }
+    SUMINC_HIP(hipGetLastError());
+    read_counters(c);
+    if (c->host_counters->done) break;
+    if (group < 32) group *= 2;
+  }
+  const Counters* h = c->host_counters;
+  out.reserved[5] = (double)h->rounds;  // sketch rounds
+  SUMINC_REQUIRE(!h->aux[0], SUMINC_E_NUMERIC, "NaN/Inf in the sketch residual");
+  // 3. flush the exits' pending inflow, 4. correction
+  k_lists_clear<<<1, 64, 0, c->stream>>>(cnt);
+  k_sk_gather<T><<<g, kBlock, 0, c->stream>>>(a, c->pr_delta.as<T>(), s.bound.as<T>(), s.spnode.as<T>(),
+                                              c->pr_rank.as<T>(), c->pr_comp.as<float>(), c->wl1.as<T>(),
+                                              c->tmp3.as<T>(), eps, d, 1);
+  sk_push_pass<T>(c);
+  k_lists_clear<<<1, 64, 0, c->stream>>>(cnt);
+  if (s.K)
+    k_sk_correct<T><<<s.K, kBlock, 0, c->stream>>>(s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(),
+                                                   s.cl_nexit.as<uint32_t>(), s.cl_eoff.as<uint32_t>(),
+                                                   s.cl_ent.as<uint32_t>(), s.sc_base.as<unsigned long long>(),
+                                                   s.sc_R.as<T>(), s.sc_D.as<T>(), s.spnode.as<T>(),
+                                                   c->pr_delta.as<T>(), c->pr_rank.as<T>(), c->pr_comp.as<float>());
+  SUMINC_HIP(hipGetLastError());
+  SUMINC_HIP(hipEventRecord(c->ev_stop, c->stream));
+  SUMINC_HIP(hipEventSynchronize(c->ev_stop));
+  float ms_sketch = 0;
+  SUMINC_HIP(hipEventElapsedTime(&ms_sketch, c->ev_start, c->ev_stop));
+  unsigned long long wk = 0;
+  SUMINC_HIP(hipMemcpy(&wk, work.p, 8, hipMemcpyDeviceToHost));
+  const double sketch_edges = (double)h->edges;
+  // 5. residual phase: plain Ingress rounds on the full graph
+  suminc_stats rs{};
+  pr_run(c, &rs);
+  out.rounds = rs.rounds + out.reserved[5];
+  out.edges_traversed = rs.edges_traversed + sketch_edges + (double)wk;
+  out.residual = rs.residual;
+  out.kernel_ms = rs.kernel_ms + ms_sketch;
+  out.push_rounds = rs.push_rounds;
+  out.pull_rounds = rs.pull_rounds;
+  out.reserved[6] = ms_sketch;
+  out.reserved[7] = rs.rounds;  // residual-phase rounds
+  out.reserved[8] = (double)wk;  // internal edges read by precompute
+  c->stats = out;
+  if (st) *st = out;
+}
+
+void sum_pr_run(suminc_ctx* c, suminc_stats* st) {
+  SUMINC_REQUIRE(c->pr_ready, SUMINC_E_STATE, "PageRank state not initialised");
+  sketch_refresh(c);
+  if (c->pr_prec == SUMINC_PR_FP64) sum_pr_run_t<double>(c, st);
+  else sum_pr_run_t<float>(c, st);
+}
+
+void sketch_info(suminc_ctx* c, suminc_sketch_info* out) {
+  SUMINC_REQUIRE(out, SUMINC_E_INVALID, "null output");
+  sketch_refresh(c);
+  Sketch& s = c->sk;
+  std::memset(out, 0, sizeof(*out));
+  out->clusters = s.K;
+  out->raw_clusters = s.Kraw;
+  out->entries = s.E;
+  out->clustered_vertices = s.M;
+  for (int t = 0; t < 5; ++t) out->node_type_count[t] = s.counts[t];
+  out->sketch_edges = s.ib_m;
+  out->internal_edges = s.ie_m;
+  out->shortcut_slots = s.sc_len;
+  out->max_cluster = s.max_cluster;
+}
+
+void sketch_download(suminc_ctx* c, uint32_t* cluster_of, uint8_t* node_type) {
+  sketch_refresh(c);
+  Sketch& s = c->sk;
+  if (cluster_of) SUMINC_HIP(hipMemcpy(cluster_of, s.cid.p, (size_t)c->n * 4, hipMemcpyDeviceToHost));
+  if (node_type) SUMINC_HIP(hipMemcpy(node_type, s.ntype.p, c->n, hipMemcpyDeviceToHost));
+}
+
+// Shortcut row of entry vertex v: members of its cluster (exits first) with the
+// reach mass and residual (PageRank, FP64 copies).
+void sketch_pr_shortcut(suminc_ctx* c, uint32_t v, uint32_t cap, uint32_t* members, double* reach, double* residual,
+                        uint32_t* count, uint32_t* nexit) {
+  SUMINC_REQUIRE(c->pr_ready, SUMINC_E_STATE, "PageRank state not initialised");
+  sketch_refresh(c);
+  Sketch& s = c->sk;
+  SUMINC_REQUIRE(v < c->n, SUMINC_E_INVALID, "vertex out of range");
+  if (s.pr_rows_version != c->graph_version || s.pr_rows_for != c->pr_prec) {
+    if (c->pr_prec == SUMINC_PR_FP64) compute_pr_rows<double>(c);
+    else compute_pr_rows<float>(c);
+  }
+  uint32_t e = INF32, k = INF32;
+  SUMINC_HIP(hipMemcpy(&e, s.ent_of.as<uint32_t>() + v, 4, hipMemcpyDeviceToHost));
+  SUMINC_REQUIRE(e != INF32, SUMINC_E_INVALID, "vertex is not an entry of a supernode");
+  SUMINC_HIP(hipMemcpy(&k, s.cid.as<uint32_t>() + v, 4, hipMemcpyDeviceToHost));
+  uint32_t co[2], ne = 0;
+  unsigned long long base = 0;
+  SUMINC_HIP(hipMemcpy(co, s.cl_off.as<uint32_t>() + k, 8, hipMemcpyDeviceToHost));
+  SUMINC_HIP(hipMemcpy(&ne, s.cl_nexit.as<uint32_t>() + k, 4, hipMemcpyDeviceToHost));
+  SUMINC_HIP(hipMemcpy(&base, s.sc_base.as<unsigned long long>() + e, 8, hipMemcpyDeviceToHost));
+  const uint32_t size = co[1] - co[0];
+  if (count) *count = size;
+  if (nexit) *nexit = ne;
+  SUMINC_REQUIRE(cap >= size || !(members || reach || residual), SUMINC_E_INVALID, "output capacity too small");
+  if (members) SUMINC_HIP(hipMemcpy(members, s.cl_mem.as<uint32_t>() + co[0], (size_t)size * 4, hipMemcpyDeviceToHost));
+  if (reach || residual) {
+    if (c->pr_prec == SUMINC_PR_FP64) {
+      if (reach) SUMINC_HIP(hipMemcpy(reach, s.sc_R.as<double>() + base, (size_t)size * 8, hipMemcpyDeviceToHost));
+      if (residual) SUMINC_HIP(hipMemcpy(residual, s.sc_D.as<double>() + base, (size_t)size * 8, hipMemcpyDeviceToHost));
+    } else {
+      std::vector<float> tmp(size);
+      if (reach) {
+        SUMINC_HIP(hipMemcpy(tmp.data(), s.sc_R.as<float>() + base, (size_t)size * 4, hipMemcpyDeviceToHost));
+        for (uint32_t j = 0; j < size; ++j) reach[j] = tmp[j];
+      }
+      if (residual) {
+        SUMINC_HIP(hipMemcpy(tmp.data(), s.sc_D.as<float>() + base, (size_t)size * 4, hipMemcpyDeviceToHost));
+        for (uint32_t j = 0; j < size; ++j) residual[j] = tmp[j];
+      }
+    }
+  }
+}
+
+}  // namespace suminc

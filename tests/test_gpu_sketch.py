@@ -1,0 +1,141 @@
+"""Supernode sketch on the device vs the oracle: clustering, acceptance and node
+types bit-exact; shortcut rows vs the dense closed form; the SumInc PageRank
+query (precompute, sketch rounds, correction, residual phase) vs the FP64 fixed
+point, from scratch and after update batches."""
+import numpy as np
+import pytest
+
+import oracle
+from tests.conftest import DAMPING
+from tests.test_oracle_sketch import shortcut_dense
+
+pytestmark = pytest.mark.gpu
+INF = 0xFFFFFFFF
+
+
+def blocks(n, upto, size):
+    return np.where(np.arange(n) < upto, np.arange(n) // size, INF).astype(np.uint32)
+
+
+@pytest.mark.parametrize("scale,max_node,min_node", [(12, 1024, 2), (14, 4, 3), (16, 1024, 2)])
+def test_leaf_clusters_equal_oracle(gpu, scale, max_node, min_node):
+    g = oracle.Graph.rmat(scale, 16, seed=2)
+    off, col, _ = g.csr()
+    with gpu.Context() as ctx:
+        ctx.upload_csr(off, col)
+        raw = ctx.cluster_leaves(max_node, min_node)
+    assert np.array_equal(raw, g.leaf_clusters(max_node, min_node))
+
+
+@pytest.mark.parametrize("kind,thr", [("leaf", 1.0), ("blocks", 1.0), ("blocks", -1e9)])
+def test_structure_equals_oracle(gpu, kind, thr):
+    g = oracle.Graph.rmat(13, 16, seed=1)
+    off, col, _ = g.csr()
+    with gpu.Context() as ctx:
+        ctx.upload_csr(off, col)
+        if kind == "leaf":
+            raw = ctx.cluster_leaves(1024, 2)
+        else:
+            raw = blocks(g.n, 4096, 8)
+            ctx.set_clusters(raw)
+        info = ctx.sum_build(2, 1024, thr)
+        cid, typ = ctx.sum_download()
+    ocid, otyp = g.sum_structure(raw, 2, 1024, thr)
+    assert np.array_equal(cid, ocid)
+    assert np.array_equal(typ, otyp)
+    assert info["node_type_count"] == np.bincount(otyp, minlength=5).tolist()
+    assert info["clusters"] == len(np.unique(ocid[ocid != INF]))
+    assert info["entries"] == int(np.count_nonzero((otyp == 2) | (otyp == 3)))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_shortcut_rows_equal_dense_solve(gpu, prec):
+    g = oracle.Graph.rmat(12, 16, seed=1)
+    off, col, _ = g.csr()
+    p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
+    with gpu.Context(pr_precision=p, termcheck_threshold=1e-30, shortcut_max_rounds=3000) as ctx:
+        ctx.upload_csr(off, col)
+        ctx.set_clusters(blocks(g.n, 1024, 16))
+        ctx.sum_build(2, 1024, -1e9)
+        cid, typ = ctx.sum_download()
+        ctx.pr_init()
+        entries = np.nonzero((typ == 2) | (typ == 3))[0]
+        assert len(entries) > 20
+        for s in entries[:20]:
+            mem, reach, resid, nexit = ctx.sum_pr_shortcut(int(s))
+            omem, x = shortcut_dense(g, cid, int(s), DAMPING)
+            order = np.argsort(mem)
+            assert np.array_equal(mem[order], omem)
+            tol = 1e-12 if prec == "fp64" else 2e-6
+            assert np.abs(reach[order] - x).max() < tol
+            assert np.abs(resid).sum() < tol
+            # exits come first in the row
+            assert np.all(np.isin(typ[mem[:nexit]], [1, 3])) and not np.any(np.isin(typ[mem[nexit:]], [1, 3]))
+
+
+@pytest.mark.parametrize("kind", ["leaf", "blocks"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_sum_pr_query_equals_fixed_point(gpu, kind, prec):
+    g = oracle.Graph.rmat(14, 16, seed=1)
+    off, col, _ = g.csr()
+    rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
+    T = 1e-10 * g.n
+    p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
+    with gpu.Context(pr_precision=p, termcheck_threshold=T) as ctx:
+        ctx.upload_csr(off, col)
+        if kind == "leaf":
+            ctx.cluster_leaves(1024, 2)
+            ctx.sum_build(2, 1024, 1.0)
+        else:
+            ctx.set_clusters(blocks(g.n, 8192, 8))
+            ctx.sum_build(2, 1024, -1e9)
+        ctx.pr_init()
+        st = ctx.sum_pr_run()
+        r, d = ctx.pr_download()
+    err = np.abs(r - rs).sum()
+    assert err <= st["residual"] / (1 - DAMPING) + (1e-11 if prec == "fp64" else 2e-6), (err, st)
+    assert st["reserved"][5] > 0  # sketch rounds ran
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_sum_pr_incremental(gpu, prec):
+    g = oracle.Graph.rmat(13, 16, seed=3)
+    off, col, _ = g.csr()
+    T = 1e-10 * g.n
+    p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
+    with gpu.Context(pr_precision=p, termcheck_threshold=T) as ctx:
+        ctx.upload_csr(off, col)
+        ctx.set_clusters(blocks(g.n, 4096, 8))
+        ctx.sum_build(2, 1024, -1e9)
+        ctx.pr_init()
+        ctx.sum_pr_run()
+        for b in range(3):
+            n_ops = int(0.01 * g.m)
+            ops = g.gen_batch(seed=3, batch_id=b, n_del=n_ops // 2, n_ins=n_ops - n_ops // 2)
+            ctx.apply_batch(ops)
+            g, _ = g.apply_batch(ops)
+            st = ctx.sum_pr_run()
+            r, _ = ctx.pr_download()
+            rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
+            err = np.abs(r - rs).sum()
+            assert err <= st["residual"] / (1 - DAMPING) + (1e-11 if prec == "fp64" else 2e-6), (b, err, st)
+        # the structure follows the new graph, acceptance is kept
+        cid, typ = ctx.sum_download()
+    ocid, otyp = g.sum_structure(blocks(g.n, 4096, 8), 2, 1024, -1e9)
+    assert np.array_equal(cid, ocid) and np.array_equal(typ, otyp)
+
+
+def test_sum_build_errors(gpu):
+    g = oracle.Graph.rmat(10, 8, seed=1)
+    off, col, _ = g.csr()
+    with gpu.Context() as ctx:
+        ctx.upload_csr(off, col)
+        with pytest.raises(gpu.SumincError) as e:
+            ctx.sum_build()
+        assert e.value.code == -4  # no cluster assignment yet
+        bad = np.zeros(g.n, np.uint32)
+        bad[0] = g.n + 5
+        with pytest.raises(gpu.SumincError):
+            ctx.set_clusters(bad)
+        with pytest.raises(gpu.SumincError):
+            ctx.cluster_leaves(5000, 2)  # beyond the 2048-member LDS bound
