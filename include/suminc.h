@@ -209,6 +209,21 @@ int suminc_sum_pr_shortcut(suminc_ctx* ctx, uint32_t entry, uint32_t cap, uint32
  * tjn::g_function_compr + OutMirrorSyncToMaster + deltaSum and the host correction
  * (sum_sync_iter_worker.h:932-933, 1129-1146, 1385-1491). */
 int suminc_sum_pr_run(suminc_ctx* ctx, suminc_stats* stats);
+/* SSSP shortcut of one entry: its cluster's members (exits first), in-cluster
+ * shortest distance from the entry and in-cluster parent per member (the entry
+ * itself: 0 and SUMINC_INF_U32; unreachable: SUMINC_INF_U32 twice).  Replaces the
+ * traversal index of TravCompressor::build_trav_index_mirror /
+ * fianl_build_trav_index_mirror (grape/fragment/trav_compressor.h:244-431). */
+int suminc_sum_sssp_shortcut(suminc_ctx* ctx, uint32_t entry, uint32_t cap, uint32_t* members, uint32_t* dist,
+                             uint32_t* parent, uint32_t* count, uint32_t* nexit);
+/* The SumInc SSSP query from the current state (after suminc_sssp_init or an
+ * update batch): precompute of the clusters holding the source or a frontier
+ * member, sketch rounds (Singles and exits relax their sketch edges, entries
+ * their shortcut row to the exits), correction of every member.  Distances and
+ * parents equal suminc_sssp_run's.  Replaces the compressed branch of
+ * SumSyncTraversalWorker::Query (grape/worker/sum_sync_traversal_worker.h:1160-1233)
+ * and its correction (:1322-1406). */
+int suminc_sum_sssp_run(suminc_ctx* ctx, suminc_stats* stats);
 
 /* ---- stats ---------------------------------------------------------------------*/
 int suminc_get_stats(suminc_ctx* ctx, suminc_stats* stats);

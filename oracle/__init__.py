@@ -55,6 +55,8 @@ def lib():
             "orc_sum_structure": (None, [vp, vp, C.c_uint32, C.c_uint32, C.c_double, vp, vp]),
             "orc_sum_shortcut": (C.c_int, [vp, vp, C.c_uint32, C.c_double, C.c_double, C.c_int, vp, vp]),
             "orc_sum_pr": (None, [vp, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int, vp, vp, vp, vp]),
+            "orc_sum_sssp_shortcut": (None, [vp, vp, C.c_uint32, vp]),
+            "orc_sum_sssp": (None, [vp, vp, vp, C.c_uint32, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -211,3 +213,18 @@ class Graph:
         st = np.zeros(4)
         lib().orc_sum_pr(self.h, _p(cid), _p(typ), d, T, max_rounds, int(init), _p(dv), _p(r), _p(delta), _p(st))
         return r, delta, {"sketch_rounds": st[0], "plain_rounds": st[1], "edges": st[2], "residual": st[3]}
+
+    def sum_sssp_shortcut(self, cid, entry):
+        cid = np.ascontiguousarray(cid, np.uint32)
+        dist = np.zeros(self.n, np.uint32)
+        lib().orc_sum_sssp_shortcut(self.h, _p(cid), entry, _p(dist))
+        return dist
+
+    def sum_sssp(self, cid, typ, src=0):
+        cid = np.ascontiguousarray(cid, np.uint32)
+        typ = np.ascontiguousarray(typ, np.uint8)
+        dist = np.zeros(self.n, np.uint32)
+        par = np.zeros(self.n, np.uint32)
+        st = np.zeros(2)
+        lib().orc_sum_sssp(self.h, _p(cid), _p(typ), src, _p(dist), _p(par), _p(st))
+        return dist, par, {"rounds": st[0], "edges": st[1]}

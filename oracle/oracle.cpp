@@ -18,8 +18,11 @@
 //                           (closed form), mass invariant, error bound.
 //   * orc_pr_amend       -- mass invariant, reseeded fixed point == scratch solve.
 //   * orc_sssp_dijkstra  -- scipy.sparse.csgraph.dijkstra, edge invariants.
-//   * orc_pr_sum         -- equals the closed form (same fixed point), shortcut
+//   * orc_sum_pr         -- equals the closed form (same fixed point), shortcut
 //                           rows equal a dense per-cluster solve.
+//   * orc_sum_sssp*      -- equals orc_sssp_dijkstra / scipy Dijkstra (distances
+//                           and smallest-id parents); shortcut rows equal scipy
+//                           Dijkstra on the cluster's induced subgraph.
 //   * orc_wpr_*          -- dense numpy solve + central finite differences.
 #include <algorithm>
 #include <atomic>
@@ -735,6 +738,127 @@ void orc_sum_pr(void* h, const uint32_t* cid, const uint8_t* type, double d, dou
   double st[3];
   orc_pr_ingress(h, d, T, max_rounds, 0, r, delta, st);
   if (stats) { stats[0] = sk_rounds; stats[1] = st[0]; stats[2] = edges + st[1]; stats[3] = st[2]; }
+}
+
+// In-cluster shortest distances from `s` (the traversal index of
+// TravCompressor::build_trav_index_mirror + run_to_convergence_mirror,
+// trav_compressor.h:244-300: value/delta worklist over the edges that stay inside
+// the cluster).  out: dist[x] for every vertex (INF outside the cluster or
+// unreachable), dist[s] = 0.
+void orc_sum_sssp_shortcut(void* h, const uint32_t* cid, uint32_t s, uint32_t* dist) {
+  Graph* g = (Graph*)h;
+  const uint32_t n = g->n, INF = 0xFFFFFFFFu;
+  const uint32_t k = cid[s];
+  for (uint32_t v = 0; v < n; ++v) dist[v] = INF;
+  dist[s] = 0;
+  std::vector<uint32_t> work{s};
+  std::vector<uint8_t> in_work(n, 0);
+  in_work[s] = 1;
+  while (!work.empty()) {
+    std::vector<uint32_t> next;
+    for (uint32_t u : work) {
+      in_work[u] = 0;
+      for (uint64_t e = g->off[u]; e < g->off[u + 1]; ++e) {
+        const uint32_t v = g->col[e];
+        if (cid[v] != k) continue;
+        const uint64_t nd = (uint64_t)dist[u] + (g->wu.empty() ? 1u : g->wu[e]);
+        if (nd < dist[v]) {
+          dist[v] = (uint32_t)nd;
+          if (!in_work[v]) { in_work[v] = 1; next.push_back(v); }
+        }
+      }
+    }
+    work.swap(next);
+  }
+}
+
+// The SumInc SSSP query restated from scratch, mirror-free
+// (sum_sync_traversal_worker.h:1160-1233 compressed rounds by node type,
+// 1322-1406 correction).  Shortcut rows from orc_sum_sssp_shortcut.  If the
+// source is clustered its cluster is first relaxed in-cluster from it (the
+// reference seeds the source's supernode through GetInitDelta, trav_compressor.h:
+// 244-262).  Rounds are frontier-synchronous: a Single relaxes all out-edges, an
+// exit its edges that leave the cluster, an entry pushes dist + shortcut to every
+// exit of its cluster.  Correction: every member takes min over entries e of
+// dist(e) + shortcut(e, x).  Parents: smallest-id predecessor on a shortest path
+// (as orc_sssp_dijkstra).  stats: [0] rounds, [1] sketch edges relaxed.
+void orc_sum_sssp(void* h, const uint32_t* cid, const uint8_t* type, uint32_t src, uint32_t* dist,
+                  uint32_t* parent, double* stats) {
+  Graph* g = (Graph*)h;
+  const uint32_t n = g->n, INF = 0xFFFFFFFFu;
+  auto w = [&](uint64_t e) -> uint64_t { return g->wu.empty() ? 1u : g->wu[e]; };
+  uint32_t K = 0;
+  for (uint32_t v = 0; v < n; ++v) if (cid[v] != INF) K = std::max(K, cid[v] + 1);
+  std::vector<std::vector<uint32_t>> mem(K);
+  for (uint32_t v = 0; v < n; ++v) if (cid[v] != INF) mem[cid[v]].push_back(v);
+  auto is_exit = [&](uint32_t v) { return type[v] == 1 || type[v] == 3; };
+  auto is_entry = [&](uint32_t v) { return type[v] == 2 || type[v] == 3; };
+  // shortcut rows of every entry (distance to every member of its cluster)
+  std::vector<std::vector<uint32_t>> sc(n);
+  std::vector<uint32_t> tmp(n);
+  for (uint32_t v = 0; v < n; ++v) {
+    if (!is_entry(v)) continue;
+    orc_sum_sssp_shortcut(h, cid, v, tmp.data());
+    for (uint32_t x : mem[cid[v]]) sc[v].push_back(tmp[x]);
+  }
+  std::vector<uint64_t> D(n, INF);
+  D[src] = 0;
+  std::vector<uint8_t> in_next(n, 0);
+  std::vector<uint32_t> front{src};
+  if (cid[src] != INF) {
+    orc_sum_sssp_shortcut(h, cid, src, tmp.data());
+    for (uint32_t x : mem[cid[src]])
+      if (tmp[x] != INF && x != src) { D[x] = tmp[x]; if (is_exit(x)) front.push_back(x); }
+  }
+  double rounds = 0, edges = 0;
+  std::vector<uint32_t> next;
+  auto relax = [&](uint32_t v, uint64_t nd) {
+    if (nd < D[v]) {
+      D[v] = nd;
+      if (!in_next[v]) { in_next[v] = 1; next.push_back(v); }
+    }
+  };
+  while (!front.empty()) {
+    ++rounds;
+    next.clear();
+    for (uint32_t u : front) {
+      const uint64_t du = D[u];
+      const uint8_t t = type[u];
+      if (t == 0 || is_exit(u))
+        for (uint64_t e = g->off[u]; e < g->off[u + 1]; ++e) {
+          const uint32_t v = g->col[e];
+          if (t != 0 && cid[v] == cid[u]) continue;  // exits: external edges only
+          relax(v, du + w(e));
+          edges += 1;
+        }
+      if (is_entry(u)) {
+        const auto& m = mem[cid[u]];
+        for (size_t j = 0; j < m.size(); ++j)
+          if (is_exit(m[j]) && sc[u][j] != INF) { relax(m[j], du + sc[u][j]); edges += 1; }
+      }
+    }
+    for (uint32_t v : next) in_next[v] = 0;
+    front = next;
+  }
+  for (uint32_t k = 0; k < K; ++k)
+    for (uint32_t e : mem[k]) {
+      if (!is_entry(e) || D[e] == INF) continue;
+      for (size_t j = 0; j < mem[k].size(); ++j)
+        if (sc[e][j] != INF) D[mem[k][j]] = std::min(D[mem[k][j]], D[e] + sc[e][j]);
+    }
+  for (uint32_t v = 0; v < n; ++v) {
+    dist[v] = D[v] >= INF ? INF : (uint32_t)D[v];
+    parent[v] = v;
+  }
+  for (uint32_t v = 0; v < n; ++v) if (v != src && dist[v] != INF) parent[v] = INF;
+  for (uint32_t u = 0; u < n; ++u) {
+    if (dist[u] == INF) continue;
+    for (uint64_t e = g->off[u]; e < g->off[u + 1]; ++e) {
+      const uint32_t v = g->col[e];
+      if (v != src && (uint64_t)dist[u] + w(e) == dist[v] && u < parent[v]) parent[v] = u;
+    }
+  }
+  if (stats) { stats[0] = rounds; stats[1] = edges; }
 }
 
 }  // extern "C"

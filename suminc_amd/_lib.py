@@ -148,6 +148,8 @@ SIGNATURES = {
     "suminc_sum_download": (i32, [vp, vp, vp]),
     "suminc_sum_pr_shortcut": (i32, [vp, u32, u32, vp, vp, vp, C.POINTER(u32), C.POINTER(u32)]),
     "suminc_sum_pr_run": (i32, [vp, C.POINTER(Stats)]),
+    "suminc_sum_sssp_shortcut": (i32, [vp, u32, u32, vp, vp, vp, C.POINTER(u32), C.POINTER(u32)]),
+    "suminc_sum_sssp_run": (i32, [vp, C.POINTER(Stats)]),
     "suminc_profile_enable": (i32, [vp, i32]),
     "suminc_profile_get": (i32, [vp, i32, C.POINTER(KernelProfile)]),
 }

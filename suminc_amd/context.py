@@ -198,6 +198,22 @@ class Context:
         self._check(self._lib.suminc_sum_pr_run(self._h, C.byref(s)))
         return s.as_dict()
 
+    def sum_sssp_shortcut(self, entry):
+        cnt, nex = C.c_uint32(), C.c_uint32()
+        self._check(self._lib.suminc_sum_sssp_shortcut(self._h, entry, 0, None, None, None, C.byref(cnt),
+                                                       C.byref(nex)))
+        mem = np.zeros(cnt.value, np.uint32)
+        dist = np.zeros(cnt.value, np.uint32)
+        par = np.zeros(cnt.value, np.uint32)
+        self._check(self._lib.suminc_sum_sssp_shortcut(self._h, entry, cnt.value, L.ptr(mem), L.ptr(dist), L.ptr(par),
+                                                       C.byref(cnt), C.byref(nex)))
+        return mem, dist, par, nex.value
+
+    def sum_sssp_run(self) -> dict:
+        s = L.Stats()
+        self._check(self._lib.suminc_sum_sssp_run(self._h, C.byref(s)))
+        return s.as_dict()
+
     # ---- SSSP -----------------------------------------------------------------
     def sssp_init(self, source=0):
         self._check(self._lib.suminc_sssp_init(self._h, source))

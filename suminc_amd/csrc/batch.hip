@@ -658,6 +658,8 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops) {
                    "op type must be SUMINC_OP_ADD or SUMINC_OP_DEL (line " + std::to_string(i) + ")");
     SUMINC_REQUIRE(ops[i].u < c->n && ops[i].v < c->n, SUMINC_E_INVALID,
                    "vertex id out of range (line " + std::to_string(i) + ")");
+    SUMINC_REQUIRE(c->wkind != SUMINC_W_U32 || ops[i].type == SUMINC_OP_DEL || ops[i].w_u32 >= 1,
+                   SUMINC_E_INVALID, "integer weights must be at least 1 (line " + std::to_string(i) + ")");
   }
   const uint64_t cap_m = c->m + n_ops;
   SUMINC_REQUIRE(cap_m < 0xFFFFFFFFull, SUMINC_E_TOO_LARGE, "edge count could exceed 2^32-1");

@@ -246,6 +246,15 @@ int suminc_sum_pr_run(suminc_ctx* c, suminc_stats* stats) {
   return guarded(c, [&] { sum_pr_run(c, stats); });
 }
 
+int suminc_sum_sssp_shortcut(suminc_ctx* c, uint32_t entry, uint32_t cap, uint32_t* members, uint32_t* dist,
+                             uint32_t* parent, uint32_t* count, uint32_t* nexit) {
+  return guarded(c, [&] { sketch_sssp_shortcut(c, entry, cap, members, dist, parent, count, nexit); });
+}
+
+int suminc_sum_sssp_run(suminc_ctx* c, suminc_stats* stats) {
+  return guarded(c, [&] { sum_sssp_run(c, stats); });
+}
+
 int suminc_get_stats(suminc_ctx* c, suminc_stats* stats) {
   return guarded(c, [&] {
     SUMINC_REQUIRE(stats, SUMINC_E_INVALID, "null output");

@@ -119,6 +119,8 @@ struct Sketch {
   DevBuf ss_dist, ss_par;    // SSSP rows (in-cluster distance / parent) per entry
   DevBuf bound, spnode;      // PageRank sketch accumulators
   DevBuf dirty;              // K: cluster touched since its last precompute
+  DevBuf ss_dirty;           // K: SSSP precompute flags (clusters of frontier members)
+  DevBuf ss_work;            // u64: internal edges read by the SSSP shortcut kernels
   uint32_t counts[8] = {0};  // node type histogram
 };
 

@@ -224,6 +224,11 @@ void upload_csr(suminc_ctx* c, uint32_t n, uint64_t m, const uint64_t* off, cons
   for (uint32_t i = 0; i < n; ++i)
     SUMINC_REQUIRE(off[i] <= off[i + 1], SUMINC_E_INVALID, "offsets must be non-decreasing");
   for (uint64_t e = 0; e < m; ++e) SUMINC_REQUIRE(col[e] < n, SUMINC_E_INVALID, "column id out of range");
+  // integer weights start at 1: a zero-weight cycle would let the smallest-id
+  // parent rule of the SSSP state close a parent cycle away from the source
+  if (wkind == SUMINC_W_U32)
+    for (uint64_t e = 0; e < m; ++e)
+      SUMINC_REQUIRE(((const uint32_t*)w)[e] >= 1, SUMINC_E_INVALID, "integer weights must be at least 1");
   ensure_state_sizes(c);
   c->n = n;
   c->m = m;

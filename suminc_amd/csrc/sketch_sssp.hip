@@ -1,0 +1,470 @@
+// sketch_sssp.hip -- SSSP on the supernode sketch (the reference's
+// SumSyncTraversalWorker compressed phase, mirror-free): shortcuts are in-cluster
+// shortest paths from every entry (TravCompressor::build_trav_index_mirror +
+// run_to_convergence_mirror + fianl_build_trav_index_mirror,
+// trav_compressor.h:244-431); sketch rounds relax a Single's out-edges, an
+// exit's external edges and an entry's shortcut row to its exits
+// (sum_sync_traversal_worker.h:1160-1233, SSSPIngress::Compute /
+// ComputeByIndexDelta, sssp/sssp_ingress.h:131-261); the correction gives every
+// member min over entries of dist(entry) + in-cluster distance
+// (sum_sync_traversal_worker.h:1322-1403).
+//
+// State is the plain path's packed {dist:hi32, parent:lo32} array, so parents are
+// real graph edges and the batch repair (subtree reset + re-pull) applies
+// unchanged.  Exactness of parents: after the distances are final, one pass over
+// every member's internal in-edges min-reduces (dist + w, src) into the member,
+// so every vertex ends with the smallest-id predecessor on a shortest path.
+#include <cstring>
+
+#include "ctx.hpp"
+#include "device.hpp"
+#include "pr.hpp"
+#include "sketch.hpp"
+
+namespace suminc {
+
+constexpr int kMaxClusterS = 2048;
+constexpr uint64_t kInf64 = ~0ull;
+
+// In-cluster Bellman-Ford rounds (pull over internal in-edges) on packed
+// {dist, parent} words in LDS until no word changes.  Returns rounds run.
+__device__ uint32_t cluster_bf(uint32_t size, uint32_t slot0, const uint32_t* __restrict__ ie_off,
+                               const uint32_t* __restrict__ ie_src, const uint32_t* __restrict__ ie_wt,
+                               const uint32_t* __restrict__ mem_global, unsigned long long* s_dp, int* s_flag) {
+  uint32_t rounds = 0;
+  while (true) {
+    if (threadIdx.x == 0) *s_flag = 0;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+      unsigned long long best = s_dp[j];
+      for (uint32_t p = ie_off[slot0 + j]; p < ie_off[slot0 + j + 1]; ++p) {
+        const uint32_t i = ie_src[p];
+        const unsigned long long di = s_dp[i] >> 32;
+        if (di == 0xFFFFFFFFull) continue;
+        const unsigned long long nd = di + ie_wt[p];
+        if (nd >= 0xFFFFFFFFull) continue;
+        const unsigned long long cand = (nd << 32) | mem_global[i];
+        if (cand < best) best = cand;
+      }
+      if (best < s_dp[j]) {
+        atomicMin(&s_dp[j], best);
+        *s_flag = 1;
+      }
+    }
+    __syncthreads();
+    ++rounds;
+    const int f = *s_flag;
+    __syncthreads();
+    if (!f || rounds > size + 1) break;
+  }
+  return rounds;
+}
+
+// Shortcut rows: for every entry e of cluster k, in-cluster distance and parent of
+// every member (the entry itself: distance 0, parent INF = no candidate).
+__global__ void __launch_bounds__(kBlock) k_ss_shortcuts(const uint32_t* __restrict__ cl_off,
+                                                         const uint32_t* __restrict__ cl_mem,
+                                                         const uint32_t* __restrict__ cl_eoff,
+                                                         const uint32_t* __restrict__ cl_ent,
+                                                         const uint32_t* __restrict__ lidx,
+                                                         const unsigned long long* __restrict__ sc_base,
+                                                         const uint32_t* __restrict__ ie_off,
+                                                         const uint32_t* __restrict__ ie_src,
+                                                         const uint32_t* __restrict__ ie_wt, uint32_t* __restrict__ sd,
+                                                         uint32_t* __restrict__ sp, unsigned long long* work) {
+  __shared__ unsigned long long s_dp[kMaxClusterS];
+  __shared__ int s_flag;
+  const uint32_t k = blockIdx.x;
+  const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0;
+  unsigned long long edges = 0;
+  for (uint32_t e = cl_eoff[k]; e < cl_eoff[k + 1]; ++e) {
+    const uint32_t src = lidx[cl_ent[e]];
+    for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_dp[j] = j == src ? 0xFFFFFFFFull : kInf64;
+    __syncthreads();
+    uint32_t rounds = cluster_bf(size, slot0, ie_off, ie_src, ie_wt, cl_mem + slot0, s_dp, &s_flag);
+    edges += (unsigned long long)rounds * (ie_off[slot0 + size] - ie_off[slot0]);
+    const unsigned long long base = sc_base[e];
+    for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+      const unsigned long long x = s_dp[j];
+      sd[base + j] = (uint32_t)(x >> 32);
+      sp[base + j] = (uint32_t)x;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && edges) atomicAdd(work, edges);
+}
+
+struct SsArgs {
+  const uint32_t* ib_off;
+  const uint32_t* ib_col;
+  const uint32_t* ib_w;  // null: unit weights
+  const uint8_t* ntype;
+  const uint32_t* cid;
+  const uint32_t* ent_of;
+  const uint32_t* cl_off;
+  const uint32_t* cl_mem;
+  const uint32_t* cl_nexit;
+  const unsigned long long* sc_base;
+  const uint32_t* sd;
+  const uint32_t* sp;
+  unsigned long long* dp;
+  uint32_t* wl[2];
+  uint32_t* bits[2];
+  uint2* heavy;
+  Counters* cnt;
+  uint8_t* dirty;
+};
+
+__device__ __forceinline__ void ss_append(const SsArgs& a, int list, uint32_t v, bool add_in) {
+  bool add = false;
+  if (add_in) {
+    uint32_t bit = 1u << (v & 31);
+    add = !(atomicOr(&a.bits[list][v >> 5], bit) & bit);
+  }
+  unsigned long long slot = wave_append(add, &a.cnt->wl_count[list]);
+  if (add) a.wl[list][slot] = v;
+}
+
+// Marks the clusters of frontier members (exit or inner) and of the source dirty.
+__global__ void k_ss_mark(SsArgs a, uint32_t src, int mark_src) {
+  const unsigned long long nl = a.cnt->wl_count[0];
+  for (unsigned long long i = (unsigned long long)blockIdx.x * kBlock + threadIdx.x; i < nl;
+       i += (unsigned long long)gridDim.x * kBlock) {
+    const uint32_t v = a.wl[0][i];
+    const uint8_t t = a.ntype[v];
+    if (t == 1 || t == 3 || t == 4) a.dirty[a.cid[v]] = 1;
+  }
+  if (mark_src && blockIdx.x == 0 && threadIdx.x == 0 && a.cid[src] != INF32) a.dirty[a.cid[src]] = 1;
+}
+
+// Precompute of dirty clusters: in-cluster relaxation from the members' current
+// state; improved exits join the frontier (list 0).
+__global__ void __launch_bounds__(kBlock) k_ss_precompute(SsArgs a, const uint32_t* __restrict__ ie_off,
+                                                          const uint32_t* __restrict__ ie_src,
+                                                          const uint32_t* __restrict__ ie_wt,
+                                                          unsigned long long* work) {
+  __shared__ unsigned long long s_dp[kMaxClusterS];
+  __shared__ int s_flag;
+  const uint32_t k = blockIdx.x;
+  if (!a.dirty[k]) return;
+  const uint32_t slot0 = a.cl_off[k], size = a.cl_off[k + 1] - slot0, nexit = a.cl_nexit[k];
+  for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_dp[j] = a.dp[a.cl_mem[slot0 + j]];
+  __syncthreads();
+  uint32_t rounds = cluster_bf(size, slot0, ie_off, ie_src, ie_wt, a.cl_mem + slot0, s_dp, &s_flag);
+  for (uint32_t j0 = 0; j0 < size; j0 += kBlock) {  // uniform trip count for the wave ballots
+    const uint32_t j = j0 + threadIdx.x;
+    bool improved = false;
+    uint32_t v = 0;
+    if (j < size) {
+      v = a.cl_mem[slot0 + j];
+      const unsigned long long x = s_dp[j];
+      if (x < a.dp[v]) {
+        a.dp[v] = x;
+        improved = j < nexit && (x >> 32) != 0xFFFFFFFFull;
+      }
+    }
+    ss_append(a, 0, v, improved);
+  }
+  if (threadIdx.x == 0) {
+    a.dirty[k] = 0;
+    atomicAdd(work, (unsigned long long)rounds * (ie_off[slot0 + size] - ie_off[slot0]));
+  }
+}
+
+__device__ __forceinline__ void ss_relax(const SsArgs& a, int nxt, uint32_t v, unsigned long long nd, uint32_t parent) {
+  bool add = false;
+  if (nd < 0xFFFFFFFFull) {
+    const unsigned long long cand = (nd << 32) | parent;
+    const unsigned long long old = atomicMin(&a.dp[v], cand);
+    add = (old >> 32) > nd;
+  }
+  ss_append(a, nxt, v, add);
+}
+
+// One sketch round over the current frontier: Singles relax their out-edges,
+// exits their external edges (both: the ib row), entries their shortcut row.
+__global__ void __launch_bounds__(kBlock) k_ss_push(SsArgs a) {
+  __shared__ uint32_t s_scan[kBlock + 1];
+  __shared__ uint32_t s_start[kBlock];
+  __shared__ uint32_t s_warp[kBlock / 64 + 1];
+  __shared__ uint32_t s_u[kBlock], s_du[kBlock], s_iblen[kBlock], s_mbase[kBlock];
+  __shared__ unsigned long long s_rbase[kBlock];
+  __shared__ unsigned long long s_edges;
+  if (a.cnt->done) return;
+  const int cur = (int)(a.cnt->aux[1] & 1), nxt = cur ^ 1;
+  const unsigned long long nl = a.cnt->wl_count[cur];
+  const unsigned long long tiles = (nl + kBlock - 1) / kBlock;
+  if (threadIdx.x == 0) s_edges = 0;
+  __syncthreads();
+  for (unsigned long long t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const unsigned long long i = t * kBlock + threadIdx.x;
+    uint32_t len = 0;
+    if (i < nl) {
+      const uint32_t u = a.wl[cur][i];
+      atomicAnd(&a.bits[cur][u >> 5], ~(1u << (u & 31)));
+      const uint32_t du = (uint32_t)(a.dp[u] >> 32);
+      const uint8_t ty = a.ntype[u];
+      uint32_t iblen = 0, sclen = 0;
+      if (du != INF32) {
+        if (ty == 0 || ty == 1 || ty == 3) iblen = a.ib_off[u + 1] - a.ib_off[u];
+        if (ty == 2 || ty == 3) {
+          const uint32_t k = a.cid[u];
+          sclen = a.cl_nexit[k];
+          s_mbase[threadIdx.x] = a.cl_off[k];
+          s_rbase[threadIdx.x] = a.sc_base[a.ent_of[u]];
+        }
+        if (iblen > kHeavyDeg) {  // long rows go to 2048-edge chunks
+          const uint32_t nch = (iblen + kChunk - 1) / kChunk;
+          const unsigned long long slot = atomicAdd(&a.cnt->heavy_count, (unsigned long long)nch);
+          for (uint32_t c = 0; c < nch; ++c) a.heavy[slot + c] = make_uint2(u, c);
+          atomicAdd(&s_edges, (unsigned long long)iblen);
+          iblen = 0;
+        }
+      }
+      s_u[threadIdx.x] = u;
+      s_du[threadIdx.x] = du;
+      s_iblen[threadIdx.x] = iblen;
+      len = iblen + sclen;
+      if (len) atomicAdd(&s_edges, (unsigned long long)len);
+    }
+    block_adjacency<kBlock>(0, len, s_scan, s_start, s_warp, [&](int slot, uint32_t k, uint32_t) {
+      const uint32_t u = s_u[slot];
+      const unsigned long long du = s_du[slot];
+      if (k < s_iblen[slot]) {
+        const uint32_t e = a.ib_off[u] + k;
+        ss_relax(a, nxt, a.ib_col[e], du + (a.ib_w ? a.ib_w[e] : 1u), u);
+      } else {
+        const uint32_t j = k - s_iblen[slot];
+        const unsigned long long r = s_rbase[slot] + j;
+        const uint32_t dj = a.sd[r];
+        ss_relax(a, nxt, a.cl_mem[s_mbase[slot] + j], dj == INF32 ? 0xFFFFFFFFull : du + dj, a.sp[r]);
+      }
+    });
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_edges) atomicAdd(&a.cnt->edges, s_edges);
+}
+
+__global__ void __launch_bounds__(kBlock) k_ss_heavy(SsArgs a) {
+  if (a.cnt->done) return;
+  const int cur = (int)(a.cnt->aux[1] & 1), nxt = cur ^ 1;
+  const unsigned long long nh = a.cnt->heavy_count;
+  for (unsigned long long h = blockIdx.x; h < nh; h += gridDim.x) {
+    const uint2 hv = a.heavy[h];
+    const uint32_t u = hv.x;
+    const unsigned long long du = a.dp[u] >> 32;
+    const uint32_t b = a.ib_off[u] + hv.y * kChunk;
+    const uint32_t e = min(a.ib_off[u + 1], b + kChunk);
+    for (uint32_t k0 = b; k0 < e; k0 += kBlock) {
+      const uint32_t k = k0 + threadIdx.x;
+      if (k < e) ss_relax(a, nxt, a.ib_col[k], du + (a.ib_w ? a.ib_w[k] : 1u), u);
+      else ss_append(a, nxt, 0, false);
+    }
+  }
+}
+
+__global__ void k_ss_end(Counters* cnt, int max_rounds) {
+  if (threadIdx.x != 0 || cnt->done) return;
+  const int cur = (int)(cnt->aux[1] & 1), nxt = cur ^ 1;
+  cnt->aux[6] += cnt->wl_count[cur];
+  cnt->wl_count[cur] = 0;
+  cnt->heavy_count = 0;
+  cnt->rounds += 1;
+  cnt->aux[1] ^= 1ull;
+  if (cnt->wl_count[nxt] == 0 || (long long)cnt->rounds >= (long long)max_rounds) cnt->done = 1;
+}
+
+// Correction: (a) every member takes min over entries of dist(entry) + in-cluster
+// distance (parent = in-cluster predecessor); (b) with final distances, every
+// member min-reduces (dist(src) + w, src) over its internal in-edges, which fixes
+// equal-distance parents to the smallest id.
+__global__ void __launch_bounds__(kBlock) k_ss_correct(SsArgs a, const uint32_t* __restrict__ cl_eoff,
+                                                       const uint32_t* __restrict__ cl_ent,
+                                                       const uint32_t* __restrict__ ie_off,
+                                                       const uint32_t* __restrict__ ie_src,
+                                                       const uint32_t* __restrict__ ie_wt) {
+  __shared__ unsigned long long s_dp[kMaxClusterS];
+  const uint32_t k = blockIdx.x;
+  const uint32_t slot0 = a.cl_off[k], size = a.cl_off[k + 1] - slot0;
+  for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_dp[j] = a.dp[a.cl_mem[slot0 + j]];
+  __syncthreads();
+  for (uint32_t e = cl_eoff[k]; e < cl_eoff[k + 1]; ++e) {
+    const unsigned long long ds = a.dp[cl_ent[e]] >> 32;
+    if (ds == 0xFFFFFFFFull) continue;  // uniform across the block
+    const unsigned long long base = a.sc_base[e];
+    for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+      const uint32_t dj = a.sd[base + j], pj = a.sp[base + j];
+      if (dj == INF32 || pj == INF32) continue;
+      const unsigned long long nd = ds + dj;
+      if (nd >= 0xFFFFFFFFull) continue;
+      const unsigned long long cand = (nd << 32) | pj;
+      if (cand < s_dp[j]) s_dp[j] = cand;  // one writer per slot in this loop
+    }
+  }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+    unsigned long long best = s_dp[j];
+    for (uint32_t p = ie_off[slot0 + j]; p < ie_off[slot0 + j + 1]; ++p) {
+      const uint32_t i = ie_src[p];
+      const unsigned long long di = s_dp[i] >> 32;
+      if (di == 0xFFFFFFFFull) continue;
+      const unsigned long long nd = di + ie_wt[p];
+      if (nd >= 0xFFFFFFFFull) continue;
+      const unsigned long long cand = (nd << 32) | a.cl_mem[slot0 + i];
+      if (cand < best) best = cand;
+    }
+    const uint32_t v = a.cl_mem[slot0 + j];
+    if (best < a.dp[v]) a.dp[v] = best;
+  }
+}
+
+static SsArgs ss_args(suminc_ctx* c) {
+  Sketch& s = c->sk;
+  SsArgs a;
+  a.ib_off = s.ib_off.as<uint32_t>();
+  a.ib_col = s.ib_col.as<uint32_t>();
+  a.ib_w = c->wkind == SUMINC_W_U32 ? s.ib_w.as<uint32_t>() : nullptr;
+  a.ntype = s.ntype.as<uint8_t>();
+  a.cid = s.cid.as<uint32_t>();
+  a.ent_of = s.ent_of.as<uint32_t>();
+  a.cl_off = s.cl_off.as<uint32_t>();
+  a.cl_mem = s.cl_mem.as<uint32_t>();
+  a.cl_nexit = s.cl_nexit.as<uint32_t>();
+  a.sc_base = s.sc_base.as<unsigned long long>();
+  a.sd = s.ss_dist.as<uint32_t>();
+  a.sp = s.ss_par.as<uint32_t>();
+  a.dp = c->sssp_dp.as<unsigned long long>();
+  a.wl[0] = c->wl0.as<uint32_t>();
+  a.wl[1] = c->tmp5.as<uint32_t>();
+  a.bits[0] = c->sssp_flag0.as<uint32_t>();
+  a.bits[1] = c->sssp_flag1.as<uint32_t>();
+  a.heavy = c->heavy.as<uint2>();
+  a.cnt = dev_counters(c);
+  a.dirty = s.ss_dirty.as<uint8_t>();
+  return a;
+}
+
+static void compute_sssp_rows(suminc_ctx* c) {
+  Sketch& s = c->sk;
+  s.ss_dist.ensure(s.sc_len * 4 + 4);
+  s.ss_par.ensure(s.sc_len * 4 + 4);
+  s.ss_work.ensure(8);
+  SUMINC_HIP(hipMemsetAsync(s.ss_work.p, 0, 8, c->stream));
+  const uint32_t* wt = s.ie_w.as<uint32_t>() + s.ie_m + 1;  // internal edge weights follow the degrees
+  if (s.K)
+    k_ss_shortcuts<<<s.K, kBlock, 0, c->stream>>>(s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(),
+                                                  s.cl_eoff.as<uint32_t>(), s.cl_ent.as<uint32_t>(),
+                                                  s.lidx.as<uint32_t>(), s.sc_base.as<unsigned long long>(),
+                                                  s.ie_off.as<uint32_t>(), s.ie_src.as<uint32_t>(), wt,
+                                                  s.ss_dist.as<uint32_t>(), s.ss_par.as<uint32_t>(),
+                                                  s.ss_work.as<unsigned long long>());
+  SUMINC_HIP(hipGetLastError());
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  s.sssp_rows_version = c->graph_version;
+}
+
+__global__ void k_ss_begin(Counters* cnt) {
+  if (threadIdx.x) return;
+  cnt->wl_count[1] = 0;
+  cnt->heavy_count = 0;
+  cnt->aux[1] = 0;
+  cnt->aux[6] = 0;
+  cnt->done = 0;
+  cnt->rounds = 0;
+  cnt->edges = 0;
+}
+
+void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
+  SUMINC_REQUIRE(c->sssp_ready, SUMINC_E_STATE, "SSSP state not initialised");
+  sketch_refresh(c);
+  Sketch& s = c->sk;
+  if (s.sssp_rows_version != c->graph_version) compute_sssp_rows(c);
+  s.ss_dirty.ensure((size_t)s.K + 1);
+  const int max_rounds = c->prm.max_rounds > 0 ? c->prm.max_rounds : 1000000;
+  const bool first = c->sssp_pending;
+  c->sssp_pending = false;
+  SsArgs a = ss_args(c);
+  Counters* cnt = a.cnt;
+  const uint32_t* wt = s.ie_w.as<uint32_t>() + s.ie_m + 1;
+  s.ss_work.ensure(8);
+  SUMINC_HIP(hipMemsetAsync(s.ss_work.p, 0, 8, c->stream));
+  SUMINC_HIP(hipMemsetAsync(s.ss_dirty.p, 0, (size_t)s.K + 1, c->stream));
+  SUMINC_HIP(hipEventRecord(c->ev_start, c->stream));
+  // frontier list 0 holds the seeds (source after init, repair seeds after a batch)
+  k_ss_begin<<<1, 64, 0, c->stream>>>(cnt);
+  k_ss_mark<<<256, kBlock, 0, c->stream>>>(a, c->sssp_src, first ? 1 : 0);
+  if (s.K)
+    k_ss_precompute<<<s.K, kBlock, 0, c->stream>>>(a, s.ie_off.as<uint32_t>(), s.ie_src.as<uint32_t>(), wt,
+                                                   s.ss_work.as<unsigned long long>());
+  SUMINC_HIP(hipGetLastError());
+  int group = 8;
+  while (true) {
+    for (int i = 0; i < group; ++i) {
+      k_ss_push<<<push_grid(), kBlock, 0, c->stream>>>(a);
+      k_ss_heavy<<<push_grid(), kBlock, 0, c->stream>>>(a);
+      k_ss_end<<<1, 64, 0, c->stream>>>(cnt, max_rounds);
+    }
+    SUMINC_HIP(hipGetLastError());
+    read_counters(c);
+    if (c->host_counters->done) break;
+    if (group < 64) group *= 2;
+  }
+  const Counters* h = c->host_counters;
+  const int cur = (int)(h->aux[1] & 1);
+  SUMINC_REQUIRE(h->wl_count[cur] == 0 || h->rounds < (unsigned long long)max_rounds, SUMINC_E_NOT_CONVERGED,
+                 "round cap reached with a non-empty frontier");
+  const double rounds = (double)h->rounds, edges = (double)h->edges;
+  if (s.K)
+    k_ss_correct<<<s.K, kBlock, 0, c->stream>>>(a, s.cl_eoff.as<uint32_t>(), s.cl_ent.as<uint32_t>(),
+                                                s.ie_off.as<uint32_t>(), s.ie_src.as<uint32_t>(), wt);
+  SUMINC_HIP(hipGetLastError());
+  SUMINC_HIP(hipEventRecord(c->ev_stop, c->stream));
+  SUMINC_HIP(hipEventSynchronize(c->ev_stop));
+  float ms = 0;
+  SUMINC_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+  unsigned long long wk = 0;
+  SUMINC_HIP(hipMemcpy(&wk, s.ss_work.p, 8, hipMemcpyDeviceToHost));
+  if (cur != 0) {  // keep list/bitmap 0 as the current (empty) frontier
+    c->wl0.swap(c->tmp5);
+    c->sssp_flag0.swap(c->sssp_flag1);
+  }
+  suminc_stats out{};
+  out.rounds = rounds;
+  out.edges_traversed = edges + (double)wk;
+  out.kernel_ms = ms;
+  out.push_rounds = rounds;
+  out.reserved[8] = (double)wk;  // internal edges read by the precompute
+  c->stats = out;
+  if (st) *st = out;
+}
+
+}  // namespace suminc
+
+namespace suminc {
+
+void sketch_sssp_shortcut(suminc_ctx* c, uint32_t v, uint32_t cap, uint32_t* members, uint32_t* dist,
+                          uint32_t* parent, uint32_t* count, uint32_t* nexit) {
+  SUMINC_REQUIRE(c->has_graph, SUMINC_E_STATE, "no graph");
+  sketch_refresh(c);
+  Sketch& s = c->sk;
+  SUMINC_REQUIRE(v < c->n, SUMINC_E_INVALID, "vertex out of range");
+  if (s.sssp_rows_version != c->graph_version) compute_sssp_rows(c);
+  uint32_t e = INF32, k = INF32;
+  SUMINC_HIP(hipMemcpy(&e, s.ent_of.as<uint32_t>() + v, 4, hipMemcpyDeviceToHost));
+  SUMINC_REQUIRE(e != INF32, SUMINC_E_INVALID, "vertex is not an entry of a supernode");
+  SUMINC_HIP(hipMemcpy(&k, s.cid.as<uint32_t>() + v, 4, hipMemcpyDeviceToHost));
+  uint32_t co[2], ne = 0;
+  unsigned long long base = 0;
+  SUMINC_HIP(hipMemcpy(co, s.cl_off.as<uint32_t>() + k, 8, hipMemcpyDeviceToHost));
+  SUMINC_HIP(hipMemcpy(&ne, s.cl_nexit.as<uint32_t>() + k, 4, hipMemcpyDeviceToHost));
+  SUMINC_HIP(hipMemcpy(&base, s.sc_base.as<unsigned long long>() + e, 8, hipMemcpyDeviceToHost));
+  const uint32_t size = co[1] - co[0];
+  if (count) *count = size;
+  if (nexit) *nexit = ne;
+  SUMINC_REQUIRE(cap >= size || !(members || dist || parent), SUMINC_E_INVALID, "output capacity too small");
+  if (members) SUMINC_HIP(hipMemcpy(members, s.cl_mem.as<uint32_t>() + co[0], (size_t)size * 4, hipMemcpyDeviceToHost));
+  if (dist) SUMINC_HIP(hipMemcpy(dist, s.ss_dist.as<uint32_t>() + base, (size_t)size * 4, hipMemcpyDeviceToHost));
+  if (parent) SUMINC_HIP(hipMemcpy(parent, s.ss_par.as<uint32_t>() + base, (size_t)size * 4, hipMemcpyDeviceToHost));
+}
+
+}  // namespace suminc

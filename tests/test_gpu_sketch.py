@@ -139,3 +139,120 @@ def test_sum_build_errors(gpu):
             ctx.set_clusters(bad)
         with pytest.raises(gpu.SumincError):
             ctx.cluster_leaves(5000, 2)  # beyond the 2048-member LDS bound
+
+
+def _sssp_graph(weights, seed=2, scale=13):
+    return oracle.Graph.rmat(scale, 16, seed=seed, weights=weights, wmin=1, wmax=64)
+
+
+def _upload(ctx, g):
+    off, col, w = g.csr()
+    ctx.upload_csr(off, col, w)
+
+
+def _cluster(ctx, g, kind):
+    if kind == "leaf":
+        ctx.cluster_leaves(1024, 2)
+        return ctx.sum_build(2, 1024, 1.0)
+    ctx.set_clusters(blocks(g.n, 4096, 8))
+    return ctx.sum_build(2, 1024, -1e9)
+
+
+@pytest.mark.parametrize("weights", [None, "u32"])
+def test_sssp_shortcut_rows_equal_oracle(gpu, weights):
+    g = _sssp_graph(weights, scale=12)
+    off, col, w = g.csr()
+    wv = np.ones(len(col), np.uint64) if w is None else w.astype(np.uint64)
+    with gpu.Context() as ctx:
+        _upload(ctx, g)
+        _cluster(ctx, g, "blocks")
+        cid, typ = ctx.sum_download()
+        entries = np.nonzero((typ == 2) | (typ == 3))[0]
+        assert len(entries) > 20
+        for s in entries[:40]:
+            mem, dist, par, nexit = ctx.sum_sssp_shortcut(int(s))
+            exp = g.sum_sssp_shortcut(cid, int(s))
+            assert np.array_equal(dist, exp[mem]), int(s)
+            # parent: smallest in-cluster predecessor on a shortest in-cluster path
+            for j, x in enumerate(mem):
+                if x == s or exp[x] == INF:
+                    assert par[j] == INF
+                    continue
+                cands = [u for u in mem for e in range(off[u], off[u + 1])
+                         if col[e] == x and exp[u] != INF and int(exp[u]) + int(wv[e]) == int(exp[x])]
+                assert par[j] == min(cands), (int(s), int(x))
+            assert np.all(np.isin(typ[mem[:nexit]], [1, 3]))
+
+
+@pytest.mark.parametrize("weights", [None, "u32"])
+@pytest.mark.parametrize("kind", ["leaf", "blocks"])
+def test_sum_sssp_equals_dijkstra(gpu, weights, kind):
+    g = _sssp_graph(weights)
+    with gpu.Context() as ctx:
+        _upload(ctx, g)
+        _cluster(ctx, g, kind)
+        cid, typ = ctx.sum_download()
+        sources = [int(np.nonzero(typ == 0)[0][0])]
+        for t in (1, 2, 3, 4):
+            if np.any(typ == t):
+                sources.append(int(np.nonzero(typ == t)[0][0]))
+        for s in sources:
+            ctx.sssp_init(s)
+            st = ctx.sum_sssp_run()
+            dist, par = ctx.sssp_download()
+            od, op = g.dijkstra(s)
+            assert np.array_equal(dist, od), (s, int(typ[s]))
+            assert np.array_equal(par, op), (s, int(typ[s]))
+            assert st["rounds"] >= 1
+
+
+@pytest.mark.parametrize("weights", [None, "u32"])
+@pytest.mark.parametrize("kind", ["leaf", "blocks"])
+def test_sum_sssp_incremental(gpu, weights, kind):
+    g = _sssp_graph(weights, seed=5)
+    src = 0
+    with gpu.Context() as ctx:
+        _upload(ctx, g)
+        _cluster(ctx, g, kind)
+        ctx.sssp_init(src)
+        ctx.sum_sssp_run()
+        for b in range(4):
+            n_ops = int(0.01 * g.m)
+            ops = g.gen_batch(seed=5, batch_id=b, n_del=n_ops // 2, n_ins=n_ops - n_ops // 2)
+            ctx.apply_batch(ops)
+            g, _ = g.apply_batch(ops)
+            ctx.sum_sssp_run()
+            dist, par = ctx.sssp_download()
+            od, op = g.dijkstra(src)
+            assert np.array_equal(dist, od), b
+            assert np.array_equal(par, op), b
+
+
+def test_sum_sssp_mixed_with_plain_runs(gpu):
+    """Plain and sketch runs share one state: alternating them across batches keeps
+    the exact result."""
+    g = _sssp_graph("u32", seed=7)
+    with gpu.Context() as ctx:
+        _upload(ctx, g)
+        _cluster(ctx, g, "blocks")
+        ctx.sssp_init(3)
+        ctx.sssp_run()
+        for b in range(4):
+            ops = g.gen_batch(seed=7, batch_id=b, n_del=300, n_ins=300)
+            ctx.apply_batch(ops)
+            g, _ = g.apply_batch(ops)
+            (ctx.sum_sssp_run if b % 2 == 0 else ctx.sssp_run)()
+            dist, par = ctx.sssp_download()
+            od, op = g.dijkstra(3)
+            assert np.array_equal(dist, od) and np.array_equal(par, op), b
+
+
+def test_zero_integer_weight_rejected(gpu):
+    with gpu.Context() as ctx:
+        with pytest.raises(gpu.SumincError):
+            ctx.upload_csr(np.array([0, 1, 1], np.uint64), np.array([1], np.uint32), np.array([0], np.uint32))
+        ctx.upload_csr(np.array([0, 1, 1], np.uint64), np.array([1], np.uint32), np.array([3], np.uint32))
+        ops = np.zeros(1, gpu.UPDATE_DTYPE)
+        ops[0] = (gpu.OP_ADD, 1, 0, 0, 0.0, 0)
+        with pytest.raises(gpu.SumincError):
+            ctx.apply_batch(ops)

@@ -121,3 +121,54 @@ def test_sum_pr_equals_closed_form(cluster_kind):
     rs = closed_form(g, DAMPING)
     assert st["sketch_rounds"] > 0
     assert np.abs(r - rs).sum() <= st["residual"] / (1 - DAMPING) + 1e-12
+
+
+def _weighted(g):
+    off, col, w = g.csr()
+    return off, col, (np.ones(len(col)) if w is None else w.astype(np.float64))
+
+
+def test_sum_sssp_shortcut_equals_induced_dijkstra():
+    from scipy.sparse.csgraph import dijkstra
+    g = oracle.Graph.rmat(12, 16, seed=1, weights="u32", wmin=1, wmax=64)
+    raw = np.where(np.arange(g.n) < 2048, np.arange(g.n) // 16, INF).astype(np.uint32)
+    cid, typ = g.sum_structure(raw, 2, 1024, -1e9)
+    off, col, w = _weighted(g)
+    entries = np.nonzero((typ == 2) | (typ == 3))[0]
+    assert len(entries) > 20
+    for s in entries[:20]:
+        mem = np.nonzero(cid == cid[s])[0]
+        pos = {int(v): i for i, v in enumerate(mem)}
+        A = np.zeros((len(mem), len(mem)))
+        for v in mem:
+            for e in range(off[v], off[v + 1]):
+                if cid[col[e]] == cid[s]:
+                    A[pos[int(v)], pos[int(col[e])]] = w[e]
+        ref = dijkstra(sp.csr_matrix(A), indices=pos[int(s)])
+        got = g.sum_sssp_shortcut(cid, int(s))
+        exp = np.full(g.n, INF, np.uint32)
+        fin = np.isfinite(ref)
+        exp[mem[fin]] = ref[fin].astype(np.uint32)
+        assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("weights", [None, "u32"])
+@pytest.mark.parametrize("cluster_kind", ["leaf", "blocks"])
+def test_sum_sssp_equals_dijkstra(weights, cluster_kind):
+    g = oracle.Graph.rmat(12, 16, seed=2, weights=weights, wmin=1, wmax=64)
+    if cluster_kind == "leaf":
+        cid, typ = g.sum_structure(g.leaf_clusters(1024, 2), 2, 1024, 1.0)
+    else:
+        raw = np.where(np.arange(g.n) < 2048, np.arange(g.n) // 8, INF).astype(np.uint32)
+        cid, typ = g.sum_structure(raw, 2, 1024, -1e9)
+        assert set(np.unique(typ).tolist()) == {0, 1, 2, 3, 4}
+    # a Single source, and a clustered source of every member type present
+    sources = [int(np.nonzero(typ == 0)[0][0])]
+    for t in (1, 2, 3, 4):
+        if np.any(typ == t):
+            sources.append(int(np.nonzero(typ == t)[0][0]))
+    for s in sources:
+        dist, par = g.dijkstra(s)
+        d2, p2, st = g.sum_sssp(cid, typ, s)
+        assert np.array_equal(d2, dist), (s, typ[s])
+        assert np.array_equal(p2, par), (s, typ[s])
