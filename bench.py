@@ -4,14 +4,16 @@ MI355X (GTEPS, fraction of the HBM roofline, speedup vs full recompute on the
 same GPU, and the CPU oracle timed on the host as a baseline).
 
 A step = apply one seeded update batch (device CSR/CSC delta-merge + reseed or
-SSSP repair) and run the incremental iteration to its fixed point.  Inputs are
-generated on the device before the timed region; only the batch ops (a few MB)
-cross PCIe inside it.
+SSSP repair) and run the incremental query to its fixed point.  The graph is
+generated on the device and the SSSP batches are uploaded to device memory before
+the timed region, so the timed steps read their inputs from HBM (the rate with the
+ops passed as host buffers is reported beside it as host_ops_ms_per_step).
 
 Default workload (--workload sssp22): directed R-MAT scale 22 (a=.57 b=.19
 c=.19, edge factor 16, seed 2, self-loops/duplicates removed), u32 weights
-uniform in [1,64], source 0, 0.1% batches (half deletes of existing edges, half
-inserts with uniform endpoints and weights in [1,64]).
+uniform in [1,64], source 0, supernodes from leaf clustering (at most 1024
+members, compress threshold 1), 0.1% batches (half deletes of existing edges,
+half inserts with uniform endpoints and weights in [1,64]).
 """
 from __future__ import annotations
 
@@ -44,8 +46,9 @@ def host_cores() -> int:
 
 
 WORKLOADS = {
-    "sssp22": dict(kind="sssp", scale=22, seed=2, wmin=1, wmax=64, frac=0.001, source=0,
-                   desc="incremental SSSP, R-MAT s22 ef16 seed2, u32 w~U[1,64], source 0, 0.1% batches"),
+    "sssp22": dict(kind="sssp", scale=22, seed=2, wmin=1, wmax=64, frac=0.001, source=0, max_node=1024,
+                   desc="incremental SSSP with supernodes, R-MAT s22 ef16 seed2, u32 w~U[1,64], source 0, "
+                        "clusters <= 1024 members, 0.1% batches"),
     "pr24": dict(kind="pr", scale=24, seed=3, frac=0.01, T=1e-7,
                  desc="incremental FP32/Kahan delta-PageRank, R-MAT s24 ef16 seed3, d=0.85f, 1% batches"),
     "pr22": dict(kind="pr", scale=22, seed=3, frac=0.01, T=1e-7,
@@ -66,7 +69,7 @@ def roofline(ctx, kernel_ids):
     k, p = best
     achieved = p["alg_bytes"] / (p["ms"] * 1e-3) / 1e9
     name = {L.KERNEL_PR_PUSH: "k_pr_push", L.KERNEL_PR_GATHER: "k_pr_gather",
-            L.KERNEL_SSSP_PUSH: "k_sssp_push+k_sssp_heavy"}[k]
+            L.KERNEL_SSSP_PUSH: "k_sssp_push+k_sssp_heavy", L.KERNEL_SSSP_SKETCH: "k_ss_push+k_ss_heavy"}[k]
     return name, {
         "bound": "hbm",
         "achieved": round(achieved, 2),
@@ -85,38 +88,70 @@ def run_sssp(args, wl):
     import suminc_amd
     import suminc_amd._lib as L
 
+    sketch = args.supernodes == "on"
     ctx = suminc_amd.Context(max_rounds=1_000_000)
     t0 = time.time()
     ctx.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"], weights="u32", wmin=wl["wmin"],
                       wmax=wl["wmax"])
     n, m, _ = ctx.graph_info()
     log(f"graph: n={n} m={m} ({time.time() - t0:.1f}s)")
+    extra = {}
+    if sketch:
+        t0 = time.time()
+        ctx.cluster_leaves(wl["max_node"], 2)
+        info = ctx.sum_build(2, wl["max_node"], 1.0)
+        log(f"sketch: {info} ({time.time() - t0:.1f}s)")
+        extra["sketch"] = {k: info[k] for k in ("clusters", "clustered_vertices", "entries", "sketch_edges",
+                                                "internal_edges", "shortcut_slots", "max_cluster")}
+        extra["sketch"]["node_type_count"] = info["node_type_count"]
+    run = ctx.sum_sssp_run if sketch else ctx.sssp_run
     ctx.sssp_init(wl["source"])
-    st = ctx.sssp_run()
+    st = run()
     log(f"initial SSSP: rounds={st['rounds']:.0f} edges={st['edges_traversed']:.3e} {st['kernel_ms']:.1f} ms")
     n_ops = int(wl["frac"] * m)
     batches = [ctx.generate_batch(wl["seed"], b, n_ops // 2, n_ops - n_ops // 2, wl["wmin"], wl["wmax"])
                for b in range(args.warmup + args.steps)]
+    dev = [ctx.upload_batch(b) for b in batches]  # inputs resident in HBM before the timed region
     for b in range(args.warmup):
-        ctx.apply_batch(batches[b])
-        ctx.sssp_run()
+        ctx.apply_batch_device(dev[b])
+        run()
     ctx.sync()
     ctx.profile_enable(True)
     edges = 0.0
+    run_ms = 0.0
     t0 = time.perf_counter()
     for b in range(args.warmup, args.warmup + args.steps):
-        ctx.apply_batch(batches[b])
-        st = ctx.sssp_run()
+        ctx.apply_batch_device(dev[b])
+        st = run()
         edges += st["edges_traversed"]
+        run_ms += st["kernel_ms"]
     ctx.sync()
     dt = time.perf_counter() - t0
-    kernel, roof = roofline(ctx, [L.KERNEL_SSSP_PUSH])
+    kernel, roof = roofline(ctx, [L.KERNEL_SSSP_SKETCH if sketch else L.KERNEL_SSSP_PUSH])
     ctx.profile_enable(False)
+    for d in dev:
+        d.free()
     ms_per_step = 1e3 * dt / args.steps
     gteps = edges / dt / 1e9
-    log(f"incremental: {ms_per_step:.3f} ms/step, {edges / args.steps:.3e} edges/step, {gteps:.3f} GTEPS")
+    log(f"incremental: {ms_per_step:.3f} ms/step (query {run_ms / args.steps:.3f} ms), "
+        f"{edges / args.steps:.3e} edges/step, {gteps:.4f} GTEPS")
 
-    # full recompute on the same GPU, same final graph
+    # host-buffer variant of the same step (PCIe upload of the ops inside the step)
+    ctx.apply_batch(batches[0])  # re-applying an old batch is a valid batch on the current graph
+    run()
+    ctx.sync()
+    t1 = time.perf_counter()
+    for b in range(args.warmup, args.warmup + min(args.steps, 5)):
+        ctx.apply_batch(batches[b])
+        run()
+    ctx.sync()
+    host_ms = 1e3 * (time.perf_counter() - t1) / min(args.steps, 5)
+
+    # the incremental state is what gets checked
+    off, col, w = ctx.download_csr()
+    dist, par = ctx.sssp_download()
+
+    # full recompute on the same GPU, same final graph, plain path
     full = []
     for _ in range(3):
         ctx.sync()
@@ -128,13 +163,14 @@ def run_sssp(args, wl):
     full_ms = 1e3 * float(np.median(full))
     log(f"full recompute: {full_ms:.3f} ms ({stf['edges_traversed']:.3e} edges)")
 
-    extra = {"full_recompute_ms": round(full_ms, 3), "speedup_vs_full_recompute": round(full_ms / ms_per_step, 2),
-             "edges_per_step": edges / args.steps}
+    extra.update({"full_recompute_ms": round(full_ms, 3), "speedup_vs_full_recompute": round(full_ms / ms_per_step, 2),
+                  "query_ms_per_step": round(run_ms / args.steps, 4), "edges_per_step": edges / args.steps,
+                  "host_ops_ms_per_step": round(host_ms, 4)})
     cpu = None
     if not args.no_verify or not args.no_cpu:
-        off, col, w = ctx.download_csr()
-        dist, par = ctx.sssp_download()
         g = oracle.Graph.from_csr(off, col, w)
+        cores = min(host_cores(), 16)
+        oracle.set_threads(cores)
         if not args.no_verify:
             t1 = time.time()
             od, op = g.dijkstra(wl["source"])
@@ -143,14 +179,25 @@ def run_sssp(args, wl):
             log(f"verify vs Dijkstra ({time.time() - t1:.1f}s): dist {extra['bit_exact_dist']} "
                 f"parent {extra['bit_exact_parent']}")
         if not args.no_cpu:
-            cores = min(host_cores(), 16)
-            oracle.set_threads(cores)
-            t1 = time.perf_counter()
-            _, _, sto = g.sssp_ingress(wl["source"])
-            tc = time.perf_counter() - t1
-            cpu = {"value": round(sto["edges"] / tc / 1e9, 4), "unit": "GTEPS", "cores": cores, "kind": "port",
-                   "sample": f"oracle SSSP Ingress (reference CPU loop restated) from scratch on the final "
-                             f"s{wl['scale']} graph: {sto['edges']:.3e} edges in {tc:.2f}s"}
+            # one more seeded batch of the same shape, applied on the host (untimed),
+            # then the reference's incremental step restated on the CPU, repeated
+            ops = g.gen_batch(wl["seed"], 10_000, n_ops // 2, n_ops - n_ops // 2, wl["wmin"], wl["wmax"])
+            t1 = time.time()
+            g2, _ = g.apply_batch(ops)
+            log(f"cpu baseline: host graph update {time.time() - t1:.1f}s (untimed)")
+            reps, tc, ec = 0, 0.0, 0.0
+            while tc < args.cpu_seconds and reps < 1000:
+                t1 = time.perf_counter()
+                _, _, sto = g2.sssp_inc(g, wl["source"], dist, par)
+                tc += time.perf_counter() - t1
+                ec += sto["edges"]
+                reps += 1
+            cpu = {"value": round(ec / tc / 1e9, 4), "unit": "GTEPS", "cores": cores, "kind": "port",
+                   "ms_per_step": round(1e3 * tc / reps, 3),
+                   "sample": f"oracle restatement of the reference's incremental SSSP step (deltaCompute reset + "
+                             f"resend + Ingress rounds) on the s{wl['scale']} graph after one more "
+                             f"{100 * wl['frac']:g}% batch, repeated {reps}x: {ec:.3e} edges in {tc:.1f}s "
+                             f"(host graph update excluded)"}
             log(f"cpu baseline: {cpu}")
     ctx.close()
     return gteps, ms_per_step, "u32", roof, cpu, extra
@@ -235,6 +282,9 @@ def main():
     ap.add_argument("--workload", default="sssp22", choices=sorted(WORKLOADS))
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--supernodes", default="on", choices=["on", "off"],
+                    help="SSSP workload: query over the supernode sketch (on) or the plain CSR (off)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="length of the CPU baseline sample")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the multi-GPU edge-cut path is designed, not built "
@@ -259,7 +309,8 @@ def main():
         "dtype": dtype,
         "data": "synthetic (seeded R-MAT + seeded update batches, generated on the device)",
         "config": {"workload": wl["desc"], "scale": wl["scale"], "edge_factor": 16, "batch_fraction": wl["frac"],
-                   "supernodes": "off (plain CSR path)"},
+                   "supernodes": (("on (leaf clusters <= %d members, compress threshold 1)" % wl["max_node"])
+                                  if wl["kind"] == "sssp" and args.supernodes == "on" else "off (plain CSR path)")},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -14,8 +14,9 @@
  *   - Every function returns int: SUMINC_OK (0) or a negative SUMINC_E_* code.
  *     suminc_last_error() returns a message for the last failure on a context.
  *     No entry point aborts the process.
- *   - Host pointers are borrowed for the duration of the call only.  All device
- *     memory is owned by the context and released by suminc_ctx_destroy().
+ *   - Host pointers are borrowed for the duration of the call only.  Device memory
+ *     is owned by the context and released by suminc_ctx_destroy(), except buffers
+ *     from suminc_device_alloc(), which the caller frees with suminc_device_free().
  *   - Vertex ids are dense uint32 in [0, n).  Offsets at the boundary are
  *     uint64 (CSR row pointers, n+1 entries).
  *   - SSSP distances are uint32 with SUMINC_INF_U32 (0xFFFFFFFF) for "unreached"
@@ -139,6 +140,17 @@ int suminc_generate_batch(suminc_ctx* ctx, uint64_t seed, uint32_t batch_id, uin
  * is reset with its dependency subtree (sum_sync_traversal_worker.h:146-261), reset
  * vertices re-pull from surviving in-neighbours and every insert is relaxed. */
 int suminc_apply_batch(suminc_ctx* ctx, const suminc_update* ops, uint64_t n_ops);
+/* The same with the ops already in device memory of the context's GPU (for
+ * pipelines that keep update batches resident in HBM).  The ops are checked on the
+ * device before use; no counterpart in the reference, whose batches are parsed
+ * from host files (inc_fragment_builder.h:29-133). */
+int suminc_apply_batch_device(suminc_ctx* ctx, const suminc_update* dev_ops, uint64_t n_ops);
+
+/* ---- device buffers owned by the caller (hipMalloc / hipFree / copy on the
+ * context's device and stream); plumbing for device-resident inputs. */
+int suminc_device_alloc(suminc_ctx* ctx, uint64_t bytes, void** dev_ptr);
+int suminc_device_free(suminc_ctx* ctx, void* dev_ptr);
+int suminc_copy_to_device(suminc_ctx* ctx, void* dev_dst, const void* host_src, uint64_t bytes);
 
 /* ---- PageRank (plain Ingress path) --------------------------------------------
  * init: delta = (1-d)/N, rank = 0 (PageRankIngress::init_c/init_v,
@@ -237,6 +249,7 @@ int suminc_get_stats(suminc_ctx* ctx, suminc_stats* stats);
 #define SUMINC_KERNEL_PR_GATHER 1 /* PageRank vertex pass: 4 B/vertex + 16 B/active vertex */
 #define SUMINC_KERNEL_SSSP_PUSH 2 /* SSSP relax: 16 B/edge + 16 B/vertex */
 #define SUMINC_KERNEL_PR_PULL 3   /* PageRank CSC gather: 8 B/edge + 16 B/vertex (FP32) */
+#define SUMINC_KERNEL_SSSP_SKETCH 4 /* SSSP sketch rounds: 16 B/sketch edge + 20 B/shortcut slot + 16 B/vertex */
 #define SUMINC_KERNEL_COUNT 8
 typedef struct suminc_kernel_profile {
   double launches;  /* timed launches */

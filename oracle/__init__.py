@@ -51,6 +51,7 @@ def lib():
             "orc_pr_amend": (None, [vp, C.c_double, C.c_int, vp, vp, vp]),
             "orc_sssp_dijkstra": (None, [vp, C.c_uint32, vp, vp]),
             "orc_sssp_ingress": (C.c_int, [vp, C.c_uint32, vp, vp, vp]),
+            "orc_sssp_inc": (None, [vp, vp, C.c_uint32, vp, vp, vp]),
             "orc_leaf_clusters": (None, [vp, C.c_uint32, C.c_uint32, vp]),
             "orc_sum_structure": (None, [vp, vp, C.c_uint32, C.c_uint32, C.c_double, vp, vp]),
             "orc_sum_shortcut": (C.c_int, [vp, vp, C.c_uint32, C.c_double, C.c_double, C.c_int, vp, vp]),
@@ -182,6 +183,14 @@ class Graph:
         st = np.zeros(2)
         lib().orc_sssp_ingress(self.h, src, _p(dist), _p(par), _p(st))
         return dist, par, {"rounds": st[0], "edges": st[1]}
+
+    def sssp_inc(self, old, src, dist, parent):
+        """Incremental SSSP step on this (new) graph from the state on `old`."""
+        dist = np.ascontiguousarray(dist, np.uint32).copy()
+        par = np.ascontiguousarray(parent, np.uint32).copy()
+        st = np.zeros(3)
+        lib().orc_sssp_inc(old.h, self.h, src, _p(dist), _p(par), _p(st))
+        return dist, par, {"rounds": st[0], "edges": st[1], "reset": st[2]}
 
     # ---- supernode compression ------------------------------------------------
     def leaf_clusters(self, max_node=1024, min_node=2):

@@ -490,6 +490,124 @@ int orc_sssp_ingress(void* h, uint32_t src, uint32_t* dist, uint32_t* parent, do
   return rounds;
 }
 
+// The reference's incremental traversal step restated (deltaCompute,
+// sum_sync_traversal_worker.h:113-352, then the Ingress rounds of Query):
+//  1. roots: every v whose parent edge (parent(v), v) is gone from the new graph
+//     (:145-173 walks the deleted pairs; a pair whose weight changed is a root too
+//     here, since keeping its old distance would be stale);
+//  2. reset: frontier walk over the OLD out-edges, children x with parent(x) == u
+//     join; reset vertices get value/delta = identity (:180-260);
+//  3. resend: every vertex with a finite value pushes value + w along its NEW
+//     out-edges once, unconditionally (:316-329);
+//  4. Ingress rounds from the vertices that improved, as orc_sssp_ingress.
+// dist/parent: in = state on the old graph, out = state on the new graph.
+// stats: [0] rounds, [1] edges traversed (resend + rounds), [2] reset vertices.
+void orc_sssp_inc(void* h_old, void* h_new, uint32_t src, uint32_t* dist, uint32_t* parent, double* stats) {
+  Graph* go = (Graph*)h_old;
+  Graph* g = (Graph*)h_new;
+  const uint32_t n = g->n, INF = 0xFFFFFFFFu;
+  auto find = [](const Graph* gr, uint32_t u, uint32_t v) -> int64_t {
+    const uint32_t* b = gr->col.data() + gr->off[u];
+    const uint32_t* e = gr->col.data() + gr->off[u + 1];
+    const uint32_t* it = std::lower_bound(b, e, v);
+    return (it != e && *it == v) ? (int64_t)(gr->off[u] + (it - b)) : -1;
+  };
+  std::vector<uint8_t> cur(n, 0), nxt(n, 0), reset(n, 0);
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const uint32_t v = (uint32_t)i, p = parent[v];
+    if (v == src || dist[v] == INF || p == v) continue;
+    const int64_t eo = find(go, p, v), en = find(g, p, v);
+    const uint32_t wo = eo < 0 ? 1u : (go->wu.empty() ? 1u : go->wu[eo]);
+    const uint32_t wn = en < 0 ? 0u : (g->wu.empty() ? 1u : g->wu[en]);
+    if (en < 0 || wn != wo) cur[v] = 1;
+  }
+  double n_reset = 0;
+  bool any = true;
+  while (any) {
+    any = false;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : n_reset)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+      const uint32_t u = (uint32_t)i;
+      if (!cur[u]) continue;
+      for (uint64_t e = go->off[u]; e < go->off[u + 1]; ++e) {
+        const uint32_t x = go->col[e];
+        if (x != u && !reset[x] && !cur[x] && parent[x] == u && dist[x] != INF) nxt[x] = 1;
+      }
+      n_reset += 1;
+    }
+    for (uint32_t u = 0; u < n; ++u)
+      if (cur[u]) { reset[u] = 1; cur[u] = 0; }
+    for (uint32_t u = 0; u < n; ++u)
+      if (nxt[u] && !reset[u]) { cur[u] = 1; any = true; }
+    std::fill(nxt.begin(), nxt.end(), 0);
+  }
+  std::vector<uint64_t> delta(n);
+  std::vector<uint32_t> value(n);
+  for (uint32_t v = 0; v < n; ++v) {
+    if (reset[v]) { value[v] = INF; delta[v] = ((uint64_t)INF << 32) | v; }
+    else { value[v] = dist[v]; delta[v] = ((uint64_t)dist[v] << 32) | parent[v]; }
+  }
+  auto wt = [&](uint64_t e) -> uint64_t { return g->wu.empty() ? 1u : g->wu[e]; };
+  double edges = 0;
+  // resend
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : edges)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const uint32_t u = (uint32_t)i;
+    if (value[u] == INF) continue;
+    for (uint64_t e = g->off[u]; e < g->off[u + 1]; ++e) {
+      const uint32_t v = g->col[e];
+      const uint64_t nd = (uint64_t)value[u] + wt(e);
+      if (nd >= INF) continue;
+      const uint64_t cand = (nd << 32) | u;
+      std::atomic_ref<uint64_t> a(delta[v]);
+      uint64_t old = a.load(std::memory_order_relaxed);
+      while (cand < old && !a.compare_exchange_weak(old, cand, std::memory_order_relaxed)) {}
+      if (cand < old && (old >> 32) > nd) cur[v] = 1;
+    }
+    edges += (double)g->deg(u);
+  }
+  // Ingress rounds
+  int rounds = 0;
+  any = true;
+  while (any) {
+    any = false;
+    for (uint32_t v = 0; v < n; ++v) if (cur[v]) { any = true; break; }
+    if (!any) break;
+    ++rounds;
+    double e_round = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : e_round)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+      const uint32_t u = (uint32_t)i;
+      if (!cur[u]) continue;
+      cur[u] = 0;
+      const uint64_t du = std::atomic_ref<uint64_t>(delta[u]).load(std::memory_order_relaxed);
+      const uint32_t dv = (uint32_t)(du >> 32);
+      if (dv >= value[u]) continue;
+      value[u] = dv;
+      for (uint64_t e = g->off[u]; e < g->off[u + 1]; ++e) {
+        const uint32_t v = g->col[e];
+        const uint64_t nd = (uint64_t)dv + wt(e);
+        if (nd >= INF) continue;
+        const uint64_t cand = (nd << 32) | u;
+        std::atomic_ref<uint64_t> a(delta[v]);
+        uint64_t old = a.load(std::memory_order_relaxed);
+        while (cand < old && !a.compare_exchange_weak(old, cand, std::memory_order_relaxed)) {}
+        if (cand < old && (old >> 32) > nd) nxt[v] = 1;
+      }
+      e_round += (double)g->deg(u);
+    }
+    edges += e_round;
+    std::swap(cur, nxt);
+  }
+  for (uint32_t v = 0; v < n; ++v) {
+    const uint32_t dv = (uint32_t)(delta[v] >> 32);
+    dist[v] = std::min(value[v], dv);
+    parent[v] = dist[v] == INF ? v : (uint32_t)delta[v];
+  }
+  if (stats) { stats[0] = rounds; stats[1] = edges; stats[2] = n_reset; }
+}
+
 // ---------------------------------------------------------------------------
 // Supernode compression (mirror-free form, mirror_k = 1e8)
 // ---------------------------------------------------------------------------

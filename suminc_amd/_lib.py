@@ -132,6 +132,10 @@ SIGNATURES = {
     "suminc_download_csr": (i32, [vp, vp, vp, vp]),
     "suminc_generate_batch": (i32, [vp, u64, u32, u64, u64, u32, u32, vp]),
     "suminc_apply_batch": (i32, [vp, vp, u64]),
+    "suminc_apply_batch_device": (i32, [vp, vp, u64]),
+    "suminc_device_alloc": (i32, [vp, u64, C.POINTER(vp)]),
+    "suminc_device_free": (i32, [vp, vp]),
+    "suminc_copy_to_device": (i32, [vp, vp, vp, u64]),
     "suminc_pr_init": (i32, [vp]),
     "suminc_pr_run": (i32, [vp, C.POINTER(Stats)]),
     "suminc_pr_step": (i32, [vp, C.POINTER(f64)]),
@@ -158,6 +162,7 @@ KERNEL_PR_PUSH = 0
 KERNEL_PR_GATHER = 1
 KERNEL_SSSP_PUSH = 2
 KERNEL_PR_PULL = 3
+KERNEL_SSSP_SKETCH = 4
 
 
 def lib():

@@ -128,6 +128,18 @@ class Context:
         self._check(self._lib.suminc_apply_batch(self._h, L.ptr(ops), len(ops)))
         return self.stats()
 
+    def upload_batch(self, ops) -> "DeviceBatch":
+        """Copies a batch into device memory (for apply_batch_device)."""
+        ops = np.ascontiguousarray(ops, dtype=L.UPDATE_DTYPE)
+        p = C.c_void_p()
+        self._check(self._lib.suminc_device_alloc(self._h, ops.nbytes, C.byref(p)))
+        self._check(self._lib.suminc_copy_to_device(self._h, p, L.ptr(ops), ops.nbytes))
+        return DeviceBatch(self, p, len(ops))
+
+    def apply_batch_device(self, batch: "DeviceBatch"):
+        self._check(self._lib.suminc_apply_batch_device(self._h, batch.ptr, batch.n_ops))
+        return self.stats()
+
     # ---- PageRank -------------------------------------------------------------
     def pr_init(self):
         self._check(self._lib.suminc_pr_init(self._h))
@@ -229,3 +241,17 @@ class Context:
         parent = np.zeros(n, np.uint32)
         self._check(self._lib.suminc_sssp_download(self._h, L.ptr(dist), L.ptr(parent)))
         return dist, parent
+
+
+class DeviceBatch:
+    """An update batch resident in device memory, freed with its context's device."""
+
+    def __init__(self, ctx: Context, ptr, n_ops: int):
+        self._ctx = ctx
+        self.ptr = ptr
+        self.n_ops = n_ops
+
+    def free(self):
+        if self.ptr is not None and self._ctx._h:
+            self._ctx._check(self._ctx._lib.suminc_device_free(self._ctx._h, self.ptr))
+        self.ptr = None

@@ -209,37 +209,76 @@ __global__ void k_new_deg(const uint32_t* off, uint32_t n, Groups gs, uint32_t* 
   ndeg[v] = d;
 }
 
-// Copy surviving old edges to their new positions.  Edge-parallel (one thread
-// per old edge, row found by binary search over the row pointers) so that hub
-// rows do not serialise on one workgroup.
-__global__ void __launch_bounds__(kBlock) k_copy_old(uint32_t n, uint32_t m, const uint32_t* off,
-                                                     const uint32_t* colv, const uint32_t* wv, const uint32_t* noff,
-                                                     uint32_t* ncol, uint32_t* nw, Groups gs) {
-  const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
-  if (e >= m) return;
+// Copy surviving old edges to their new positions.  The old edge array is cut
+// into tiles of kCopyTile edges, one workgroup each; the rows a tile spans are
+// found once (k_tile_rows) and their offsets staged in LDS, so an edge finds its
+// row by a short LDS search.  Edges of untouched rows move by their row's shift;
+// in touched rows the row's groups give the number of deleted pairs and adds
+// before the edge.
+constexpr uint32_t kCopyTile = 4096;
+constexpr uint32_t kCopyRows = 4096;  // LDS-staged row offsets per tile
+
+// row of the first edge of every tile, and of the last edge (entry ntiles)
+__global__ void k_tile_rows(const uint32_t* off, uint32_t n, uint32_t m, uint32_t ntiles, uint32_t* trow) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntiles) return;
+  const uint32_t e = t < ntiles ? t * kCopyTile : m - 1;
   uint32_t lo = 0, hi = n - 1;  // last row with off[row] <= e
   while (lo < hi) {
-    uint32_t mid = lo + ((hi - lo + 1) >> 1);
+    const uint32_t mid = lo + ((hi - lo + 1) >> 1);
     if (off[mid] <= e) lo = mid; else hi = mid - 1;
   }
-  const uint32_t u = lo, k = e - off[u];
-  const uint32_t x = colv[e];
-  const uint32_t ti = gs.tix[u];
-  uint32_t dst;
-  if (ti == INF32) {
-    dst = noff[u] + k;
-  } else {
-    uint32_t b = gs.t_gbeg[ti], en = gs.t_gbeg[ti + 1];
-    uint32_t l = b, h = en;
-    while (l < h) {
-      uint32_t mid = (l + h) >> 1;
-      if (gs.col[mid] < x) l = mid + 1; else h = mid;
+  trow[t] = lo;
+}
+
+__global__ void __launch_bounds__(kBlock) k_copy_old(uint32_t m, const uint32_t* __restrict__ off,
+                                                     const uint32_t* __restrict__ colv, const uint32_t* __restrict__ wv,
+                                                     const uint32_t* __restrict__ noff, uint32_t* __restrict__ ncol,
+                                                     uint32_t* __restrict__ nw, const uint32_t* __restrict__ trow,
+                                                     uint32_t ntiles, Groups gs) {
+  __shared__ uint32_t s_off[kCopyRows + 1];
+  const uint32_t t = blockIdx.x;
+  const uint32_t e0 = t * kCopyTile, e1 = (uint32_t)min((uint64_t)m, (uint64_t)e0 + kCopyTile);
+  const uint32_t r0 = trow[t];
+  const uint32_t r1 = t + 1 < ntiles ? trow[t + 1] : trow[ntiles];  // last row a tile edge can be in
+  const uint32_t span = r1 - r0 + 1;
+  const bool staged = span <= kCopyRows;
+  if (staged)
+    for (uint32_t j = threadIdx.x; j <= span; j += kBlock) s_off[j] = off[r0 + j];
+  __syncthreads();
+  for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
+    uint32_t lo = 0, hi = span - 1;  // last row (relative to r0) starting at or before e
+    if (staged) {
+      while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo + 1) >> 1);
+        if (s_off[mid] <= e) lo = mid; else hi = mid - 1;
+      }
+    } else {
+      while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo + 1) >> 1);
+        if (off[r0 + mid] <= e) lo = mid; else hi = mid - 1;
+      }
     }
-    if (l < en && gs.col[l] == x) return;  // deleted or replaced by an add
-    dst = noff[u] + k - (gs.ex_pre[l] - gs.ex_pre[b]) + (gs.add_pre[l] - gs.add_pre[b]);
+    const uint32_t u = r0 + lo;
+    const uint32_t k = e - (staged ? s_off[lo] : off[u]);
+    const uint32_t x = colv[e];
+    const uint32_t ti = gs.tix[u];
+    uint32_t dst;
+    if (ti == INF32) {
+      dst = noff[u] + k;
+    } else {
+      const uint32_t b = gs.t_gbeg[ti], en = gs.t_gbeg[ti + 1];
+      uint32_t l = b, h = en;
+      while (l < h) {
+        const uint32_t mid = (l + h) >> 1;
+        if (gs.col[mid] < x) l = mid + 1; else h = mid;
+      }
+      if (l < en && gs.col[l] == x) continue;  // deleted or replaced by an add
+      dst = noff[u] + k - (gs.ex_pre[l] - gs.ex_pre[b]) + (gs.add_pre[l] - gs.add_pre[b]);
+    }
+    ncol[dst] = x;
+    if (nw) nw[dst] = wv[e];
   }
-  ncol[dst] = x;
-  if (nw) nw[dst] = wv[e];
 }
 
 __global__ void k_place_adds(const uint32_t* off, const uint32_t* colv, const uint32_t* noff, uint32_t* ncol,
@@ -569,11 +608,17 @@ static void merge_direction(suminc_ctx* c, DevBuf& off, DevBuf& col, DevBuf& w, 
     return rocprim::exclusive_scan(p, bytes, b.ndeg.as<uint32_t>(), noff.as<uint32_t>(), 0u, (size_t)n + 1,
                                    rocprim::plus<uint32_t>(), c->stream);
   });
-  if (c->m)
-    k_copy_old<<<grid_for(c->m), kBlock, 0, c->stream>>>(n, (uint32_t)c->m, off.as<uint32_t>(), col.as<uint32_t>(),
-                                                         c->wkind ? w.as<uint32_t>() : nullptr, noff.as<uint32_t>(),
-                                                         ncol.as<uint32_t>(), c->wkind ? nw.as<uint32_t>() : nullptr,
-                                                         gs);
+  if (c->m) {
+    const uint32_t m = (uint32_t)c->m;
+    const uint32_t ntiles = (m + kCopyTile - 1) / kCopyTile;
+    b.trow.ensure(((size_t)ntiles + 1) * 4);
+    k_tile_rows<<<grid_for((uint64_t)ntiles + 1), kBlock, 0, c->stream>>>(off.as<uint32_t>(), n, m, ntiles,
+                                                                         b.trow.as<uint32_t>());
+    k_copy_old<<<ntiles, kBlock, 0, c->stream>>>(m, off.as<uint32_t>(), col.as<uint32_t>(),
+                                                 c->wkind ? w.as<uint32_t>() : nullptr, noff.as<uint32_t>(),
+                                                 ncol.as<uint32_t>(), c->wkind ? nw.as<uint32_t>() : nullptr,
+                                                 b.trow.as<uint32_t>(), ntiles, gs);
+  }
   k_place_adds<<<grid_for(N), kBlock, 0, c->stream>>>(off.as<uint32_t>(), col.as<uint32_t>(), noff.as<uint32_t>(),
                                                       ncol.as<uint32_t>(), c->wkind ? nw.as<uint32_t>() : nullptr,
                                                       gs);
@@ -639,6 +684,7 @@ static void ensure_workspace(suminc_ctx* c, uint64_t n_ops) {
                     &b.tg_addu, &b.tg_expre, &b.tg_addpre, &b.tt_raw, &b.tt_list, &b.tt_gbeg})
     d->ensure((N + 1) * 4);
   b.bc.ensure(64);
+  b.val.ensure(8);
   b.ndeg.ensure(((size_t)c->n + 1) * 4);
   if (b.tix_n != c->n || !b.tix.p) {
     b.tix.ensure((size_t)c->n * 4);
@@ -649,17 +695,33 @@ static void ensure_workspace(suminc_ctx* c, uint64_t n_ops) {
   }
 }
 
-void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops) {
+// Device-side check of a device-resident batch: counts invalid ops and keeps the
+// smallest invalid index (out[0] = count, out[1] = first index).
+__global__ void k_validate_ops(const suminc_update* ops, uint32_t N, uint32_t n, int wkind, unsigned* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const suminc_update o = ops[i];
+  bool bad = (o.type != SUMINC_OP_ADD && o.type != SUMINC_OP_DEL) || o.u >= n || o.v >= n ||
+             (wkind == SUMINC_W_U32 && o.type == SUMINC_OP_ADD && o.w_u32 < 1);
+  if (bad) {
+    atomicAdd(&out[0], 1u);
+    atomicMin(&out[1], i);
+  }
+}
+
+void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool ops_on_device) {
   SUMINC_REQUIRE(c->has_graph, SUMINC_E_STATE, "no graph");
   SUMINC_REQUIRE(ops || n_ops == 0, SUMINC_E_INVALID, "null ops");
   SUMINC_REQUIRE(n_ops < 0x7FFFFFFFull, SUMINC_E_TOO_LARGE, "batch too large");
-  for (uint64_t i = 0; i < n_ops; ++i) {
-    SUMINC_REQUIRE(ops[i].type == SUMINC_OP_ADD || ops[i].type == SUMINC_OP_DEL, SUMINC_E_INVALID,
-                   "op type must be SUMINC_OP_ADD or SUMINC_OP_DEL (line " + std::to_string(i) + ")");
-    SUMINC_REQUIRE(ops[i].u < c->n && ops[i].v < c->n, SUMINC_E_INVALID,
-                   "vertex id out of range (line " + std::to_string(i) + ")");
-    SUMINC_REQUIRE(c->wkind != SUMINC_W_U32 || ops[i].type == SUMINC_OP_DEL || ops[i].w_u32 >= 1,
-                   SUMINC_E_INVALID, "integer weights must be at least 1 (line " + std::to_string(i) + ")");
+  if (!ops_on_device) {
+    for (uint64_t i = 0; i < n_ops; ++i) {
+      SUMINC_REQUIRE(ops[i].type == SUMINC_OP_ADD || ops[i].type == SUMINC_OP_DEL, SUMINC_E_INVALID,
+                     "op type must be SUMINC_OP_ADD or SUMINC_OP_DEL (line " + std::to_string(i) + ")");
+      SUMINC_REQUIRE(ops[i].u < c->n && ops[i].v < c->n, SUMINC_E_INVALID,
+                     "vertex id out of range (line " + std::to_string(i) + ")");
+      SUMINC_REQUIRE(c->wkind != SUMINC_W_U32 || ops[i].type == SUMINC_OP_DEL || ops[i].w_u32 >= 1,
+                     SUMINC_E_INVALID, "integer weights must be at least 1 (line " + std::to_string(i) + ")");
+    }
   }
   const uint64_t cap_m = c->m + n_ops;
   SUMINC_REQUIRE(cap_m < 0xFFFFFFFFull, SUMINC_E_TOO_LARGE, "edge count could exceed 2^32-1");
@@ -669,9 +731,21 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops) {
   const uint32_t N = (uint32_t)n_ops;
   Counters* cnt = dev_counters(c);
   ensure_workspace(c, n_ops);
+  BatchWs& b = c->bw;
+  if (ops_on_device) {
+    // validate before any kernel indexes by the ops' vertex ids
+    unsigned init[2] = {0u, 0xFFFFFFFFu}, res[2];
+    SUMINC_HIP(hipMemcpyAsync(b.val.p, init, 8, hipMemcpyHostToDevice, c->stream));
+    k_validate_ops<<<grid_for(N), kBlock, 0, c->stream>>>(ops, N, n, c->wkind, b.val.as<unsigned>());
+    SUMINC_HIP(hipGetLastError());
+    SUMINC_HIP(hipMemcpyAsync(res, b.val.p, 8, hipMemcpyDeviceToHost, c->stream));
+    SUMINC_HIP(hipStreamSynchronize(c->stream));
+    SUMINC_REQUIRE(res[0] == 0, SUMINC_E_INVALID,
+                   std::to_string(res[0]) + " invalid ops (first at line " + std::to_string(res[1]) +
+                       "): type must be add/delete, ids below n, integer weights at least 1");
+  }
   // heavy-chunk list sized for the largest graph this batch can produce
   if (c->pr_ready || c->sssp_ready) c->heavy.ensure((cap_m / kChunk + cap_m / kHeavyDeg + 2) * sizeof(uint4));
-  BatchWs& b = c->bw;
   unsigned* bc = b.bc.as<unsigned>();
   k_batch_zero<<<1, 64, 0, c->stream>>>(cnt);
   // An SSSP state that was initialised but never run is still the initial state:
@@ -680,7 +754,8 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops) {
   const bool sssp_reinit = c->sssp_ready && c->sssp_pending;
 
   // 1. upload + stable sort by (u,v) + one group per pair
-  SUMINC_HIP(hipMemcpyAsync(b.ops.p, ops, n_ops * sizeof(suminc_update), hipMemcpyHostToDevice, c->stream));
+  SUMINC_HIP(hipMemcpyAsync(b.ops.p, ops, n_ops * sizeof(suminc_update),
+                            ops_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
   k_op_keys<<<grid_for(N), kBlock, 0, c->stream>>>(b.ops.as<suminc_update>(), N, b.keys_a.as<uint64_t>(),
                                                    b.idx_a.as<uint32_t>());
   cub_call(c, c->tmp, [&](void* p, size_t& bytes) {
@@ -707,8 +782,9 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops) {
              b.g_expre, b.g_addpre, b.head, b.t_raw, b.t_list, b.t_gbeg, b.tix, 1);
   k_new_m<<<1, 64, 0, c->stream>>>(bc, b.g_expre.as<uint32_t>(), b.g_addpre.as<uint32_t>(), (uint32_t)c->m, bc);
   SUMINC_HIP(hipGetLastError());
-  // supernode sketch: clusters of both endpoints need a new precompute
-  sketch_mark_dirty(c, b.g_row.as<uint32_t>(), b.g_col.as<uint32_t>(), bc, N);
+  // supernode sketch: dirty clusters, entry/exit edge counts, internal edge list
+  sketch_batch(c, b.g_row.as<uint32_t>(), b.g_col.as<uint32_t>(), b.g_act.as<uint8_t>(), b.g_ex.as<uint8_t>(),
+               b.g_w.as<uint32_t>(), bc, N);
 
   // 3. PageRank reseed -1 on the old out-edges of touched sources
   if (c->pr_ready) {
@@ -806,6 +882,7 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops) {
   unsigned hbc[4] = {0, 0, 0, 0};
   SUMINC_HIP(hipMemcpyAsync(hbc, bc, sizeof(hbc), hipMemcpyDeviceToHost, c->stream));
   read_counters(c);  // synchronises the stream
+  sketch_batch_done(c);
   c->m = hbc[3];
   c->graph_version++;
   // per-round scratch sized by m may need to grow

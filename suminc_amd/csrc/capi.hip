@@ -177,7 +177,39 @@ int suminc_generate_batch(suminc_ctx* c, uint64_t seed, uint32_t batch_id, uint6
 }
 
 int suminc_apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops) {
-  return guarded(c, [&] { apply_batch(c, ops, n_ops); });
+  return guarded(c, [&] { apply_batch(c, ops, n_ops, false); });
+}
+
+int suminc_apply_batch_device(suminc_ctx* c, const suminc_update* dev_ops, uint64_t n_ops) {
+  return guarded(c, [&] { apply_batch(c, dev_ops, n_ops, true); });
+}
+
+int suminc_device_alloc(suminc_ctx* c, uint64_t bytes, void** dev_ptr) {
+  return guarded(c, [&] {
+    SUMINC_REQUIRE(dev_ptr, SUMINC_E_INVALID, "null output");
+    *dev_ptr = nullptr;
+    if (bytes) {
+      hipError_t e = hipMalloc(dev_ptr, bytes);
+      SUMINC_REQUIRE(e == hipSuccess, SUMINC_E_NOMEM, "device allocation failed");
+    }
+  });
+}
+
+int suminc_device_free(suminc_ctx* c, void* dev_ptr) {
+  return guarded(c, [&] {
+    SUMINC_HIP(hipStreamSynchronize(c->stream));
+    if (dev_ptr) SUMINC_HIP(hipFree(dev_ptr));
+  });
+}
+
+int suminc_copy_to_device(suminc_ctx* c, void* dev_dst, const void* host_src, uint64_t bytes) {
+  return guarded(c, [&] {
+    SUMINC_REQUIRE((dev_dst && host_src) || bytes == 0, SUMINC_E_INVALID, "null buffer");
+    if (bytes) {
+      SUMINC_HIP(hipMemcpyAsync(dev_dst, host_src, bytes, hipMemcpyHostToDevice, c->stream));
+      SUMINC_HIP(hipStreamSynchronize(c->stream));
+    }
+  });
 }
 
 int suminc_pr_init(suminc_ctx* c) {

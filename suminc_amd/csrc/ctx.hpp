@@ -89,13 +89,16 @@ struct BatchWs {
   DevBuf tix, ttix;  // row -> touched index, all INF32 between batches
   DevBuf ndeg, noff_out, ncol_out, nw_out, noff_in, ncol_in, nw_in;
   DevBuf rbits, rlist;
+  DevBuf trow;  // row of the first edge of every copy tile
+  DevBuf val;  // validation of device-resident ops: count, first bad index
   DevBuf bc;  // device counts: [0]=G groups, [1]=touched rows (CSR), [2]=touched rows (CSC), [3]=new m
   uint32_t tix_n = 0;
 };
 
 // Supernode sketch of the graph (mirror-free form of the reference's compressor):
 // accepted clusters, node types, cluster member layout, entry (supernode) list,
-// internal in-edges, sketch out-edges and the dense shortcut rows.
+// internal edges and the dense shortcut rows.  A Single's sketch row is its CSR
+// row; an exit's sketch row is its CSR row without the edges inside its cluster.
 struct Sketch {
   bool built = false;
   uint32_t K = 0;            // accepted clusters
@@ -103,24 +106,32 @@ struct Sketch {
   uint32_t M = 0;            // clustered vertices
   uint32_t Kraw = 0;         // raw clusters in the input assignment
   uint64_t sc_len = 0;       // total length of the dense shortcut rows
-  uint64_t ib_m = 0;         // sketch out-edges
   uint64_t ie_m = 0;         // internal edges
+  uint64_t iem_n = 0;        // live entries of the internal edge list
   uint32_t max_cluster = 0;  // largest accepted cluster
   uint64_t version = ~0ull;  // graph version the structure was built for
+  uint32_t RM = 0;           // vertices with a raw cluster id
   int pr_rows_for = -1;      // precision the PR shortcut rows were computed in (-1: none)
-  uint64_t pr_rows_version = ~0ull;
-  uint64_t sssp_rows_version = ~0ull;
+  bool ss_rows = false;      // SSSP shortcut rows exist for the current layout
+  bool pr_need = true;       // some PageRank rows are invalid
+  bool ss_need = true;       // some SSSP rows are invalid
   DevBuf raw_cid, raw_acc;   // input assignment (n), acceptance per raw cluster (Kraw)
+  DevBuf rlist;              // vertices with a raw cluster id (RM)
   DevBuf cid, ntype, lidx, is_entry, is_exit;
+  DevBuf ext_out, ext_in;    // per clustered vertex: edges leaving / entering its cluster
   DevBuf cl_off, cl_mem, cl_nexit, cl_eoff, cl_ent, ent_of, sc_base;
-  DevBuf ie_off, ie_src, ie_w;
-  DevBuf ib_off, ib_col, ib_w;
+  // internal edges: the list sorted by (src, dst), kept up to date by every batch,
+  // and its grouping by destination member slot for the in-cluster kernels
+  DevBuf iem_key, iem_w, iem_dead, iem_key2, iem_w2;
+  DevBuf ie_off, ie_src, ie_dst, ie_deg, ie_wt;
   DevBuf sc_R, sc_D;         // PageRank rows (reach mass / residual) per entry
   DevBuf ss_dist, ss_par;    // SSSP rows (in-cluster distance / parent) per entry
   DevBuf bound, spnode;      // PageRank sketch accumulators
   DevBuf dirty;              // K: cluster touched since its last precompute
   DevBuf ss_dirty;           // K: SSSP precompute flags (clusters of frontier members)
   DevBuf ss_work;            // u64: internal edges read by the SSSP shortcut kernels
+  DevBuf pr_valid, ss_valid; // K: shortcut rows of the cluster are current
+  DevBuf sc_base_old, ent_of_old, sc_R_old, sc_D_old, ss_dist_old, ss_par_old;  // previous layout
   uint32_t counts[8] = {0};  // node type histogram
 };
 

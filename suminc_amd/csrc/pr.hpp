@@ -28,6 +28,6 @@ void sssp_download(suminc_ctx* c, uint32_t* dist, uint32_t* parent);
 // batch.hip
 void generate_batch(suminc_ctx* c, uint64_t seed, uint32_t batch_id, uint64_t n_del, uint64_t n_ins, uint32_t wmin,
                     uint32_t wmax, suminc_update* ops);
-void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops);
+void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool ops_on_device);
 
 }  // namespace suminc

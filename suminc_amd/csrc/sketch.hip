@@ -183,6 +183,7 @@ void cluster_leaves(suminc_ctx* c, uint32_t max_node, uint32_t min_node, uint32_
   s.Kraw = K;
   s.built = false;
   s.raw_acc.release();
+  s.rlist.release();
   if (out) SUMINC_HIP(hipMemcpy(out, s.raw_cid.p, (size_t)n * 4, hipMemcpyDeviceToHost));
 }
 
@@ -202,34 +203,64 @@ void set_clusters(suminc_ctx* c, const uint32_t* cluster_of) {
   s.Kraw = K;
   s.built = false;
   s.raw_acc.release();
+  s.rlist.release();
 }
 
 // ---------------------------------------------------------------------------
-// structure
+// structure.  The first build scans the rows of the clustered vertices once (one
+// wave per vertex); afterwards every batch updates the per-member edge counts and
+// the internal edge list from its groups, so a rebuild costs O(M + internal
+// edges) and never reads a row.
 // ---------------------------------------------------------------------------
-__global__ void k_edge_class(uint32_t n, uint32_t m, const uint32_t* off, const uint32_t* col, const uint32_t* raw,
-                             uint32_t* inner, uint8_t* is_entry, uint8_t* is_exit) {
-  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m) return;
-  uint32_t u = row_of(off, n, e), v = col[e];
-  uint32_t cu = raw[u], cv = raw[v];
-  if (cu != INF32 && cu == cv) {
-    atomicAdd(&inner[cu], 1u);
-  } else {
-    if (cv != INF32) is_entry[v] = 1;
-    if (cu != INF32) is_exit[u] = 1;
+constexpr int kWaves = kBlock / 64;
+
+__global__ void k_raw_flag(uint32_t n, const uint32_t* raw, uint8_t* flag) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) flag[v] = raw[v] != INF32;
+}
+
+// Edges leaving / entering the raw cluster of each listed vertex and its internal
+// in-edges (judge_out_bound_node / get_nodetype_mirror, compressor_base.h:
+// 3924-3956, 4144-4182); `inner` gets the internal edge count of every raw cluster.
+__global__ void __launch_bounds__(kBlock) k_member_scan(const uint32_t* __restrict__ list, uint32_t L,
+                                                        const uint32_t* __restrict__ raw,
+                                                        const uint32_t* __restrict__ ooff,
+                                                        const uint32_t* __restrict__ ocol,
+                                                        const uint32_t* __restrict__ ioff,
+                                                        const uint32_t* __restrict__ icol, uint32_t* ext_out,
+                                                        uint32_t* ext_in, uint32_t* inner) {
+  const uint32_t w = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (w >= L) return;  // whole wave
+  const int lane = lane_id();
+  const uint32_t v = list[w], k = raw[v];
+  uint32_t ext = 0, inn = 0, exin = 0;
+  for (uint32_t e = ooff[v] + lane; e < ooff[v + 1]; e += 64) ext += raw[ocol[e]] != k;
+  for (uint32_t e = ioff[v] + lane; e < ioff[v + 1]; e += 64) {
+    const bool same = raw[icol[e]] == k;
+    inn += same;
+    exin += !same;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    ext += __shfl_xor(ext, o);
+    inn += __shfl_xor(inn, o);
+    exin += __shfl_xor(exin, o);
+  }
+  if (lane == 0) {
+    ext_out[v] = ext;
+    ext_in[v] = exin;
+    if (inn) atomicAdd(&inner[k], inn);
   }
 }
 
-__global__ void k_raw_counts(uint32_t n, const uint32_t* raw, const uint8_t* is_entry, const uint8_t* is_exit,
-                             uint32_t* size, uint32_t* ns, uint32_t* nb) {
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  uint32_t k = raw[v];
-  if (k == INF32) return;
+__global__ void k_raw_counts(const uint32_t* list, uint32_t L, const uint32_t* raw, const uint32_t* ext_out,
+                             const uint32_t* ext_in, uint32_t* size, uint32_t* ns, uint32_t* nb) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L) return;
+  uint32_t v = list[i], k = raw[v];
   atomicAdd(&size[k], 1u);
-  if (is_entry[v]) atomicAdd(&ns[k], 1u);
-  if (is_exit[v]) atomicAdd(&nb[k], 1u);
+  if (ext_in[v]) atomicAdd(&ns[k], 1u);
+  if (ext_out[v]) atomicAdd(&nb[k], 1u);
 }
 
 // benefit[0] of get_init_supernode_by_clusters, computed in double and compared
@@ -247,32 +278,42 @@ __global__ void k_acc_u32(uint32_t K, const uint8_t* acc, uint32_t* out) {
   if (k <= K) out[k] = k < K ? acc[k] : 0u;
 }
 
-__global__ void k_assign(uint32_t n, const uint32_t* raw, const uint8_t* acc, const uint32_t* newid,
-                         const uint8_t* is_entry, const uint8_t* is_exit, uint32_t* cid, uint8_t* ntype,
-                         unsigned long long* keys, uint32_t* hist, int kbits) {
+// accepted cluster id of every vertex (fixed after the first build)
+__global__ void k_assign_cid(uint32_t n, const uint32_t* raw, const uint8_t* acc, const uint32_t* newid,
+                             uint32_t* cid, uint8_t* ntype) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  uint32_t k = raw[v];
+  uint32_t id = (k != INF32 && acc[k]) ? newid[k] : INF32;
+  cid[v] = id;
+  ntype[v] = 0;  // SingleNode until typed below
+}
+
+// node type of every raw member from its edge counts, the member sort key
+// (cluster, exits first, id; rejected raw members last) and the type histogram
+__global__ void k_member_types(const uint32_t* list, uint32_t L, const uint32_t* cid, const uint32_t* ext_out,
+                               const uint32_t* ext_in, uint8_t* ntype, uint8_t* is_entry, uint8_t* is_exit,
+                               unsigned long long* keys, uint32_t* hist) {
   __shared__ uint32_t s_h[8];
   if (threadIdx.x < 8) s_h[threadIdx.x] = 0;
   __syncthreads();
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < n) {
-    uint32_t k = raw[v];
-    uint32_t id = (k != INF32 && acc[k]) ? newid[k] : INF32;
-    cid[v] = id;
-    uint8_t t;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < L) {
+    const uint32_t v = list[i], id = cid[v];
     if (id == INF32) {
-      t = 0;  // SingleNode
+      keys[i] = ~0ull;
     } else {
-      bool en = is_entry[v], ex = is_exit[v];
-      t = en && ex ? 3 : en ? 2 : ex ? 1 : 4;  // BothOutIn / OnlyIn / OnlyOut / Inner
+      const bool en = ext_in[v] != 0, ex = ext_out[v] != 0;
+      const uint8_t t = en && ex ? 3 : en ? 2 : ex ? 1 : 4;  // BothOutIn / OnlyIn / OnlyOut / Inner
+      ntype[v] = t;
+      is_entry[v] = en;
+      is_exit[v] = ex;
+      atomicAdd(&s_h[t], 1u);
+      keys[i] = ((unsigned long long)id << 33) | ((unsigned long long)(ex ? 0 : 1) << 32) | v;
     }
-    ntype[v] = t;
-    atomicAdd(&s_h[t], 1u);
-    keys[v] = id == INF32 ? ~0ull
-                          : (((unsigned long long)id << 33) | ((unsigned long long)(is_exit[v] ? 0 : 1) << 32) | v);
   }
   __syncthreads();
   if (threadIdx.x < 8 && s_h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], s_h[threadIdx.x]);
-  (void)kbits;
 }
 
 __global__ void k_members(const unsigned long long* keys, uint32_t M, uint32_t* cl_mem, uint32_t* cl_key) {
@@ -282,11 +323,12 @@ __global__ void k_members(const unsigned long long* keys, uint32_t M, uint32_t* 
   cl_key[i] = (uint32_t)(keys[i] >> 33);
 }
 
-__global__ void k_row_ptr32(const uint32_t* rowkey, uint32_t M, uint32_t K, uint32_t* off) {
+template <class K>
+__global__ void k_row_ptr32(const K* rowkey, uint32_t M, uint32_t K_, uint32_t* off, int shift) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > M) return;
-  int64_t row = (i == M) ? (int64_t)K : (int64_t)rowkey[i];
-  int64_t prev = (i == 0) ? -1 : (int64_t)rowkey[i - 1];
+  int64_t row = (i == M) ? (int64_t)K_ : (int64_t)(rowkey[i] >> shift);
+  int64_t prev = (i == 0) ? -1 : (int64_t)(rowkey[i - 1] >> shift);
   for (int64_t r = prev + 1; r <= row; ++r) off[r] = i;
 }
 
@@ -315,49 +357,66 @@ __global__ void k_entries(uint32_t E, const uint32_t* pos, const uint32_t* cl_me
   rowlen[e] = cl_off[k + 1] - cl_off[k];
 }
 
-// internal in-edges per member slot, and sketch out-edge counts per vertex
-__global__ void k_count_internal(uint32_t n, uint32_t m, const uint32_t* off, const uint32_t* col,
-                                 const uint32_t* cid, const uint32_t* lidx, const uint32_t* cl_off, uint32_t* ie_cnt,
-                                 uint32_t* ext_cnt) {
-  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m) return;
-  uint32_t u = row_of(off, n, e), v = col[e];
-  uint32_t cu = cid[u], cv = cid[v];
-  if (cu != INF32 && cu == cv) atomicAdd(&ie_cnt[cl_off[cu] + lidx[v]], 1u);
-  else if (cu != INF32) atomicAdd(&ext_cnt[u], 1u);
+__global__ void k_ent_of_clear(uint32_t M, const uint32_t* cl_mem, uint32_t* ent_of) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < M) ent_of[cl_mem[i]] = INF32;
 }
 
-__global__ void k_ib_len(uint32_t n, const uint32_t* off, const uint32_t* cid, const uint8_t* ntype,
-                         const uint32_t* ext_cnt, uint32_t* len) {
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v > n) return;
-  if (v == n) { len[n] = 0; return; }
-  uint8_t t = ntype[v];
-  len[v] = cid[v] == INF32 ? off[v + 1] - off[v] : ((t == 1 || t == 3) ? ext_cnt[v] : 0u);
-}
-
-__global__ void k_fill_internal(uint32_t n, uint32_t m, const uint32_t* off, const uint32_t* col, const uint32_t* w,
-                                const uint32_t* cid, const uint32_t* lidx, const uint32_t* cl_off,
-                                const uint32_t* ie_off, uint32_t* ie_cur, uint32_t* ie_src, uint32_t* ie_deg,
-                                uint32_t* ie_wt, const uint32_t* ib_off, uint32_t* ib_cur, uint32_t* ib_col,
-                                uint32_t* ib_w) {
-  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m) return;
-  uint32_t u = row_of(off, n, e), v = col[e];
-  uint32_t cu = cid[u], cv = cid[v];
-  if (cu != INF32 && cu == cv) {
-    uint32_t slot = cl_off[cu] + lidx[v];
-    uint32_t p = ie_off[slot] + atomicAdd(&ie_cur[slot], 1u);
-    ie_src[p] = lidx[u];
-    ie_deg[p] = off[u + 1] - off[u];
-    ie_wt[p] = w ? w[e] : 1u;
-  } else {
-    uint32_t p;
-    if (cu == INF32) p = ib_off[u] + (e - off[u]);       // Single: the whole row, in order
-    else p = ib_off[u] + atomicAdd(&ib_cur[u], 1u);      // exit: its external edges
-    ib_col[p] = v;
-    if (ib_w) ib_w[p] = w ? w[e] : 1u;
+// First build: the internal in-edges of every accepted member, one wave per
+// member, appended as (src, dst) keys with their weights.
+__global__ void __launch_bounds__(kBlock) k_internal_collect(const uint32_t* __restrict__ list, uint32_t L,
+                                                             const uint32_t* __restrict__ cid,
+                                                             const uint32_t* __restrict__ ioff,
+                                                             const uint32_t* __restrict__ icol,
+                                                             const uint32_t* __restrict__ iw,
+                                                             unsigned long long* key, uint32_t* wout,
+                                                             unsigned long long* count) {
+  const uint32_t i = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (i >= L) return;  // whole wave
+  const int lane = lane_id();
+  const uint32_t v = list[i], k = cid[v];
+  if (k == INF32) return;  // whole wave
+  const uint32_t b = ioff[v], e = ioff[v + 1];
+  for (uint32_t e0 = b; e0 < e; e0 += 64) {
+    const uint32_t x = e0 + lane;
+    uint32_t src = 0;
+    bool in = false;
+    if (x < e) {
+      src = icol[x];
+      in = cid[src] == k;
+    }
+    const unsigned long long slot = wave_append(in, count);
+    if (in) {
+      key[slot] = ((unsigned long long)src << 32) | v;
+      wout[slot] = iw ? iw[x] : 1u;
+    }
   }
+}
+
+// Groups the internal edge list by destination member slot: sort key
+// (slot << 32 | list index).
+__global__ void k_internal_slot_keys(uint64_t N, const unsigned long long* key, const uint32_t* cid,
+                                     const uint32_t* cl_off, const uint32_t* lidx, unsigned long long* skey,
+                                     uint32_t* idx) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const uint32_t dst = (uint32_t)key[i];
+  skey[i] = ((unsigned long long)(cl_off[cid[dst]] + lidx[dst]) << 32) | (uint32_t)i;
+  idx[i] = (uint32_t)i;
+}
+
+__global__ void k_internal_fill(uint64_t N, const unsigned long long* skey, const unsigned long long* key,
+                                const uint32_t* w, const uint32_t* lidx, const uint32_t* ooff, uint32_t* ie_src,
+                                uint32_t* ie_dst, uint32_t* ie_deg, uint32_t* ie_wt) {
+  uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const uint32_t i = (uint32_t)skey[p];
+  const unsigned long long kk = key[i];
+  const uint32_t src = (uint32_t)(kk >> 32), dst = (uint32_t)kk;
+  ie_src[p] = lidx[src];
+  ie_dst[p] = lidx[dst];
+  ie_deg[p] = ooff[src + 1] - ooff[src];
+  ie_wt[p] = w[i];
 }
 
 __global__ void k_max_cluster(uint32_t K, const uint32_t* cl_off, uint32_t* out) {
@@ -365,63 +424,72 @@ __global__ void k_max_cluster(uint32_t K, const uint32_t* cl_off, uint32_t* out)
   if (k < K) atomicMax(out, cl_off[k + 1] - cl_off[k]);
 }
 
-// Builds (or rebuilds after a batch) the sketch structure from the raw cluster
-// assignment.  Acceptance is decided on the first build and then kept, like the
-// reference keeps its supernodes across update batches.
-void sketch_build(suminc_ctx* c, uint32_t min_node, uint32_t max_node, double threshold) {
-  SUMINC_REQUIRE(c->has_graph, SUMINC_E_STATE, "no graph");
-  Sketch& s = c->sk;
-  SUMINC_REQUIRE(s.raw_cid.p, SUMINC_E_STATE, "no cluster assignment (call suminc_cluster_* first)");
-  const uint32_t n = c->n, m = (uint32_t)c->m;
-  const uint32_t Kr = s.Kraw;
-  DevBuf inner, size, ns, nb, acc32, newid, keys_a, keys_b, ext_cnt, ibl, hist, ckey, entf, epos, ecount, ekey,
-      rowlen, iecnt, iecur, ibcur, mx;
-  s.is_entry.ensure(n); s.is_exit.ensure(n);
-  SUMINC_HIP(hipMemsetAsync(s.is_entry.p, 0, n, c->stream));
-  SUMINC_HIP(hipMemsetAsync(s.is_exit.p, 0, n, c->stream));
-  inner.ensure((size_t)Kr * 4 + 4);
-  SUMINC_HIP(hipMemsetAsync(inner.p, 0, (size_t)Kr * 4 + 4, c->stream));
-  if (m)
-    k_edge_class<<<grid_for(m), kBlock, 0, c->stream>>>(n, m, c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
-                                                        s.raw_cid.as<uint32_t>(), inner.as<uint32_t>(),
-                                                        s.is_entry.as<uint8_t>(), s.is_exit.as<uint8_t>());
-  const bool decide = !s.raw_acc.p;
-  if (decide) {
-    size.ensure((size_t)Kr * 4 + 4); ns.ensure((size_t)Kr * 4 + 4); nb.ensure((size_t)Kr * 4 + 4);
-    for (DevBuf* b : {&size, &ns, &nb}) SUMINC_HIP(hipMemsetAsync(b->p, 0, (size_t)Kr * 4 + 4, c->stream));
-    k_raw_counts<<<grid_for(n), kBlock, 0, c->stream>>>(n, s.raw_cid.as<uint32_t>(), s.is_entry.as<uint8_t>(),
-                                                        s.is_exit.as<uint8_t>(), size.as<uint32_t>(),
-                                                        ns.as<uint32_t>(), nb.as<uint32_t>());
-    s.raw_acc.ensure(Kr + 1);
-    if (Kr)
-      k_accept<<<grid_for(Kr), kBlock, 0, c->stream>>>(Kr, size.as<uint32_t>(), inner.as<uint32_t>(),
-                                                       ns.as<uint32_t>(), nb.as<uint32_t>(), min_node, max_node,
-                                                       (float)threshold, s.raw_acc.as<uint8_t>());
+// Moves the shortcut rows of clusters the batch did not touch from the previous
+// layout: one workgroup per entry of the new layout.
+template <class T>
+__global__ void k_rows_move(uint32_t E, const uint32_t* __restrict__ cl_ent, const uint32_t* __restrict__ cid,
+                            const uint32_t* __restrict__ cl_off, const unsigned long long* __restrict__ sc_base,
+                            const uint32_t* __restrict__ old_ent_of,
+                            const unsigned long long* __restrict__ old_sc_base, const uint8_t* __restrict__ valid,
+                            const T* __restrict__ oldA, T* __restrict__ newA, const T* __restrict__ oldB,
+                            T* __restrict__ newB) {
+  const uint32_t e = blockIdx.x;
+  if (e >= E) return;
+  const uint32_t v = cl_ent[e], k = cid[v];
+  const uint32_t oe = old_ent_of[v];
+  if (!valid[k] || oe == INF32) return;
+  const uint32_t size = cl_off[k + 1] - cl_off[k];
+  const unsigned long long nb = sc_base[e], ob = old_sc_base[oe];
+  for (uint32_t j = threadIdx.x; j < size; j += blockDim.x) {
+    newA[nb + j] = oldA[ob + j];
+    newB[nb + j] = oldB[ob + j];
   }
-  acc32.ensure(((size_t)Kr + 1) * 4); newid.ensure(((size_t)Kr + 1) * 4);
-  k_acc_u32<<<grid_for((uint64_t)Kr + 1), kBlock, 0, c->stream>>>(Kr, s.raw_acc.as<uint8_t>(), acc32.as<uint32_t>());
-  cub(c, [&](void* p, size_t& b) {
-    return rocprim::exclusive_scan(p, b, acc32.as<uint32_t>(), newid.as<uint32_t>(), 0u, (size_t)Kr + 1,
-                                   rocprim::plus<uint32_t>(), c->stream);
-  });
-  uint32_t K = 0;
-  SUMINC_HIP(hipMemcpyAsync(&K, newid.as<uint32_t>() + Kr, 4, hipMemcpyDeviceToHost, c->stream));
-  // node types + member keys
-  s.cid.ensure((size_t)n * 4); s.ntype.ensure(n); s.lidx.ensure((size_t)n * 4);
-  keys_a.ensure((size_t)n * 8); keys_b.ensure((size_t)n * 8); hist.ensure(32);
+}
+
+// Grows a buffer keeping its first `keep` bytes.
+static void grow_keep(suminc_ctx* c, DevBuf& b, size_t bytes, size_t keep) {
+  if (bytes <= b.cap && b.p) return;
+  DevBuf t;
+  t.ensure(bytes);
+  if (keep && b.p) SUMINC_HIP(hipMemcpyAsync(t.p, b.p, keep, hipMemcpyDeviceToDevice, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  b.swap(t);
+}
+
+// Layout of the current structure from the per-member edge counts and the
+// internal edge list: node types, member order, entries, internal edges grouped
+// by destination slot, moved shortcut rows.
+static void sketch_layout(suminc_ctx* c, bool rebuild) {
+  Sketch& s = c->sk;
+  const uint32_t n = c->n, K = s.K, RM = s.RM;
+  DevBuf keys_a, keys_b, hist, ckey, entf, epos, ecount, ekey, rowlen, mx, skey_a, skey_b, sidx_a, sidx_b;
+  keys_a.ensure((size_t)RM * 8 + 8); keys_b.ensure((size_t)RM * 8 + 8); hist.ensure(32);
   SUMINC_HIP(hipMemsetAsync(hist.p, 0, 32, c->stream));
-  k_assign<<<grid_for(n), kBlock, 0, c->stream>>>(n, s.raw_cid.as<uint32_t>(), s.raw_acc.as<uint8_t>(),
-                                                  newid.as<uint32_t>(), s.is_entry.as<uint8_t>(),
-                                                  s.is_exit.as<uint8_t>(), s.cid.as<uint32_t>(), s.ntype.as<uint8_t>(),
-                                                  keys_a.as<unsigned long long>(), hist.as<uint32_t>(), 0);
-  cub(c, [&](void* p, size_t& b) {
-    return rocprim::radix_sort_keys(p, b, keys_a.as<uint64_t>(), keys_b.as<uint64_t>(), (size_t)n, 0, 64, c->stream);
-  });
+  if (RM) {
+    k_member_types<<<grid_for(RM), kBlock, 0, c->stream>>>(s.rlist.as<uint32_t>(), RM, s.cid.as<uint32_t>(),
+                                                           s.ext_out.as<uint32_t>(), s.ext_in.as<uint32_t>(),
+                                                           s.ntype.as<uint8_t>(), s.is_entry.as<uint8_t>(),
+                                                           s.is_exit.as<uint8_t>(), keys_a.as<unsigned long long>(),
+                                                           hist.as<uint32_t>());
+    cub(c, [&](void* p, size_t& b) {
+      return rocprim::radix_sort_keys(p, b, keys_a.as<uint64_t>(), keys_b.as<uint64_t>(), (size_t)RM, 0, 64,
+                                      c->stream);
+    });
+  }
   SUMINC_HIP(hipMemcpyAsync(s.counts, hist.p, 32, hipMemcpyDeviceToHost, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
-  const uint32_t M = n - s.counts[0];
-  s.K = K;
+  uint32_t M = 0;
+  for (int t = 1; t < 5; ++t) M += s.counts[t];
+  s.counts[0] = n - M;
+  SUMINC_REQUIRE(!rebuild || M == s.M, SUMINC_E_STATE, "supernode membership changed on a rebuild");
   s.M = M;
+  // the previous layout is kept for moving untouched shortcut rows
+  if (rebuild) {
+    s.sc_base.swap(s.sc_base_old);
+    s.ent_of.swap(s.ent_of_old);
+    s.sc_R.swap(s.sc_R_old); s.sc_D.swap(s.sc_D_old);
+    s.ss_dist.swap(s.ss_dist_old); s.ss_par.swap(s.ss_par_old);
+  }
   // cluster member layout
   s.cl_mem.ensure((size_t)M * 4 + 4); ckey.ensure((size_t)M * 4 + 4); s.cl_off.ensure(((size_t)K + 1) * 4);
   s.cl_nexit.ensure((size_t)K * 4 + 4); entf.ensure(M + 1);
@@ -429,8 +497,8 @@ void sketch_build(suminc_ctx* c, uint32_t min_node, uint32_t max_node, double th
   if (M) {
     k_members<<<grid_for(M), kBlock, 0, c->stream>>>(keys_b.as<unsigned long long>(), M, s.cl_mem.as<uint32_t>(),
                                                      ckey.as<uint32_t>());
-    k_row_ptr32<<<grid_for((uint64_t)M + 1), kBlock, 0, c->stream>>>(ckey.as<uint32_t>(), M, K,
-                                                                     s.cl_off.as<uint32_t>());
+    k_row_ptr32<uint32_t><<<grid_for((uint64_t)M + 1), kBlock, 0, c->stream>>>(ckey.as<uint32_t>(), M, K,
+                                                                               s.cl_off.as<uint32_t>(), 0);
     k_member_index<<<grid_for(M), kBlock, 0, c->stream>>>(M, s.cl_mem.as<uint32_t>(), ckey.as<uint32_t>(),
                                                           s.cl_off.as<uint32_t>(), s.is_exit.as<uint8_t>(),
                                                           s.is_entry.as<uint8_t>(), s.lidx.as<uint32_t>(),
@@ -450,63 +518,50 @@ void sketch_build(suminc_ctx* c, uint32_t min_node, uint32_t max_node, double th
   SUMINC_HIP(hipMemcpyAsync(&E, ecount.p, 4, hipMemcpyDeviceToHost, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
   s.E = E;
-  s.cl_ent.ensure((size_t)E * 4 + 4); ekey.ensure((size_t)E * 4 + 4); s.ent_of.ensure((size_t)n * 4);
+  s.cl_ent.ensure((size_t)E * 4 + 4); ekey.ensure((size_t)E * 4 + 4);
+  const bool fresh_ent_of = !rebuild || !s.ent_of.p || s.ent_of.cap < (size_t)n * 4;
+  s.ent_of.ensure((size_t)n * 4);
   s.cl_eoff.ensure(((size_t)K + 1) * 4); rowlen.ensure(((size_t)E + 1) * 8); s.sc_base.ensure(((size_t)E + 1) * 8);
-  k_fill32<<<grid_for(n), kBlock, 0, c->stream>>>(s.ent_of.as<uint32_t>(), n, INF32);
+  if (fresh_ent_of) k_fill32<<<grid_for(n), kBlock, 0, c->stream>>>(s.ent_of.as<uint32_t>(), n, INF32);
+  else if (M) k_ent_of_clear<<<grid_for(M), kBlock, 0, c->stream>>>(M, s.cl_mem.as<uint32_t>(), s.ent_of.as<uint32_t>());
   k_entries<<<grid_for((uint64_t)E + 1), kBlock, 0, c->stream>>>(E, epos.as<uint32_t>(), s.cl_mem.as<uint32_t>(),
                                                                  ckey.as<uint32_t>(), s.cl_off.as<uint32_t>(),
                                                                  s.cl_ent.as<uint32_t>(), ekey.as<uint32_t>(),
                                                                  s.ent_of.as<uint32_t>(),
                                                                  rowlen.as<unsigned long long>());
-  k_row_ptr32<<<grid_for((uint64_t)E + 1), kBlock, 0, c->stream>>>(ekey.as<uint32_t>(), E, K,
-                                                                   s.cl_eoff.as<uint32_t>());
+  k_row_ptr32<uint32_t><<<grid_for((uint64_t)E + 1), kBlock, 0, c->stream>>>(ekey.as<uint32_t>(), E, K,
+                                                                             s.cl_eoff.as<uint32_t>(), 0);
   cub(c, [&](void* p, size_t& b) {
     return rocprim::exclusive_scan(p, b, rowlen.as<unsigned long long>(), s.sc_base.as<unsigned long long>(), 0ull,
                                    (size_t)E + 1, rocprim::plus<unsigned long long>(), c->stream);
   });
   unsigned long long sc_len = 0;
   SUMINC_HIP(hipMemcpyAsync(&sc_len, s.sc_base.as<unsigned long long>() + E, 8, hipMemcpyDeviceToHost, c->stream));
-  // internal in-edges and sketch out-edges
-  iecnt.ensure(((size_t)M + 1) * 4); ext_cnt.ensure((size_t)n * 4); ibl.ensure(((size_t)n + 1) * 4);
-  SUMINC_HIP(hipMemsetAsync(iecnt.p, 0, ((size_t)M + 1) * 4, c->stream));
-  SUMINC_HIP(hipMemsetAsync(ext_cnt.p, 0, (size_t)n * 4, c->stream));
-  if (m)
-    k_count_internal<<<grid_for(m), kBlock, 0, c->stream>>>(n, m, c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
-                                                            s.cid.as<uint32_t>(), s.lidx.as<uint32_t>(),
-                                                            s.cl_off.as<uint32_t>(), iecnt.as<uint32_t>(),
-                                                            ext_cnt.as<uint32_t>());
-  k_ib_len<<<grid_for((uint64_t)n + 1), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), s.cid.as<uint32_t>(),
-                                                                s.ntype.as<uint8_t>(), ext_cnt.as<uint32_t>(),
-                                                                ibl.as<uint32_t>());
-  s.ie_off.ensure(((size_t)M + 1) * 4); s.ib_off.ensure(((size_t)n + 1) * 4);
-  cub(c, [&](void* p, size_t& b) {
-    return rocprim::exclusive_scan(p, b, iecnt.as<uint32_t>(), s.ie_off.as<uint32_t>(), 0u, (size_t)M + 1,
-                                   rocprim::plus<uint32_t>(), c->stream);
-  });
-  cub(c, [&](void* p, size_t& b) {
-    return rocprim::exclusive_scan(p, b, ibl.as<uint32_t>(), s.ib_off.as<uint32_t>(), 0u, (size_t)n + 1,
-                                   rocprim::plus<uint32_t>(), c->stream);
-  });
-  uint32_t ie_m = 0, ib_m = 0;
-  SUMINC_HIP(hipMemcpyAsync(&ie_m, s.ie_off.as<uint32_t>() + M, 4, hipMemcpyDeviceToHost, c->stream));
-  SUMINC_HIP(hipMemcpyAsync(&ib_m, s.ib_off.as<uint32_t>() + n, 4, hipMemcpyDeviceToHost, c->stream));
-  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  // internal edges grouped by destination slot
+  const uint64_t ie_m = s.iem_n;
   s.ie_m = ie_m;
-  s.ib_m = ib_m;
-  s.sc_len = sc_len;
-  s.ie_src.ensure((size_t)ie_m * 4 + 4); s.ie_w.ensure(((size_t)ie_m * 2 + 2) * 4);
-  s.ib_col.ensure((size_t)ib_m * 4 + 4);
-  if (c->wkind == SUMINC_W_U32) s.ib_w.ensure((size_t)ib_m * 4 + 4);
-  iecur.ensure(((size_t)M + 1) * 4); ibcur.ensure((size_t)n * 4);
-  SUMINC_HIP(hipMemsetAsync(iecur.p, 0, ((size_t)M + 1) * 4, c->stream));
-  SUMINC_HIP(hipMemsetAsync(ibcur.p, 0, (size_t)n * 4, c->stream));
-  if (m)
-    k_fill_internal<<<grid_for(m), kBlock, 0, c->stream>>>(
-        n, m, c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
-        c->wkind == SUMINC_W_U32 ? c->out_w.as<uint32_t>() : nullptr, s.cid.as<uint32_t>(), s.lidx.as<uint32_t>(),
-        s.cl_off.as<uint32_t>(), s.ie_off.as<uint32_t>(), iecur.as<uint32_t>(), s.ie_src.as<uint32_t>(),
-        s.ie_w.as<uint32_t>(), s.ie_w.as<uint32_t>() + ie_m + 1, s.ib_off.as<uint32_t>(), ibcur.as<uint32_t>(),
-        s.ib_col.as<uint32_t>(), c->wkind == SUMINC_W_U32 ? s.ib_w.as<uint32_t>() : nullptr);
+  s.ie_off.ensure(((size_t)M + 1) * 4);
+  for (DevBuf* b : {&s.ie_src, &s.ie_dst, &s.ie_deg, &s.ie_wt}) b->ensure(ie_m * 4 + 4);
+  if (ie_m) {
+    skey_a.ensure(ie_m * 8); skey_b.ensure(ie_m * 8); sidx_a.ensure(ie_m * 4); sidx_b.ensure(ie_m * 4);
+    k_internal_slot_keys<<<grid_for(ie_m), kBlock, 0, c->stream>>>(ie_m, s.iem_key.as<unsigned long long>(),
+                                                                   s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(),
+                                                                   s.lidx.as<uint32_t>(),
+                                                                   skey_a.as<unsigned long long>(),
+                                                                   sidx_a.as<uint32_t>());
+    cub(c, [&](void* p, size_t& b) {
+      return rocprim::radix_sort_keys(p, b, skey_a.as<uint64_t>(), skey_b.as<uint64_t>(), (size_t)ie_m, 0, 64,
+                                      c->stream);
+    });
+    k_internal_fill<<<grid_for(ie_m), kBlock, 0, c->stream>>>(
+        ie_m, skey_b.as<unsigned long long>(), s.iem_key.as<unsigned long long>(), s.iem_w.as<uint32_t>(),
+        s.lidx.as<uint32_t>(), c->out_off.as<uint32_t>(), s.ie_src.as<uint32_t>(), s.ie_dst.as<uint32_t>(),
+        s.ie_deg.as<uint32_t>(), s.ie_wt.as<uint32_t>());
+    k_row_ptr32<unsigned long long><<<grid_for(ie_m + 1), kBlock, 0, c->stream>>>(
+        skey_b.as<unsigned long long>(), (uint32_t)ie_m, M, s.ie_off.as<uint32_t>(), 32);
+  } else {
+    SUMINC_HIP(hipMemsetAsync(s.ie_off.p, 0, ((size_t)M + 1) * 4, c->stream));
+  }
   mx.ensure(4);
   SUMINC_HIP(hipMemsetAsync(mx.p, 0, 4, c->stream));
   if (K) k_max_cluster<<<grid_for(K), kBlock, 0, c->stream>>>(K, s.cl_off.as<uint32_t>(), mx.as<uint32_t>());
@@ -516,40 +571,264 @@ void sketch_build(suminc_ctx* c, uint32_t min_node, uint32_t max_node, double th
   SUMINC_HIP(hipStreamSynchronize(c->stream));
   SUMINC_REQUIRE(maxc <= (uint32_t)kMaxCluster, SUMINC_E_TOO_LARGE, "an accepted cluster has more than 2048 members");
   s.max_cluster = maxc;
-  // every cluster needs a precompute after a (re)build of the PageRank side
-  s.dirty.ensure((size_t)K + 1);
-  if (!s.built || s.K != K) SUMINC_HIP(hipMemsetAsync(s.dirty.p, 1, (size_t)K + 1, c->stream));
+  s.sc_len = sc_len;
+  // shortcut rows: validity per cluster (PageRank, SSSP)
+  s.pr_valid.ensure((size_t)K + 1); s.ss_valid.ensure((size_t)K + 1);
+  if (!rebuild) {
+    SUMINC_HIP(hipMemsetAsync(s.pr_valid.p, 0, (size_t)K + 1, c->stream));
+    SUMINC_HIP(hipMemsetAsync(s.ss_valid.p, 0, (size_t)K + 1, c->stream));
+    s.pr_rows_for = -1;
+    s.ss_rows = false;
+  } else {
+    if (s.pr_rows_for >= 0) {
+      const size_t es = s.pr_rows_for == SUMINC_PR_FP64 ? 8 : 4;
+      s.sc_R.ensure(sc_len * es + es); s.sc_D.ensure(sc_len * es + es);
+      if (E && es == 8)
+        k_rows_move<double><<<E, kBlock, 0, c->stream>>>(
+            E, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(),
+            s.sc_base.as<unsigned long long>(), s.ent_of_old.as<uint32_t>(), s.sc_base_old.as<unsigned long long>(),
+            s.pr_valid.as<uint8_t>(), s.sc_R_old.as<double>(), s.sc_R.as<double>(), s.sc_D_old.as<double>(),
+            s.sc_D.as<double>());
+      else if (E)
+        k_rows_move<float><<<E, kBlock, 0, c->stream>>>(
+            E, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(),
+            s.sc_base.as<unsigned long long>(), s.ent_of_old.as<uint32_t>(), s.sc_base_old.as<unsigned long long>(),
+            s.pr_valid.as<uint8_t>(), s.sc_R_old.as<float>(), s.sc_R.as<float>(), s.sc_D_old.as<float>(),
+            s.sc_D.as<float>());
+    }
+    if (s.ss_rows) {
+      s.ss_dist.ensure(sc_len * 4 + 4); s.ss_par.ensure(sc_len * 4 + 4);
+      if (E)
+        k_rows_move<uint32_t><<<E, kBlock, 0, c->stream>>>(
+            E, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(),
+            s.sc_base.as<unsigned long long>(), s.ent_of_old.as<uint32_t>(), s.sc_base_old.as<unsigned long long>(),
+            s.ss_valid.as<uint8_t>(), s.ss_dist_old.as<uint32_t>(), s.ss_dist.as<uint32_t>(),
+            s.ss_par_old.as<uint32_t>(), s.ss_par.as<uint32_t>());
+    }
+    SUMINC_HIP(hipGetLastError());
+  }
+  s.pr_need = s.ss_need = true;
   s.version = c->graph_version;
-  s.built = true;
-  s.pr_rows_version = ~0ull;
-  s.sssp_rows_version = ~0ull;
   SUMINC_HIP(hipStreamSynchronize(c->stream));
 }
 
-// A batch marks the clusters of both endpoints of every op dirty (the reference's
-// update_cluster_ids, compressor_base.h:5634-5852); the structure is rebuilt on
-// the next query.
-__global__ void k_mark_dirty(const uint32_t* rows, const uint32_t* cols, const unsigned* countp, uint32_t N,
-                             const uint32_t* cid, uint8_t* dirty) {
-  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= N || g >= *countp) return;
-  uint32_t a = cid[rows[g]], b = cid[cols[g]];
-  if (a != INF32) dirty[a] = 1;
-  if (b != INF32) dirty[b] = 1;
+// First build from the raw cluster assignment: scans the rows of the raw members,
+// decides acceptance (kept across batches, as the reference keeps its supernodes)
+// and collects the internal edge list; then lays the structure out.
+void sketch_build(suminc_ctx* c, uint32_t min_node, uint32_t max_node, double threshold) {
+  SUMINC_REQUIRE(c->has_graph, SUMINC_E_STATE, "no graph");
+  Sketch& s = c->sk;
+  SUMINC_REQUIRE(s.raw_cid.p, SUMINC_E_STATE, "no cluster assignment (call suminc_cluster_* first)");
+  const uint32_t n = c->n;
+  const uint32_t Kr = s.Kraw;
+  DevBuf inner, size, ns, nb, acc32, newid, rflag, rcount, icount;
+  // raw member list
+  rflag.ensure(n + 1); rcount.ensure(8);
+  s.rlist.ensure((size_t)n * 4 + 4);
+  k_raw_flag<<<grid_for(n), kBlock, 0, c->stream>>>(n, s.raw_cid.as<uint32_t>(), rflag.as<uint8_t>());
+  cub(c, [&](void* p, size_t& b) {
+    return rocprim::select(p, b, rocprim::counting_iterator<uint32_t>(0), rflag.as<uint8_t>(), s.rlist.as<uint32_t>(),
+                           rcount.as<unsigned>(), (size_t)n, c->stream);
+  });
+  unsigned rm = 0;
+  SUMINC_HIP(hipMemcpyAsync(&rm, rcount.p, 4, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  s.RM = rm;
+  const uint32_t RM = rm;
+  s.is_entry.ensure(n); s.is_exit.ensure(n); s.ext_out.ensure((size_t)n * 4); s.ext_in.ensure((size_t)n * 4);
+  SUMINC_HIP(hipMemsetAsync(s.is_entry.p, 0, n, c->stream));
+  SUMINC_HIP(hipMemsetAsync(s.is_exit.p, 0, n, c->stream));
+  inner.ensure((size_t)Kr * 4 + 4);
+  SUMINC_HIP(hipMemsetAsync(inner.p, 0, (size_t)Kr * 4 + 4, c->stream));
+  if (RM)
+    k_member_scan<<<(RM + kWaves - 1) / kWaves, kBlock, 0, c->stream>>>(
+        s.rlist.as<uint32_t>(), RM, s.raw_cid.as<uint32_t>(), c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
+        c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(), s.ext_out.as<uint32_t>(), s.ext_in.as<uint32_t>(),
+        inner.as<uint32_t>());
+  size.ensure((size_t)Kr * 4 + 4); ns.ensure((size_t)Kr * 4 + 4); nb.ensure((size_t)Kr * 4 + 4);
+  for (DevBuf* b : {&size, &ns, &nb}) SUMINC_HIP(hipMemsetAsync(b->p, 0, (size_t)Kr * 4 + 4, c->stream));
+  if (RM)
+    k_raw_counts<<<grid_for(RM), kBlock, 0, c->stream>>>(s.rlist.as<uint32_t>(), RM, s.raw_cid.as<uint32_t>(),
+                                                         s.ext_out.as<uint32_t>(), s.ext_in.as<uint32_t>(),
+                                                         size.as<uint32_t>(), ns.as<uint32_t>(), nb.as<uint32_t>());
+  s.raw_acc.ensure(Kr + 1);
+  if (Kr)
+    k_accept<<<grid_for(Kr), kBlock, 0, c->stream>>>(Kr, size.as<uint32_t>(), inner.as<uint32_t>(), ns.as<uint32_t>(),
+                                                     nb.as<uint32_t>(), min_node, max_node, (float)threshold,
+                                                     s.raw_acc.as<uint8_t>());
+  acc32.ensure(((size_t)Kr + 1) * 4); newid.ensure(((size_t)Kr + 1) * 4);
+  k_acc_u32<<<grid_for((uint64_t)Kr + 1), kBlock, 0, c->stream>>>(Kr, s.raw_acc.as<uint8_t>(), acc32.as<uint32_t>());
+  cub(c, [&](void* p, size_t& b) {
+    return rocprim::exclusive_scan(p, b, acc32.as<uint32_t>(), newid.as<uint32_t>(), 0u, (size_t)Kr + 1,
+                                   rocprim::plus<uint32_t>(), c->stream);
+  });
+  uint32_t K = 0;
+  SUMINC_HIP(hipMemcpyAsync(&K, newid.as<uint32_t>() + Kr, 4, hipMemcpyDeviceToHost, c->stream));
+  s.cid.ensure((size_t)n * 4); s.ntype.ensure(n); s.lidx.ensure((size_t)n * 4);
+  k_assign_cid<<<grid_for(n), kBlock, 0, c->stream>>>(n, s.raw_cid.as<uint32_t>(), s.raw_acc.as<uint8_t>(),
+                                                      newid.as<uint32_t>(), s.cid.as<uint32_t>(), s.ntype.as<uint8_t>());
+  // internal edges of the accepted clusters, sorted by (src, dst)
+  icount.ensure(8);
+  SUMINC_HIP(hipMemsetAsync(icount.p, 0, 8, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  s.K = K;
+  {
+    // the internal edges of the accepted clusters size the list
+    std::vector<uint32_t> hin(Kr + 1);
+    std::vector<uint8_t> hacc(Kr + 1);
+    SUMINC_HIP(hipMemcpy(hin.data(), inner.p, (size_t)Kr * 4, hipMemcpyDeviceToHost));
+    SUMINC_HIP(hipMemcpy(hacc.data(), s.raw_acc.p, Kr, hipMemcpyDeviceToHost));
+    uint64_t tot = 0;
+    for (uint32_t k = 0; k < Kr; ++k)
+      if (hacc[k]) tot += hin[k];
+    const size_t cap = tot + 4096;
+    s.iem_key.ensure(cap * 8); s.iem_w.ensure(cap * 4); s.iem_key2.ensure(cap * 8); s.iem_w2.ensure(cap * 4);
+    if (RM)
+      k_internal_collect<<<(RM + kWaves - 1) / kWaves, kBlock, 0, c->stream>>>(
+          s.rlist.as<uint32_t>(), RM, s.cid.as<uint32_t>(), c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
+          c->wkind == SUMINC_W_U32 ? c->in_w.as<uint32_t>() : nullptr, s.iem_key2.as<unsigned long long>(),
+          s.iem_w2.as<uint32_t>(), icount.as<unsigned long long>());
+    SUMINC_HIP(hipGetLastError());
+    unsigned long long got = 0;
+    SUMINC_HIP(hipMemcpyAsync(&got, icount.p, 8, hipMemcpyDeviceToHost, c->stream));
+    SUMINC_HIP(hipStreamSynchronize(c->stream));
+    SUMINC_REQUIRE(got == tot, SUMINC_E_STATE, "internal edge count mismatch");
+    if (got)
+      cub(c, [&](void* p, size_t& b) {
+        return rocprim::radix_sort_pairs(p, b, s.iem_key2.as<uint64_t>(), s.iem_key.as<uint64_t>(),
+                                         s.iem_w2.as<uint32_t>(), s.iem_w.as<uint32_t>(), (size_t)got, 0, 64,
+                                         c->stream);
+      });
+    s.iem_n = got;
+    s.iem_dead.ensure(cap);
+    SUMINC_HIP(hipMemsetAsync(s.iem_dead.p, 0, cap, c->stream));
+  }
+  sketch_layout(c, false);
+  // every cluster needs a precompute after the first build of the PageRank side
+  s.dirty.ensure((size_t)K + 1);
+  SUMINC_HIP(hipMemsetAsync(s.dirty.p, 1, (size_t)K + 1, c->stream));
+  s.built = true;
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
 }
 
-void sketch_mark_dirty(suminc_ctx* c, const uint32_t* rows, const uint32_t* cols, const unsigned* countp, uint32_t N) {
+// ---------------------------------------------------------------------------
+// batches (the reference's inc_compress_mirror bookkeeping, compressor_base.h:
+// 5634-5852, and inc_judge_out_bound_node, :3958-3990): every group of a batch
+// marks the clusters of its endpoints dirty (precompute) and their shortcut rows
+// stale, updates the edge counts that decide entry/exit status, and updates the
+// internal edge list.  The layout is redone on the next query.
+// ---------------------------------------------------------------------------
+__global__ void k_sk_batch(const uint32_t* __restrict__ row, const uint32_t* __restrict__ col,
+                           const uint8_t* __restrict__ act, const uint8_t* __restrict__ ex,
+                           const uint32_t* __restrict__ gw, const unsigned* countp, uint32_t N, int wu32,
+                           const uint32_t* __restrict__ cid, uint8_t* dirty, uint8_t* pr_valid, uint8_t* ss_valid,
+                           uint32_t* ext_out, uint32_t* ext_in, const unsigned long long* __restrict__ iem_key,
+                           uint64_t iem_n, uint8_t* iem_dead, unsigned long long* add_key, uint32_t* add_w,
+                           unsigned long long* add_cnt, unsigned long long* delta) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  bool app = false;
+  unsigned long long key = 0;
+  uint32_t w = 1;
+  if (g < N && g < *countp) {
+    const uint32_t u = row[g], v = col[g];
+    const uint32_t cu = cid[u], cv = cid[v];
+    if (cu != INF32) { dirty[cu] = 1; pr_valid[cu] = 0; ss_valid[cu] = 0; }
+    if (cv != INF32) { dirty[cv] = 1; pr_valid[cv] = 0; ss_valid[cv] = 0; }
+    const bool existed = ex[g], add = act[g] == 0;
+    const bool removed = existed && !add, added = !existed && add;
+    if ((removed || added) && cu != cv) {
+      if (cu != INF32) atomicAdd(&ext_out[u], added ? 1u : 0xFFFFFFFFu);
+      if (cv != INF32) atomicAdd(&ext_in[v], added ? 1u : 0xFFFFFFFFu);
+    }
+    if (cu != INF32 && cu == cv) {
+      key = ((unsigned long long)u << 32) | v;
+      if (existed) {  // deleted or re-weighted: the old entry goes
+        uint64_t lo = 0, hi = iem_n;
+        while (lo < hi) {
+          const uint64_t mid = (lo + hi) >> 1;
+          if (iem_key[mid] < key) lo = mid + 1; else hi = mid;
+        }
+        if (lo < iem_n && iem_key[lo] == key) {
+          iem_dead[lo] = 1;
+          atomicAdd(delta, 0xFFFFFFFFFFFFFFFFull);
+        }
+      }
+      if (add) {
+        app = true;
+        w = wu32 ? gw[g] : 1u;
+      }
+    }
+  }
+  const unsigned long long slot = wave_append(app, add_cnt);
+  if (app) {
+    add_key[slot] = key;
+    add_w[slot] = w;
+    atomicAdd(delta, 1ull);
+  }
+}
+
+__global__ void k_sk_batch_prep(unsigned long long* key_tail, uint32_t N, unsigned long long* add_cnt,
+                                unsigned long long* delta) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) key_tail[i] = ~0ull;
+  if (i == 0) {
+    *add_cnt = 0;
+    *delta = 0;
+  }
+}
+
+__global__ void k_sk_drop_dead(uint64_t n, unsigned long long* key, uint8_t* dead) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && dead[i]) {
+    key[i] = ~0ull;
+    dead[i] = 0;
+  }
+}
+
+void sketch_batch(suminc_ctx* c, const uint32_t* rows, const uint32_t* cols, const uint8_t* act, const uint8_t* ex,
+                  const uint32_t* w, const unsigned* countp, uint32_t N) {
   Sketch& s = c->sk;
   if (!s.built || !N) return;
-  k_mark_dirty<<<grid_for(N), kBlock, 0, c->stream>>>(rows, cols, countp, N, s.cid.as<uint32_t>(),
-                                                      s.dirty.as<uint8_t>());
+  const uint64_t n0 = s.iem_n, tot = n0 + N;
+  grow_keep(c, s.iem_key, tot * 8 + 8, n0 * 8);
+  grow_keep(c, s.iem_w, tot * 4 + 4, n0 * 4);
+  grow_keep(c, s.iem_dead, tot + 1, n0);
+  SUMINC_HIP(hipMemsetAsync(s.iem_dead.as<uint8_t>() + n0, 0, N + 1, c->stream));
+  s.iem_key2.ensure(tot * 8 + 8);
+  s.iem_w2.ensure(tot * 4 + 4);
+  Counters* cnt = dev_counters(c);
+  unsigned long long* add_cnt = &cnt->aux[7];
+  unsigned long long* delta = &cnt->aux[8];
+  k_sk_batch_prep<<<grid_for(N), kBlock, 0, c->stream>>>(s.iem_key.as<unsigned long long>() + n0, N, add_cnt, delta);
+  k_sk_batch<<<grid_for(N), kBlock, 0, c->stream>>>(
+      rows, cols, act, ex, w, countp, N, c->wkind == SUMINC_W_U32, s.cid.as<uint32_t>(), s.dirty.as<uint8_t>(),
+      s.pr_valid.as<uint8_t>(), s.ss_valid.as<uint8_t>(), s.ext_out.as<uint32_t>(), s.ext_in.as<uint32_t>(),
+      s.iem_key.as<unsigned long long>(), n0, s.iem_dead.as<uint8_t>(), s.iem_key.as<unsigned long long>() + n0,
+      s.iem_w.as<uint32_t>() + n0, add_cnt, delta);
+  if (n0)
+    k_sk_drop_dead<<<grid_for(n0), kBlock, 0, c->stream>>>(n0, s.iem_key.as<unsigned long long>(),
+                                                          s.iem_dead.as<uint8_t>());
+  cub(c, [&](void* p, size_t& b) {
+    return rocprim::radix_sort_pairs(p, b, s.iem_key.as<uint64_t>(), s.iem_key2.as<uint64_t>(),
+                                     s.iem_w.as<uint32_t>(), s.iem_w2.as<uint32_t>(), (size_t)tot, 0, 64, c->stream);
+  });
+  s.iem_key.swap(s.iem_key2);
+  s.iem_w.swap(s.iem_w2);
   SUMINC_HIP(hipGetLastError());
+  s.pr_need = s.ss_need = true;
+}
+
+// After the batch's final synchronisation: the live length of the internal list.
+void sketch_batch_done(suminc_ctx* c) {
+  Sketch& s = c->sk;
+  if (!s.built) return;
+  s.iem_n = (uint64_t)((long long)s.iem_n + (long long)c->host_counters->aux[8]);
 }
 
 void sketch_refresh(suminc_ctx* c) {
   Sketch& s = c->sk;
   SUMINC_REQUIRE(s.built, SUMINC_E_STATE, "no sketch (call suminc_sum_build first)");
-  if (s.version != c->graph_version) sketch_build(c, 0, 0, 0.0);  // acceptance kept
+  if (s.version != c->graph_version) sketch_layout(c, true);  // acceptance and membership kept
 }
 
 // ---------------------------------------------------------------------------
@@ -568,23 +847,25 @@ __device__ __forceinline__ T block_sum(T v, T* s_red) {
 }
 
 // Jacobi rounds inside one cluster held in LDS: r += delta; delta <- sum over
-// internal in-edges of delta[src] * d / outdeg(src); stops when the mass
-// processed in a round is <= thr or after maxit rounds.  Returns the rounds run
-// and the internal edges read.
+// internal edges of delta[src] * d / outdeg(src), edge-parallel with LDS atomics
+// (a hub's in-edges spread over the workgroup); stops when the mass processed in
+// a round is <= thr or after maxit rounds.  Returns the rounds run.
 template <class T>
-__device__ uint32_t cluster_iterate(uint32_t size, uint32_t slot0, const uint32_t* __restrict__ ie_off,
-                                    const uint32_t* __restrict__ ie_src, const uint32_t* __restrict__ ie_deg, T d,
-                                    double thr, int maxit, T* s_r, T* s_d, T* s_n, T* s_red) {
+__device__ uint32_t cluster_iterate(uint32_t size, uint32_t pb, uint32_t pe, const Internal& ie, T d, double thr,
+                                    int maxit, T* s_r, T* s_d, T* s_n, T* s_red) {
   uint32_t rounds = 0;
   for (int it = 0; it < maxit; ++it) {
     T processed = T(0);
     for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
-      T acc = T(0);
-      for (uint32_t p = ie_off[slot0 + j]; p < ie_off[slot0 + j + 1]; ++p) acc += s_d[ie_src[p]] * d / (T)ie_deg[p];
-      s_n[j] = acc;
+      s_n[j] = T(0);
       processed += fabs_t(s_d[j]);
     }
-    T tot = block_sum(processed, s_red);
+    __syncthreads();
+    for (uint32_t p = pb + threadIdx.x; p < pe; p += kBlock) {
+      const T x = s_d[ie.src[p]];
+      if (x != T(0)) atomicAdd(&s_n[ie.dst[p]], x * d / (T)ie.deg[p]);
+    }
+    T tot = block_sum(processed, s_red);  // synchronises the workgroup
     for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
       s_r[j] += s_d[j];
       s_d[j] = s_n[j];
@@ -602,14 +883,13 @@ __global__ void __launch_bounds__(kBlock) k_pr_shortcuts(const uint32_t* __restr
                                                          const uint32_t* __restrict__ cl_ent,
                                                          const uint32_t* __restrict__ lidx,
                                                          const unsigned long long* __restrict__ sc_base,
-                                                         const uint32_t* __restrict__ ie_off,
-                                                         const uint32_t* __restrict__ ie_src,
-                                                         const uint32_t* __restrict__ ie_deg, T d, double thr_per_vertex,
+                                                         Internal ie, T d, double thr_per_vertex,
                                                          int maxit, T* __restrict__ R, T* __restrict__ D,
-                                                         unsigned long long* work) {
+                                                         const uint8_t* __restrict__ valid, unsigned long long* work) {
   __shared__ T s_r[kMaxCluster], s_d[kMaxCluster], s_n[kMaxCluster];
   __shared__ T s_red[kBlock / 64];
   const uint32_t k = blockIdx.x;
+  if (valid[k]) return;
   const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0;
   const double thr = thr_per_vertex * size;
   unsigned long long edges = 0;
@@ -620,8 +900,9 @@ __global__ void __launch_bounds__(kBlock) k_pr_shortcuts(const uint32_t* __restr
       s_d[j] = j == src ? T(1) : T(0);
     }
     __syncthreads();
-    uint32_t rounds = cluster_iterate<T>(size, slot0, ie_off, ie_src, ie_deg, d, thr, maxit, s_r, s_d, s_n, s_red);
-    edges += (unsigned long long)rounds * (ie_off[slot0 + size] - ie_off[slot0]);
+    const uint32_t pb = ie.off[slot0], pe = ie.off[slot0 + size];
+    uint32_t rounds = cluster_iterate<T>(size, pb, pe, ie, d, thr, maxit, s_r, s_d, s_n, s_red);
+    edges += (unsigned long long)rounds * (pe - pb);
     const unsigned long long base = sc_base[e];
     for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
       R[base + j] = s_r[j];
@@ -632,12 +913,18 @@ __global__ void __launch_bounds__(kBlock) k_pr_shortcuts(const uint32_t* __restr
   if (threadIdx.x == 0 && edges) atomicAdd(work, edges);
 }
 
+// Recomputes the PageRank rows of the clusters whose rows are not current (all of
+// them after the first build or a precision change, the touched ones after a batch).
 template <class T>
 static void compute_pr_rows(suminc_ctx* c) {
   Sketch& s = c->sk;
+  if (!s.pr_need && s.pr_rows_for == c->pr_prec) return;
   const size_t es = sizeof(T);
-  s.sc_R.ensure(s.sc_len * es + es);
-  s.sc_D.ensure(s.sc_len * es + es);
+  if (s.pr_rows_for != c->pr_prec) SUMINC_HIP(hipMemsetAsync(s.pr_valid.p, 0, (size_t)s.K + 1, c->stream));
+  if (s.pr_rows_for != c->pr_prec || s.sc_R.cap < s.sc_len * es + es) {
+    s.sc_R.ensure(s.sc_len * es + es);
+    s.sc_D.ensure(s.sc_len * es + es);
+  }
   DevBuf work;
   work.ensure(8);
   SUMINC_HIP(hipMemsetAsync(work.p, 0, 8, c->stream));
@@ -646,15 +933,16 @@ static void compute_pr_rows(suminc_ctx* c) {
   if (s.K)
     k_pr_shortcuts<T><<<s.K, kBlock, 0, c->stream>>>(
         s.cl_off.as<uint32_t>(), s.cl_eoff.as<uint32_t>(), s.cl_ent.as<uint32_t>(), s.lidx.as<uint32_t>(),
-        s.sc_base.as<unsigned long long>(), s.ie_off.as<uint32_t>(), s.ie_src.as<uint32_t>(), s.ie_w.as<uint32_t>(),
-        (T)c->prm.damping, thr_pv, maxit, s.sc_R.as<T>(), s.sc_D.as<T>(), work.as<unsigned long long>());
+        s.sc_base.as<unsigned long long>(), internal_of(s), (T)c->prm.damping, thr_pv, maxit, s.sc_R.as<T>(), s.sc_D.as<T>(), s.pr_valid.as<uint8_t>(),
+        work.as<unsigned long long>());
   SUMINC_HIP(hipGetLastError());
+  SUMINC_HIP(hipMemsetAsync(s.pr_valid.p, 1, (size_t)s.K + 1, c->stream));
   unsigned long long w = 0;
   SUMINC_HIP(hipMemcpyAsync(&w, work.p, 8, hipMemcpyDeviceToHost, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
   c->stats.reserved[4] = (double)w;  // internal edges read by the shortcut build
   s.pr_rows_for = c->pr_prec;
-  s.pr_rows_version = c->graph_version;
+  s.pr_need = false;
 }
 
 // ---------------------------------------------------------------------------
@@ -706,7 +994,6 @@ __device__ __forceinline__ void push_item(bool want, uint32_t v, T cval, uint32_
 struct SkArgs {
   uint32_t n;
   const uint32_t* off;     // full out-CSR row pointers (degrees)
-  const uint32_t* ib_off;  // sketch out-edge rows
   const uint8_t* ntype;
   const uint32_t* ent_of;
   uint32_t* wl_v;
@@ -782,7 +1069,7 @@ __global__ void __launch_bounds__(kBlock) k_sk_gather(SkArgs a, T* __restrict__ 
   if (v < a.n) {
     const uint8_t t = a.ntype[v];
     const uint32_t deg = a.off[v + 1] - a.off[v];
-    rowlen = a.ib_off[v + 1] - a.ib_off[v];
+    rowlen = deg;  // the push skips an exit's edges inside its cluster
     if (t == 0 && !flush) {  // SingleNode: plain push over its out-list
       T dv = delta[v];
       if ((double)fabs_t(dv) > eps) {
@@ -904,10 +1191,8 @@ __global__ void k_lists_clear(Counters* cnt) {
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_sk_precompute(SkArgs a, const uint32_t* __restrict__ cl_off,
                                                           const uint32_t* __restrict__ cl_mem,
-                                                          const uint32_t* __restrict__ cl_nexit,
-                                                          const uint32_t* __restrict__ ie_off,
-                                                          const uint32_t* __restrict__ ie_src,
-                                                          const uint32_t* __restrict__ ie_deg, uint8_t* dirty,
+                                                          const uint32_t* __restrict__ cl_nexit, Internal ie,
+                                                          uint8_t* dirty,
                                                           T* __restrict__ delta, T* __restrict__ rank,
                                                           float* __restrict__ comp, T* __restrict__ wl_c, T d,
                                                           double thr_per_vertex, int maxit,
@@ -923,8 +1208,8 @@ __global__ void __launch_bounds__(kBlock) k_sk_precompute(SkArgs a, const uint32
     s_d[j] = delta[v];
   }
   __syncthreads();
-  uint32_t rounds = cluster_iterate<T>(size, slot0, ie_off, ie_src, ie_deg, d, thr_per_vertex * size, maxit, s_r,
-                                       s_d, s_n, s_red);
+  const uint32_t pb = ie.off[slot0], pe = ie.off[slot0 + size];
+  uint32_t rounds = cluster_iterate<T>(size, pb, pe, ie, d, thr_per_vertex * size, maxit, s_r, s_d, s_n, s_red);
   for (uint32_t j0 = 0; j0 < size; j0 += kBlock) {  // uniform trip count: push_item uses wave ballots
     const uint32_t j = j0 + threadIdx.x;
     bool want = false;
@@ -936,15 +1221,15 @@ __global__ void __launch_bounds__(kBlock) k_sk_precompute(SkArgs a, const uint32
       if (s_r[j] != T(0)) kahan_add(rank, comp, v, s_r[j]);
       if (j < nexit && s_r[j] != T(0)) {
         const uint32_t deg = a.off[v + 1] - a.off[v];
-        rowlen = a.ib_off[v + 1] - a.ib_off[v];
-        if (deg && rowlen) { want = true; cval = d * s_r[j] / (T)deg; }
+        rowlen = deg;
+        if (deg) { want = true; cval = d * s_r[j] / (T)deg; }
       }
     }
     push_item<T>(want, v, cval, rowlen, a.wl_v, wl_c, a.heavy, a.cnt);
   }
   if (threadIdx.x == 0) {
     dirty[k] = 0;
-    atomicAdd(work, (unsigned long long)rounds * (ie_off[slot0 + size] - ie_off[slot0]));
+    atomicAdd(work, (unsigned long long)rounds * (pe - pb));
   }
 }
 
@@ -993,9 +1278,9 @@ __global__ void __launch_bounds__(kBlock) k_sk_correct(const uint32_t* __restric
 template <class T>
 static void sk_push_pass(suminc_ctx* c) {
   Sketch& s = c->sk;
-  k_pr_push<T><<<push_grid(), kBlock, 0, c->stream>>>(s.ib_off.as<uint32_t>(), s.ib_col.as<uint32_t>(),
-                                                      c->pr_delta.as<T>(), c->wl0.as<uint32_t>(), c->wl1.as<T>(),
-                                                      c->heavy.as<uint4>(), dev_counters(c));
+  k_pr_push<T, true><<<push_grid(), kBlock, 0, c->stream>>>(
+      c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(), c->pr_delta.as<T>(), c->wl0.as<uint32_t>(),
+      c->wl1.as<T>(), c->heavy.as<uint4>(), dev_counters(c), s.ntype.as<uint8_t>(), s.cid.as<uint32_t>());
 }
 
 template <class T>
@@ -1009,7 +1294,7 @@ static void sum_pr_run_t(suminc_ctx* c, suminc_stats* st) {
   const int maxit = c->prm.shortcut_max_rounds > 0 ? c->prm.shortcut_max_rounds : 100;
   const T d = (T)c->prm.damping;
   const size_t es = sizeof(T);
-  if (s.pr_rows_version != c->graph_version || s.pr_rows_for != c->pr_prec) compute_pr_rows<T>(c);
+  compute_pr_rows<T>(c);
   s.bound.ensure((size_t)n * es);
   s.spnode.ensure((size_t)n * es);
   SUMINC_HIP(hipMemsetAsync(s.bound.p, 0, (size_t)n * es, c->stream));
@@ -1019,7 +1304,6 @@ static void sum_pr_run_t(suminc_ctx* c, suminc_stats* st) {
   SkArgs a;
   a.n = n;
   a.off = c->out_off.as<uint32_t>();
-  a.ib_off = s.ib_off.as<uint32_t>();
   a.ntype = s.ntype.as<uint8_t>();
   a.ent_of = s.ent_of.as<uint32_t>();
   a.wl_v = c->wl0.as<uint32_t>();
@@ -1038,8 +1322,7 @@ static void sum_pr_run_t(suminc_ctx* c, suminc_stats* st) {
   k_lists_clear<<<1, 64, 0, c->stream>>>(cnt);
   if (s.K)
     k_sk_precompute<T><<<s.K, kBlock, 0, c->stream>>>(a, s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(),
-                                                      s.cl_nexit.as<uint32_t>(), s.ie_off.as<uint32_t>(),
-                                                      s.ie_src.as<uint32_t>(), s.ie_w.as<uint32_t>(),
+                                                      s.cl_nexit.as<uint32_t>(), internal_of(s),
                                                       s.dirty.as<uint8_t>(), c->pr_delta.as<T>(), c->pr_rank.as<T>(),
                                                       c->pr_comp.as<float>(), c->wl1.as<T>(), d, eps, maxit,
                                                       work.as<unsigned long long>());
@@ -1125,7 +1408,7 @@ void sketch_info(suminc_ctx* c, suminc_sketch_info* out) {
   out->entries = s.E;
   out->clustered_vertices = s.M;
   for (int t = 0; t < 5; ++t) out->node_type_count[t] = s.counts[t];
-  out->sketch_edges = s.ib_m;
+  out->sketch_edges = c->m - s.ie_m;  // Singles' CSR rows + the exits' external edges
   out->internal_edges = s.ie_m;
   out->shortcut_slots = s.sc_len;
   out->max_cluster = s.max_cluster;
@@ -1146,10 +1429,8 @@ void sketch_pr_shortcut(suminc_ctx* c, uint32_t v, uint32_t cap, uint32_t* membe
   sketch_refresh(c);
   Sketch& s = c->sk;
   SUMINC_REQUIRE(v < c->n, SUMINC_E_INVALID, "vertex out of range");
-  if (s.pr_rows_version != c->graph_version || s.pr_rows_for != c->pr_prec) {
-    if (c->pr_prec == SUMINC_PR_FP64) compute_pr_rows<double>(c);
-    else compute_pr_rows<float>(c);
-  }
+  if (c->pr_prec == SUMINC_PR_FP64) compute_pr_rows<double>(c);
+  else compute_pr_rows<float>(c);
   uint32_t e = INF32, k = INF32;
   SUMINC_HIP(hipMemcpy(&e, s.ent_of.as<uint32_t>() + v, 4, hipMemcpyDeviceToHost));
   SUMINC_REQUIRE(e != INF32, SUMINC_E_INVALID, "vertex is not an entry of a supernode");

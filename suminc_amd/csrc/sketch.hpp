@@ -4,10 +4,27 @@
 
 namespace suminc {
 
+// Internal edges of one cluster as the in-cluster kernels read them: the edges
+// with destination slot in [slot0, slot0 + size) occupy [off[slot0], off[slot0 + size]).
+struct Internal {
+  const uint32_t* off;  // per member slot (M + 1)
+  const uint32_t* src;  // local index of the source
+  const uint32_t* dst;  // local index of the destination
+  const uint32_t* deg;  // out-degree of the source
+  const uint32_t* wt;   // integer weight (1 without weights)
+};
+
+inline Internal internal_of(const Sketch& s) {
+  return Internal{s.ie_off.as<uint32_t>(), s.ie_src.as<uint32_t>(), s.ie_dst.as<uint32_t>(), s.ie_deg.as<uint32_t>(),
+                  s.ie_wt.as<uint32_t>()};
+}
+
 void cluster_leaves(suminc_ctx* c, uint32_t max_node, uint32_t min_node, uint32_t* out);
 void set_clusters(suminc_ctx* c, const uint32_t* cluster_of);
 void sketch_build(suminc_ctx* c, uint32_t min_node, uint32_t max_node, double threshold);
-void sketch_mark_dirty(suminc_ctx* c, const uint32_t* rows, const uint32_t* cols, const unsigned* countp, uint32_t N);
+void sketch_batch(suminc_ctx* c, const uint32_t* rows, const uint32_t* cols, const uint8_t* act, const uint8_t* ex,
+                  const uint32_t* w, const unsigned* countp, uint32_t N);
+void sketch_batch_done(suminc_ctx* c);
 void sketch_refresh(suminc_ctx* c);
 void sum_pr_run(suminc_ctx* c, suminc_stats* st);
 void sum_sssp_run(suminc_ctx* c, suminc_stats* st);

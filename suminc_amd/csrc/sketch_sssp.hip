@@ -26,28 +26,26 @@ namespace suminc {
 constexpr int kMaxClusterS = 2048;
 constexpr uint64_t kInf64 = ~0ull;
 
-// In-cluster Bellman-Ford rounds (pull over internal in-edges) on packed
-// {dist, parent} words in LDS until no word changes.  Returns rounds run.
-__device__ uint32_t cluster_bf(uint32_t size, uint32_t slot0, const uint32_t* __restrict__ ie_off,
-                               const uint32_t* __restrict__ ie_src, const uint32_t* __restrict__ ie_wt,
-                               const uint32_t* __restrict__ mem_global, unsigned long long* s_dp, int* s_flag) {
+// In-cluster Bellman-Ford rounds on packed {dist, parent} words in LDS,
+// edge-parallel over the cluster's internal edges with LDS atomic min, until a
+// round changes nothing.  s_id holds the members' vertex ids.  Returns rounds run.
+__device__ uint32_t cluster_bf(uint32_t size, uint32_t pb, uint32_t pe, const uint32_t* __restrict__ src,
+                               const uint32_t* __restrict__ dst, const uint32_t* __restrict__ wt,
+                               const uint32_t* s_id, unsigned long long* s_dp, int* s_flag) {
   uint32_t rounds = 0;
   while (true) {
     if (threadIdx.x == 0) *s_flag = 0;
     __syncthreads();
-    for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
-      unsigned long long best = s_dp[j];
-      for (uint32_t p = ie_off[slot0 + j]; p < ie_off[slot0 + j + 1]; ++p) {
-        const uint32_t i = ie_src[p];
-        const unsigned long long di = s_dp[i] >> 32;
-        if (di == 0xFFFFFFFFull) continue;
-        const unsigned long long nd = di + ie_wt[p];
-        if (nd >= 0xFFFFFFFFull) continue;
-        const unsigned long long cand = (nd << 32) | mem_global[i];
-        if (cand < best) best = cand;
-      }
-      if (best < s_dp[j]) {
-        atomicMin(&s_dp[j], best);
+    for (uint32_t p = pb + threadIdx.x; p < pe; p += kBlock) {
+      const uint32_t i = src[p];
+      const unsigned long long di = s_dp[i] >> 32;
+      if (di == 0xFFFFFFFFull) continue;
+      const unsigned long long nd = di + wt[p];
+      if (nd >= 0xFFFFFFFFull) continue;
+      const unsigned long long cand = (nd << 32) | s_id[i];
+      const uint32_t j = dst[p];
+      if (cand < s_dp[j]) {
+        atomicMin(&s_dp[j], cand);
         *s_flag = 1;
       }
     }
@@ -68,21 +66,24 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts(const uint32_t* __restr
                                                          const uint32_t* __restrict__ cl_ent,
                                                          const uint32_t* __restrict__ lidx,
                                                          const unsigned long long* __restrict__ sc_base,
-                                                         const uint32_t* __restrict__ ie_off,
-                                                         const uint32_t* __restrict__ ie_src,
-                                                         const uint32_t* __restrict__ ie_wt, uint32_t* __restrict__ sd,
-                                                         uint32_t* __restrict__ sp, unsigned long long* work) {
+                                                         Internal ie, uint32_t* __restrict__ sd,
+                                                         uint32_t* __restrict__ sp, const uint8_t* __restrict__ valid,
+                                                         unsigned long long* work) {
   __shared__ unsigned long long s_dp[kMaxClusterS];
+  __shared__ uint32_t s_id[kMaxClusterS];
   __shared__ int s_flag;
   const uint32_t k = blockIdx.x;
+  if (valid[k]) return;
   const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0;
+  const uint32_t pb = ie.off[slot0], pe = ie.off[slot0 + size];
+  for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_id[j] = cl_mem[slot0 + j];
   unsigned long long edges = 0;
   for (uint32_t e = cl_eoff[k]; e < cl_eoff[k + 1]; ++e) {
     const uint32_t src = lidx[cl_ent[e]];
     for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_dp[j] = j == src ? 0xFFFFFFFFull : kInf64;
     __syncthreads();
-    uint32_t rounds = cluster_bf(size, slot0, ie_off, ie_src, ie_wt, cl_mem + slot0, s_dp, &s_flag);
-    edges += (unsigned long long)rounds * (ie_off[slot0 + size] - ie_off[slot0]);
+    uint32_t rounds = cluster_bf(size, pb, pe, ie.src, ie.dst, ie.wt, s_id, s_dp, &s_flag);
+    edges += (unsigned long long)rounds * (pe - pb);
     const unsigned long long base = sc_base[e];
     for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
       const unsigned long long x = s_dp[j];
@@ -95,9 +96,9 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts(const uint32_t* __restr
 }
 
 struct SsArgs {
-  const uint32_t* ib_off;
-  const uint32_t* ib_col;
-  const uint32_t* ib_w;  // null: unit weights
+  const uint32_t* off;   // CSR (an exit's sketch row skips its edges inside the cluster)
+  const uint32_t* col;
+  const uint32_t* w;     // null: unit weights
   const uint8_t* ntype;
   const uint32_t* cid;
   const uint32_t* ent_of;
@@ -139,18 +140,21 @@ __global__ void k_ss_mark(SsArgs a, uint32_t src, int mark_src) {
 
 // Precompute of dirty clusters: in-cluster relaxation from the members' current
 // state; improved exits join the frontier (list 0).
-__global__ void __launch_bounds__(kBlock) k_ss_precompute(SsArgs a, const uint32_t* __restrict__ ie_off,
-                                                          const uint32_t* __restrict__ ie_src,
-                                                          const uint32_t* __restrict__ ie_wt,
-                                                          unsigned long long* work) {
+__global__ void __launch_bounds__(kBlock) k_ss_precompute(SsArgs a, Internal ie, unsigned long long* work) {
   __shared__ unsigned long long s_dp[kMaxClusterS];
+  __shared__ uint32_t s_id[kMaxClusterS];
   __shared__ int s_flag;
   const uint32_t k = blockIdx.x;
   if (!a.dirty[k]) return;
   const uint32_t slot0 = a.cl_off[k], size = a.cl_off[k + 1] - slot0, nexit = a.cl_nexit[k];
-  for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_dp[j] = a.dp[a.cl_mem[slot0 + j]];
+  const uint32_t pb = ie.off[slot0], pe = ie.off[slot0 + size];
+  for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+    const uint32_t v = a.cl_mem[slot0 + j];
+    s_id[j] = v;
+    s_dp[j] = a.dp[v];
+  }
   __syncthreads();
-  uint32_t rounds = cluster_bf(size, slot0, ie_off, ie_src, ie_wt, a.cl_mem + slot0, s_dp, &s_flag);
+  uint32_t rounds = cluster_bf(size, pb, pe, ie.src, ie.dst, ie.wt, s_id, s_dp, &s_flag);
   for (uint32_t j0 = 0; j0 < size; j0 += kBlock) {  // uniform trip count for the wave ballots
     const uint32_t j = j0 + threadIdx.x;
     bool improved = false;
@@ -165,10 +169,7 @@ __global__ void __launch_bounds__(kBlock) k_ss_precompute(SsArgs a, const uint32
     }
     ss_append(a, 0, v, improved);
   }
-  if (threadIdx.x == 0) {
-    a.dirty[k] = 0;
-    atomicAdd(work, (unsigned long long)rounds * (ie_off[slot0 + size] - ie_off[slot0]));
-  }
+  if (threadIdx.x == 0) atomicAdd(work, (unsigned long long)rounds * (pe - pb));
 }
 
 __device__ __forceinline__ void ss_relax(const SsArgs& a, int nxt, uint32_t v, unsigned long long nd, uint32_t parent) {
@@ -181,20 +182,20 @@ __device__ __forceinline__ void ss_relax(const SsArgs& a, int nxt, uint32_t v, u
   ss_append(a, nxt, v, add);
 }
 
-// One sketch round over the current frontier: Singles relax their out-edges,
-// exits their external edges (both: the ib row), entries their shortcut row.
+// One sketch round over the current frontier: Singles relax their CSR row,
+// exits their external edges, entries their shortcut row to the exits.
 __global__ void __launch_bounds__(kBlock) k_ss_push(SsArgs a) {
   __shared__ uint32_t s_scan[kBlock + 1];
   __shared__ uint32_t s_start[kBlock];
   __shared__ uint32_t s_warp[kBlock / 64 + 1];
-  __shared__ uint32_t s_u[kBlock], s_du[kBlock], s_iblen[kBlock], s_mbase[kBlock];
+  __shared__ uint32_t s_u[kBlock], s_du[kBlock], s_iblen[kBlock], s_mbase[kBlock], s_rb[kBlock], s_k[kBlock];
   __shared__ unsigned long long s_rbase[kBlock];
-  __shared__ unsigned long long s_edges;
+  __shared__ unsigned long long s_edges, s_sc;
   if (a.cnt->done) return;
   const int cur = (int)(a.cnt->aux[1] & 1), nxt = cur ^ 1;
   const unsigned long long nl = a.cnt->wl_count[cur];
   const unsigned long long tiles = (nl + kBlock - 1) / kBlock;
-  if (threadIdx.x == 0) s_edges = 0;
+  if (threadIdx.x == 0) s_edges = s_sc = 0;
   __syncthreads();
   for (unsigned long long t = blockIdx.x; t < tiles; t += gridDim.x) {
     const unsigned long long i = t * kBlock + threadIdx.x;
@@ -205,10 +206,15 @@ __global__ void __launch_bounds__(kBlock) k_ss_push(SsArgs a) {
       const uint32_t du = (uint32_t)(a.dp[u] >> 32);
       const uint8_t ty = a.ntype[u];
       uint32_t iblen = 0, sclen = 0;
+      s_k[threadIdx.x] = ty == 0 ? INF32 : a.cid[u];
       if (du != INF32) {
-        if (ty == 0 || ty == 1 || ty == 3) iblen = a.ib_off[u + 1] - a.ib_off[u];
+        if (ty == 0 || ty == 1 || ty == 3) {
+          s_rb[threadIdx.x] = a.off[u];
+          iblen = a.off[u + 1] - a.off[u];
+        }
         if (ty == 2 || ty == 3) {
           const uint32_t k = a.cid[u];
+          a.dirty[k] = 1;  // the correction revisits this cluster
           sclen = a.cl_nexit[k];
           s_mbase[threadIdx.x] = a.cl_off[k];
           s_rbase[threadIdx.x] = a.sc_base[a.ent_of[u]];
@@ -226,13 +232,16 @@ __global__ void __launch_bounds__(kBlock) k_ss_push(SsArgs a) {
       s_iblen[threadIdx.x] = iblen;
       len = iblen + sclen;
       if (len) atomicAdd(&s_edges, (unsigned long long)len);
+      if (sclen) atomicAdd(&s_sc, (unsigned long long)sclen);
     }
     block_adjacency<kBlock>(0, len, s_scan, s_start, s_warp, [&](int slot, uint32_t k, uint32_t) {
       const uint32_t u = s_u[slot];
       const unsigned long long du = s_du[slot];
       if (k < s_iblen[slot]) {
-        const uint32_t e = a.ib_off[u] + k;
-        ss_relax(a, nxt, a.ib_col[e], du + (a.ib_w ? a.ib_w[e] : 1u), u);
+        const uint32_t e = s_rb[slot] + k;
+        const uint32_t x = a.col[e];
+        const bool inside = s_k[slot] != INF32 && a.cid[x] == s_k[slot];  // an exit's edge inside its cluster
+        ss_relax(a, nxt, x, inside ? 0xFFFFFFFFull : du + (a.w ? a.w[e] : 1u), u);
       } else {
         const uint32_t j = k - s_iblen[slot];
         const unsigned long long r = s_rbase[slot] + j;
@@ -243,6 +252,7 @@ __global__ void __launch_bounds__(kBlock) k_ss_push(SsArgs a) {
   }
   __syncthreads();
   if (threadIdx.x == 0 && s_edges) atomicAdd(&a.cnt->edges, s_edges);
+  if (threadIdx.x == 0 && s_sc) atomicAdd(&a.cnt->aux[7], s_sc);
 }
 
 __global__ void __launch_bounds__(kBlock) k_ss_heavy(SsArgs a) {
@@ -253,12 +263,18 @@ __global__ void __launch_bounds__(kBlock) k_ss_heavy(SsArgs a) {
     const uint2 hv = a.heavy[h];
     const uint32_t u = hv.x;
     const unsigned long long du = a.dp[u] >> 32;
-    const uint32_t b = a.ib_off[u] + hv.y * kChunk;
-    const uint32_t e = min(a.ib_off[u + 1], b + kChunk);
+    const uint32_t kc = a.ntype[u] == 0 ? INF32 : a.cid[u];
+    const uint32_t b = a.off[u] + hv.y * kChunk;
+    const uint32_t e = min(a.off[u + 1], b + kChunk);
     for (uint32_t k0 = b; k0 < e; k0 += kBlock) {
       const uint32_t k = k0 + threadIdx.x;
-      if (k < e) ss_relax(a, nxt, a.ib_col[k], du + (a.ib_w ? a.ib_w[k] : 1u), u);
-      else ss_append(a, nxt, 0, false);
+      if (k < e) {
+        const uint32_t x = a.col[k];
+        const bool inside = kc != INF32 && a.cid[x] == kc;
+        ss_relax(a, nxt, x, inside ? 0xFFFFFFFFull : du + (a.w ? a.w[k] : 1u), u);
+      } else {
+        ss_append(a, nxt, 0, false);
+      }
     }
   }
 }
@@ -279,14 +295,17 @@ __global__ void k_ss_end(Counters* cnt, int max_rounds) {
 // member min-reduces (dist(src) + w, src) over its internal in-edges, which fixes
 // equal-distance parents to the smallest id.
 __global__ void __launch_bounds__(kBlock) k_ss_correct(SsArgs a, const uint32_t* __restrict__ cl_eoff,
-                                                       const uint32_t* __restrict__ cl_ent,
-                                                       const uint32_t* __restrict__ ie_off,
-                                                       const uint32_t* __restrict__ ie_src,
-                                                       const uint32_t* __restrict__ ie_wt) {
-  __shared__ unsigned long long s_dp[kMaxClusterS];
+                                                       const uint32_t* __restrict__ cl_ent, Internal ie) {
+  __shared__ unsigned long long s_dp[kMaxClusterS], s_best[kMaxClusterS];
+  __shared__ uint32_t s_id[kMaxClusterS];
   const uint32_t k = blockIdx.x;
+  if (!a.dirty[k]) return;  // no member or entry of this cluster moved in this run
   const uint32_t slot0 = a.cl_off[k], size = a.cl_off[k + 1] - slot0;
-  for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_dp[j] = a.dp[a.cl_mem[slot0 + j]];
+  for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+    const uint32_t v = a.cl_mem[slot0 + j];
+    s_id[j] = v;
+    s_dp[j] = a.dp[v];
+  }
   __syncthreads();
   for (uint32_t e = cl_eoff[k]; e < cl_eoff[k + 1]; ++e) {
     const unsigned long long ds = a.dp[cl_ent[e]] >> 32;
@@ -302,28 +321,33 @@ __global__ void __launch_bounds__(kBlock) k_ss_correct(SsArgs a, const uint32_t*
     }
   }
   __syncthreads();
-  for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
-    unsigned long long best = s_dp[j];
-    for (uint32_t p = ie_off[slot0 + j]; p < ie_off[slot0 + j + 1]; ++p) {
-      const uint32_t i = ie_src[p];
-      const unsigned long long di = s_dp[i] >> 32;
-      if (di == 0xFFFFFFFFull) continue;
-      const unsigned long long nd = di + ie_wt[p];
-      if (nd >= 0xFFFFFFFFull) continue;
-      const unsigned long long cand = (nd << 32) | a.cl_mem[slot0 + i];
-      if (cand < best) best = cand;
-    }
-    const uint32_t v = a.cl_mem[slot0 + j];
-    if (best < a.dp[v]) a.dp[v] = best;
+  for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_best[j] = s_dp[j];
+  __syncthreads();
+  const uint32_t pb = ie.off[slot0], pe = ie.off[slot0 + size];
+  for (uint32_t p = pb + threadIdx.x; p < pe; p += kBlock) {
+    const uint32_t i = ie.src[p];
+    const unsigned long long di = s_dp[i] >> 32;
+    if (di == 0xFFFFFFFFull) continue;
+    const unsigned long long nd = di + ie.wt[p];
+    if (nd >= 0xFFFFFFFFull) continue;
+    const unsigned long long cand = (nd << 32) | s_id[i];
+    const uint32_t j = ie.dst[p];
+    if (cand < s_best[j]) atomicMin(&s_best[j], cand);
   }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+    const uint32_t v = s_id[j];
+    if (s_best[j] < a.dp[v]) a.dp[v] = s_best[j];
+  }
+  if (threadIdx.x == 0) a.dirty[k] = 0;
 }
 
 static SsArgs ss_args(suminc_ctx* c) {
   Sketch& s = c->sk;
   SsArgs a;
-  a.ib_off = s.ib_off.as<uint32_t>();
-  a.ib_col = s.ib_col.as<uint32_t>();
-  a.ib_w = c->wkind == SUMINC_W_U32 ? s.ib_w.as<uint32_t>() : nullptr;
+  a.off = c->out_off.as<uint32_t>();
+  a.col = c->out_col.as<uint32_t>();
+  a.w = c->wkind == SUMINC_W_U32 ? c->out_w.as<uint32_t>() : nullptr;
   a.ntype = s.ntype.as<uint8_t>();
   a.cid = s.cid.as<uint32_t>();
   a.ent_of = s.ent_of.as<uint32_t>();
@@ -344,23 +368,29 @@ static SsArgs ss_args(suminc_ctx* c) {
   return a;
 }
 
+// Recomputes the SSSP rows of the clusters whose rows are not current.
 static void compute_sssp_rows(suminc_ctx* c) {
   Sketch& s = c->sk;
-  s.ss_dist.ensure(s.sc_len * 4 + 4);
-  s.ss_par.ensure(s.sc_len * 4 + 4);
+  if (!s.ss_need && s.ss_rows) return;
+  if (!s.ss_rows) SUMINC_HIP(hipMemsetAsync(s.ss_valid.p, 0, (size_t)s.K + 1, c->stream));
+  if (!s.ss_rows || s.ss_dist.cap < s.sc_len * 4 + 4) {
+    s.ss_dist.ensure(s.sc_len * 4 + 4);
+    s.ss_par.ensure(s.sc_len * 4 + 4);
+  }
   s.ss_work.ensure(8);
   SUMINC_HIP(hipMemsetAsync(s.ss_work.p, 0, 8, c->stream));
-  const uint32_t* wt = s.ie_w.as<uint32_t>() + s.ie_m + 1;  // internal edge weights follow the degrees
   if (s.K)
     k_ss_shortcuts<<<s.K, kBlock, 0, c->stream>>>(s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(),
                                                   s.cl_eoff.as<uint32_t>(), s.cl_ent.as<uint32_t>(),
                                                   s.lidx.as<uint32_t>(), s.sc_base.as<unsigned long long>(),
-                                                  s.ie_off.as<uint32_t>(), s.ie_src.as<uint32_t>(), wt,
+                                                  internal_of(s),
                                                   s.ss_dist.as<uint32_t>(), s.ss_par.as<uint32_t>(),
-                                                  s.ss_work.as<unsigned long long>());
+                                                  s.ss_valid.as<uint8_t>(), s.ss_work.as<unsigned long long>());
   SUMINC_HIP(hipGetLastError());
+  SUMINC_HIP(hipMemsetAsync(s.ss_valid.p, 1, (size_t)s.K + 1, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
-  s.sssp_rows_version = c->graph_version;
+  s.ss_rows = true;
+  s.ss_need = false;
 }
 
 __global__ void k_ss_begin(Counters* cnt) {
@@ -369,6 +399,7 @@ __global__ void k_ss_begin(Counters* cnt) {
   cnt->heavy_count = 0;
   cnt->aux[1] = 0;
   cnt->aux[6] = 0;
+  cnt->aux[7] = 0;
   cnt->done = 0;
   cnt->rounds = 0;
   cnt->edges = 0;
@@ -378,14 +409,13 @@ void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
   SUMINC_REQUIRE(c->sssp_ready, SUMINC_E_STATE, "SSSP state not initialised");
   sketch_refresh(c);
   Sketch& s = c->sk;
-  if (s.sssp_rows_version != c->graph_version) compute_sssp_rows(c);
+  compute_sssp_rows(c);
   s.ss_dirty.ensure((size_t)s.K + 1);
   const int max_rounds = c->prm.max_rounds > 0 ? c->prm.max_rounds : 1000000;
   const bool first = c->sssp_pending;
   c->sssp_pending = false;
   SsArgs a = ss_args(c);
   Counters* cnt = a.cnt;
-  const uint32_t* wt = s.ie_w.as<uint32_t>() + s.ie_m + 1;
   s.ss_work.ensure(8);
   SUMINC_HIP(hipMemsetAsync(s.ss_work.p, 0, 8, c->stream));
   SUMINC_HIP(hipMemsetAsync(s.ss_dirty.p, 0, (size_t)s.K + 1, c->stream));
@@ -394,14 +424,17 @@ void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
   k_ss_begin<<<1, 64, 0, c->stream>>>(cnt);
   k_ss_mark<<<256, kBlock, 0, c->stream>>>(a, c->sssp_src, first ? 1 : 0);
   if (s.K)
-    k_ss_precompute<<<s.K, kBlock, 0, c->stream>>>(a, s.ie_off.as<uint32_t>(), s.ie_src.as<uint32_t>(), wt,
+    k_ss_precompute<<<s.K, kBlock, 0, c->stream>>>(a, internal_of(s),
                                                    s.ss_work.as<unsigned long long>());
   SUMINC_HIP(hipGetLastError());
   int group = 8;
+  size_t slot = 0;
   while (true) {
-    for (int i = 0; i < group; ++i) {
+    for (int i = 0; i < group; ++i, ++slot) {
+      if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 2 * slot), c->stream));
       k_ss_push<<<push_grid(), kBlock, 0, c->stream>>>(a);
       k_ss_heavy<<<push_grid(), kBlock, 0, c->stream>>>(a);
+      if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 2 * slot + 1), c->stream));
       k_ss_end<<<1, 64, 0, c->stream>>>(cnt, max_rounds);
     }
     SUMINC_HIP(hipGetLastError());
@@ -414,9 +447,20 @@ void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
   SUMINC_REQUIRE(h->wl_count[cur] == 0 || h->rounds < (unsigned long long)max_rounds, SUMINC_E_NOT_CONVERGED,
                  "round cap reached with a non-empty frontier");
   const double rounds = (double)h->rounds, edges = (double)h->edges;
+  if (c->prof_on) {
+    prof_collect(c, SUMINC_KERNEL_SSSP_SKETCH, (size_t)h->rounds, 0, 2);
+    suminc_kernel_profile& p = c->prof[SUMINC_KERNEL_SSSP_SKETCH];
+    // algorithmic bytes: a sketch edge moves column + weight (8 B) and a 32-bit
+    // distance read-modify-write (8 B); a shortcut slot moves distance + parent +
+    // member id (12 B) and the read-modify-write (8 B); a frontier vertex 16 B
+    const double sc = (double)h->aux[7];
+    p.edges += edges;
+    p.vertices += (double)h->aux[6];
+    p.alg_bytes += 16.0 * (edges - sc) + 20.0 * sc + 16.0 * (double)h->aux[6];
+  }
   if (s.K)
     k_ss_correct<<<s.K, kBlock, 0, c->stream>>>(a, s.cl_eoff.as<uint32_t>(), s.cl_ent.as<uint32_t>(),
-                                                s.ie_off.as<uint32_t>(), s.ie_src.as<uint32_t>(), wt);
+                                                internal_of(s));
   SUMINC_HIP(hipGetLastError());
   SUMINC_HIP(hipEventRecord(c->ev_stop, c->stream));
   SUMINC_HIP(hipEventSynchronize(c->ev_stop));
@@ -448,7 +492,7 @@ void sketch_sssp_shortcut(suminc_ctx* c, uint32_t v, uint32_t cap, uint32_t* mem
   sketch_refresh(c);
   Sketch& s = c->sk;
   SUMINC_REQUIRE(v < c->n, SUMINC_E_INVALID, "vertex out of range");
-  if (s.sssp_rows_version != c->graph_version) compute_sssp_rows(c);
+  compute_sssp_rows(c);
   uint32_t e = INF32, k = INF32;
   SUMINC_HIP(hipMemcpy(&e, s.ent_of.as<uint32_t>() + v, 4, hipMemcpyDeviceToHost));
   SUMINC_REQUIRE(e != INF32, SUMINC_E_INVALID, "vertex is not an entry of a supernode");

@@ -116,3 +116,46 @@ def test_apply_batch_rejects_bad_ops(gpu):
         # the graph is unchanged after a rejected batch
         o2, c2, _ = ctx.download_csr()
         assert np.array_equal(o2, off) and np.array_equal(c2, col)
+
+
+@pytest.mark.parametrize("weights", [None, "u32"])
+def test_apply_batch_device_equals_host_path(gpu, weights):
+    """Device-resident ops give the same graph and the same SSSP repair as host ops."""
+    g = oracle.Graph.rmat(12, 16, seed=4, weights=weights)
+    off, col, w = g.csr()
+    with gpu.Context() as a, gpu.Context() as b:
+        for ctx in (a, b):
+            ctx.upload_csr(off, col, w)
+            ctx.sssp_init(0)
+            ctx.sssp_run()
+        for i in range(3):
+            ops = g.gen_batch(seed=4, batch_id=i, n_del=500, n_ins=500)
+            a.apply_batch(ops)
+            a.sssp_run()
+            db = b.upload_batch(ops)
+            b.apply_batch_device(db)
+            b.sssp_run()
+            db.free()
+            for x, y in zip(a.download_csr(), b.download_csr()):
+                assert (x is None and y is None) or np.array_equal(x, y)
+            for x, y in zip(a.sssp_download(), b.sssp_download()):
+                assert np.array_equal(x, y)
+
+
+def test_apply_batch_device_rejects_bad_ops(gpu):
+    g = oracle.Graph.rmat(8, 8, seed=1, weights="u32")
+    off, col, w = g.csr()
+    with gpu.Context() as ctx:
+        ctx.upload_csr(off, col, w)
+        ops = np.zeros(4, gpu.UPDATE_DTYPE)
+        ops[0] = (0, 1, 2, 1, 0, 0)
+        ops[1] = (0, 1, 3, 1, 0, 0)
+        ops[2] = (0, 1, g.n + 3, 1, 0, 0)   # id out of range
+        ops[3] = (0, 1, 4, 0, 0, 0)         # zero integer weight
+        db = ctx.upload_batch(ops)
+        with pytest.raises(gpu.SumincError) as e:
+            ctx.apply_batch_device(db)
+        assert "2 invalid ops" in str(e.value) and "line 2" in str(e.value)
+        db.free()
+        o2, c2, _ = ctx.download_csr()
+        assert np.array_equal(o2, off) and np.array_equal(c2, col)

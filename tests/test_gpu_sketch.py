@@ -256,3 +256,40 @@ def test_zero_integer_weight_rejected(gpu):
         ops[0] = (gpu.OP_ADD, 1, 0, 0, 0.0, 0)
         with pytest.raises(gpu.SumincError):
             ctx.apply_batch(ops)
+
+
+def test_rows_follow_small_batches(gpu):
+    """Small batches leave most clusters untouched: their shortcut rows are moved
+    to the new layout, the touched ones are recomputed.  Both kinds must equal the
+    oracle's rows on the new graph, and both queries stay exact."""
+    g = _sssp_graph("u32", seed=8, scale=12)
+    T = 1e-10 * g.n
+    with gpu.Context(pr_precision=gpu.PR_FP64, termcheck_threshold=T) as ctx:
+        _upload(ctx, g)
+        ctx.set_clusters(blocks(g.n, 4096, 8))
+        ctx.sum_build(2, 1024, -1e9)
+        ctx.sssp_init(0)
+        ctx.sum_sssp_run()
+        ctx.pr_init()
+        ctx.sum_pr_run()
+        for b in range(3):
+            ops = g.gen_batch(seed=8, batch_id=b, n_del=10, n_ins=10)
+            ctx.apply_batch(ops)
+            g, _ = g.apply_batch(ops)
+            ctx.sum_sssp_run()
+            dist, par = ctx.sssp_download()
+            od, op = g.dijkstra(0)
+            assert np.array_equal(dist, od) and np.array_equal(par, op), b
+            st = ctx.sum_pr_run()
+            r, _ = ctx.pr_download()
+            rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
+            assert np.abs(r - rs).sum() <= st["residual"] / (1 - DAMPING) + 1e-11, b
+            cid, typ = ctx.sum_download()
+            entries = np.nonzero((typ == 2) | (typ == 3))[0]
+            for s in entries[::max(1, len(entries) // 40)]:
+                mem, sd, _, _ = ctx.sum_sssp_shortcut(int(s))
+                assert np.array_equal(sd, g.sum_sssp_shortcut(cid, int(s))[mem]), (b, int(s))
+                mem, reach, _, _ = ctx.sum_pr_shortcut(int(s))
+                omem, x = shortcut_dense(g, cid, int(s), DAMPING)
+                order = np.argsort(mem)
+                assert np.abs(reach[order] - x).max() < 1e-6, (b, int(s))

@@ -265,3 +265,29 @@ def test_dijkstra_and_ingress_vs_scipy(scale, src):
     di, pi, st = g.sssp_ingress(src)
     assert np.array_equal(di, dist) and np.array_equal(pi, par)
     assert st["edges"] > 0
+
+
+@pytest.mark.parametrize("weights", [None, "u32"])
+def test_sssp_incremental_step_equals_dijkstra(weights):
+    """The restated deltaCompute + resend + Ingress step reaches the new graph's
+    Dijkstra result (distances and smallest-id parents) batch after batch,
+    including re-weighted parent edges."""
+    g = oracle.Graph.rmat(12, 16, seed=6, weights=weights, wmin=1, wmax=64)
+    src = 0
+    dist, par = g.dijkstra(src)
+    for b in range(4):
+        ops = g.gen_batch(seed=6, batch_id=b, n_del=200, n_ins=200)
+        if weights and b == 1:
+            # re-weight ten parent edges upward: their children must be reset
+            off, col, w = g.csr()
+            kids = [v for v in range(g.n) if dist[v] not in (0, 0xFFFFFFFF)][:10]
+            extra = np.zeros(len(kids), ops.dtype)
+            for i, v in enumerate(kids):
+                extra[i] = (0, par[v], v, 200, 0.0, 0)
+            ops = np.concatenate([ops, extra])
+        g2, _ = g.apply_batch(ops)
+        d2, p2, st = g2.sssp_inc(g, src, dist, par)
+        od, op = g2.dijkstra(src)
+        assert np.array_equal(d2, od), b
+        assert np.array_equal(p2, op), b
+        g, dist, par = g2, d2, p2
