@@ -237,6 +237,25 @@ int suminc_sum_sssp_shortcut(suminc_ctx* ctx, uint32_t entry, uint32_t cap, uint
  * and its correction (:1322-1406). */
 int suminc_sum_sssp_run(suminc_ctx* ctx, suminc_stats* stats);
 
+/* ---- weighted personalized PageRank and its adjoint ------------------------------
+ * No counterpart in the reference (its nearest app is the weighted personalized
+ * PHP Ingress, php/php_ingress.h:15-120; nothing there has gradients).  Needs a
+ * graph with SUMINC_W_F32 weights.  Forward: r = (1-d) p + d W^T r with
+ * W[u,v] = w_uv / sum_k w_uk (dangling rows zero), as the delta series
+ * delta_0 = (1-d) p, delta_{t+1} = d W^T delta_t, stopped when the FP64 sum|delta| <
+ * termcheck_threshold.  Backward for L = sum_i c_i r_i: the same series on the
+ * transpose graph from mu_0 = c, stopped when sum|mu| < termcheck_threshold * sum|c|,
+ * gives lambda; dL/dp = (1-d) lambda, dL/dw_uv = (d r_u / s_u)(lambda_v - sum_k W[u,k]
+ * lambda_k).  State in FP32 with Kahan sums or FP64 (pr_precision).  Vectors are n
+ * floats, edge weights and their gradient m floats in out-CSR order; every pointer
+ * may be host or device memory of the context's GPU.  All four synchronise. */
+int suminc_wpr_set_weights(suminc_ctx* ctx, const float* weights);
+int suminc_wpr_forward(suminc_ctx* ctx, const float* personalization, float* rank, suminc_stats* stats);
+int suminc_wpr_backward(suminc_ctx* ctx, const float* grad_rank, float* grad_personalization, float* grad_weights,
+                        suminc_stats* stats);
+/* FP64 copies (host memory) of the last forward's r and the last backward's lambda. */
+int suminc_wpr_download(suminc_ctx* ctx, double* rank, double* adjoint);
+
 /* ---- stats ---------------------------------------------------------------------*/
 int suminc_get_stats(suminc_ctx* ctx, suminc_stats* stats);
 

@@ -58,6 +58,8 @@ def lib():
             "orc_sum_pr": (None, [vp, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int, vp, vp, vp, vp]),
             "orc_sum_sssp_shortcut": (None, [vp, vp, C.c_uint32, vp]),
             "orc_sum_sssp": (None, [vp, vp, vp, C.c_uint32, vp, vp, vp]),
+            "orc_wpr_forward": (C.c_int, [vp, vp, vp, C.c_double, C.c_double, C.c_int, vp]),
+            "orc_wpr_adjoint": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_double, C.c_int, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -237,3 +239,21 @@ class Graph:
         st = np.zeros(2)
         lib().orc_sum_sssp(self.h, _p(cid), _p(typ), src, _p(dist), _p(par), _p(st))
         return dist, par, {"rounds": st[0], "edges": st[1]}
+
+    # ---- weighted personalized PageRank + adjoint (FP64) -----------------------
+    def wpr_forward(self, w, p, d, tol=1e-15, max_rounds=100000):
+        w = np.ascontiguousarray(w, np.float32)
+        p = np.ascontiguousarray(p, np.float64)
+        r = np.zeros(self.n)
+        rounds = lib().orc_wpr_forward(self.h, _p(w), _p(p), d, tol, max_rounds, _p(r))
+        return r, rounds
+
+    def wpr_adjoint(self, w, r, c, d, tol=1e-15, max_rounds=100000):
+        w = np.ascontiguousarray(w, np.float32)
+        r = np.ascontiguousarray(r, np.float64)
+        c = np.ascontiguousarray(c, np.float64)
+        lam = np.zeros(self.n)
+        gp = np.zeros(self.n)
+        gw = np.zeros(len(w))
+        lib().orc_wpr_adjoint(self.h, _p(w), _p(r), _p(c), d, tol, max_rounds, _p(lam), _p(gp), _p(gw))
+        return lam, gp, gw

@@ -154,6 +154,10 @@ SIGNATURES = {
     "suminc_sum_pr_run": (i32, [vp, C.POINTER(Stats)]),
     "suminc_sum_sssp_shortcut": (i32, [vp, u32, u32, vp, vp, vp, C.POINTER(u32), C.POINTER(u32)]),
     "suminc_sum_sssp_run": (i32, [vp, C.POINTER(Stats)]),
+    "suminc_wpr_set_weights": (i32, [vp, vp]),
+    "suminc_wpr_forward": (i32, [vp, vp, vp, C.POINTER(Stats)]),
+    "suminc_wpr_backward": (i32, [vp, vp, vp, vp, C.POINTER(Stats)]),
+    "suminc_wpr_download": (i32, [vp, vp, vp]),
     "suminc_profile_enable": (i32, [vp, i32]),
     "suminc_profile_get": (i32, [vp, i32, C.POINTER(KernelProfile)]),
 }

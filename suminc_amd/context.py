@@ -1,8 +1,7 @@
 """Python handle over one libsuminc context (one device, one stream).
 
-This is plumbing for tests and the benchmark; the host-side worker that mirrors
-the reference's SumSyncIterWorker / SumSyncTraversalWorker phase loop lives in
-suminc_amd/worker.py and drives the same C ABI.
+This is plumbing for tests, the benchmark and the autograd wrapper
+(suminc_amd/wpr.py); every call goes through the C ABI of include/suminc.h.
 """
 from __future__ import annotations
 
@@ -11,6 +10,17 @@ import ctypes as C
 import numpy as np
 
 from . import _lib as L
+
+
+def _buf(x, dtype, count):
+    """Pointer to `count` elements of `dtype` in a numpy array or a device tensor."""
+    if hasattr(x, "data_ptr"):
+        if x.numel() != count or x.element_size() != np.dtype(dtype).itemsize or not x.is_contiguous():
+            raise ValueError(f"expected a contiguous tensor of {count} {np.dtype(dtype).name} values")
+        return C.c_void_p(x.data_ptr())
+    if not isinstance(x, np.ndarray) or x.dtype != dtype or x.size != count or not x.flags["C_CONTIGUOUS"]:
+        raise ValueError(f"expected a contiguous numpy array of {count} {np.dtype(dtype).name} values")
+    return L.ptr(x)
 
 
 class Context:
@@ -225,6 +235,36 @@ class Context:
         s = L.Stats()
         self._check(self._lib.suminc_sum_sssp_run(self._h, C.byref(s)))
         return s.as_dict()
+
+    # ---- weighted personalized PageRank + adjoint ------------------------------
+    # Arrays may be numpy arrays (host) or torch tensors on this context's GPU.
+    def wpr_set_weights(self, w):
+        self._check(self._lib.suminc_wpr_set_weights(self._h, _buf(w, np.float32, self.graph_info()[1])))
+
+    def wpr_forward(self, p, out=None) -> tuple:
+        n = self.graph_info()[0]
+        r = np.zeros(n, np.float32) if out is None else out
+        s = L.Stats()
+        self._check(self._lib.suminc_wpr_forward(self._h, _buf(p, np.float32, n), _buf(r, np.float32, n),
+                                                 C.byref(s)))
+        return r, s.as_dict()
+
+    def wpr_backward(self, c, grad_p=None, grad_w=None) -> tuple:
+        n, m, _ = self.graph_info()
+        gp = np.zeros(n, np.float32) if grad_p is None else grad_p
+        gw = np.zeros(m, np.float32) if grad_w is None else grad_w
+        s = L.Stats()
+        self._check(self._lib.suminc_wpr_backward(self._h, _buf(c, np.float32, n), _buf(gp, np.float32, n),
+                                                  _buf(gw, np.float32, m), C.byref(s)))
+        return gp, gw, s.as_dict()
+
+    def wpr_download(self, adjoint=False):
+        """FP64 r of the last forward and, with adjoint=True, lambda of the last backward."""
+        n = self.graph_info()[0]
+        r = np.zeros(n)
+        lam = np.zeros(n) if adjoint else None
+        self._check(self._lib.suminc_wpr_download(self._h, L.ptr(r), L.ptr(lam)))
+        return r, lam
 
     # ---- SSSP -----------------------------------------------------------------
     def sssp_init(self, source=0):

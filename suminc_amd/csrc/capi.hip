@@ -123,6 +123,7 @@ int suminc_ctx_destroy(suminc_ctx* c) {
   if (c->ev_start) (void)hipEventDestroy(c->ev_start);
   if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
   hipStream_t s = c->stream;
+  wpr_release(c);
   delete c;  // DevBuf destructors free device memory
   if (s) (void)hipStreamDestroy(s);
   return SUMINC_OK;
@@ -285,6 +286,23 @@ int suminc_sum_sssp_shortcut(suminc_ctx* c, uint32_t entry, uint32_t cap, uint32
 
 int suminc_sum_sssp_run(suminc_ctx* c, suminc_stats* stats) {
   return guarded(c, [&] { sum_sssp_run(c, stats); });
+}
+
+int suminc_wpr_set_weights(suminc_ctx* c, const float* weights) {
+  return guarded(c, [&] { wpr_set_weights(c, weights); });
+}
+
+int suminc_wpr_forward(suminc_ctx* c, const float* personalization, float* rank, suminc_stats* stats) {
+  return guarded(c, [&] { wpr_forward(c, personalization, rank, stats); });
+}
+
+int suminc_wpr_backward(suminc_ctx* c, const float* grad_rank, float* grad_personalization, float* grad_weights,
+                        suminc_stats* stats) {
+  return guarded(c, [&] { wpr_backward(c, grad_rank, grad_personalization, grad_weights, stats); });
+}
+
+int suminc_wpr_download(suminc_ctx* c, double* rank, double* adjoint) {
+  return guarded(c, [&] { wpr_download(c, rank, adjoint); });
 }
 
 int suminc_get_stats(suminc_ctx* c, suminc_stats* stats) {

@@ -166,6 +166,9 @@ struct suminc_ctx {
   uint32_t sssp_src = 0;
   suminc::DevBuf sssp_dp, sssp_flag0, sssp_flag1;
 
+  // weighted PageRank / adjoint state (wpr.hip), created on first use
+  void* wpr = nullptr;
+
   // work lists and scratch
   suminc::DevBuf wl0, wl1, heavy, partial, tmp, tmp2, tmp3, tmp4, tmp5;
   suminc::BatchWs bw;

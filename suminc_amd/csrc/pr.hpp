@@ -30,4 +30,11 @@ void generate_batch(suminc_ctx* c, uint64_t seed, uint32_t batch_id, uint64_t n_
                     uint32_t wmax, suminc_update* ops);
 void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool ops_on_device);
 
+// weighted PageRank + adjoint (wpr.hip)
+void wpr_set_weights(suminc_ctx* c, const float* w);
+void wpr_forward(suminc_ctx* c, const float* p, float* r, suminc_stats* st);
+void wpr_backward(suminc_ctx* c, const float* cvec, float* grad_p, float* grad_w, suminc_stats* st);
+void wpr_download(suminc_ctx* c, double* r, double* lam);
+void wpr_release(suminc_ctx* c);
+
 }  // namespace suminc

@@ -1,0 +1,598 @@
+// wpr.hip -- weighted personalized PageRank and its adjoint on the device.
+//
+// No reference counterpart: the reference's nearest app is the weighted
+// personalized PHP Ingress (php/php_ingress.h:15-120), and nothing in it has
+// gradients.  The forward keeps the delta-accumulative form of the plain path:
+//   r = (1-d) p + d W^T r,  W[u,v] = w_uv / s_u,  s_u = sum_k w_uk (rows of
+//   dangling vertices are zero, their mass leaks like the plain path's),
+// computed as r = sum_t delta_t with delta_0 = (1-d) p, delta_{t+1} = d W^T delta_t:
+// a pull over the in-CSR (CSC) with per-edge coefficient a_e = d w_e / s_src.
+// The adjoint of L = sum_i c_i r_i is the same delta iteration on the transpose
+// graph (the out-CSR): lambda = sum_t mu_t, mu_0 = c, mu_{t+1} = d W mu_t, and
+//   dL/dp_v    = (1-d) lambda_v
+//   dL/dw_uv   = (d r_u / s_u) (lambda_v - sum_k W[u,k] lambda_k).
+//
+// Rows are classified once per graph version and direction (short rows: one
+// thread; medium: one wave; long: 2048-edge chunks with one atomic each), so a
+// round has no per-edge atomics.  State: FP32 deltas with Kahan-compensated FP32
+// sums, or FP64; the stop rule and every residual are FP64 block partials.
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include "ctx.hpp"
+#include "device.hpp"
+#include "pr.hpp"
+
+namespace suminc {
+
+// ---- row classes of one direction -----------------------------------------------
+__global__ void k_wpr_lists(uint32_t n, const uint32_t* off, uint32_t* short_list, uint32_t* wave_list, uint2* heavy,
+                            unsigned long long* counts) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t deg = v < n ? off[v + 1] - off[v] : 0;
+  const bool is_short = deg >= 1 && deg <= kPullThread;
+  const bool is_wave = deg > kPullThread && deg <= kHeavyDeg;
+  unsigned long long s1 = wave_append(is_short, &counts[0]);
+  if (is_short) short_list[s1] = v;
+  unsigned long long s2 = wave_append(is_wave, &counts[1]);
+  if (is_wave) wave_list[s2] = v;
+  if (deg > kHeavyDeg) {
+    uint32_t nch = (deg + kChunk - 1) / kChunk;
+    unsigned long long slot = atomicAdd(&counts[2], (unsigned long long)nch);
+    for (uint32_t k = 0; k < nch; ++k) heavy[slot + k] = make_uint2(v, k);
+  }
+}
+
+struct RowLists {
+  const uint32_t* off;
+  const uint32_t* col;
+  const uint32_t* shrt;
+  const uint32_t* wave;
+  const uint2* heavy;
+  uint32_t n_short, n_wave;
+  uint64_t n_heavy;
+};
+
+// y[v] = sum_{e in row v} a[e] x[col[e]]; y must be zero before (heavy rows add).
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_spmv_short(RowLists L, const T* __restrict__ a, const T* __restrict__ x,
+                                                       T* __restrict__ y, const Counters* cnt) {
+  if (cnt && cnt->done) return;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < L.n_short; i += gridDim.x * kBlock) {
+    const uint32_t v = L.shrt[i];
+    T acc = T(0);
+    for (uint32_t e = L.off[v]; e < L.off[v + 1]; ++e) acc += a[e] * x[L.col[e]];
+    y[v] = acc;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_spmv_wave(RowLists L, const T* __restrict__ a, const T* __restrict__ x,
+                                                      T* __restrict__ y, const Counters* cnt) {
+  if (cnt && cnt->done) return;
+  const uint32_t waves = gridDim.x * (kBlock / 64);
+  for (uint32_t i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); i < L.n_wave; i += waves) {
+    const uint32_t v = L.wave[i];
+    T acc = T(0);
+    for (uint32_t e = L.off[v] + lane_id(); e < L.off[v + 1]; e += 64) acc += a[e] * x[L.col[e]];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane_id() == 0) y[v] = acc;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_spmv_heavy(RowLists L, const T* __restrict__ a, const T* __restrict__ x,
+                                                       T* __restrict__ y, const Counters* cnt) {
+  __shared__ T s_red[kBlock / 64];
+  if (cnt && cnt->done) return;
+  for (uint64_t h = blockIdx.x; h < L.n_heavy; h += gridDim.x) {
+    const uint2 ch = L.heavy[h];
+    const uint32_t b = L.off[ch.x] + ch.y * kChunk;
+    const uint32_t e = min(L.off[ch.x + 1], b + kChunk);
+    T acc = T(0);
+    for (uint32_t k = b + threadIdx.x; k < e; k += kBlock) acc += a[k] * x[L.col[k]];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane_id() == 0) s_red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      T s = T(0);
+      for (int w = 0; w < kBlock / 64; ++w) s += s_red[w];
+      if (s != T(0)) atomicAdd(&y[ch.x], s);
+    }
+    __syncthreads();
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void kahan(T* sum, float* comp, uint32_t v, T x) {
+  if constexpr (sizeof(T) == 4) {
+    float r = sum[v], cc = comp[v];
+    float yv = (float)x - cc;
+    float t = r + yv;
+    comp[v] = (t - r) - yv;
+    sum[v] = t;
+  } else {
+    sum[v] += x;
+  }
+}
+
+// One vertex pass of a round: sum += x (compensated), x <- y, y <- 0, and the
+// FP64 block partial of sum |x_new|.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_wpr_step(uint32_t n, T* __restrict__ x, T* __restrict__ y,
+                                                     T* __restrict__ sum, float* __restrict__ comp,
+                                                     double* __restrict__ partial, const Counters* cnt) {
+  __shared__ double s_p[kBlock / 64];
+  if (cnt->done) return;
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  double a = 0;
+  if (v < n) {
+    const T xo = x[v], xn = y[v];
+    if (xo != T(0)) kahan(sum, comp, v, xo);
+    x[v] = xn;
+    y[v] = T(0);
+    a = fabs((double)xn);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+  if (lane_id() == 0) s_p[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += s_p[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_wpr_decide(const double* __restrict__ partial, uint32_t nparts,
+                                                     Counters* cnt, double tol, int max_rounds,
+                                                     unsigned long long edges_per_round) {
+  __shared__ double s[1024];
+  if (cnt->done) return;
+  double acc = 0;
+  for (uint32_t i = threadIdx.x; i < nparts; i += 1024) acc += partial[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x) return;
+  const double r = s[0];
+  cnt->residual = r;
+  cnt->rounds += 1;
+  cnt->edges += edges_per_round;
+  if (r != r || r > 1e300) cnt->aux[0] = 1;
+  if (!(r >= tol) || (long long)cnt->rounds >= (long long)max_rounds || cnt->aux[0]) cnt->done = 1;
+}
+
+// x = scale * seed, y = sum = 0; block partials of sum |x|
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_wpr_seed(uint32_t n, const float* __restrict__ src, T scale,
+                                                     T* __restrict__ x, T* __restrict__ y, T* __restrict__ sum,
+                                                     float* __restrict__ comp, double* __restrict__ partial) {
+  __shared__ double s_p[kBlock / 64];
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  double a = 0;
+  if (v < n) {
+    const T s = scale * (T)src[v];
+    x[v] = s;
+    y[v] = T(0);
+    sum[v] = T(0);
+    comp[v] = 0.f;
+    a = fabs((double)s);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+  if (lane_id() == 0) s_p[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += s_p[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_sum_partials(const double* __restrict__ partial, uint32_t nparts,
+                                                       double* out) {
+  __shared__ double s[1024];
+  double acc = 0;
+  for (uint32_t i = threadIdx.x; i < nparts; i += 1024) acc += partial[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
+// sum += x at the stop (the last delta belongs to the series)
+template <class T>
+__global__ void k_wpr_finish(uint32_t n, const T* __restrict__ x, T* __restrict__ sum, float* __restrict__ comp,
+                             float* __restrict__ out32, double* __restrict__ out64) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const T xo = x[v];
+  if (xo != T(0)) kahan(sum, comp, v, xo);
+  double val = (double)sum[v];
+  if constexpr (sizeof(T) == 4) val -= (double)comp[v];
+  if (out32) out32[v] = (float)val;
+  out64[v] = val;
+}
+
+// ---- weights: row sums and the per-edge coefficients of both directions ---------
+__global__ void __launch_bounds__(kBlock) k_wpr_rowsum(uint32_t n, const uint32_t* __restrict__ off,
+                                                       const float* __restrict__ w, double* __restrict__ s) {
+  const uint32_t v = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (v >= n) return;  // whole wave
+  double acc = 0;
+  for (uint32_t e = off[v] + lane_id(); e < off[v + 1]; e += 64) acc += (double)w[e];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane_id() == 0) s[v] = acc;
+}
+
+// a_e = d w_e / s_src for the out-CSR (edge-parallel over rows via the tile rows)
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_wpr_coef_csr(uint32_t n, const uint32_t* __restrict__ off,
+                                                         const float* __restrict__ w, const double* __restrict__ s,
+                                                         double d, T* __restrict__ a) {
+  const uint32_t v = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (v >= n) return;
+  const double inv = s[v] > 0 ? d / s[v] : 0.0;
+  for (uint32_t e = off[v] + lane_id(); e < off[v + 1]; e += 64) a[e] = (T)((double)w[e] * inv);
+}
+
+// the in-CSR copy: a_csc[i] = a_csr[eid[i]]
+template <class T>
+__global__ void k_wpr_gather(uint64_t m, const uint32_t* __restrict__ eid, const T* __restrict__ src,
+                             T* __restrict__ dst) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) dst[i] = src[eid[i]];
+}
+
+__global__ void k_wpr_gather_f(uint64_t m, const uint32_t* __restrict__ eid, const float* __restrict__ src,
+                               uint32_t* __restrict__ dst) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) dst[i] = __float_as_uint(src[eid[i]]);
+}
+
+// CSR edge id of every CSC position: sort (dst, src) keys with the edge index
+__global__ void k_wpr_edge_keys(const uint32_t* __restrict__ off, uint32_t n, uint64_t m,
+                                const uint32_t* __restrict__ col, unsigned long long* keys, uint32_t* idx) {
+  const uint32_t v = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (v >= n) return;
+  for (uint32_t e = off[v] + lane_id(); e < off[v + 1]; e += 64) {
+    keys[e] = ((unsigned long long)col[e] << 32) | v;
+    idx[e] = e;
+  }
+  (void)m;
+}
+
+// ---- gradients -------------------------------------------------------------------
+// grad_w[e] for CSR edge e = (u -> v): (d r_u / s_u) lambda_v - (r_u / s_u) y_u with
+// y_u = sum_k a_uk lambda_k = d sum_k W[u,k] lambda_k.  One wave per row.
+__global__ void __launch_bounds__(kBlock) k_wpr_grad_w(uint32_t n, const uint32_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ col, const double* __restrict__ s,
+                                                       const double* __restrict__ r, const double* __restrict__ lam,
+                                                       const double* __restrict__ yl, double d,
+                                                       float* __restrict__ g32, double* __restrict__ g64) {
+  const uint32_t u = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (u >= n) return;
+  const double ru = s[u] > 0 ? r[u] / s[u] : 0.0;  // a row without weight mass has no gradient
+  const double sub = ru * yl[u];
+  for (uint32_t e = off[u] + lane_id(); e < off[u + 1]; e += 64) {
+    const double g = d * ru * lam[col[e]] - sub;
+    if (g32) g32[e] = (float)g;
+    if (g64) g64[e] = g;
+  }
+}
+
+__global__ void k_wpr_grad_p(uint32_t n, const double* __restrict__ lam, double d, float* __restrict__ g32,
+                             double* __restrict__ g64) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const double g = (1.0 - d) * lam[v];
+  if (g32) g32[v] = (float)g;
+  if (g64) g64[v] = g;
+}
+
+template <class T>
+__global__ void k_to_t(uint32_t n, const double* __restrict__ src, T* __restrict__ dst) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) dst[v] = (T)src[v];
+}
+
+template <class T>
+__global__ void k_to_f64(uint32_t n, const T* __restrict__ src, double* __restrict__ dst) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) dst[v] = (double)src[v];
+}
+
+// ---------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------
+struct WprDir {
+  DevBuf shrt, wave, heavy, counts;
+  uint64_t n_short = 0, n_wave = 0, n_heavy = 0;
+};
+
+static void build_lists(suminc_ctx* c, WprDir& L, const DevBuf& off) {
+  const uint32_t n = c->n;
+  L.shrt.ensure((size_t)n * 4 + 4);
+  L.wave.ensure((size_t)n * 4 + 4);
+  L.heavy.ensure((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint2));
+  L.counts.ensure(32);
+  SUMINC_HIP(hipMemsetAsync(L.counts.p, 0, 32, c->stream));
+  k_wpr_lists<<<grid_for(n), kBlock, 0, c->stream>>>(n, off.as<uint32_t>(), L.shrt.as<uint32_t>(),
+                                                     L.wave.as<uint32_t>(), L.heavy.as<uint2>(),
+                                                     L.counts.as<unsigned long long>());
+  unsigned long long h[3];
+  SUMINC_HIP(hipMemcpyAsync(h, L.counts.p, 24, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  L.n_short = h[0];
+  L.n_wave = h[1];
+  L.n_heavy = h[2];
+}
+
+struct WprState {
+  uint64_t graph_version = ~0ull;    // version the lists / edge ids were built for
+  uint64_t coef_version = ~0ull;     // weights version the coefficients belong to
+  int coef_prec = -1;
+  uint64_t weights_version = 0;
+  bool fwd_ready = false;
+  uint64_t fwd_coef_version = ~0ull;
+  WprDir out, in;
+  DevBuf eid;                        // CSR edge id per CSC position
+  DevBuf s;                          // row sums (f64)
+  DevBuf a_out, a_in;                // coefficients d w / s (T) in CSR / CSC order
+  DevBuf x, y, sum, comp;            // round state (T)
+  DevBuf r64, lam64, yl64;           // FP64 results of the last forward / backward
+  DevBuf l1;                         // sum |seed|
+};
+
+static WprState& wstate(suminc_ctx* c) {
+  if (!c->wpr) c->wpr = new WprState();
+  return *reinterpret_cast<WprState*>(c->wpr);
+}
+
+void wpr_release(suminc_ctx* c) {
+  delete reinterpret_cast<WprState*>(c->wpr);
+  c->wpr = nullptr;
+}
+
+static void wpr_prepare_graph(suminc_ctx* c, WprState& S) {
+  if (S.graph_version == c->graph_version) return;
+  const uint32_t n = c->n;
+  const uint64_t m = c->m;
+  build_lists(c, S.out, c->out_off);
+  build_lists(c, S.in, c->in_off);
+  S.eid.ensure(m * 4 + 4);
+  if (m) {
+    DevBuf ka, kb, ia;
+    ka.ensure(m * 8); kb.ensure(m * 8); ia.ensure(m * 4);
+    k_wpr_edge_keys<<<grid_for(n, kBlock / 64), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), n, m,
+                                                                       c->out_col.as<uint32_t>(),
+                                                                       ka.as<unsigned long long>(), ia.as<uint32_t>());
+    size_t bytes = 0;
+    SUMINC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, ka.as<uint64_t>(), kb.as<uint64_t>(), ia.as<uint32_t>(),
+                                         S.eid.as<uint32_t>(), (size_t)m, 0, 64, c->stream));
+    c->tmp.ensure(bytes);
+    SUMINC_HIP(rocprim::radix_sort_pairs(c->tmp.p, bytes, ka.as<uint64_t>(), kb.as<uint64_t>(), ia.as<uint32_t>(),
+                                         S.eid.as<uint32_t>(), (size_t)m, 0, 64, c->stream));
+    SUMINC_HIP(hipStreamSynchronize(c->stream));
+  }
+  S.graph_version = c->graph_version;
+  S.coef_version = ~0ull;
+  S.fwd_ready = false;
+}
+
+template <class T>
+static void wpr_prepare_coef(suminc_ctx* c, WprState& S) {
+  const int prec = sizeof(T) == 8 ? SUMINC_PR_FP64 : SUMINC_PR_FP32_KAHAN;
+  if (S.coef_version == S.weights_version && S.coef_prec == prec) return;
+  const uint32_t n = c->n;
+  const uint64_t m = c->m;
+  S.s.ensure((size_t)n * 8 + 8);
+  S.a_out.ensure(m * sizeof(T) + 8);
+  S.a_in.ensure(m * sizeof(T) + 8);
+  const unsigned gw = grid_for(n, kBlock / 64);
+  k_wpr_rowsum<<<gw, kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), c->out_w.as<float>(), S.s.as<double>());
+  k_wpr_coef_csr<T><<<gw, kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), c->out_w.as<float>(),
+                                                  S.s.as<double>(), c->prm.damping, S.a_out.as<T>());
+  if (m) k_wpr_gather<T><<<grid_for(m), kBlock, 0, c->stream>>>(m, S.eid.as<uint32_t>(), S.a_out.as<T>(), S.a_in.as<T>());
+  SUMINC_HIP(hipGetLastError());
+  S.coef_version = S.weights_version;
+  S.coef_prec = prec;
+}
+
+static RowLists lists_of(const WprDir& L, const DevBuf& off, const DevBuf& col) {
+  RowLists R;
+  R.off = off.as<uint32_t>();
+  R.col = col.as<uint32_t>();
+  R.shrt = L.shrt.as<uint32_t>();
+  R.wave = L.wave.as<uint32_t>();
+  R.heavy = L.heavy.as<uint2>();
+  R.n_short = (uint32_t)L.n_short;
+  R.n_wave = (uint32_t)L.n_wave;
+  R.n_heavy = L.n_heavy;
+  return R;
+}
+
+template <class T>
+static void spmv(suminc_ctx* c, const RowLists& R, const T* a, const T* x, T* y, const Counters* cnt) {
+  if (R.n_short)
+    k_spmv_short<T><<<grid_for(R.n_short, kBlock, 8192), kBlock, 0, c->stream>>>(R, a, x, y, cnt);
+  if (R.n_wave)
+    k_spmv_wave<T><<<grid_for(R.n_wave, kBlock / 64, 8192), kBlock, 0, c->stream>>>(R, a, x, y, cnt);
+  if (R.n_heavy) k_spmv_heavy<T><<<push_grid(), kBlock, 0, c->stream>>>(R, a, x, y, cnt);
+}
+
+// Runs sum = sum_t x_t with x_{t+1} = A x_t from x_0 = scale * seed until
+// sum|x| < tol.  Result (FP64) in out64, optionally FP32 in out32.
+template <class T>
+static void wpr_series(suminc_ctx* c, WprState& S, const RowLists& R, const T* a, const float* seed_dev, T scale,
+                       double tol_abs, bool relative, float* out32, double* out64, suminc_stats* st) {
+  const uint32_t n = c->n;
+  S.x.ensure((size_t)n * sizeof(T) + 8);
+  S.y.ensure((size_t)n * sizeof(T) + 8);
+  S.sum.ensure((size_t)n * sizeof(T) + 8);
+  S.comp.ensure((size_t)n * 4 + 4);
+  S.l1.ensure(8);
+  const unsigned g = grid_for(n);
+  c->partial.ensure((size_t)(g + 1) * 8 * 3);
+  Counters* cnt = dev_counters(c);
+  SUMINC_HIP(hipMemsetAsync(cnt, 0, sizeof(Counters), c->stream));
+  k_wpr_seed<T><<<g, kBlock, 0, c->stream>>>(n, seed_dev, scale, S.x.as<T>(), S.y.as<T>(), S.sum.as<T>(),
+                                             S.comp.as<float>(), c->partial.as<double>());
+  k_sum_partials<<<1, 1024, 0, c->stream>>>(c->partial.as<double>(), g, S.l1.as<double>());
+  double tol = tol_abs;
+  if (relative) {
+    double l1 = 0;
+    SUMINC_HIP(hipMemcpyAsync(&l1, S.l1.p, 8, hipMemcpyDeviceToHost, c->stream));
+    SUMINC_HIP(hipStreamSynchronize(c->stream));
+    tol = tol_abs * l1;
+  }
+  const int max_rounds = c->prm.max_rounds > 0 ? c->prm.max_rounds : 100000;
+  SUMINC_HIP(hipEventRecord(c->ev_start, c->stream));
+  int group = 8;
+  while (true) {
+    for (int i = 0; i < group; ++i) {
+      spmv<T>(c, R, a, S.x.as<T>(), S.y.as<T>(), cnt);
+      k_wpr_step<T><<<g, kBlock, 0, c->stream>>>(n, S.x.as<T>(), S.y.as<T>(), S.sum.as<T>(), S.comp.as<float>(),
+                                                 c->partial.as<double>(), cnt);
+      k_wpr_decide<<<1, 1024, 0, c->stream>>>(c->partial.as<double>(), g, cnt, tol, max_rounds, c->m);
+    }
+    SUMINC_HIP(hipGetLastError());
+    read_counters(c);
+    if (c->host_counters->done) break;
+    if (group < 64) group *= 2;
+  }
+  k_wpr_finish<T><<<g, kBlock, 0, c->stream>>>(n, S.x.as<T>(), S.sum.as<T>(), S.comp.as<float>(), out32, out64);
+  SUMINC_HIP(hipGetLastError());
+  SUMINC_HIP(hipEventRecord(c->ev_stop, c->stream));
+  SUMINC_HIP(hipEventSynchronize(c->ev_stop));
+  float ms = 0;
+  SUMINC_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+  const Counters* h = c->host_counters;
+  SUMINC_REQUIRE(!h->aux[0], SUMINC_E_NUMERIC, "NaN/Inf in the weighted PageRank state");
+  SUMINC_REQUIRE(!(h->residual >= tol), SUMINC_E_NOT_CONVERGED, "round cap reached before sum|delta| < tolerance");
+  suminc_stats out{};
+  out.rounds = (double)h->rounds;
+  out.edges_traversed = (double)h->edges;
+  out.residual = h->residual;
+  out.kernel_ms = ms;
+  out.pull_rounds = (double)h->rounds;
+  c->stats = out;
+  if (st) *st = out;
+}
+
+static void check_weighted(suminc_ctx* c) {
+  SUMINC_REQUIRE(c->has_graph, SUMINC_E_STATE, "no graph");
+  SUMINC_REQUIRE(c->wkind == SUMINC_W_F32, SUMINC_E_STATE, "weighted PageRank needs SUMINC_W_F32 edge weights");
+}
+
+void wpr_set_weights(suminc_ctx* c, const float* w) {
+  check_weighted(c);
+  SUMINC_REQUIRE(w || c->m == 0, SUMINC_E_INVALID, "null weights");
+  WprState& S = wstate(c);
+  if (c->m) {
+    SUMINC_HIP(hipMemcpyAsync(c->out_w.p, w, c->m * 4, hipMemcpyDefault, c->stream));
+    wpr_prepare_graph(c, S);
+    k_wpr_gather_f<<<grid_for(c->m), kBlock, 0, c->stream>>>(c->m, S.eid.as<uint32_t>(), c->out_w.as<float>(),
+                                                             c->in_w.as<uint32_t>());
+    SUMINC_HIP(hipGetLastError());
+    SUMINC_HIP(hipStreamSynchronize(c->stream));
+  }
+  S.weights_version++;
+  S.fwd_ready = false;
+}
+
+template <class T>
+static void wpr_forward_t(suminc_ctx* c, const float* p, float* r, suminc_stats* st) {
+  WprState& S = wstate(c);
+  wpr_prepare_graph(c, S);
+  wpr_prepare_coef<T>(c, S);
+  const uint32_t n = c->n;
+  DevBuf pd, r32;
+  pd.ensure((size_t)n * 4);
+  SUMINC_HIP(hipMemcpyAsync(pd.p, p, (size_t)n * 4, hipMemcpyDefault, c->stream));
+  S.r64.ensure((size_t)n * 8 + 8);
+  r32.ensure((size_t)n * 4 + 4);
+  const RowLists R = lists_of(S.in, c->in_off, c->in_col);
+  wpr_series<T>(c, S, R, S.a_in.as<T>(), pd.as<float>(), (T)(1.0 - c->prm.damping), c->prm.termcheck_threshold,
+                false, r32.as<float>(), S.r64.as<double>(), st);
+  if (r) SUMINC_HIP(hipMemcpyAsync(r, r32.p, (size_t)n * 4, hipMemcpyDefault, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  S.fwd_ready = true;
+  S.fwd_coef_version = S.weights_version;
+}
+
+template <class T>
+static void wpr_backward_t(suminc_ctx* c, const float* cvec, float* grad_p, float* grad_w, suminc_stats* st) {
+  WprState& S = wstate(c);
+  SUMINC_REQUIRE(S.fwd_ready && S.fwd_coef_version == S.weights_version && S.graph_version == c->graph_version,
+                 SUMINC_E_STATE, "backward needs a forward on the current graph and weights");
+  wpr_prepare_coef<T>(c, S);
+  const uint32_t n = c->n;
+  const uint64_t m = c->m;
+  DevBuf cd, gp32, gw32;
+  cd.ensure((size_t)n * 4);
+  SUMINC_HIP(hipMemcpyAsync(cd.p, cvec, (size_t)n * 4, hipMemcpyDefault, c->stream));
+  S.lam64.ensure((size_t)n * 8 + 8);
+  S.yl64.ensure((size_t)n * 8 + 8);
+  const RowLists R = lists_of(S.out, c->out_off, c->out_col);
+  wpr_series<T>(c, S, R, S.a_out.as<T>(), cd.as<float>(), (T)1, c->prm.termcheck_threshold, true, nullptr,
+                S.lam64.as<double>(), st);
+  // y_u = sum_k a_uk lambda_k (one more product, in FP64)
+  DevBuf lam_t, y_t;
+  lam_t.ensure((size_t)n * sizeof(T) + 8);
+  y_t.ensure((size_t)n * sizeof(T) + 8);
+  SUMINC_HIP(hipMemsetAsync(y_t.p, 0, (size_t)n * sizeof(T), c->stream));
+  k_to_t<T><<<grid_for(n), kBlock, 0, c->stream>>>(n, S.lam64.as<double>(), lam_t.as<T>());
+  spmv<T>(c, R, S.a_out.as<T>(), lam_t.as<T>(), y_t.as<T>(), nullptr);
+  DevBuf yd;
+  yd.ensure((size_t)n * 8 + 8);
+  k_to_f64<T><<<grid_for(n), kBlock, 0, c->stream>>>(n, y_t.as<T>(), yd.as<double>());
+  SUMINC_HIP(hipGetLastError());
+  gp32.ensure((size_t)n * 4 + 4);
+  gw32.ensure(m * 4 + 4);
+  k_wpr_grad_p<<<grid_for(n), kBlock, 0, c->stream>>>(n, S.lam64.as<double>(), c->prm.damping, gp32.as<float>(),
+                                                      nullptr);
+  k_wpr_grad_w<<<grid_for(n, kBlock / 64), kBlock, 0, c->stream>>>(
+      n, c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(), S.s.as<double>(), S.r64.as<double>(),
+      S.lam64.as<double>(), yd.as<double>(), c->prm.damping, gw32.as<float>(), nullptr);
+  SUMINC_HIP(hipGetLastError());
+  if (grad_p) SUMINC_HIP(hipMemcpyAsync(grad_p, gp32.p, (size_t)n * 4, hipMemcpyDefault, c->stream));
+  if (grad_w && m) SUMINC_HIP(hipMemcpyAsync(grad_w, gw32.p, m * 4, hipMemcpyDefault, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+}
+
+void wpr_forward(suminc_ctx* c, const float* p, float* r, suminc_stats* st) {
+  check_weighted(c);
+  SUMINC_REQUIRE(p, SUMINC_E_INVALID, "null personalization");
+  if (c->prm.pr_precision == SUMINC_PR_FP64) wpr_forward_t<double>(c, p, r, st);
+  else wpr_forward_t<float>(c, p, r, st);
+}
+
+void wpr_backward(suminc_ctx* c, const float* cvec, float* grad_p, float* grad_w, suminc_stats* st) {
+  check_weighted(c);
+  SUMINC_REQUIRE(cvec, SUMINC_E_INVALID, "null loss weights");
+  if (c->prm.pr_precision == SUMINC_PR_FP64) wpr_backward_t<double>(c, cvec, grad_p, grad_w, st);
+  else wpr_backward_t<float>(c, cvec, grad_p, grad_w, st);
+}
+
+void wpr_download(suminc_ctx* c, double* r, double* lam) {
+  WprState& S = wstate(c);
+  SUMINC_REQUIRE(S.fwd_ready, SUMINC_E_STATE, "no weighted PageRank forward yet");
+  if (r) SUMINC_HIP(hipMemcpy(r, S.r64.p, (size_t)c->n * 8, hipMemcpyDeviceToHost));
+  if (lam) {
+    SUMINC_REQUIRE(S.lam64.p, SUMINC_E_STATE, "no backward yet");
+    SUMINC_HIP(hipMemcpy(lam, S.lam64.p, (size_t)c->n * 8, hipMemcpyDeviceToHost));
+  }
+}
+
+}  // namespace suminc
