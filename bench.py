@@ -56,6 +56,17 @@ WORKLOADS = {
 }
 
 
+def kernel_names():
+    import suminc_amd._lib as L
+
+    return {L.KERNEL_PR_PUSH: "k_pr_push", L.KERNEL_PR_GATHER: "k_pr_gather",
+            L.KERNEL_SSSP_PUSH: "k_sssp_push+k_sssp_heavy", L.KERNEL_SSSP_SKETCH: "k_ss_push+k_ss_heavy",
+            L.KERNEL_PR_PULL: "k_pr_pull", L.KERNEL_BATCH_COPY: "k_copy_old"}
+
+
+KERNEL_NAMES = {}
+
+
 def roofline(ctx, kernel_ids):
     import suminc_amd._lib as L
 
@@ -68,19 +79,44 @@ def roofline(ctx, kernel_ids):
         return None, None
     k, p = best
     achieved = p["alg_bytes"] / (p["ms"] * 1e-3) / 1e9
-    name = {L.KERNEL_PR_PUSH: "k_pr_push", L.KERNEL_PR_GATHER: "k_pr_gather",
-            L.KERNEL_SSSP_PUSH: "k_sssp_push+k_sssp_heavy", L.KERNEL_SSSP_SKETCH: "k_ss_push+k_ss_heavy"}[k]
+    name = KERNEL_NAMES[k]
+    alg = p["alg_bytes"] / p["launches"]
+    traffic = measured_traffic(name)
     return name, {
         "bound": "hbm",
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": None if traffic is None else round(traffic),
         "kernel": name,
         "avg_launch_us": round(1e3 * p["ms"] / p["launches"], 2),
-        "alg_bytes_per_launch": round(p["alg_bytes"] / p["launches"]),
+        "alg_bytes_per_launch": round(alg),
+        "kernel_ms_total": round(p["ms"], 3),
     }
+
+
+def kernel_table(ctx, kernel_ids):
+    out = {}
+    for k in kernel_ids:
+        p = ctx.profile(k)
+        if p["launches"]:
+            out[KERNEL_NAMES[k]] = {"launches": int(p["launches"]), "ms": round(p["ms"], 3),
+                                    "GB/s": round(p["alg_bytes"] / (p["ms"] * 1e-3) / 1e9, 1)}
+    return out
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the default workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the counters'
+    reading for a plain copy of known size), or None when there is none."""
+    f = ROOT / "profiles" / f"{kernel}_sssp22_traffic.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def run_sssp(args, wl):
@@ -127,7 +163,9 @@ def run_sssp(args, wl):
         run_ms += st["kernel_ms"]
     ctx.sync()
     dt = time.perf_counter() - t0
-    kernel, roof = roofline(ctx, [L.KERNEL_SSSP_SKETCH if sketch else L.KERNEL_SSSP_PUSH])
+    kids = [L.KERNEL_SSSP_SKETCH if sketch else L.KERNEL_SSSP_PUSH, L.KERNEL_BATCH_COPY]
+    kernel, roof = roofline(ctx, kids)
+    extra["kernels"] = kernel_table(ctx, kids)
     ctx.profile_enable(False)
     for d in dev:
         d.free()
@@ -233,7 +271,8 @@ def run_pr(args, wl):
         edges += st["edges_traversed"] + sa["reserved"][3]
     ctx.sync()
     dt = time.perf_counter() - t0
-    kernel, roof = roofline(ctx, [L.KERNEL_PR_PUSH, L.KERNEL_PR_GATHER])
+    kids = [L.KERNEL_PR_PUSH, L.KERNEL_PR_GATHER, L.KERNEL_BATCH_COPY]
+    kernel, roof = roofline(ctx, kids)
     ctx.profile_enable(False)
     ms_per_step = 1e3 * dt / args.steps
     gteps = edges / dt / 1e9
@@ -249,7 +288,8 @@ def run_pr(args, wl):
         full.append(time.perf_counter() - t1)
     full_ms = 1e3 * float(np.median(full))
     extra = {"full_recompute_ms": round(full_ms, 3), "speedup_vs_full_recompute": round(full_ms / ms_per_step, 2),
-             "edges_per_step": edges / args.steps, "termcheck_threshold": wl["T"]}
+             "edges_per_step": edges / args.steps, "termcheck_threshold": wl["T"],
+             "kernels": kernel_table(ctx, kids)}
     cpu = None
     if not args.no_verify or not args.no_cpu:
         off, col, _ = ctx.download_csr()
@@ -274,6 +314,26 @@ def run_pr(args, wl):
     return gteps, ms_per_step, "f32", roof, cpu, extra
 
 
+def calibrate_copy(nbytes):
+    """One device-to-device copy of nbytes (the runtime's copy kernel), so that the
+    FETCH_SIZE / WRITE_SIZE counters can be read against a known byte count."""
+    import ctypes as C
+
+    import suminc_amd
+
+    ctx = suminc_amd.Context()
+    lib = ctx._lib
+    a, b = C.c_void_p(), C.c_void_p()
+    ctx._check(lib.suminc_device_alloc(ctx._h, nbytes, C.byref(a)))
+    ctx._check(lib.suminc_device_alloc(ctx._h, nbytes, C.byref(b)))
+    for _ in range(3):
+        ctx._check(lib.suminc_copy_to_device(ctx._h, b, a, nbytes))
+    ctx._check(lib.suminc_device_free(ctx._h, a))
+    ctx._check(lib.suminc_device_free(ctx._h, b))
+    ctx.close()
+    log(f"copied {nbytes} bytes device to device, 3 times")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -285,7 +345,13 @@ def main():
     ap.add_argument("--supernodes", default="on", choices=["on", "off"],
                     help="SSSP workload: query over the supernode sketch (on) or the plain CSR (off)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="length of the CPU baseline sample")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES device-to-device (counter calibration for rocprofv3 --pmc) and exit")
     args = ap.parse_args()
+    KERNEL_NAMES.update(kernel_names())
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the multi-GPU edge-cut path is designed, not built "
               "(run with --gpus 1)", file=sys.stderr)

@@ -150,6 +150,7 @@ int suminc_apply_batch_device(suminc_ctx* ctx, const suminc_update* dev_ops, uin
  * context's device and stream); plumbing for device-resident inputs. */
 int suminc_device_alloc(suminc_ctx* ctx, uint64_t bytes, void** dev_ptr);
 int suminc_device_free(suminc_ctx* ctx, void* dev_ptr);
+/* src may be host memory or device memory of the context's GPU */
 int suminc_copy_to_device(suminc_ctx* ctx, void* dev_dst, const void* host_src, uint64_t bytes);
 
 /* ---- PageRank (plain Ingress path) --------------------------------------------
@@ -269,6 +270,7 @@ int suminc_get_stats(suminc_ctx* ctx, suminc_stats* stats);
 #define SUMINC_KERNEL_SSSP_PUSH 2 /* SSSP relax: 16 B/edge + 16 B/vertex */
 #define SUMINC_KERNEL_PR_PULL 3   /* PageRank CSC gather: 8 B/edge + 16 B/vertex (FP32) */
 #define SUMINC_KERNEL_SSSP_SKETCH 4 /* SSSP sketch rounds: 16 B/sketch edge + 20 B/shortcut slot + 16 B/vertex */
+#define SUMINC_KERNEL_BATCH_COPY 5  /* batch merge, old-edge copy: 8 B/edge per array (read + write), CSR + CSC */
 #define SUMINC_KERNEL_COUNT 8
 typedef struct suminc_kernel_profile {
   double launches;  /* timed launches */

@@ -167,6 +167,7 @@ KERNEL_PR_GATHER = 1
 KERNEL_SSSP_PUSH = 2
 KERNEL_PR_PULL = 3
 KERNEL_SSSP_SKETCH = 4
+KERNEL_BATCH_COPY = 5
 
 
 def lib():

@@ -597,12 +597,12 @@ static void cub_call(suminc_ctx* c, DevBuf& tmp, F&& f) {
 // Rebuilds one adjacency direction into the workspace buffers, then swaps them
 // with the live arrays (the old arrays become next batch's workspace).
 static void merge_direction(suminc_ctx* c, DevBuf& off, DevBuf& col, DevBuf& w, DevBuf& noff, DevBuf& ncol,
-                            DevBuf& nw, const Groups& gs, uint32_t N, uint64_t cap_m) {
+                            DevBuf& nw, const Groups& gs, uint32_t N, uint64_t cap_m, size_t prof_slot) {
   const uint32_t n = c->n;
   BatchWs& b = c->bw;
   noff.ensure(((size_t)n + 1) * 4);
-  ncol.ensure(cap_m * 4 + 4);
-  if (c->wkind) nw.ensure(cap_m * 4 + 4);
+  ncol.ensure_slack(cap_m * 4 + 4);
+  if (c->wkind) nw.ensure_slack(cap_m * 4 + 4);
   k_new_deg<<<grid_for((uint64_t)n + 1), kBlock, 0, c->stream>>>(off.as<uint32_t>(), n, gs, b.ndeg.as<uint32_t>());
   cub_call(c, c->tmp, [&](void* p, size_t& bytes) {
     return rocprim::exclusive_scan(p, bytes, b.ndeg.as<uint32_t>(), noff.as<uint32_t>(), 0u, (size_t)n + 1,
@@ -614,10 +614,12 @@ static void merge_direction(suminc_ctx* c, DevBuf& off, DevBuf& col, DevBuf& w, 
     b.trow.ensure(((size_t)ntiles + 1) * 4);
     k_tile_rows<<<grid_for((uint64_t)ntiles + 1), kBlock, 0, c->stream>>>(off.as<uint32_t>(), n, m, ntiles,
                                                                          b.trow.as<uint32_t>());
+    if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, prof_slot), c->stream));
     k_copy_old<<<ntiles, kBlock, 0, c->stream>>>(m, off.as<uint32_t>(), col.as<uint32_t>(),
                                                  c->wkind ? w.as<uint32_t>() : nullptr, noff.as<uint32_t>(),
                                                  ncol.as<uint32_t>(), c->wkind ? nw.as<uint32_t>() : nullptr,
                                                  b.trow.as<uint32_t>(), ntiles, gs);
+    if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, prof_slot + 1), c->stream));
   }
   k_place_adds<<<grid_for(N), kBlock, 0, c->stream>>>(off.as<uint32_t>(), col.as<uint32_t>(), noff.as<uint32_t>(),
                                                       ncol.as<uint32_t>(), c->wkind ? nw.as<uint32_t>() : nullptr,
@@ -828,7 +830,8 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   gs.w = b.g_w.as<uint32_t>(); gs.ex = b.g_ex.as<uint8_t>();
   gs.ex_pre = b.g_expre.as<uint32_t>(); gs.add_pre = b.g_addpre.as<uint32_t>();
   gs.tix = b.tix.as<uint32_t>(); gs.t_gbeg = b.t_gbeg.as<uint32_t>();
-  merge_direction(c, c->out_off, c->out_col, c->out_w, b.noff_out, b.ncol_out, b.nw_out, gs, N, cap_m);
+  const uint64_t m_old = c->m;
+  merge_direction(c, c->out_off, c->out_col, c->out_w, b.noff_out, b.ncol_out, b.nw_out, gs, N, cap_m, 0);
 
   k_transpose_group_keys<<<grid_for(N), kBlock, 0, c->stream>>>(b.g_row.as<uint32_t>(), b.g_col.as<uint32_t>(), bc,
                                                                 N, b.keys_a.as<uint64_t>(), b.idx_a.as<uint32_t>());
@@ -850,7 +853,7 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   gt.w = b.tg_w.as<uint32_t>(); gt.ex = b.tg_ex.as<uint8_t>();
   gt.ex_pre = b.tg_expre.as<uint32_t>(); gt.add_pre = b.tg_addpre.as<uint32_t>();
   gt.tix = b.ttix.as<uint32_t>(); gt.t_gbeg = b.tt_gbeg.as<uint32_t>();
-  merge_direction(c, c->in_off, c->in_col, c->in_w, b.noff_in, b.ncol_in, b.nw_in, gt, N, cap_m);
+  merge_direction(c, c->in_off, c->in_col, c->in_w, b.noff_in, b.ncol_in, b.nw_in, gt, N, cap_m, 2);
   k_untouch<<<grid_for(N), kBlock, 0, c->stream>>>(b.tt_list.as<uint32_t>(), bc, 2, N, b.ttix.as<uint32_t>());
 
   // 6. PageRank reseed +1 on the new out-edges
@@ -883,6 +886,15 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   SUMINC_HIP(hipMemcpyAsync(hbc, bc, sizeof(hbc), hipMemcpyDeviceToHost, c->stream));
   read_counters(c);  // synchronises the stream
   sketch_batch_done(c);
+  if (c->prof_on && m_old) {
+    prof_collect(c, SUMINC_KERNEL_BATCH_COPY, 2, 0, 2);
+    suminc_kernel_profile& p = c->prof[SUMINC_KERNEL_BATCH_COPY];
+    // algorithmic bytes of the old-edge copy: every old edge's column (and weight)
+    // read once and written once, per direction
+    const double per_edge = c->wkind ? 16.0 : 8.0;
+    p.edges += 2.0 * (double)m_old;
+    p.alg_bytes += 2.0 * per_edge * (double)m_old;
+  }
   c->m = hbc[3];
   c->graph_version++;
   // per-round scratch sized by m may need to grow

@@ -207,7 +207,7 @@ int suminc_copy_to_device(suminc_ctx* c, void* dev_dst, const void* host_src, ui
   return guarded(c, [&] {
     SUMINC_REQUIRE((dev_dst && host_src) || bytes == 0, SUMINC_E_INVALID, "null buffer");
     if (bytes) {
-      SUMINC_HIP(hipMemcpyAsync(dev_dst, host_src, bytes, hipMemcpyHostToDevice, c->stream));
+      SUMINC_HIP(hipMemcpyAsync(dev_dst, host_src, bytes, hipMemcpyDefault, c->stream));
       SUMINC_HIP(hipStreamSynchronize(c->stream));
     }
   });

@@ -53,6 +53,12 @@ struct DevBuf {
     SUMINC_HIP(hipMalloc(&p, b));
     cap = b;
   }
+  // Like ensure, but a reallocation leaves 1/16 headroom: buffers whose size
+  // drifts from batch to batch are not reallocated every time.
+  void ensure_slack(size_t bytes) {
+    if (bytes <= cap && p) return;
+    ensure(bytes + bytes / 16);
+  }
   template <class T>
   T* as() const { return reinterpret_cast<T*>(p); }
   void swap(DevBuf& o) {
@@ -132,6 +138,9 @@ struct Sketch {
   DevBuf ss_work;            // u64: internal edges read by the SSSP shortcut kernels
   DevBuf pr_valid, ss_valid; // K: shortcut rows of the cluster are current
   DevBuf sc_base_old, ent_of_old, sc_R_old, sc_D_old, ss_dist_old, ss_par_old;  // previous layout
+  // scratch of the layout pass, kept between batches
+  DevBuf lay_keys_a, lay_keys_b, lay_hist, lay_ckey, lay_entf, lay_epos, lay_ecount, lay_ekey, lay_rowlen, lay_mx,
+      lay_skey_a, lay_skey_b, lay_sidx_a;
   uint32_t counts[8] = {0};  // node type histogram
 };
 

@@ -462,7 +462,10 @@ static void grow_keep(suminc_ctx* c, DevBuf& b, size_t bytes, size_t keep) {
 static void sketch_layout(suminc_ctx* c, bool rebuild) {
   Sketch& s = c->sk;
   const uint32_t n = c->n, K = s.K, RM = s.RM;
-  DevBuf keys_a, keys_b, hist, ckey, entf, epos, ecount, ekey, rowlen, mx, skey_a, skey_b, sidx_a, sidx_b;
+  DevBuf &keys_a = s.lay_keys_a, &keys_b = s.lay_keys_b, &hist = s.lay_hist, &ckey = s.lay_ckey,
+         &entf = s.lay_entf, &epos = s.lay_epos, &ecount = s.lay_ecount, &ekey = s.lay_ekey,
+         &rowlen = s.lay_rowlen, &mx = s.lay_mx, &skey_a = s.lay_skey_a, &skey_b = s.lay_skey_b,
+         &sidx_a = s.lay_sidx_a;
   keys_a.ensure((size_t)RM * 8 + 8); keys_b.ensure((size_t)RM * 8 + 8); hist.ensure(32);
   SUMINC_HIP(hipMemsetAsync(hist.p, 0, 32, c->stream));
   if (RM) {
@@ -518,10 +521,11 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
   SUMINC_HIP(hipMemcpyAsync(&E, ecount.p, 4, hipMemcpyDeviceToHost, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
   s.E = E;
-  s.cl_ent.ensure((size_t)E * 4 + 4); ekey.ensure((size_t)E * 4 + 4);
+  s.cl_ent.ensure_slack((size_t)E * 4 + 4); ekey.ensure_slack((size_t)E * 4 + 4);
   const bool fresh_ent_of = !rebuild || !s.ent_of.p || s.ent_of.cap < (size_t)n * 4;
   s.ent_of.ensure((size_t)n * 4);
-  s.cl_eoff.ensure(((size_t)K + 1) * 4); rowlen.ensure(((size_t)E + 1) * 8); s.sc_base.ensure(((size_t)E + 1) * 8);
+  s.cl_eoff.ensure(((size_t)K + 1) * 4); rowlen.ensure_slack(((size_t)E + 1) * 8);
+  s.sc_base.ensure_slack(((size_t)E + 1) * 8);
   if (fresh_ent_of) k_fill32<<<grid_for(n), kBlock, 0, c->stream>>>(s.ent_of.as<uint32_t>(), n, INF32);
   else if (M) k_ent_of_clear<<<grid_for(M), kBlock, 0, c->stream>>>(M, s.cl_mem.as<uint32_t>(), s.ent_of.as<uint32_t>());
   k_entries<<<grid_for((uint64_t)E + 1), kBlock, 0, c->stream>>>(E, epos.as<uint32_t>(), s.cl_mem.as<uint32_t>(),
@@ -541,9 +545,9 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
   const uint64_t ie_m = s.iem_n;
   s.ie_m = ie_m;
   s.ie_off.ensure(((size_t)M + 1) * 4);
-  for (DevBuf* b : {&s.ie_src, &s.ie_dst, &s.ie_deg, &s.ie_wt}) b->ensure(ie_m * 4 + 4);
+  for (DevBuf* b : {&s.ie_src, &s.ie_dst, &s.ie_deg, &s.ie_wt}) b->ensure_slack(ie_m * 4 + 4);
   if (ie_m) {
-    skey_a.ensure(ie_m * 8); skey_b.ensure(ie_m * 8); sidx_a.ensure(ie_m * 4); sidx_b.ensure(ie_m * 4);
+    skey_a.ensure_slack(ie_m * 8); skey_b.ensure_slack(ie_m * 8); sidx_a.ensure_slack(ie_m * 4);
     k_internal_slot_keys<<<grid_for(ie_m), kBlock, 0, c->stream>>>(ie_m, s.iem_key.as<unsigned long long>(),
                                                                    s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(),
                                                                    s.lidx.as<uint32_t>(),
@@ -582,7 +586,7 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
   } else {
     if (s.pr_rows_for >= 0) {
       const size_t es = s.pr_rows_for == SUMINC_PR_FP64 ? 8 : 4;
-      s.sc_R.ensure(sc_len * es + es); s.sc_D.ensure(sc_len * es + es);
+      s.sc_R.ensure_slack(sc_len * es + es); s.sc_D.ensure_slack(sc_len * es + es);
       if (E && es == 8)
         k_rows_move<double><<<E, kBlock, 0, c->stream>>>(
             E, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(),
@@ -597,7 +601,7 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
             s.sc_D.as<float>());
     }
     if (s.ss_rows) {
-      s.ss_dist.ensure(sc_len * 4 + 4); s.ss_par.ensure(sc_len * 4 + 4);
+      s.ss_dist.ensure_slack(sc_len * 4 + 4); s.ss_par.ensure_slack(sc_len * 4 + 4);
       if (E)
         k_rows_move<uint32_t><<<E, kBlock, 0, c->stream>>>(
             E, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(),
@@ -922,8 +926,8 @@ static void compute_pr_rows(suminc_ctx* c) {
   const size_t es = sizeof(T);
   if (s.pr_rows_for != c->pr_prec) SUMINC_HIP(hipMemsetAsync(s.pr_valid.p, 0, (size_t)s.K + 1, c->stream));
   if (s.pr_rows_for != c->pr_prec || s.sc_R.cap < s.sc_len * es + es) {
-    s.sc_R.ensure(s.sc_len * es + es);
-    s.sc_D.ensure(s.sc_len * es + es);
+    s.sc_R.ensure_slack(s.sc_len * es + es);
+    s.sc_D.ensure_slack(s.sc_len * es + es);
   }
   DevBuf work;
   work.ensure(8);

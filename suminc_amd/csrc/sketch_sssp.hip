@@ -374,8 +374,8 @@ static void compute_sssp_rows(suminc_ctx* c) {
   if (!s.ss_need && s.ss_rows) return;
   if (!s.ss_rows) SUMINC_HIP(hipMemsetAsync(s.ss_valid.p, 0, (size_t)s.K + 1, c->stream));
   if (!s.ss_rows || s.ss_dist.cap < s.sc_len * 4 + 4) {
-    s.ss_dist.ensure(s.sc_len * 4 + 4);
-    s.ss_par.ensure(s.sc_len * 4 + 4);
+    s.ss_dist.ensure_slack(s.sc_len * 4 + 4);
+    s.ss_par.ensure_slack(s.sc_len * 4 + 4);
   }
   s.ss_work.ensure(8);
   SUMINC_HIP(hipMemsetAsync(s.ss_work.p, 0, 8, c->stream));
