@@ -222,9 +222,11 @@ int suminc_sum_pr_shortcut(suminc_ctx* ctx, uint32_t entry, uint32_t cap, uint32
  * tjn::g_function_compr + OutMirrorSyncToMaster + deltaSum and the host correction
  * (sum_sync_iter_worker.h:932-933, 1129-1146, 1385-1491). */
 int suminc_sum_pr_run(suminc_ctx* ctx, suminc_stats* stats);
-/* SSSP shortcut of one entry: its cluster's members (exits first), in-cluster
- * shortest distance from the entry and in-cluster parent per member (the entry
- * itself: 0 and SUMINC_INF_U32; unreachable: SUMINC_INF_U32 twice).  Replaces the
+/* SSSP shortcut of one entry: its cluster's members (exits first, count of them),
+ * and for each of the first nexit members (the exits) the in-cluster shortest
+ * distance from the entry and the in-cluster parent (the entry itself: 0 and
+ * SUMINC_INF_U32; unreachable: SUMINC_INF_U32 twice); slots past the exits are
+ * SUMINC_INF_U32 (the row keeps exits only).  Replaces the
  * traversal index of TravCompressor::build_trav_index_mirror /
  * fianl_build_trav_index_mirror (grape/fragment/trav_compressor.h:244-431). */
 int suminc_sum_sssp_shortcut(suminc_ctx* ctx, uint32_t entry, uint32_t cap, uint32_t* members, uint32_t* dist,

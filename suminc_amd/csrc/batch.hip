@@ -611,7 +611,7 @@ static void merge_direction(suminc_ctx* c, DevBuf& off, DevBuf& col, DevBuf& w, 
   if (c->m) {
     const uint32_t m = (uint32_t)c->m;
     const uint32_t ntiles = (m + kCopyTile - 1) / kCopyTile;
-    b.trow.ensure(((size_t)ntiles + 1) * 4);
+    b.trow.ensure_slack(((size_t)ntiles + 1) * 4);
     k_tile_rows<<<grid_for((uint64_t)ntiles + 1), kBlock, 0, c->stream>>>(off.as<uint32_t>(), n, m, ntiles,
                                                                          b.trow.as<uint32_t>());
     if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, prof_slot), c->stream));
@@ -747,7 +747,7 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
                        "): type must be add/delete, ids below n, integer weights at least 1");
   }
   // heavy-chunk list sized for the largest graph this batch can produce
-  if (c->pr_ready || c->sssp_ready) c->heavy.ensure((cap_m / kChunk + cap_m / kHeavyDeg + 2) * sizeof(uint4));
+  if (c->pr_ready || c->sssp_ready) c->heavy.ensure_slack((cap_m / kChunk + cap_m / kHeavyDeg + 2) * sizeof(uint4));
   unsigned* bc = b.bc.as<unsigned>();
   k_batch_zero<<<1, 64, 0, c->stream>>>(cnt);
   // An SSSP state that was initialised but never run is still the initial state:
@@ -898,7 +898,7 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   c->m = hbc[3];
   c->graph_version++;
   // per-round scratch sized by m may need to grow
-  if (c->pr_ready || c->sssp_ready) c->heavy.ensure((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
+  if (c->pr_ready || c->sssp_ready) c->heavy.ensure_slack((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
   c->stats.reserved[1] = (double)hbc[0];
   c->stats.reserved[2] = (double)hbc[1];
   c->stats.reserved[3] = (double)c->host_counters->aux[5];  // reseed edges

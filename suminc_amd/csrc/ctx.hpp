@@ -111,7 +111,8 @@ struct Sketch {
   uint32_t E = 0;            // entries (one supernode each)
   uint32_t M = 0;            // clustered vertices
   uint32_t Kraw = 0;         // raw clusters in the input assignment
-  uint64_t sc_len = 0;       // total length of the dense shortcut rows
+  uint64_t sc_len = 0;       // total length of the dense PageRank shortcut rows (one slot per member)
+  uint64_t ss_len = 0;       // total length of the SSSP shortcut rows (one slot per exit)
   uint64_t ie_m = 0;         // internal edges
   uint64_t iem_n = 0;        // live entries of the internal edge list
   uint32_t max_cluster = 0;  // largest accepted cluster
@@ -125,21 +126,23 @@ struct Sketch {
   DevBuf rlist;              // vertices with a raw cluster id (RM)
   DevBuf cid, ntype, lidx, is_entry, is_exit;
   DevBuf ext_out, ext_in;    // per clustered vertex: edges leaving / entering its cluster
-  DevBuf cl_off, cl_mem, cl_nexit, cl_eoff, cl_ent, ent_of, sc_base;
+  DevBuf cl_off, cl_mem, cl_nexit, cl_eoff, cl_ent, ent_of, sc_base, ss_base;
   // internal edges: the list sorted by (src, dst), kept up to date by every batch,
   // and its grouping by destination member slot for the in-cluster kernels
   DevBuf iem_key, iem_w, iem_dead, iem_key2, iem_w2;
   DevBuf ie_off, ie_src, ie_dst, ie_deg, ie_wt;
   DevBuf sc_R, sc_D;         // PageRank rows (reach mass / residual) per entry
-  DevBuf ss_dist, ss_par;    // SSSP rows (in-cluster distance / parent) per entry
+  DevBuf ss_dist, ss_par;    // SSSP rows (in-cluster distance / parent to every exit) per entry
   DevBuf bound, spnode;      // PageRank sketch accumulators
   DevBuf dirty;              // K: cluster touched since its last precompute
   DevBuf ss_dirty;           // K: SSSP precompute flags (clusters of frontier members)
-  DevBuf ss_work;            // u64: internal edges read by the SSSP shortcut kernels
+  DevBuf ss_work;            // u64 x2: internal edges read by the SSSP shortcut kernels, list length
+  DevBuf ss_list;            // K: clusters the next per-cluster kernel visits
   DevBuf pr_valid, ss_valid; // K: shortcut rows of the cluster are current
-  DevBuf sc_base_old, ent_of_old, sc_R_old, sc_D_old, ss_dist_old, ss_par_old;  // previous layout
+  DevBuf sc_base_old, ss_base_old, ent_of_old, sc_R_old, sc_D_old, ss_dist_old, ss_par_old;  // previous layout
   // scratch of the layout pass, kept between batches
   DevBuf lay_keys_a, lay_keys_b, lay_hist, lay_ckey, lay_entf, lay_epos, lay_ecount, lay_ekey, lay_rowlen, lay_mx,
+      lay_rowlen2,
       lay_skey_a, lay_skey_b, lay_sidx_a;
   uint32_t counts[8] = {0};  // node type histogram
 };

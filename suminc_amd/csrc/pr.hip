@@ -350,7 +350,7 @@ void pr_alloc(suminc_ctx* c) {
   c->wl0.ensure(n * 4 + 4);
   c->wl1.ensure(n * es + 8);
   // heavy chunks: at most m / kChunk + (number of heavy vertices) entries
-  c->heavy.ensure((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
+  c->heavy.ensure_slack((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
   c->partial.ensure(((size_t)grid_for(n) + 1) * 3 * 8);  // residual | edges | active vertices
   c->tmp4.ensure(65536);  // per-round scatter mode history
 }

@@ -344,17 +344,18 @@ __global__ void k_member_index(uint32_t M, const uint32_t* cl_mem, const uint32_
 }
 
 __global__ void k_entries(uint32_t E, const uint32_t* pos, const uint32_t* cl_mem, const uint32_t* cl_key,
-                          const uint32_t* cl_off, uint32_t* cl_ent, uint32_t* ent_key, uint32_t* ent_of,
-                          unsigned long long* rowlen) {
+                          const uint32_t* cl_off, const uint32_t* cl_nexit, uint32_t* cl_ent, uint32_t* ent_key,
+                          uint32_t* ent_of, unsigned long long* rowlen, unsigned long long* rowlen_exits) {
   uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e > E) return;
-  if (e == E) { rowlen[E] = 0; return; }
+  if (e == E) { rowlen[E] = 0; rowlen_exits[E] = 0; return; }
   uint32_t i = pos[e];
   uint32_t v = cl_mem[i], k = cl_key[i];
   cl_ent[e] = v;
   ent_key[e] = k;
   ent_of[v] = e;
   rowlen[e] = cl_off[k + 1] - cl_off[k];
+  rowlen_exits[e] = cl_nexit[k];
 }
 
 __global__ void k_ent_of_clear(uint32_t M, const uint32_t* cl_mem, uint32_t* ent_of) {
@@ -428,7 +429,8 @@ __global__ void k_max_cluster(uint32_t K, const uint32_t* cl_off, uint32_t* out)
 // layout: one workgroup per entry of the new layout.
 template <class T>
 __global__ void k_rows_move(uint32_t E, const uint32_t* __restrict__ cl_ent, const uint32_t* __restrict__ cid,
-                            const uint32_t* __restrict__ cl_off, const unsigned long long* __restrict__ sc_base,
+                            const uint32_t* __restrict__ cl_off, const uint32_t* __restrict__ cl_nexit,
+                            const unsigned long long* __restrict__ sc_base,
                             const uint32_t* __restrict__ old_ent_of,
                             const unsigned long long* __restrict__ old_sc_base, const uint8_t* __restrict__ valid,
                             const T* __restrict__ oldA, T* __restrict__ newA, const T* __restrict__ oldB,
@@ -438,7 +440,7 @@ __global__ void k_rows_move(uint32_t E, const uint32_t* __restrict__ cl_ent, con
   const uint32_t v = cl_ent[e], k = cid[v];
   const uint32_t oe = old_ent_of[v];
   if (!valid[k] || oe == INF32) return;
-  const uint32_t size = cl_off[k + 1] - cl_off[k];
+  const uint32_t size = cl_nexit ? cl_nexit[k] : cl_off[k + 1] - cl_off[k];
   const unsigned long long nb = sc_base[e], ob = old_sc_base[oe];
   for (uint32_t j = threadIdx.x; j < size; j += blockDim.x) {
     newA[nb + j] = oldA[ob + j];
@@ -450,7 +452,7 @@ __global__ void k_rows_move(uint32_t E, const uint32_t* __restrict__ cl_ent, con
 static void grow_keep(suminc_ctx* c, DevBuf& b, size_t bytes, size_t keep) {
   if (bytes <= b.cap && b.p) return;
   DevBuf t;
-  t.ensure(bytes);
+  t.ensure(bytes + bytes / 16);
   if (keep && b.p) SUMINC_HIP(hipMemcpyAsync(t.p, b.p, keep, hipMemcpyDeviceToDevice, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
   b.swap(t);
@@ -464,7 +466,8 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
   const uint32_t n = c->n, K = s.K, RM = s.RM;
   DevBuf &keys_a = s.lay_keys_a, &keys_b = s.lay_keys_b, &hist = s.lay_hist, &ckey = s.lay_ckey,
          &entf = s.lay_entf, &epos = s.lay_epos, &ecount = s.lay_ecount, &ekey = s.lay_ekey,
-         &rowlen = s.lay_rowlen, &mx = s.lay_mx, &skey_a = s.lay_skey_a, &skey_b = s.lay_skey_b,
+         &rowlen = s.lay_rowlen, &rowlen2 = s.lay_rowlen2, &mx = s.lay_mx, &skey_a = s.lay_skey_a,
+         &skey_b = s.lay_skey_b,
          &sidx_a = s.lay_sidx_a;
   keys_a.ensure((size_t)RM * 8 + 8); keys_b.ensure((size_t)RM * 8 + 8); hist.ensure(32);
   SUMINC_HIP(hipMemsetAsync(hist.p, 0, 32, c->stream));
@@ -489,6 +492,7 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
   // the previous layout is kept for moving untouched shortcut rows
   if (rebuild) {
     s.sc_base.swap(s.sc_base_old);
+    s.ss_base.swap(s.ss_base_old);
     s.ent_of.swap(s.ent_of_old);
     s.sc_R.swap(s.sc_R_old); s.sc_D.swap(s.sc_D_old);
     s.ss_dist.swap(s.ss_dist_old); s.ss_par.swap(s.ss_par_old);
@@ -525,22 +529,27 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
   const bool fresh_ent_of = !rebuild || !s.ent_of.p || s.ent_of.cap < (size_t)n * 4;
   s.ent_of.ensure((size_t)n * 4);
   s.cl_eoff.ensure(((size_t)K + 1) * 4); rowlen.ensure_slack(((size_t)E + 1) * 8);
-  s.sc_base.ensure_slack(((size_t)E + 1) * 8);
+  rowlen2.ensure_slack(((size_t)E + 1) * 8);
+  s.sc_base.ensure_slack(((size_t)E + 1) * 8); s.ss_base.ensure_slack(((size_t)E + 1) * 8);
   if (fresh_ent_of) k_fill32<<<grid_for(n), kBlock, 0, c->stream>>>(s.ent_of.as<uint32_t>(), n, INF32);
   else if (M) k_ent_of_clear<<<grid_for(M), kBlock, 0, c->stream>>>(M, s.cl_mem.as<uint32_t>(), s.ent_of.as<uint32_t>());
-  k_entries<<<grid_for((uint64_t)E + 1), kBlock, 0, c->stream>>>(E, epos.as<uint32_t>(), s.cl_mem.as<uint32_t>(),
-                                                                 ckey.as<uint32_t>(), s.cl_off.as<uint32_t>(),
-                                                                 s.cl_ent.as<uint32_t>(), ekey.as<uint32_t>(),
-                                                                 s.ent_of.as<uint32_t>(),
-                                                                 rowlen.as<unsigned long long>());
+  k_entries<<<grid_for((uint64_t)E + 1), kBlock, 0, c->stream>>>(
+      E, epos.as<uint32_t>(), s.cl_mem.as<uint32_t>(), ckey.as<uint32_t>(), s.cl_off.as<uint32_t>(),
+      s.cl_nexit.as<uint32_t>(), s.cl_ent.as<uint32_t>(), ekey.as<uint32_t>(), s.ent_of.as<uint32_t>(),
+      rowlen.as<unsigned long long>(), rowlen2.as<unsigned long long>());
   k_row_ptr32<uint32_t><<<grid_for((uint64_t)E + 1), kBlock, 0, c->stream>>>(ekey.as<uint32_t>(), E, K,
                                                                              s.cl_eoff.as<uint32_t>(), 0);
   cub(c, [&](void* p, size_t& b) {
     return rocprim::exclusive_scan(p, b, rowlen.as<unsigned long long>(), s.sc_base.as<unsigned long long>(), 0ull,
                                    (size_t)E + 1, rocprim::plus<unsigned long long>(), c->stream);
   });
-  unsigned long long sc_len = 0;
+  cub(c, [&](void* p, size_t& b) {
+    return rocprim::exclusive_scan(p, b, rowlen2.as<unsigned long long>(), s.ss_base.as<unsigned long long>(), 0ull,
+                                   (size_t)E + 1, rocprim::plus<unsigned long long>(), c->stream);
+  });
+  unsigned long long sc_len = 0, ss_len = 0;
   SUMINC_HIP(hipMemcpyAsync(&sc_len, s.sc_base.as<unsigned long long>() + E, 8, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipMemcpyAsync(&ss_len, s.ss_base.as<unsigned long long>() + E, 8, hipMemcpyDeviceToHost, c->stream));
   // internal edges grouped by destination slot
   const uint64_t ie_m = s.iem_n;
   s.ie_m = ie_m;
@@ -576,6 +585,7 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
   SUMINC_REQUIRE(maxc <= (uint32_t)kMaxCluster, SUMINC_E_TOO_LARGE, "an accepted cluster has more than 2048 members");
   s.max_cluster = maxc;
   s.sc_len = sc_len;
+  s.ss_len = ss_len;
   // shortcut rows: validity per cluster (PageRank, SSSP)
   s.pr_valid.ensure((size_t)K + 1); s.ss_valid.ensure((size_t)K + 1);
   if (!rebuild) {
@@ -589,23 +599,23 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
       s.sc_R.ensure_slack(sc_len * es + es); s.sc_D.ensure_slack(sc_len * es + es);
       if (E && es == 8)
         k_rows_move<double><<<E, kBlock, 0, c->stream>>>(
-            E, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(),
+            E, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(), nullptr,
             s.sc_base.as<unsigned long long>(), s.ent_of_old.as<uint32_t>(), s.sc_base_old.as<unsigned long long>(),
             s.pr_valid.as<uint8_t>(), s.sc_R_old.as<double>(), s.sc_R.as<double>(), s.sc_D_old.as<double>(),
             s.sc_D.as<double>());
       else if (E)
         k_rows_move<float><<<E, kBlock, 0, c->stream>>>(
-            E, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(),
+            E, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(), nullptr,
             s.sc_base.as<unsigned long long>(), s.ent_of_old.as<uint32_t>(), s.sc_base_old.as<unsigned long long>(),
             s.pr_valid.as<uint8_t>(), s.sc_R_old.as<float>(), s.sc_R.as<float>(), s.sc_D_old.as<float>(),
             s.sc_D.as<float>());
     }
     if (s.ss_rows) {
-      s.ss_dist.ensure_slack(sc_len * 4 + 4); s.ss_par.ensure_slack(sc_len * 4 + 4);
+      s.ss_dist.ensure_slack(ss_len * 4 + 4); s.ss_par.ensure_slack(ss_len * 4 + 4);
       if (E)
         k_rows_move<uint32_t><<<E, kBlock, 0, c->stream>>>(
-            E, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(),
-            s.sc_base.as<unsigned long long>(), s.ent_of_old.as<uint32_t>(), s.sc_base_old.as<unsigned long long>(),
+            E, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(), s.cl_nexit.as<uint32_t>(),
+            s.ss_base.as<unsigned long long>(), s.ent_of_old.as<uint32_t>(), s.ss_base_old.as<unsigned long long>(),
             s.ss_valid.as<uint8_t>(), s.ss_dist_old.as<uint32_t>(), s.ss_dist.as<uint32_t>(),
             s.ss_par_old.as<uint32_t>(), s.ss_par.as<uint32_t>());
     }
@@ -798,8 +808,8 @@ void sketch_batch(suminc_ctx* c, const uint32_t* rows, const uint32_t* cols, con
   grow_keep(c, s.iem_w, tot * 4 + 4, n0 * 4);
   grow_keep(c, s.iem_dead, tot + 1, n0);
   SUMINC_HIP(hipMemsetAsync(s.iem_dead.as<uint8_t>() + n0, 0, N + 1, c->stream));
-  s.iem_key2.ensure(tot * 8 + 8);
-  s.iem_w2.ensure(tot * 4 + 4);
+  s.iem_key2.ensure_slack(tot * 8 + 8);
+  s.iem_w2.ensure_slack(tot * 4 + 4);
   Counters* cnt = dev_counters(c);
   unsigned long long* add_cnt = &cnt->aux[7];
   unsigned long long* delta = &cnt->aux[8];

@@ -14,6 +14,7 @@
 // unchanged.  Exactness of parents: after the distances are final, one pass over
 // every member's internal in-edges min-reduces (dist + w, src) into the member,
 // so every vertex ends with the smallest-id predecessor on a shortest path.
+#include <algorithm>
 #include <cstring>
 
 #include "ctx.hpp"
@@ -58,39 +59,59 @@ __device__ uint32_t cluster_bf(uint32_t size, uint32_t pb, uint32_t pe, const ui
   return rounds;
 }
 
+// Clusters whose flag equals `want`, appended in any order to list (count on the
+// device): the per-cluster kernels below loop over such a list with a fixed grid.
+__global__ void k_cluster_list(uint32_t K, const uint8_t* __restrict__ flag, uint8_t want, uint32_t* list,
+                               unsigned long long* count) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool take = k < K && flag[k] == want;
+  const unsigned long long slot = wave_append(take, count);
+  if (take) list[slot] = k;
+}
+
+constexpr unsigned kClusterGrid = 2048;
+
 // Shortcut rows: for every entry e of cluster k, in-cluster distance and parent of
-// every member (the entry itself: distance 0, parent INF = no candidate).
+// every exit of k (the first nexit members; the entry itself if it is an exit:
+// distance 0, parent INF = no candidate).  Only the exits are kept: the sketch
+// rounds need entry -> exit distances, and the correction re-runs the in-cluster
+// relaxation from the entries' final distances instead of reading rows.
 __global__ void __launch_bounds__(kBlock) k_ss_shortcuts(const uint32_t* __restrict__ cl_off,
+                                                         const uint32_t* __restrict__ cl_nexit,
                                                          const uint32_t* __restrict__ cl_mem,
                                                          const uint32_t* __restrict__ cl_eoff,
                                                          const uint32_t* __restrict__ cl_ent,
                                                          const uint32_t* __restrict__ lidx,
-                                                         const unsigned long long* __restrict__ sc_base,
+                                                         const unsigned long long* __restrict__ ss_base,
                                                          Internal ie, uint32_t* __restrict__ sd,
-                                                         uint32_t* __restrict__ sp, const uint8_t* __restrict__ valid,
+                                                         uint32_t* __restrict__ sp, const uint32_t* __restrict__ list,
+                                                         const unsigned long long* __restrict__ count,
                                                          unsigned long long* work) {
   __shared__ unsigned long long s_dp[kMaxClusterS];
   __shared__ uint32_t s_id[kMaxClusterS];
   __shared__ int s_flag;
-  const uint32_t k = blockIdx.x;
-  if (valid[k]) return;
-  const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0;
+  const uint32_t nk = (uint32_t)*count;
+  unsigned long long edges = 0;
+  for (uint32_t li = blockIdx.x; li < nk; li += gridDim.x) {
+  const uint32_t k = list[li];
+  const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0, nexit = cl_nexit[k];
   const uint32_t pb = ie.off[slot0], pe = ie.off[slot0 + size];
   for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_id[j] = cl_mem[slot0 + j];
-  unsigned long long edges = 0;
   for (uint32_t e = cl_eoff[k]; e < cl_eoff[k + 1]; ++e) {
     const uint32_t src = lidx[cl_ent[e]];
     for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_dp[j] = j == src ? 0xFFFFFFFFull : kInf64;
     __syncthreads();
     uint32_t rounds = cluster_bf(size, pb, pe, ie.src, ie.dst, ie.wt, s_id, s_dp, &s_flag);
     edges += (unsigned long long)rounds * (pe - pb);
-    const unsigned long long base = sc_base[e];
-    for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+    const unsigned long long base = ss_base[e];
+    for (uint32_t j = threadIdx.x; j < nexit; j += kBlock) {
       const unsigned long long x = s_dp[j];
       sd[base + j] = (uint32_t)(x >> 32);
       sp[base + j] = (uint32_t)x;
     }
     __syncthreads();
+  }
+  __syncthreads();  // s_id of the next cluster
   }
   if (threadIdx.x == 0 && edges) atomicAdd(work, edges);
 }
@@ -105,7 +126,7 @@ struct SsArgs {
   const uint32_t* cl_off;
   const uint32_t* cl_mem;
   const uint32_t* cl_nexit;
-  const unsigned long long* sc_base;
+  const unsigned long long* ss_base;
   const uint32_t* sd;
   const uint32_t* sp;
   unsigned long long* dp;
@@ -138,14 +159,22 @@ __global__ void k_ss_mark(SsArgs a, uint32_t src, int mark_src) {
   if (mark_src && blockIdx.x == 0 && threadIdx.x == 0 && a.cid[src] != INF32) a.dirty[a.cid[src]] = 1;
 }
 
-// Precompute of dirty clusters: in-cluster relaxation from the members' current
-// state; improved exits join the frontier (list 0).
-__global__ void __launch_bounds__(kBlock) k_ss_precompute(SsArgs a, Internal ie, unsigned long long* work) {
+// In-cluster relaxation of the listed clusters from the members' current state.
+// Precompute (final = 0): improved exits join the frontier (list 0).
+// Correction (final = 1, after the sketch rounds): the entries and exits hold their
+// final distances, so the relaxation gives every member its distance, and the
+// packed {dist, parent} minimum over the internal in-edges (together with what the
+// external relaxations left in the member's word) its smallest-id parent.
+__global__ void __launch_bounds__(kBlock) k_ss_relax_clusters(SsArgs a, Internal ie,
+                                                              const uint32_t* __restrict__ list,
+                                                              const unsigned long long* __restrict__ count,
+                                                              unsigned long long* work, int final_pass) {
   __shared__ unsigned long long s_dp[kMaxClusterS];
   __shared__ uint32_t s_id[kMaxClusterS];
   __shared__ int s_flag;
-  const uint32_t k = blockIdx.x;
-  if (!a.dirty[k]) return;
+  const uint32_t nk = (uint32_t)*count;
+  for (uint32_t li = blockIdx.x; li < nk; li += gridDim.x) {
+  const uint32_t k = list[li];
   const uint32_t slot0 = a.cl_off[k], size = a.cl_off[k + 1] - slot0, nexit = a.cl_nexit[k];
   const uint32_t pb = ie.off[slot0], pe = ie.off[slot0 + size];
   for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
@@ -164,12 +193,17 @@ __global__ void __launch_bounds__(kBlock) k_ss_precompute(SsArgs a, Internal ie,
       const unsigned long long x = s_dp[j];
       if (x < a.dp[v]) {
         a.dp[v] = x;
-        improved = j < nexit && (x >> 32) != 0xFFFFFFFFull;
+        improved = !final_pass && j < nexit && (x >> 32) != 0xFFFFFFFFull;
       }
     }
     ss_append(a, 0, v, improved);
   }
-  if (threadIdx.x == 0) atomicAdd(work, (unsigned long long)rounds * (pe - pb));
+  if (threadIdx.x == 0) {
+    atomicAdd(work, (unsigned long long)rounds * (pe - pb));
+    if (final_pass) a.dirty[k] = 0;
+  }
+  __syncthreads();  // LDS of the next cluster
+  }
 }
 
 __device__ __forceinline__ void ss_relax(const SsArgs& a, int nxt, uint32_t v, unsigned long long nd, uint32_t parent) {
@@ -217,7 +251,7 @@ __global__ void __launch_bounds__(kBlock) k_ss_push(SsArgs a) {
           a.dirty[k] = 1;  // the correction revisits this cluster
           sclen = a.cl_nexit[k];
           s_mbase[threadIdx.x] = a.cl_off[k];
-          s_rbase[threadIdx.x] = a.sc_base[a.ent_of[u]];
+          s_rbase[threadIdx.x] = a.ss_base[a.ent_of[u]];
         }
         if (iblen > kHeavyDeg) {  // long rows go to 2048-edge chunks
           const uint32_t nch = (iblen + kChunk - 1) / kChunk;
@@ -294,54 +328,6 @@ __global__ void k_ss_end(Counters* cnt, int max_rounds) {
 // distance (parent = in-cluster predecessor); (b) with final distances, every
 // member min-reduces (dist(src) + w, src) over its internal in-edges, which fixes
 // equal-distance parents to the smallest id.
-__global__ void __launch_bounds__(kBlock) k_ss_correct(SsArgs a, const uint32_t* __restrict__ cl_eoff,
-                                                       const uint32_t* __restrict__ cl_ent, Internal ie) {
-  __shared__ unsigned long long s_dp[kMaxClusterS], s_best[kMaxClusterS];
-  __shared__ uint32_t s_id[kMaxClusterS];
-  const uint32_t k = blockIdx.x;
-  if (!a.dirty[k]) return;  // no member or entry of this cluster moved in this run
-  const uint32_t slot0 = a.cl_off[k], size = a.cl_off[k + 1] - slot0;
-  for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
-    const uint32_t v = a.cl_mem[slot0 + j];
-    s_id[j] = v;
-    s_dp[j] = a.dp[v];
-  }
-  __syncthreads();
-  for (uint32_t e = cl_eoff[k]; e < cl_eoff[k + 1]; ++e) {
-    const unsigned long long ds = a.dp[cl_ent[e]] >> 32;
-    if (ds == 0xFFFFFFFFull) continue;  // uniform across the block
-    const unsigned long long base = a.sc_base[e];
-    for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
-      const uint32_t dj = a.sd[base + j], pj = a.sp[base + j];
-      if (dj == INF32 || pj == INF32) continue;
-      const unsigned long long nd = ds + dj;
-      if (nd >= 0xFFFFFFFFull) continue;
-      const unsigned long long cand = (nd << 32) | pj;
-      if (cand < s_dp[j]) s_dp[j] = cand;  // one writer per slot in this loop
-    }
-  }
-  __syncthreads();
-  for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_best[j] = s_dp[j];
-  __syncthreads();
-  const uint32_t pb = ie.off[slot0], pe = ie.off[slot0 + size];
-  for (uint32_t p = pb + threadIdx.x; p < pe; p += kBlock) {
-    const uint32_t i = ie.src[p];
-    const unsigned long long di = s_dp[i] >> 32;
-    if (di == 0xFFFFFFFFull) continue;
-    const unsigned long long nd = di + ie.wt[p];
-    if (nd >= 0xFFFFFFFFull) continue;
-    const unsigned long long cand = (nd << 32) | s_id[i];
-    const uint32_t j = ie.dst[p];
-    if (cand < s_best[j]) atomicMin(&s_best[j], cand);
-  }
-  __syncthreads();
-  for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
-    const uint32_t v = s_id[j];
-    if (s_best[j] < a.dp[v]) a.dp[v] = s_best[j];
-  }
-  if (threadIdx.x == 0) a.dirty[k] = 0;
-}
-
 static SsArgs ss_args(suminc_ctx* c) {
   Sketch& s = c->sk;
   SsArgs a;
@@ -354,7 +340,7 @@ static SsArgs ss_args(suminc_ctx* c) {
   a.cl_off = s.cl_off.as<uint32_t>();
   a.cl_mem = s.cl_mem.as<uint32_t>();
   a.cl_nexit = s.cl_nexit.as<uint32_t>();
-  a.sc_base = s.sc_base.as<unsigned long long>();
+  a.ss_base = s.ss_base.as<unsigned long long>();
   a.sd = s.ss_dist.as<uint32_t>();
   a.sp = s.ss_par.as<uint32_t>();
   a.dp = c->sssp_dp.as<unsigned long long>();
@@ -373,19 +359,23 @@ static void compute_sssp_rows(suminc_ctx* c) {
   Sketch& s = c->sk;
   if (!s.ss_need && s.ss_rows) return;
   if (!s.ss_rows) SUMINC_HIP(hipMemsetAsync(s.ss_valid.p, 0, (size_t)s.K + 1, c->stream));
-  if (!s.ss_rows || s.ss_dist.cap < s.sc_len * 4 + 4) {
-    s.ss_dist.ensure_slack(s.sc_len * 4 + 4);
-    s.ss_par.ensure_slack(s.sc_len * 4 + 4);
+  if (!s.ss_rows || s.ss_dist.cap < s.ss_len * 4 + 4) {
+    s.ss_dist.ensure_slack(s.ss_len * 4 + 4);
+    s.ss_par.ensure_slack(s.ss_len * 4 + 4);
   }
-  s.ss_work.ensure(8);
-  SUMINC_HIP(hipMemsetAsync(s.ss_work.p, 0, 8, c->stream));
-  if (s.K)
-    k_ss_shortcuts<<<s.K, kBlock, 0, c->stream>>>(s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(),
-                                                  s.cl_eoff.as<uint32_t>(), s.cl_ent.as<uint32_t>(),
-                                                  s.lidx.as<uint32_t>(), s.sc_base.as<unsigned long long>(),
-                                                  internal_of(s),
-                                                  s.ss_dist.as<uint32_t>(), s.ss_par.as<uint32_t>(),
-                                                  s.ss_valid.as<uint8_t>(), s.ss_work.as<unsigned long long>());
+  s.ss_work.ensure(16);
+  s.ss_list.ensure((size_t)s.K * 4 + 4);
+  SUMINC_HIP(hipMemsetAsync(s.ss_work.p, 0, 16, c->stream));
+  unsigned long long* lcount = s.ss_work.as<unsigned long long>() + 1;
+  if (s.K) {
+    k_cluster_list<<<grid_for(s.K), kBlock, 0, c->stream>>>(s.K, s.ss_valid.as<uint8_t>(), 0, s.ss_list.as<uint32_t>(),
+                                                            lcount);
+    k_ss_shortcuts<<<std::min(s.K, kClusterGrid), kBlock, 0, c->stream>>>(
+        s.cl_off.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.cl_eoff.as<uint32_t>(),
+        s.cl_ent.as<uint32_t>(), s.lidx.as<uint32_t>(), s.ss_base.as<unsigned long long>(), internal_of(s),
+        s.ss_dist.as<uint32_t>(),
+        s.ss_par.as<uint32_t>(), s.ss_list.as<uint32_t>(), lcount, s.ss_work.as<unsigned long long>());
+  }
   SUMINC_HIP(hipGetLastError());
   SUMINC_HIP(hipMemsetAsync(s.ss_valid.p, 1, (size_t)s.K + 1, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
@@ -416,16 +406,21 @@ void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
   c->sssp_pending = false;
   SsArgs a = ss_args(c);
   Counters* cnt = a.cnt;
-  s.ss_work.ensure(8);
-  SUMINC_HIP(hipMemsetAsync(s.ss_work.p, 0, 8, c->stream));
+  s.ss_work.ensure(16);
+  s.ss_list.ensure((size_t)s.K * 4 + 4);
+  SUMINC_HIP(hipMemsetAsync(s.ss_work.p, 0, 16, c->stream));
+  unsigned long long* lcount = s.ss_work.as<unsigned long long>() + 1;
   SUMINC_HIP(hipMemsetAsync(s.ss_dirty.p, 0, (size_t)s.K + 1, c->stream));
   SUMINC_HIP(hipEventRecord(c->ev_start, c->stream));
   // frontier list 0 holds the seeds (source after init, repair seeds after a batch)
   k_ss_begin<<<1, 64, 0, c->stream>>>(cnt);
   k_ss_mark<<<256, kBlock, 0, c->stream>>>(a, c->sssp_src, first ? 1 : 0);
-  if (s.K)
-    k_ss_precompute<<<s.K, kBlock, 0, c->stream>>>(a, internal_of(s),
-                                                   s.ss_work.as<unsigned long long>());
+  if (s.K) {
+    k_cluster_list<<<grid_for(s.K), kBlock, 0, c->stream>>>(s.K, s.ss_dirty.as<uint8_t>(), 1,
+                                                            s.ss_list.as<uint32_t>(), lcount);
+    k_ss_relax_clusters<<<std::min(s.K, kClusterGrid), kBlock, 0, c->stream>>>(
+        a, internal_of(s), s.ss_list.as<uint32_t>(), lcount, s.ss_work.as<unsigned long long>(), 0);
+  }
   SUMINC_HIP(hipGetLastError());
   int group = 8;
   size_t slot = 0;
@@ -458,9 +453,13 @@ void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
     p.vertices += (double)h->aux[6];
     p.alg_bytes += 16.0 * (edges - sc) + 20.0 * sc + 16.0 * (double)h->aux[6];
   }
-  if (s.K)
-    k_ss_correct<<<s.K, kBlock, 0, c->stream>>>(a, s.cl_eoff.as<uint32_t>(), s.cl_ent.as<uint32_t>(),
-                                                internal_of(s));
+  if (s.K) {
+    SUMINC_HIP(hipMemsetAsync(lcount, 0, 8, c->stream));
+    k_cluster_list<<<grid_for(s.K), kBlock, 0, c->stream>>>(s.K, s.ss_dirty.as<uint8_t>(), 1,
+                                                            s.ss_list.as<uint32_t>(), lcount);
+    k_ss_relax_clusters<<<std::min(s.K, kClusterGrid), kBlock, 0, c->stream>>>(
+        a, internal_of(s), s.ss_list.as<uint32_t>(), lcount, s.ss_work.as<unsigned long long>(), 1);
+  }
   SUMINC_HIP(hipGetLastError());
   SUMINC_HIP(hipEventRecord(c->ev_stop, c->stream));
   SUMINC_HIP(hipEventSynchronize(c->ev_stop));
@@ -501,14 +500,21 @@ void sketch_sssp_shortcut(suminc_ctx* c, uint32_t v, uint32_t cap, uint32_t* mem
   unsigned long long base = 0;
   SUMINC_HIP(hipMemcpy(co, s.cl_off.as<uint32_t>() + k, 8, hipMemcpyDeviceToHost));
   SUMINC_HIP(hipMemcpy(&ne, s.cl_nexit.as<uint32_t>() + k, 4, hipMemcpyDeviceToHost));
-  SUMINC_HIP(hipMemcpy(&base, s.sc_base.as<unsigned long long>() + e, 8, hipMemcpyDeviceToHost));
+  SUMINC_HIP(hipMemcpy(&base, s.ss_base.as<unsigned long long>() + e, 8, hipMemcpyDeviceToHost));
   const uint32_t size = co[1] - co[0];
   if (count) *count = size;
   if (nexit) *nexit = ne;
   SUMINC_REQUIRE(cap >= size || !(members || dist || parent), SUMINC_E_INVALID, "output capacity too small");
   if (members) SUMINC_HIP(hipMemcpy(members, s.cl_mem.as<uint32_t>() + co[0], (size_t)size * 4, hipMemcpyDeviceToHost));
-  if (dist) SUMINC_HIP(hipMemcpy(dist, s.ss_dist.as<uint32_t>() + base, (size_t)size * 4, hipMemcpyDeviceToHost));
-  if (parent) SUMINC_HIP(hipMemcpy(parent, s.ss_par.as<uint32_t>() + base, (size_t)size * 4, hipMemcpyDeviceToHost));
+  // rows hold the exits (the first nexit members) only
+  if (dist) {
+    for (uint32_t j = ne; j < size; ++j) dist[j] = INF32;
+    if (ne) SUMINC_HIP(hipMemcpy(dist, s.ss_dist.as<uint32_t>() + base, (size_t)ne * 4, hipMemcpyDeviceToHost));
+  }
+  if (parent) {
+    for (uint32_t j = ne; j < size; ++j) parent[j] = INF32;
+    if (ne) SUMINC_HIP(hipMemcpy(parent, s.ss_par.as<uint32_t>() + base, (size_t)ne * 4, hipMemcpyDeviceToHost));
+  }
 }
 
 }  // namespace suminc

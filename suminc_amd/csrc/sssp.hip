@@ -172,7 +172,7 @@ void sssp_alloc(suminc_ctx* c) {
   c->sssp_flag1.ensure(words * 4);
   c->wl0.ensure(n * 4 + 4);
   c->tmp5.ensure(n * 4 + 4);
-  c->heavy.ensure((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
+  c->heavy.ensure_slack((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
 }
 
 void sssp_clear_flags(suminc_ctx* c) {

@@ -172,9 +172,11 @@ def test_sssp_shortcut_rows_equal_oracle(gpu, weights):
         for s in entries[:40]:
             mem, dist, par, nexit = ctx.sum_sssp_shortcut(int(s))
             exp = g.sum_sssp_shortcut(cid, int(s))
-            assert np.array_equal(dist, exp[mem]), int(s)
+            # rows keep the exits (the first nexit members)
+            assert np.array_equal(dist[:nexit], exp[mem[:nexit]]), int(s)
+            assert np.all(dist[nexit:] == INF) and np.all(par[nexit:] == INF)
             # parent: smallest in-cluster predecessor on a shortest in-cluster path
-            for j, x in enumerate(mem):
+            for j, x in enumerate(mem[:nexit]):
                 if x == s or exp[x] == INF:
                     assert par[j] == INF
                     continue
@@ -287,8 +289,8 @@ def test_rows_follow_small_batches(gpu):
             cid, typ = ctx.sum_download()
             entries = np.nonzero((typ == 2) | (typ == 3))[0]
             for s in entries[::max(1, len(entries) // 40)]:
-                mem, sd, _, _ = ctx.sum_sssp_shortcut(int(s))
-                assert np.array_equal(sd, g.sum_sssp_shortcut(cid, int(s))[mem]), (b, int(s))
+                mem, sd, _, ne = ctx.sum_sssp_shortcut(int(s))
+                assert np.array_equal(sd[:ne], g.sum_sssp_shortcut(cid, int(s))[mem[:ne]]), (b, int(s))
                 mem, reach, _, _ = ctx.sum_pr_shortcut(int(s))
                 omem, x = shortcut_dense(g, cid, int(s), DAMPING)
                 order = np.argsort(mem)
