@@ -121,6 +121,10 @@ int suminc_generate_rmat(suminc_ctx* ctx, int scale, int edge_factor, double a, 
                          double c, uint64_t seed, int weight_kind, uint32_t wmin, uint32_t wmax);
 int suminc_graph_info(suminc_ctx* ctx, uint32_t* n, uint64_t* m, int* weight_kind);
 int suminc_download_csr(suminc_ctx* ctx, uint64_t* offsets, uint32_t* cols, void* weights);
+/* The in-adjacency (CSC) the device keeps beside the CSR: sources of each vertex's
+ * in-edges, sorted, with their weights (the fragment's incoming adjacency,
+ * GetIncomingAdjList in immutable_edgecut_fragment.h). */
+int suminc_download_csc(suminc_ctx* ctx, uint64_t* offsets, uint32_t* rows, void* weights);
 
 /* ---- update batches ------------------------------------------------------------
  * Seeded batch: n_del deletes of existing edges, then n_ins inserts with uniform

@@ -130,6 +130,7 @@ SIGNATURES = {
     "suminc_generate_rmat": (i32, [vp, i32, i32, f64, f64, f64, u64, i32, u32, u32]),
     "suminc_graph_info": (i32, [vp, C.POINTER(u32), C.POINTER(u64), C.POINTER(i32)]),
     "suminc_download_csr": (i32, [vp, vp, vp, vp]),
+    "suminc_download_csc": (i32, [vp, vp, vp, vp]),
     "suminc_generate_batch": (i32, [vp, u64, u32, u64, u64, u32, u32, vp]),
     "suminc_apply_batch": (i32, [vp, vp, u64]),
     "suminc_apply_batch_device": (i32, [vp, vp, u64]),

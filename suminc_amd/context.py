@@ -127,6 +127,18 @@ class Context:
         self._check(self._lib.suminc_download_csr(self._h, L.ptr(off), L.ptr(col), L.ptr(w)))
         return off, col, w
 
+    def download_csc(self):
+        n, m, kind = self.graph_info()
+        off = np.zeros(n + 1, np.uint64)
+        row = np.zeros(m, np.uint32)
+        w = None
+        if kind == L.W_U32:
+            w = np.zeros(m, np.uint32)
+        elif kind == L.W_F32:
+            w = np.zeros(m, np.float32)
+        self._check(self._lib.suminc_download_csc(self._h, L.ptr(off), L.ptr(row), L.ptr(w)))
+        return off, row, w
+
     # ---- batches --------------------------------------------------------------
     def generate_batch(self, seed, batch_id, n_del, n_ins, wmin=1, wmax=64):
         ops = np.zeros(n_del + n_ins, L.UPDATE_DTYPE)

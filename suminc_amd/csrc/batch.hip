@@ -19,6 +19,7 @@
 #include <cstring>
 #include <rocprim/rocprim.hpp>
 
+#include <algorithm>
 #include <vector>
 
 #include "ctx.hpp"
@@ -120,6 +121,7 @@ __global__ void k_group_heads(const uint64_t* skeys, const uint32_t* sidx, const
 // transposed view for the CSC (row=v, col=u).  Sizes live on the device.
 struct Groups {
   const unsigned* countp;   // number of groups G
+  const uint32_t* pos;      // position of the pair in the old edge array (or its insertion point)
   const uint32_t* row;
   const uint32_t* col;
   const uint8_t* act;       // 0 add, 1 delete
@@ -141,8 +143,11 @@ __global__ void k_split_rec(const uint4* rec, const unsigned* bc, uint32_t N, ui
 }
 
 // existence probe in the old out-CSR; also records the old weight bits
+// existence probe in the old adjacency of one direction; also records the old
+// weight bits (ex / eold_w may be null) and the pair's position in the old edge
+// array (its slot if it exists, else the slot it is inserted before)
 __global__ void k_probe(const uint32_t* off, const uint32_t* colv, const uint32_t* wv, const uint32_t* grow,
-                        const uint32_t* gcol, const unsigned* bc, uint8_t* ex, uint32_t* eold_w) {
+                        const uint32_t* gcol, const unsigned* bc, uint8_t* ex, uint32_t* eold_w, uint32_t* pos) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= bc[0]) return;
   uint32_t u = grow[g], v = gcol[g];
@@ -152,8 +157,9 @@ __global__ void k_probe(const uint32_t* off, const uint32_t* colv, const uint32_
     if (colv[mid] < v) lo = mid + 1; else hi = mid;
   }
   bool found = lo < off[u + 1] && colv[lo] == v;
-  ex[g] = found;
-  eold_w[g] = (found && wv) ? wv[lo] : 0u;
+  if (ex) ex[g] = found;
+  if (eold_w) eold_w[g] = (found && wv) ? wv[lo] : 0u;
+  pos[g] = lo;
 }
 
 __global__ void k_flags_u32(const uint8_t* ex, const uint8_t* act, const unsigned* bc, uint32_t N, uint32_t* exu,
@@ -209,75 +215,133 @@ __global__ void k_new_deg(const uint32_t* off, uint32_t n, Groups gs, uint32_t* 
   ndeg[v] = d;
 }
 
-// Copy surviving old edges to their new positions.  The old edge array is cut
-// into tiles of kCopyTile edges, one workgroup each; the rows a tile spans are
-// found once (k_tile_rows) and their offsets staged in LDS, so an edge finds its
-// row by a short LDS search.  Edges of untouched rows move by their row's shift;
-// in touched rows the row's groups give the number of deleted pairs and adds
-// before the edge.
+// Copy surviving old edges to their new positions.  Between two consecutive group
+// positions the destination is the source shifted by a constant: an old edge e
+// moves to e - (existing pairs grouped before e) + (adds grouped at or before e),
+// and is dropped when its own pair is grouped as existing (deleted, or re-added
+// with a new weight, which k_place_adds writes).  Groups are sorted by (row, col),
+// so their positions are non-decreasing; each workgroup copies a tile of
+// kCopyTile edges, finds its first group once and stages the tile's groups in LDS.
 constexpr uint32_t kCopyTile = 4096;
-constexpr uint32_t kCopyRows = 4096;  // LDS-staged row offsets per tile
+constexpr uint32_t kCopyStage = 2048;  // LDS-staged groups per tile
 
-// row of the first edge of every tile, and of the last edge (entry ntiles)
-__global__ void k_tile_rows(const uint32_t* off, uint32_t n, uint32_t m, uint32_t ntiles, uint32_t* trow) {
+// first group of every tile (groups with pos >= t * kCopyTile); entry ntiles = G
+__global__ void k_tile_groups(const unsigned* countp, const uint32_t* __restrict__ pos, uint32_t ntiles,
+                              uint32_t* tstart) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t > ntiles) return;
-  const uint32_t e = t < ntiles ? t * kCopyTile : m - 1;
-  uint32_t lo = 0, hi = n - 1;  // last row with off[row] <= e
+  const uint32_t G = *countp;
+  if (t == ntiles) { tstart[t] = G; return; }
+  const uint32_t e0 = t * kCopyTile;
+  uint32_t lo = 0, hi = G;
   while (lo < hi) {
-    const uint32_t mid = lo + ((hi - lo + 1) >> 1);
-    if (off[mid] <= e) lo = mid; else hi = mid - 1;
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pos[mid] < e0) lo = mid + 1; else hi = mid;
   }
-  trow[t] = lo;
+  tstart[t] = lo;
 }
 
-__global__ void __launch_bounds__(kBlock) k_copy_old(uint32_t m, const uint32_t* __restrict__ off,
-                                                     const uint32_t* __restrict__ colv, const uint32_t* __restrict__ wv,
-                                                     const uint32_t* __restrict__ noff, uint32_t* __restrict__ ncol,
-                                                     uint32_t* __restrict__ nw, const uint32_t* __restrict__ trow,
-                                                     uint32_t ntiles, Groups gs) {
-  __shared__ uint32_t s_off[kCopyRows + 1];
-  const uint32_t t = blockIdx.x;
-  const uint32_t e0 = t * kCopyTile, e1 = (uint32_t)min((uint64_t)m, (uint64_t)e0 + kCopyTile);
-  const uint32_t r0 = trow[t];
-  const uint32_t r1 = t + 1 < ntiles ? trow[t + 1] : trow[ntiles];  // last row a tile edge can be in
-  const uint32_t span = r1 - r0 + 1;
-  const bool staged = span <= kCopyRows;
-  if (staged)
-    for (uint32_t j = threadIdx.x; j <= span; j += kBlock) s_off[j] = off[r0 + j];
+// Position bookkeeping of one old edge e inside a tile: i = the tile's groups with
+// pos < e, k = those with pos <= e (the caller keeps them for the next edge).
+struct TileGroups {
+  const uint32_t* s_pos;
+  const uint32_t* s_ex;   // ex_pre | (pair exists) << 31
+  const uint32_t* s_add;  // add_pre + (group is an add)
+  uint32_t cnt, ex_base, add_base;
+  __device__ __forceinline__ void seek(uint32_t e, uint32_t& i) const {
+    uint32_t lo = i, hi = cnt;  // i only moves forward
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_pos[mid] < e) lo = mid + 1; else hi = mid;
+    }
+    i = lo;
+  }
+  // destination of e, or INF32 when e's pair is grouped as existing
+  __device__ __forceinline__ uint32_t dest(uint32_t e, uint32_t i) const {
+    uint32_t k = i;
+    bool dropped = false;
+    while (k < cnt && s_pos[k] == e) {
+      dropped |= (s_ex[k] >> 31) != 0;
+      ++k;
+    }
+    if (dropped) return INF32;
+    const uint32_t drops = i ? (s_ex[i - 1] & 0x7FFFFFFFu) + (s_ex[i - 1] >> 31) : ex_base;
+    const uint32_t adds = k ? s_add[k - 1] : add_base;
+    return e - drops + adds;
+  }
+};
+
+__global__ void __launch_bounds__(kBlock) k_copy_old(uint32_t m, const uint32_t* __restrict__ colv,
+                                                     const uint32_t* __restrict__ wv, uint32_t* __restrict__ ncol,
+                                                     uint32_t* __restrict__ nw, const uint32_t* __restrict__ tstart,
+                                                     Groups gs) {
+  __shared__ uint32_t s_pos[kCopyStage];
+  __shared__ uint32_t s_ex[kCopyStage];
+  __shared__ uint32_t s_add[kCopyStage];
+  const uint32_t e0 = blockIdx.x * kCopyTile;
+  const uint32_t e1 = (uint32_t)min((uint64_t)m, (uint64_t)e0 + kCopyTile);
+  const uint32_t j0 = tstart[blockIdx.x], cnt = tstart[blockIdx.x + 1] - j0;
+  if (cnt > kCopyStage) {
+    // a tile with more groups than fit in LDS: one edge per thread, global reads
+    for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
+      uint32_t lo = 0, hi = cnt;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (gs.pos[j0 + mid] < e) lo = mid + 1; else hi = mid;
+      }
+      uint32_t k = lo;
+      bool dropped = false;
+      while (k < cnt && gs.pos[j0 + k] == e) dropped |= gs.ex[j0 + k++] != 0;
+      if (dropped) continue;
+      const uint32_t dst = e - gs.ex_pre[j0 + lo] + gs.add_pre[j0 + k];
+      ncol[dst] = colv[e];
+      if (nw) nw[dst] = wv[e];
+    }
+    return;
+  }
+  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
+    const uint32_t g = j0 + i;
+    s_pos[i] = gs.pos[g];
+    s_ex[i] = gs.ex_pre[g] | (gs.ex[g] ? 0x80000000u : 0u);
+    s_add[i] = gs.add_pre[g] + (gs.act[g] == 0 ? 1u : 0u);
+  }
   __syncthreads();
-  for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
-    uint32_t lo = 0, hi = span - 1;  // last row (relative to r0) starting at or before e
-    if (staged) {
-      while (lo < hi) {
-        const uint32_t mid = lo + ((hi - lo + 1) >> 1);
-        if (s_off[mid] <= e) lo = mid; else hi = mid - 1;
-      }
+  const TileGroups T{s_pos, s_ex, s_add, cnt, gs.ex_pre[j0], gs.add_pre[j0]};
+  // every thread copies runs of 4 consecutive edges with 16-byte loads; all loads
+  // of the tile are issued before the first store
+  constexpr int kRuns = kCopyTile / (kBlock * 4);
+  uint4 c4[kRuns], w4[kRuns];
+#pragma unroll
+  for (int r = 0; r < kRuns; ++r) {
+    const uint32_t e = e0 + 4 * (r * kBlock + threadIdx.x);
+    if (e + 3 < e1) {
+      c4[r] = *reinterpret_cast<const uint4*>(colv + e);
+      if (nw) w4[r] = *reinterpret_cast<const uint4*>(wv + e);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kRuns; ++r) {
+    const uint32_t e = e0 + 4 * (r * kBlock + threadIdx.x);
+    if (e >= e1) continue;
+    uint32_t cv[4], wvv[4];
+    const uint32_t len = min(4u, e1 - e);
+    if (len == 4) {
+      cv[0] = c4[r].x; cv[1] = c4[r].y; cv[2] = c4[r].z; cv[3] = c4[r].w;
+      if (nw) { wvv[0] = w4[r].x; wvv[1] = w4[r].y; wvv[2] = w4[r].z; wvv[3] = w4[r].w; }
     } else {
-      while (lo < hi) {
-        const uint32_t mid = lo + ((hi - lo + 1) >> 1);
-        if (off[r0 + mid] <= e) lo = mid; else hi = mid - 1;
+      for (uint32_t q = 0; q < len; ++q) {
+        cv[q] = colv[e + q];
+        if (nw) wvv[q] = wv[e + q];
       }
     }
-    const uint32_t u = r0 + lo;
-    const uint32_t k = e - (staged ? s_off[lo] : off[u]);
-    const uint32_t x = colv[e];
-    const uint32_t ti = gs.tix[u];
-    uint32_t dst;
-    if (ti == INF32) {
-      dst = noff[u] + k;
-    } else {
-      const uint32_t b = gs.t_gbeg[ti], en = gs.t_gbeg[ti + 1];
-      uint32_t l = b, h = en;
-      while (l < h) {
-        const uint32_t mid = (l + h) >> 1;
-        if (gs.col[mid] < x) l = mid + 1; else h = mid;
-      }
-      if (l < en && gs.col[l] == x) continue;  // deleted or replaced by an add
-      dst = noff[u] + k - (gs.ex_pre[l] - gs.ex_pre[b]) + (gs.add_pre[l] - gs.add_pre[b]);
+    uint32_t i = 0;
+    for (uint32_t q = 0; q < len; ++q) {
+      T.seek(e + q, i);
+      const uint32_t dst = T.dest(e + q, i);
+      if (dst == INF32) continue;
+      ncol[dst] = cv[q];
+      if (nw) nw[dst] = wvv[q];
     }
-    ncol[dst] = x;
-    if (nw) nw[dst] = wv[e];
   }
 }
 
@@ -467,6 +531,7 @@ __global__ void __launch_bounds__(kBlock) k_reset_round(const uint32_t* off, con
 
 __global__ void k_reset_end(Counters* cnt) {
   if (threadIdx.x || cnt->done) return;
+  cnt->aux[9] += 1;  // walk rounds
   cnt->aux[2] = cnt->aux[3];
   cnt->aux[3] = cnt->aux[4];
   if (cnt->aux[3] == cnt->aux[2]) cnt->done = 1;
@@ -562,6 +627,7 @@ __global__ void k_sssp_repair_prepare(Counters* cnt) {
   cnt->aux[2] = 0;
   cnt->aux[3] = 0;
   cnt->aux[4] = 0;
+  cnt->aux[9] = 0;
   cnt->done = 0;
   cnt->rounds = 0;
   cnt->edges = 0;
@@ -611,14 +677,13 @@ static void merge_direction(suminc_ctx* c, DevBuf& off, DevBuf& col, DevBuf& w, 
   if (c->m) {
     const uint32_t m = (uint32_t)c->m;
     const uint32_t ntiles = (m + kCopyTile - 1) / kCopyTile;
-    b.trow.ensure_slack(((size_t)ntiles + 1) * 4);
-    k_tile_rows<<<grid_for((uint64_t)ntiles + 1), kBlock, 0, c->stream>>>(off.as<uint32_t>(), n, m, ntiles,
-                                                                         b.trow.as<uint32_t>());
+    b.tstart.ensure_slack(((size_t)ntiles + 1) * 4);
+    k_tile_groups<<<grid_for((uint64_t)ntiles + 1), kBlock, 0, c->stream>>>(gs.countp, gs.pos, ntiles,
+                                                                           b.tstart.as<uint32_t>());
     if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, prof_slot), c->stream));
-    k_copy_old<<<ntiles, kBlock, 0, c->stream>>>(m, off.as<uint32_t>(), col.as<uint32_t>(),
-                                                 c->wkind ? w.as<uint32_t>() : nullptr, noff.as<uint32_t>(),
+    k_copy_old<<<ntiles, kBlock, 0, c->stream>>>(m, col.as<uint32_t>(), c->wkind ? w.as<uint32_t>() : nullptr,
                                                  ncol.as<uint32_t>(), c->wkind ? nw.as<uint32_t>() : nullptr,
-                                                 b.trow.as<uint32_t>(), ntiles, gs);
+                                                 b.tstart.as<uint32_t>(), gs);
     if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, prof_slot + 1), c->stream));
   }
   k_place_adds<<<grid_for(N), kBlock, 0, c->stream>>>(off.as<uint32_t>(), col.as<uint32_t>(), noff.as<uint32_t>(),
@@ -680,7 +745,8 @@ static void ensure_workspace(suminc_ctx* c, uint64_t n_ops) {
   b.ops.ensure(N * sizeof(suminc_update));
   b.keys_a.ensure(N * 8); b.keys_b.ensure(N * 8); b.idx_a.ensure(N * 4); b.idx_b.ensure(N * 4);
   b.rec.ensure(N * 16); b.grec.ensure(N * 16); b.flags.ensure(N);
-  for (DevBuf* d : {&b.g_row, &b.g_col, &b.g_w, &b.g_eoldw, &b.tg_row, &b.tg_col, &b.tg_w}) d->ensure(N * 4 + 4);
+  for (DevBuf* d : {&b.g_row, &b.g_col, &b.g_w, &b.g_eoldw, &b.tg_row, &b.tg_col, &b.tg_w, &b.g_pos, &b.tg_pos})
+    d->ensure(N * 4 + 4);
   for (DevBuf* d : {&b.g_act, &b.g_ex, &b.head, &b.tg_act, &b.tg_ex, &b.thead}) d->ensure(N + 1);
   for (DevBuf* d : {&b.g_exu, &b.g_addu, &b.g_expre, &b.g_addpre, &b.t_raw, &b.t_list, &b.t_gbeg, &b.tg_exu,
                     &b.tg_addu, &b.tg_expre, &b.tg_addpre, &b.tt_raw, &b.tt_list, &b.tt_gbeg})
@@ -779,7 +845,7 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   k_probe<<<grid_for(N), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
                                                  c->wkind ? c->out_w.as<uint32_t>() : nullptr, b.g_row.as<uint32_t>(),
                                                  b.g_col.as<uint32_t>(), bc, b.g_ex.as<uint8_t>(),
-                                                 b.g_eoldw.as<uint32_t>());
+                                                 b.g_eoldw.as<uint32_t>(), b.g_pos.as<uint32_t>());
   build_view(c, N, b.g_row.as<uint32_t>(), b.g_ex.as<uint8_t>(), b.g_act.as<uint8_t>(), b.g_exu, b.g_addu,
              b.g_expre, b.g_addpre, b.head, b.t_raw, b.t_list, b.t_gbeg, b.tix, 1);
   k_new_m<<<1, 64, 0, c->stream>>>(bc, b.g_expre.as<uint32_t>(), b.g_addpre.as<uint32_t>(), (uint32_t)c->m, bc);
@@ -807,7 +873,9 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
                                                         c->sssp_dp.as<unsigned long long>(), c->sssp_src,
                                                         b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(), cnt);
     k_reset_start<<<1, 64, 0, c->stream>>>(cnt);
-    int group = 4;
+    // rounds are enqueued in groups (the walk's depth of the previous batch + 1
+    // first), the host checks the device's stop flag between groups
+    int group = c->hint_reset;
     while (true) {
       for (int i = 0; i < group; ++i) {
         k_reset_round<<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
@@ -820,6 +888,7 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
       if (c->host_counters->done) break;
       group = 16;
     }
+    c->hint_reset = std::max(2, (int)c->host_counters->aux[9] + 1);
     c->stats.reserved[0] = (double)c->host_counters->aux[4];  // reset vertex count
   }
 
@@ -830,6 +899,7 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   gs.w = b.g_w.as<uint32_t>(); gs.ex = b.g_ex.as<uint8_t>();
   gs.ex_pre = b.g_expre.as<uint32_t>(); gs.add_pre = b.g_addpre.as<uint32_t>();
   gs.tix = b.tix.as<uint32_t>(); gs.t_gbeg = b.t_gbeg.as<uint32_t>();
+  gs.pos = b.g_pos.as<uint32_t>();
   const uint64_t m_old = c->m;
   merge_direction(c, c->out_off, c->out_col, c->out_w, b.noff_out, b.ncol_out, b.nw_out, gs, N, cap_m, 0);
 
@@ -853,6 +923,11 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   gt.w = b.tg_w.as<uint32_t>(); gt.ex = b.tg_ex.as<uint8_t>();
   gt.ex_pre = b.tg_expre.as<uint32_t>(); gt.add_pre = b.tg_addpre.as<uint32_t>();
   gt.tix = b.ttix.as<uint32_t>(); gt.t_gbeg = b.tt_gbeg.as<uint32_t>();
+  // positions of the transposed pairs in the old in-CSR
+  k_probe<<<grid_for(N), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(), nullptr,
+                                                 b.tg_row.as<uint32_t>(), b.tg_col.as<uint32_t>(), bc, nullptr,
+                                                 nullptr, b.tg_pos.as<uint32_t>());
+  gt.pos = b.tg_pos.as<uint32_t>();
   merge_direction(c, c->in_off, c->in_col, c->in_w, b.noff_in, b.ncol_in, b.nw_in, gt, N, cap_m, 2);
   k_untouch<<<grid_for(N), kBlock, 0, c->stream>>>(b.tt_list.as<uint32_t>(), bc, 2, N, b.ttix.as<uint32_t>());
 

@@ -172,6 +172,10 @@ int suminc_download_csr(suminc_ctx* c, uint64_t* offsets, uint32_t* cols, void* 
   return guarded(c, [&] { download_csr(c, offsets, cols, weights); });
 }
 
+int suminc_download_csc(suminc_ctx* c, uint64_t* offsets, uint32_t* rows, void* weights) {
+  return guarded(c, [&] { download_csr(c, offsets, rows, weights, true); });
+}
+
 int suminc_generate_batch(suminc_ctx* c, uint64_t seed, uint32_t batch_id, uint64_t n_del, uint64_t n_ins,
                           uint32_t wmin, uint32_t wmax, suminc_update* ops_out) {
   return guarded(c, [&] { generate_batch(c, seed, batch_id, n_del, n_ins, wmin, wmax, ops_out); });

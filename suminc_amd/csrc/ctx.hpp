@@ -91,11 +91,12 @@ struct Counters {
 struct BatchWs {
   DevBuf ops, keys_a, keys_b, idx_a, idx_b, rec, flags, grec;
   DevBuf g_row, g_col, g_act, g_w, g_ex, g_eoldw, g_exu, g_addu, g_expre, g_addpre, head, t_raw, t_list, t_gbeg;
+  DevBuf g_pos, tg_pos;  // old position of each group's pair (CSR / CSC)
+  DevBuf tstart;         // first group of every copy tile
   DevBuf tg_row, tg_col, tg_act, tg_w, tg_ex, tg_exu, tg_addu, tg_expre, tg_addpre, thead, tt_raw, tt_list, tt_gbeg;
   DevBuf tix, ttix;  // row -> touched index, all INF32 between batches
   DevBuf ndeg, noff_out, ncol_out, nw_out, noff_in, ncol_in, nw_in;
   DevBuf rbits, rlist;
-  DevBuf trow;  // row of the first edge of every copy tile
   DevBuf val;  // validation of device-resident ops: count, first bad index
   DevBuf bc;  // device counts: [0]=G groups, [1]=touched rows (CSR), [2]=touched rows (CSC), [3]=new m
   uint32_t tix_n = 0;
@@ -177,6 +178,9 @@ struct suminc_ctx {
   bool sssp_pending = false;  // initialised, not yet run
   uint32_t sssp_src = 0;
   suminc::DevBuf sssp_dp, sssp_flag0, sssp_flag1;
+
+  // rounds enqueued before the first host check (last run's rounds + 1)
+  int hint_reset = 6, hint_sssp = 8, hint_sssp_sketch = 8;
 
   // weighted PageRank / adjoint state (wpr.hip), created on first use
   void* wpr = nullptr;

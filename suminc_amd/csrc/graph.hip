@@ -251,16 +251,19 @@ void upload_csr(suminc_ctx* c, uint32_t n, uint64_t m, const uint64_t* off, cons
   c->graph_version++;
 }
 
-void download_csr(suminc_ctx* c, uint64_t* off, uint32_t* col, void* w) {
+void download_csr(suminc_ctx* c, uint64_t* off, uint32_t* col, void* w, bool in_edges) {
   SUMINC_REQUIRE(c->has_graph, SUMINC_E_STATE, "no graph");
+  const DevBuf& O = in_edges ? c->in_off : c->out_off;
+  const DevBuf& C = in_edges ? c->in_col : c->out_col;
+  const DevBuf& W = in_edges ? c->in_w : c->out_w;
   if (off) {
     std::vector<uint32_t> off32(c->n + 1);
-    SUMINC_HIP(hipMemcpyAsync(off32.data(), c->out_off.p, ((size_t)c->n + 1) * 4, hipMemcpyDeviceToHost, c->stream));
+    SUMINC_HIP(hipMemcpyAsync(off32.data(), O.p, ((size_t)c->n + 1) * 4, hipMemcpyDeviceToHost, c->stream));
     SUMINC_HIP(hipStreamSynchronize(c->stream));
     for (uint32_t i = 0; i <= c->n; ++i) off[i] = off32[i];
   }
-  if (col && c->m) SUMINC_HIP(hipMemcpyAsync(col, c->out_col.p, c->m * 4, hipMemcpyDeviceToHost, c->stream));
-  if (w && c->wkind && c->m) SUMINC_HIP(hipMemcpyAsync(w, c->out_w.p, c->m * 4, hipMemcpyDeviceToHost, c->stream));
+  if (col && c->m) SUMINC_HIP(hipMemcpyAsync(col, C.p, c->m * 4, hipMemcpyDeviceToHost, c->stream));
+  if (w && c->wkind && c->m) SUMINC_HIP(hipMemcpyAsync(w, W.p, c->m * 4, hipMemcpyDeviceToHost, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
 }
 

@@ -9,7 +9,7 @@ void generate_rmat(suminc_ctx* c, int scale, int edge_factor, double a, double b
                    int wkind, uint32_t wmin, uint32_t wmax);
 void upload_csr(suminc_ctx* c, uint32_t n, uint64_t m, const uint64_t* off, const uint32_t* col, int wkind,
                 const void* w);
-void download_csr(suminc_ctx* c, uint64_t* off, uint32_t* col, void* w);
+void download_csr(suminc_ctx* c, uint64_t* off, uint32_t* col, void* w, bool in_edges = false);
 
 // pr.hip
 void pr_alloc(suminc_ctx* c);

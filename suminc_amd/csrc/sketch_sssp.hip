@@ -422,7 +422,7 @@ void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
         a, internal_of(s), s.ss_list.as<uint32_t>(), lcount, s.ss_work.as<unsigned long long>(), 0);
   }
   SUMINC_HIP(hipGetLastError());
-  int group = 8;
+  int group = c->hint_sssp_sketch;  // rounds of the previous query + 1 first
   size_t slot = 0;
   while (true) {
     for (int i = 0; i < group; ++i, ++slot) {
@@ -442,6 +442,7 @@ void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
   SUMINC_REQUIRE(h->wl_count[cur] == 0 || h->rounds < (unsigned long long)max_rounds, SUMINC_E_NOT_CONVERGED,
                  "round cap reached with a non-empty frontier");
   const double rounds = (double)h->rounds, edges = (double)h->edges;
+  c->hint_sssp_sketch = std::max(2, std::min(64, (int)h->rounds + 1));
   if (c->prof_on) {
     prof_collect(c, SUMINC_KERNEL_SSSP_SKETCH, (size_t)h->rounds, 0, 2);
     suminc_kernel_profile& p = c->prof[SUMINC_KERNEL_SSSP_SKETCH];

@@ -14,6 +14,8 @@
 // drops is appended to the next list once (atomicOr on its bitmap word, then a
 // wave64-aggregated append).  Per round: k_sssp_push (light tiles, heavy
 // vertices deferred to 4096-edge chunks), k_sssp_heavy, k_sssp_end.
+#include <algorithm>
+
 #include "ctx.hpp"
 #include "device.hpp"
 #include "pr.hpp"
@@ -203,7 +205,7 @@ void sssp_run(suminc_ctx* c, suminc_stats* st) {
   SsspArgs a = sssp_args(c);
   c->sssp_pending = false;
   SUMINC_HIP(hipEventRecord(c->ev_start, c->stream));
-  int group = 8;
+  int group = c->hint_sssp;  // rounds of the previous query + 1 first
   size_t slot = 0;
   while (true) {
     for (int i = 0; i < group; ++i, ++slot) {
@@ -223,6 +225,7 @@ void sssp_run(suminc_ctx* c, suminc_stats* st) {
   float ms = 0;
   SUMINC_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
   Counters* h = c->host_counters;
+  c->hint_sssp = std::max(2, std::min(64, (int)h->rounds + 1));
   if (c->prof_on) {
     prof_collect(c, SUMINC_KERNEL_SSSP_PUSH, (size_t)h->rounds, 0, 2);
     suminc_kernel_profile& p = c->prof[SUMINC_KERNEL_SSSP_PUSH];

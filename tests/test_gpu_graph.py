@@ -97,6 +97,12 @@ def test_apply_batch_equals_oracle(gpu, weights):
             assert np.array_equal(col2, o_col)
             if weights:
                 assert np.array_equal(w2.view(np.uint32), o_w.view(np.uint32))
+            # the in-adjacency follows the same batch
+            c_off, c_row, c_w = csc_from_csr(o_off, o_col, o_w)
+            ioff, irow, iw = ctx.download_csc()
+            assert np.array_equal(ioff, c_off) and np.array_equal(irow, c_row)
+            if weights:
+                assert np.array_equal(iw.view(np.uint32), c_w.view(np.uint32))
 
 
 def test_apply_batch_rejects_bad_ops(gpu):
