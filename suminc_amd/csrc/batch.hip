@@ -496,9 +496,29 @@ __global__ void k_sssp_roots(const uint32_t* grow, const uint32_t* gcol, const u
 
 // one round of the subtree walk over the OLD out-CSR: children x of u with
 // parent(x) == u join the reset set (sum_sync_traversal_worker.h:170-194)
+// Rows longer than kHeavyDeg are not walked by the tile that holds them: they are
+// cut into kChunk-edge chunks (heavy list) that the *_heavy kernels spread over
+// all workgroups, so a hub does not serialise one workgroup.
+__device__ __forceinline__ void reset_child(const unsigned long long* dp, uint32_t u, uint32_t x, uint32_t* rbits,
+                                            uint32_t* rlist, Counters* cnt) {
+  bool add = false;
+  if (x != u && (uint32_t)dp[x] == u && (uint32_t)(dp[x] >> 32) != INF32) {
+    uint32_t bit = 1u << (x & 31);
+    add = !(atomicOr(&rbits[x >> 5], bit) & bit);
+  }
+  unsigned long long slot = wave_append(add, &cnt->aux[4]);
+  if (add) rlist[slot] = x;
+}
+
+__device__ __forceinline__ void heavy_chunks(uint32_t v, uint32_t len, uint2* heavy, Counters* cnt) {
+  const uint32_t nch = (len + kChunk - 1) / kChunk;
+  const unsigned long long slot = atomicAdd(&cnt->heavy_count, (unsigned long long)nch);
+  for (uint32_t k = 0; k < nch; ++k) heavy[slot + k] = make_uint2(v, k);
+}
+
 __global__ void __launch_bounds__(kBlock) k_reset_round(const uint32_t* off, const uint32_t* colv,
                                                         const unsigned long long* dp, uint32_t* rbits,
-                                                        uint32_t* rlist, Counters* cnt) {
+                                                        uint32_t* rlist, uint2* heavy, Counters* cnt) {
   __shared__ uint32_t s_scan[kBlock + 1];
   __shared__ uint32_t s_start[kBlock];
   __shared__ uint32_t s_warp[kBlock / 64 + 1];
@@ -514,23 +534,42 @@ __global__ void __launch_bounds__(kBlock) k_reset_round(const uint32_t* off, con
       s_u[threadIdx.x] = u;
       start = off[u];
       len = off[u + 1] - start;
+      if (len > kHeavyDeg) {
+        heavy_chunks(u, len, heavy, cnt);
+        len = 0;
+      }
     }
     block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t, uint32_t k) {
-      uint32_t u = s_u[slot];
-      uint32_t x = colv[k];
-      bool add = false;
-      if (x != u && (uint32_t)dp[x] == u && (uint32_t)(dp[x] >> 32) != INF32) {
-        uint32_t bit = 1u << (x & 31);
-        add = !(atomicOr(&rbits[x >> 5], bit) & bit);
-      }
-      unsigned long long slot2 = wave_append(add, &cnt->aux[4]);
-      if (add) rlist[slot2] = x;
+      reset_child(dp, s_u[slot], colv[k], rbits, rlist, cnt);
     });
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_reset_round_heavy(const uint32_t* off, const uint32_t* colv,
+                                                              const unsigned long long* dp, uint32_t* rbits,
+                                                              uint32_t* rlist, const uint2* heavy, Counters* cnt) {
+  if (cnt->done) return;
+  const unsigned long long nh = cnt->heavy_count;
+  for (unsigned long long h = blockIdx.x; h < nh; h += gridDim.x) {
+    const uint2 hv = heavy[h];
+    const uint32_t u = hv.x;
+    const uint32_t b = off[u] + hv.y * kChunk;
+    const uint32_t e = min(off[u + 1], b + kChunk);
+    for (uint32_t k0 = b; k0 < e; k0 += kBlock) {  // same trip count for every lane (wave ballots)
+      const uint32_t k = k0 + threadIdx.x;
+      if (k < e) {
+        reset_child(dp, u, colv[k], rbits, rlist, cnt);
+      } else {
+        unsigned long long unused = wave_append(false, &cnt->aux[4]);
+        (void)unused;
+      }
+    }
   }
 }
 
 __global__ void k_reset_end(Counters* cnt) {
   if (threadIdx.x || cnt->done) return;
+  cnt->heavy_count = 0;
   cnt->aux[9] += 1;  // walk rounds
   cnt->aux[2] = cnt->aux[3];
   cnt->aux[3] = cnt->aux[4];
@@ -545,10 +584,22 @@ __global__ void k_reset_apply(const uint32_t* rlist, const Counters* cnt, unsign
 }
 
 // reset vertices pull from surviving (non-reset) in-neighbours over the NEW CSC
+__device__ __forceinline__ unsigned long long pull_candidate(const uint32_t* icol, const uint32_t* iw,
+                                                             const uint32_t* rbits, const unsigned long long* dp,
+                                                             uint32_t k) {
+  const uint32_t u = icol[k];
+  if ((rbits[u >> 5] >> (u & 31)) & 1u) return ~0ull;
+  const uint32_t du = (uint32_t)(dp[u] >> 32);
+  if (du == INF32) return ~0ull;
+  const unsigned long long nd = (unsigned long long)du + (iw ? iw[k] : 1u);
+  if (nd >= INF32) return ~0ull;
+  return (nd << 32) | u;
+}
+
 __global__ void __launch_bounds__(kBlock) k_reset_pull(const uint32_t* ioff, const uint32_t* icol,
                                                        const uint32_t* iw, const uint32_t* rbits,
-                                                       const uint32_t* rlist, const Counters* cnt,
-                                                       unsigned long long* dp) {
+                                                       const uint32_t* rlist, Counters* cnt,
+                                                       unsigned long long* dp, uint2* heavy) {
   __shared__ uint32_t s_scan[kBlock + 1];
   __shared__ uint32_t s_start[kBlock];
   __shared__ uint32_t s_warp[kBlock / 64 + 1];
@@ -563,17 +614,53 @@ __global__ void __launch_bounds__(kBlock) k_reset_pull(const uint32_t* ioff, con
       s_x[threadIdx.x] = x;
       start = ioff[x];
       len = ioff[x + 1] - start;
+      if (len > kHeavyDeg) {
+        heavy_chunks(x, len, heavy, cnt);
+        len = 0;
+      }
     }
     block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t, uint32_t k) {
-      uint32_t u = icol[k];
-      if ((rbits[u >> 5] >> (u & 31)) & 1u) return;
-      uint32_t du = (uint32_t)(dp[u] >> 32);
-      if (du == INF32) return;
-      unsigned long long nd = (unsigned long long)du + (iw ? iw[k] : 1u);
-      if (nd >= INF32) return;
-      atomicMin(&dp[s_x[slot]], (nd << 32) | u);
+      const unsigned long long cand = pull_candidate(icol, iw, rbits, dp, k);
+      if (cand != ~0ull) atomicMin(&dp[s_x[slot]], cand);
     });
   }
+}
+
+// heavy in-rows of reset vertices: block minimum per chunk, one atomicMin
+__global__ void __launch_bounds__(kBlock) k_reset_pull_heavy(const uint32_t* ioff, const uint32_t* icol,
+                                                             const uint32_t* iw, const uint32_t* rbits,
+                                                             const uint2* heavy, Counters* cnt,
+                                                             unsigned long long* dp) {
+  __shared__ unsigned long long s_min[kBlock / 64];
+  const unsigned long long nh = cnt->heavy_count;
+  for (unsigned long long h = blockIdx.x; h < nh; h += gridDim.x) {
+    const uint2 hv = heavy[h];
+    const uint32_t x = hv.x;
+    const uint32_t b = ioff[x] + hv.y * kChunk;
+    const uint32_t e = min(ioff[x + 1], b + kChunk);
+    unsigned long long best = ~0ull;
+    for (uint32_t k = b + threadIdx.x; k < e; k += kBlock) {
+      const unsigned long long cand = pull_candidate(icol, iw, rbits, dp, k);
+      best = cand < best ? cand : best;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long y = __shfl_xor(best, o);
+      best = y < best ? y : best;
+    }
+    if (lane_id() == 0) s_min[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long m = ~0ull;
+      for (int w = 0; w < kBlock / 64; ++w) m = s_min[w] < m ? s_min[w] : m;
+      if (m != ~0ull) atomicMin(&dp[x], m);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_heavy_clear(Counters* cnt) {
+  if (threadIdx.x == 0) cnt->heavy_count = 0;
 }
 
 // Seeds SSSP list 0 with every finite reset vertex.
@@ -880,7 +967,11 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
       for (int i = 0; i < group; ++i) {
         k_reset_round<<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
                                                              c->sssp_dp.as<unsigned long long>(),
-                                                             b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(), cnt);
+                                                             b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(),
+                                                             c->heavy.as<uint2>(), cnt);
+        k_reset_round_heavy<<<push_grid(), kBlock, 0, c->stream>>>(
+            c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(), c->sssp_dp.as<unsigned long long>(),
+            b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(), c->heavy.as<uint2>(), cnt);
         k_reset_end<<<1, 64, 0, c->stream>>>(cnt);
       }
       SUMINC_HIP(hipGetLastError());
@@ -943,10 +1034,15 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   if (sssp_repair) {
     k_reset_apply<<<grid_for(n), kBlock, 0, c->stream>>>(b.rlist.as<uint32_t>(), cnt,
                                                          c->sssp_dp.as<unsigned long long>());
-    k_reset_pull<<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
-                                                        c->wkind == SUMINC_W_U32 ? c->in_w.as<uint32_t>() : nullptr,
+    const uint32_t* iw = c->wkind == SUMINC_W_U32 ? c->in_w.as<uint32_t>() : nullptr;
+    k_heavy_clear<<<1, 64, 0, c->stream>>>(cnt);
+    k_reset_pull<<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(), iw,
                                                         b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(), cnt,
-                                                        c->sssp_dp.as<unsigned long long>());
+                                                        c->sssp_dp.as<unsigned long long>(), c->heavy.as<uint2>());
+    k_reset_pull_heavy<<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(), iw,
+                                                              b.rbits.as<uint32_t>(), c->heavy.as<uint2>(), cnt,
+                                                              c->sssp_dp.as<unsigned long long>());
+    k_heavy_clear<<<1, 64, 0, c->stream>>>(cnt);
     k_reset_seed<<<grid_for(n), kBlock, 0, c->stream>>>(b.rlist.as<uint32_t>(), c->sssp_dp.as<unsigned long long>(),
                                                         c->wl0.as<uint32_t>(), c->sssp_flag0.as<uint32_t>(), cnt);
     k_insert_relax<<<grid_for(N), kBlock, 0, c->stream>>>(b.g_row.as<uint32_t>(), b.g_col.as<uint32_t>(),
