@@ -126,11 +126,14 @@ struct Sketch {
   DevBuf raw_cid, raw_acc;   // input assignment (n), acceptance per raw cluster (Kraw)
   DevBuf rlist;              // vertices with a raw cluster id (RM)
   DevBuf cid, ntype, lidx, is_entry, is_exit;
+  DevBuf base_mem, bslot;    // members in (cluster, id) order (M), base slot of each member (n); fixed
+  bool batch_pending = false;  // a batch updated the internal list; merge pending
   DevBuf ext_out, ext_in;    // per clustered vertex: edges leaving / entering its cluster
   DevBuf cl_off, cl_mem, cl_nexit, cl_eoff, cl_ent, ent_of, sc_base, ss_base;
-  // internal edges: the list sorted by (src, dst), kept up to date by every batch,
-  // and its grouping by destination member slot for the in-cluster kernels
-  DevBuf iem_key, iem_w, iem_dead, iem_key2, iem_w2;
+  // internal edges: the list sorted by (base slot of dst, src), kept up to date by
+  // every batch (sorted adds merged in, dead entries dropped), and the
+  // per-cluster arrays the in-cluster kernels read
+  DevBuf iem_key, iem_w, iem_dead, iem_key2, iem_w2, iem_pre, add_key, add_w;
   DevBuf ie_off, ie_src, ie_dst, ie_deg, ie_wt;
   DevBuf sc_R, sc_D;         // PageRank rows (reach mass / residual) per entry
   DevBuf ss_dist, ss_par;    // SSSP rows (in-cluster distance / parent to every exit) per entry
@@ -140,11 +143,9 @@ struct Sketch {
   DevBuf ss_work;            // u64 x2: internal edges read by the SSSP shortcut kernels, list length
   DevBuf ss_list;            // K: clusters the next per-cluster kernel visits
   DevBuf pr_valid, ss_valid; // K: shortcut rows of the cluster are current
-  DevBuf sc_base_old, ss_base_old, ent_of_old, sc_R_old, sc_D_old, ss_dist_old, ss_par_old;  // previous layout
-  // scratch of the layout pass, kept between batches
-  DevBuf lay_keys_a, lay_keys_b, lay_hist, lay_ckey, lay_entf, lay_epos, lay_ecount, lay_ekey, lay_rowlen, lay_mx,
-      lay_rowlen2,
-      lay_skey_a, lay_skey_b, lay_sidx_a;
+  DevBuf sc_R_old, sc_D_old, ss_dist_old, ss_par_old;  // rows of the previous layout
+  // layout pass: per-cluster counts, their prefix (current and previous layout), type histogram
+  DevBuf lay_cnt, lay_base, lay_base_old, lay_hist;
   uint32_t counts[8] = {0};  // node type histogram
 };
 

@@ -289,38 +289,39 @@ __global__ void k_assign_cid(uint32_t n, const uint32_t* raw, const uint8_t* acc
   ntype[v] = 0;  // SingleNode until typed below
 }
 
-// node type of every raw member from its edge counts, the member sort key
-// (cluster, exits first, id; rejected raw members last) and the type histogram
-__global__ void k_member_types(const uint32_t* list, uint32_t L, const uint32_t* cid, const uint32_t* ext_out,
-                               const uint32_t* ext_in, uint8_t* ntype, uint8_t* is_entry, uint8_t* is_exit,
-                               unsigned long long* keys, uint32_t* hist) {
-  __shared__ uint32_t s_h[8];
-  if (threadIdx.x < 8) s_h[threadIdx.x] = 0;
-  __syncthreads();
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < L) {
-    const uint32_t v = list[i], id = cid[v];
-    if (id == INF32) {
-      keys[i] = ~0ull;
-    } else {
-      const bool en = ext_in[v] != 0, ex = ext_out[v] != 0;
-      const uint8_t t = en && ex ? 3 : en ? 2 : ex ? 1 : 4;  // BothOutIn / OnlyIn / OnlyOut / Inner
-      ntype[v] = t;
-      is_entry[v] = en;
-      is_exit[v] = ex;
-      atomicAdd(&s_h[t], 1u);
-      keys[i] = ((unsigned long long)id << 33) | ((unsigned long long)(ex ? 0 : 1) << 32) | v;
-    }
+// ---------------------------------------------------------------------------
+// Member order.  The first build sorts the accepted members by (cluster, id)
+// once; that base order, the cluster offsets cl_off and every member's base slot
+// stay fixed, because membership never changes after the first build.  Every
+// layout then partitions each cluster stably into its exits followed by the
+// other members (the shortcut rows of SSSP hold the exit slots only), and lays
+// out the entries and the shortcut-row bases from one scan over the clusters.
+// ---------------------------------------------------------------------------
+struct ClBase {  // per-cluster counts, and after the scan their prefix
+  unsigned long long sc;  // PageRank row slots: entries x members
+  unsigned long long ss;  // SSSP row slots: entries x exits
+  uint32_t ent;           // entries
+  uint32_t pad;
+};
+struct ClBasePlus {
+  __host__ __device__ ClBase operator()(const ClBase& a, const ClBase& b) const {
+    return ClBase{a.sc + b.sc, a.ss + b.ss, a.ent + b.ent, 0u};
   }
-  __syncthreads();
-  if (threadIdx.x < 8 && s_h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], s_h[threadIdx.x]);
+};
+
+__global__ void k_base_keys(const uint32_t* list, uint32_t L, const uint32_t* cid, unsigned long long* keys) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L) return;
+  const uint32_t v = list[i], k = cid[v];
+  keys[i] = k == INF32 ? ~0ull : ((unsigned long long)k << 32) | v;
 }
 
-__global__ void k_members(const unsigned long long* keys, uint32_t M, uint32_t* cl_mem, uint32_t* cl_key) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_base_order(const unsigned long long* keys, uint32_t M, uint32_t* base_mem, uint32_t* bslot) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
-  cl_mem[i] = (uint32_t)keys[i];
-  cl_key[i] = (uint32_t)(keys[i] >> 33);
+  const uint32_t v = (uint32_t)keys[i];
+  base_mem[i] = v;
+  bslot[v] = i;
 }
 
 template <class K>
@@ -332,41 +333,135 @@ __global__ void k_row_ptr32(const K* rowkey, uint32_t M, uint32_t K_, uint32_t* 
   for (int64_t r = prev + 1; r <= row; ++r) off[r] = i;
 }
 
-__global__ void k_member_index(uint32_t M, const uint32_t* cl_mem, const uint32_t* cl_key, const uint32_t* cl_off,
-                               const uint8_t* is_exit, const uint8_t* is_entry, uint32_t* lidx, uint32_t* nexit,
-                               uint8_t* ent_flag) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M) return;
-  uint32_t v = cl_mem[i], k = cl_key[i];
-  lidx[v] = i - cl_off[k];
-  if (is_exit[v]) atomicAdd(&nexit[k], 1u);
-  ent_flag[i] = is_entry[v];
+// One wave per cluster: node types from the edge counts (BothOutIn / OnlyIn /
+// OnlyOut / Inner), the stable exits-first member order, local indices, the
+// per-cluster counts for the scan, and the type histogram.
+constexpr uint32_t kLayoutBlocks = 1024;
+__global__ void __launch_bounds__(kBlock) k_member_partition(uint32_t K, const uint32_t* __restrict__ cl_off,
+                                                             const uint32_t* __restrict__ base_mem,
+                                                             const uint32_t* __restrict__ ext_out,
+                                                             const uint32_t* __restrict__ ext_in, uint8_t* ntype,
+                                                             uint8_t* is_entry, uint8_t* is_exit, uint32_t* cl_mem,
+                                                             uint32_t* lidx, uint32_t* cl_nexit, ClBase* cl_cnt,
+                                                             uint32_t* hist) {
+  __shared__ uint32_t s_h[8];
+  if (threadIdx.x < 8) s_h[threadIdx.x] = 0;
+  __syncthreads();
+  const int lane = lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  uint32_t h1 = 0, h2 = 0, h3 = 0, h4 = 0;
+  const uint32_t nw = gridDim.x * kWaves;
+  for (uint32_t k = blockIdx.x * kWaves + (threadIdx.x >> 6); k < K; k += nw) {
+    const uint32_t b = cl_off[k], size = cl_off[k + 1] - b;
+    uint32_t nex = 0, nen = 0, nboth = 0;
+    for (uint32_t j0 = 0; j0 < size; j0 += 64) {
+      const uint32_t j = j0 + lane;
+      bool en = false, ex = false;
+      if (j < size) {
+        const uint32_t v = base_mem[b + j];
+        en = ext_in[v] != 0;
+        ex = ext_out[v] != 0;
+        ntype[v] = en && ex ? 3 : en ? 2 : ex ? 1 : 4;
+        is_entry[v] = en;
+        is_exit[v] = ex;
+      }
+      const unsigned long long mex = __ballot(ex), men = __ballot(en);
+      nex += __popcll(mex);
+      nen += __popcll(men);
+      nboth += __popcll(mex & men);
+    }
+    uint32_t pe = 0, pi = nex;
+    for (uint32_t j0 = 0; j0 < size; j0 += 64) {
+      const uint32_t j = j0 + lane;
+      const bool in = j < size;
+      uint32_t v = 0;
+      bool ex = false;
+      if (in) {
+        v = base_mem[b + j];
+        ex = ext_out[v] != 0;
+      }
+      const unsigned long long mex = __ballot(ex), mrest = __ballot(in && !ex);
+      if (in) {
+        const uint32_t pos = ex ? pe + __popcll(mex & below) : pi + __popcll(mrest & below);
+        cl_mem[b + pos] = v;
+        lidx[v] = pos;
+      }
+      pe += __popcll(mex);
+      pi += __popcll(mrest);
+    }
+    h1 += nex - nboth;
+    h2 += nen - nboth;
+    h3 += nboth;
+    h4 += size - (nex + nen - nboth);
+    if (lane == 0) {
+      cl_nexit[k] = nex;
+      cl_cnt[k] = ClBase{(unsigned long long)nen * size, (unsigned long long)nen * nex, nen, 0u};
+    }
+  }
+  if (lane == 0) {
+    atomicAdd(&s_h[1], h1);
+    atomicAdd(&s_h[2], h2);
+    atomicAdd(&s_h[3], h3);
+    atomicAdd(&s_h[4], h4);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) cl_cnt[K] = ClBase{0ull, 0ull, 0u, 0u};
+  __syncthreads();
+  if (threadIdx.x < 8 && s_h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], s_h[threadIdx.x]);
 }
 
-__global__ void k_entries(uint32_t E, const uint32_t* pos, const uint32_t* cl_mem, const uint32_t* cl_key,
-                          const uint32_t* cl_off, const uint32_t* cl_nexit, uint32_t* cl_ent, uint32_t* ent_key,
-                          uint32_t* ent_of, unsigned long long* rowlen, unsigned long long* rowlen_exits) {
-  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e > E) return;
-  if (e == E) { rowlen[E] = 0; rowlen_exits[E] = 0; return; }
-  uint32_t i = pos[e];
-  uint32_t v = cl_mem[i], k = cl_key[i];
-  cl_ent[e] = v;
-  ent_key[e] = k;
-  ent_of[v] = e;
-  rowlen[e] = cl_off[k + 1] - cl_off[k];
-  rowlen_exits[e] = cl_nexit[k];
-}
-
-__global__ void k_ent_of_clear(uint32_t M, const uint32_t* cl_mem, uint32_t* ent_of) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < M) ent_of[cl_mem[i]] = INF32;
+// One wave per cluster: entries in member order, their shortcut-row bases, and
+// ent_of for every member (INF32 for non-entries).
+__global__ void __launch_bounds__(kBlock) k_entry_layout(uint32_t K, const uint32_t* __restrict__ cl_off,
+                                                         const uint32_t* __restrict__ cl_mem,
+                                                         const uint32_t* __restrict__ cl_nexit,
+                                                         const uint8_t* __restrict__ is_entry,
+                                                         const ClBase* __restrict__ base, uint32_t* cl_eoff,
+                                                         uint32_t* cl_ent, uint32_t* ent_of,
+                                                         unsigned long long* sc_base, unsigned long long* ss_base) {
+  const int lane = lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const uint32_t nw = gridDim.x * kWaves;
+  for (uint32_t k = blockIdx.x * kWaves + (threadIdx.x >> 6); k < K; k += nw) {
+    const uint32_t b = cl_off[k], size = cl_off[k + 1] - b, nex = cl_nexit[k];
+    const ClBase cb = base[k];
+    uint32_t r = 0;
+    for (uint32_t j0 = 0; j0 < size; j0 += 64) {
+      const uint32_t j = j0 + lane;
+      uint32_t v = 0;
+      bool en = false;
+      if (j < size) {
+        v = cl_mem[b + j];
+        en = is_entry[v] != 0;
+      }
+      const unsigned long long men = __ballot(en);
+      if (j < size) {
+        if (en) {
+          const uint32_t q = r + __popcll(men & below), e = cb.ent + q;
+          cl_ent[e] = v;
+          ent_of[v] = e;
+          sc_base[e] = cb.sc + (unsigned long long)q * size;
+          ss_base[e] = cb.ss + (unsigned long long)q * nex;
+        } else {
+          ent_of[v] = INF32;
+        }
+      }
+      r += __popcll(men);
+    }
+    if (lane == 0) cl_eoff[k] = cb.ent;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const ClBase t = base[K];
+    cl_eoff[K] = t.ent;
+    sc_base[t.ent] = t.sc;
+    ss_base[t.ent] = t.ss;
+  }
 }
 
 // First build: the internal in-edges of every accepted member, one wave per
-// member, appended as (src, dst) keys with their weights.
+// member, appended as keys (base slot of dst << 32 | src) with their weights.
 __global__ void __launch_bounds__(kBlock) k_internal_collect(const uint32_t* __restrict__ list, uint32_t L,
                                                              const uint32_t* __restrict__ cid,
+                                                             const uint32_t* __restrict__ bslot,
                                                              const uint32_t* __restrict__ ioff,
                                                              const uint32_t* __restrict__ icol,
                                                              const uint32_t* __restrict__ iw,
@@ -377,6 +472,7 @@ __global__ void __launch_bounds__(kBlock) k_internal_collect(const uint32_t* __r
   const int lane = lane_id();
   const uint32_t v = list[i], k = cid[v];
   if (k == INF32) return;  // whole wave
+  const unsigned long long hi = (unsigned long long)bslot[v] << 32;
   const uint32_t b = ioff[v], e = ioff[v + 1];
   for (uint32_t e0 = b; e0 < e; e0 += 64) {
     const uint32_t x = e0 + lane;
@@ -388,63 +484,54 @@ __global__ void __launch_bounds__(kBlock) k_internal_collect(const uint32_t* __r
     }
     const unsigned long long slot = wave_append(in, count);
     if (in) {
-      key[slot] = ((unsigned long long)src << 32) | v;
+      key[slot] = hi | src;
       wout[slot] = iw ? iw[x] : 1u;
     }
   }
 }
 
-// Groups the internal edge list by destination member slot: sort key
-// (slot << 32 | list index).
-__global__ void k_internal_slot_keys(uint64_t N, const unsigned long long* key, const uint32_t* cid,
-                                     const uint32_t* cl_off, const uint32_t* lidx, unsigned long long* skey,
-                                     uint32_t* idx) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const uint32_t dst = (uint32_t)key[i];
-  skey[i] = ((unsigned long long)(cl_off[cid[dst]] + lidx[dst]) << 32) | (uint32_t)i;
-  idx[i] = (uint32_t)i;
-}
-
-__global__ void k_internal_fill(uint64_t N, const unsigned long long* skey, const unsigned long long* key,
-                                const uint32_t* w, const uint32_t* lidx, const uint32_t* ooff, uint32_t* ie_src,
+// The internal edge list is sorted by (base slot of dst, src), hence grouped by
+// cluster: the in-cluster arrays are a direct map of it, ie_off per cluster.
+__global__ void k_internal_fill(uint64_t N, uint32_t K, const unsigned long long* __restrict__ key,
+                                const uint32_t* __restrict__ w, const uint32_t* __restrict__ base_mem,
+                                const uint32_t* __restrict__ cid, const uint32_t* __restrict__ lidx,
+                                const uint32_t* __restrict__ ooff, uint32_t* ie_off, uint32_t* ie_src,
                                 uint32_t* ie_dst, uint32_t* ie_deg, uint32_t* ie_wt) {
-  uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= N) return;
-  const uint32_t i = (uint32_t)skey[p];
-  const unsigned long long kk = key[i];
-  const uint32_t src = (uint32_t)(kk >> 32), dst = (uint32_t)kk;
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > N) return;
+  const int64_t row = p == N ? (int64_t)K : (int64_t)cid[base_mem[key[p] >> 32]];
+  const int64_t prev = p == 0 ? -1 : (int64_t)cid[base_mem[key[p - 1] >> 32]];
+  for (int64_t r = prev + 1; r <= row; ++r) ie_off[r] = (uint32_t)p;
+  if (p == N) return;
+  const uint32_t src = (uint32_t)key[p], dst = base_mem[key[p] >> 32];
   ie_src[p] = lidx[src];
   ie_dst[p] = lidx[dst];
   ie_deg[p] = ooff[src + 1] - ooff[src];
-  ie_wt[p] = w[i];
+  ie_wt[p] = w[p];
 }
 
-__global__ void k_max_cluster(uint32_t K, const uint32_t* cl_off, uint32_t* out) {
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < K) atomicMax(out, cl_off[k + 1] - cl_off[k]);
-}
-
-// Moves the shortcut rows of clusters the batch did not touch from the previous
-// layout: one workgroup per entry of the new layout.
-template <class T>
-__global__ void k_rows_move(uint32_t E, const uint32_t* __restrict__ cl_ent, const uint32_t* __restrict__ cid,
-                            const uint32_t* __restrict__ cl_off, const uint32_t* __restrict__ cl_nexit,
-                            const unsigned long long* __restrict__ sc_base,
-                            const uint32_t* __restrict__ old_ent_of,
-                            const unsigned long long* __restrict__ old_sc_base, const uint8_t* __restrict__ valid,
-                            const T* __restrict__ oldA, T* __restrict__ newA, const T* __restrict__ oldB,
-                            T* __restrict__ newB) {
-  const uint32_t e = blockIdx.x;
-  if (e >= E) return;
-  const uint32_t v = cl_ent[e], k = cid[v];
-  const uint32_t oe = old_ent_of[v];
-  if (!valid[k] || oe == INF32) return;
-  const uint32_t size = cl_nexit ? cl_nexit[k] : cl_off[k + 1] - cl_off[k];
-  const unsigned long long nb = sc_base[e], ob = old_sc_base[oe];
-  for (uint32_t j = threadIdx.x; j < size; j += blockDim.x) {
-    newA[nb + j] = oldA[ob + j];
-    newB[nb + j] = oldB[ob + j];
+// Moves the shortcut rows of the clusters the batches did not touch from the
+// previous layout.  A cluster's rows are contiguous in both layouts and keep
+// their length (its members' entry/exit status can only change through an edge
+// with an endpoint in it).
+template <class T, bool kSsspRows>
+__global__ void k_rows_move(uint64_t len, uint32_t K, const ClBase* __restrict__ nb, const ClBase* __restrict__ ob,
+                            const uint8_t* __restrict__ valid, const T* __restrict__ oldA, T* __restrict__ newA,
+                            const T* __restrict__ oldB, T* __restrict__ newB) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < len; p += stride) {
+    uint32_t lo = 0, hi = K;  // last cluster whose base is <= p (nb[K] = len > p)
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      const unsigned long long bm = kSsspRows ? nb[mid].ss : nb[mid].sc;
+      if (bm <= p) lo = mid; else hi = mid - 1;
+    }
+    if (!valid[lo]) continue;
+    const unsigned long long nbase = kSsspRows ? nb[lo].ss : nb[lo].sc;
+    const unsigned long long obase = kSsspRows ? ob[lo].ss : ob[lo].sc;
+    const unsigned long long q = obase + (p - nbase);
+    newA[p] = oldA[q];
+    newB[p] = oldB[q];
   }
 }
 
@@ -458,134 +545,72 @@ static void grow_keep(suminc_ctx* c, DevBuf& b, size_t bytes, size_t keep) {
   b.swap(t);
 }
 
+static uint32_t layout_grid(uint32_t K) {
+  return std::max<uint32_t>(1u, std::min<uint32_t>(kLayoutBlocks, (K + kWaves - 1) / kWaves));
+}
+
+static uint32_t move_grid(uint64_t len) {
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(4096, (len + kBlock - 1) / kBlock));
+}
+
 // Layout of the current structure from the per-member edge counts and the
-// internal edge list: node types, member order, entries, internal edges grouped
-// by destination slot, moved shortcut rows.
+// internal edge list: node types, member order, entries and their shortcut-row
+// bases, the in-cluster edge arrays, and the rows of untouched clusters moved
+// over from the previous layout.
 static void sketch_layout(suminc_ctx* c, bool rebuild) {
   Sketch& s = c->sk;
-  const uint32_t n = c->n, K = s.K, RM = s.RM;
-  DevBuf &keys_a = s.lay_keys_a, &keys_b = s.lay_keys_b, &hist = s.lay_hist, &ckey = s.lay_ckey,
-         &entf = s.lay_entf, &epos = s.lay_epos, &ecount = s.lay_ecount, &ekey = s.lay_ekey,
-         &rowlen = s.lay_rowlen, &rowlen2 = s.lay_rowlen2, &mx = s.lay_mx, &skey_a = s.lay_skey_a,
-         &skey_b = s.lay_skey_b,
-         &sidx_a = s.lay_sidx_a;
-  keys_a.ensure((size_t)RM * 8 + 8); keys_b.ensure((size_t)RM * 8 + 8); hist.ensure(32);
-  SUMINC_HIP(hipMemsetAsync(hist.p, 0, 32, c->stream));
-  if (RM) {
-    k_member_types<<<grid_for(RM), kBlock, 0, c->stream>>>(s.rlist.as<uint32_t>(), RM, s.cid.as<uint32_t>(),
-                                                           s.ext_out.as<uint32_t>(), s.ext_in.as<uint32_t>(),
-                                                           s.ntype.as<uint8_t>(), s.is_entry.as<uint8_t>(),
-                                                           s.is_exit.as<uint8_t>(), keys_a.as<unsigned long long>(),
-                                                           hist.as<uint32_t>());
-    cub(c, [&](void* p, size_t& b) {
-      return rocprim::radix_sort_keys(p, b, keys_a.as<uint64_t>(), keys_b.as<uint64_t>(), (size_t)RM, 0, 64,
-                                      c->stream);
-    });
+  const uint32_t n = c->n, K = s.K, M = s.M;
+  if (rebuild) s.lay_base.swap(s.lay_base_old);  // the previous bases locate the rows to move
+  s.lay_hist.ensure(32);
+  s.lay_cnt.ensure(((size_t)K + 1) * sizeof(ClBase));
+  s.lay_base.ensure(((size_t)K + 1) * sizeof(ClBase));
+  s.cl_mem.ensure((size_t)M * 4 + 4); s.cl_nexit.ensure((size_t)K * 4 + 4); s.cl_eoff.ensure(((size_t)K + 1) * 4);
+  s.cl_ent.ensure((size_t)M * 4 + 4);
+  s.sc_base.ensure(((size_t)M + 1) * 8); s.ss_base.ensure(((size_t)M + 1) * 8);
+  if (!rebuild || !s.ent_of.p) {
+    s.ent_of.ensure((size_t)n * 4);
+    k_fill32<<<grid_for(n), kBlock, 0, c->stream>>>(s.ent_of.as<uint32_t>(), n, INF32);
   }
-  SUMINC_HIP(hipMemcpyAsync(s.counts, hist.p, 32, hipMemcpyDeviceToHost, c->stream));
-  SUMINC_HIP(hipStreamSynchronize(c->stream));
-  uint32_t M = 0;
-  for (int t = 1; t < 5; ++t) M += s.counts[t];
-  s.counts[0] = n - M;
-  SUMINC_REQUIRE(!rebuild || M == s.M, SUMINC_E_STATE, "supernode membership changed on a rebuild");
-  s.M = M;
-  // the previous layout is kept for moving untouched shortcut rows
-  if (rebuild) {
-    s.sc_base.swap(s.sc_base_old);
-    s.ss_base.swap(s.ss_base_old);
-    s.ent_of.swap(s.ent_of_old);
-    s.sc_R.swap(s.sc_R_old); s.sc_D.swap(s.sc_D_old);
-    s.ss_dist.swap(s.ss_dist_old); s.ss_par.swap(s.ss_par_old);
-  }
-  // cluster member layout
-  s.cl_mem.ensure((size_t)M * 4 + 4); ckey.ensure((size_t)M * 4 + 4); s.cl_off.ensure(((size_t)K + 1) * 4);
-  s.cl_nexit.ensure((size_t)K * 4 + 4); entf.ensure(M + 1);
-  SUMINC_HIP(hipMemsetAsync(s.cl_nexit.p, 0, (size_t)K * 4 + 4, c->stream));
-  if (M) {
-    k_members<<<grid_for(M), kBlock, 0, c->stream>>>(keys_b.as<unsigned long long>(), M, s.cl_mem.as<uint32_t>(),
-                                                     ckey.as<uint32_t>());
-    k_row_ptr32<uint32_t><<<grid_for((uint64_t)M + 1), kBlock, 0, c->stream>>>(ckey.as<uint32_t>(), M, K,
-                                                                               s.cl_off.as<uint32_t>(), 0);
-    k_member_index<<<grid_for(M), kBlock, 0, c->stream>>>(M, s.cl_mem.as<uint32_t>(), ckey.as<uint32_t>(),
-                                                          s.cl_off.as<uint32_t>(), s.is_exit.as<uint8_t>(),
-                                                          s.is_entry.as<uint8_t>(), s.lidx.as<uint32_t>(),
-                                                          s.cl_nexit.as<uint32_t>(), entf.as<uint8_t>());
-  } else {
-    SUMINC_HIP(hipMemsetAsync(s.cl_off.p, 0, ((size_t)K + 1) * 4, c->stream));
-  }
-  // entries
-  epos.ensure((size_t)M * 4 + 4); ecount.ensure(8);
-  SUMINC_HIP(hipMemsetAsync(ecount.p, 0, 8, c->stream));
-  if (M)
-    cub(c, [&](void* p, size_t& b) {
-      return rocprim::select(p, b, rocprim::counting_iterator<uint32_t>(0), entf.as<uint8_t>(), epos.as<uint32_t>(),
-                             ecount.as<unsigned>(), (size_t)M, c->stream);
-    });
-  uint32_t E = 0;
-  SUMINC_HIP(hipMemcpyAsync(&E, ecount.p, 4, hipMemcpyDeviceToHost, c->stream));
-  SUMINC_HIP(hipStreamSynchronize(c->stream));
-  s.E = E;
-  s.cl_ent.ensure_slack((size_t)E * 4 + 4); ekey.ensure_slack((size_t)E * 4 + 4);
-  const bool fresh_ent_of = !rebuild || !s.ent_of.p || s.ent_of.cap < (size_t)n * 4;
-  s.ent_of.ensure((size_t)n * 4);
-  s.cl_eoff.ensure(((size_t)K + 1) * 4); rowlen.ensure_slack(((size_t)E + 1) * 8);
-  rowlen2.ensure_slack(((size_t)E + 1) * 8);
-  s.sc_base.ensure_slack(((size_t)E + 1) * 8); s.ss_base.ensure_slack(((size_t)E + 1) * 8);
-  if (fresh_ent_of) k_fill32<<<grid_for(n), kBlock, 0, c->stream>>>(s.ent_of.as<uint32_t>(), n, INF32);
-  else if (M) k_ent_of_clear<<<grid_for(M), kBlock, 0, c->stream>>>(M, s.cl_mem.as<uint32_t>(), s.ent_of.as<uint32_t>());
-  k_entries<<<grid_for((uint64_t)E + 1), kBlock, 0, c->stream>>>(
-      E, epos.as<uint32_t>(), s.cl_mem.as<uint32_t>(), ckey.as<uint32_t>(), s.cl_off.as<uint32_t>(),
-      s.cl_nexit.as<uint32_t>(), s.cl_ent.as<uint32_t>(), ekey.as<uint32_t>(), s.ent_of.as<uint32_t>(),
-      rowlen.as<unsigned long long>(), rowlen2.as<unsigned long long>());
-  k_row_ptr32<uint32_t><<<grid_for((uint64_t)E + 1), kBlock, 0, c->stream>>>(ekey.as<uint32_t>(), E, K,
-                                                                             s.cl_eoff.as<uint32_t>(), 0);
+  SUMINC_HIP(hipMemsetAsync(s.lay_hist.p, 0, 32, c->stream));
+  ClBase* cnt = s.lay_cnt.as<ClBase>();
+  ClBase* base = s.lay_base.as<ClBase>();
+  k_member_partition<<<layout_grid(K), kBlock, 0, c->stream>>>(
+      K, s.cl_off.as<uint32_t>(), s.base_mem.as<uint32_t>(), s.ext_out.as<uint32_t>(), s.ext_in.as<uint32_t>(),
+      s.ntype.as<uint8_t>(), s.is_entry.as<uint8_t>(), s.is_exit.as<uint8_t>(), s.cl_mem.as<uint32_t>(),
+      s.lidx.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), cnt, s.lay_hist.as<uint32_t>());
   cub(c, [&](void* p, size_t& b) {
-    return rocprim::exclusive_scan(p, b, rowlen.as<unsigned long long>(), s.sc_base.as<unsigned long long>(), 0ull,
-                                   (size_t)E + 1, rocprim::plus<unsigned long long>(), c->stream);
+    return rocprim::exclusive_scan(p, b, cnt, base, ClBase{0ull, 0ull, 0u, 0u}, (size_t)K + 1, ClBasePlus(),
+                                   c->stream);
   });
-  cub(c, [&](void* p, size_t& b) {
-    return rocprim::exclusive_scan(p, b, rowlen2.as<unsigned long long>(), s.ss_base.as<unsigned long long>(), 0ull,
-                                   (size_t)E + 1, rocprim::plus<unsigned long long>(), c->stream);
-  });
-  unsigned long long sc_len = 0, ss_len = 0;
-  SUMINC_HIP(hipMemcpyAsync(&sc_len, s.sc_base.as<unsigned long long>() + E, 8, hipMemcpyDeviceToHost, c->stream));
-  SUMINC_HIP(hipMemcpyAsync(&ss_len, s.ss_base.as<unsigned long long>() + E, 8, hipMemcpyDeviceToHost, c->stream));
-  // internal edges grouped by destination slot
+  k_entry_layout<<<layout_grid(K), kBlock, 0, c->stream>>>(
+      K, s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), s.is_entry.as<uint8_t>(),
+      base, s.cl_eoff.as<uint32_t>(), s.cl_ent.as<uint32_t>(), s.ent_of.as<uint32_t>(),
+      s.sc_base.as<unsigned long long>(), s.ss_base.as<unsigned long long>());
+  // in-cluster edge arrays
   const uint64_t ie_m = s.iem_n;
   s.ie_m = ie_m;
-  s.ie_off.ensure(((size_t)M + 1) * 4);
+  s.ie_off.ensure(((size_t)K + 1) * 4);
   for (DevBuf* b : {&s.ie_src, &s.ie_dst, &s.ie_deg, &s.ie_wt}) b->ensure_slack(ie_m * 4 + 4);
-  if (ie_m) {
-    skey_a.ensure_slack(ie_m * 8); skey_b.ensure_slack(ie_m * 8); sidx_a.ensure_slack(ie_m * 4);
-    k_internal_slot_keys<<<grid_for(ie_m), kBlock, 0, c->stream>>>(ie_m, s.iem_key.as<unsigned long long>(),
-                                                                   s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(),
-                                                                   s.lidx.as<uint32_t>(),
-                                                                   skey_a.as<unsigned long long>(),
-                                                                   sidx_a.as<uint32_t>());
-    cub(c, [&](void* p, size_t& b) {
-      return rocprim::radix_sort_keys(p, b, skey_a.as<uint64_t>(), skey_b.as<uint64_t>(), (size_t)ie_m, 0, 64,
-                                      c->stream);
-    });
-    k_internal_fill<<<grid_for(ie_m), kBlock, 0, c->stream>>>(
-        ie_m, skey_b.as<unsigned long long>(), s.iem_key.as<unsigned long long>(), s.iem_w.as<uint32_t>(),
-        s.lidx.as<uint32_t>(), c->out_off.as<uint32_t>(), s.ie_src.as<uint32_t>(), s.ie_dst.as<uint32_t>(),
-        s.ie_deg.as<uint32_t>(), s.ie_wt.as<uint32_t>());
-    k_row_ptr32<unsigned long long><<<grid_for(ie_m + 1), kBlock, 0, c->stream>>>(
-        skey_b.as<unsigned long long>(), (uint32_t)ie_m, M, s.ie_off.as<uint32_t>(), 32);
-  } else {
-    SUMINC_HIP(hipMemsetAsync(s.ie_off.p, 0, ((size_t)M + 1) * 4, c->stream));
-  }
-  mx.ensure(4);
-  SUMINC_HIP(hipMemsetAsync(mx.p, 0, 4, c->stream));
-  if (K) k_max_cluster<<<grid_for(K), kBlock, 0, c->stream>>>(K, s.cl_off.as<uint32_t>(), mx.as<uint32_t>());
-  uint32_t maxc = 0;
-  SUMINC_HIP(hipMemcpyAsync(&maxc, mx.p, 4, hipMemcpyDeviceToHost, c->stream));
+  k_internal_fill<<<grid_for(ie_m + 1), kBlock, 0, c->stream>>>(
+      ie_m, K, s.iem_key.as<unsigned long long>(), s.iem_w.as<uint32_t>(), s.base_mem.as<uint32_t>(),
+      s.cid.as<uint32_t>(), s.lidx.as<uint32_t>(), c->out_off.as<uint32_t>(), s.ie_off.as<uint32_t>(),
+      s.ie_src.as<uint32_t>(), s.ie_dst.as<uint32_t>(), s.ie_deg.as<uint32_t>(), s.ie_wt.as<uint32_t>());
   SUMINC_HIP(hipGetLastError());
+  ClBase tot{};
+  uint32_t hist[8];
+  SUMINC_HIP(hipMemcpyAsync(hist, s.lay_hist.p, 32, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipMemcpyAsync(&tot, base + K, sizeof(ClBase), hipMemcpyDeviceToHost, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
-  SUMINC_REQUIRE(maxc <= (uint32_t)kMaxCluster, SUMINC_E_TOO_LARGE, "an accepted cluster has more than 2048 members");
-  s.max_cluster = maxc;
-  s.sc_len = sc_len;
-  s.ss_len = ss_len;
+  uint32_t typed = 0;
+  for (int t = 1; t < 5; ++t) {
+    s.counts[t] = hist[t];
+    typed += hist[t];
+  }
+  SUMINC_REQUIRE(typed == M, SUMINC_E_STATE, "supernode member count changed");
+  s.counts[0] = n - M;
+  s.E = tot.ent;
+  s.sc_len = tot.sc;
+  s.ss_len = tot.ss;
   // shortcut rows: validity per cluster (PageRank, SSSP)
   s.pr_valid.ensure((size_t)K + 1); s.ss_valid.ensure((size_t)K + 1);
   if (!rebuild) {
@@ -594,36 +619,32 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
     s.pr_rows_for = -1;
     s.ss_rows = false;
   } else {
+    const ClBase* old = s.lay_base_old.as<ClBase>();
     if (s.pr_rows_for >= 0) {
+      s.sc_R.swap(s.sc_R_old); s.sc_D.swap(s.sc_D_old);
       const size_t es = s.pr_rows_for == SUMINC_PR_FP64 ? 8 : 4;
-      s.sc_R.ensure_slack(sc_len * es + es); s.sc_D.ensure_slack(sc_len * es + es);
-      if (E && es == 8)
-        k_rows_move<double><<<E, kBlock, 0, c->stream>>>(
-            E, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(), nullptr,
-            s.sc_base.as<unsigned long long>(), s.ent_of_old.as<uint32_t>(), s.sc_base_old.as<unsigned long long>(),
-            s.pr_valid.as<uint8_t>(), s.sc_R_old.as<double>(), s.sc_R.as<double>(), s.sc_D_old.as<double>(),
-            s.sc_D.as<double>());
-      else if (E)
-        k_rows_move<float><<<E, kBlock, 0, c->stream>>>(
-            E, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(), nullptr,
-            s.sc_base.as<unsigned long long>(), s.ent_of_old.as<uint32_t>(), s.sc_base_old.as<unsigned long long>(),
-            s.pr_valid.as<uint8_t>(), s.sc_R_old.as<float>(), s.sc_R.as<float>(), s.sc_D_old.as<float>(),
-            s.sc_D.as<float>());
+      s.sc_R.ensure_slack(s.sc_len * es + es); s.sc_D.ensure_slack(s.sc_len * es + es);
+      if (s.sc_len && es == 8)
+        k_rows_move<double, false><<<move_grid(s.sc_len), kBlock, 0, c->stream>>>(
+            s.sc_len, K, base, old, s.pr_valid.as<uint8_t>(), s.sc_R_old.as<double>(), s.sc_R.as<double>(),
+            s.sc_D_old.as<double>(), s.sc_D.as<double>());
+      else if (s.sc_len)
+        k_rows_move<float, false><<<move_grid(s.sc_len), kBlock, 0, c->stream>>>(
+            s.sc_len, K, base, old, s.pr_valid.as<uint8_t>(), s.sc_R_old.as<float>(), s.sc_R.as<float>(),
+            s.sc_D_old.as<float>(), s.sc_D.as<float>());
     }
     if (s.ss_rows) {
-      s.ss_dist.ensure_slack(ss_len * 4 + 4); s.ss_par.ensure_slack(ss_len * 4 + 4);
-      if (E)
-        k_rows_move<uint32_t><<<E, kBlock, 0, c->stream>>>(
-            E, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(), s.cl_nexit.as<uint32_t>(),
-            s.ss_base.as<unsigned long long>(), s.ent_of_old.as<uint32_t>(), s.ss_base_old.as<unsigned long long>(),
-            s.ss_valid.as<uint8_t>(), s.ss_dist_old.as<uint32_t>(), s.ss_dist.as<uint32_t>(),
-            s.ss_par_old.as<uint32_t>(), s.ss_par.as<uint32_t>());
+      s.ss_dist.swap(s.ss_dist_old); s.ss_par.swap(s.ss_par_old);
+      s.ss_dist.ensure_slack(s.ss_len * 4 + 4); s.ss_par.ensure_slack(s.ss_len * 4 + 4);
+      if (s.ss_len)
+        k_rows_move<uint32_t, true><<<move_grid(s.ss_len), kBlock, 0, c->stream>>>(
+            s.ss_len, K, base, old, s.ss_valid.as<uint8_t>(), s.ss_dist_old.as<uint32_t>(),
+            s.ss_dist.as<uint32_t>(), s.ss_par_old.as<uint32_t>(), s.ss_par.as<uint32_t>());
     }
     SUMINC_HIP(hipGetLastError());
   }
   s.pr_need = s.ss_need = true;
   s.version = c->graph_version;
-  SUMINC_HIP(hipStreamSynchronize(c->stream));
 }
 
 // First build from the raw cluster assignment: scans the rows of the raw members,
@@ -688,20 +709,47 @@ void sketch_build(suminc_ctx* c, uint32_t min_node, uint32_t max_node, double th
   s.K = K;
   {
     // the internal edges of the accepted clusters size the list
-    std::vector<uint32_t> hin(Kr + 1);
+    std::vector<uint32_t> hin(Kr + 1), hsize(Kr + 1);
     std::vector<uint8_t> hacc(Kr + 1);
     SUMINC_HIP(hipMemcpy(hin.data(), inner.p, (size_t)Kr * 4, hipMemcpyDeviceToHost));
+    SUMINC_HIP(hipMemcpy(hsize.data(), size.p, (size_t)Kr * 4, hipMemcpyDeviceToHost));
     SUMINC_HIP(hipMemcpy(hacc.data(), s.raw_acc.p, Kr, hipMemcpyDeviceToHost));
-    uint64_t tot = 0;
+    uint64_t tot = 0, M = 0;
+    uint32_t maxc = 0;
     for (uint32_t k = 0; k < Kr; ++k)
-      if (hacc[k]) tot += hin[k];
+      if (hacc[k]) {
+        tot += hin[k];
+        M += hsize[k];
+        maxc = std::max(maxc, hsize[k]);
+      }
+    SUMINC_REQUIRE(maxc <= (uint32_t)kMaxCluster, SUMINC_E_TOO_LARGE, "an accepted cluster has more than 2048 members");
+    s.M = (uint32_t)M;
+    s.max_cluster = maxc;
+    // base member order (cluster, id), cluster offsets, base slots
+    DevBuf keys_a, keys_b;
+    keys_a.ensure((size_t)RM * 8 + 8); keys_b.ensure((size_t)RM * 8 + 8);
+    s.base_mem.ensure(M * 4 + 4); s.bslot.ensure((size_t)n * 4); s.cl_off.ensure(((size_t)K + 1) * 4);
+    if (RM) {
+      k_base_keys<<<grid_for(RM), kBlock, 0, c->stream>>>(s.rlist.as<uint32_t>(), RM, s.cid.as<uint32_t>(),
+                                                          keys_a.as<unsigned long long>());
+      cub(c, [&](void* p, size_t& b) {
+        return rocprim::radix_sort_keys(p, b, keys_a.as<uint64_t>(), keys_b.as<uint64_t>(), (size_t)RM, 0, 64,
+                                        c->stream);
+      });
+    }
+    if (M)
+      k_base_order<<<grid_for(M), kBlock, 0, c->stream>>>(keys_b.as<unsigned long long>(), (uint32_t)M,
+                                                          s.base_mem.as<uint32_t>(), s.bslot.as<uint32_t>());
+    k_row_ptr32<unsigned long long><<<grid_for(M + 1), kBlock, 0, c->stream>>>(
+        keys_b.as<unsigned long long>(), (uint32_t)M, K, s.cl_off.as<uint32_t>(), 32);
+    // internal edges, sorted by (base slot of dst, src)
     const size_t cap = tot + 4096;
     s.iem_key.ensure(cap * 8); s.iem_w.ensure(cap * 4); s.iem_key2.ensure(cap * 8); s.iem_w2.ensure(cap * 4);
     if (RM)
       k_internal_collect<<<(RM + kWaves - 1) / kWaves, kBlock, 0, c->stream>>>(
-          s.rlist.as<uint32_t>(), RM, s.cid.as<uint32_t>(), c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
-          c->wkind == SUMINC_W_U32 ? c->in_w.as<uint32_t>() : nullptr, s.iem_key2.as<unsigned long long>(),
-          s.iem_w2.as<uint32_t>(), icount.as<unsigned long long>());
+          s.rlist.as<uint32_t>(), RM, s.cid.as<uint32_t>(), s.bslot.as<uint32_t>(), c->in_off.as<uint32_t>(),
+          c->in_col.as<uint32_t>(), c->wkind == SUMINC_W_U32 ? c->in_w.as<uint32_t>() : nullptr,
+          s.iem_key2.as<unsigned long long>(), s.iem_w2.as<uint32_t>(), icount.as<unsigned long long>());
     SUMINC_HIP(hipGetLastError());
     unsigned long long got = 0;
     SUMINC_HIP(hipMemcpyAsync(&got, icount.p, 8, hipMemcpyDeviceToHost, c->stream));
@@ -736,7 +784,8 @@ __global__ void k_sk_batch(const uint32_t* __restrict__ row, const uint32_t* __r
                            const uint8_t* __restrict__ act, const uint8_t* __restrict__ ex,
                            const uint32_t* __restrict__ gw, const unsigned* countp, uint32_t N, int wu32,
                            const uint32_t* __restrict__ cid, uint8_t* dirty, uint8_t* pr_valid, uint8_t* ss_valid,
-                           uint32_t* ext_out, uint32_t* ext_in, const unsigned long long* __restrict__ iem_key,
+                           uint32_t* ext_out, uint32_t* ext_in, const uint32_t* __restrict__ bslot,
+                           const unsigned long long* __restrict__ iem_key,
                            uint64_t iem_n, uint8_t* iem_dead, unsigned long long* add_key, uint32_t* add_w,
                            unsigned long long* add_cnt, unsigned long long* delta) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -755,7 +804,7 @@ __global__ void k_sk_batch(const uint32_t* __restrict__ row, const uint32_t* __r
       if (cv != INF32) atomicAdd(&ext_in[v], added ? 1u : 0xFFFFFFFFu);
     }
     if (cu != INF32 && cu == cv) {
-      key = ((unsigned long long)u << 32) | v;
+      key = ((unsigned long long)bslot[v] << 32) | u;
       if (existed) {  // deleted or re-weighted: the old entry goes
         uint64_t lo = 0, hi = iem_n;
         while (lo < hi) {
@@ -781,62 +830,151 @@ __global__ void k_sk_batch(const uint32_t* __restrict__ row, const uint32_t* __r
   }
 }
 
-__global__ void k_sk_batch_prep(unsigned long long* key_tail, uint32_t N, unsigned long long* add_cnt,
-                                unsigned long long* delta) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < N) key_tail[i] = ~0ull;
-  if (i == 0) {
+__global__ void k_sk_batch_prep(unsigned long long* add_cnt, unsigned long long* delta) {
+  if (threadIdx.x == 0) {
     *add_cnt = 0;
     *delta = 0;
-  }
-}
-
-__global__ void k_sk_drop_dead(uint64_t n, unsigned long long* key, uint8_t* dead) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && dead[i]) {
-    key[i] = ~0ull;
-    dead[i] = 0;
   }
 }
 
 void sketch_batch(suminc_ctx* c, const uint32_t* rows, const uint32_t* cols, const uint8_t* act, const uint8_t* ex,
                   const uint32_t* w, const unsigned* countp, uint32_t N) {
   Sketch& s = c->sk;
+  s.batch_pending = false;
   if (!s.built || !N) return;
   const uint64_t n0 = s.iem_n, tot = n0 + N;
   grow_keep(c, s.iem_key, tot * 8 + 8, n0 * 8);
   grow_keep(c, s.iem_w, tot * 4 + 4, n0 * 4);
   grow_keep(c, s.iem_dead, tot + 1, n0);
-  SUMINC_HIP(hipMemsetAsync(s.iem_dead.as<uint8_t>() + n0, 0, N + 1, c->stream));
   s.iem_key2.ensure_slack(tot * 8 + 8);
   s.iem_w2.ensure_slack(tot * 4 + 4);
   Counters* cnt = dev_counters(c);
   unsigned long long* add_cnt = &cnt->aux[7];
   unsigned long long* delta = &cnt->aux[8];
-  k_sk_batch_prep<<<grid_for(N), kBlock, 0, c->stream>>>(s.iem_key.as<unsigned long long>() + n0, N, add_cnt, delta);
+  k_sk_batch_prep<<<1, 64, 0, c->stream>>>(add_cnt, delta);
   k_sk_batch<<<grid_for(N), kBlock, 0, c->stream>>>(
       rows, cols, act, ex, w, countp, N, c->wkind == SUMINC_W_U32, s.cid.as<uint32_t>(), s.dirty.as<uint8_t>(),
       s.pr_valid.as<uint8_t>(), s.ss_valid.as<uint8_t>(), s.ext_out.as<uint32_t>(), s.ext_in.as<uint32_t>(),
-      s.iem_key.as<unsigned long long>(), n0, s.iem_dead.as<uint8_t>(), s.iem_key.as<unsigned long long>() + n0,
-      s.iem_w.as<uint32_t>() + n0, add_cnt, delta);
-  if (n0)
-    k_sk_drop_dead<<<grid_for(n0), kBlock, 0, c->stream>>>(n0, s.iem_key.as<unsigned long long>(),
-                                                          s.iem_dead.as<uint8_t>());
-  cub(c, [&](void* p, size_t& b) {
-    return rocprim::radix_sort_pairs(p, b, s.iem_key.as<uint64_t>(), s.iem_key2.as<uint64_t>(),
-                                     s.iem_w.as<uint32_t>(), s.iem_w2.as<uint32_t>(), (size_t)tot, 0, 64, c->stream);
-  });
-  s.iem_key.swap(s.iem_key2);
-  s.iem_w.swap(s.iem_w2);
+      s.bslot.as<uint32_t>(), s.iem_key.as<unsigned long long>(), n0, s.iem_dead.as<uint8_t>(),
+      s.iem_key.as<unsigned long long>() + n0, s.iem_w.as<uint32_t>() + n0, add_cnt, delta);
   SUMINC_HIP(hipGetLastError());
+  s.batch_pending = true;
   s.pr_need = s.ss_need = true;
 }
 
-// After the batch's final synchronisation: the live length of the internal list.
+// Sorts up to kSmallSort (key, weight) pairs in one workgroup (bitonic network in LDS).
+constexpr uint32_t kSmallSort = 2048;
+__global__ void __launch_bounds__(1024) k_sort_small(const unsigned long long* __restrict__ key,
+                                                     const uint32_t* __restrict__ w, uint32_t n,
+                                                     unsigned long long* okey, uint32_t* ow) {
+  __shared__ unsigned long long s_k[kSmallSort];
+  __shared__ uint32_t s_w[kSmallSort];
+  uint32_t P = 64;
+  while (P < n) P <<= 1;
+  for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+    s_k[i] = i < n ? key[i] : ~0ull;
+    s_w[i] = i < n ? w[i] : 0u;
+  }
+  __syncthreads();
+  for (uint32_t k = 2; k <= P; k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+        const uint32_t l = i ^ j;
+        if (l > i) {
+          const unsigned long long x = s_k[i], y = s_k[l];
+          if ((i & k) == 0 ? x > y : x < y) {
+            s_k[i] = y;
+            s_k[l] = x;
+            const uint32_t t = s_w[i];
+            s_w[i] = s_w[l];
+            s_w[l] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    okey[i] = s_k[i];
+    ow[i] = s_w[i];
+  }
+}
+
+struct LiveOne {
+  __host__ __device__ uint32_t operator()(uint8_t dead) const { return dead ? 0u : 1u; }
+};
+
+// Merges the sorted list without its dead entries with the sorted adds: an old
+// entry lands after the live entries and the adds below it, an add after the
+// adds and the live entries below it (keys are unique among live entries and adds).
+__global__ void k_iem_merge(uint64_t n0, uint64_t A, uint64_t live, const unsigned long long* __restrict__ key,
+                            const uint32_t* __restrict__ w, const uint8_t* __restrict__ dead,
+                            const uint32_t* __restrict__ pre, const unsigned long long* __restrict__ akey,
+                            const uint32_t* __restrict__ aw, unsigned long long* okey, uint32_t* ow) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n0) {
+    if (dead[i]) return;
+    const unsigned long long x = key[i];
+    uint64_t lo = 0, hi = A;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (akey[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    const uint64_t q = pre[i] + lo;
+    okey[q] = x;
+    ow[q] = w[i];
+  } else if (i < n0 + A) {
+    const uint64_t j = i - n0;
+    const unsigned long long x = akey[j];
+    uint64_t lo = 0, hi = n0;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (key[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    const uint64_t q = j + (lo < n0 ? pre[lo] : live);
+    okey[q] = x;
+    ow[q] = aw[j];
+  }
+}
+
+// After the batch's final synchronisation (the add and dead counts are on the
+// host): merges the batch's internal adds into the sorted internal edge list and
+// drops its dead entries.
 void sketch_batch_done(suminc_ctx* c) {
   Sketch& s = c->sk;
-  if (!s.built) return;
-  s.iem_n = (uint64_t)((long long)s.iem_n + (long long)c->host_counters->aux[8]);
+  if (!s.built || !s.batch_pending) return;
+  s.batch_pending = false;
+  const uint64_t n0 = s.iem_n;
+  const uint64_t A = c->host_counters->aux[7];
+  const long long delta = (long long)c->host_counters->aux[8];
+  const uint64_t n1 = (uint64_t)((long long)n0 + delta);
+  const uint64_t D = (uint64_t)((long long)A - delta);
+  if (!A && !D) return;
+  s.add_key.ensure_slack(A * 8 + 8); s.add_w.ensure_slack(A * 4 + 4);
+  s.iem_pre.ensure_slack(n0 * 4 + 4);
+  const unsigned long long* tail_k = s.iem_key.as<unsigned long long>() + n0;
+  const uint32_t* tail_w = s.iem_w.as<uint32_t>() + n0;
+  if (A && A <= kSmallSort)
+    k_sort_small<<<1, 1024, 0, c->stream>>>(tail_k, tail_w, (uint32_t)A, s.add_key.as<unsigned long long>(),
+                                            s.add_w.as<uint32_t>());
+  else if (A)
+    cub(c, [&](void* p, size_t& b) {
+      return rocprim::radix_sort_pairs(p, b, tail_k, s.add_key.as<unsigned long long>(), tail_w,
+                                       s.add_w.as<uint32_t>(), (size_t)A, 0, 64, c->stream);
+    });
+  if (n0)
+    cub(c, [&](void* p, size_t& b) {
+      return rocprim::exclusive_scan(p, b, rocprim::make_transform_iterator(s.iem_dead.as<uint8_t>(), LiveOne()),
+                                     s.iem_pre.as<uint32_t>(), 0u, (size_t)n0, rocprim::plus<uint32_t>(), c->stream);
+    });
+  k_iem_merge<<<grid_for(n0 + A), kBlock, 0, c->stream>>>(
+      n0, A, n0 - D, s.iem_key.as<unsigned long long>(), s.iem_w.as<uint32_t>(), s.iem_dead.as<uint8_t>(),
+      s.iem_pre.as<uint32_t>(), s.add_key.as<unsigned long long>(), s.add_w.as<uint32_t>(),
+      s.iem_key2.as<unsigned long long>(), s.iem_w2.as<uint32_t>());
+  SUMINC_HIP(hipGetLastError());
+  s.iem_key.swap(s.iem_key2);
+  s.iem_w.swap(s.iem_w2);
+  SUMINC_HIP(hipMemsetAsync(s.iem_dead.p, 0, std::max(n0, n1) + 1, c->stream));  // grown tails were never cleared
+  s.iem_n = n1;
 }
 
 void sketch_refresh(suminc_ctx* c) {
@@ -914,7 +1052,7 @@ __global__ void __launch_bounds__(kBlock) k_pr_shortcuts(const uint32_t* __restr
       s_d[j] = j == src ? T(1) : T(0);
     }
     __syncthreads();
-    const uint32_t pb = ie.off[slot0], pe = ie.off[slot0 + size];
+    const uint32_t pb = ie.off[k], pe = ie.off[k + 1];
     uint32_t rounds = cluster_iterate<T>(size, pb, pe, ie, d, thr, maxit, s_r, s_d, s_n, s_red);
     edges += (unsigned long long)rounds * (pe - pb);
     const unsigned long long base = sc_base[e];
@@ -1222,7 +1360,7 @@ __global__ void __launch_bounds__(kBlock) k_sk_precompute(SkArgs a, const uint32
     s_d[j] = delta[v];
   }
   __syncthreads();
-  const uint32_t pb = ie.off[slot0], pe = ie.off[slot0 + size];
+  const uint32_t pb = ie.off[k], pe = ie.off[k + 1];
   uint32_t rounds = cluster_iterate<T>(size, pb, pe, ie, d, thr_per_vertex * size, maxit, s_r, s_d, s_n, s_red);
   for (uint32_t j0 = 0; j0 < size; j0 += kBlock) {  // uniform trip count: push_item uses wave ballots
     const uint32_t j = j0 + threadIdx.x;

@@ -5,9 +5,9 @@
 namespace suminc {
 
 // Internal edges of one cluster as the in-cluster kernels read them: the edges
-// with destination slot in [slot0, slot0 + size) occupy [off[slot0], off[slot0 + size]).
+// of cluster k (both endpoints in it) occupy [off[k], off[k + 1]).
 struct Internal {
-  const uint32_t* off;  // per member slot (M + 1)
+  const uint32_t* off;  // per cluster (K + 1)
   const uint32_t* src;  // local index of the source
   const uint32_t* dst;  // local index of the destination
   const uint32_t* deg;  // out-degree of the source

@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts(const uint32_t* __restr
   for (uint32_t li = blockIdx.x; li < nk; li += gridDim.x) {
   const uint32_t k = list[li];
   const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0, nexit = cl_nexit[k];
-  const uint32_t pb = ie.off[slot0], pe = ie.off[slot0 + size];
+  const uint32_t pb = ie.off[k], pe = ie.off[k + 1];
   for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_id[j] = cl_mem[slot0 + j];
   for (uint32_t e = cl_eoff[k]; e < cl_eoff[k + 1]; ++e) {
     const uint32_t src = lidx[cl_ent[e]];
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(kBlock) k_ss_relax_clusters(SsArgs a, Internal
   for (uint32_t li = blockIdx.x; li < nk; li += gridDim.x) {
   const uint32_t k = list[li];
   const uint32_t slot0 = a.cl_off[k], size = a.cl_off[k + 1] - slot0, nexit = a.cl_nexit[k];
-  const uint32_t pb = ie.off[slot0], pe = ie.off[slot0 + size];
+  const uint32_t pb = ie.off[k], pe = ie.off[k + 1];
   for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
     const uint32_t v = a.cl_mem[slot0 + j];
     s_id[j] = v;
