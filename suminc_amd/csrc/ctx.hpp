@@ -125,11 +125,13 @@ struct Sketch {
   bool ss_need = true;       // some SSSP rows are invalid
   DevBuf raw_cid, raw_acc;   // input assignment (n), acceptance per raw cluster (Kraw)
   DevBuf rlist;              // vertices with a raw cluster id (RM)
-  DevBuf cid, ntype, lidx, is_entry, is_exit;
+  DevBuf cid, ntype, lidx;
   DevBuf base_mem, bslot;    // members in (cluster, id) order (M), base slot of each member (n); fixed
   bool batch_pending = false;  // a batch updated the internal list; merge pending
   DevBuf ext_out, ext_in;    // per clustered vertex: edges leaving / entering its cluster
   DevBuf cl_off, cl_mem, cl_nexit, cl_eoff, cl_ent, ent_of, sc_base, ss_base;
+  DevBuf slot_ent;           // M: the member in this slot is an entry
+  DevBuf touched;            // K: cluster touched by a batch since the last layout
   // internal edges: the list sorted by (base slot of dst, src), kept up to date by
   // every batch (sorted adds merged in, dead entries dropped), and the
   // per-cluster arrays the in-cluster kernels read

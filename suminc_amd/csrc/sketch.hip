@@ -333,88 +333,95 @@ __global__ void k_row_ptr32(const K* rowkey, uint32_t M, uint32_t K_, uint32_t* 
   for (int64_t r = prev + 1; r <= row; ++r) off[r] = i;
 }
 
-// One wave per cluster: node types from the edge counts (BothOutIn / OnlyIn /
-// OnlyOut / Inner), the stable exits-first member order, local indices, the
-// per-cluster counts for the scan, and the type histogram.
-constexpr uint32_t kLayoutBlocks = 1024;
+// One wave per cluster touched since the last layout (untouched clusters keep
+// their types, order and counts): node types from the edge counts (BothOutIn /
+// OnlyIn / OnlyOut / Inner), the stable exits-first member order with an entry
+// flag per slot, local indices and the per-cluster counts for the scan.  The
+// first 64 members stay in registers between the two passes.
 __global__ void __launch_bounds__(kBlock) k_member_partition(uint32_t K, const uint32_t* __restrict__ cl_off,
                                                              const uint32_t* __restrict__ base_mem,
                                                              const uint32_t* __restrict__ ext_out,
-                                                             const uint32_t* __restrict__ ext_in, uint8_t* ntype,
-                                                             uint8_t* is_entry, uint8_t* is_exit, uint32_t* cl_mem,
-                                                             uint32_t* lidx, uint32_t* cl_nexit, ClBase* cl_cnt,
-                                                             uint32_t* hist) {
-  __shared__ uint32_t s_h[8];
-  if (threadIdx.x < 8) s_h[threadIdx.x] = 0;
-  __syncthreads();
+                                                             const uint32_t* __restrict__ ext_in,
+                                                             const uint8_t* __restrict__ touched, uint8_t* ntype,
+                                                             uint32_t* cl_mem, uint8_t* slot_ent, uint32_t* lidx,
+                                                             uint32_t* cl_nexit, ClBase* cl_cnt) {
   const int lane = lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
-  uint32_t h1 = 0, h2 = 0, h3 = 0, h4 = 0;
   const uint32_t nw = gridDim.x * kWaves;
   for (uint32_t k = blockIdx.x * kWaves + (threadIdx.x >> 6); k < K; k += nw) {
+    if (!touched[k]) continue;  // whole wave
     const uint32_t b = cl_off[k], size = cl_off[k + 1] - b;
-    uint32_t nex = 0, nen = 0, nboth = 0;
+    uint32_t nex = 0, nen = 0;
+    uint32_t v0 = 0;
+    bool en0 = false, ex0 = false;
     for (uint32_t j0 = 0; j0 < size; j0 += 64) {
       const uint32_t j = j0 + lane;
       bool en = false, ex = false;
+      uint32_t v = 0;
       if (j < size) {
-        const uint32_t v = base_mem[b + j];
+        v = base_mem[b + j];
         en = ext_in[v] != 0;
         ex = ext_out[v] != 0;
-        ntype[v] = en && ex ? 3 : en ? 2 : ex ? 1 : 4;
-        is_entry[v] = en;
-        is_exit[v] = ex;
+        const uint8_t t = en && ex ? 3 : en ? 2 : ex ? 1 : 4;
+        if (ntype[v] != t) ntype[v] = t;
+      }
+      if (j0 == 0) {
+        v0 = v;
+        en0 = en;
+        ex0 = ex;
       }
       const unsigned long long mex = __ballot(ex), men = __ballot(en);
       nex += __popcll(mex);
       nen += __popcll(men);
-      nboth += __popcll(mex & men);
     }
     uint32_t pe = 0, pi = nex;
     for (uint32_t j0 = 0; j0 < size; j0 += 64) {
       const uint32_t j = j0 + lane;
       const bool in = j < size;
-      uint32_t v = 0;
-      bool ex = false;
-      if (in) {
+      uint32_t v = v0;
+      bool ex = ex0, en = en0;
+      if (in && j0 != 0) {
         v = base_mem[b + j];
         ex = ext_out[v] != 0;
+        en = ext_in[v] != 0;
       }
-      const unsigned long long mex = __ballot(ex), mrest = __ballot(in && !ex);
+      const unsigned long long mex = __ballot(in && ex), mrest = __ballot(in && !ex);
       if (in) {
         const uint32_t pos = ex ? pe + __popcll(mex & below) : pi + __popcll(mrest & below);
         cl_mem[b + pos] = v;
+        slot_ent[b + pos] = en;
         lidx[v] = pos;
       }
       pe += __popcll(mex);
       pi += __popcll(mrest);
     }
-    h1 += nex - nboth;
-    h2 += nen - nboth;
-    h3 += nboth;
-    h4 += size - (nex + nen - nboth);
     if (lane == 0) {
       cl_nexit[k] = nex;
       cl_cnt[k] = ClBase{(unsigned long long)nen * size, (unsigned long long)nen * nex, nen, 0u};
     }
   }
-  if (lane == 0) {
-    atomicAdd(&s_h[1], h1);
-    atomicAdd(&s_h[2], h2);
-    atomicAdd(&s_h[3], h3);
-    atomicAdd(&s_h[4], h4);
-  }
   if (blockIdx.x == 0 && threadIdx.x == 0) cl_cnt[K] = ClBase{0ull, 0ull, 0u, 0u};
+}
+
+// Node-type histogram of the members (for the structure report only).
+__global__ void k_type_hist(uint32_t M, const uint32_t* __restrict__ base_mem, const uint8_t* __restrict__ ntype,
+                            uint32_t* hist) {
+  __shared__ uint32_t s_h[8];
+  if (threadIdx.x < 8) s_h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < M) atomicAdd(&s_h[ntype[base_mem[i]] & 7], 1u);
   __syncthreads();
   if (threadIdx.x < 8 && s_h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], s_h[threadIdx.x]);
 }
 
-// One wave per cluster: entries in member order, their shortcut-row bases, and
-// ent_of for every member (INF32 for non-entries).
+// One wave per cluster: entries in member order, their shortcut-row bases and
+// ent_of (INF32 for the other members of touched clusters; untouched clusters
+// have the same non-entries as before).  Clears the touched flags.
 __global__ void __launch_bounds__(kBlock) k_entry_layout(uint32_t K, const uint32_t* __restrict__ cl_off,
                                                          const uint32_t* __restrict__ cl_mem,
                                                          const uint32_t* __restrict__ cl_nexit,
-                                                         const uint8_t* __restrict__ is_entry,
+                                                         const uint8_t* __restrict__ slot_ent, uint8_t* touched,
                                                          const ClBase* __restrict__ base, uint32_t* cl_eoff,
                                                          uint32_t* cl_ent, uint32_t* ent_of,
                                                          unsigned long long* sc_base, unsigned long long* ss_base) {
@@ -423,31 +430,29 @@ __global__ void __launch_bounds__(kBlock) k_entry_layout(uint32_t K, const uint3
   const uint32_t nw = gridDim.x * kWaves;
   for (uint32_t k = blockIdx.x * kWaves + (threadIdx.x >> 6); k < K; k += nw) {
     const uint32_t b = cl_off[k], size = cl_off[k + 1] - b, nex = cl_nexit[k];
+    const bool tk = touched[k] != 0;
     const ClBase cb = base[k];
     uint32_t r = 0;
     for (uint32_t j0 = 0; j0 < size; j0 += 64) {
       const uint32_t j = j0 + lane;
-      uint32_t v = 0;
-      bool en = false;
-      if (j < size) {
-        v = cl_mem[b + j];
-        en = is_entry[v] != 0;
-      }
+      const bool en = j < size && slot_ent[b + j] != 0;
       const unsigned long long men = __ballot(en);
-      if (j < size) {
-        if (en) {
-          const uint32_t q = r + __popcll(men & below), e = cb.ent + q;
-          cl_ent[e] = v;
-          ent_of[v] = e;
-          sc_base[e] = cb.sc + (unsigned long long)q * size;
-          ss_base[e] = cb.ss + (unsigned long long)q * nex;
-        } else {
-          ent_of[v] = INF32;
-        }
+      if (en) {
+        const uint32_t v = cl_mem[b + j];
+        const uint32_t q = r + __popcll(men & below), e = cb.ent + q;
+        cl_ent[e] = v;
+        ent_of[v] = e;
+        sc_base[e] = cb.sc + (unsigned long long)q * size;
+        ss_base[e] = cb.ss + (unsigned long long)q * nex;
+      } else if (tk && j < size) {
+        ent_of[cl_mem[b + j]] = INF32;
       }
       r += __popcll(men);
     }
-    if (lane == 0) cl_eoff[k] = cb.ent;
+    if (lane == 0) {
+      cl_eoff[k] = cb.ent;
+      if (tk) touched[k] = 0;
+    }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     const ClBase t = base[K];
@@ -545,9 +550,7 @@ static void grow_keep(suminc_ctx* c, DevBuf& b, size_t bytes, size_t keep) {
   b.swap(t);
 }
 
-static uint32_t layout_grid(uint32_t K) {
-  return std::max<uint32_t>(1u, std::min<uint32_t>(kLayoutBlocks, (K + kWaves - 1) / kWaves));
-}
+static uint32_t layout_grid(uint32_t K) { return std::max<uint32_t>(1u, (K + kWaves - 1) / kWaves); }
 
 static uint32_t move_grid(uint64_t len) {
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(4096, (len + kBlock - 1) / kBlock));
@@ -564,27 +567,31 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
   s.lay_hist.ensure(32);
   s.lay_cnt.ensure(((size_t)K + 1) * sizeof(ClBase));
   s.lay_base.ensure(((size_t)K + 1) * sizeof(ClBase));
-  s.cl_mem.ensure((size_t)M * 4 + 4); s.cl_nexit.ensure((size_t)K * 4 + 4); s.cl_eoff.ensure(((size_t)K + 1) * 4);
+  s.cl_mem.ensure((size_t)M * 4 + 4); s.slot_ent.ensure((size_t)M + 1); s.cl_nexit.ensure((size_t)K * 4 + 4);
+  s.cl_eoff.ensure(((size_t)K + 1) * 4);
   s.cl_ent.ensure((size_t)M * 4 + 4);
   s.sc_base.ensure(((size_t)M + 1) * 8); s.ss_base.ensure(((size_t)M + 1) * 8);
   if (!rebuild || !s.ent_of.p) {
     s.ent_of.ensure((size_t)n * 4);
     k_fill32<<<grid_for(n), kBlock, 0, c->stream>>>(s.ent_of.as<uint32_t>(), n, INF32);
   }
-  SUMINC_HIP(hipMemsetAsync(s.lay_hist.p, 0, 32, c->stream));
   ClBase* cnt = s.lay_cnt.as<ClBase>();
   ClBase* base = s.lay_base.as<ClBase>();
+  if (!rebuild) {  // every cluster is laid out by the first layout
+    s.touched.ensure((size_t)K + 1);
+    SUMINC_HIP(hipMemsetAsync(s.touched.p, 1, (size_t)K + 1, c->stream));
+  }
   k_member_partition<<<layout_grid(K), kBlock, 0, c->stream>>>(
       K, s.cl_off.as<uint32_t>(), s.base_mem.as<uint32_t>(), s.ext_out.as<uint32_t>(), s.ext_in.as<uint32_t>(),
-      s.ntype.as<uint8_t>(), s.is_entry.as<uint8_t>(), s.is_exit.as<uint8_t>(), s.cl_mem.as<uint32_t>(),
-      s.lidx.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), cnt, s.lay_hist.as<uint32_t>());
+      s.touched.as<uint8_t>(), s.ntype.as<uint8_t>(), s.cl_mem.as<uint32_t>(), s.slot_ent.as<uint8_t>(),
+      s.lidx.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), cnt);
   cub(c, [&](void* p, size_t& b) {
     return rocprim::exclusive_scan(p, b, cnt, base, ClBase{0ull, 0ull, 0u, 0u}, (size_t)K + 1, ClBasePlus(),
                                    c->stream);
   });
   k_entry_layout<<<layout_grid(K), kBlock, 0, c->stream>>>(
-      K, s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), s.is_entry.as<uint8_t>(),
-      base, s.cl_eoff.as<uint32_t>(), s.cl_ent.as<uint32_t>(), s.ent_of.as<uint32_t>(),
+      K, s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), s.slot_ent.as<uint8_t>(),
+      s.touched.as<uint8_t>(), base, s.cl_eoff.as<uint32_t>(), s.cl_ent.as<uint32_t>(), s.ent_of.as<uint32_t>(),
       s.sc_base.as<unsigned long long>(), s.ss_base.as<unsigned long long>());
   // in-cluster edge arrays
   const uint64_t ie_m = s.iem_n;
@@ -597,17 +604,8 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
       s.ie_src.as<uint32_t>(), s.ie_dst.as<uint32_t>(), s.ie_deg.as<uint32_t>(), s.ie_wt.as<uint32_t>());
   SUMINC_HIP(hipGetLastError());
   ClBase tot{};
-  uint32_t hist[8];
-  SUMINC_HIP(hipMemcpyAsync(hist, s.lay_hist.p, 32, hipMemcpyDeviceToHost, c->stream));
   SUMINC_HIP(hipMemcpyAsync(&tot, base + K, sizeof(ClBase), hipMemcpyDeviceToHost, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
-  uint32_t typed = 0;
-  for (int t = 1; t < 5; ++t) {
-    s.counts[t] = hist[t];
-    typed += hist[t];
-  }
-  SUMINC_REQUIRE(typed == M, SUMINC_E_STATE, "supernode member count changed");
-  s.counts[0] = n - M;
   s.E = tot.ent;
   s.sc_len = tot.sc;
   s.ss_len = tot.ss;
@@ -670,9 +668,7 @@ void sketch_build(suminc_ctx* c, uint32_t min_node, uint32_t max_node, double th
   SUMINC_HIP(hipStreamSynchronize(c->stream));
   s.RM = rm;
   const uint32_t RM = rm;
-  s.is_entry.ensure(n); s.is_exit.ensure(n); s.ext_out.ensure((size_t)n * 4); s.ext_in.ensure((size_t)n * 4);
-  SUMINC_HIP(hipMemsetAsync(s.is_entry.p, 0, n, c->stream));
-  SUMINC_HIP(hipMemsetAsync(s.is_exit.p, 0, n, c->stream));
+  s.ext_out.ensure((size_t)n * 4); s.ext_in.ensure((size_t)n * 4);
   inner.ensure((size_t)Kr * 4 + 4);
   SUMINC_HIP(hipMemsetAsync(inner.p, 0, (size_t)Kr * 4 + 4, c->stream));
   if (RM)
@@ -784,6 +780,7 @@ __global__ void k_sk_batch(const uint32_t* __restrict__ row, const uint32_t* __r
                            const uint8_t* __restrict__ act, const uint8_t* __restrict__ ex,
                            const uint32_t* __restrict__ gw, const unsigned* countp, uint32_t N, int wu32,
                            const uint32_t* __restrict__ cid, uint8_t* dirty, uint8_t* pr_valid, uint8_t* ss_valid,
+                           uint8_t* touched,
                            uint32_t* ext_out, uint32_t* ext_in, const uint32_t* __restrict__ bslot,
                            const unsigned long long* __restrict__ iem_key,
                            uint64_t iem_n, uint8_t* iem_dead, unsigned long long* add_key, uint32_t* add_w,
@@ -795,8 +792,8 @@ __global__ void k_sk_batch(const uint32_t* __restrict__ row, const uint32_t* __r
   if (g < N && g < *countp) {
     const uint32_t u = row[g], v = col[g];
     const uint32_t cu = cid[u], cv = cid[v];
-    if (cu != INF32) { dirty[cu] = 1; pr_valid[cu] = 0; ss_valid[cu] = 0; }
-    if (cv != INF32) { dirty[cv] = 1; pr_valid[cv] = 0; ss_valid[cv] = 0; }
+    if (cu != INF32) { dirty[cu] = 1; pr_valid[cu] = 0; ss_valid[cu] = 0; touched[cu] = 1; }
+    if (cv != INF32) { dirty[cv] = 1; pr_valid[cv] = 0; ss_valid[cv] = 0; touched[cv] = 1; }
     const bool existed = ex[g], add = act[g] == 0;
     const bool removed = existed && !add, added = !existed && add;
     if ((removed || added) && cu != cv) {
@@ -854,8 +851,8 @@ void sketch_batch(suminc_ctx* c, const uint32_t* rows, const uint32_t* cols, con
   k_sk_batch_prep<<<1, 64, 0, c->stream>>>(add_cnt, delta);
   k_sk_batch<<<grid_for(N), kBlock, 0, c->stream>>>(
       rows, cols, act, ex, w, countp, N, c->wkind == SUMINC_W_U32, s.cid.as<uint32_t>(), s.dirty.as<uint8_t>(),
-      s.pr_valid.as<uint8_t>(), s.ss_valid.as<uint8_t>(), s.ext_out.as<uint32_t>(), s.ext_in.as<uint32_t>(),
-      s.bslot.as<uint32_t>(), s.iem_key.as<unsigned long long>(), n0, s.iem_dead.as<uint8_t>(),
+      s.pr_valid.as<uint8_t>(), s.ss_valid.as<uint8_t>(), s.touched.as<uint8_t>(), s.ext_out.as<uint32_t>(),
+      s.ext_in.as<uint32_t>(), s.bslot.as<uint32_t>(), s.iem_key.as<unsigned long long>(), n0, s.iem_dead.as<uint8_t>(),
       s.iem_key.as<unsigned long long>() + n0, s.iem_w.as<uint32_t>() + n0, add_cnt, delta);
   SUMINC_HIP(hipGetLastError());
   s.batch_pending = true;
@@ -1555,6 +1552,21 @@ void sketch_info(suminc_ctx* c, suminc_sketch_info* out) {
   sketch_refresh(c);
   Sketch& s = c->sk;
   std::memset(out, 0, sizeof(*out));
+  uint32_t hist[8] = {0};
+  s.lay_hist.ensure(32);
+  SUMINC_HIP(hipMemsetAsync(s.lay_hist.p, 0, 32, c->stream));
+  if (s.M)
+    k_type_hist<<<grid_for(s.M), kBlock, 0, c->stream>>>(s.M, s.base_mem.as<uint32_t>(), s.ntype.as<uint8_t>(),
+                                                         s.lay_hist.as<uint32_t>());
+  SUMINC_HIP(hipMemcpyAsync(hist, s.lay_hist.p, 32, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  uint32_t typed = 0;
+  for (int t = 1; t < 5; ++t) {
+    s.counts[t] = hist[t];
+    typed += hist[t];
+  }
+  SUMINC_REQUIRE(typed == s.M, SUMINC_E_STATE, "supernode member count changed");
+  s.counts[0] = c->n - s.M;
   out->clusters = s.K;
   out->raw_clusters = s.Kraw;
   out->entries = s.E;

@@ -59,17 +59,83 @@ __device__ uint32_t cluster_bf(uint32_t size, uint32_t pb, uint32_t pe, const ui
   return rounds;
 }
 
-// Clusters whose flag equals `want`, appended in any order to list (count on the
-// device): the per-cluster kernels below loop over such a list with a fixed grid.
-__global__ void k_cluster_list(uint32_t K, const uint8_t* __restrict__ flag, uint8_t want, uint32_t* list,
-                               unsigned long long* count) {
+constexpr unsigned kClusterGrid = 2048;
+constexpr uint32_t kSmallCluster = 64;  // clusters up to this size run one per wave
+
+// Clusters whose flag equals `want`, split by size: clusters of at most
+// kSmallCluster members at list[0, counts[0]), larger ones at
+// list[K - counts[1], K) (filled from the back).
+__global__ void k_cluster_lists(uint32_t K, const uint8_t* __restrict__ flag, uint8_t want,
+                                const uint32_t* __restrict__ cl_off, uint32_t* list, unsigned long long* counts) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool take = k < K && flag[k] == want;
-  const unsigned long long slot = wave_append(take, count);
-  if (take) list[slot] = k;
+  const bool small = take && cl_off[k + 1] - cl_off[k] <= kSmallCluster;
+  const unsigned long long s1 = wave_append(small, &counts[0]);
+  const unsigned long long s2 = wave_append(take && !small, &counts[1]);
+  if (small) list[s1] = k;
+  else if (take) list[K - 1 - s2] = k;
 }
 
-constexpr unsigned kClusterGrid = 2048;
+// Orders this wave's LDS traffic before what follows (wave-local exchange).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// cluster_bf for a cluster of at most 64 members run by one wave (s_id, s_dp are
+// this wave's LDS slots): up to 128 internal edges stay in registers across the
+// rounds; a round ends with a wave vote instead of workgroup barriers.
+__device__ uint32_t cluster_bf_wave(uint32_t size, uint32_t pb, uint32_t pe, const uint32_t* __restrict__ src,
+                                    const uint32_t* __restrict__ dst, const uint32_t* __restrict__ wt,
+                                    const uint32_t* s_id, unsigned long long* s_dp) {
+  const int lane = lane_id();
+  auto relax = [&](uint32_t i, uint32_t j, uint32_t w) -> bool {
+    const unsigned long long di = s_dp[i] >> 32;
+    if (di == 0xFFFFFFFFull) return false;
+    const unsigned long long nd = di + w;
+    if (nd >= 0xFFFFFFFFull) return false;
+    const unsigned long long cand = (nd << 32) | s_id[i];
+    if (cand < s_dp[j]) {
+      atomicMin(&s_dp[j], cand);
+      return true;
+    }
+    return false;
+  };
+  uint32_t rounds = 0;
+  if (pe - pb <= 128) {
+    uint32_t ei[2] = {0, 0}, ej[2] = {0, 0}, ew[2] = {0, 0};
+    bool ev[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const uint32_t p = pb + lane + 64 * r;
+      ev[r] = p < pe;
+      if (ev[r]) {
+        ei[r] = src[p];
+        ej[r] = dst[p];
+        ew[r] = wt[p];
+      }
+    }
+    while (true) {
+      bool changed = false;
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+        if (ev[r]) changed |= relax(ei[r], ej[r], ew[r]);
+      wave_lds_sync();
+      ++rounds;
+      if (!__any(changed) || rounds > size + 1) break;
+    }
+  } else {
+    while (true) {
+      bool changed = false;
+      for (uint32_t p = pb + lane; p < pe; p += 64) changed |= relax(src[p], dst[p], wt[p]);
+      wave_lds_sync();
+      ++rounds;
+      if (!__any(changed) || rounds > size + 1) break;
+    }
+  }
+  return rounds;
+}
 
 // Shortcut rows: for every entry e of cluster k, in-cluster distance and parent of
 // every exit of k (the first nexit members; the entry itself if it is an exit:
@@ -85,15 +151,15 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts(const uint32_t* __restr
                                                          const unsigned long long* __restrict__ ss_base,
                                                          Internal ie, uint32_t* __restrict__ sd,
                                                          uint32_t* __restrict__ sp, const uint32_t* __restrict__ list,
-                                                         const unsigned long long* __restrict__ count,
+                                                         uint32_t K, const unsigned long long* __restrict__ counts,
                                                          unsigned long long* work) {
   __shared__ unsigned long long s_dp[kMaxClusterS];
   __shared__ uint32_t s_id[kMaxClusterS];
   __shared__ int s_flag;
-  const uint32_t nk = (uint32_t)*count;
+  const uint32_t nk = (uint32_t)counts[1];  // the large clusters, at the back of the list
   unsigned long long edges = 0;
   for (uint32_t li = blockIdx.x; li < nk; li += gridDim.x) {
-  const uint32_t k = list[li];
+  const uint32_t k = list[K - 1 - li];
   const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0, nexit = cl_nexit[k];
   const uint32_t pb = ie.off[k], pe = ie.off[k + 1];
   for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_id[j] = cl_mem[slot0 + j];
@@ -114,6 +180,54 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts(const uint32_t* __restr
   __syncthreads();  // s_id of the next cluster
   }
   if (threadIdx.x == 0 && edges) atomicAdd(work, edges);
+}
+
+// k_ss_shortcuts for the small clusters (list[0, counts[0])), one wave each.
+__global__ void __launch_bounds__(kBlock) k_ss_shortcuts_small(const uint32_t* __restrict__ cl_off,
+                                                               const uint32_t* __restrict__ cl_nexit,
+                                                               const uint32_t* __restrict__ cl_mem,
+                                                               const uint32_t* __restrict__ cl_eoff,
+                                                               const uint32_t* __restrict__ cl_ent,
+                                                               const uint32_t* __restrict__ lidx,
+                                                               const unsigned long long* __restrict__ ss_base,
+                                                               Internal ie, uint32_t* __restrict__ sd,
+                                                               uint32_t* __restrict__ sp,
+                                                               const uint32_t* __restrict__ list,
+                                                               const unsigned long long* __restrict__ counts,
+                                                               unsigned long long* work) {
+  __shared__ unsigned long long s_dp_w[kBlock / 64][kSmallCluster];
+  __shared__ uint32_t s_id_w[kBlock / 64][kSmallCluster];
+  __shared__ unsigned long long s_edges;
+  const int wv = threadIdx.x >> 6, lane = lane_id();
+  unsigned long long* s_dp = s_dp_w[wv];
+  uint32_t* s_id = s_id_w[wv];
+  if (threadIdx.x == 0) s_edges = 0;
+  __syncthreads();
+  const uint32_t nk = (uint32_t)counts[0];
+  unsigned long long edges = 0;
+  for (uint32_t li = blockIdx.x * (kBlock / 64) + wv; li < nk; li += gridDim.x * (kBlock / 64)) {
+    const uint32_t k = list[li];
+    const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0, nexit = cl_nexit[k];
+    const uint32_t pb = ie.off[k], pe = ie.off[k + 1];
+    if (lane < (int)size) s_id[lane] = cl_mem[slot0 + lane];
+    for (uint32_t e = cl_eoff[k]; e < cl_eoff[k + 1]; ++e) {
+      const uint32_t src = lidx[cl_ent[e]];
+      if (lane < (int)size) s_dp[lane] = (uint32_t)lane == src ? 0xFFFFFFFFull : kInf64;
+      wave_lds_sync();
+      const uint32_t rounds = cluster_bf_wave(size, pb, pe, ie.src, ie.dst, ie.wt, s_id, s_dp);
+      edges += (unsigned long long)rounds * (pe - pb);
+      if (lane < (int)nexit) {
+        const unsigned long long x = s_dp[lane];
+        const unsigned long long base = ss_base[e];
+        sd[base + lane] = (uint32_t)(x >> 32);
+        sp[base + lane] = (uint32_t)x;
+      }
+      wave_lds_sync();
+    }
+  }
+  if (lane == 0 && edges) atomicAdd(&s_edges, edges);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_edges) atomicAdd(work, s_edges);
 }
 
 struct SsArgs {
@@ -166,15 +280,15 @@ __global__ void k_ss_mark(SsArgs a, uint32_t src, int mark_src) {
 // packed {dist, parent} minimum over the internal in-edges (together with what the
 // external relaxations left in the member's word) its smallest-id parent.
 __global__ void __launch_bounds__(kBlock) k_ss_relax_clusters(SsArgs a, Internal ie,
-                                                              const uint32_t* __restrict__ list,
-                                                              const unsigned long long* __restrict__ count,
+                                                              const uint32_t* __restrict__ list, uint32_t K,
+                                                              const unsigned long long* __restrict__ counts,
                                                               unsigned long long* work, int final_pass) {
   __shared__ unsigned long long s_dp[kMaxClusterS];
   __shared__ uint32_t s_id[kMaxClusterS];
   __shared__ int s_flag;
-  const uint32_t nk = (uint32_t)*count;
+  const uint32_t nk = (uint32_t)counts[1];  // the large clusters, at the back of the list
   for (uint32_t li = blockIdx.x; li < nk; li += gridDim.x) {
-  const uint32_t k = list[li];
+  const uint32_t k = list[K - 1 - li];
   const uint32_t slot0 = a.cl_off[k], size = a.cl_off[k + 1] - slot0, nexit = a.cl_nexit[k];
   const uint32_t pb = ie.off[k], pe = ie.off[k + 1];
   for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
@@ -204,6 +318,52 @@ __global__ void __launch_bounds__(kBlock) k_ss_relax_clusters(SsArgs a, Internal
   }
   __syncthreads();  // LDS of the next cluster
   }
+}
+
+// k_ss_relax_clusters for the small clusters (list[0, counts[0])), one wave each.
+__global__ void __launch_bounds__(kBlock) k_ss_relax_small(SsArgs a, Internal ie, const uint32_t* __restrict__ list,
+                                                           const unsigned long long* __restrict__ counts,
+                                                           unsigned long long* work, int final_pass) {
+  __shared__ unsigned long long s_dp_w[kBlock / 64][kSmallCluster];
+  __shared__ uint32_t s_id_w[kBlock / 64][kSmallCluster];
+  __shared__ unsigned long long s_edges;
+  const int wv = threadIdx.x >> 6, lane = lane_id();
+  unsigned long long* s_dp = s_dp_w[wv];
+  uint32_t* s_id = s_id_w[wv];
+  if (threadIdx.x == 0) s_edges = 0;
+  __syncthreads();
+  const uint32_t nk = (uint32_t)counts[0];
+  unsigned long long edges = 0;
+  for (uint32_t li = blockIdx.x * (kBlock / 64) + wv; li < nk; li += gridDim.x * (kBlock / 64)) {
+    const uint32_t k = list[li];
+    const uint32_t slot0 = a.cl_off[k], size = a.cl_off[k + 1] - slot0, nexit = a.cl_nexit[k];
+    const uint32_t pb = ie.off[k], pe = ie.off[k + 1];
+    uint32_t v = 0;
+    unsigned long long cur = kInf64;
+    if (lane < (int)size) {
+      v = a.cl_mem[slot0 + lane];
+      cur = a.dp[v];
+      s_id[lane] = v;
+      s_dp[lane] = cur;
+    }
+    wave_lds_sync();
+    const uint32_t rounds = cluster_bf_wave(size, pb, pe, ie.src, ie.dst, ie.wt, s_id, s_dp);
+    edges += (unsigned long long)rounds * (pe - pb);
+    bool improved = false;
+    if (lane < (int)size) {
+      const unsigned long long x = s_dp[lane];
+      if (x < cur) {
+        a.dp[v] = x;
+        improved = !final_pass && (uint32_t)lane < nexit && (x >> 32) != 0xFFFFFFFFull;
+      }
+    }
+    ss_append(a, 0, v, improved);
+    if (lane == 0 && final_pass) a.dirty[k] = 0;
+    wave_lds_sync();
+  }
+  if (lane == 0 && edges) atomicAdd(&s_edges, edges);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_edges) atomicAdd(work, s_edges);
 }
 
 __device__ __forceinline__ void ss_relax(const SsArgs& a, int nxt, uint32_t v, unsigned long long nd, uint32_t parent) {
@@ -354,6 +514,20 @@ static SsArgs ss_args(suminc_ctx* c) {
   return a;
 }
 
+static unsigned small_grid(uint32_t K) {
+  return std::max(1u, std::min(kClusterGrid, (K + kBlock / 64 - 1) / (kBlock / 64)));
+}
+
+// In-cluster relaxation of the listed clusters: small ones one per wave, large
+// ones one per workgroup.
+static void relax_clusters(suminc_ctx* c, const SsArgs& a, const unsigned long long* lcounts, int final_pass) {
+  Sketch& s = c->sk;
+  k_ss_relax_small<<<small_grid(s.K), kBlock, 0, c->stream>>>(a, internal_of(s), s.ss_list.as<uint32_t>(), lcounts,
+                                                             s.ss_work.as<unsigned long long>(), final_pass);
+  k_ss_relax_clusters<<<std::min(s.K, kClusterGrid), kBlock, 0, c->stream>>>(
+      a, internal_of(s), s.ss_list.as<uint32_t>(), s.K, lcounts, s.ss_work.as<unsigned long long>(), final_pass);
+}
+
 // Recomputes the SSSP rows of the clusters whose rows are not current.
 static void compute_sssp_rows(suminc_ctx* c) {
   Sketch& s = c->sk;
@@ -363,18 +537,24 @@ static void compute_sssp_rows(suminc_ctx* c) {
     s.ss_dist.ensure_slack(s.ss_len * 4 + 4);
     s.ss_par.ensure_slack(s.ss_len * 4 + 4);
   }
-  s.ss_work.ensure(16);
+  s.ss_work.ensure(32);
   s.ss_list.ensure((size_t)s.K * 4 + 4);
-  SUMINC_HIP(hipMemsetAsync(s.ss_work.p, 0, 16, c->stream));
-  unsigned long long* lcount = s.ss_work.as<unsigned long long>() + 1;
+  SUMINC_HIP(hipMemsetAsync(s.ss_work.p, 0, 32, c->stream));
+  unsigned long long* lcounts = s.ss_work.as<unsigned long long>() + 1;
   if (s.K) {
-    k_cluster_list<<<grid_for(s.K), kBlock, 0, c->stream>>>(s.K, s.ss_valid.as<uint8_t>(), 0, s.ss_list.as<uint32_t>(),
-                                                            lcount);
+    k_cluster_lists<<<grid_for(s.K), kBlock, 0, c->stream>>>(s.K, s.ss_valid.as<uint8_t>(), 0,
+                                                             s.cl_off.as<uint32_t>(), s.ss_list.as<uint32_t>(),
+                                                             lcounts);
+    k_ss_shortcuts_small<<<small_grid(s.K), kBlock, 0, c->stream>>>(
+        s.cl_off.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.cl_eoff.as<uint32_t>(),
+        s.cl_ent.as<uint32_t>(), s.lidx.as<uint32_t>(), s.ss_base.as<unsigned long long>(), internal_of(s),
+        s.ss_dist.as<uint32_t>(), s.ss_par.as<uint32_t>(), s.ss_list.as<uint32_t>(), lcounts,
+        s.ss_work.as<unsigned long long>());
     k_ss_shortcuts<<<std::min(s.K, kClusterGrid), kBlock, 0, c->stream>>>(
         s.cl_off.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.cl_eoff.as<uint32_t>(),
         s.cl_ent.as<uint32_t>(), s.lidx.as<uint32_t>(), s.ss_base.as<unsigned long long>(), internal_of(s),
-        s.ss_dist.as<uint32_t>(),
-        s.ss_par.as<uint32_t>(), s.ss_list.as<uint32_t>(), lcount, s.ss_work.as<unsigned long long>());
+        s.ss_dist.as<uint32_t>(), s.ss_par.as<uint32_t>(), s.ss_list.as<uint32_t>(), s.K, lcounts,
+        s.ss_work.as<unsigned long long>());
   }
   SUMINC_HIP(hipGetLastError());
   SUMINC_HIP(hipMemsetAsync(s.ss_valid.p, 1, (size_t)s.K + 1, c->stream));
@@ -406,20 +586,20 @@ void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
   c->sssp_pending = false;
   SsArgs a = ss_args(c);
   Counters* cnt = a.cnt;
-  s.ss_work.ensure(16);
+  s.ss_work.ensure(32);
   s.ss_list.ensure((size_t)s.K * 4 + 4);
-  SUMINC_HIP(hipMemsetAsync(s.ss_work.p, 0, 16, c->stream));
-  unsigned long long* lcount = s.ss_work.as<unsigned long long>() + 1;
+  SUMINC_HIP(hipMemsetAsync(s.ss_work.p, 0, 32, c->stream));
+  unsigned long long* lcounts = s.ss_work.as<unsigned long long>() + 1;
   SUMINC_HIP(hipMemsetAsync(s.ss_dirty.p, 0, (size_t)s.K + 1, c->stream));
   SUMINC_HIP(hipEventRecord(c->ev_start, c->stream));
   // frontier list 0 holds the seeds (source after init, repair seeds after a batch)
   k_ss_begin<<<1, 64, 0, c->stream>>>(cnt);
   k_ss_mark<<<256, kBlock, 0, c->stream>>>(a, c->sssp_src, first ? 1 : 0);
   if (s.K) {
-    k_cluster_list<<<grid_for(s.K), kBlock, 0, c->stream>>>(s.K, s.ss_dirty.as<uint8_t>(), 1,
-                                                            s.ss_list.as<uint32_t>(), lcount);
-    k_ss_relax_clusters<<<std::min(s.K, kClusterGrid), kBlock, 0, c->stream>>>(
-        a, internal_of(s), s.ss_list.as<uint32_t>(), lcount, s.ss_work.as<unsigned long long>(), 0);
+    k_cluster_lists<<<grid_for(s.K), kBlock, 0, c->stream>>>(s.K, s.ss_dirty.as<uint8_t>(), 1,
+                                                             s.cl_off.as<uint32_t>(), s.ss_list.as<uint32_t>(),
+                                                             lcounts);
+    relax_clusters(c, a, lcounts, 0);
   }
   SUMINC_HIP(hipGetLastError());
   int group = c->hint_sssp_sketch;  // rounds of the previous query + 1 first
@@ -455,11 +635,11 @@ void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
     p.alg_bytes += 16.0 * (edges - sc) + 20.0 * sc + 16.0 * (double)h->aux[6];
   }
   if (s.K) {
-    SUMINC_HIP(hipMemsetAsync(lcount, 0, 8, c->stream));
-    k_cluster_list<<<grid_for(s.K), kBlock, 0, c->stream>>>(s.K, s.ss_dirty.as<uint8_t>(), 1,
-                                                            s.ss_list.as<uint32_t>(), lcount);
-    k_ss_relax_clusters<<<std::min(s.K, kClusterGrid), kBlock, 0, c->stream>>>(
-        a, internal_of(s), s.ss_list.as<uint32_t>(), lcount, s.ss_work.as<unsigned long long>(), 1);
+    SUMINC_HIP(hipMemsetAsync(lcounts, 0, 16, c->stream));
+    k_cluster_lists<<<grid_for(s.K), kBlock, 0, c->stream>>>(s.K, s.ss_dirty.as<uint8_t>(), 1,
+                                                             s.cl_off.as<uint32_t>(), s.ss_list.as<uint32_t>(),
+                                                             lcounts);
+    relax_clusters(c, a, lcounts, 1);
   }
   SUMINC_HIP(hipGetLastError());
   SUMINC_HIP(hipEventRecord(c->ev_stop, c->stream));
