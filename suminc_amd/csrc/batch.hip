@@ -131,6 +131,8 @@ struct Groups {
   const uint32_t* add_pre;  // exclusive prefix of (act == 0) (G+1 valid)
   const uint32_t* tix;      // row -> index in touched list or INF32
   const uint32_t* t_gbeg;   // touched row -> first group (T+1)
+  const uint32_t* t_list;   // touched rows, ascending (T)
+  const unsigned* tcount;   // T
 };
 
 __global__ void k_split_rec(const uint4* rec, const unsigned* bc, uint32_t N, uint32_t* row, uint32_t* col,
@@ -202,17 +204,46 @@ __global__ void k_new_m(const unsigned* bc_in, const uint32_t* ex_pre, const uin
   bc[3] = m + add_pre[G] - ex_pre[G];
 }
 
-__global__ void k_new_deg(const uint32_t* off, uint32_t n, Groups gs, uint32_t* ndeg) {
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v > n) return;
-  if (v == n) { ndeg[n] = 0; return; }
-  uint32_t d = off[v + 1] - off[v];
-  uint32_t t = gs.tix[v];
-  if (t != INF32) {
-    uint32_t b = gs.t_gbeg[t], e = gs.t_gbeg[t + 1];
-    d = d + (gs.add_pre[e] - gs.add_pre[b]) - (gs.ex_pre[e] - gs.ex_pre[b]);
+// New row pointers: a row pointer moves by the net change (adds - existing pairs)
+// of the groups of all rows before it, off'[v] = off[v] + add_pre[g] - ex_pre[g]
+// with g the first group of the first touched row >= v (g = G past the last).
+// Vertices go in tiles of kOffTile; vts[t] is the first touched row of tile t
+// (touched rows are ascending), and a workgroup stages its tile's touched rows
+// and their shifts in LDS.
+constexpr uint32_t kOffTile = 8 * kBlock;
+
+__global__ void k_off_tiles(const uint32_t* __restrict__ t_list, const unsigned* tcount, uint32_t ntiles,
+                            uint32_t* vts) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t T = *tcount;
+  if (i > T) return;
+  const int64_t cur = i == T ? (int64_t)ntiles : (int64_t)(t_list[i] / kOffTile);
+  const int64_t prev = i == 0 ? -1 : (int64_t)(t_list[i - 1] / kOffTile);
+  for (int64_t t = prev + 1; t <= cur; ++t) vts[t] = i;
+}
+
+__global__ void __launch_bounds__(kBlock) k_new_off(const uint32_t* __restrict__ off, uint32_t n, Groups gs,
+                                                    const uint32_t* __restrict__ vts, uint32_t* __restrict__ noff) {
+  __shared__ uint32_t s_row[kOffTile];
+  __shared__ uint32_t s_delta[kOffTile + 1];
+  const uint32_t tile = blockIdx.x;
+  const uint32_t lo = vts[tile], hi = vts[tile + 1], r = hi - lo;  // touched rows of the tile (<= kOffTile)
+  for (uint32_t i = threadIdx.x; i <= r; i += kBlock) {
+    const uint32_t g = gs.t_gbeg[lo + i];
+    if (i < r) s_row[i] = gs.t_list[lo + i];
+    s_delta[i] = gs.add_pre[g] - gs.ex_pre[g];
   }
-  ndeg[v] = d;
+  __syncthreads();
+  for (uint32_t j = 0; j < 8; ++j) {
+    const uint32_t v = tile * kOffTile + j * kBlock + threadIdx.x;
+    if (v > n) break;
+    uint32_t a = 0, b = r;  // first touched row >= v
+    while (a < b) {
+      const uint32_t mid = (a + b) >> 1;
+      if (s_row[mid] < v) a = mid + 1; else b = mid;
+    }
+    noff[v] = off[v] + s_delta[a];
+  }
 }
 
 // Copy surviving old edges to their new positions.  Between two consecutive group
@@ -307,41 +338,28 @@ __global__ void __launch_bounds__(kBlock) k_copy_old(uint32_t m, const uint32_t*
   }
   __syncthreads();
   const TileGroups T{s_pos, s_ex, s_add, cnt, gs.ex_pre[j0], gs.add_pre[j0]};
-  // every thread copies runs of 4 consecutive edges with 16-byte loads; all loads
-  // of the tile are issued before the first store
-  constexpr int kRuns = kCopyTile / (kBlock * 4);
-  uint4 c4[kRuns], w4[kRuns];
+  // lane-consecutive edges: every load and (between group positions) every store
+  // of a wave covers 256 contiguous bytes; all loads of the tile are issued first
+  constexpr int kPer = kCopyTile / kBlock;
+  uint32_t cv[kPer], wvv[kPer];
 #pragma unroll
-  for (int r = 0; r < kRuns; ++r) {
-    const uint32_t e = e0 + 4 * (r * kBlock + threadIdx.x);
-    if (e + 3 < e1) {
-      c4[r] = *reinterpret_cast<const uint4*>(colv + e);
-      if (nw) w4[r] = *reinterpret_cast<const uint4*>(wv + e);
+  for (int r = 0; r < kPer; ++r) {
+    const uint32_t e = e0 + r * kBlock + threadIdx.x;
+    if (e < e1) {
+      cv[r] = colv[e];
+      if (nw) wvv[r] = wv[e];
     }
   }
+  uint32_t i = 0;
 #pragma unroll
-  for (int r = 0; r < kRuns; ++r) {
-    const uint32_t e = e0 + 4 * (r * kBlock + threadIdx.x);
-    if (e >= e1) continue;
-    uint32_t cv[4], wvv[4];
-    const uint32_t len = min(4u, e1 - e);
-    if (len == 4) {
-      cv[0] = c4[r].x; cv[1] = c4[r].y; cv[2] = c4[r].z; cv[3] = c4[r].w;
-      if (nw) { wvv[0] = w4[r].x; wvv[1] = w4[r].y; wvv[2] = w4[r].z; wvv[3] = w4[r].w; }
-    } else {
-      for (uint32_t q = 0; q < len; ++q) {
-        cv[q] = colv[e + q];
-        if (nw) wvv[q] = wv[e + q];
-      }
-    }
-    uint32_t i = 0;
-    for (uint32_t q = 0; q < len; ++q) {
-      T.seek(e + q, i);
-      const uint32_t dst = T.dest(e + q, i);
-      if (dst == INF32) continue;
-      ncol[dst] = cv[q];
-      if (nw) nw[dst] = wvv[q];
-    }
+  for (int r = 0; r < kPer; ++r) {
+    const uint32_t e = e0 + r * kBlock + threadIdx.x;
+    if (e >= e1) break;
+    T.seek(e, i);
+    const uint32_t dst = T.dest(e, i);
+    if (dst == INF32) continue;
+    ncol[dst] = cv[r];
+    if (nw) nw[dst] = wvv[r];
   }
 }
 
@@ -352,12 +370,7 @@ __global__ void k_place_adds(const uint32_t* off, const uint32_t* colv, const ui
   uint32_t u = gs.row[g], v = gs.col[g];
   uint32_t ti = gs.tix[u];
   uint32_t b = gs.t_gbeg[ti];
-  uint32_t lo = off[u], hi = off[u + 1];
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (colv[mid] < v) lo = mid + 1; else hi = mid;
-  }
-  uint32_t old_before = lo - off[u];
+  uint32_t old_before = gs.pos[g] - off[u];  // the pair's insertion point in the old row
   uint32_t kept_before = old_before - (gs.ex_pre[g] - gs.ex_pre[b]);
   uint32_t adds_before = gs.add_pre[g] - gs.add_pre[b];
   uint32_t dst = noff[u] + kept_before + adds_before;
@@ -756,11 +769,10 @@ static void merge_direction(suminc_ctx* c, DevBuf& off, DevBuf& col, DevBuf& w, 
   noff.ensure(((size_t)n + 1) * 4);
   ncol.ensure_slack(cap_m * 4 + 4);
   if (c->wkind) nw.ensure_slack(cap_m * 4 + 4);
-  k_new_deg<<<grid_for((uint64_t)n + 1), kBlock, 0, c->stream>>>(off.as<uint32_t>(), n, gs, b.ndeg.as<uint32_t>());
-  cub_call(c, c->tmp, [&](void* p, size_t& bytes) {
-    return rocprim::exclusive_scan(p, bytes, b.ndeg.as<uint32_t>(), noff.as<uint32_t>(), 0u, (size_t)n + 1,
-                                   rocprim::plus<uint32_t>(), c->stream);
-  });
+  const uint32_t vtiles = (n + 1 + kOffTile - 1) / kOffTile;
+  b.vts.ensure(((size_t)vtiles + 1) * 4);
+  k_off_tiles<<<grid_for((uint64_t)N + 1), kBlock, 0, c->stream>>>(gs.t_list, gs.tcount, vtiles, b.vts.as<uint32_t>());
+  k_new_off<<<vtiles, kBlock, 0, c->stream>>>(off.as<uint32_t>(), n, gs, b.vts.as<uint32_t>(), noff.as<uint32_t>());
   if (c->m) {
     const uint32_t m = (uint32_t)c->m;
     const uint32_t ntiles = (m + kCopyTile - 1) / kCopyTile;
@@ -840,7 +852,6 @@ static void ensure_workspace(suminc_ctx* c, uint64_t n_ops) {
     d->ensure((N + 1) * 4);
   b.bc.ensure(64);
   b.val.ensure(8);
-  b.ndeg.ensure(((size_t)c->n + 1) * 4);
   if (b.tix_n != c->n || !b.tix.p) {
     b.tix.ensure((size_t)c->n * 4);
     b.ttix.ensure((size_t)c->n * 4);
@@ -990,6 +1001,7 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   gs.w = b.g_w.as<uint32_t>(); gs.ex = b.g_ex.as<uint8_t>();
   gs.ex_pre = b.g_expre.as<uint32_t>(); gs.add_pre = b.g_addpre.as<uint32_t>();
   gs.tix = b.tix.as<uint32_t>(); gs.t_gbeg = b.t_gbeg.as<uint32_t>();
+  gs.t_list = b.t_list.as<uint32_t>(); gs.tcount = bc + 1;
   gs.pos = b.g_pos.as<uint32_t>();
   const uint64_t m_old = c->m;
   merge_direction(c, c->out_off, c->out_col, c->out_w, b.noff_out, b.ncol_out, b.nw_out, gs, N, cap_m, 0);
@@ -1014,6 +1026,7 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   gt.w = b.tg_w.as<uint32_t>(); gt.ex = b.tg_ex.as<uint8_t>();
   gt.ex_pre = b.tg_expre.as<uint32_t>(); gt.add_pre = b.tg_addpre.as<uint32_t>();
   gt.tix = b.ttix.as<uint32_t>(); gt.t_gbeg = b.tt_gbeg.as<uint32_t>();
+  gt.t_list = b.tt_list.as<uint32_t>(); gt.tcount = bc + 2;
   // positions of the transposed pairs in the old in-CSR
   k_probe<<<grid_for(N), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(), nullptr,
                                                  b.tg_row.as<uint32_t>(), b.tg_col.as<uint32_t>(), bc, nullptr,

@@ -95,7 +95,7 @@ struct BatchWs {
   DevBuf tstart;         // first group of every copy tile
   DevBuf tg_row, tg_col, tg_act, tg_w, tg_ex, tg_exu, tg_addu, tg_expre, tg_addpre, thead, tt_raw, tt_list, tt_gbeg;
   DevBuf tix, ttix;  // row -> touched index, all INF32 between batches
-  DevBuf ndeg, noff_out, ncol_out, nw_out, noff_in, ncol_in, nw_in;
+  DevBuf vts, noff_out, ncol_out, nw_out, noff_in, ncol_in, nw_in;
   DevBuf rbits, rlist;
   DevBuf val;  // validation of device-resident ops: count, first bad index
   DevBuf bc;  // device counts: [0]=G groups, [1]=touched rows (CSR), [2]=touched rows (CSC), [3]=new m
@@ -138,14 +138,14 @@ struct Sketch {
   DevBuf iem_key, iem_w, iem_dead, iem_key2, iem_w2, iem_pre, add_key, add_w;
   DevBuf ie_off, ie_src, ie_dst, ie_deg, ie_wt;
   DevBuf sc_R, sc_D;         // PageRank rows (reach mass / residual) per entry
-  DevBuf ss_dist, ss_par;    // SSSP rows (in-cluster distance / parent to every exit) per entry
+  DevBuf ss_dist;            // SSSP rows (in-cluster distance to every exit) per entry
   DevBuf bound, spnode;      // PageRank sketch accumulators
   DevBuf dirty;              // K: cluster touched since its last precompute
   DevBuf ss_dirty;           // K: SSSP precompute flags (clusters of frontier members)
   DevBuf ss_work;            // u64 x2: internal edges read by the SSSP shortcut kernels, list length
   DevBuf ss_list;            // K: clusters the next per-cluster kernel visits
   DevBuf pr_valid, ss_valid; // K: shortcut rows of the cluster are current
-  DevBuf sc_R_old, sc_D_old, ss_dist_old, ss_par_old;  // rows of the previous layout
+  DevBuf sc_R_old, sc_D_old, ss_dist_old;  // rows of the previous layout
   // layout pass: per-cluster counts, their prefix (current and previous layout), type histogram
   DevBuf lay_cnt, lay_base, lay_base_old, lay_hist;
   uint32_t counts[8] = {0};  // node type histogram

@@ -297,12 +297,6 @@ __global__ void k_assign_cid(uint32_t n, const uint32_t* raw, const uint8_t* acc
 // other members (the shortcut rows of SSSP hold the exit slots only), and lays
 // out the entries and the shortcut-row bases from one scan over the clusters.
 // ---------------------------------------------------------------------------
-struct ClBase {  // per-cluster counts, and after the scan their prefix
-  unsigned long long sc;  // PageRank row slots: entries x members
-  unsigned long long ss;  // SSSP row slots: entries x exits
-  uint32_t ent;           // entries
-  uint32_t pad;
-};
 struct ClBasePlus {
   __host__ __device__ ClBase operator()(const ClBase& a, const ClBase& b) const {
     return ClBase{a.sc + b.sc, a.ss + b.ss, a.ent + b.ent, 0u};
@@ -536,7 +530,7 @@ __global__ void k_rows_move(uint64_t len, uint32_t K, const ClBase* __restrict__
     const unsigned long long obase = kSsspRows ? ob[lo].ss : ob[lo].sc;
     const unsigned long long q = obase + (p - nbase);
     newA[p] = oldA[q];
-    newB[p] = oldB[q];
+    if (newB) newB[p] = oldB[q];
   }
 }
 
@@ -632,12 +626,12 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
             s.sc_D_old.as<float>(), s.sc_D.as<float>());
     }
     if (s.ss_rows) {
-      s.ss_dist.swap(s.ss_dist_old); s.ss_par.swap(s.ss_par_old);
-      s.ss_dist.ensure_slack(s.ss_len * 4 + 4); s.ss_par.ensure_slack(s.ss_len * 4 + 4);
+      s.ss_dist.swap(s.ss_dist_old);
+      s.ss_dist.ensure_slack(s.ss_len * 4 + 4);
       if (s.ss_len)
         k_rows_move<uint32_t, true><<<move_grid(s.ss_len), kBlock, 0, c->stream>>>(
             s.ss_len, K, base, old, s.ss_valid.as<uint8_t>(), s.ss_dist_old.as<uint32_t>(),
-            s.ss_dist.as<uint32_t>(), s.ss_par_old.as<uint32_t>(), s.ss_par.as<uint32_t>());
+            s.ss_dist.as<uint32_t>(), nullptr, nullptr);
     }
     SUMINC_HIP(hipGetLastError());
   }

@@ -14,6 +14,15 @@ struct Internal {
   const uint32_t* wt;   // integer weight (1 without weights)
 };
 
+// Per-cluster counts of the layout, and after the layout's scan their prefix: the
+// first entry of cluster k and the first slot of its shortcut rows.
+struct ClBase {
+  unsigned long long sc;  // PageRank row slots: entries x members
+  unsigned long long ss;  // SSSP row slots: entries x exits
+  uint32_t ent;           // entries
+  uint32_t pad;
+};
+
 inline Internal internal_of(const Sketch& s) {
   return Internal{s.ie_off.as<uint32_t>(), s.ie_src.as<uint32_t>(), s.ie_dst.as<uint32_t>(), s.ie_deg.as<uint32_t>(),
                   s.ie_wt.as<uint32_t>()};

@@ -27,34 +27,66 @@ namespace suminc {
 constexpr int kMaxClusterS = 2048;
 constexpr uint64_t kInf64 = ~0ull;
 
+// Relaxes the internal edge i -> j (local indices) of weight w on packed
+// {dist, parent} words in LDS; true when it lowered j.
+__device__ __forceinline__ bool relax_lds(uint32_t i, uint32_t j, uint32_t w, const uint32_t* s_id,
+                                          unsigned long long* s_dp) {
+  const unsigned long long di = s_dp[i] >> 32;
+  if (di == 0xFFFFFFFFull) return false;
+  const unsigned long long nd = di + w;
+  if (nd >= 0xFFFFFFFFull) return false;
+  const unsigned long long cand = (nd << 32) | s_id[i];
+  if (cand < s_dp[j]) {
+    atomicMin(&s_dp[j], cand);
+    return true;
+  }
+  return false;
+}
+
+// The first kEdgeCache * kBlock internal edges of a cluster, kept in registers
+// across the Bellman-Ford rounds of a workgroup (from -> to: the relaxation
+// direction; the reversed edges give distances towards a member).
+constexpr int kEdgeCache = 8;
+struct EdgeCache {
+  uint32_t a[kEdgeCache], b[kEdgeCache], w[kEdgeCache];
+  __device__ __forceinline__ void load(uint32_t pb, uint32_t pe, const uint32_t* __restrict__ from,
+                                       const uint32_t* __restrict__ to, const uint32_t* __restrict__ wt) {
+#pragma unroll
+    for (int r = 0; r < kEdgeCache; ++r) {
+      const uint32_t p = pb + r * kBlock + threadIdx.x;
+      a[r] = p < pe ? from[p] : INF32;
+      b[r] = p < pe ? to[p] : 0u;
+      w[r] = p < pe ? wt[p] : 0u;
+    }
+  }
+};
+
 // In-cluster Bellman-Ford rounds on packed {dist, parent} words in LDS,
 // edge-parallel over the cluster's internal edges with LDS atomic min, until a
-// round changes nothing.  s_id holds the members' vertex ids.  Returns rounds run.
-__device__ uint32_t cluster_bf(uint32_t size, uint32_t pb, uint32_t pe, const uint32_t* __restrict__ src,
-                               const uint32_t* __restrict__ dst, const uint32_t* __restrict__ wt,
-                               const uint32_t* s_id, unsigned long long* s_dp, int* s_flag) {
+// round changes nothing.  s_id holds the members' vertex ids; edges past the
+// cache are read from memory.  One barrier per round: the round's change flag
+// rotates over three LDS words (the word a round sets was last read two rounds
+// earlier).  Returns rounds run.
+__device__ uint32_t cluster_bf(uint32_t size, uint32_t pb, uint32_t pe, const EdgeCache& ec,
+                               const uint32_t* __restrict__ from, const uint32_t* __restrict__ to,
+                               const uint32_t* __restrict__ wt, const uint32_t* s_id, unsigned long long* s_dp,
+                               int* s_flag) {
+  if (threadIdx.x == 0) s_flag[0] = 0;
+  __syncthreads();
+  const uint32_t pc = pb + kEdgeCache * kBlock;  // first edge past the cache
   uint32_t rounds = 0;
   while (true) {
-    if (threadIdx.x == 0) *s_flag = 0;
-    __syncthreads();
-    for (uint32_t p = pb + threadIdx.x; p < pe; p += kBlock) {
-      const uint32_t i = src[p];
-      const unsigned long long di = s_dp[i] >> 32;
-      if (di == 0xFFFFFFFFull) continue;
-      const unsigned long long nd = di + wt[p];
-      if (nd >= 0xFFFFFFFFull) continue;
-      const unsigned long long cand = (nd << 32) | s_id[i];
-      const uint32_t j = dst[p];
-      if (cand < s_dp[j]) {
-        atomicMin(&s_dp[j], cand);
-        *s_flag = 1;
-      }
-    }
+    const int cur = (int)(rounds % 3);
+    if (threadIdx.x == 0) s_flag[(rounds + 1) % 3] = 0;
+    bool changed = false;
+#pragma unroll
+    for (int r = 0; r < kEdgeCache; ++r)
+      if (ec.a[r] != INF32) changed |= relax_lds(ec.a[r], ec.b[r], ec.w[r], s_id, s_dp);
+    for (uint32_t p = pc + threadIdx.x; p < pe; p += kBlock) changed |= relax_lds(from[p], to[p], wt[p], s_id, s_dp);
+    if (changed) s_flag[cur] = 1;
     __syncthreads();
     ++rounds;
-    const int f = *s_flag;
-    __syncthreads();
-    if (!f || rounds > size + 1) break;
+    if (!s_flag[cur] || rounds > size + 1) break;
   }
   return rounds;
 }
@@ -90,18 +122,7 @@ __device__ uint32_t cluster_bf_wave(uint32_t size, uint32_t pb, uint32_t pe, con
                                     const uint32_t* __restrict__ dst, const uint32_t* __restrict__ wt,
                                     const uint32_t* s_id, unsigned long long* s_dp) {
   const int lane = lane_id();
-  auto relax = [&](uint32_t i, uint32_t j, uint32_t w) -> bool {
-    const unsigned long long di = s_dp[i] >> 32;
-    if (di == 0xFFFFFFFFull) return false;
-    const unsigned long long nd = di + w;
-    if (nd >= 0xFFFFFFFFull) return false;
-    const unsigned long long cand = (nd << 32) | s_id[i];
-    if (cand < s_dp[j]) {
-      atomicMin(&s_dp[j], cand);
-      return true;
-    }
-    return false;
-  };
+  auto relax = [&](uint32_t i, uint32_t j, uint32_t w) -> bool { return relax_lds(i, j, w, s_id, s_dp); };
   uint32_t rounds = 0;
   if (pe - pb <= 128) {
     uint32_t ei[2] = {0, 0}, ej[2] = {0, 0}, ew[2] = {0, 0};
@@ -137,11 +158,13 @@ __device__ uint32_t cluster_bf_wave(uint32_t size, uint32_t pb, uint32_t pe, con
   return rounds;
 }
 
-// Shortcut rows: for every entry e of cluster k, in-cluster distance and parent of
-// every exit of k (the first nexit members; the entry itself if it is an exit:
-// distance 0, parent INF = no candidate).  Only the exits are kept: the sketch
-// rounds need entry -> exit distances, and the correction re-runs the in-cluster
-// relaxation from the entries' final distances instead of reading rows.
+// Shortcut rows: for every entry e of cluster k, the in-cluster distance to every
+// exit of k (the first nexit members; 0 for the entry itself if it is an exit).
+// The rows hold distances only: an exit reached through a row takes parent INF32
+// for the moment, and the correction pass, which re-relaxes the cluster's
+// internal edges once the distances are final, gives it its smallest-id
+// in-cluster predecessor.  A cluster runs one Bellman-Ford per entry, or one per
+// exit over the reversed internal edges when it has fewer exits than entries.
 __global__ void __launch_bounds__(kBlock) k_ss_shortcuts(const uint32_t* __restrict__ cl_off,
                                                          const uint32_t* __restrict__ cl_nexit,
                                                          const uint32_t* __restrict__ cl_mem,
@@ -150,48 +173,62 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts(const uint32_t* __restr
                                                          const uint32_t* __restrict__ lidx,
                                                          const unsigned long long* __restrict__ ss_base,
                                                          Internal ie, uint32_t* __restrict__ sd,
-                                                         uint32_t* __restrict__ sp, const uint32_t* __restrict__ list,
-                                                         uint32_t K, const unsigned long long* __restrict__ counts,
+                                                         const uint32_t* __restrict__ list, uint32_t K,
+                                                         const unsigned long long* __restrict__ counts,
                                                          unsigned long long* work) {
   __shared__ unsigned long long s_dp[kMaxClusterS];
   __shared__ uint32_t s_id[kMaxClusterS];
-  __shared__ int s_flag;
+  __shared__ int s_flag[3];
   const uint32_t nk = (uint32_t)counts[1];  // the large clusters, at the back of the list
   unsigned long long edges = 0;
   for (uint32_t li = blockIdx.x; li < nk; li += gridDim.x) {
-  const uint32_t k = list[K - 1 - li];
-  const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0, nexit = cl_nexit[k];
-  const uint32_t pb = ie.off[k], pe = ie.off[k + 1];
-  for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_id[j] = cl_mem[slot0 + j];
-  for (uint32_t e = cl_eoff[k]; e < cl_eoff[k + 1]; ++e) {
-    const uint32_t src = lidx[cl_ent[e]];
-    for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_dp[j] = j == src ? 0xFFFFFFFFull : kInf64;
-    __syncthreads();
-    uint32_t rounds = cluster_bf(size, pb, pe, ie.src, ie.dst, ie.wt, s_id, s_dp, &s_flag);
-    edges += (unsigned long long)rounds * (pe - pb);
-    const unsigned long long base = ss_base[e];
-    for (uint32_t j = threadIdx.x; j < nexit; j += kBlock) {
-      const unsigned long long x = s_dp[j];
-      sd[base + j] = (uint32_t)(x >> 32);
-      sp[base + j] = (uint32_t)x;
+    const uint32_t k = list[K - 1 - li];
+    const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0, nexit = cl_nexit[k];
+    const uint32_t pb = ie.off[k], pe = ie.off[k + 1];
+    const uint32_t eb = cl_eoff[k], ee = cl_eoff[k + 1], nent = ee - eb;
+    if (nexit == 0 || nent == 0) continue;
+    for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_id[j] = cl_mem[slot0 + j];
+    const bool fwd = nent <= nexit;
+    EdgeCache ec;
+    ec.load(pb, pe, fwd ? ie.src : ie.dst, fwd ? ie.dst : ie.src, ie.wt);
+    if (fwd) {
+      for (uint32_t e = eb; e < ee; ++e) {
+        const uint32_t src = lidx[cl_ent[e]];
+        const unsigned long long base = ss_base[e];
+        if (nexit == 1 && src == 0) {  // the only exit is the entry itself
+          if (threadIdx.x == 0) sd[base] = 0;
+          continue;
+        }
+        for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_dp[j] = j == src ? 0xFFFFFFFFull : kInf64;
+        const uint32_t rounds = cluster_bf(size, pb, pe, ec, ie.src, ie.dst, ie.wt, s_id, s_dp, s_flag);
+        edges += (unsigned long long)rounds * (pe - pb);
+        for (uint32_t j = threadIdx.x; j < nexit; j += kBlock) sd[base + j] = (uint32_t)(s_dp[j] >> 32);
+        __syncthreads();
+      }
+    } else {
+      for (uint32_t x = 0; x < nexit; ++x) {
+        for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_dp[j] = j == x ? 0xFFFFFFFFull : kInf64;
+        const uint32_t rounds = cluster_bf(size, pb, pe, ec, ie.dst, ie.src, ie.wt, s_id, s_dp, s_flag);
+        edges += (unsigned long long)rounds * (pe - pb);
+        for (uint32_t e = eb + threadIdx.x; e < ee; e += kBlock)
+          sd[ss_base[e] + x] = (uint32_t)(s_dp[lidx[cl_ent[e]]] >> 32);
+        __syncthreads();
+      }
     }
-    __syncthreads();
-  }
-  __syncthreads();  // s_id of the next cluster
+    __syncthreads();  // s_id of the next cluster
   }
   if (threadIdx.x == 0 && edges) atomicAdd(work, edges);
 }
 
-// k_ss_shortcuts for the small clusters (list[0, counts[0])), one wave each.
+// k_ss_shortcuts for the small clusters (list[0, counts[0])), one wave each: the
+// entries come from the slot flags, the row of the q-th entry starts at the
+// cluster's row base + q * nexit.
 __global__ void __launch_bounds__(kBlock) k_ss_shortcuts_small(const uint32_t* __restrict__ cl_off,
                                                                const uint32_t* __restrict__ cl_nexit,
                                                                const uint32_t* __restrict__ cl_mem,
-                                                               const uint32_t* __restrict__ cl_eoff,
-                                                               const uint32_t* __restrict__ cl_ent,
-                                                               const uint32_t* __restrict__ lidx,
-                                                               const unsigned long long* __restrict__ ss_base,
-                                                               Internal ie, uint32_t* __restrict__ sd,
-                                                               uint32_t* __restrict__ sp,
+                                                               const uint8_t* __restrict__ slot_ent,
+                                                               const ClBase* __restrict__ clb, Internal ie,
+                                                               uint32_t* __restrict__ sd,
                                                                const uint32_t* __restrict__ list,
                                                                const unsigned long long* __restrict__ counts,
                                                                unsigned long long* work) {
@@ -199,6 +236,7 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts_small(const uint32_t* _
   __shared__ uint32_t s_id_w[kBlock / 64][kSmallCluster];
   __shared__ unsigned long long s_edges;
   const int wv = threadIdx.x >> 6, lane = lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
   unsigned long long* s_dp = s_dp_w[wv];
   uint32_t* s_id = s_id_w[wv];
   if (threadIdx.x == 0) s_edges = 0;
@@ -209,25 +247,70 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts_small(const uint32_t* _
     const uint32_t k = list[li];
     const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0, nexit = cl_nexit[k];
     const uint32_t pb = ie.off[k], pe = ie.off[k + 1];
-    if (lane < (int)size) s_id[lane] = cl_mem[slot0 + lane];
-    for (uint32_t e = cl_eoff[k]; e < cl_eoff[k + 1]; ++e) {
-      const uint32_t src = lidx[cl_ent[e]];
-      if (lane < (int)size) s_dp[lane] = (uint32_t)lane == src ? 0xFFFFFFFFull : kInf64;
-      wave_lds_sync();
-      const uint32_t rounds = cluster_bf_wave(size, pb, pe, ie.src, ie.dst, ie.wt, s_id, s_dp);
-      edges += (unsigned long long)rounds * (pe - pb);
-      if (lane < (int)nexit) {
-        const unsigned long long x = s_dp[lane];
-        const unsigned long long base = ss_base[e];
-        sd[base + lane] = (uint32_t)(x >> 32);
-        sp[base + lane] = (uint32_t)x;
+    const unsigned long long rbase = clb[k].ss;
+    bool en = false;
+    if (lane < (int)size) {
+      s_id[lane] = cl_mem[slot0 + lane];
+      en = slot_ent[slot0 + lane] != 0;
+    }
+    const unsigned long long ents = __ballot(en);
+    const uint32_t nent = (uint32_t)__popcll(ents);
+    if (nexit == 0 || nent == 0) continue;
+    if (nent <= nexit) {
+      unsigned long long left = ents;
+      for (uint32_t q = 0; left; ++q) {
+        const uint32_t src = (uint32_t)__ffsll((long long)left) - 1;
+        left &= left - 1;
+        const unsigned long long r0 = rbase + (unsigned long long)q * nexit;
+        if (nexit == 1 && src == 0) {  // the only exit is the entry itself
+          if (lane == 0) sd[r0] = 0;
+          continue;
+        }
+        if (lane < (int)size) s_dp[lane] = (uint32_t)lane == src ? 0xFFFFFFFFull : kInf64;
+        wave_lds_sync();
+        const uint32_t rounds = cluster_bf_wave(size, pb, pe, ie.src, ie.dst, ie.wt, s_id, s_dp);
+        edges += (unsigned long long)rounds * (pe - pb);
+        if (lane < (int)nexit) sd[r0 + lane] = (uint32_t)(s_dp[lane] >> 32);
+        wave_lds_sync();
       }
-      wave_lds_sync();
+    } else {
+      const unsigned long long q = __popcll(ents & below);  // this lane's entry rank
+      for (uint32_t x = 0; x < nexit; ++x) {
+        if (lane < (int)size) s_dp[lane] = (uint32_t)lane == x ? 0xFFFFFFFFull : kInf64;
+        wave_lds_sync();
+        const uint32_t rounds = cluster_bf_wave(size, pb, pe, ie.dst, ie.src, ie.wt, s_id, s_dp);
+        edges += (unsigned long long)rounds * (pe - pb);
+        if (en) sd[rbase + q * nexit + x] = (uint32_t)(s_dp[lane] >> 32);
+        wave_lds_sync();
+      }
     }
   }
   if (lane == 0 && edges) atomicAdd(&s_edges, edges);
   __syncthreads();
   if (threadIdx.x == 0 && s_edges) atomicAdd(work, s_edges);
+}
+
+// One shortcut row with parents, for inspection: forward Bellman-Ford from the
+// entry in slot src of cluster k; distance and parent of every exit.
+__global__ void __launch_bounds__(kBlock) k_ss_row_one(const uint32_t* __restrict__ cl_off,
+                                                       const uint32_t* __restrict__ cl_nexit,
+                                                       const uint32_t* __restrict__ cl_mem, Internal ie, uint32_t k,
+                                                       uint32_t src, uint32_t* out_d, uint32_t* out_p) {
+  __shared__ unsigned long long s_dp[kMaxClusterS];
+  __shared__ uint32_t s_id[kMaxClusterS];
+  __shared__ int s_flag[3];
+  const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0, nexit = cl_nexit[k];
+  for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+    s_id[j] = cl_mem[slot0 + j];
+    s_dp[j] = j == src ? 0xFFFFFFFFull : kInf64;
+  }
+  EdgeCache ec;
+  ec.load(ie.off[k], ie.off[k + 1], ie.src, ie.dst, ie.wt);
+  cluster_bf(size, ie.off[k], ie.off[k + 1], ec, ie.src, ie.dst, ie.wt, s_id, s_dp, s_flag);
+  for (uint32_t j = threadIdx.x; j < nexit; j += kBlock) {
+    out_d[j] = (uint32_t)(s_dp[j] >> 32);
+    out_p[j] = (uint32_t)s_dp[j];
+  }
 }
 
 struct SsArgs {
@@ -242,7 +325,6 @@ struct SsArgs {
   const uint32_t* cl_nexit;
   const unsigned long long* ss_base;
   const uint32_t* sd;
-  const uint32_t* sp;
   unsigned long long* dp;
   uint32_t* wl[2];
   uint32_t* bits[2];
@@ -285,7 +367,7 @@ __global__ void __launch_bounds__(kBlock) k_ss_relax_clusters(SsArgs a, Internal
                                                               unsigned long long* work, int final_pass) {
   __shared__ unsigned long long s_dp[kMaxClusterS];
   __shared__ uint32_t s_id[kMaxClusterS];
-  __shared__ int s_flag;
+  __shared__ int s_flag[3];
   const uint32_t nk = (uint32_t)counts[1];  // the large clusters, at the back of the list
   for (uint32_t li = blockIdx.x; li < nk; li += gridDim.x) {
   const uint32_t k = list[K - 1 - li];
@@ -297,7 +379,9 @@ __global__ void __launch_bounds__(kBlock) k_ss_relax_clusters(SsArgs a, Internal
     s_dp[j] = a.dp[v];
   }
   __syncthreads();
-  uint32_t rounds = cluster_bf(size, pb, pe, ie.src, ie.dst, ie.wt, s_id, s_dp, &s_flag);
+  EdgeCache ec;
+  ec.load(pb, pe, ie.src, ie.dst, ie.wt);
+  uint32_t rounds = cluster_bf(size, pb, pe, ec, ie.src, ie.dst, ie.wt, s_id, s_dp, s_flag);
   for (uint32_t j0 = 0; j0 < size; j0 += kBlock) {  // uniform trip count for the wave ballots
     const uint32_t j = j0 + threadIdx.x;
     bool improved = false;
@@ -440,7 +524,7 @@ __global__ void __launch_bounds__(kBlock) k_ss_push(SsArgs a) {
         const uint32_t j = k - s_iblen[slot];
         const unsigned long long r = s_rbase[slot] + j;
         const uint32_t dj = a.sd[r];
-        ss_relax(a, nxt, a.cl_mem[s_mbase[slot] + j], dj == INF32 ? 0xFFFFFFFFull : du + dj, a.sp[r]);
+        ss_relax(a, nxt, a.cl_mem[s_mbase[slot] + j], dj == INF32 ? 0xFFFFFFFFull : du + dj, INF32);
       }
     });
   }
@@ -502,7 +586,6 @@ static SsArgs ss_args(suminc_ctx* c) {
   a.cl_nexit = s.cl_nexit.as<uint32_t>();
   a.ss_base = s.ss_base.as<unsigned long long>();
   a.sd = s.ss_dist.as<uint32_t>();
-  a.sp = s.ss_par.as<uint32_t>();
   a.dp = c->sssp_dp.as<unsigned long long>();
   a.wl[0] = c->wl0.as<uint32_t>();
   a.wl[1] = c->tmp5.as<uint32_t>();
@@ -535,7 +618,6 @@ static void compute_sssp_rows(suminc_ctx* c) {
   if (!s.ss_rows) SUMINC_HIP(hipMemsetAsync(s.ss_valid.p, 0, (size_t)s.K + 1, c->stream));
   if (!s.ss_rows || s.ss_dist.cap < s.ss_len * 4 + 4) {
     s.ss_dist.ensure_slack(s.ss_len * 4 + 4);
-    s.ss_par.ensure_slack(s.ss_len * 4 + 4);
   }
   s.ss_work.ensure(32);
   s.ss_list.ensure((size_t)s.K * 4 + 4);
@@ -546,14 +628,13 @@ static void compute_sssp_rows(suminc_ctx* c) {
                                                              s.cl_off.as<uint32_t>(), s.ss_list.as<uint32_t>(),
                                                              lcounts);
     k_ss_shortcuts_small<<<small_grid(s.K), kBlock, 0, c->stream>>>(
-        s.cl_off.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.cl_eoff.as<uint32_t>(),
-        s.cl_ent.as<uint32_t>(), s.lidx.as<uint32_t>(), s.ss_base.as<unsigned long long>(), internal_of(s),
-        s.ss_dist.as<uint32_t>(), s.ss_par.as<uint32_t>(), s.ss_list.as<uint32_t>(), lcounts,
+        s.cl_off.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.slot_ent.as<uint8_t>(),
+        s.lay_base.as<ClBase>(), internal_of(s), s.ss_dist.as<uint32_t>(), s.ss_list.as<uint32_t>(), lcounts,
         s.ss_work.as<unsigned long long>());
     k_ss_shortcuts<<<std::min(s.K, kClusterGrid), kBlock, 0, c->stream>>>(
         s.cl_off.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.cl_eoff.as<uint32_t>(),
         s.cl_ent.as<uint32_t>(), s.lidx.as<uint32_t>(), s.ss_base.as<unsigned long long>(), internal_of(s),
-        s.ss_dist.as<uint32_t>(), s.ss_par.as<uint32_t>(), s.ss_list.as<uint32_t>(), s.K, lcounts,
+        s.ss_dist.as<uint32_t>(), s.ss_list.as<uint32_t>(), s.K, lcounts,
         s.ss_work.as<unsigned long long>());
   }
   SUMINC_HIP(hipGetLastError());
@@ -692,9 +773,20 @@ void sketch_sssp_shortcut(suminc_ctx* c, uint32_t v, uint32_t cap, uint32_t* mem
     for (uint32_t j = ne; j < size; ++j) dist[j] = INF32;
     if (ne) SUMINC_HIP(hipMemcpy(dist, s.ss_dist.as<uint32_t>() + base, (size_t)ne * 4, hipMemcpyDeviceToHost));
   }
-  if (parent) {
+  if (parent) {  // the rows keep distances only: the parents come from one forward run
     for (uint32_t j = ne; j < size; ++j) parent[j] = INF32;
-    if (ne) SUMINC_HIP(hipMemcpy(parent, s.ss_par.as<uint32_t>() + base, (size_t)ne * 4, hipMemcpyDeviceToHost));
+    if (ne) {
+      uint32_t src = 0;
+      SUMINC_HIP(hipMemcpy(&src, s.lidx.as<uint32_t>() + v, 4, hipMemcpyDeviceToHost));
+      DevBuf out;
+      out.ensure((size_t)ne * 8);
+      k_ss_row_one<<<1, kBlock, 0, c->stream>>>(s.cl_off.as<uint32_t>(), s.cl_nexit.as<uint32_t>(),
+                                                 s.cl_mem.as<uint32_t>(), internal_of(s), k, src,
+                                                 out.as<uint32_t>(), out.as<uint32_t>() + ne);
+      SUMINC_HIP(hipGetLastError());
+      SUMINC_HIP(hipStreamSynchronize(c->stream));
+      SUMINC_HIP(hipMemcpy(parent, out.as<uint32_t>() + ne, (size_t)ne * 4, hipMemcpyDeviceToHost));
+    }
   }
 }
 
