@@ -174,16 +174,18 @@ def run_sssp(args, wl):
     log(f"incremental: {ms_per_step:.3f} ms/step (query {run_ms / args.steps:.3f} ms), "
         f"{edges / args.steps:.3e} edges/step, {gteps:.4f} GTEPS")
 
-    # host-buffer variant of the same step (PCIe upload of the ops inside the step)
-    ctx.apply_batch(batches[0])  # re-applying an old batch is a valid batch on the current graph
-    run()
+    # host-buffer variant of the same step (PCIe upload of the ops inside the step),
+    # on further fresh batches of the same size
+    n_host = min(args.steps, 5)
+    host_batches = [ctx.generate_batch(wl["seed"], b, n_ops // 2, n_ops - n_ops // 2, wl["wmin"], wl["wmax"])
+                    for b in range(args.warmup + args.steps, args.warmup + args.steps + n_host)]
     ctx.sync()
     t1 = time.perf_counter()
-    for b in range(args.warmup, args.warmup + min(args.steps, 5)):
-        ctx.apply_batch(batches[b])
+    for hb in host_batches:
+        ctx.apply_batch(hb)
         run()
     ctx.sync()
-    host_ms = 1e3 * (time.perf_counter() - t1) / min(args.steps, 5)
+    host_ms = 1e3 * (time.perf_counter() - t1) / n_host
 
     # the incremental state is what gets checked
     off, col, w = ctx.download_csr()
