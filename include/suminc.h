@@ -102,6 +102,8 @@ void suminc_default_params(suminc_params* p);
 int suminc_ctx_create(int device, const suminc_params* params, suminc_ctx** out);
 int suminc_ctx_destroy(suminc_ctx* ctx);
 int suminc_set_params(suminc_ctx* ctx, const suminc_params* params);
+/* With ctx == NULL: the message of the last failed call without a context on the
+ * calling thread (the text-format readers below). */
 const char* suminc_last_error(const suminc_ctx* ctx);
 int suminc_sync(suminc_ctx* ctx);
 /* hipStream_t the context launches on, as an opaque pointer (for event timing). */
@@ -289,6 +291,52 @@ typedef struct suminc_kernel_profile {
 /* on != 0 enables event timing and clears the accumulators. */
 int suminc_profile_enable(suminc_ctx* ctx, int on);
 int suminc_profile_get(suminc_ctx* ctx, int kernel, suminc_kernel_profile* out);
+
+/* ---- the reference's text formats (host only, no device work) -------------------
+ * Parsed with one thread per newline-aligned chunk of a memory-mapped file.  These
+ * calls take no context; on failure suminc_last_error(NULL) holds the message
+ * (with the line number). */
+
+/* Edge file: one "u v [w]" per line; empty lines and lines starting with '#' are
+ * skipped, and so is a line that does not parse (counted in skipped_lines), as the
+ * fragment loaders do (grape/fragment/e_fragment_loader.h:107-170; tokens as
+ * grape/io/tsv_line_parser.h reads them).  weight_kind selects the third column
+ * (SUMINC_W_U32: integer >= 1, SUMINC_W_F32: float, SUMINC_W_NONE: ignored).
+ * directed == 0 adds both directions of every line (the loader's spec.directed).
+ * Vertex ids are taken as dense ids: n = largest id + 1.  The result is a CSR with
+ * sorted rows; of repeated (u, v) lines the first one is kept.  Free with
+ * suminc_edges_free. */
+typedef struct suminc_edges suminc_edges;
+int suminc_edges_read_tsv(const char* path, int weight_kind, int directed, suminc_edges** out);
+int suminc_edges_info(const suminc_edges* edges, uint32_t* n, uint64_t* m, int* weight_kind,
+                      uint64_t* skipped_lines);
+/* Copies the CSR out: offsets[n + 1], cols[m], weights[m] (4-byte bits); any may be NULL. */
+int suminc_edges_copy(const suminc_edges* edges, uint64_t* offsets, uint32_t* cols, void* weights);
+/* suminc_upload_csr of the edge list (replaces the loader + fragment build,
+ * grape/fragment/loader.h:30-61 and immutable_edgecut_fragment.h:129-330). */
+int suminc_edges_upload(suminc_ctx* ctx, const suminc_edges* edges);
+void suminc_edges_free(suminc_edges* edges);
+
+/* Update file: "a u v [w]" / "d u v" per line (IncFragmentBuilder's delta file,
+ * grape/fragment/inc_fragment_builder.h:29-72), blank and '#' lines skipped, any
+ * other op type rejected.  Adds carry the weight column when weight_kind is not
+ * SUMINC_W_NONE.  directed == 0 emits both directions of every line.  *count gets
+ * the number of records; they are written to ops when ops != NULL (cap must be at
+ * least *count).  The records go to suminc_apply_batch unchanged. */
+int suminc_updates_read_tsv(const char* path, int weight_kind, int directed, suminc_update* ops, uint64_t cap,
+                            uint64_t* count);
+
+/* Cluster file "<efile>.c_<max_node_num>": whitespace-separated records
+ * "<size> <id> ... <id>", one cluster each (compressor_base.h:3153-3250).  Records
+ * with fewer than min_node_num distinct ids are skipped; the kept ones are numbered
+ * 0.. in file order and cluster_of[v] gets the number (SUMINC_INF_U32 for vertices
+ * in none; a vertex listed in several kept clusters gets the last).  The result
+ * goes to suminc_set_clusters. */
+int suminc_clusters_read(const char* path, uint32_t n, uint32_t min_node_num, uint32_t* cluster_of,
+                         uint32_t* n_clusters);
+/* Writes cluster_of (ids < n or SUMINC_INF_U32) in the same format: one line per
+ * non-empty cluster in id order, members ascending. */
+int suminc_clusters_write(const char* path, uint32_t n, const uint32_t* cluster_of);
 
 #ifdef __cplusplus
 }

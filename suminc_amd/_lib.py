@@ -161,6 +161,14 @@ SIGNATURES = {
     "suminc_wpr_download": (i32, [vp, vp, vp]),
     "suminc_profile_enable": (i32, [vp, i32]),
     "suminc_profile_get": (i32, [vp, i32, C.POINTER(KernelProfile)]),
+    "suminc_edges_read_tsv": (i32, [C.c_char_p, i32, i32, C.POINTER(vp)]),
+    "suminc_edges_info": (i32, [vp, C.POINTER(u32), C.POINTER(u64), C.POINTER(i32), C.POINTER(u64)]),
+    "suminc_edges_copy": (i32, [vp, vp, vp, vp]),
+    "suminc_edges_upload": (i32, [vp, vp]),
+    "suminc_edges_free": (None, [vp]),
+    "suminc_updates_read_tsv": (i32, [C.c_char_p, i32, i32, vp, u64, C.POINTER(u64)]),
+    "suminc_clusters_read": (i32, [C.c_char_p, u32, u32, vp, C.POINTER(u32)]),
+    "suminc_clusters_write": (i32, [C.c_char_p, u32, vp]),
 }
 
 KERNEL_PR_PUSH = 0

@@ -96,6 +96,15 @@ class Context:
         kind = {None: L.W_NONE, "u32": L.W_U32, "f32": L.W_F32}[weights]
         self._check(self._lib.suminc_generate_rmat(self._h, scale, edge_factor, a, b, c, seed, kind, wmin, wmax))
 
+    def load_edge_file(self, path, weights=None, directed=True) -> dict:
+        """Reads an edge file ("u v [w]" lines, the reference's efile) on the host and
+        uploads it; returns n, m and the number of lines that did not parse."""
+        from .textio import EdgeFile
+
+        with EdgeFile(path, weights=weights, directed=directed) as ef:
+            ef.upload(self)
+            return {"n": ef.n, "m": ef.m, "skipped_lines": ef.skipped_lines}
+
     def upload_csr(self, offsets, cols, weights=None):
         off = np.ascontiguousarray(offsets, dtype=np.uint64)
         col = np.ascontiguousarray(cols, dtype=np.uint32)

@@ -136,7 +136,11 @@ int suminc_set_params(suminc_ctx* c, const suminc_params* p) {
   });
 }
 
-const char* suminc_last_error(const suminc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* suminc_last_error(const suminc_ctx* c) { return c ? c->err.c_str() : suminc::io_last_error(); }
+
+int suminc_edges_upload(suminc_ctx* c, const suminc_edges* e) {
+  return guarded(c, [&] { upload_edges(c, e); });
+}
 
 int suminc_sync(suminc_ctx* c) {
   return guarded(c, [&] { SUMINC_HIP(hipStreamSynchronize(c->stream)); });

@@ -28,6 +28,11 @@ struct Error : std::runtime_error {
                             std::string(#expr) + ": " + hipGetErrorString(e_));       \
   } while (0)
 
+// Message of the last failed call without a context on this thread (io.hip).
+const char* io_last_error();
+// Uploads an edge list read by suminc_edges_read_tsv (io.hip).
+void upload_edges(suminc_ctx* c, const suminc_edges* e);
+
 #define SUMINC_REQUIRE(cond, code, msg) \
   do {                                  \
     if (!(cond)) throw ::suminc::Error((code), (msg)); \

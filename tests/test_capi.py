@@ -44,7 +44,8 @@ def test_null_arguments_return_codes():
     assert lib.suminc_ctx_create(0, None, None) == L.E_INVALID
     assert lib.suminc_ctx_destroy(None) == L.E_INVALID
     assert lib.suminc_pr_init(None) == L.E_INVALID
-    assert lib.suminc_last_error(None) == b"null context"
+    # without a context: the last failed context-free call of this thread (none yet here)
+    assert lib.suminc_last_error(None) is not None
 
 
 def test_context_creation_without_gpu_fails_cleanly():
