@@ -60,7 +60,7 @@ def kernel_names():
     import suminc_amd._lib as L
 
     return {L.KERNEL_PR_PUSH: "k_pr_push", L.KERNEL_PR_GATHER: "k_pr_gather",
-            L.KERNEL_SSSP_PUSH: "k_sssp_push+k_sssp_heavy", L.KERNEL_SSSP_SKETCH: "k_ss_push+k_ss_heavy",
+            L.KERNEL_SSSP_PUSH: "k_sssp_push+k_sssp_heavy", L.KERNEL_SSSP_SKETCH: "k_ss_push",
             L.KERNEL_PR_PULL: "k_pr_pull", L.KERNEL_BATCH_COPY: "k_copy_old"}
 
 

@@ -513,31 +513,62 @@ __global__ void k_sssp_roots(const uint32_t* grow, const uint32_t* gcol, const u
 // cut into kChunk-edge chunks (heavy list) that the *_heavy kernels spread over
 // all workgroups, so a hub does not serialise one workgroup.
 __device__ __forceinline__ void reset_child(const unsigned long long* dp, uint32_t u, uint32_t x, uint32_t* rbits,
-                                            uint32_t* rlist, Counters* cnt) {
+                                            uint32_t* rlist, unsigned long long base, unsigned long long* ctr) {
   bool add = false;
   if (x != u && (uint32_t)dp[x] == u && (uint32_t)(dp[x] >> 32) != INF32) {
     uint32_t bit = 1u << (x & 31);
     add = !(atomicOr(&rbits[x >> 5], bit) & bit);
   }
-  unsigned long long slot = wave_append(add, &cnt->aux[4]);
-  if (add) rlist[slot] = x;
+  unsigned long long slot = wave_append(add, ctr);
+  if (add) rlist[base + slot] = x;
 }
 
-__device__ __forceinline__ void heavy_chunks(uint32_t v, uint32_t len, uint2* heavy, Counters* cnt) {
+__device__ __forceinline__ void heavy_chunks(uint32_t v, uint32_t len, uint2* heavy, unsigned long long* ctr) {
   const uint32_t nch = (len + kChunk - 1) / kChunk;
-  const unsigned long long slot = atomicAdd(&cnt->heavy_count, (unsigned long long)nch);
+  const unsigned long long slot = atomicAdd(ctr, (unsigned long long)nch);
   for (uint32_t k = 0; k < nch; ++k) heavy[slot + k] = make_uint2(v, k);
 }
 
+// The walk's rounds are numbered on the host, so no kernel closes a round.  rlist
+// holds the roots (counted by aux[4]) followed by every round's new vertices;
+// round r's frontier is rlist[S_r, S_r + C_r) with S_r in aux[2 + r % 2] (S_0 = 0)
+// and C_r the count of round r - 1's appends, rc[r % 3] (C_0 = aux[4]); round r
+// appends at S_r + C_r, counting in rc[(r + 1) % 3].  Rows longer than kHeavyDeg
+// become kChunk-edge chunks walked by the next round (the walk is a reachability
+// closure, so the order does not matter): round r writes chunks to heavy half
+// r % 2 counted by hc[r % 3] and walks the previous round's (hc[(r - 1) % 3]).  Its
+// first workgroup records S_{r+1}, clears rc[(r + 2) % 3] and hc[(r + 1) % 3] (last
+// read by round r - 1), and when nothing is left sets done, the total in aux[4]
+// and the depth in aux[9].
 __global__ void __launch_bounds__(kBlock) k_reset_round(const uint32_t* off, const uint32_t* colv,
                                                         const unsigned long long* dp, uint32_t* rbits,
-                                                        uint32_t* rlist, uint2* heavy, Counters* cnt) {
+                                                        uint32_t* rlist, uint2* heavy, uint64_t heavy_half,
+                                                        Counters* cnt, uint32_t r) {
   __shared__ uint32_t s_scan[kBlock + 1];
   __shared__ uint32_t s_start[kBlock];
   __shared__ uint32_t s_warp[kBlock / 64 + 1];
   __shared__ uint32_t s_u[kBlock];
   if (cnt->done) return;
-  const unsigned long long b = cnt->aux[2], e = cnt->aux[3];
+  const unsigned long long S = r == 0 ? 0ull : cnt->aux[2 + (r & 1)];
+  const unsigned long long C = r == 0 ? cnt->aux[4] : cnt->rc[r % 3];
+  const unsigned long long nh = r ? cnt->hc[(r - 1) % 3] : 0ull;
+  unsigned long long* out = &cnt->rc[(r + 1) % 3];
+  unsigned long long* hcount = &cnt->hc[r % 3];
+  uint2* hnew = heavy + (r & 1) * heavy_half;
+  const uint2* hold = heavy + ((r + 1) & 1) * heavy_half;
+  const bool stop = C == 0 && nh == 0;  // counts of earlier launches: the whole launch agrees
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    cnt->aux[2 + ((r + 1) & 1)] = S + C;
+    cnt->rc[(r + 2) % 3] = 0;
+    cnt->hc[(r + 1) % 3] = 0;
+    if (stop) {
+      cnt->done = 1;
+      cnt->aux[4] = S;  // every reset vertex
+      cnt->aux[9] = r;  // walk rounds
+    }
+  }
+  if (stop) return;
+  const unsigned long long b = S, e = S + C, base = e;
   const unsigned long long tiles = (e - b + kBlock - 1) / kBlock;
   for (unsigned long long t = blockIdx.x; t < tiles; t += gridDim.x) {
     unsigned long long i = b + t * kBlock + threadIdx.x;
@@ -548,45 +579,30 @@ __global__ void __launch_bounds__(kBlock) k_reset_round(const uint32_t* off, con
       start = off[u];
       len = off[u + 1] - start;
       if (len > kHeavyDeg) {
-        heavy_chunks(u, len, heavy, cnt);
+        heavy_chunks(u, len, hnew, hcount);
         len = 0;
       }
     }
     block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t, uint32_t k) {
-      reset_child(dp, s_u[slot], colv[k], rbits, rlist, cnt);
+      reset_child(dp, s_u[slot], colv[k], rbits, rlist, base, out);
     });
   }
-}
-
-__global__ void __launch_bounds__(kBlock) k_reset_round_heavy(const uint32_t* off, const uint32_t* colv,
-                                                              const unsigned long long* dp, uint32_t* rbits,
-                                                              uint32_t* rlist, const uint2* heavy, Counters* cnt) {
-  if (cnt->done) return;
-  const unsigned long long nh = cnt->heavy_count;
+  // the previous round's heavy chunks
   for (unsigned long long h = blockIdx.x; h < nh; h += gridDim.x) {
-    const uint2 hv = heavy[h];
+    const uint2 hv = hold[h];
     const uint32_t u = hv.x;
-    const uint32_t b = off[u] + hv.y * kChunk;
-    const uint32_t e = min(off[u + 1], b + kChunk);
-    for (uint32_t k0 = b; k0 < e; k0 += kBlock) {  // same trip count for every lane (wave ballots)
+    const uint32_t cb = off[u] + hv.y * kChunk;
+    const uint32_t ce = min(off[u + 1], cb + kChunk);
+    for (uint32_t k0 = cb; k0 < ce; k0 += kBlock) {  // same trip count for every lane (wave ballots)
       const uint32_t k = k0 + threadIdx.x;
-      if (k < e) {
-        reset_child(dp, u, colv[k], rbits, rlist, cnt);
+      if (k < ce) {
+        reset_child(dp, u, colv[k], rbits, rlist, base, out);
       } else {
-        unsigned long long unused = wave_append(false, &cnt->aux[4]);
+        unsigned long long unused = wave_append(false, out);
         (void)unused;
       }
     }
   }
-}
-
-__global__ void k_reset_end(Counters* cnt) {
-  if (threadIdx.x || cnt->done) return;
-  cnt->heavy_count = 0;
-  cnt->aux[9] += 1;  // walk rounds
-  cnt->aux[2] = cnt->aux[3];
-  cnt->aux[3] = cnt->aux[4];
-  if (cnt->aux[3] == cnt->aux[2]) cnt->done = 1;
 }
 
 __global__ void k_reset_apply(const uint32_t* rlist, const Counters* cnt, unsigned long long* dp) {
@@ -628,7 +644,7 @@ __global__ void __launch_bounds__(kBlock) k_reset_pull(const uint32_t* ioff, con
       start = ioff[x];
       len = ioff[x + 1] - start;
       if (len > kHeavyDeg) {
-        heavy_chunks(x, len, heavy, cnt);
+        heavy_chunks(x, len, heavy, &cnt->heavy_count);
         len = 0;
       }
     }
@@ -728,15 +744,11 @@ __global__ void k_sssp_repair_prepare(Counters* cnt) {
   cnt->aux[3] = 0;
   cnt->aux[4] = 0;
   cnt->aux[9] = 0;
+  cnt->rc[0] = cnt->rc[1] = cnt->rc[2] = 0;
+  cnt->hc[0] = cnt->hc[1] = cnt->hc[2] = 0;
   cnt->done = 0;
   cnt->rounds = 0;
   cnt->edges = 0;
-}
-__global__ void k_reset_start(Counters* cnt) {
-  if (threadIdx.x) return;
-  cnt->aux[2] = 0;
-  cnt->aux[3] = cnt->aux[4];
-  cnt->done = cnt->aux[4] == 0;
 }
 __global__ void k_undone(Counters* cnt) {
   if (threadIdx.x == 0) cnt->done = 0;
@@ -970,25 +982,23 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
                                                         b.g_ex.as<uint8_t>(), b.g_eoldw.as<uint32_t>(), bc, N,
                                                         c->sssp_dp.as<unsigned long long>(), c->sssp_src,
                                                         b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(), cnt);
-    k_reset_start<<<1, 64, 0, c->stream>>>(cnt);
     // rounds are enqueued in groups (the walk's depth of the previous batch + 1
     // first), the host checks the device's stop flag between groups
     int group = c->hint_reset;
+    int more = 4;  // then a few more rounds at a time, widening
+    uint32_t r = 0;
     while (true) {
-      for (int i = 0; i < group; ++i) {
-        k_reset_round<<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
-                                                             c->sssp_dp.as<unsigned long long>(),
-                                                             b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(),
-                                                             c->heavy.as<uint2>(), cnt);
-        k_reset_round_heavy<<<push_grid(), kBlock, 0, c->stream>>>(
+      for (int i = 0; i < group; ++i, ++r) {
+        k_reset_round<<<push_grid(), kBlock, 0, c->stream>>>(
             c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(), c->sssp_dp.as<unsigned long long>(),
-            b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(), c->heavy.as<uint2>(), cnt);
-        k_reset_end<<<1, 64, 0, c->stream>>>(cnt);
+            b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(), c->heavy.as<uint2>(), c->heavy.cap / sizeof(uint2) / 2,
+            cnt, r);
       }
       SUMINC_HIP(hipGetLastError());
       read_counters(c);
       if (c->host_counters->done) break;
-      group = 16;
+      group = more;
+      more = std::min(more * 2, 16);
     }
     c->hint_reset = std::max(2, (int)c->host_counters->aux[9] + 1);
     c->stats.reserved[0] = (double)c->host_counters->aux[4];  // reset vertex count

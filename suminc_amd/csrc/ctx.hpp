@@ -89,6 +89,8 @@ struct Counters {
   double residual;                  // last FP64 sum|delta|
   double residual_hist[2];
   double dpad[5];
+  unsigned long long rc[3];         // round-indexed frontier counts (sketch SSSP rounds, reset walk)
+  unsigned long long hc[3];         // round-indexed heavy-chunk counts
 };
 
 // Persistent scratch of the update-batch path (grows, never shrinks), so a batch

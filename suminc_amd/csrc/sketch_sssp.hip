@@ -328,18 +328,21 @@ struct SsArgs {
   unsigned long long* dp;
   uint32_t* wl[2];
   uint32_t* bits[2];
-  uint2* heavy;
+  uint2* heavy;          // two halves of heavy_half entries: chunks made in even / odd rounds
+  uint64_t heavy_half;
   Counters* cnt;
   uint8_t* dirty;
 };
 
-__device__ __forceinline__ void ss_append(const SsArgs& a, int list, uint32_t v, bool add_in) {
+// Appends v to frontier list `list` (bitmap dedup); ctr counts the list.
+__device__ __forceinline__ void ss_append(const SsArgs& a, int list, unsigned long long* ctr, uint32_t v,
+                                          bool add_in) {
   bool add = false;
   if (add_in) {
     uint32_t bit = 1u << (v & 31);
     add = !(atomicOr(&a.bits[list][v >> 5], bit) & bit);
   }
-  unsigned long long slot = wave_append(add, &a.cnt->wl_count[list]);
+  unsigned long long slot = wave_append(add, ctr);
   if (add) a.wl[list][slot] = v;
 }
 
@@ -394,7 +397,7 @@ __global__ void __launch_bounds__(kBlock) k_ss_relax_clusters(SsArgs a, Internal
         improved = !final_pass && j < nexit && (x >> 32) != 0xFFFFFFFFull;
       }
     }
-    ss_append(a, 0, v, improved);
+    ss_append(a, 0, &a.cnt->wl_count[0], v, improved);
   }
   if (threadIdx.x == 0) {
     atomicAdd(work, (unsigned long long)rounds * (pe - pb));
@@ -441,7 +444,7 @@ __global__ void __launch_bounds__(kBlock) k_ss_relax_small(SsArgs a, Internal ie
         improved = !final_pass && (uint32_t)lane < nexit && (x >> 32) != 0xFFFFFFFFull;
       }
     }
-    ss_append(a, 0, v, improved);
+    ss_append(a, 0, &a.cnt->wl_count[0], v, improved);
     if (lane == 0 && final_pass) a.dirty[k] = 0;
     wave_lds_sync();
   }
@@ -450,28 +453,63 @@ __global__ void __launch_bounds__(kBlock) k_ss_relax_small(SsArgs a, Internal ie
   if (threadIdx.x == 0 && s_edges) atomicAdd(work, s_edges);
 }
 
-__device__ __forceinline__ void ss_relax(const SsArgs& a, int nxt, uint32_t v, unsigned long long nd, uint32_t parent) {
+__device__ __forceinline__ void ss_relax(const SsArgs& a, int nxt, unsigned long long* ctr, uint32_t v,
+                                         unsigned long long nd, uint32_t parent) {
   bool add = false;
   if (nd < 0xFFFFFFFFull) {
     const unsigned long long cand = (nd << 32) | parent;
     const unsigned long long old = atomicMin(&a.dp[v], cand);
     add = (old >> 32) > nd;
   }
-  ss_append(a, nxt, v, add);
+  ss_append(a, nxt, ctr, v, add);
+}
+
+// Rounds are numbered on the host and passed to the kernels, so no kernel has to
+// close a round: round r reads list r % 2 counted by rc[r % 3] (round 0: the seeds
+// in list 0, counted by wl_count[0]), appends to list (r + 1) % 2 counted by
+// rc[(r + 1) % 3] and clears rc[(r + 2) % 3] for round r + 1 (last read by round
+// r - 1).  Rows longer than kHeavyDeg become kChunk-edge chunks that the next
+// round relaxes (label-correcting rounds converge in any order; a chunk reads its
+// row's distance when it runs): round r writes chunks to heavy half r % 2 counted
+// by hc[r % 3], relaxes the previous round's from the other half (hc[(r - 1) % 3])
+// and clears hc[(r + 1) % 3].  An empty frontier with no pending chunks, or the
+// round cap, sets done in round r's first workgroup; every later launch returns.
+__device__ __forceinline__ unsigned long long* round_in(Counters* c, uint32_t r) {
+  return r == 0 ? &c->wl_count[0] : &c->rc[r % 3];
 }
 
 // One sketch round over the current frontier: Singles relax their CSR row,
 // exits their external edges, entries their shortcut row to the exits.
-__global__ void __launch_bounds__(kBlock) k_ss_push(SsArgs a) {
+__global__ void __launch_bounds__(kBlock) k_ss_push(SsArgs a, uint32_t r, uint32_t max_rounds) {
   __shared__ uint32_t s_scan[kBlock + 1];
   __shared__ uint32_t s_start[kBlock];
   __shared__ uint32_t s_warp[kBlock / 64 + 1];
   __shared__ uint32_t s_u[kBlock], s_du[kBlock], s_iblen[kBlock], s_mbase[kBlock], s_rb[kBlock], s_k[kBlock];
   __shared__ unsigned long long s_rbase[kBlock];
   __shared__ unsigned long long s_edges, s_sc;
-  if (a.cnt->done) return;
-  const int cur = (int)(a.cnt->aux[1] & 1), nxt = cur ^ 1;
-  const unsigned long long nl = a.cnt->wl_count[cur];
+  Counters* cnt = a.cnt;
+  if (cnt->done) return;
+  const int cur = (int)(r & 1), nxt = cur ^ 1;
+  const unsigned long long nl = *round_in(cnt, r);
+  const unsigned long long nh = r ? cnt->hc[(r - 1) % 3] : 0ull;  // chunks the previous round left
+  unsigned long long* out = &cnt->rc[(r + 1) % 3];
+  unsigned long long* hcount = &cnt->hc[r % 3];
+  uint2* hnew = a.heavy + (r & 1) * a.heavy_half;
+  const uint2* hold = a.heavy + ((r + 1) & 1) * a.heavy_half;
+  const bool stop = (nl == 0 && nh == 0) || r >= max_rounds;  // counts of earlier launches: all agree
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    cnt->rc[(r + 2) % 3] = 0;
+    cnt->hc[(r + 1) % 3] = 0;
+    if (r == 1) cnt->wl_count[0] = 0;  // the seeds were round 0's input
+    if (stop) {
+      cnt->done = 1;
+      if (nl || nh) cnt->aux[1] = 1;  // stopped by the round cap
+    } else {
+      cnt->aux[6] += nl;
+      cnt->rounds = r + 1;
+    }
+  }
+  if (stop) return;
   const unsigned long long tiles = (nl + kBlock - 1) / kBlock;
   if (threadIdx.x == 0) s_edges = s_sc = 0;
   __syncthreads();
@@ -499,8 +537,8 @@ __global__ void __launch_bounds__(kBlock) k_ss_push(SsArgs a) {
         }
         if (iblen > kHeavyDeg) {  // long rows go to 2048-edge chunks
           const uint32_t nch = (iblen + kChunk - 1) / kChunk;
-          const unsigned long long slot = atomicAdd(&a.cnt->heavy_count, (unsigned long long)nch);
-          for (uint32_t c = 0; c < nch; ++c) a.heavy[slot + c] = make_uint2(u, c);
+          const unsigned long long slot = atomicAdd(hcount, (unsigned long long)nch);
+          for (uint32_t c = 0; c < nch; ++c) hnew[slot + c] = make_uint2(u, c);
           atomicAdd(&s_edges, (unsigned long long)iblen);
           iblen = 0;
         }
@@ -519,53 +557,37 @@ __global__ void __launch_bounds__(kBlock) k_ss_push(SsArgs a) {
         const uint32_t e = s_rb[slot] + k;
         const uint32_t x = a.col[e];
         const bool inside = s_k[slot] != INF32 && a.cid[x] == s_k[slot];  // an exit's edge inside its cluster
-        ss_relax(a, nxt, x, inside ? 0xFFFFFFFFull : du + (a.w ? a.w[e] : 1u), u);
+        ss_relax(a, nxt, out, x, inside ? 0xFFFFFFFFull : du + (a.w ? a.w[e] : 1u), u);
       } else {
         const uint32_t j = k - s_iblen[slot];
         const unsigned long long r = s_rbase[slot] + j;
         const uint32_t dj = a.sd[r];
-        ss_relax(a, nxt, a.cl_mem[s_mbase[slot] + j], dj == INF32 ? 0xFFFFFFFFull : du + dj, INF32);
+        ss_relax(a, nxt, out, a.cl_mem[s_mbase[slot] + j], dj == INF32 ? 0xFFFFFFFFull : du + dj, INF32);
       }
     });
   }
-  __syncthreads();
-  if (threadIdx.x == 0 && s_edges) atomicAdd(&a.cnt->edges, s_edges);
-  if (threadIdx.x == 0 && s_sc) atomicAdd(&a.cnt->aux[7], s_sc);
-}
-
-__global__ void __launch_bounds__(kBlock) k_ss_heavy(SsArgs a) {
-  if (a.cnt->done) return;
-  const int cur = (int)(a.cnt->aux[1] & 1), nxt = cur ^ 1;
-  const unsigned long long nh = a.cnt->heavy_count;
+  // the previous round's heavy chunks
   for (unsigned long long h = blockIdx.x; h < nh; h += gridDim.x) {
-    const uint2 hv = a.heavy[h];
+    const uint2 hv = hold[h];
     const uint32_t u = hv.x;
     const unsigned long long du = a.dp[u] >> 32;
     const uint32_t kc = a.ntype[u] == 0 ? INF32 : a.cid[u];
     const uint32_t b = a.off[u] + hv.y * kChunk;
     const uint32_t e = min(a.off[u + 1], b + kChunk);
-    for (uint32_t k0 = b; k0 < e; k0 += kBlock) {
+    for (uint32_t k0 = b; k0 < e; k0 += kBlock) {  // same trip count for every lane (wave ballots)
       const uint32_t k = k0 + threadIdx.x;
       if (k < e) {
         const uint32_t x = a.col[k];
         const bool inside = kc != INF32 && a.cid[x] == kc;
-        ss_relax(a, nxt, x, inside ? 0xFFFFFFFFull : du + (a.w ? a.w[k] : 1u), u);
+        ss_relax(a, nxt, out, x, inside ? 0xFFFFFFFFull : du + (a.w ? a.w[k] : 1u), u);
       } else {
-        ss_append(a, nxt, 0, false);
+        ss_append(a, nxt, out, 0, false);
       }
     }
   }
-}
-
-__global__ void k_ss_end(Counters* cnt, int max_rounds) {
-  if (threadIdx.x != 0 || cnt->done) return;
-  const int cur = (int)(cnt->aux[1] & 1), nxt = cur ^ 1;
-  cnt->aux[6] += cnt->wl_count[cur];
-  cnt->wl_count[cur] = 0;
-  cnt->heavy_count = 0;
-  cnt->rounds += 1;
-  cnt->aux[1] ^= 1ull;
-  if (cnt->wl_count[nxt] == 0 || (long long)cnt->rounds >= (long long)max_rounds) cnt->done = 1;
+  __syncthreads();
+  if (threadIdx.x == 0 && s_edges) atomicAdd(&a.cnt->edges, s_edges);
+  if (threadIdx.x == 0 && s_sc) atomicAdd(&a.cnt->aux[7], s_sc);
 }
 
 // Correction: (a) every member takes min over entries of dist(entry) + in-cluster
@@ -592,6 +614,7 @@ static SsArgs ss_args(suminc_ctx* c) {
   a.bits[0] = c->sssp_flag0.as<uint32_t>();
   a.bits[1] = c->sssp_flag1.as<uint32_t>();
   a.heavy = c->heavy.as<uint2>();
+  a.heavy_half = c->heavy.cap / sizeof(uint2) / 2;
   a.cnt = dev_counters(c);
   a.dirty = s.ss_dirty.as<uint8_t>();
   return a;
@@ -648,6 +671,8 @@ __global__ void k_ss_begin(Counters* cnt) {
   if (threadIdx.x) return;
   cnt->wl_count[1] = 0;
   cnt->heavy_count = 0;
+  cnt->rc[0] = cnt->rc[1] = cnt->rc[2] = 0;
+  cnt->hc[0] = cnt->hc[1] = cnt->hc[2] = 0;
   cnt->aux[1] = 0;
   cnt->aux[6] = 0;
   cnt->aux[7] = 0;
@@ -684,24 +709,23 @@ void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
   }
   SUMINC_HIP(hipGetLastError());
   int group = c->hint_sssp_sketch;  // rounds of the previous query + 1 first
+  int more = 4;                     // then a few more rounds at a time, widening
   size_t slot = 0;
   while (true) {
     for (int i = 0; i < group; ++i, ++slot) {
       if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 2 * slot), c->stream));
-      k_ss_push<<<push_grid(), kBlock, 0, c->stream>>>(a);
-      k_ss_heavy<<<push_grid(), kBlock, 0, c->stream>>>(a);
+      k_ss_push<<<push_grid(), kBlock, 0, c->stream>>>(a, (uint32_t)slot, (uint32_t)max_rounds);
       if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 2 * slot + 1), c->stream));
-      k_ss_end<<<1, 64, 0, c->stream>>>(cnt, max_rounds);
     }
     SUMINC_HIP(hipGetLastError());
     read_counters(c);
     if (c->host_counters->done) break;
-    if (group < 64) group *= 2;
+    group = more;
+    more = std::min(more * 2, 64);
   }
   const Counters* h = c->host_counters;
-  const int cur = (int)(h->aux[1] & 1);
-  SUMINC_REQUIRE(h->wl_count[cur] == 0 || h->rounds < (unsigned long long)max_rounds, SUMINC_E_NOT_CONVERGED,
-                 "round cap reached with a non-empty frontier");
+  const int cur = (int)(h->rounds & 1);  // the list the last (empty) round read
+  SUMINC_REQUIRE(h->aux[1] == 0, SUMINC_E_NOT_CONVERGED, "round cap reached with a non-empty frontier");
   const double rounds = (double)h->rounds, edges = (double)h->edges;
   c->hint_sssp_sketch = std::max(2, std::min(64, (int)h->rounds + 1));
   if (c->prof_on) {
