@@ -692,14 +692,16 @@ __global__ void k_heavy_clear(Counters* cnt) {
   if (threadIdx.x == 0) cnt->heavy_count = 0;
 }
 
-// Seeds SSSP list 0 with every finite reset vertex.
-__global__ void k_reset_seed(const uint32_t* rlist, const unsigned long long* dp, uint32_t* wl0, uint32_t* bits0,
-                             Counters* cnt) {
+// Seeds SSSP list 0 with every finite reset vertex and clears the reset bitmap
+// (the next repair starts from a clear one).
+__global__ void k_reset_seed(const uint32_t* rlist, uint32_t* rbits, const unsigned long long* dp, uint32_t* wl0,
+                             uint32_t* bits0, Counters* cnt) {
   unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   bool add = false;
   uint32_t x = 0;
   if (i < cnt->aux[4]) {
     x = rlist[i];
+    atomicAnd(&rbits[x >> 5], ~(1u << (x & 31)));
     if ((uint32_t)(dp[x] >> 32) != INF32) {
       uint32_t bit = 1u << (x & 31);
       add = !(atomicOr(&bits0[x >> 5], bit) & bit);
@@ -750,11 +752,18 @@ __global__ void k_sssp_repair_prepare(Counters* cnt) {
   cnt->rounds = 0;
   cnt->edges = 0;
 }
-__global__ void k_undone(Counters* cnt) {
-  if (threadIdx.x == 0) cnt->done = 0;
+// End of a batch: the group counts go out with the counters (one copy to the
+// host), and a finished repair walk leaves the stop flag clear for the query.
+__global__ void k_batch_end(Counters* cnt, const unsigned* bc, int undone) {
+  if (threadIdx.x) return;
+  for (int i = 0; i < 4; ++i) cnt->bcopy[i] = bc[i];
+  if (undone) cnt->done = 0;
 }
-__global__ void k_batch_zero(Counters* cnt) {
-  if (threadIdx.x == 0) cnt->aux[5] = 0;
+__global__ void k_batch_zero(Counters* cnt, unsigned* val) {
+  if (threadIdx.x) return;
+  cnt->aux[5] = 0;
+  val[0] = 0;            // invalid ops
+  val[1] = 0xFFFFFFFFu;  // first invalid line
 }
 __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -910,10 +919,10 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   Counters* cnt = dev_counters(c);
   ensure_workspace(c, n_ops);
   BatchWs& b = c->bw;
+  k_batch_zero<<<1, 64, 0, c->stream>>>(cnt, b.val.as<unsigned>());
   if (ops_on_device) {
     // validate before any kernel indexes by the ops' vertex ids
-    unsigned init[2] = {0u, 0xFFFFFFFFu}, res[2];
-    SUMINC_HIP(hipMemcpyAsync(b.val.p, init, 8, hipMemcpyHostToDevice, c->stream));
+    unsigned res[2];
     k_validate_ops<<<grid_for(N), kBlock, 0, c->stream>>>(ops, N, n, c->wkind, b.val.as<unsigned>());
     SUMINC_HIP(hipGetLastError());
     SUMINC_HIP(hipMemcpyAsync(res, b.val.p, 8, hipMemcpyDeviceToHost, c->stream));
@@ -925,23 +934,25 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   // heavy-chunk list sized for the largest graph this batch can produce
   if (c->pr_ready || c->sssp_ready) c->heavy.ensure_slack((cap_m / kChunk + cap_m / kHeavyDeg + 2) * sizeof(uint4));
   unsigned* bc = b.bc.as<unsigned>();
-  k_batch_zero<<<1, 64, 0, c->stream>>>(cnt);
   // An SSSP state that was initialised but never run is still the initial state:
   // it is simply re-initialised on the new graph instead of repaired.
   const bool sssp_repair = c->sssp_ready && !c->sssp_pending;
   const bool sssp_reinit = c->sssp_ready && c->sssp_pending;
 
   // 1. upload + stable sort by (u,v) + one group per pair
-  SUMINC_HIP(hipMemcpyAsync(b.ops.p, ops, n_ops * sizeof(suminc_update),
-                            ops_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
-  k_op_keys<<<grid_for(N), kBlock, 0, c->stream>>>(b.ops.as<suminc_update>(), N, b.keys_a.as<uint64_t>(),
+  const suminc_update* dops = ops;  // device ops are read in place
+  if (!ops_on_device) {
+    SUMINC_HIP(hipMemcpyAsync(b.ops.p, ops, n_ops * sizeof(suminc_update), hipMemcpyHostToDevice, c->stream));
+    dops = b.ops.as<suminc_update>();
+  }
+  k_op_keys<<<grid_for(N), kBlock, 0, c->stream>>>(dops, N, b.keys_a.as<uint64_t>(),
                                                    b.idx_a.as<uint32_t>());
   cub_call(c, c->tmp, [&](void* p, size_t& bytes) {
     return rocprim::radix_sort_pairs(p, bytes, b.keys_a.as<uint64_t>(), b.keys_b.as<uint64_t>(),
                                      b.idx_a.as<uint32_t>(), b.idx_b.as<uint32_t>(), (size_t)N, 0, 64, c->stream);
   });
   k_group_heads<<<grid_for(N), kBlock, 0, c->stream>>>(b.keys_b.as<uint64_t>(), b.idx_b.as<uint32_t>(),
-                                                       b.ops.as<suminc_update>(), N, c->wkind, b.rec.as<uint4>(),
+                                                       dops, N, c->wkind, b.rec.as<uint4>(),
                                                        b.flags.as<uint8_t>());
   cub_call(c, c->tmp, [&](void* p, size_t& bytes) {
     return rocprim::select(p, bytes, b.rec.as<uint4>(), b.flags.as<uint8_t>(), b.grec.as<uint4>(), bc, (size_t)N,
@@ -973,9 +984,12 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   // 4. SSSP: reset roots + subtree walk on the old graph
   if (sssp_repair) {
     const size_t words = ((size_t)n + 31) / 32 + 1;
-    b.rbits.ensure(words * 4);
+    if (b.rbits.cap < words * 4 || !b.rbits_clear) {  // otherwise the last repair left it clear
+      b.rbits.ensure(words * 4);
+      SUMINC_HIP(hipMemsetAsync(b.rbits.p, 0, words * 4, c->stream));
+    }
+    b.rbits_clear = false;
     b.rlist.ensure((size_t)n * 4 + 4);
-    SUMINC_HIP(hipMemsetAsync(b.rbits.p, 0, words * 4, c->stream));
     k_sssp_repair_prepare<<<1, 64, 0, c->stream>>>(cnt);
     k_sssp_roots<<<grid_for(N), kBlock, 0, c->stream>>>(b.g_row.as<uint32_t>(), b.g_col.as<uint32_t>(),
                                                         b.g_act.as<uint8_t>(), b.g_w.as<uint32_t>(),
@@ -1066,19 +1080,21 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
                                                               b.rbits.as<uint32_t>(), c->heavy.as<uint2>(), cnt,
                                                               c->sssp_dp.as<unsigned long long>());
     k_heavy_clear<<<1, 64, 0, c->stream>>>(cnt);
-    k_reset_seed<<<grid_for(n), kBlock, 0, c->stream>>>(b.rlist.as<uint32_t>(), c->sssp_dp.as<unsigned long long>(),
+    k_reset_seed<<<grid_for(n), kBlock, 0, c->stream>>>(b.rlist.as<uint32_t>(), b.rbits.as<uint32_t>(),
+                                                        c->sssp_dp.as<unsigned long long>(),
                                                         c->wl0.as<uint32_t>(), c->sssp_flag0.as<uint32_t>(), cnt);
     k_insert_relax<<<grid_for(N), kBlock, 0, c->stream>>>(b.g_row.as<uint32_t>(), b.g_col.as<uint32_t>(),
                                                           b.g_act.as<uint8_t>(), b.g_w.as<uint32_t>(), bc, N,
                                                           c->wkind != SUMINC_W_U32,
                                                           c->sssp_dp.as<unsigned long long>(), c->wl0.as<uint32_t>(),
                                                           c->sssp_flag0.as<uint32_t>(), cnt);
-    k_undone<<<1, 64, 0, c->stream>>>(cnt);
   }
+  k_batch_end<<<1, 64, 0, c->stream>>>(cnt, bc, sssp_repair ? 1 : 0);
   SUMINC_HIP(hipGetLastError());
-  unsigned hbc[4] = {0, 0, 0, 0};
-  SUMINC_HIP(hipMemcpyAsync(hbc, bc, sizeof(hbc), hipMemcpyDeviceToHost, c->stream));
   read_counters(c);  // synchronises the stream
+  if (sssp_repair) b.rbits_clear = true;
+  unsigned hbc[4];
+  for (int i = 0; i < 4; ++i) hbc[i] = (unsigned)c->host_counters->bcopy[i];
   sketch_batch_done(c);
   if (c->prof_on && m_old) {
     prof_collect(c, SUMINC_KERNEL_BATCH_COPY, 2, 0, 2);

@@ -91,6 +91,7 @@ struct Counters {
   double dpad[5];
   unsigned long long rc[3];         // round-indexed frontier counts (sketch SSSP rounds, reset walk)
   unsigned long long hc[3];         // round-indexed heavy-chunk counts
+  unsigned long long bcopy[4];      // a batch's group counts, copied out with the counters
 };
 
 // Persistent scratch of the update-batch path (grows, never shrinks), so a batch
@@ -107,6 +108,7 @@ struct BatchWs {
   DevBuf val;  // validation of device-resident ops: count, first bad index
   DevBuf bc;  // device counts: [0]=G groups, [1]=touched rows (CSR), [2]=touched rows (CSC), [3]=new m
   uint32_t tix_n = 0;
+  bool rbits_clear = false;  // the reset bitmap is all zero (a repair completed)
 };
 
 // Supernode sketch of the graph (mirror-free form of the reference's compressor):
@@ -149,7 +151,8 @@ struct Sketch {
   DevBuf bound, spnode;      // PageRank sketch accumulators
   DevBuf dirty;              // K: cluster touched since its last precompute
   DevBuf ss_dirty;           // K: SSSP precompute flags (clusters of frontier members)
-  DevBuf ss_work;            // u64 x2: internal edges read by the SSSP shortcut kernels, list length
+  bool ss_dirty_clear = false;  // a finished query left every flag clear
+  DevBuf ss_work;            // u64 x8: internal edges read by the row kernels, then the (small, large) counts of the row, precompute and correction cluster lists
   DevBuf ss_list;            // K: clusters the next per-cluster kernel visits
   DevBuf pr_valid, ss_valid; // K: shortcut rows of the cluster are current
   DevBuf sc_R_old, sc_D_old, ss_dist_old;  // rows of the previous layout

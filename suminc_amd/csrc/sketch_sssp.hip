@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts(const uint32_t* __restr
                                                          Internal ie, uint32_t* __restrict__ sd,
                                                          const uint32_t* __restrict__ list, uint32_t K,
                                                          const unsigned long long* __restrict__ counts,
-                                                         unsigned long long* work) {
+                                                         uint8_t* valid, unsigned long long* work) {
   __shared__ unsigned long long s_dp[kMaxClusterS];
   __shared__ uint32_t s_id[kMaxClusterS];
   __shared__ int s_flag[3];
@@ -186,6 +186,7 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts(const uint32_t* __restr
     const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0, nexit = cl_nexit[k];
     const uint32_t pb = ie.off[k], pe = ie.off[k + 1];
     const uint32_t eb = cl_eoff[k], ee = cl_eoff[k + 1], nent = ee - eb;
+    if (threadIdx.x == 0) valid[k] = 1;
     if (nexit == 0 || nent == 0) continue;
     for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_id[j] = cl_mem[slot0 + j];
     const bool fwd = nent <= nexit;
@@ -231,7 +232,7 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts_small(const uint32_t* _
                                                                uint32_t* __restrict__ sd,
                                                                const uint32_t* __restrict__ list,
                                                                const unsigned long long* __restrict__ counts,
-                                                               unsigned long long* work) {
+                                                               uint8_t* valid, unsigned long long* work) {
   __shared__ unsigned long long s_dp_w[kBlock / 64][kSmallCluster];
   __shared__ uint32_t s_id_w[kBlock / 64][kSmallCluster];
   __shared__ unsigned long long s_edges;
@@ -248,6 +249,7 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts_small(const uint32_t* _
     const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0, nexit = cl_nexit[k];
     const uint32_t pb = ie.off[k], pe = ie.off[k + 1];
     const unsigned long long rbase = clb[k].ss;
+    if (lane == 0) valid[k] = 1;
     bool en = false;
     if (lane < (int)size) {
       s_id[lane] = cl_mem[slot0 + lane];
@@ -629,23 +631,27 @@ static unsigned small_grid(uint32_t K) {
 static void relax_clusters(suminc_ctx* c, const SsArgs& a, const unsigned long long* lcounts, int final_pass) {
   Sketch& s = c->sk;
   k_ss_relax_small<<<small_grid(s.K), kBlock, 0, c->stream>>>(a, internal_of(s), s.ss_list.as<uint32_t>(), lcounts,
-                                                             s.ss_work.as<unsigned long long>(), final_pass);
+                                                             &a.cnt->aux[8], final_pass);
   k_ss_relax_clusters<<<std::min(s.K, kClusterGrid), kBlock, 0, c->stream>>>(
-      a, internal_of(s), s.ss_list.as<uint32_t>(), s.K, lcounts, s.ss_work.as<unsigned long long>(), final_pass);
+      a, internal_of(s), s.ss_list.as<uint32_t>(), s.K, lcounts, &a.cnt->aux[8], final_pass);
 }
 
-// Recomputes the SSSP rows of the clusters whose rows are not current.
-static void compute_sssp_rows(suminc_ctx* c) {
+// Recomputes the SSSP rows of the clusters whose rows are not current; the row
+// kernels mark the clusters they filled valid.  `zeroed`: the list counts in
+// ss_work[1..2] are already zero.
+static void compute_sssp_rows(suminc_ctx* c, bool zeroed) {
   Sketch& s = c->sk;
   if (!s.ss_need && s.ss_rows) return;
   if (!s.ss_rows) SUMINC_HIP(hipMemsetAsync(s.ss_valid.p, 0, (size_t)s.K + 1, c->stream));
   if (!s.ss_rows || s.ss_dist.cap < s.ss_len * 4 + 4) {
     s.ss_dist.ensure_slack(s.ss_len * 4 + 4);
   }
-  s.ss_work.ensure(32);
+  s.ss_work.ensure(64);
   s.ss_list.ensure((size_t)s.K * 4 + 4);
-  SUMINC_HIP(hipMemsetAsync(s.ss_work.p, 0, 32, c->stream));
+  if (!zeroed) SUMINC_HIP(hipMemsetAsync(s.ss_work.p, 0, 64, c->stream));
   unsigned long long* lcounts = s.ss_work.as<unsigned long long>() + 1;
+  // internal edges read by the row kernels: kept apart from the query's traversed edges
+  unsigned long long* work = s.ss_work.as<unsigned long long>();
   if (s.K) {
     k_cluster_lists<<<grid_for(s.K), kBlock, 0, c->stream>>>(s.K, s.ss_valid.as<uint8_t>(), 0,
                                                              s.cl_off.as<uint32_t>(), s.ss_list.as<uint32_t>(),
@@ -653,22 +659,21 @@ static void compute_sssp_rows(suminc_ctx* c) {
     k_ss_shortcuts_small<<<small_grid(s.K), kBlock, 0, c->stream>>>(
         s.cl_off.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.slot_ent.as<uint8_t>(),
         s.lay_base.as<ClBase>(), internal_of(s), s.ss_dist.as<uint32_t>(), s.ss_list.as<uint32_t>(), lcounts,
-        s.ss_work.as<unsigned long long>());
+        s.ss_valid.as<uint8_t>(), work);
     k_ss_shortcuts<<<std::min(s.K, kClusterGrid), kBlock, 0, c->stream>>>(
         s.cl_off.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.cl_eoff.as<uint32_t>(),
         s.cl_ent.as<uint32_t>(), s.lidx.as<uint32_t>(), s.ss_base.as<unsigned long long>(), internal_of(s),
-        s.ss_dist.as<uint32_t>(), s.ss_list.as<uint32_t>(), s.K, lcounts,
-        s.ss_work.as<unsigned long long>());
+        s.ss_dist.as<uint32_t>(), s.ss_list.as<uint32_t>(), s.K, lcounts, s.ss_valid.as<uint8_t>(), work);
   }
   SUMINC_HIP(hipGetLastError());
-  SUMINC_HIP(hipMemsetAsync(s.ss_valid.p, 1, (size_t)s.K + 1, c->stream));
-  SUMINC_HIP(hipStreamSynchronize(c->stream));
   s.ss_rows = true;
   s.ss_need = false;
 }
 
-__global__ void k_ss_begin(Counters* cnt) {
+__global__ void k_ss_begin(Counters* cnt, unsigned long long* lists) {
+  if (threadIdx.x < 8) lists[threadIdx.x] = 0;  // work slot and the three list-count pairs
   if (threadIdx.x) return;
+  cnt->aux[8] = 0;  // internal edges read by the in-cluster kernels
   cnt->wl_count[1] = 0;
   cnt->heavy_count = 0;
   cnt->rc[0] = cnt->rc[1] = cnt->rc[2] = 0;
@@ -685,21 +690,23 @@ void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
   SUMINC_REQUIRE(c->sssp_ready, SUMINC_E_STATE, "SSSP state not initialised");
   sketch_refresh(c);
   Sketch& s = c->sk;
-  compute_sssp_rows(c);
-  s.ss_dirty.ensure((size_t)s.K + 1);
+  s.ss_work.ensure(64);
+  k_ss_begin<<<1, 64, 0, c->stream>>>(dev_counters(c), s.ss_work.as<unsigned long long>());
+  compute_sssp_rows(c, true);
+  if (!s.ss_dirty.p || s.ss_dirty.cap < (size_t)s.K + 1 || !s.ss_dirty_clear) {
+    s.ss_dirty.ensure((size_t)s.K + 1);
+    SUMINC_HIP(hipMemsetAsync(s.ss_dirty.p, 0, (size_t)s.K + 1, c->stream));
+  }
+  s.ss_dirty_clear = false;
   const int max_rounds = c->prm.max_rounds > 0 ? c->prm.max_rounds : 1000000;
   const bool first = c->sssp_pending;
   c->sssp_pending = false;
   SsArgs a = ss_args(c);
   Counters* cnt = a.cnt;
-  s.ss_work.ensure(32);
   s.ss_list.ensure((size_t)s.K * 4 + 4);
-  SUMINC_HIP(hipMemsetAsync(s.ss_work.p, 0, 32, c->stream));
-  unsigned long long* lcounts = s.ss_work.as<unsigned long long>() + 1;
-  SUMINC_HIP(hipMemsetAsync(s.ss_dirty.p, 0, (size_t)s.K + 1, c->stream));
+  unsigned long long* lcounts = s.ss_work.as<unsigned long long>() + 3;
   SUMINC_HIP(hipEventRecord(c->ev_start, c->stream));
   // frontier list 0 holds the seeds (source after init, repair seeds after a batch)
-  k_ss_begin<<<1, 64, 0, c->stream>>>(cnt);
   k_ss_mark<<<256, kBlock, 0, c->stream>>>(a, c->sssp_src, first ? 1 : 0);
   if (s.K) {
     k_cluster_lists<<<grid_for(s.K), kBlock, 0, c->stream>>>(s.K, s.ss_dirty.as<uint8_t>(), 1,
@@ -740,19 +747,19 @@ void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
     p.alg_bytes += 16.0 * (edges - sc) + 20.0 * sc + 16.0 * (double)h->aux[6];
   }
   if (s.K) {
-    SUMINC_HIP(hipMemsetAsync(lcounts, 0, 16, c->stream));
+    unsigned long long* fcounts = s.ss_work.as<unsigned long long>() + 5;
     k_cluster_lists<<<grid_for(s.K), kBlock, 0, c->stream>>>(s.K, s.ss_dirty.as<uint8_t>(), 1,
                                                              s.cl_off.as<uint32_t>(), s.ss_list.as<uint32_t>(),
-                                                             lcounts);
-    relax_clusters(c, a, lcounts, 1);
+                                                             fcounts);
+    relax_clusters(c, a, fcounts, 1);  // clears the dirty flags it visits
   }
   SUMINC_HIP(hipGetLastError());
   SUMINC_HIP(hipEventRecord(c->ev_stop, c->stream));
-  SUMINC_HIP(hipEventSynchronize(c->ev_stop));
+  read_counters(c);  // synchronises the stream
+  s.ss_dirty_clear = true;
   float ms = 0;
   SUMINC_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
-  unsigned long long wk = 0;
-  SUMINC_HIP(hipMemcpy(&wk, s.ss_work.p, 8, hipMemcpyDeviceToHost));
+  const unsigned long long wk = c->host_counters->aux[8];
   if (cur != 0) {  // keep list/bitmap 0 as the current (empty) frontier
     c->wl0.swap(c->tmp5);
     c->sssp_flag0.swap(c->sssp_flag1);
@@ -777,7 +784,7 @@ void sketch_sssp_shortcut(suminc_ctx* c, uint32_t v, uint32_t cap, uint32_t* mem
   sketch_refresh(c);
   Sketch& s = c->sk;
   SUMINC_REQUIRE(v < c->n, SUMINC_E_INVALID, "vertex out of range");
-  compute_sssp_rows(c);
+  compute_sssp_rows(c, false);
   uint32_t e = INF32, k = INF32;
   SUMINC_HIP(hipMemcpy(&e, s.ent_of.as<uint32_t>() + v, 4, hipMemcpyDeviceToHost));
   SUMINC_REQUIRE(e != INF32, SUMINC_E_INVALID, "vertex is not an entry of a supernode");
