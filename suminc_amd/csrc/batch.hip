@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(kBlock) k_new_off(const uint32_t* __restrict__
 // with a new weight, which k_place_adds writes).  Groups are sorted by (row, col),
 // so their positions are non-decreasing; each workgroup copies a tile of
 // kCopyTile edges, finds its first group once and stages the tile's groups in LDS.
-constexpr uint32_t kCopyTile = 4096;
+constexpr uint32_t kCopyTile = 2048;
 constexpr uint32_t kCopyStage = 2048;  // LDS-staged groups per tile
 
 // first group of every tile (groups with pos >= t * kCopyTile); entry ntiles = G
