@@ -49,9 +49,9 @@ WORKLOADS = {
     "sssp22": dict(kind="sssp", scale=22, seed=2, wmin=1, wmax=64, frac=0.001, source=0, max_node=1024,
                    desc="incremental SSSP with supernodes, R-MAT s22 ef16 seed2, u32 w~U[1,64], source 0, "
                         "clusters <= 1024 members, 0.1% batches"),
-    "pr24": dict(kind="pr", scale=24, seed=3, frac=0.01, T=1e-7,
+    "pr24": dict(kind="pr", scale=24, seed=3, frac=0.01, T=1e-7, max_node=1024,
                  desc="incremental FP32/Kahan delta-PageRank, R-MAT s24 ef16 seed3, d=0.85f, 1% batches"),
-    "pr22": dict(kind="pr", scale=22, seed=3, frac=0.01, T=1e-7,
+    "pr22": dict(kind="pr", scale=22, seed=3, frac=0.01, T=1e-7, max_node=1024,
                  desc="incremental FP32/Kahan delta-PageRank, R-MAT s22 ef16 seed3, d=0.85f, 1% batches"),
 }
 
@@ -254,22 +254,32 @@ def run_pr(args, wl):
     ctx.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"])
     n, m, _ = ctx.graph_info()
     log(f"graph: n={n} m={m} ({time.time() - t0:.1f}s)")
+    sketch = args.supernodes == "on" and wl.get("max_node")
+    run = ctx.sum_pr_run if sketch else ctx.pr_run
+    extra_sk = {}
+    if sketch:
+        t0 = time.time()
+        ctx.cluster_leaves(wl["max_node"], 2)
+        info = ctx.sum_build(2, wl["max_node"], 1.0)
+        log(f"sketch: {info} ({time.time() - t0:.1f}s)")
+        extra_sk = {k: info[k] for k in ("clusters", "clustered_vertices", "entries", "sketch_edges",
+                                         "internal_edges", "shortcut_slots", "max_cluster")}
     ctx.pr_init()
-    st = ctx.pr_run()
+    st = run()
     log(f"initial PR: rounds={st['rounds']:.0f} edges={st['edges_traversed']:.3e} {st['kernel_ms']:.1f} ms")
     n_ops = int(wl["frac"] * m)
     batches = [ctx.generate_batch(wl["seed"], b, n_ops // 2, n_ops - n_ops // 2)
                for b in range(args.warmup + args.steps)]
     for b in range(args.warmup):
         ctx.apply_batch(batches[b])
-        ctx.pr_run()
+        run()
     ctx.sync()
     ctx.profile_enable(True)
     edges = 0.0
     t0 = time.perf_counter()
     for b in range(args.warmup, args.warmup + args.steps):
         sa = ctx.apply_batch(batches[b])
-        st = ctx.pr_run()
+        st = run()
         edges += st["edges_traversed"] + sa["reserved"][3]
     ctx.sync()
     dt = time.perf_counter() - t0
@@ -292,6 +302,8 @@ def run_pr(args, wl):
     extra = {"full_recompute_ms": round(full_ms, 3), "speedup_vs_full_recompute": round(full_ms / ms_per_step, 2),
              "edges_per_step": edges / args.steps, "termcheck_threshold": wl["T"],
              "kernels": kernel_table(ctx, kids)}
+    if sketch:
+        extra["sketch"] = extra_sk
     cpu = None
     if not args.no_verify or not args.no_cpu:
         off, col, _ = ctx.download_csr()
@@ -378,7 +390,7 @@ def main():
         "data": "synthetic (seeded R-MAT + seeded update batches, generated on the device)",
         "config": {"workload": wl["desc"], "scale": wl["scale"], "edge_factor": 16, "batch_fraction": wl["frac"],
                    "supernodes": (("on (leaf clusters <= %d members, compress threshold 1)" % wl["max_node"])
-                                  if wl["kind"] == "sssp" and args.supernodes == "on" else "off (plain CSR path)")},
+                                  if args.supernodes == "on" and wl.get("max_node") else "off (plain CSR path)")},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -184,37 +184,53 @@ __global__ void __launch_bounds__(kBlock) k_pr_gather(uint32_t n, const uint32_t
 
 // Pull, short rows (in-degree 1..kPullThread): one thread per vertex sums its
 // in-edges in order; no atomics (each vertex has one owner).
-template <class T>
+// kSketch: the sketch rounds' pull, where an in-edge from a member of v's own
+// cluster is internal (carried by the shortcuts) and skipped.
+template <bool kSketch>
+__device__ __forceinline__ bool pull_skip(const uint32_t* cid, uint32_t u, uint32_t cv) {
+  if constexpr (kSketch) return cv != INF32 && cid[u] == cv;
+  return false;
+}
+
+template <class T, bool kSketch = false>
 __global__ void __launch_bounds__(kBlock) k_pr_pull_short(const uint32_t* __restrict__ list, uint32_t count,
                                                           const uint32_t* __restrict__ ioff,
                                                           const uint32_t* __restrict__ icol,
                                                           const T* __restrict__ contrib, T* __restrict__ delta,
-                                                          const Counters* cnt) {
+                                                          const Counters* cnt, const uint32_t* __restrict__ cid) {
   if (cnt->done || cnt->mode != 1) return;
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
     const uint32_t v = list[i];
     const uint32_t b = ioff[v], e = ioff[v + 1];
+    const uint32_t cv = kSketch ? cid[v] : INF32;
     T acc = T(0);
-    for (uint32_t k = b; k < e; ++k) acc += contrib[icol[k]];
+    for (uint32_t k = b; k < e; ++k) {
+      const uint32_t u = icol[k];
+      if (!pull_skip<kSketch>(cid, u, cv)) acc += contrib[u];
+    }
     if (acc != T(0)) delta[v] += acc;
   }
 }
 
 // Pull, medium rows (kPullThread < in-degree <= kHeavyDeg): one wave per vertex,
 // lanes stride the row (coalesced column reads), shuffle reduction.
-template <class T>
+template <class T, bool kSketch = false>
 __global__ void __launch_bounds__(kBlock) k_pr_pull_wave(const uint32_t* __restrict__ list, uint32_t count,
                                                          const uint32_t* __restrict__ ioff,
                                                          const uint32_t* __restrict__ icol,
                                                          const T* __restrict__ contrib, T* __restrict__ delta,
-                                                         const Counters* cnt) {
+                                                         const Counters* cnt, const uint32_t* __restrict__ cid) {
   if (cnt->done || cnt->mode != 1) return;
   const uint32_t waves = gridDim.x * (kBlock / 64);
   for (uint32_t i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); i < count; i += waves) {
     const uint32_t v = list[i];
     const uint32_t b = ioff[v], e = ioff[v + 1];
+    const uint32_t cv = kSketch ? cid[v] : INF32;
     T acc = T(0);
-    for (uint32_t k = b + lane_id(); k < e; k += 64) acc += contrib[icol[k]];
+    for (uint32_t k = b + lane_id(); k < e; k += 64) {
+      const uint32_t u = icol[k];
+      if (!pull_skip<kSketch>(cid, u, cv)) acc += contrib[u];
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
     if (lane_id() == 0 && acc != T(0)) delta[v] += acc;
@@ -222,20 +238,24 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_wave(const uint32_t* __restr
 }
 
 // Pull, heavy part: 2048-edge chunks of vertices with in-degree > 256.
-template <class T>
+template <class T, bool kSketch = false>
 __global__ void __launch_bounds__(kBlock) k_pr_pull_heavy(const uint32_t* __restrict__ ioff,
                                                           const uint32_t* __restrict__ icol,
                                                           const T* __restrict__ contrib, T* __restrict__ delta,
                                                           const uint2* __restrict__ chunks, uint64_t nchunks,
-                                                          const Counters* cnt) {
+                                                          const Counters* cnt, const uint32_t* __restrict__ cid) {
   __shared__ T s_red[kBlock / 64];
   if (cnt->done || cnt->mode != 1) return;
   for (uint64_t h = blockIdx.x; h < nchunks; h += gridDim.x) {
     const uint2 ch = chunks[h];
     const uint32_t b = ioff[ch.x] + ch.y * kChunk;
     const uint32_t e = min(ioff[ch.x + 1], b + kChunk);
+    const uint32_t cv = kSketch ? cid[ch.x] : INF32;
     T acc = T(0);
-    for (uint32_t k = b + threadIdx.x; k < e; k += kBlock) acc += contrib[icol[k]];
+    for (uint32_t k = b + threadIdx.x; k < e; k += kBlock) {
+      const uint32_t u = icol[k];
+      if (!pull_skip<kSketch>(cid, u, cv)) acc += contrib[u];
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
     if (lane_id() == 0) s_red[threadIdx.x >> 6] = acc;
@@ -371,7 +391,7 @@ void pr_init(suminc_ctx* c) {
   c->pr_ready = true;
 }
 
-static void ensure_pull_heavy(suminc_ctx* c) {
+void ensure_pull_heavy(suminc_ctx* c) {
   if (c->pull_heavy_version == c->graph_version) return;
   c->pull_heavy.ensure((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint2));
   c->pull_short.ensure((size_t)c->n * 4 + 4);
@@ -391,6 +411,32 @@ static void ensure_pull_heavy(suminc_ctx* c) {
   c->pull_heavy_count = h[2];
   c->pull_heavy_version = c->graph_version;
 }
+
+// The pull launches of a round (kernels return unless the round chose pull).
+// cid != nullptr: the sketch rounds' pull (internal in-edges skipped).
+template <class T>
+void pull_scatter(suminc_ctx* c, const uint32_t* cid) {
+  Counters* cnt = dev_counters(c);
+  const bool sk = cid != nullptr;
+  if (c->pull_short_count) {
+    auto k = sk ? k_pr_pull_short<T, true> : k_pr_pull_short<T, false>;
+    k<<<grid_for(c->pull_short_count, kBlock, 8192), kBlock, 0, c->stream>>>(
+        c->pull_short.as<uint32_t>(), (uint32_t)c->pull_short_count, c->in_off.as<uint32_t>(),
+        c->in_col.as<uint32_t>(), c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, cid);
+  }
+  if (c->pull_wave_count) {
+    auto k = sk ? k_pr_pull_wave<T, true> : k_pr_pull_wave<T, false>;
+    k<<<grid_for(c->pull_wave_count, kBlock / 64, 8192), kBlock, 0, c->stream>>>(
+        c->pull_wave.as<uint32_t>(), (uint32_t)c->pull_wave_count, c->in_off.as<uint32_t>(),
+        c->in_col.as<uint32_t>(), c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, cid);
+  }
+  auto kh = sk ? k_pr_pull_heavy<T, true> : k_pr_pull_heavy<T, false>;
+  kh<<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
+                                            c->pr_contrib.as<T>(), c->pr_delta.as<T>(), c->pull_heavy.as<uint2>(),
+                                            c->pull_heavy_count, cnt, cid);
+}
+template void pull_scatter<float>(suminc_ctx* c, const uint32_t* cid);
+template void pull_scatter<double>(suminc_ctx* c, const uint32_t* cid);
 
 // Enqueues one round; with profiling on, events 4*slot .. 4*slot+3 bracket the
 // count+decide, gather and scatter (push or pull) launches.
@@ -413,19 +459,7 @@ static void enqueue_round(suminc_ctx* c, double eps, double T_stop, int max_roun
   k_pr_push<T><<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
                                                       c->pr_delta.as<T>(), c->wl0.as<uint32_t>(), c->wl1.as<T>(),
                                                       c->heavy.as<uint4>(), cnt);
-  if (div > 0) {
-    if (c->pull_short_count)
-      k_pr_pull_short<T><<<grid_for(c->pull_short_count, kBlock, 8192), kBlock, 0, c->stream>>>(
-          c->pull_short.as<uint32_t>(), (uint32_t)c->pull_short_count, c->in_off.as<uint32_t>(),
-          c->in_col.as<uint32_t>(), c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt);
-    if (c->pull_wave_count)
-      k_pr_pull_wave<T><<<grid_for(c->pull_wave_count, kBlock / 64, 8192), kBlock, 0, c->stream>>>(
-          c->pull_wave.as<uint32_t>(), (uint32_t)c->pull_wave_count, c->in_off.as<uint32_t>(),
-          c->in_col.as<uint32_t>(), c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt);
-    k_pr_pull_heavy<T><<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
-                                                              c->pr_contrib.as<T>(), c->pr_delta.as<T>(),
-                                                              c->pull_heavy.as<uint2>(), c->pull_heavy_count, cnt);
-  }
+  if (div > 0) pull_scatter<T>(c, nullptr);
   if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 4 * slot + 3), c->stream));
   k_pr_end<<<1, 1024, 0, c->stream>>>(cnt, reinterpret_cast<unsigned long long*>(c->partial.as<double>()) + 2 * g, g);
 }

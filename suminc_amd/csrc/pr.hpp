@@ -19,6 +19,11 @@ void pr_step(suminc_ctx* c, double* residual);
 void pr_download(suminc_ctx* c, double* rank, double* delta);
 void pr_upload(suminc_ctx* c, const double* rank, const double* delta);
 unsigned push_grid();
+// in-degree classes of the CSC for the pull rounds (rebuilt per graph version)
+void ensure_pull_heavy(suminc_ctx* c);
+// the pull launches of a PageRank round; cid != nullptr: the sketch rounds' pull
+template <class T>
+void pull_scatter(suminc_ctx* c, const uint32_t* cid);
 
 // sssp.hip
 void sssp_init(suminc_ctx* c, uint32_t source);

@@ -1148,40 +1148,75 @@ struct SkArgs {
   const uint32_t* cl_nexit;
 };
 
-// residual of the sketch phase: |delta| of Single/entry vertices + |bound| of exits
+// residual of the sketch phase: |delta| of Single/entry vertices + |bound| of exits,
+// and the out-edges the round's senders will scatter (the push / pull choice)
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_sk_count(SkArgs a, const T* __restrict__ delta,
-                                                     const T* __restrict__ bound) {
+                                                     const T* __restrict__ bound, double eps) {
   __shared__ double s_sum[kBlock / 64];
+  __shared__ unsigned long long s_edges[kBlock / 64];
   if (a.cnt->done) return;
   const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
   double x = 0;
+  unsigned long long ed = 0;
   if (v < a.n) {
-    uint8_t t = a.ntype[v];
-    if (t != 4 && t != 1) x += (double)fabs_t(delta[v]);
-    if (t == 1 || t == 3) x += (double)fabs_t(bound[v]);
+    const uint8_t t = a.ntype[v];
+    const uint32_t deg = a.off[v + 1] - a.off[v];
+    if (t != 4 && t != 1) {
+      const double dv = (double)fabs_t(delta[v]);
+      x += dv;
+      if (t == 0 && dv > eps) ed += deg;
+    }
+    if (t == 1 || t == 3) {
+      const double b = (double)fabs_t(bound[v]);
+      x += b;
+      if (b > eps) ed += deg;
+    }
   }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-  if (lane_id() == 0) s_sum[threadIdx.x >> 6] = x;
+  for (int o = 32; o > 0; o >>= 1) {
+    x += __shfl_xor(x, o);
+    ed += __shfl_xor(ed, o);
+  }
+  if (lane_id() == 0) {
+    s_sum[threadIdx.x >> 6] = x;
+    s_edges[threadIdx.x >> 6] = ed;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = 0;
-    for (int w = 0; w < kBlock / 64; ++w) t += s_sum[w];
+    unsigned long long te = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      t += s_sum[w];
+      te += s_edges[w];
+    }
     a.partial[blockIdx.x] = t;
+    reinterpret_cast<unsigned long long*>(a.partial)[2 * gridDim.x + blockIdx.x] = te;
   }
 }
 
+// Stop rule of the sketch phase and the round's scatter: pull over the CSC when
+// the senders' out-edges exceed m / div (as the plain rounds decide), else push.
 __global__ void __launch_bounds__(1024) k_sk_decide(const double* __restrict__ partial, uint32_t nparts, Counters* cnt,
-                                                    double T, int max_rounds) {
+                                                    double T, int max_rounds, uint64_t m, int div) {
   __shared__ double s[1024];
+  __shared__ unsigned long long se[1024];
   if (cnt->done) return;
+  const unsigned long long* pe = reinterpret_cast<const unsigned long long*>(partial) + 2 * nparts;
   double acc = 0;
-  for (uint32_t i = threadIdx.x; i < nparts; i += 1024) acc += partial[i];
+  unsigned long long acc_e = 0;
+  for (uint32_t i = threadIdx.x; i < nparts; i += 1024) {
+    acc += partial[i];
+    acc_e += pe[i];
+  }
   s[threadIdx.x] = acc;
+  se[threadIdx.x] = acc_e;
   __syncthreads();
   for (int o = 512; o > 0; o >>= 1) {
-    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    if (threadIdx.x < o) {
+      s[threadIdx.x] += s[threadIdx.x + o];
+      se[threadIdx.x] += se[threadIdx.x + o];
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -1192,7 +1227,7 @@ __global__ void __launch_bounds__(1024) k_sk_decide(const double* __restrict__ p
       cnt->done = 1;
       return;
     }
-    cnt->mode = 0;
+    cnt->mode = div > 0 && se[0] * (unsigned long long)div > m ? 1 : 0;
     cnt->rounds += 1;
   }
 }
@@ -1203,8 +1238,10 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) k_sk_gather(SkArgs a, T* __restrict__ delta, T* __restrict__ bound,
                                                       T* __restrict__ spnode, T* __restrict__ rank,
                                                       float* __restrict__ comp, T* __restrict__ wl_c,
-                                                      T* __restrict__ sc_x, double eps, T d, int flush) {
+                                                      T* __restrict__ sc_x, T* __restrict__ contrib, double eps, T d,
+                                                      int flush) {
   if (a.cnt->done) return;
+  const bool pull = a.cnt->mode != 0;  // contributions for the CSC pull instead of push items
   const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
   bool want_push = false, want_sc = false;
   T cval = T(0), sval = T(0);
@@ -1239,7 +1276,8 @@ __global__ void __launch_bounds__(kBlock) k_sk_gather(SkArgs a, T* __restrict__ 
       }
     }
   }
-  push_item<T>(want_push, v, cval, rowlen, a.wl_v, wl_c, a.heavy, a.cnt);
+  if (pull && v < a.n) contrib[v] = want_push ? cval : T(0);
+  push_item<T>(want_push && !pull, v, cval, rowlen, a.wl_v, wl_c, a.heavy, a.cnt);
   unsigned long long slot = wave_append(want_sc, &a.cnt->wl_count[1]);
   if (want_sc) {
     a.sc_e[slot] = a.ent_of[v];
@@ -1437,6 +1475,8 @@ static void sum_pr_run_t(suminc_ctx* c, suminc_stats* st) {
   const int maxit = c->prm.shortcut_max_rounds > 0 ? c->prm.shortcut_max_rounds : 100;
   const T d = (T)c->prm.damping;
   const size_t es = sizeof(T);
+  const int div = c->prm.pull_threshold_div;
+  if (div > 0) ensure_pull_heavy(c);
   compute_pr_rows<T>(c);
   s.bound.ensure((size_t)n * es);
   s.spnode.ensure((size_t)n * es);
@@ -1477,12 +1517,13 @@ static void sum_pr_run_t(suminc_ctx* c, suminc_stats* st) {
   int group = 4;
   while (true) {
     for (int i = 0; i < group; ++i) {
-      k_sk_count<T><<<g, kBlock, 0, c->stream>>>(a, c->pr_delta.as<T>(), s.bound.as<T>());
-      k_sk_decide<<<1, 1024, 0, c->stream>>>(a.partial, g, cnt, T_stop, max_rounds);
+      k_sk_count<T><<<g, kBlock, 0, c->stream>>>(a, c->pr_delta.as<T>(), s.bound.as<T>(), eps);
+      k_sk_decide<<<1, 1024, 0, c->stream>>>(a.partial, g, cnt, T_stop, max_rounds, c->m, div);
       k_sk_gather<T><<<g, kBlock, 0, c->stream>>>(a, c->pr_delta.as<T>(), s.bound.as<T>(), s.spnode.as<T>(),
                                                   c->pr_rank.as<T>(), c->pr_comp.as<float>(), c->wl1.as<T>(),
-                                                  c->tmp3.as<T>(), eps, d, 0);
+                                                  c->tmp3.as<T>(), c->pr_contrib.as<T>(), eps, d, 0);
       sk_push_pass<T>(c);
+      if (div > 0) pull_scatter<T>(c, s.cid.as<uint32_t>());
       k_sk_shortcut<T><<<push_grid(), kBlock, 0, c->stream>>>(
           c->tmp2.as<uint32_t>(), c->tmp3.as<T>(), cnt, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(),
           s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.cl_nexit.as<uint32_t>(),
@@ -1501,7 +1542,7 @@ static void sum_pr_run_t(suminc_ctx* c, suminc_stats* st) {
   k_lists_clear<<<1, 64, 0, c->stream>>>(cnt);
   k_sk_gather<T><<<g, kBlock, 0, c->stream>>>(a, c->pr_delta.as<T>(), s.bound.as<T>(), s.spnode.as<T>(),
                                               c->pr_rank.as<T>(), c->pr_comp.as<float>(), c->wl1.as<T>(),
-                                              c->tmp3.as<T>(), eps, d, 1);
+                                              c->tmp3.as<T>(), c->pr_contrib.as<T>(), eps, d, 1);
   sk_push_pass<T>(c);
   k_lists_clear<<<1, 64, 0, c->stream>>>(cnt);
   if (s.K)
