@@ -283,8 +283,9 @@ def run_pr(args, wl):
         edges += st["edges_traversed"] + sa["reserved"][3]
     ctx.sync()
     dt = time.perf_counter() - t0
-    kids = [L.KERNEL_PR_PUSH, L.KERNEL_PR_GATHER, L.KERNEL_BATCH_COPY]
+    kids = [L.KERNEL_PR_PUSH, L.KERNEL_PR_PULL, L.KERNEL_PR_GATHER, L.KERNEL_BATCH_COPY]
     kernel, roof = roofline(ctx, kids)
+    ktab = kernel_table(ctx, kids)
     ctx.profile_enable(False)
     ms_per_step = 1e3 * dt / args.steps
     gteps = edges / dt / 1e9
@@ -301,7 +302,7 @@ def run_pr(args, wl):
     full_ms = 1e3 * float(np.median(full))
     extra = {"full_recompute_ms": round(full_ms, 3), "speedup_vs_full_recompute": round(full_ms / ms_per_step, 2),
              "edges_per_step": edges / args.steps, "termcheck_threshold": wl["T"],
-             "kernels": kernel_table(ctx, kids)}
+             "kernels": ktab}
     if sketch:
         extra["sketch"] = extra_sk
     cpu = None

@@ -400,16 +400,20 @@ __global__ void k_gather_groups(const uint32_t* sidx, const unsigned* bc, const 
 }
 
 // ---------------------------------------------------------------------------
-// PageRank reseed: list (u, type*d*r_u/deg) for touched u over the current CSR
+// PageRank reseed: (u, type*d*r_u/deg) for touched u over the current CSR, as a
+// push list and as the pull contribution (contrib is zero elsewhere); the edge
+// total picks the scatter like a round does (k_reseed_mode)
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ void k_reseed_list(const uint32_t* t_list, const unsigned* bc, uint32_t N, const uint32_t* off,
-                              const T* rank, const float* comp, T d, T type, uint32_t* wl_v, T* wl_c, uint4* heavy,
-                              Counters* cnt) {
+__global__ void __launch_bounds__(kBlock) k_reseed_list(const uint32_t* t_list, const unsigned* bc, uint32_t N,
+                                                        const uint32_t* off, const T* rank, const float* comp, T d,
+                                                        T type, uint32_t* wl_v, T* wl_c, uint4* heavy, T* contrib,
+                                                        Counters* cnt) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   bool light = false;
   uint32_t u = 0;
   T c = T(0);
+  unsigned long long edges = 0;
   if (t < N && t < bc[1]) {
     u = t_list[t];
     uint32_t deg = off[u + 1] - off[u];
@@ -417,7 +421,8 @@ __global__ void k_reseed_list(const uint32_t* t_list, const unsigned* bc, uint32
     if constexpr (sizeof(T) == 4) r = (T)((double)r - (double)comp[u]);
     if (deg > 0 && r != T(0)) {
       c = type * d * r / (T)deg;
-      atomicAdd(&cnt->aux[5], (unsigned long long)deg);
+      contrib[u] = c;
+      edges = deg;
       if (deg <= kHeavyDeg) {
         light = true;
       } else {
@@ -438,6 +443,18 @@ __global__ void k_reseed_list(const uint32_t* t_list, const unsigned* bc, uint32
   }
   unsigned long long slot = wave_append(light, &cnt->wl_count[0]);
   if (light) { wl_v[slot] = u; wl_c[slot] = c; }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) edges += __shfl_xor(edges, o);
+  if (lane_id() == 0 && edges) atomicAdd(&cnt->round_edges, edges);
+}
+
+// Scatter of the reseed: pull over the CSC when its edges exceed m / div (the
+// rounds' rule), else push; the edges add to the batch's reseed total (aux[5]).
+__global__ void k_reseed_mode(Counters* cnt, uint64_t m, int div) {
+  if (threadIdx.x) return;
+  cnt->reseed_mode = div > 0 && cnt->round_edges * (unsigned long long)div > m ? 1ull : 0ull;
+  cnt->aux[5] += cnt->round_edges;
+  cnt->round_edges = 0;
 }
 
 // Same scatter as the PageRank push round, over the reseed list.
@@ -450,6 +467,7 @@ __global__ void __launch_bounds__(kBlock) k_reseed_push(const uint32_t* __restri
   __shared__ uint32_t s_start[kBlock];
   __shared__ uint32_t s_warp[kBlock / 64 + 1];
   __shared__ T s_c[kBlock];
+  if (cnt->reseed_mode != 0) return;
   const unsigned long long nl = cnt->wl_count[0];
   const unsigned long long nh = cnt->heavy_count;
   const unsigned long long tiles = (nl + kBlock - 1) / kBlock;
@@ -480,6 +498,8 @@ __global__ void k_reset_lists(Counters* cnt) {
   if (threadIdx.x) return;
   cnt->wl_count[0] = 0;
   cnt->heavy_count = 0;
+  cnt->round_edges = 0;
+  cnt->reseed_mode = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -815,19 +835,26 @@ static void merge_direction(suminc_ctx* c, DevBuf& off, DevBuf& col, DevBuf& w, 
   if (c->wkind) w.swap(nw);
 }
 
+// AmendValue over the touched sources' out-edges of the current graph: push for a
+// few edges, or, when they exceed m / div, one pull over the CSC (whose work
+// lists must be current: `lists` false leaves the pull out).
 template <class T>
-static void pr_reseed(suminc_ctx* c, uint32_t N, double type) {
+static void pr_reseed(suminc_ctx* c, uint32_t N, double type, bool lists) {
   Counters* cnt = dev_counters(c);
   BatchWs& b = c->bw;
+  const int div = lists ? c->prm.pull_threshold_div : 0;
   k_reset_lists<<<1, 64, 0, c->stream>>>(cnt);
+  if (div > 0) SUMINC_HIP(hipMemsetAsync(c->pr_contrib.p, 0, (size_t)c->n * sizeof(T), c->stream));
   k_reseed_list<T><<<grid_for(N), kBlock, 0, c->stream>>>(b.t_list.as<uint32_t>(), b.bc.as<unsigned>(), N,
                                                           c->out_off.as<uint32_t>(), c->pr_rank.as<T>(),
                                                           c->pr_comp.as<float>(), (T)c->prm.damping, (T)type,
                                                           c->wl0.as<uint32_t>(), c->wl1.as<T>(), c->heavy.as<uint4>(),
-                                                          cnt);
+                                                          c->pr_contrib.as<T>(), cnt);
+  k_reseed_mode<<<1, 64, 0, c->stream>>>(cnt, c->m, div);
   k_reseed_push<T><<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
                                                           c->pr_delta.as<T>(), c->wl0.as<uint32_t>(), c->wl1.as<T>(),
                                                           c->heavy.as<uint4>(), cnt);
+  if (div > 0) pull_scatter<T>(c, nullptr, nullptr, 1);
   k_reset_lists<<<1, 64, 0, c->stream>>>(cnt);
   SUMINC_HIP(hipGetLastError());
 }
@@ -976,9 +1003,11 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
                b.g_w.as<uint32_t>(), bc, N);
 
   // 3. PageRank reseed -1 on the old out-edges of touched sources
+  const bool pr_pull = c->pr_ready && c->prm.pull_threshold_div > 0;
+  if (pr_pull) ensure_pull_heavy(c);
   if (c->pr_ready) {
-    if (c->pr_prec == SUMINC_PR_FP64) pr_reseed<double>(c, N, -1.0);
-    else pr_reseed<float>(c, N, -1.0);
+    if (c->pr_prec == SUMINC_PR_FP64) pr_reseed<double>(c, N, -1.0, pr_pull);
+    else pr_reseed<float>(c, N, -1.0, pr_pull);
   }
 
   // 4. SSSP: reset roots + subtree walk on the old graph
@@ -1059,10 +1088,12 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   merge_direction(c, c->in_off, c->in_col, c->in_w, b.noff_in, b.ncol_in, b.nw_in, gt, N, cap_m, 2);
   k_untouch<<<grid_for(N), kBlock, 0, c->stream>>>(b.tt_list.as<uint32_t>(), bc, 2, N, b.ttix.as<uint32_t>());
 
-  // 6. PageRank reseed +1 on the new out-edges
+  // 6. PageRank reseed +1 on the new out-edges; the pull lists of the new CSC are
+  // built here (they serve the next query as well)
+  if (pr_pull) build_pull_lists(c, cap_m);
   if (c->pr_ready) {
-    if (c->pr_prec == SUMINC_PR_FP64) pr_reseed<double>(c, N, +1.0);
-    else pr_reseed<float>(c, N, +1.0);
+    if (c->pr_prec == SUMINC_PR_FP64) pr_reseed<double>(c, N, +1.0, pr_pull);
+    else pr_reseed<float>(c, N, +1.0, pr_pull);
   }
   k_untouch<<<grid_for(N), kBlock, 0, c->stream>>>(b.t_list.as<uint32_t>(), bc, 1, N, b.tix.as<uint32_t>());
 
@@ -1107,6 +1138,7 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   }
   c->m = hbc[3];
   c->graph_version++;
+  if (pr_pull) c->pull_heavy_version = c->graph_version;  // built in step 6
   // per-round scratch sized by m may need to grow
   if (c->pr_ready || c->sssp_ready) c->heavy.ensure_slack((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
   c->stats.reserved[1] = (double)hbc[0];

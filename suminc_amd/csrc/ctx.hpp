@@ -92,6 +92,7 @@ struct Counters {
   unsigned long long rc[3];         // round-indexed frontier counts (sketch SSSP rounds, reset walk)
   unsigned long long hc[3];         // round-indexed heavy-chunk counts
   unsigned long long bcopy[4];      // a batch's group counts, copied out with the counters
+  unsigned long long reseed_mode;   // PageRank reseed scatter: 0 push, 1 pull
 };
 
 // Persistent scratch of the update-batch path (grows, never shrinks), so a batch
@@ -149,6 +150,7 @@ struct Sketch {
   DevBuf sc_R, sc_D;         // PageRank rows (reach mass / residual) per entry
   DevBuf ss_dist;            // SSSP rows (in-cluster distance to every exit) per entry
   DevBuf bound, spnode;      // PageRank sketch accumulators
+  DevBuf pull_chk;           // n: an internal in-edge from an exit of the vertex's cluster (sketch pull)
   DevBuf dirty;              // K: cluster touched since its last precompute
   DevBuf ss_dirty;           // K: SSSP precompute flags (clusters of frontier members)
   bool ss_dirty_clear = false;  // a finished query left every flag clear
@@ -183,7 +185,9 @@ struct suminc_ctx {
   suminc::DevBuf pr_delta, pr_rank, pr_comp, pr_contrib;
   // pull mode: static chunk list of high in-degree vertices, rebuilt per graph version
   suminc::DevBuf pull_heavy, pull_short, pull_wave;
-  uint64_t pull_heavy_count = 0, pull_short_count = 0, pull_wave_count = 0, pull_heavy_version = ~0ull;
+  suminc::DevBuf pull_cnt;    // u64 x3: lengths of the short, wave and chunk lists
+  suminc::DevBuf pull_parts;  // per-block class counts of the list build
+  uint64_t pull_heavy_version = ~0ull;
   uint64_t graph_version = 0;
 
   // SSSP state: packed {dist:hi32, parent:lo32}

@@ -182,8 +182,12 @@ __global__ void __launch_bounds__(kBlock) k_pr_gather(uint32_t n, const uint32_t
   }
 }
 
-// Pull, short rows (in-degree 1..kPullThread): one thread per vertex sums its
-// in-edges in order; no atomics (each vertex has one owner).
+// Whether the pull launches of this point in the stream run: gate 0, a round
+// that chose pull (not stopped); gate 1, a reseed that chose pull.
+__device__ __forceinline__ bool pull_runs(const Counters* cnt, int gate) {
+  return gate ? cnt->reseed_mode == 1 : (!cnt->done && cnt->mode == 1);
+}
+
 // kSketch: the sketch rounds' pull, where an in-edge from a member of v's own
 // cluster is internal (carried by the shortcuts) and skipped.
 template <bool kSketch>
@@ -192,17 +196,33 @@ __device__ __forceinline__ bool pull_skip(const uint32_t* cid, uint32_t u, uint3
   return false;
 }
 
+// The cluster whose in-edges v must check.  In the sketch pull only Singles and
+// exits contribute, so only an internal in-edge from an exit of v's cluster must
+// be skipped: chk[v] != 0 marks the vertices that have one.  For the others an
+// internal in-edge adds zero and needs no check.
+template <bool kSketch>
+__device__ __forceinline__ uint32_t pull_cluster(const uint32_t* cid, const uint8_t* chk, uint32_t v) {
+  if constexpr (kSketch) return chk[v] ? cid[v] : INF32;
+  return INF32;
+}
+
+// Pull, short rows (in-degree 1..kPullThread): one thread per vertex sums its
+// in-edges in order; no atomics (each vertex has one owner).
 template <class T, bool kSketch = false>
-__global__ void __launch_bounds__(kBlock) k_pr_pull_short(const uint32_t* __restrict__ list, uint32_t count,
+__global__ void __launch_bounds__(kBlock) k_pr_pull_short(const uint32_t* __restrict__ list,
+                                                          const unsigned long long* __restrict__ counts,
                                                           const uint32_t* __restrict__ ioff,
                                                           const uint32_t* __restrict__ icol,
                                                           const T* __restrict__ contrib, T* __restrict__ delta,
-                                                          const Counters* cnt, const uint32_t* __restrict__ cid) {
-  if (cnt->done || cnt->mode != 1) return;
+                                                          const Counters* cnt, int gate,
+                                                          const uint32_t* __restrict__ cid,
+                                                          const uint8_t* __restrict__ chk) {
+  if (!pull_runs(cnt, gate)) return;
+  const uint32_t count = (uint32_t)counts[0];
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
     const uint32_t v = list[i];
     const uint32_t b = ioff[v], e = ioff[v + 1];
-    const uint32_t cv = kSketch ? cid[v] : INF32;
+    const uint32_t cv = pull_cluster<kSketch>(cid, chk, v);
     T acc = T(0);
     for (uint32_t k = b; k < e; ++k) {
       const uint32_t u = icol[k];
@@ -215,17 +235,21 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_short(const uint32_t* __rest
 // Pull, medium rows (kPullThread < in-degree <= kHeavyDeg): one wave per vertex,
 // lanes stride the row (coalesced column reads), shuffle reduction.
 template <class T, bool kSketch = false>
-__global__ void __launch_bounds__(kBlock) k_pr_pull_wave(const uint32_t* __restrict__ list, uint32_t count,
+__global__ void __launch_bounds__(kBlock) k_pr_pull_wave(const uint32_t* __restrict__ list,
+                                                         const unsigned long long* __restrict__ counts,
                                                          const uint32_t* __restrict__ ioff,
                                                          const uint32_t* __restrict__ icol,
                                                          const T* __restrict__ contrib, T* __restrict__ delta,
-                                                         const Counters* cnt, const uint32_t* __restrict__ cid) {
-  if (cnt->done || cnt->mode != 1) return;
+                                                         const Counters* cnt, int gate,
+                                                         const uint32_t* __restrict__ cid,
+                                                         const uint8_t* __restrict__ chk) {
+  if (!pull_runs(cnt, gate)) return;
+  const uint32_t count = (uint32_t)counts[1];
   const uint32_t waves = gridDim.x * (kBlock / 64);
   for (uint32_t i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); i < count; i += waves) {
     const uint32_t v = list[i];
     const uint32_t b = ioff[v], e = ioff[v + 1];
-    const uint32_t cv = kSketch ? cid[v] : INF32;
+    const uint32_t cv = pull_cluster<kSketch>(cid, chk, v);
     T acc = T(0);
     for (uint32_t k = b + lane_id(); k < e; k += 64) {
       const uint32_t u = icol[k];
@@ -242,15 +266,19 @@ template <class T, bool kSketch = false>
 __global__ void __launch_bounds__(kBlock) k_pr_pull_heavy(const uint32_t* __restrict__ ioff,
                                                           const uint32_t* __restrict__ icol,
                                                           const T* __restrict__ contrib, T* __restrict__ delta,
-                                                          const uint2* __restrict__ chunks, uint64_t nchunks,
-                                                          const Counters* cnt, const uint32_t* __restrict__ cid) {
+                                                          const uint2* __restrict__ chunks,
+                                                          const unsigned long long* __restrict__ counts,
+                                                          const Counters* cnt, int gate,
+                                                          const uint32_t* __restrict__ cid,
+                                                          const uint8_t* __restrict__ chk) {
   __shared__ T s_red[kBlock / 64];
-  if (cnt->done || cnt->mode != 1) return;
+  if (!pull_runs(cnt, gate)) return;
+  const uint64_t nchunks = counts[2];
   for (uint64_t h = blockIdx.x; h < nchunks; h += gridDim.x) {
     const uint2 ch = chunks[h];
     const uint32_t b = ioff[ch.x] + ch.y * kChunk;
     const uint32_t e = min(ioff[ch.x + 1], b + kChunk);
-    const uint32_t cv = kSketch ? cid[ch.x] : INF32;
+    const uint32_t cv = pull_cluster<kSketch>(cid, chk, ch.x);
     T acc = T(0);
     for (uint32_t k = b + threadIdx.x; k < e; k += kBlock) {
       const uint32_t u = icol[k];
@@ -295,23 +323,82 @@ __global__ void __launch_bounds__(1024) k_pr_end(Counters* cnt, const unsigned l
   cnt->heavy_count = 0;
 }
 
-// Static pull work lists, rebuilt per graph version: vertices by in-degree
-// class (short: 1..kPullThread, wave: ..kHeavyDeg) and 2048-edge chunks above.
-__global__ void k_pull_lists(uint32_t n, const uint32_t* ioff, uint32_t* short_list, uint32_t* wave_list,
-                             uint2* heavy, unsigned long long* counts) {
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t deg = v < n ? ioff[v + 1] - ioff[v] : 0;
-  const bool is_short = deg >= 1 && deg <= kPullThread;
-  const bool is_wave = deg > kPullThread && deg <= kHeavyDeg;
-  unsigned long long s1 = wave_append(is_short, &counts[0]);
-  if (is_short) short_list[s1] = v;
-  unsigned long long s2 = wave_append(is_wave, &counts[1]);
-  if (is_wave) wave_list[s2] = v;
-  if (deg > kHeavyDeg) {
-    uint32_t nch = (deg + kChunk - 1) / kChunk;
-    unsigned long long slot = atomicAdd(&counts[2], (unsigned long long)nch);
-    for (uint32_t k = 0; k < nch; ++k) heavy[slot + k] = make_uint2(v, k);
+// Static pull work lists, rebuilt per graph version, in vertex order and without
+// atomics: vertices by in-degree class (short: 1..kPullThread, wave: ..kHeavyDeg)
+// and the 2048-edge chunks of the longer rows.  Three passes: class counts per
+// block, a one-block exclusive scan of the counts (totals to counts[0..2]), and
+// the ordered fill.
+__device__ __forceinline__ void pull_classes(uint32_t deg, uint32_t& s, uint32_t& w, uint32_t& h) {
+  s = deg >= 1 && deg <= kPullThread;
+  w = deg > kPullThread && deg <= kHeavyDeg;
+  h = deg > kHeavyDeg ? (deg + kChunk - 1) / kChunk : 0u;
+}
+
+__global__ void __launch_bounds__(kBlock) k_pull_count(uint32_t n, const uint32_t* __restrict__ ioff,
+                                                       uint32_t* __restrict__ parts) {
+  __shared__ uint32_t s_w[3][kBlock / 64];
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t cs = 0, cw = 0, ch = 0;
+  if (v < n) pull_classes(ioff[v + 1] - ioff[v], cs, cw, ch);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    cs += __shfl_xor(cs, o);
+    cw += __shfl_xor(cw, o);
+    ch += __shfl_xor(ch, o);
   }
+  const int wid = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    s_w[0][wid] = cs;
+    s_w[1][wid] = cw;
+    s_w[2][wid] = ch;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    uint32_t t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += s_w[threadIdx.x][w];
+    parts[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// One block: each thread scans a contiguous run of the g block counts of a class.
+__global__ void __launch_bounds__(1024) k_pull_scan(uint32_t g, uint32_t* __restrict__ parts,
+                                                    unsigned long long* __restrict__ counts) {
+  __shared__ uint32_t s_warp[1024 / 64 + 1];
+  const uint32_t per = (g + 1023) / 1024;
+  const uint32_t b = min(g, threadIdx.x * per), e = min(g, b + per);
+  for (int k = 0; k < 3; ++k) {
+    uint32_t* p = parts + (size_t)k * g;
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < e; ++i) sum += p[i];
+    uint32_t total;
+    uint32_t run = block_exclusive_scan<1024>(sum, s_warp, &total);
+    for (uint32_t i = b; i < e; ++i) {
+      const uint32_t x = p[i];
+      p[i] = run;
+      run += x;
+    }
+    if (threadIdx.x == 0) counts[k] = total;
+    __syncthreads();  // s_warp is reused by the next class
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_pull_fill(uint32_t n, const uint32_t* __restrict__ ioff,
+                                                      const uint32_t* __restrict__ parts,
+                                                      uint32_t* __restrict__ short_list,
+                                                      uint32_t* __restrict__ wave_list, uint2* __restrict__ heavy) {
+  __shared__ uint32_t s_warp[kBlock / 64 + 1];
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t cs = 0, cw = 0, ch = 0, tot;
+  if (v < n) pull_classes(ioff[v + 1] - ioff[v], cs, cw, ch);
+  const uint32_t ps = block_exclusive_scan<kBlock>(cs, s_warp, &tot);
+  __syncthreads();
+  const uint32_t pw = block_exclusive_scan<kBlock>(cw, s_warp, &tot);
+  __syncthreads();
+  const uint32_t ph = block_exclusive_scan<kBlock>(ch, s_warp, &tot);
+  const uint32_t g = gridDim.x;
+  if (cs) short_list[parts[blockIdx.x] + ps] = v;
+  if (cw) wave_list[parts[g + blockIdx.x] + pw] = v;
+  for (uint32_t k = 0; k < ch; ++k) heavy[parts[2 * g + blockIdx.x] + ph + k] = make_uint2(v, k);
 }
 
 template <class T>
@@ -391,52 +478,52 @@ void pr_init(suminc_ctx* c) {
   c->pr_ready = true;
 }
 
-void ensure_pull_heavy(suminc_ctx* c) {
-  if (c->pull_heavy_version == c->graph_version) return;
-  c->pull_heavy.ensure((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint2));
+void build_pull_lists(suminc_ctx* c, uint64_t m_cap) {
+  const unsigned g = grid_for(c->n);
+  c->pull_heavy.ensure_slack((m_cap / kChunk + m_cap / kHeavyDeg + 2) * sizeof(uint2));
   c->pull_short.ensure((size_t)c->n * 4 + 4);
   c->pull_wave.ensure((size_t)c->n * 4 + 4);
-  DevBuf cnt;
-  cnt.ensure(32);
-  SUMINC_HIP(hipMemsetAsync(cnt.p, 0, 32, c->stream));
-  k_pull_lists<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->in_off.as<uint32_t>(), c->pull_short.as<uint32_t>(),
-                                                        c->pull_wave.as<uint32_t>(), c->pull_heavy.as<uint2>(),
-                                                        cnt.as<unsigned long long>());
+  c->pull_parts.ensure((size_t)g * 3 * 4);
+  c->pull_cnt.ensure(4 * 8);
+  k_pull_count<<<g, kBlock, 0, c->stream>>>(c->n, c->in_off.as<uint32_t>(), c->pull_parts.as<uint32_t>());
+  k_pull_scan<<<1, 1024, 0, c->stream>>>(g, c->pull_parts.as<uint32_t>(), c->pull_cnt.as<unsigned long long>());
+  k_pull_fill<<<g, kBlock, 0, c->stream>>>(c->n, c->in_off.as<uint32_t>(), c->pull_parts.as<uint32_t>(),
+                                           c->pull_short.as<uint32_t>(), c->pull_wave.as<uint32_t>(),
+                                           c->pull_heavy.as<uint2>());
   SUMINC_HIP(hipGetLastError());
-  unsigned long long h[4] = {0, 0, 0, 0};
-  SUMINC_HIP(hipMemcpyAsync(h, cnt.p, 32, hipMemcpyDeviceToHost, c->stream));
-  SUMINC_HIP(hipStreamSynchronize(c->stream));
-  c->pull_short_count = h[0];
-  c->pull_wave_count = h[1];
-  c->pull_heavy_count = h[2];
+}
+
+void ensure_pull_heavy(suminc_ctx* c) {
+  if (c->pull_heavy_version == c->graph_version) return;
+  build_pull_lists(c, c->m);
   c->pull_heavy_version = c->graph_version;
 }
 
-// The pull launches of a round (kernels return unless the round chose pull).
-// cid != nullptr: the sketch rounds' pull (internal in-edges skipped).
+// The pull launches of a round (kernels return unless the gate opens, see
+// pull_runs).  The list lengths stay on the device.  cid != nullptr: the sketch
+// rounds' pull (internal in-edges skipped; chk as in pull_cluster).
 template <class T>
-void pull_scatter(suminc_ctx* c, const uint32_t* cid) {
+void pull_scatter(suminc_ctx* c, const uint32_t* cid, const uint8_t* chk, int gate) {
   Counters* cnt = dev_counters(c);
   const bool sk = cid != nullptr;
-  if (c->pull_short_count) {
-    auto k = sk ? k_pr_pull_short<T, true> : k_pr_pull_short<T, false>;
-    k<<<grid_for(c->pull_short_count, kBlock, 8192), kBlock, 0, c->stream>>>(
-        c->pull_short.as<uint32_t>(), (uint32_t)c->pull_short_count, c->in_off.as<uint32_t>(),
-        c->in_col.as<uint32_t>(), c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, cid);
-  }
-  if (c->pull_wave_count) {
-    auto k = sk ? k_pr_pull_wave<T, true> : k_pr_pull_wave<T, false>;
-    k<<<grid_for(c->pull_wave_count, kBlock / 64, 8192), kBlock, 0, c->stream>>>(
-        c->pull_wave.as<uint32_t>(), (uint32_t)c->pull_wave_count, c->in_off.as<uint32_t>(),
-        c->in_col.as<uint32_t>(), c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, cid);
-  }
+  const unsigned long long* counts = c->pull_cnt.as<unsigned long long>();
+  auto ks = sk ? k_pr_pull_short<T, true> : k_pr_pull_short<T, false>;
+  ks<<<grid_for(c->n, kBlock, 8192), kBlock, 0, c->stream>>>(c->pull_short.as<uint32_t>(), counts,
+                                                              c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
+                                                              c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, gate,
+                                                              cid, chk);
+  auto kw = sk ? k_pr_pull_wave<T, true> : k_pr_pull_wave<T, false>;
+  kw<<<grid_for(c->n, kBlock / 64, 8192), kBlock, 0, c->stream>>>(c->pull_wave.as<uint32_t>(), counts,
+                                                                   c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
+                                                                   c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt,
+                                                                   gate, cid, chk);
   auto kh = sk ? k_pr_pull_heavy<T, true> : k_pr_pull_heavy<T, false>;
   kh<<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
                                             c->pr_contrib.as<T>(), c->pr_delta.as<T>(), c->pull_heavy.as<uint2>(),
-                                            c->pull_heavy_count, cnt, cid);
+                                            counts, cnt, gate, cid, chk);
 }
-template void pull_scatter<float>(suminc_ctx* c, const uint32_t* cid);
-template void pull_scatter<double>(suminc_ctx* c, const uint32_t* cid);
+template void pull_scatter<float>(suminc_ctx* c, const uint32_t* cid, const uint8_t* chk, int gate);
+template void pull_scatter<double>(suminc_ctx* c, const uint32_t* cid, const uint8_t* chk, int gate);
 
 // Enqueues one round; with profiling on, events 4*slot .. 4*slot+3 bracket the
 // count+decide, gather and scatter (push or pull) launches.
@@ -459,7 +546,7 @@ static void enqueue_round(suminc_ctx* c, double eps, double T_stop, int max_roun
   k_pr_push<T><<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
                                                       c->pr_delta.as<T>(), c->wl0.as<uint32_t>(), c->wl1.as<T>(),
                                                       c->heavy.as<uint4>(), cnt);
-  if (div > 0) pull_scatter<T>(c, nullptr);
+  if (div > 0) pull_scatter<T>(c, nullptr, nullptr, 0);
   if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 4 * slot + 3), c->stream));
   k_pr_end<<<1, 1024, 0, c->stream>>>(cnt, reinterpret_cast<unsigned long long*>(c->partial.as<double>()) + 2 * g, g);
 }

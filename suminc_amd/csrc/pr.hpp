@@ -19,11 +19,16 @@ void pr_step(suminc_ctx* c, double* residual);
 void pr_download(suminc_ctx* c, double* rank, double* delta);
 void pr_upload(suminc_ctx* c, const double* rank, const double* delta);
 unsigned push_grid();
-// in-degree classes of the CSC for the pull rounds (rebuilt per graph version)
+// in-degree classes of the CSC for the pull rounds: built on the stream (lengths
+// stay on the device) for the current CSC, sized for up to m_cap edges
+void build_pull_lists(suminc_ctx* c, uint64_t m_cap);
+// the same, once per graph version
 void ensure_pull_heavy(suminc_ctx* c);
-// the pull launches of a PageRank round; cid != nullptr: the sketch rounds' pull
+// the pull launches of a PageRank round (gate 0) or reseed (gate 1); cid != nullptr:
+// the sketch rounds' pull, chk[v] != 0 for the vertices with an internal in-edge
+// from an exit
 template <class T>
-void pull_scatter(suminc_ctx* c, const uint32_t* cid);
+void pull_scatter(suminc_ctx* c, const uint32_t* cid, const uint8_t* chk, int gate);
 
 // sssp.hip
 void sssp_init(suminc_ctx* c, uint32_t source);

@@ -1141,7 +1141,6 @@ struct SkArgs {
   const uint32_t* ent_of;
   uint32_t* wl_v;
   uint4* heavy;
-  uint32_t* sc_e;          // shortcut items: entry index
   Counters* cnt;
   double* partial;
   const uint32_t* cid;
@@ -1266,6 +1265,7 @@ __global__ void __launch_bounds__(kBlock) k_sk_gather(SkArgs a, T* __restrict__ 
         want_sc = true;
         sval = dv;
       }
+      sc_x[a.ent_of[v]] = sval;  // every entry's slot is written each round (0: idle)
     }
     if (t == 1 || t == 3) {  // exit: inflow becomes rank, d/outdeg of it leaves the cluster
       T b = bound[v];
@@ -1278,11 +1278,6 @@ __global__ void __launch_bounds__(kBlock) k_sk_gather(SkArgs a, T* __restrict__ 
   }
   if (pull && v < a.n) contrib[v] = want_push ? cval : T(0);
   push_item<T>(want_push && !pull, v, cval, rowlen, a.wl_v, wl_c, a.heavy, a.cnt);
-  unsigned long long slot = wave_append(want_sc, &a.cnt->wl_count[1]);
-  if (want_sc) {
-    a.sc_e[slot] = a.ent_of[v];
-    sc_x[slot] = sval;
-  }
   // edges of this round: sketch out-edges pushed + shortcut slots used
   __shared__ unsigned long long s_e[kBlock / 64];
   unsigned long long ed = (want_push ? rowlen : 0u) + (want_sc ? a.cl_nexit[a.cid[v]] : 0u);
@@ -1298,9 +1293,11 @@ __global__ void __launch_bounds__(kBlock) k_sk_gather(SkArgs a, T* __restrict__ 
 }
 
 // Shortcut items: bound[x] += R[e][x] * delta for the exits x of the entry's
-// cluster (g_index_function, pagerank_ingress.h:102-125).
+// cluster (g_index_function, pagerank_ingress.h:102-125).  sc_x holds the delta
+// each entry injected this round (0 when idle), indexed by entry: a fixed list
+// instead of an appended one, so the gather needs no same-address atomics.
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_sk_shortcut(const uint32_t* __restrict__ sc_e, const T* __restrict__ sc_x,
+__global__ void __launch_bounds__(kBlock) k_sk_shortcut(uint32_t E, const T* __restrict__ sc_x,
                                                         const Counters* cnt, const uint32_t* __restrict__ cl_ent,
                                                         const uint32_t* __restrict__ cid,
                                                         const uint32_t* __restrict__ cl_off,
@@ -1315,18 +1312,17 @@ __global__ void __launch_bounds__(kBlock) k_sk_shortcut(const uint32_t* __restri
   __shared__ unsigned long long s_rbase[kBlock];
   __shared__ T s_x[kBlock];
   if (cnt->done) return;
-  const unsigned long long ni = cnt->wl_count[1];
-  const unsigned long long tiles = (ni + kBlock - 1) / kBlock;
-  for (unsigned long long t = blockIdx.x; t < tiles; t += gridDim.x) {
-    unsigned long long i = t * kBlock + threadIdx.x;
+  const uint32_t tiles = (E + kBlock - 1) / kBlock;
+  for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const uint32_t e = t * kBlock + threadIdx.x;
     uint32_t len = 0;
-    if (i < ni) {
-      uint32_t e = sc_e[i];
+    const T x = e < E ? sc_x[e] : T(0);
+    if (x != T(0)) {
       uint32_t k = cid[cl_ent[e]];
       len = cl_nexit[k];
       s_mbase[threadIdx.x] = cl_off[k];
       s_rbase[threadIdx.x] = sc_base[e];
-      s_x[threadIdx.x] = sc_x[i];
+      s_x[threadIdx.x] = x;
     }
     block_adjacency<kBlock>(0, len, s_scan, s_start, s_warp, [&](int slot, uint32_t j, uint32_t) {
       T w = R[s_rbase[slot] + j];
@@ -1456,6 +1452,23 @@ __global__ void __launch_bounds__(kBlock) k_sk_correct(const uint32_t* __restric
   }
 }
 
+// chk[v] = 1 iff an internal edge of v's cluster runs from one of its exits
+// (local index < nexit) to v.  Only such in-edges carry a contribution the
+// sketch pull must skip; every other vertex sums all its in-edges unchecked.
+// chk is all zero on entry (the caller clears it).
+__global__ void __launch_bounds__(kBlock) k_pull_marks(uint32_t K, const uint32_t* __restrict__ ie_off,
+                                                       const uint32_t* __restrict__ ie_src,
+                                                       const uint32_t* __restrict__ ie_dst,
+                                                       const uint32_t* __restrict__ cl_off,
+                                                       const uint32_t* __restrict__ cl_mem,
+                                                       const uint32_t* __restrict__ cl_nexit, uint8_t* __restrict__ chk) {
+  const uint32_t k = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);  // one wave per cluster
+  if (k >= K) return;
+  const uint32_t nx = cl_nexit[k], base = cl_off[k];
+  for (uint32_t i = ie_off[k] + lane_id(); i < ie_off[k + 1]; i += 64)
+    if (ie_src[i] < nx) chk[cl_mem[base + ie_dst[i]]] = 1;
+}
+
 template <class T>
 static void sk_push_pass(suminc_ctx* c) {
   Sketch& s = c->sk;
@@ -1482,8 +1495,15 @@ static void sum_pr_run_t(suminc_ctx* c, suminc_stats* st) {
   s.spnode.ensure((size_t)n * es);
   SUMINC_HIP(hipMemsetAsync(s.bound.p, 0, (size_t)n * es, c->stream));
   SUMINC_HIP(hipMemsetAsync(s.spnode.p, 0, (size_t)n * es, c->stream));
-  c->tmp2.ensure((size_t)n * 4 + 4);
   c->tmp3.ensure((size_t)n * es + 8);
+  if (div > 0) {
+    s.pull_chk.ensure((size_t)n);
+    SUMINC_HIP(hipMemsetAsync(s.pull_chk.p, 0, (size_t)n, c->stream));
+    if (s.K)
+      k_pull_marks<<<grid_for(s.K, kBlock / 64), kBlock, 0, c->stream>>>(
+          s.K, s.ie_off.as<uint32_t>(), s.ie_src.as<uint32_t>(), s.ie_dst.as<uint32_t>(), s.cl_off.as<uint32_t>(),
+          s.cl_mem.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), s.pull_chk.as<uint8_t>());
+  }
   SkArgs a;
   a.n = n;
   a.off = c->out_off.as<uint32_t>();
@@ -1491,7 +1511,6 @@ static void sum_pr_run_t(suminc_ctx* c, suminc_stats* st) {
   a.ent_of = s.ent_of.as<uint32_t>();
   a.wl_v = c->wl0.as<uint32_t>();
   a.heavy = c->heavy.as<uint4>();
-  a.sc_e = c->tmp2.as<uint32_t>();
   a.cnt = cnt;
   a.partial = c->partial.as<double>();
   a.cid = s.cid.as<uint32_t>();
@@ -1523,9 +1542,9 @@ static void sum_pr_run_t(suminc_ctx* c, suminc_stats* st) {
                                                   c->pr_rank.as<T>(), c->pr_comp.as<float>(), c->wl1.as<T>(),
                                                   c->tmp3.as<T>(), c->pr_contrib.as<T>(), eps, d, 0);
       sk_push_pass<T>(c);
-      if (div > 0) pull_scatter<T>(c, s.cid.as<uint32_t>());
-      k_sk_shortcut<T><<<push_grid(), kBlock, 0, c->stream>>>(
-          c->tmp2.as<uint32_t>(), c->tmp3.as<T>(), cnt, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(),
+      if (div > 0) pull_scatter<T>(c, s.cid.as<uint32_t>(), s.pull_chk.as<uint8_t>(), 0);
+      k_sk_shortcut<T><<<grid_for(s.E, kBlock, push_grid()), kBlock, 0, c->stream>>>(
+          s.E, c->tmp3.as<T>(), cnt, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(),
           s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.cl_nexit.as<uint32_t>(),
           s.sc_base.as<unsigned long long>(), s.sc_R.as<T>(), s.bound.as<T>());
       k_sk_end<<<1, 1024, 0, c->stream>>>(cnt, reinterpret_cast<unsigned long long*>(a.partial) + g, g);
