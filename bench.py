@@ -53,6 +53,11 @@ WORKLOADS = {
                  desc="incremental FP32/Kahan delta-PageRank, R-MAT s24 ef16 seed3, d=0.85f, 1% batches"),
     "pr22": dict(kind="pr", scale=22, seed=3, frac=0.01, T=1e-7, max_node=1024,
                  desc="incremental FP32/Kahan delta-PageRank, R-MAT s22 ef16 seed3, d=0.85f, 1% batches"),
+    "web26": dict(kind="web", scale=26, seed=5, wmin=1, wmax=255, frac=0.01, T=1e-7, source=0, max_node=1024,
+                  adjoint_seeds=1024,
+                  desc="web-like incremental run, R-MAT s26 ef16 seed5, successive 1% batches; per batch "
+                       "FP32/Kahan delta-PageRank (plain CSR) + SSSP over supernodes (u32 w~U[1,255], source 0, "
+                       "clusters <= 1024 members); weighted-PageRank forward + adjoint on the final graph"),
 }
 
 
@@ -329,6 +334,161 @@ def run_pr(args, wl):
     return gteps, ms_per_step, "f32", roof, cpu, extra
 
 
+def run_web(args, wl):
+    """Successive batches on one large graph: each step applies one batch (CSR/CSC
+    merge, PageRank reseed, SSSP repair, sketch bookkeeping) and brings both the
+    PageRank and the SSSP state back to their fixed points.  Checks on the final
+    graph: SSSP equals a full plain recompute on the same GPU bit for bit, PageRank
+    is compared with an FP64 run to 1e-12 on the same GPU (L1, Linf), and the
+    weighted-PageRank forward + adjoint run once."""
+    import oracle
+    import suminc_amd
+    import suminc_amd._lib as L
+
+    ctx = suminc_amd.Context(termcheck_threshold=wl["T"], pr_precision=suminc_amd.PR_FP32_KAHAN,
+                             max_rounds=1_000_000)
+    t0 = time.time()
+    ctx.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"], weights="u32", wmin=wl["wmin"],
+                      wmax=wl["wmax"])
+    n, m, _ = ctx.graph_info()
+    log(f"graph: n={n} m={m} ({time.time() - t0:.1f}s)")
+    t0 = time.time()
+    ctx.cluster_leaves(wl["max_node"], 2)
+    info = ctx.sum_build(2, wl["max_node"], 1.0)
+    log(f"sketch: {info} ({time.time() - t0:.1f}s)")
+    extra = {"sketch": {k: info[k] for k in ("clusters", "clustered_vertices", "entries", "sketch_edges",
+                                             "internal_edges", "shortcut_slots", "max_cluster")}}
+    ctx.pr_init()
+    t0 = time.time()
+    st = ctx.pr_run()
+    log(f"initial PR: rounds={st['rounds']:.0f} edges={st['edges_traversed']:.3e} ({time.time() - t0:.1f}s)")
+    ctx.sssp_init(wl["source"])
+    st = ctx.sum_sssp_run()
+    log(f"initial SSSP: rounds={st['rounds']:.0f} edges={st['edges_traversed']:.3e}")
+    n_ops = int(wl["frac"] * m)
+    nb = args.warmup + args.steps
+    dev = []
+    for b in range(nb):  # inputs resident in HBM before the timed region
+        ops = ctx.generate_batch(wl["seed"], b, n_ops // 2, n_ops - n_ops // 2, wl["wmin"], wl["wmax"])
+        dev.append(ctx.upload_batch(ops))
+    del ops
+    for b in range(args.warmup):
+        ctx.apply_batch_device(dev[b])
+        ctx.pr_run()
+        ctx.sum_sssp_run()
+    ctx.sync()
+    ctx.profile_enable(True)
+    per = []
+    edges = 0.0
+    t_all = time.perf_counter()
+    for b in range(args.warmup, nb):
+        t0 = time.perf_counter()
+        sa = ctx.apply_batch_device(dev[b])
+        ctx.sync()
+        t1 = time.perf_counter()
+        sp = ctx.pr_run()
+        ctx.sync()
+        t2 = time.perf_counter()
+        ss = ctx.sum_sssp_run()
+        ctx.sync()
+        t3 = time.perf_counter()
+        e = sp["edges_traversed"] + sa["reserved"][3] + ss["edges_traversed"]
+        edges += e
+        per.append({"batch": b, "ms": round(1e3 * (t3 - t0), 3), "apply_ms": round(1e3 * (t1 - t0), 3),
+                    "pr_ms": round(1e3 * (t2 - t1), 3), "sssp_ms": round(1e3 * (t3 - t2), 3),
+                    "pr_rounds": sp["rounds"], "sssp_rounds": ss["rounds"], "edges": e,
+                    "GTEPS": round(e / (t3 - t0) / 1e9, 3)})
+        log(f"batch {b}: {per[-1]}")
+    dt = time.perf_counter() - t_all
+    kids = [L.KERNEL_PR_PUSH, L.KERNEL_PR_PULL, L.KERNEL_PR_GATHER, L.KERNEL_SSSP_SKETCH, L.KERNEL_BATCH_COPY]
+    kernel, roof = roofline(ctx, kids)
+    extra["kernels"] = kernel_table(ctx, kids)
+    ctx.profile_enable(False)
+    for d in dev:
+        d.free()
+    ms_per_step = 1e3 * dt / args.steps
+    gteps = edges / dt / 1e9
+    log(f"incremental: {ms_per_step:.3f} ms/step, {edges / args.steps:.3e} edges/step, {gteps:.3f} GTEPS")
+    extra["batches"] = per
+
+    # the incremental states, then full recomputes on the same GPU and final graph
+    rank, _ = ctx.pr_download()
+    dist, par = ctx.sssp_download()
+    t0 = time.perf_counter()
+    ctx.pr_init()
+    ctx.pr_run()
+    ctx.sync()
+    full_pr = 1e3 * (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    ctx.sssp_init(wl["source"])
+    ctx.sssp_run()
+    ctx.sync()
+    full_ss = 1e3 * (time.perf_counter() - t0)
+    d2, p2 = ctx.sssp_download()
+    extra["sssp_equals_full_recompute"] = bool(np.array_equal(dist, d2) and np.array_equal(par, p2))
+    del d2, p2
+    mean = lambda k: float(np.mean([x[k] for x in per]))  # noqa: E731
+    extra.update({"full_recompute_ms": {"pagerank": round(full_pr, 3), "sssp": round(full_ss, 3)},
+                  "incremental_ms": {"pagerank": round(mean("pr_ms") + mean("apply_ms"), 3),
+                                     "sssp": round(mean("sssp_ms"), 3)},
+                  "speedup_vs_full_recompute": {
+                      "pagerank": round(full_pr / (mean("pr_ms") + mean("apply_ms")), 2),
+                      "sssp": round(full_ss / mean("sssp_ms"), 2),
+                      "step": round((full_pr + full_ss) / ms_per_step, 2)},
+                  "edges_per_step": edges / args.steps, "termcheck_threshold": wl["T"]})
+    log(f"full recompute: pagerank {full_pr:.1f} ms, sssp {full_ss:.1f} ms; sssp bit-exact vs full: "
+        f"{extra['sssp_equals_full_recompute']}")
+    if not args.no_verify:
+        ctx.set_params(pr_precision=suminc_amd.PR_FP64, termcheck_threshold=1e-12)
+        ctx.pr_init()
+        st = ctx.pr_run()
+        r64, _ = ctx.pr_download()
+        extra["l1_vs_fp64"] = float(np.abs(rank - r64).sum())
+        extra["linf_vs_fp64"] = float(np.abs(rank - r64).max())
+        extra["fp64_reference"] = f"plain FP64 delta-PageRank on the same GPU to sum|delta| < 1e-12 " \
+                                  f"({st['rounds']:.0f} rounds)"
+        log(f"L1 vs FP64 = {extra['l1_vs_fp64']:.3e}, Linf {extra['linf_vs_fp64']:.3e}")
+        del r64
+        ctx.set_params(pr_precision=suminc_amd.PR_FP32_KAHAN, termcheck_threshold=wl["T"])
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline: downloading the final graph")
+        off, col, _ = ctx.download_csr()
+        g = oracle.Graph.from_csr(off, col)
+        del off, col
+        cores = min(host_cores(), 16)
+        oracle.set_threads(cores)
+        t1 = time.perf_counter()
+        _, _, sto = g.pr_ingress(DAMPING, wl["T"], max_rounds=3)
+        tc = time.perf_counter() - t1
+        cpu = {"value": round(sto["edges"] / tc / 1e9, 4), "unit": "GTEPS", "cores": cores, "kind": "port",
+               "sample": f"oracle delta-PageRank Ingress (reference CPU loop restated, FP64) from scratch on "
+                         f"the final s{wl['scale']} graph, first {sto['rounds']:.0f} rounds: "
+                         f"{sto['edges']:.3e} edges in {tc:.2f}s"}
+        log(f"cpu baseline: {cpu}")
+    ctx.close()
+    # weighted personalized PageRank + adjoint on the same R-MAT graph with FP32
+    # weights from the generator (a context of its own: the SSSP graph's weights
+    # are integers), Dirichlet(1) personalization over random seeds, c ~ N(0,1)
+    with suminc_amd.Context(termcheck_threshold=wl["T"], pr_precision=suminc_amd.PR_FP32_KAHAN,
+                            max_rounds=1_000_000) as cw:
+        cw.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"], weights="f32")
+        rng = np.random.default_rng(wl["seed"])
+        pvec = np.zeros(n, np.float32)
+        seeds = rng.choice(n, wl["adjoint_seeds"], replace=False)
+        pvec[seeds] = rng.dirichlet(np.ones(wl["adjoint_seeds"])).astype(np.float32)
+        cvec = rng.standard_normal(n, dtype=np.float32)
+        _, sf = cw.wpr_forward(pvec)
+        _, gw, sb = cw.wpr_backward(cvec)
+        mw = cw.graph_info()[1]
+        extra["adjoint"] = {"edges": mw, "forward_ms": round(sf["kernel_ms"], 3), "forward_rounds": sf["rounds"],
+                            "backward_ms": round(sb["kernel_ms"], 3), "backward_rounds": sb["rounds"],
+                            "grad_w_finite": bool(np.isfinite(gw).all())}
+        log(f"adjoint: {extra['adjoint']}")
+        del gw
+    return gteps, ms_per_step, "f32+u32", roof, cpu, extra
+
+
 def calibrate_copy(nbytes):
     """One device-to-device copy of nbytes (the runtime's copy kernel), so that the
     FETCH_SIZE / WRITE_SIZE counters can be read against a known byte count."""
@@ -374,6 +534,8 @@ def main():
     wl = WORKLOADS[args.workload]
     if wl["kind"] == "sssp":
         value, ms, dtype, roof, cpu, extra = run_sssp(args, wl)
+    elif wl["kind"] == "web":
+        value, ms, dtype, roof, cpu, extra = run_web(args, wl)
     else:
         value, ms, dtype, roof, cpu, extra = run_pr(args, wl)
     out = {
@@ -391,7 +553,9 @@ def main():
         "data": "synthetic (seeded R-MAT + seeded update batches, generated on the device)",
         "config": {"workload": wl["desc"], "scale": wl["scale"], "edge_factor": 16, "batch_fraction": wl["frac"],
                    "supernodes": (("on (leaf clusters <= %d members, compress threshold 1)" % wl["max_node"])
-                                  if args.supernodes == "on" and wl.get("max_node") else "off (plain CSR path)")},
+                                  if args.supernodes == "on" and wl.get("max_node") else "off (plain CSR path)")
+                   if wl["kind"] != "web" else "SSSP over supernodes (leaf clusters <= %d members), PageRank on "
+                                               "the plain CSR" % wl["max_node"]},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -715,7 +715,7 @@ __global__ void k_heavy_clear(Counters* cnt) {
 // Seeds SSSP list 0 with every finite reset vertex and clears the reset bitmap
 // (the next repair starts from a clear one).
 __global__ void k_reset_seed(const uint32_t* rlist, uint32_t* rbits, const unsigned long long* dp, uint32_t* wl0,
-                             uint32_t* bits0, Counters* cnt) {
+                             uint32_t* bits0, const Counters* cnt, unsigned long long* seeds) {
   unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   bool add = false;
   uint32_t x = 0;
@@ -727,14 +727,14 @@ __global__ void k_reset_seed(const uint32_t* rlist, uint32_t* rbits, const unsig
       add = !(atomicOr(&bits0[x >> 5], bit) & bit);
     }
   }
-  unsigned long long slot = wave_append(add, &cnt->wl_count[0]);
+  unsigned long long slot = wave_append(add, seeds);
   if (add) wl0[slot] = x;
 }
 
 // relax every add (new pair or new weight) from its current source distance
 __global__ void k_insert_relax(const uint32_t* grow, const uint32_t* gcol, const uint8_t* gact, const uint32_t* gw,
                                const unsigned* bc, uint32_t N, int unit, unsigned long long* dp, uint32_t* wl0,
-                               uint32_t* bits0, Counters* cnt) {
+                               uint32_t* bits0, unsigned long long* seeds) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   bool add = false;
   uint32_t v = 0;
@@ -752,7 +752,7 @@ __global__ void k_insert_relax(const uint32_t* grow, const uint32_t* gcol, const
       }
     }
   }
-  unsigned long long slot = wave_append(add, &cnt->wl_count[0]);
+  unsigned long long slot = wave_append(add, seeds);
   if (add) wl0[slot] = v;
 }
 
@@ -1113,12 +1113,14 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
     k_heavy_clear<<<1, 64, 0, c->stream>>>(cnt);
     k_reset_seed<<<grid_for(n), kBlock, 0, c->stream>>>(b.rlist.as<uint32_t>(), b.rbits.as<uint32_t>(),
                                                         c->sssp_dp.as<unsigned long long>(),
-                                                        c->wl0.as<uint32_t>(), c->sssp_flag0.as<uint32_t>(), cnt);
+                                                        c->sssp_wl0.as<uint32_t>(), c->sssp_flag0.as<uint32_t>(), cnt,
+                                                        c->sssp_seeds.as<unsigned long long>());
     k_insert_relax<<<grid_for(N), kBlock, 0, c->stream>>>(b.g_row.as<uint32_t>(), b.g_col.as<uint32_t>(),
                                                           b.g_act.as<uint8_t>(), b.g_w.as<uint32_t>(), bc, N,
                                                           c->wkind != SUMINC_W_U32,
-                                                          c->sssp_dp.as<unsigned long long>(), c->wl0.as<uint32_t>(),
-                                                          c->sssp_flag0.as<uint32_t>(), cnt);
+                                                          c->sssp_dp.as<unsigned long long>(),
+                                                          c->sssp_wl0.as<uint32_t>(), c->sssp_flag0.as<uint32_t>(),
+                                                          c->sssp_seeds.as<unsigned long long>());
   }
   k_batch_end<<<1, 64, 0, c->stream>>>(cnt, bc, sssp_repair ? 1 : 0);
   SUMINC_HIP(hipGetLastError());

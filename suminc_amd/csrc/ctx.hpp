@@ -195,6 +195,10 @@ struct suminc_ctx {
   bool sssp_pending = false;  // initialised, not yet run
   uint32_t sssp_src = 0;
   suminc::DevBuf sssp_dp, sssp_flag0, sssp_flag1;
+  // frontier lists 0/1 and the length of list 0's seeds (set by init, appended by
+  // batches, taken by the next query): owned by SSSP, so that PageRank and the
+  // weighted PageRank, which reset the shared counters, leave pending seeds intact
+  suminc::DevBuf sssp_wl0, sssp_wl1, sssp_seeds;
 
   // rounds enqueued before the first host check (last run's rounds + 1)
   int hint_reset = 6, hint_sssp = 8, hint_sssp_sketch = 8;

@@ -611,8 +611,8 @@ static SsArgs ss_args(suminc_ctx* c) {
   a.ss_base = s.ss_base.as<unsigned long long>();
   a.sd = s.ss_dist.as<uint32_t>();
   a.dp = c->sssp_dp.as<unsigned long long>();
-  a.wl[0] = c->wl0.as<uint32_t>();
-  a.wl[1] = c->tmp5.as<uint32_t>();
+  a.wl[0] = c->sssp_wl0.as<uint32_t>();
+  a.wl[1] = c->sssp_wl1.as<uint32_t>();
   a.bits[0] = c->sssp_flag0.as<uint32_t>();
   a.bits[1] = c->sssp_flag1.as<uint32_t>();
   a.heavy = c->heavy.as<uint2>();
@@ -670,9 +670,11 @@ static void compute_sssp_rows(suminc_ctx* c, bool zeroed) {
   s.ss_need = false;
 }
 
-__global__ void k_ss_begin(Counters* cnt, unsigned long long* lists) {
+__global__ void k_ss_begin(Counters* cnt, unsigned long long* lists, unsigned long long* seeds) {
   if (threadIdx.x < 8) lists[threadIdx.x] = 0;  // work slot and the three list-count pairs
   if (threadIdx.x) return;
+  cnt->wl_count[0] = *seeds;  // list 0 holds the pending seeds
+  *seeds = 0;
   cnt->aux[8] = 0;  // internal edges read by the in-cluster kernels
   cnt->wl_count[1] = 0;
   cnt->heavy_count = 0;
@@ -691,7 +693,8 @@ void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
   sketch_refresh(c);
   Sketch& s = c->sk;
   s.ss_work.ensure(64);
-  k_ss_begin<<<1, 64, 0, c->stream>>>(dev_counters(c), s.ss_work.as<unsigned long long>());
+  k_ss_begin<<<1, 64, 0, c->stream>>>(dev_counters(c), s.ss_work.as<unsigned long long>(),
+                                      c->sssp_seeds.as<unsigned long long>());
   compute_sssp_rows(c, true);
   if (!s.ss_dirty.p || s.ss_dirty.cap < (size_t)s.K + 1 || !s.ss_dirty_clear) {
     s.ss_dirty.ensure((size_t)s.K + 1);
@@ -761,7 +764,7 @@ void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
   SUMINC_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
   const unsigned long long wk = c->host_counters->aux[8];
   if (cur != 0) {  // keep list/bitmap 0 as the current (empty) frontier
-    c->wl0.swap(c->tmp5);
+    c->sssp_wl0.swap(c->sssp_wl1);
     c->sssp_flag0.swap(c->sssp_flag1);
   }
   suminc_stats out{};

@@ -130,17 +130,25 @@ __global__ void k_sssp_init(uint32_t n, unsigned long long* dp, uint32_t src) {
   dp[v] = (v == src) ? (unsigned long long)src : ((unsigned long long)INF32 << 32) | v;
 }
 
-__global__ void k_sssp_seed(Counters* cnt, uint32_t* wl0, uint32_t* bits0, uint32_t src) {
+__global__ void k_sssp_seed(unsigned long long* seeds, uint32_t* wl0, uint32_t* bits0, uint32_t src) {
   if (threadIdx.x) return;
-  cnt->wl_count[0] = 1;
+  *seeds = 1;
+  wl0[0] = src;
+  bits0[src >> 5] = 1u << (src & 31);
+}
+
+// Start of a query: list 0 holds the pending seeds; the round state starts clear.
+__global__ void k_sssp_begin(Counters* cnt, unsigned long long* seeds) {
+  if (threadIdx.x) return;
+  cnt->wl_count[0] = *seeds;
+  *seeds = 0;
   cnt->wl_count[1] = 0;
   cnt->heavy_count = 0;
   cnt->aux[1] = 0;
+  cnt->aux[6] = 0;
   cnt->done = 0;
   cnt->rounds = 0;
   cnt->edges = 0;
-  wl0[0] = src;
-  bits0[src >> 5] = 1u << (src & 31);
 }
 
 __global__ void k_sssp_unpack(uint32_t n, const unsigned long long* dp, uint32_t* dist, uint32_t* parent) {
@@ -157,8 +165,8 @@ SsspArgs sssp_args(suminc_ctx* c) {
   a.col = c->out_col.as<uint32_t>();
   a.w = c->wkind == SUMINC_W_U32 ? c->out_w.as<uint32_t>() : nullptr;
   a.dp = c->sssp_dp.as<unsigned long long>();
-  a.wl[0] = c->wl0.as<uint32_t>();
-  a.wl[1] = c->tmp5.as<uint32_t>();
+  a.wl[0] = c->sssp_wl0.as<uint32_t>();
+  a.wl[1] = c->sssp_wl1.as<uint32_t>();
   a.bits[0] = c->sssp_flag0.as<uint32_t>();
   a.bits[1] = c->sssp_flag1.as<uint32_t>();
   a.heavy = c->heavy.as<uint2>();
@@ -172,8 +180,9 @@ void sssp_alloc(suminc_ctx* c) {
   const size_t words = (n + 31) / 32 + 1;
   c->sssp_flag0.ensure(words * 4);
   c->sssp_flag1.ensure(words * 4);
-  c->wl0.ensure(n * 4 + 4);
-  c->tmp5.ensure(n * 4 + 4);
+  c->sssp_wl0.ensure(n * 4 + 4);
+  c->sssp_wl1.ensure(n * 4 + 4);
+  c->sssp_seeds.ensure(8);
   c->heavy.ensure_slack((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
 }
 
@@ -190,8 +199,8 @@ void sssp_init(suminc_ctx* c, uint32_t source) {
   c->sssp_src = source;
   k_sssp_init<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->sssp_dp.as<unsigned long long>(), source);
   sssp_clear_flags(c);
-  k_sssp_seed<<<1, 64, 0, c->stream>>>(dev_counters(c), c->wl0.as<uint32_t>(), c->sssp_flag0.as<uint32_t>(),
-                                       source);
+  k_sssp_seed<<<1, 64, 0, c->stream>>>(c->sssp_seeds.as<unsigned long long>(), c->sssp_wl0.as<uint32_t>(),
+                                       c->sssp_flag0.as<uint32_t>(), source);
   SUMINC_HIP(hipGetLastError());
   c->sssp_ready = true;
   c->sssp_pending = true;
@@ -205,6 +214,7 @@ void sssp_run(suminc_ctx* c, suminc_stats* st) {
   SsspArgs a = sssp_args(c);
   c->sssp_pending = false;
   SUMINC_HIP(hipEventRecord(c->ev_start, c->stream));
+  k_sssp_begin<<<1, 64, 0, c->stream>>>(a.cnt, c->sssp_seeds.as<unsigned long long>());
   int group = c->hint_sssp;  // rounds of the previous query + 1 first
   size_t slot = 0;
   while (true) {
@@ -249,7 +259,7 @@ void sssp_run(suminc_ctx* c, suminc_stats* st) {
                  "round cap reached with a non-empty frontier");
   if (cur != 0) {
     // swap buffers so that list/bitmap 0 is current
-    c->wl0.swap(c->tmp5);
+    c->sssp_wl0.swap(c->sssp_wl1);
     c->sssp_flag0.swap(c->sssp_flag1);
   }
 }

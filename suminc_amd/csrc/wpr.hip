@@ -377,12 +377,16 @@ static void wpr_prepare_graph(suminc_ctx* c, WprState& S) {
     k_wpr_edge_keys<<<grid_for(n, kBlock / 64), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), n, m,
                                                                        c->out_col.as<uint32_t>(),
                                                                        ka.as<unsigned long long>(), ia.as<uint32_t>());
+    // keys (dst << 32 | src): the bits above 32 + log2(n) are zero
+    int hi_bits = 1;
+    while (hi_bits < 32 && (1ull << hi_bits) < n) ++hi_bits;
+    const unsigned end_bit = 32 + hi_bits;
     size_t bytes = 0;
     SUMINC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, ka.as<uint64_t>(), kb.as<uint64_t>(), ia.as<uint32_t>(),
-                                         S.eid.as<uint32_t>(), (size_t)m, 0, 64, c->stream));
+                                         S.eid.as<uint32_t>(), (size_t)m, 0, end_bit, c->stream));
     c->tmp.ensure(bytes);
     SUMINC_HIP(rocprim::radix_sort_pairs(c->tmp.p, bytes, ka.as<uint64_t>(), kb.as<uint64_t>(), ia.as<uint32_t>(),
-                                         S.eid.as<uint32_t>(), (size_t)m, 0, 64, c->stream));
+                                         S.eid.as<uint32_t>(), (size_t)m, 0, end_bit, c->stream));
     SUMINC_HIP(hipStreamSynchronize(c->stream));
   }
   S.graph_version = c->graph_version;
