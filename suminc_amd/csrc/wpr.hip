@@ -223,16 +223,21 @@ __global__ void k_wpr_finish(uint32_t n, const T* __restrict__ x, T* __restrict_
   out64[v] = val;
 }
 
+// Wave-per-row kernels stride over the rows with at most this many workgroups (a
+// grid of one wave per row would pass 2^32 threads at 2^26 rows).
+constexpr unsigned kWaveGrid = 65536;
+
 // ---- weights: row sums and the per-edge coefficients of both directions ---------
 __global__ void __launch_bounds__(kBlock) k_wpr_rowsum(uint32_t n, const uint32_t* __restrict__ off,
                                                        const float* __restrict__ w, double* __restrict__ s) {
-  const uint32_t v = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  if (v >= n) return;  // whole wave
-  double acc = 0;
-  for (uint32_t e = off[v] + lane_id(); e < off[v + 1]; e += 64) acc += (double)w[e];
+  const uint32_t waves = gridDim.x * (kBlock / 64);
+  for (uint32_t v = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); v < n; v += waves) {  // whole waves
+    double acc = 0;
+    for (uint32_t e = off[v] + lane_id(); e < off[v + 1]; e += 64) acc += (double)w[e];
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-  if (lane_id() == 0) s[v] = acc;
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane_id() == 0) s[v] = acc;
+  }
 }
 
 // a_e = d w_e / s_src for the out-CSR (edge-parallel over rows via the tile rows)
@@ -240,10 +245,11 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) k_wpr_coef_csr(uint32_t n, const uint32_t* __restrict__ off,
                                                          const float* __restrict__ w, const double* __restrict__ s,
                                                          double d, T* __restrict__ a) {
-  const uint32_t v = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  if (v >= n) return;
-  const double inv = s[v] > 0 ? d / s[v] : 0.0;
-  for (uint32_t e = off[v] + lane_id(); e < off[v + 1]; e += 64) a[e] = (T)((double)w[e] * inv);
+  const uint32_t waves = gridDim.x * (kBlock / 64);
+  for (uint32_t v = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); v < n; v += waves) {
+    const double inv = s[v] > 0 ? d / s[v] : 0.0;
+    for (uint32_t e = off[v] + lane_id(); e < off[v + 1]; e += 64) a[e] = (T)((double)w[e] * inv);
+  }
 }
 
 // the in-CSR copy: a_csc[i] = a_csr[eid[i]]
@@ -263,11 +269,12 @@ __global__ void k_wpr_gather_f(uint64_t m, const uint32_t* __restrict__ eid, con
 // CSR edge id of every CSC position: sort (dst, src) keys with the edge index
 __global__ void k_wpr_edge_keys(const uint32_t* __restrict__ off, uint32_t n, uint64_t m,
                                 const uint32_t* __restrict__ col, unsigned long long* keys, uint32_t* idx) {
-  const uint32_t v = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  if (v >= n) return;
-  for (uint32_t e = off[v] + lane_id(); e < off[v + 1]; e += 64) {
-    keys[e] = ((unsigned long long)col[e] << 32) | v;
-    idx[e] = e;
+  const uint32_t waves = gridDim.x * (kBlock / 64);
+  for (uint32_t v = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); v < n; v += waves) {
+    for (uint32_t e = off[v] + lane_id(); e < off[v + 1]; e += 64) {
+      keys[e] = ((unsigned long long)col[e] << 32) | v;
+      idx[e] = e;
+    }
   }
   (void)m;
 }
@@ -280,14 +287,15 @@ __global__ void __launch_bounds__(kBlock) k_wpr_grad_w(uint32_t n, const uint32_
                                                        const double* __restrict__ r, const double* __restrict__ lam,
                                                        const double* __restrict__ yl, double d,
                                                        float* __restrict__ g32, double* __restrict__ g64) {
-  const uint32_t u = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  if (u >= n) return;
-  const double ru = s[u] > 0 ? r[u] / s[u] : 0.0;  // a row without weight mass has no gradient
-  const double sub = ru * yl[u];
-  for (uint32_t e = off[u] + lane_id(); e < off[u + 1]; e += 64) {
-    const double g = d * ru * lam[col[e]] - sub;
-    if (g32) g32[e] = (float)g;
-    if (g64) g64[e] = g;
+  const uint32_t waves = gridDim.x * (kBlock / 64);
+  for (uint32_t u = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); u < n; u += waves) {
+    const double ru = s[u] > 0 ? r[u] / s[u] : 0.0;  // a row without weight mass has no gradient
+    const double sub = ru * yl[u];
+    for (uint32_t e = off[u] + lane_id(); e < off[u + 1]; e += 64) {
+      const double g = d * ru * lam[col[e]] - sub;
+      if (g32) g32[e] = (float)g;
+      if (g64) g64[e] = g;
+    }
   }
 }
 
@@ -374,7 +382,7 @@ static void wpr_prepare_graph(suminc_ctx* c, WprState& S) {
   if (m) {
     DevBuf ka, kb, ia;
     ka.ensure(m * 8); kb.ensure(m * 8); ia.ensure(m * 4);
-    k_wpr_edge_keys<<<grid_for(n, kBlock / 64), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), n, m,
+    k_wpr_edge_keys<<<grid_for(n, kBlock / 64, kWaveGrid), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), n, m,
                                                                        c->out_col.as<uint32_t>(),
                                                                        ka.as<unsigned long long>(), ia.as<uint32_t>());
     // keys (dst << 32 | src): the bits above 32 + log2(n) are zero
@@ -403,7 +411,7 @@ static void wpr_prepare_coef(suminc_ctx* c, WprState& S) {
   S.s.ensure((size_t)n * 8 + 8);
   S.a_out.ensure(m * sizeof(T) + 8);
   S.a_in.ensure(m * sizeof(T) + 8);
-  const unsigned gw = grid_for(n, kBlock / 64);
+  const unsigned gw = grid_for(n, kBlock / 64, kWaveGrid);
   k_wpr_rowsum<<<gw, kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), c->out_w.as<float>(), S.s.as<double>());
   k_wpr_coef_csr<T><<<gw, kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), c->out_w.as<float>(),
                                                   S.s.as<double>(), c->prm.damping, S.a_out.as<T>());
@@ -566,7 +574,7 @@ static void wpr_backward_t(suminc_ctx* c, const float* cvec, float* grad_p, floa
   gw32.ensure(m * 4 + 4);
   k_wpr_grad_p<<<grid_for(n), kBlock, 0, c->stream>>>(n, S.lam64.as<double>(), c->prm.damping, gp32.as<float>(),
                                                       nullptr);
-  k_wpr_grad_w<<<grid_for(n, kBlock / 64), kBlock, 0, c->stream>>>(
+  k_wpr_grad_w<<<grid_for(n, kBlock / 64, kWaveGrid), kBlock, 0, c->stream>>>(
       n, c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(), S.s.as<double>(), S.r64.as<double>(),
       S.lam64.as<double>(), yd.as<double>(), c->prm.damping, gw32.as<float>(), nullptr);
   SUMINC_HIP(hipGetLastError());
