@@ -53,6 +53,8 @@ WORKLOADS = {
                  desc="incremental FP32/Kahan delta-PageRank, R-MAT s24 ef16 seed3, d=0.85f, 1% batches"),
     "pr22": dict(kind="pr", scale=22, seed=3, frac=0.01, T=1e-7, max_node=1024,
                  desc="incremental FP32/Kahan delta-PageRank, R-MAT s22 ef16 seed3, d=0.85f, 1% batches"),
+    "pr26": dict(kind="pr", scale=26, seed=5, frac=0.01, T=1e-7, max_node=1024,
+                 desc="incremental FP32/Kahan delta-PageRank, R-MAT s26 ef16 seed5, d=0.85f, 1% batches"),
     "web26": dict(kind="web", scale=26, seed=5, wmin=1, wmax=255, frac=0.01, T=1e-7, source=0, max_node=1024,
                   adjoint_seeds=1024,
                   desc="web-like incremental run, R-MAT s26 ef16 seed5, successive 1% batches; per batch "
@@ -254,7 +256,7 @@ def run_pr(args, wl):
     import suminc_amd._lib as L
 
     ctx = suminc_amd.Context(termcheck_threshold=wl["T"], pr_precision=suminc_amd.PR_FP32_KAHAN,
-                             max_rounds=1_000_000)
+                             max_rounds=1_000_000, pull_band_kib=args.pull_band_kib)
     t0 = time.time()
     ctx.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"])
     n, m, _ = ctx.graph_info()
@@ -346,7 +348,7 @@ def run_web(args, wl):
     import suminc_amd._lib as L
 
     ctx = suminc_amd.Context(termcheck_threshold=wl["T"], pr_precision=suminc_amd.PR_FP32_KAHAN,
-                             max_rounds=1_000_000)
+                             max_rounds=1_000_000, pull_band_kib=args.pull_band_kib)
     t0 = time.time()
     ctx.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"], weights="u32", wmin=wl["wmin"],
                       wmax=wl["wmax"])
@@ -471,7 +473,7 @@ def run_web(args, wl):
     # weights from the generator (a context of its own: the SSSP graph's weights
     # are integers), Dirichlet(1) personalization over random seeds, c ~ N(0,1)
     with suminc_amd.Context(termcheck_threshold=wl["T"], pr_precision=suminc_amd.PR_FP32_KAHAN,
-                            max_rounds=1_000_000) as cw:
+                            max_rounds=1_000_000, pull_band_kib=args.pull_band_kib) as cw:
         cw.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"], weights="f32")
         rng = np.random.default_rng(wl["seed"])
         pvec = np.zeros(n, np.float32)
@@ -520,6 +522,8 @@ def main():
     ap.add_argument("--supernodes", default="on", choices=["on", "off"],
                     help="SSSP workload: query over the supernode sketch (on) or the plain CSR (off)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="length of the CPU baseline sample")
+    ap.add_argument("--pull-band-kib", type=int, default=0,
+                    help="PageRank workloads: source band of the banded pull in KiB (0: library default, -1: off)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device (counter calibration for rocprofv3 --pmc) and exit")
     args = ap.parse_args()

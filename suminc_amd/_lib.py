@@ -52,7 +52,8 @@ class Params(C.Structure):
         ("sssp_source", C.c_uint32),
         ("pull_threshold_div", C.c_int32),
         ("shortcut_max_rounds", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("pull_band_kib", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 

@@ -188,6 +188,12 @@ struct suminc_ctx {
   suminc::DevBuf pull_cnt;    // u64 x3: lengths of the short, wave and chunk lists
   suminc::DevBuf pull_parts;  // per-block class counts of the list build
   uint64_t pull_heavy_version = ~0ull;
+  // banded pull (graphs whose contributions outgrow the caches): the wave rows'
+  // in-edges split at every band boundary of the sources, and the band-major
+  // chunks of the heavy rows; band_P bands of band_B source vertices
+  suminc::DevBuf band_split, band_split_s, band_cnt, band_pre, band_chunks, band_off;
+  uint32_t band_P = 1, band_B = 0;
+  uint64_t band_W = 0, band_S = 0;
   uint64_t graph_version = 0;
 
   // SSSP state: packed {dist:hi32, parent:lo32}

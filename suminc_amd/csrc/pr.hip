@@ -23,6 +23,11 @@
 //                > 256 are summed per 2048-edge chunk and added with one atomic.
 //   k_pr_end     list counters reset, statistics.
 // Kernels not selected for the round return at entry.
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+
 #include "ctx.hpp"
 #include "device.hpp"
 #include "pr.hpp"
@@ -478,6 +483,188 @@ void pr_init(suminc_ctx* c) {
   c->pr_ready = true;
 }
 
+// ---------------------------------------------------------------------------
+// Banded pull.  A dense round gathers contrib[u] for every in-edge; once the
+// contributions outgrow the last-level cache each gather costs a DRAM sector.
+// The CSC rows are sorted by source, so a source band [p*B, (p+1)*B) is one
+// sub-range of every row: the wave and heavy rows are pulled band by band, each
+// band's slice of contrib staying cache resident while all workgroups read it.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t row_lower_bound(const uint32_t* __restrict__ icol, uint32_t b, uint32_t e,
+                                                    uint64_t x) {
+  while (b < e) {
+    const uint32_t mid = b + ((e - b) >> 1);
+    if ((uint64_t)icol[mid] < x) b = mid + 1; else e = mid;
+  }
+  return b;
+}
+
+// split[p * W + i]: first in-edge of listed row i with source >= p * B (p = 0..P)
+__global__ void k_band_split(const uint32_t* __restrict__ wave_list, uint32_t W, const uint32_t* __restrict__ ioff,
+                             const uint32_t* __restrict__ icol, uint32_t P, uint32_t B, uint32_t* __restrict__ split) {
+  const uint64_t total = (uint64_t)W * (P + 1);
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t p = (uint32_t)(t / W), i = (uint32_t)(t % W);
+    const uint32_t v = wave_list[i];
+    const uint32_t b = ioff[v], e = ioff[v + 1];
+    split[t] = p == 0 ? b : (p == P ? e : row_lower_bound(icol, b, e, (uint64_t)p * B));
+  }
+}
+
+// Band p of heavy row (chunk entry h with k == 0): its in-edge range and chunks.
+__device__ __forceinline__ uint32_t band_range(const uint2* chunks, uint32_t h, uint32_t p, uint32_t P, uint32_t B,
+                                               const uint32_t* ioff, const uint32_t* icol, uint32_t& v,
+                                               uint32_t& lo, uint32_t& hi) {
+  const uint2 ch = chunks[h];
+  if (ch.y != 0) return 0;
+  v = ch.x;
+  const uint32_t b = ioff[v], e = ioff[v + 1];
+  lo = p == 0 ? b : row_lower_bound(icol, b, e, (uint64_t)p * B);
+  hi = p + 1 == P ? e : row_lower_bound(icol, b, e, (uint64_t)(p + 1) * B);
+  return (hi - lo + kChunk - 1) / kChunk;
+}
+
+__global__ void k_band_heavy_count(const uint2* __restrict__ chunks, uint32_t Hc, const uint32_t* __restrict__ ioff,
+                                   const uint32_t* __restrict__ icol, uint32_t P, uint32_t B,
+                                   uint32_t* __restrict__ cnt) {
+  const uint64_t total = (uint64_t)Hc * P;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t v, lo, hi;
+    cnt[t] = band_range(chunks, (uint32_t)(t % Hc), (uint32_t)(t / Hc), P, B, ioff, icol, v, lo, hi);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) cnt[total] = 0;
+}
+
+// Band-major chunk list {row, first edge, end edge}; off[p] = first chunk of band p.
+__global__ void k_band_heavy_fill(const uint2* __restrict__ chunks, uint32_t Hc, const uint32_t* __restrict__ ioff,
+                                  const uint32_t* __restrict__ icol, uint32_t P, uint32_t B,
+                                  const uint32_t* __restrict__ pre, uint4* __restrict__ out, uint32_t* __restrict__ off) {
+  const uint64_t total = (uint64_t)Hc * P;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t h = (uint32_t)(t % Hc), p = (uint32_t)(t / Hc);
+    if (h == 0) off[p] = pre[t];
+    uint32_t v = 0, lo = 0, hi = 0;
+    const uint32_t nch = band_range(chunks, h, p, P, B, ioff, icol, v, lo, hi);
+    for (uint32_t j = 0; j < nch; ++j)
+      out[pre[t] + j] = make_uint4(v, lo + j * kChunk, min(hi, lo + (j + 1) * kChunk), 0u);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) off[P] = pre[total];
+}
+
+// Band p of the wave rows: tiles of 256 rows, the block walks their band slices
+// with coalesced column reads and sums per row in LDS, one delta update per row.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pr_pull_band_rows(const uint32_t* __restrict__ list, uint32_t W,
+                                                              const uint32_t* __restrict__ split, uint32_t p,
+                                                              const uint32_t* __restrict__ icol,
+                                                              const T* __restrict__ contrib, T* __restrict__ delta,
+                                                              const Counters* cnt, int gate) {
+  __shared__ uint32_t s_scan[kBlock + 1];
+  __shared__ uint32_t s_start[kBlock];
+  __shared__ uint32_t s_warp[kBlock / 64 + 1];
+  __shared__ T s_acc[kBlock];
+  if (!pull_runs(cnt, gate)) return;
+  const uint32_t tiles = (W + kBlock - 1) / kBlock;
+  for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const uint32_t i = t * kBlock + threadIdx.x;
+    uint32_t lo = 0, len = 0, v = 0;
+    if (i < W) {
+      v = list[i];
+      lo = split[(size_t)p * W + i];
+      len = split[(size_t)(p + 1) * W + i] - lo;
+    }
+    s_acc[threadIdx.x] = T(0);  // visible before the edge loop: block_adjacency syncs after its scan
+    block_adjacency<kBlock>(lo, len, s_scan, s_start, s_warp,
+                            [&](int slot, uint32_t, uint32_t e) { atomicAdd(&s_acc[slot], contrib[icol[e]]); });
+    if (i < W && s_acc[threadIdx.x] != T(0)) delta[v] += s_acc[threadIdx.x];
+  }
+}
+
+// Band p of the heavy rows: one chunk (at most 2048 in-edges) per block at a time.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pr_pull_band_heavy(const uint4* __restrict__ chunks,
+                                                               const uint32_t* __restrict__ off, uint32_t p,
+                                                               const uint32_t* __restrict__ icol,
+                                                               const T* __restrict__ contrib, T* __restrict__ delta,
+                                                               const Counters* cnt, int gate) {
+  __shared__ T s_red[kBlock / 64];
+  if (!pull_runs(cnt, gate)) return;
+  const uint32_t h1 = off[p + 1];
+  for (uint32_t h = off[p] + blockIdx.x; h < h1; h += gridDim.x) {
+    const uint4 ch = chunks[h];
+    T acc = T(0);
+    for (uint32_t k = ch.y + threadIdx.x; k < ch.z; k += kBlock) acc += contrib[icol[k]];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane_id() == 0) s_red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      T s = T(0);
+      for (int w = 0; w < kBlock / 64; ++w) s += s_red[w];
+      if (s != T(0)) unsafeAtomicAdd(&delta[ch.x], s);
+    }
+    __syncthreads();
+  }
+}
+
+// Band tables for the lists just built (sizes read back from the device).
+static void build_band_tables(suminc_ctx* c, uint64_t m_cap) {
+  const int kib = c->prm.pull_band_kib;
+  const uint64_t es = c->pr_prec == SUMINC_PR_FP64 ? 8 : 4;
+  c->band_P = 1;
+  if (kib < 0) return;
+  // bands of at most kib KiB of contributions, equal in width (measured on MI355X:
+  // 32-64 MiB bands are fastest; 4 MiB bands lose to the unbanded pull at 2^22
+  // vertices, 128 MiB ones at 2^26)
+  const uint64_t band_bytes = std::max<uint64_t>(4096, (uint64_t)(kib ? kib : 49152) * 1024);
+  const uint64_t P = std::min<uint64_t>(64, ((uint64_t)c->n * es + band_bytes - 1) / band_bytes);
+  if (P <= 1) return;
+  const uint64_t B = ((uint64_t)c->n + P - 1) / P;
+  unsigned long long h[3];
+  SUMINC_HIP(hipMemcpyAsync(h, c->pull_cnt.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  const uint32_t S = (uint32_t)h[0], W = (uint32_t)h[1], Hc = (uint32_t)h[2];
+  c->band_split.ensure_slack(((size_t)W * (P + 1) + 1) * 4);
+  c->band_split_s.ensure_slack(((size_t)S * (P + 1) + 1) * 4);
+  if (W)
+    k_band_split<<<grid_for((uint64_t)W * (P + 1), kBlock, 65536), kBlock, 0, c->stream>>>(
+        c->pull_wave.as<uint32_t>(), W, c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(), (uint32_t)P,
+        (uint32_t)B, c->band_split.as<uint32_t>());
+  if (S)
+    k_band_split<<<grid_for((uint64_t)S * (P + 1), kBlock, 65536), kBlock, 0, c->stream>>>(
+        c->pull_short.as<uint32_t>(), S, c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(), (uint32_t)P,
+        (uint32_t)B, c->band_split_s.as<uint32_t>());
+  const uint64_t nc = (uint64_t)Hc * P;
+  c->band_cnt.ensure_slack((nc + 1) * 4);
+  c->band_pre.ensure_slack((nc + 1) * 4);
+  c->band_off.ensure((P + 1) * 4);
+  // a row of len edges has at most len / kChunk + 1 chunks in each band
+  c->band_chunks.ensure_slack(((size_t)m_cap / kChunk + (size_t)P * Hc + 1) * sizeof(uint4));
+  SUMINC_HIP(hipMemsetAsync(c->band_off.p, 0, (P + 1) * 4, c->stream));
+  const unsigned gc = grid_for(nc + 1, kBlock, 65536);
+  k_band_heavy_count<<<gc, kBlock, 0, c->stream>>>(c->pull_heavy.as<uint2>(), Hc, c->in_off.as<uint32_t>(),
+                                                   c->in_col.as<uint32_t>(), (uint32_t)P, (uint32_t)B,
+                                                   c->band_cnt.as<uint32_t>());
+  size_t bytes = 0;
+  SUMINC_HIP(rocprim::exclusive_scan(nullptr, bytes, c->band_cnt.as<uint32_t>(), c->band_pre.as<uint32_t>(), 0u,
+                                     (size_t)nc + 1, rocprim::plus<uint32_t>(), c->stream));
+  c->tmp.ensure(bytes);
+  SUMINC_HIP(rocprim::exclusive_scan(c->tmp.p, bytes, c->band_cnt.as<uint32_t>(), c->band_pre.as<uint32_t>(), 0u,
+                                     (size_t)nc + 1, rocprim::plus<uint32_t>(), c->stream));
+  k_band_heavy_fill<<<gc, kBlock, 0, c->stream>>>(c->pull_heavy.as<uint2>(), Hc, c->in_off.as<uint32_t>(),
+                                                  c->in_col.as<uint32_t>(), (uint32_t)P, (uint32_t)B,
+                                                  c->band_pre.as<uint32_t>(), c->band_chunks.as<uint4>(),
+                                                  c->band_off.as<uint32_t>());
+  SUMINC_HIP(hipGetLastError());
+  c->band_P = (uint32_t)P;
+  c->band_B = (uint32_t)B;
+  c->band_W = W;
+  c->band_S = S;
+}
+
 void build_pull_lists(suminc_ctx* c, uint64_t m_cap) {
   const unsigned g = grid_for(c->n);
   c->pull_heavy.ensure_slack((m_cap / kChunk + m_cap / kHeavyDeg + 2) * sizeof(uint2));
@@ -491,6 +678,7 @@ void build_pull_lists(suminc_ctx* c, uint64_t m_cap) {
                                            c->pull_short.as<uint32_t>(), c->pull_wave.as<uint32_t>(),
                                            c->pull_heavy.as<uint2>());
   SUMINC_HIP(hipGetLastError());
+  build_band_tables(c, m_cap);
 }
 
 void ensure_pull_heavy(suminc_ctx* c) {
@@ -507,6 +695,23 @@ void pull_scatter(suminc_ctx* c, const uint32_t* cid, const uint8_t* chk, int ga
   Counters* cnt = dev_counters(c);
   const bool sk = cid != nullptr;
   const unsigned long long* counts = c->pull_cnt.as<unsigned long long>();
+  if (!sk && c->band_P > 1) {  // every row band by band
+    const uint32_t W = (uint32_t)c->band_W, S = (uint32_t)c->band_S;
+    for (uint32_t p = 0; p < c->band_P; ++p) {
+      if (S)
+        k_pr_pull_band_rows<T><<<grid_for(S, kBlock, 8192), kBlock, 0, c->stream>>>(
+            c->pull_short.as<uint32_t>(), S, c->band_split_s.as<uint32_t>(), p, c->in_col.as<uint32_t>(),
+            c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, gate);
+      if (W)
+        k_pr_pull_band_rows<T><<<grid_for(W, kBlock, 8192), kBlock, 0, c->stream>>>(
+            c->pull_wave.as<uint32_t>(), W, c->band_split.as<uint32_t>(), p, c->in_col.as<uint32_t>(),
+            c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, gate);
+      k_pr_pull_band_heavy<T><<<push_grid(), kBlock, 0, c->stream>>>(
+          c->band_chunks.as<uint4>(), c->band_off.as<uint32_t>(), p, c->in_col.as<uint32_t>(),
+          c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, gate);
+    }
+    return;
+  }
   auto ks = sk ? k_pr_pull_short<T, true> : k_pr_pull_short<T, false>;
   ks<<<grid_for(c->n, kBlock, 8192), kBlock, 0, c->stream>>>(c->pull_short.as<uint32_t>(), counts,
                                                               c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),

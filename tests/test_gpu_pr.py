@@ -40,15 +40,18 @@ def test_golden_test_base(gpu):
         assert err <= tol, (prec, err, tol)
 
 
-@pytest.mark.parametrize("scale", [12, 16])
+# band_kib=16: the banded pull (source bands of 4096 / 2048 vertices at FP32 /
+# FP64 on 2^16 vertices), which the default 48 MiB bands only use above 12.6M
+# vertices (FP32)
+@pytest.mark.parametrize("scale,band_kib", [(12, 0), (16, 0), (16, 16)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_pagerank_vs_fp64_oracle(gpu, scale, prec):
+def test_pagerank_vs_fp64_oracle(gpu, scale, band_kib, prec):
     g = oracle.Graph.rmat(scale, 16, seed=1)
     off, col, _ = g.csr()
     rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
     T = 1e-9 * g.n  # per-vertex threshold 1e-9 (the C1 setting)
     p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
-    with gpu.Context(pr_precision=p, termcheck_threshold=T) as ctx:
+    with gpu.Context(pr_precision=p, termcheck_threshold=T, pull_band_kib=band_kib) as ctx:
         ctx.upload_csr(off, col)
         ctx.pr_init()
         st = ctx.pr_run()
@@ -107,13 +110,14 @@ def test_step_and_upload(gpu):
     assert np.abs(r - ro).max() < 1e-15 and np.abs(d - do).max() < 1e-15
 
 
+@pytest.mark.parametrize("band_kib", [0, 16])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_incremental_reseed_matches_scratch(gpu, prec):
+def test_incremental_reseed_matches_scratch(gpu, prec, band_kib):
     g = oracle.Graph.rmat(14, 16, seed=3)
     off, col, _ = g.csr()
     T = 1e-10 * g.n
     p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
-    with gpu.Context(pr_precision=p, termcheck_threshold=T) as ctx:
+    with gpu.Context(pr_precision=p, termcheck_threshold=T, pull_band_kib=band_kib) as ctx:
         ctx.upload_csr(off, col)
         ctx.pr_init()
         ctx.pr_run()
