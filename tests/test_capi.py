@@ -55,3 +55,37 @@ def test_context_creation_without_gpu_fails_cleanly():
     rc = L.lib().suminc_ctx_create(0, None, C.byref(h))
     assert rc in (L.E_HIP, L.E_INVALID)
     assert not h.value
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors have the C structs' sizes and field offsets (a C probe
+    compiled against include/suminc.h prints them)."""
+    import numpy as np
+
+    mirrors = {"suminc_params": L.Params, "suminc_stats": L.Stats, "suminc_sketch_info": L.SketchInfo,
+               "suminc_kernel_profile": L.KernelProfile}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "suminc.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append('  printf("suminc_update size %zu\\n", sizeof(suminc_update));')
+    for f in L.UPDATE_DTYPE.names:
+        lines.append(f'  printf("suminc_update {f} %zu\\n", offsetof(suminc_update, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        s, f, v = line.split()
+        got[(s, f)] = int(v)
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+    dt = np.dtype(L.UPDATE_DTYPE)
+    assert got[("suminc_update", "size")] == dt.itemsize
+    for f in dt.names:
+        assert got[("suminc_update", f)] == dt.fields[f][1], f
