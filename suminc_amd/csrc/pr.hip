@@ -211,6 +211,29 @@ __device__ __forceinline__ uint32_t pull_cluster(const uint32_t* cid, const uint
   return INF32;
 }
 
+// Sum of contrib over the in-edges k = k0 + r * stride (r < U, k < e).  All U
+// column loads are issued before the U gathers, so a thread keeps U independent
+// gathers in flight instead of one dependent (column, gather) pair per edge: the
+// gathers are cache-miss latency bound, not bandwidth bound.
+template <class T, bool kSketch, int U>
+__device__ __forceinline__ T gather_sum(const uint32_t* __restrict__ icol, const T* __restrict__ contrib,
+                                        const uint32_t* __restrict__ cid, uint32_t cv, uint32_t k0,
+                                        uint32_t stride, uint32_t e) {
+  uint32_t u[U];
+#pragma unroll
+  for (int r = 0; r < U; ++r) {
+    const uint32_t k = k0 + r * stride;
+    u[r] = k < e ? icol[k] : INF32;
+  }
+  T x[U];
+#pragma unroll
+  for (int r = 0; r < U; ++r) x[r] = u[r] != INF32 && !pull_skip<kSketch>(cid, u[r], cv) ? contrib[u[r]] : T(0);
+  T acc = T(0);
+#pragma unroll
+  for (int r = 0; r < U; ++r) acc += x[r];
+  return acc;
+}
+
 // Pull, short rows (in-degree 1..kPullThread): one thread per vertex sums its
 // in-edges in order; no atomics (each vertex has one owner).
 template <class T, bool kSketch = false>
@@ -229,10 +252,7 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_short(const uint32_t* __rest
     const uint32_t b = ioff[v], e = ioff[v + 1];
     const uint32_t cv = pull_cluster<kSketch>(cid, chk, v);
     T acc = T(0);
-    for (uint32_t k = b; k < e; ++k) {
-      const uint32_t u = icol[k];
-      if (!pull_skip<kSketch>(cid, u, cv)) acc += contrib[u];
-    }
+    for (uint32_t k = b; k < e; k += 4) acc += gather_sum<T, kSketch, 4>(icol, contrib, cid, cv, k, 1, e);
     if (acc != T(0)) delta[v] += acc;
   }
 }
@@ -256,10 +276,8 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_wave(const uint32_t* __restr
     const uint32_t b = ioff[v], e = ioff[v + 1];
     const uint32_t cv = pull_cluster<kSketch>(cid, chk, v);
     T acc = T(0);
-    for (uint32_t k = b + lane_id(); k < e; k += 64) {
-      const uint32_t u = icol[k];
-      if (!pull_skip<kSketch>(cid, u, cv)) acc += contrib[u];
-    }
+    for (uint32_t k = b + lane_id(); k < e; k += 4 * 64)  // one pass for rows up to 256
+      acc += gather_sum<T, kSketch, 4>(icol, contrib, cid, cv, k, 64, e);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
     if (lane_id() == 0 && acc != T(0)) delta[v] += acc;
@@ -284,11 +302,7 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_heavy(const uint32_t* __rest
     const uint32_t b = ioff[ch.x] + ch.y * kChunk;
     const uint32_t e = min(ioff[ch.x + 1], b + kChunk);
     const uint32_t cv = pull_cluster<kSketch>(cid, chk, ch.x);
-    T acc = T(0);
-    for (uint32_t k = b + threadIdx.x; k < e; k += kBlock) {
-      const uint32_t u = icol[k];
-      if (!pull_skip<kSketch>(cid, u, cv)) acc += contrib[u];
-    }
+    T acc = gather_sum<T, kSketch, kChunk / kBlock>(icol, contrib, cid, cv, b + threadIdx.x, kBlock, e);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
     if (lane_id() == 0) s_red[threadIdx.x >> 6] = acc;
@@ -576,10 +590,41 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_band_rows(const uint32_t* __
       lo = split[(size_t)p * W + i];
       len = split[(size_t)(p + 1) * W + i] - lo;
     }
-    s_acc[threadIdx.x] = T(0);  // visible before the edge loop: block_adjacency syncs after its scan
-    block_adjacency<kBlock>(lo, len, s_scan, s_start, s_warp,
-                            [&](int slot, uint32_t, uint32_t e) { atomicAdd(&s_acc[slot], contrib[icol[e]]); });
+    s_acc[threadIdx.x] = T(0);
+    uint32_t total;
+    s_scan[threadIdx.x] = block_exclusive_scan<kBlock>(len, s_warp, &total);  // syncs: s_acc is clear
+    s_start[threadIdx.x] = lo;
+    if (threadIdx.x == 0) s_scan[kBlock] = total;
+    __syncthreads();
+    // four edges per thread at a time, loads first (as gather_sum)
+    for (uint32_t j0 = threadIdx.x; j0 < total; j0 += 4 * kBlock) {
+      int slot[4];
+      uint32_t u[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t j = j0 + r * kBlock;
+        slot[r] = -1;
+        u[r] = INF32;
+        if (j < total) {
+          int a = 0, z = kBlock - 1;  // owner: last row whose first edge is <= j
+          while (a < z) {
+            const int mid = (a + z + 1) >> 1;
+            if (s_scan[mid] <= j) a = mid; else z = mid - 1;
+          }
+          slot[r] = a;
+          u[r] = icol[s_start[a] + (j - s_scan[a])];
+        }
+      }
+      T x[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) x[r] = slot[r] >= 0 ? contrib[u[r]] : T(0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (slot[r] >= 0) atomicAdd(&s_acc[slot[r]], x[r]);
+    }
+    __syncthreads();
     if (i < W && s_acc[threadIdx.x] != T(0)) delta[v] += s_acc[threadIdx.x];
+    __syncthreads();  // s_scan / s_start / s_acc are rewritten by the next tile
   }
 }
 
@@ -595,8 +640,7 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_band_heavy(const uint4* __re
   const uint32_t h1 = off[p + 1];
   for (uint32_t h = off[p] + blockIdx.x; h < h1; h += gridDim.x) {
     const uint4 ch = chunks[h];
-    T acc = T(0);
-    for (uint32_t k = ch.y + threadIdx.x; k < ch.z; k += kBlock) acc += contrib[icol[k]];
+    T acc = gather_sum<T, false, kChunk / kBlock>(icol, contrib, nullptr, INF32, ch.y + threadIdx.x, kBlock, ch.z);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
     if (lane_id() == 0) s_red[threadIdx.x >> 6] = acc;
@@ -619,7 +663,7 @@ static void build_band_tables(suminc_ctx* c, uint64_t m_cap) {
   // bands of at most kib KiB of contributions, equal in width (measured on MI355X:
   // 32-64 MiB bands are fastest; 4 MiB bands lose to the unbanded pull at 2^22
   // vertices, 128 MiB ones at 2^26)
-  const uint64_t band_bytes = std::max<uint64_t>(4096, (uint64_t)(kib ? kib : 49152) * 1024);
+  const uint64_t band_bytes = std::max<uint64_t>(4096, (uint64_t)(kib ? kib : 65536) * 1024);
   const uint64_t P = std::min<uint64_t>(64, ((uint64_t)c->n * es + band_bytes - 1) / band_bytes);
   if (P <= 1) return;
   const uint64_t B = ((uint64_t)c->n + P - 1) / P;
