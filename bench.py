@@ -74,7 +74,7 @@ def kernel_names():
 KERNEL_NAMES = {}
 
 
-def roofline(ctx, kernel_ids):
+def roofline(ctx, kernel_ids, workload):
     import suminc_amd._lib as L
 
     best = None
@@ -88,7 +88,8 @@ def roofline(ctx, kernel_ids):
     achieved = p["alg_bytes"] / (p["ms"] * 1e-3) / 1e9
     name = KERNEL_NAMES[k]
     alg = p["alg_bytes"] / p["launches"]
-    traffic = measured_traffic(name)
+    # the s26 PageRank pull was measured on the PageRank-only run of the same graph
+    traffic = measured_traffic(name, "pr26" if workload == "web26" else workload)
     return name, {
         "bound": "hbm",
         "achieved": round(achieved, 2),
@@ -113,16 +114,17 @@ def kernel_table(ctx, kernel_ids):
     return out
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` on the default workload, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the counters'
-    reading for a plain copy of known size), or None when there is none."""
-    f = ROOT / "profiles" / f"{kernel}_sssp22_traffic.txt"
-    if not f.exists():
-        return None
-    for line in f.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload`, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summaries (scaled by the counters' reading for a plain
+    copy of known size), or None when there is none.  A PageRank pull "launch" is
+    one dense round's pull launches."""
+    for f, key in ((ROOT / "profiles" / f"{kernel}_{workload}_traffic.txt", "traffic_bytes_per_launch:"),
+                   (ROOT / "profiles" / f"{kernel}_traffic.txt", f"traffic_bytes_per_launch[{workload}]:")):
+        if f.exists():
+            for line in f.read_text().splitlines():
+                if line.startswith(key):
+                    return float(line.split(":", 1)[1])
     return None
 
 
@@ -171,7 +173,7 @@ def run_sssp(args, wl):
     ctx.sync()
     dt = time.perf_counter() - t0
     kids = [L.KERNEL_SSSP_SKETCH if sketch else L.KERNEL_SSSP_PUSH, L.KERNEL_BATCH_COPY]
-    kernel, roof = roofline(ctx, kids)
+    kernel, roof = roofline(ctx, kids, args.workload)
     extra["kernels"] = kernel_table(ctx, kids)
     ctx.profile_enable(False)
     for d in dev:
@@ -291,7 +293,7 @@ def run_pr(args, wl):
     ctx.sync()
     dt = time.perf_counter() - t0
     kids = [L.KERNEL_PR_PUSH, L.KERNEL_PR_PULL, L.KERNEL_PR_GATHER, L.KERNEL_BATCH_COPY]
-    kernel, roof = roofline(ctx, kids)
+    kernel, roof = roofline(ctx, kids, args.workload)
     ktab = kernel_table(ctx, kids)
     ctx.profile_enable(False)
     ms_per_step = 1e3 * dt / args.steps
@@ -403,7 +405,7 @@ def run_web(args, wl):
         log(f"batch {b}: {per[-1]}")
     dt = time.perf_counter() - t_all
     kids = [L.KERNEL_PR_PUSH, L.KERNEL_PR_PULL, L.KERNEL_PR_GATHER, L.KERNEL_SSSP_SKETCH, L.KERNEL_BATCH_COPY]
-    kernel, roof = roofline(ctx, kids)
+    kernel, roof = roofline(ctx, kids, args.workload)
     extra["kernels"] = kernel_table(ctx, kids)
     ctx.profile_enable(False)
     for d in dev:

@@ -80,7 +80,7 @@ typedef struct suminc_params {
   int32_t shortcut_max_rounds;/* cap on the in-cluster rounds of a shortcut / precompute (the reference's 100,
                                  iter_compressor.h:422) */
   int32_t pull_band_kib;      /* banded pull: a dense PageRank round gathers the contributions of one source
-                                 band of at most this many KiB at a time (0: 65536; negative: one band) */
+                                 band of at most this many KiB at a time (0: 61440; negative: one band) */
   int32_t reserved[6];
 } suminc_params;
 

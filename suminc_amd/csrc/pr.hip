@@ -661,9 +661,9 @@ static void build_band_tables(suminc_ctx* c, uint64_t m_cap) {
   c->band_P = 1;
   if (kib < 0) return;
   // bands of at most kib KiB of contributions, equal in width (measured on MI355X:
-  // 32-64 MiB bands are fastest; 4 MiB bands lose to the unbanded pull at 2^22
-  // vertices, 128 MiB ones at 2^26)
-  const uint64_t band_bytes = std::max<uint64_t>(4096, (uint64_t)(kib ? kib : 65536) * 1024);
+  // 2 bands of 32 MiB at 2^24 vertices and 4 of 64 MiB at 2^26 are fastest; 4 MiB
+  // bands lose to the unbanded pull at 2^22 vertices, 128 MiB ones at 2^26)
+  const uint64_t band_bytes = std::max<uint64_t>(4096, (uint64_t)(kib ? kib : 61440) * 1024);
   const uint64_t P = std::min<uint64_t>(64, ((uint64_t)c->n * es + band_bytes - 1) / band_bytes);
   if (P <= 1) return;
   const uint64_t B = ((uint64_t)c->n + P - 1) / P;

@@ -41,7 +41,7 @@ def test_golden_test_base(gpu):
 
 
 # band_kib=16: the banded pull (source bands of 4096 / 2048 vertices at FP32 /
-# FP64 on 2^16 vertices), which the default 64 MiB bands only use above 16.8M
+# FP64 on 2^16 vertices), which the default 60 MiB bands only use above 15.7M
 # vertices (FP32)
 @pytest.mark.parametrize("scale,band_kib", [(12, 0), (16, 0), (16, 16)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
