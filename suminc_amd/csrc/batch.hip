@@ -401,14 +401,15 @@ __global__ void k_gather_groups(const uint32_t* sidx, const unsigned* bc, const 
 
 // ---------------------------------------------------------------------------
 // PageRank reseed: (u, type*d*r_u/deg) for touched u over the current CSR, as a
-// push list and as the pull contribution (contrib is zero elsewhere); the edge
-// total picks the scatter like a round does (k_reseed_mode)
+// push list and, with pos, as the pull contribution at u's compact source
+// position (contrib is zero elsewhere); the edge total picks the scatter like a
+// round does (k_reseed_mode)
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_reseed_list(const uint32_t* t_list, const unsigned* bc, uint32_t N,
                                                         const uint32_t* off, const T* rank, const float* comp, T d,
                                                         T type, uint32_t* wl_v, T* wl_c, uint4* heavy, T* contrib,
-                                                        Counters* cnt) {
+                                                        const uint32_t* __restrict__ pos, Counters* cnt) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   bool light = false;
   uint32_t u = 0;
@@ -421,7 +422,7 @@ __global__ void __launch_bounds__(kBlock) k_reseed_list(const uint32_t* t_list, 
     if constexpr (sizeof(T) == 4) r = (T)((double)r - (double)comp[u]);
     if (deg > 0 && r != T(0)) {
       c = type * d * r / (T)deg;
-      contrib[u] = c;
+      if (pos) contrib[pos[u]] = c;  // deg > 0: u has a position
       edges = deg;
       if (deg <= kHeavyDeg) {
         light = true;
@@ -849,7 +850,8 @@ static void pr_reseed(suminc_ctx* c, uint32_t N, double type, bool lists) {
                                                           c->out_off.as<uint32_t>(), c->pr_rank.as<T>(),
                                                           c->pr_comp.as<float>(), (T)c->prm.damping, (T)type,
                                                           c->wl0.as<uint32_t>(), c->wl1.as<T>(), c->heavy.as<uint4>(),
-                                                          c->pr_contrib.as<T>(), cnt);
+                                                          c->pr_contrib.as<T>(),
+                                                          div > 0 ? c->src_pos.as<uint32_t>() : nullptr, cnt);
   k_reseed_mode<<<1, 64, 0, c->stream>>>(cnt, c->m, div);
   k_reseed_push<T><<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
                                                           c->pr_delta.as<T>(), c->wl0.as<uint32_t>(), c->wl1.as<T>(),

@@ -194,6 +194,11 @@ struct suminc_ctx {
   suminc::DevBuf band_split, band_split_s, band_cnt, band_pre, band_chunks, band_off;
   uint32_t band_P = 1, band_B = 0;
   uint64_t band_W = 0, band_S = 0;
+  // compact source positions (per graph version, with the pull lists): pos[u] =
+  // rank of u among the vertices with out-edges, INF32 for dangling ones, and the
+  // CSC columns renamed to positions, so a plain pull round gathers from an array
+  // of the n_src non-dangling contributions only
+  suminc::DevBuf src_pos, src_pre, in_col_c;
   uint64_t graph_version = 0;
 
   // SSSP state: packed {dist:hi32, parent:lo32}

@@ -117,7 +117,8 @@ __global__ void __launch_bounds__(kBlock) k_pr_gather(uint32_t n, const uint32_t
                                                       T* __restrict__ rank, float* __restrict__ comp, double eps, T d,
                                                       uint32_t* __restrict__ wl_v, T* __restrict__ wl_c,
                                                       uint4* __restrict__ heavy, T* __restrict__ contrib,
-                                                      Counters* cnt, unsigned long long* __restrict__ act_partial) {
+                                                      const uint32_t* __restrict__ pos, Counters* cnt,
+                                                      unsigned long long* __restrict__ act_partial) {
   __shared__ unsigned long long s_act[kBlock / 64];
   if (cnt->done) return;
   const bool pull = cnt->mode != 0;
@@ -166,7 +167,10 @@ __global__ void __launch_bounds__(kBlock) k_pr_gather(uint32_t n, const uint32_t
         }
       }
     }
-    if (pull) contrib[v] = c;
+    if (pull) {  // the contribution of every vertex with out-edges, at its compact position
+      const uint32_t p = pos[v];
+      if (p != INF32) contrib[p] = c;
+    }
   }
   if (!pull) {
     unsigned long long slot = wave_append(light, &cnt->wl_count[0]);
@@ -654,7 +658,51 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_band_heavy(const uint4* __re
   }
 }
 
-// Band tables for the lists just built (sizes read back from the device).
+// Compact source positions of the current graph and the renamed CSC columns
+// (every CSC column is a source with an out-edge, so its position is valid).
+__global__ void k_src_flags(uint32_t n, const uint32_t* __restrict__ off, uint32_t* __restrict__ flag) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) flag[v] = off[v + 1] > off[v] ? 1u : 0u;
+  else if (v == n) flag[n] = 0u;
+}
+
+__global__ void k_src_pos(uint32_t n, const uint32_t* __restrict__ off, const uint32_t* __restrict__ pre,
+                          uint32_t* __restrict__ pos) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) pos[v] = off[v + 1] > off[v] ? pre[v] : INF32;
+}
+
+__global__ void k_col_pos(uint32_t n, const uint32_t* __restrict__ ioff, const uint32_t* __restrict__ icol,
+                          const uint32_t* __restrict__ pos, uint32_t* __restrict__ icol_c) {
+  const uint64_t m = ioff[n];
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (uint64_t)gridDim.x * blockDim.x)
+    icol_c[e] = pos[icol[e]];
+}
+
+static void build_src_positions(suminc_ctx* c, uint64_t m_cap) {
+  const uint32_t n = c->n;
+  c->src_pos.ensure((size_t)n * 4 + 4);
+  c->src_pre.ensure((size_t)n * 8 + 8);  // flags, then their exclusive prefix
+  c->in_col_c.ensure_slack(m_cap * 4 + 4);
+  uint32_t* flag = c->src_pre.as<uint32_t>();
+  uint32_t* pre = flag + n + 1;
+  k_src_flags<<<grid_for((uint64_t)n + 1), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), flag);
+  size_t bytes = 0;
+  SUMINC_HIP(rocprim::exclusive_scan(nullptr, bytes, flag, pre, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(),
+                                     c->stream));
+  c->tmp.ensure(bytes);
+  SUMINC_HIP(rocprim::exclusive_scan(c->tmp.p, bytes, flag, pre, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(),
+                                     c->stream));
+  k_src_pos<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), pre, c->src_pos.as<uint32_t>());
+  k_col_pos<<<grid_for(m_cap, kBlock, 65536), kBlock, 0, c->stream>>>(n, c->in_off.as<uint32_t>(),
+                                                                      c->in_col.as<uint32_t>(),
+                                                                      c->src_pos.as<uint32_t>(),
+                                                                      c->in_col_c.as<uint32_t>());
+  SUMINC_HIP(hipGetLastError());
+}
+
+// Band tables for the lists just built (sizes read back from the device).  Bands
+// cut the compact source positions (CSC rows are sorted by them too).
 static void build_band_tables(suminc_ctx* c, uint64_t m_cap) {
   const int kib = c->prm.pull_band_kib;
   const uint64_t es = c->pr_prec == SUMINC_PR_FP64 ? 8 : 4;
@@ -664,22 +712,26 @@ static void build_band_tables(suminc_ctx* c, uint64_t m_cap) {
   // 2 bands of 32 MiB at 2^24 vertices and 4 of 64 MiB at 2^26 are fastest; 4 MiB
   // bands lose to the unbanded pull at 2^22 vertices, 128 MiB ones at 2^26)
   const uint64_t band_bytes = std::max<uint64_t>(4096, (uint64_t)(kib ? kib : 61440) * 1024);
-  const uint64_t P = std::min<uint64_t>(64, ((uint64_t)c->n * es + band_bytes - 1) / band_bytes);
-  if (P <= 1) return;
-  const uint64_t B = ((uint64_t)c->n + P - 1) / P;
+  if ((uint64_t)c->n * es <= band_bytes) return;  // n_src <= n
   unsigned long long h[3];
+  uint32_t n_src = 0;
   SUMINC_HIP(hipMemcpyAsync(h, c->pull_cnt.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipMemcpyAsync(&n_src, c->src_pre.as<uint32_t>() + 2 * (size_t)c->n + 1, 4, hipMemcpyDeviceToHost,
+                            c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
+  const uint64_t P = std::min<uint64_t>(64, ((uint64_t)n_src * es + band_bytes - 1) / band_bytes);
+  if (P <= 1) return;
+  const uint64_t B = ((uint64_t)n_src + P - 1) / P;
   const uint32_t S = (uint32_t)h[0], W = (uint32_t)h[1], Hc = (uint32_t)h[2];
   c->band_split.ensure_slack(((size_t)W * (P + 1) + 1) * 4);
   c->band_split_s.ensure_slack(((size_t)S * (P + 1) + 1) * 4);
   if (W)
     k_band_split<<<grid_for((uint64_t)W * (P + 1), kBlock, 65536), kBlock, 0, c->stream>>>(
-        c->pull_wave.as<uint32_t>(), W, c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(), (uint32_t)P,
+        c->pull_wave.as<uint32_t>(), W, c->in_off.as<uint32_t>(), c->in_col_c.as<uint32_t>(), (uint32_t)P,
         (uint32_t)B, c->band_split.as<uint32_t>());
   if (S)
     k_band_split<<<grid_for((uint64_t)S * (P + 1), kBlock, 65536), kBlock, 0, c->stream>>>(
-        c->pull_short.as<uint32_t>(), S, c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(), (uint32_t)P,
+        c->pull_short.as<uint32_t>(), S, c->in_off.as<uint32_t>(), c->in_col_c.as<uint32_t>(), (uint32_t)P,
         (uint32_t)B, c->band_split_s.as<uint32_t>());
   const uint64_t nc = (uint64_t)Hc * P;
   c->band_cnt.ensure_slack((nc + 1) * 4);
@@ -690,7 +742,7 @@ static void build_band_tables(suminc_ctx* c, uint64_t m_cap) {
   SUMINC_HIP(hipMemsetAsync(c->band_off.p, 0, (P + 1) * 4, c->stream));
   const unsigned gc = grid_for(nc + 1, kBlock, 65536);
   k_band_heavy_count<<<gc, kBlock, 0, c->stream>>>(c->pull_heavy.as<uint2>(), Hc, c->in_off.as<uint32_t>(),
-                                                   c->in_col.as<uint32_t>(), (uint32_t)P, (uint32_t)B,
+                                                   c->in_col_c.as<uint32_t>(), (uint32_t)P, (uint32_t)B,
                                                    c->band_cnt.as<uint32_t>());
   size_t bytes = 0;
   SUMINC_HIP(rocprim::exclusive_scan(nullptr, bytes, c->band_cnt.as<uint32_t>(), c->band_pre.as<uint32_t>(), 0u,
@@ -699,7 +751,7 @@ static void build_band_tables(suminc_ctx* c, uint64_t m_cap) {
   SUMINC_HIP(rocprim::exclusive_scan(c->tmp.p, bytes, c->band_cnt.as<uint32_t>(), c->band_pre.as<uint32_t>(), 0u,
                                      (size_t)nc + 1, rocprim::plus<uint32_t>(), c->stream));
   k_band_heavy_fill<<<gc, kBlock, 0, c->stream>>>(c->pull_heavy.as<uint2>(), Hc, c->in_off.as<uint32_t>(),
-                                                  c->in_col.as<uint32_t>(), (uint32_t)P, (uint32_t)B,
+                                                  c->in_col_c.as<uint32_t>(), (uint32_t)P, (uint32_t)B,
                                                   c->band_pre.as<uint32_t>(), c->band_chunks.as<uint4>(),
                                                   c->band_off.as<uint32_t>());
   SUMINC_HIP(hipGetLastError());
@@ -722,6 +774,7 @@ void build_pull_lists(suminc_ctx* c, uint64_t m_cap) {
                                            c->pull_short.as<uint32_t>(), c->pull_wave.as<uint32_t>(),
                                            c->pull_heavy.as<uint2>());
   SUMINC_HIP(hipGetLastError());
+  build_src_positions(c, m_cap);
   build_band_tables(c, m_cap);
 }
 
@@ -739,35 +792,38 @@ void pull_scatter(suminc_ctx* c, const uint32_t* cid, const uint8_t* chk, int ga
   Counters* cnt = dev_counters(c);
   const bool sk = cid != nullptr;
   const unsigned long long* counts = c->pull_cnt.as<unsigned long long>();
+  // the plain pull gathers compact contributions through the renamed columns; the
+  // sketch pull keeps vertex ids (its skip test reads the source's cluster)
+  const uint32_t* icol = sk ? c->in_col.as<uint32_t>() : c->in_col_c.as<uint32_t>();
   if (!sk && c->band_P > 1) {  // every row band by band
     const uint32_t W = (uint32_t)c->band_W, S = (uint32_t)c->band_S;
     for (uint32_t p = 0; p < c->band_P; ++p) {
       if (S)
         k_pr_pull_band_rows<T><<<grid_for(S, kBlock, 8192), kBlock, 0, c->stream>>>(
-            c->pull_short.as<uint32_t>(), S, c->band_split_s.as<uint32_t>(), p, c->in_col.as<uint32_t>(),
+            c->pull_short.as<uint32_t>(), S, c->band_split_s.as<uint32_t>(), p, icol,
             c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, gate);
       if (W)
         k_pr_pull_band_rows<T><<<grid_for(W, kBlock, 8192), kBlock, 0, c->stream>>>(
-            c->pull_wave.as<uint32_t>(), W, c->band_split.as<uint32_t>(), p, c->in_col.as<uint32_t>(),
+            c->pull_wave.as<uint32_t>(), W, c->band_split.as<uint32_t>(), p, icol,
             c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, gate);
       k_pr_pull_band_heavy<T><<<push_grid(), kBlock, 0, c->stream>>>(
-          c->band_chunks.as<uint4>(), c->band_off.as<uint32_t>(), p, c->in_col.as<uint32_t>(),
+          c->band_chunks.as<uint4>(), c->band_off.as<uint32_t>(), p, icol,
           c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, gate);
     }
     return;
   }
   auto ks = sk ? k_pr_pull_short<T, true> : k_pr_pull_short<T, false>;
   ks<<<grid_for(c->n, kBlock, 8192), kBlock, 0, c->stream>>>(c->pull_short.as<uint32_t>(), counts,
-                                                              c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
+                                                              c->in_off.as<uint32_t>(), icol,
                                                               c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, gate,
                                                               cid, chk);
   auto kw = sk ? k_pr_pull_wave<T, true> : k_pr_pull_wave<T, false>;
   kw<<<grid_for(c->n, kBlock / 64, 8192), kBlock, 0, c->stream>>>(c->pull_wave.as<uint32_t>(), counts,
-                                                                   c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
+                                                                   c->in_off.as<uint32_t>(), icol,
                                                                    c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt,
                                                                    gate, cid, chk);
   auto kh = sk ? k_pr_pull_heavy<T, true> : k_pr_pull_heavy<T, false>;
-  kh<<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
+  kh<<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), icol,
                                             c->pr_contrib.as<T>(), c->pr_delta.as<T>(), c->pull_heavy.as<uint2>(),
                                             counts, cnt, gate, cid, chk);
 }
@@ -789,7 +845,7 @@ static void enqueue_round(suminc_ctx* c, double eps, double T_stop, int max_roun
   k_pr_gather<T><<<g, kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), c->pr_delta.as<T>(),
                                               c->pr_rank.as<T>(), c->pr_comp.as<float>(), eps, (T)c->prm.damping,
                                               c->wl0.as<uint32_t>(), c->wl1.as<T>(), c->heavy.as<uint4>(),
-                                              c->pr_contrib.as<T>(), cnt,
+                                              c->pr_contrib.as<T>(), c->src_pos.as<uint32_t>(), cnt,
                                               reinterpret_cast<unsigned long long*>(c->partial.as<double>()) + 2 * g);
   if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 4 * slot + 2), c->stream));
   k_pr_push<T><<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
