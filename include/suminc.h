@@ -81,7 +81,10 @@ typedef struct suminc_params {
                                  iter_compressor.h:422) */
   int32_t pull_band_kib;      /* banded pull: a dense PageRank round gathers the contributions of one source
                                  band of at most this many KiB at a time (0: 61440; negative: one band) */
-  int32_t reserved[6];
+  int32_t pull_compact_kib;   /* the plain pull gathers contributions of the vertices with out-edges only,
+                                 renamed to dense positions, once the contributions exceed this many KiB
+                                 (0: 32768; negative: never) */
+  int32_t reserved[5];
 } suminc_params;
 
 typedef struct suminc_stats {

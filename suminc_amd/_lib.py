@@ -53,7 +53,8 @@ class Params(C.Structure):
         ("pull_threshold_div", C.c_int32),
         ("shortcut_max_rounds", C.c_int32),
         ("pull_band_kib", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("pull_compact_kib", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 

@@ -422,7 +422,7 @@ __global__ void __launch_bounds__(kBlock) k_reseed_list(const uint32_t* t_list, 
     if constexpr (sizeof(T) == 4) r = (T)((double)r - (double)comp[u]);
     if (deg > 0 && r != T(0)) {
       c = type * d * r / (T)deg;
-      if (pos) contrib[pos[u]] = c;  // deg > 0: u has a position
+      contrib[pos ? pos[u] : u] = c;  // deg > 0: u has a position
       edges = deg;
       if (deg <= kHeavyDeg) {
         light = true;
@@ -851,7 +851,7 @@ static void pr_reseed(suminc_ctx* c, uint32_t N, double type, bool lists) {
                                                           c->pr_comp.as<float>(), (T)c->prm.damping, (T)type,
                                                           c->wl0.as<uint32_t>(), c->wl1.as<T>(), c->heavy.as<uint4>(),
                                                           c->pr_contrib.as<T>(),
-                                                          div > 0 ? c->src_pos.as<uint32_t>() : nullptr, cnt);
+                                                          div > 0 ? pull_pos(c) : nullptr, cnt);
   k_reseed_mode<<<1, 64, 0, c->stream>>>(cnt, c->m, div);
   k_reseed_push<T><<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
                                                           c->pr_delta.as<T>(), c->wl0.as<uint32_t>(), c->wl1.as<T>(),

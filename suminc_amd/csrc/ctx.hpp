@@ -199,6 +199,7 @@ struct suminc_ctx {
   // CSC columns renamed to positions, so a plain pull round gathers from an array
   // of the n_src non-dangling contributions only
   suminc::DevBuf src_pos, src_pre, in_col_c;
+  bool pull_compact = false;  // the positions are in use (contributions above 32 MiB)
   uint64_t graph_version = 0;
 
   // SSSP state: packed {dist:hi32, parent:lo32}

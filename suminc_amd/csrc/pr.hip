@@ -167,8 +167,8 @@ __global__ void __launch_bounds__(kBlock) k_pr_gather(uint32_t n, const uint32_t
         }
       }
     }
-    if (pull) {  // the contribution of every vertex with out-edges, at its compact position
-      const uint32_t p = pos[v];
+    if (pull) {  // the contribution of every vertex (pos: of every vertex with out-edges, at its compact position)
+      const uint32_t p = pos ? pos[v] : v;
       if (p != INF32) contrib[p] = c;
     }
   }
@@ -470,6 +470,13 @@ __global__ void k_zero_counters(Counters* cnt) {
 // ---------------------------------------------------------------------------
 unsigned push_grid() { return 2048; }
 
+// Contribution index of the plain pull: compact source positions when in use
+// (nullptr: vertex ids), and the CSC columns the plain pull reads.
+const uint32_t* pull_pos(suminc_ctx* c) { return c->pull_compact ? c->src_pos.as<uint32_t>() : nullptr; }
+const uint32_t* pull_cols(suminc_ctx* c) {
+  return c->pull_compact ? c->in_col_c.as<uint32_t>() : c->in_col.as<uint32_t>();
+}
+
 void pr_alloc(suminc_ctx* c) {
   const size_t n = c->n;
   const size_t es = c->pr_prec == SUMINC_PR_FP64 ? 8 : 4;
@@ -681,6 +688,12 @@ __global__ void k_col_pos(uint32_t n, const uint32_t* __restrict__ ioff, const u
 
 static void build_src_positions(suminc_ctx* c, uint64_t m_cap) {
   const uint32_t n = c->n;
+  // measured on MI355X: at 2^22 vertices (16 MiB of FP32 contributions) renaming
+  // costs more per batch than it saves; from 2^24 on it pays
+  const uint64_t es = c->pr_prec == SUMINC_PR_FP64 ? 8 : 4;
+  const int kib = c->prm.pull_compact_kib;
+  c->pull_compact = kib >= 0 && (uint64_t)n * es > (uint64_t)(kib ? kib : 32768) * 1024;
+  if (!c->pull_compact) return;
   c->src_pos.ensure((size_t)n * 4 + 4);
   c->src_pre.ensure((size_t)n * 8 + 8);  // flags, then their exclusive prefix
   c->in_col_c.ensure_slack(m_cap * 4 + 4);
@@ -714,10 +727,11 @@ static void build_band_tables(suminc_ctx* c, uint64_t m_cap) {
   const uint64_t band_bytes = std::max<uint64_t>(4096, (uint64_t)(kib ? kib : 61440) * 1024);
   if ((uint64_t)c->n * es <= band_bytes) return;  // n_src <= n
   unsigned long long h[3];
-  uint32_t n_src = 0;
+  uint32_t n_src = c->n;
   SUMINC_HIP(hipMemcpyAsync(h, c->pull_cnt.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-  SUMINC_HIP(hipMemcpyAsync(&n_src, c->src_pre.as<uint32_t>() + 2 * (size_t)c->n + 1, 4, hipMemcpyDeviceToHost,
-                            c->stream));
+  if (c->pull_compact)
+    SUMINC_HIP(hipMemcpyAsync(&n_src, c->src_pre.as<uint32_t>() + 2 * (size_t)c->n + 1, 4, hipMemcpyDeviceToHost,
+                              c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
   const uint64_t P = std::min<uint64_t>(64, ((uint64_t)n_src * es + band_bytes - 1) / band_bytes);
   if (P <= 1) return;
@@ -727,11 +741,11 @@ static void build_band_tables(suminc_ctx* c, uint64_t m_cap) {
   c->band_split_s.ensure_slack(((size_t)S * (P + 1) + 1) * 4);
   if (W)
     k_band_split<<<grid_for((uint64_t)W * (P + 1), kBlock, 65536), kBlock, 0, c->stream>>>(
-        c->pull_wave.as<uint32_t>(), W, c->in_off.as<uint32_t>(), c->in_col_c.as<uint32_t>(), (uint32_t)P,
+        c->pull_wave.as<uint32_t>(), W, c->in_off.as<uint32_t>(), pull_cols(c), (uint32_t)P,
         (uint32_t)B, c->band_split.as<uint32_t>());
   if (S)
     k_band_split<<<grid_for((uint64_t)S * (P + 1), kBlock, 65536), kBlock, 0, c->stream>>>(
-        c->pull_short.as<uint32_t>(), S, c->in_off.as<uint32_t>(), c->in_col_c.as<uint32_t>(), (uint32_t)P,
+        c->pull_short.as<uint32_t>(), S, c->in_off.as<uint32_t>(), pull_cols(c), (uint32_t)P,
         (uint32_t)B, c->band_split_s.as<uint32_t>());
   const uint64_t nc = (uint64_t)Hc * P;
   c->band_cnt.ensure_slack((nc + 1) * 4);
@@ -742,7 +756,7 @@ static void build_band_tables(suminc_ctx* c, uint64_t m_cap) {
   SUMINC_HIP(hipMemsetAsync(c->band_off.p, 0, (P + 1) * 4, c->stream));
   const unsigned gc = grid_for(nc + 1, kBlock, 65536);
   k_band_heavy_count<<<gc, kBlock, 0, c->stream>>>(c->pull_heavy.as<uint2>(), Hc, c->in_off.as<uint32_t>(),
-                                                   c->in_col_c.as<uint32_t>(), (uint32_t)P, (uint32_t)B,
+                                                   pull_cols(c), (uint32_t)P, (uint32_t)B,
                                                    c->band_cnt.as<uint32_t>());
   size_t bytes = 0;
   SUMINC_HIP(rocprim::exclusive_scan(nullptr, bytes, c->band_cnt.as<uint32_t>(), c->band_pre.as<uint32_t>(), 0u,
@@ -751,7 +765,7 @@ static void build_band_tables(suminc_ctx* c, uint64_t m_cap) {
   SUMINC_HIP(rocprim::exclusive_scan(c->tmp.p, bytes, c->band_cnt.as<uint32_t>(), c->band_pre.as<uint32_t>(), 0u,
                                      (size_t)nc + 1, rocprim::plus<uint32_t>(), c->stream));
   k_band_heavy_fill<<<gc, kBlock, 0, c->stream>>>(c->pull_heavy.as<uint2>(), Hc, c->in_off.as<uint32_t>(),
-                                                  c->in_col_c.as<uint32_t>(), (uint32_t)P, (uint32_t)B,
+                                                  pull_cols(c), (uint32_t)P, (uint32_t)B,
                                                   c->band_pre.as<uint32_t>(), c->band_chunks.as<uint4>(),
                                                   c->band_off.as<uint32_t>());
   SUMINC_HIP(hipGetLastError());
@@ -794,7 +808,7 @@ void pull_scatter(suminc_ctx* c, const uint32_t* cid, const uint8_t* chk, int ga
   const unsigned long long* counts = c->pull_cnt.as<unsigned long long>();
   // the plain pull gathers compact contributions through the renamed columns; the
   // sketch pull keeps vertex ids (its skip test reads the source's cluster)
-  const uint32_t* icol = sk ? c->in_col.as<uint32_t>() : c->in_col_c.as<uint32_t>();
+  const uint32_t* icol = sk ? c->in_col.as<uint32_t>() : pull_cols(c);
   if (!sk && c->band_P > 1) {  // every row band by band
     const uint32_t W = (uint32_t)c->band_W, S = (uint32_t)c->band_S;
     for (uint32_t p = 0; p < c->band_P; ++p) {
@@ -845,7 +859,7 @@ static void enqueue_round(suminc_ctx* c, double eps, double T_stop, int max_roun
   k_pr_gather<T><<<g, kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), c->pr_delta.as<T>(),
                                               c->pr_rank.as<T>(), c->pr_comp.as<float>(), eps, (T)c->prm.damping,
                                               c->wl0.as<uint32_t>(), c->wl1.as<T>(), c->heavy.as<uint4>(),
-                                              c->pr_contrib.as<T>(), c->src_pos.as<uint32_t>(), cnt,
+                                              c->pr_contrib.as<T>(), pull_pos(c), cnt,
                                               reinterpret_cast<unsigned long long*>(c->partial.as<double>()) + 2 * g);
   if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 4 * slot + 2), c->stream));
   k_pr_push<T><<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),

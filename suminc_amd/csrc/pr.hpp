@@ -19,6 +19,9 @@ void pr_step(suminc_ctx* c, double* residual);
 void pr_download(suminc_ctx* c, double* rank, double* delta);
 void pr_upload(suminc_ctx* c, const double* rank, const double* delta);
 unsigned push_grid();
+// the plain pull's contribution index (nullptr: vertex ids) and CSC columns
+const uint32_t* pull_pos(suminc_ctx* c);
+const uint32_t* pull_cols(suminc_ctx* c);
 // in-degree classes of the CSC for the pull rounds: built on the stream (lengths
 // stay on the device) for the current CSC, sized for up to m_cap edges
 void build_pull_lists(suminc_ctx* c, uint64_t m_cap);

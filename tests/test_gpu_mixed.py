@@ -28,12 +28,15 @@ def check_sssp(ctx, g, src):
     assert np.array_equal(p, op)
 
 
-@pytest.mark.parametrize("sketch", [False, True])
-def test_pagerank_between_batch_and_sssp(gpu, sketch):
+# compact=1: the plain pull over compact source positions in 16 KiB source bands
+# (both used by default only on large graphs)
+@pytest.mark.parametrize("sketch,compact", [(False, 0), (True, 0), (False, 1), (True, 1)])
+def test_pagerank_between_batch_and_sssp(gpu, sketch, compact):
     g = oracle.Graph.rmat(14, 16, seed=5, weights="u32", wmin=1, wmax=255)
     off, col, w = g.csr()
     src = 0
-    with gpu.Context(termcheck_threshold=1e-9) as ctx:
+    kw = {"pull_compact_kib": 1, "pull_band_kib": 16} if compact else {}
+    with gpu.Context(termcheck_threshold=1e-9, **kw) as ctx:
         ctx.upload_csr(off, col, w)
         if sketch:
             ctx.cluster_leaves(1024, 2)

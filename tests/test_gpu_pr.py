@@ -42,16 +42,18 @@ def test_golden_test_base(gpu):
 
 # band_kib=16: the banded pull (source bands of 4096 / 2048 vertices at FP32 /
 # FP64 on 2^16 vertices), which the default 60 MiB bands only use above 15.7M
-# vertices (FP32)
-@pytest.mark.parametrize("scale,band_kib", [(12, 0), (16, 0), (16, 16)])
+# vertices (FP32); compact_kib=1: the pull over compact source positions, used by
+# default above 32 MiB of contributions (8.4M vertices in FP32)
+@pytest.mark.parametrize("scale,band_kib,compact_kib", [(12, 0, 0), (16, 0, 0), (16, 16, 0), (16, 0, 1), (16, 16, 1)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_pagerank_vs_fp64_oracle(gpu, scale, band_kib, prec):
+def test_pagerank_vs_fp64_oracle(gpu, scale, band_kib, compact_kib, prec):
     g = oracle.Graph.rmat(scale, 16, seed=1)
     off, col, _ = g.csr()
     rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
     T = 1e-9 * g.n  # per-vertex threshold 1e-9 (the C1 setting)
     p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
-    with gpu.Context(pr_precision=p, termcheck_threshold=T, pull_band_kib=band_kib) as ctx:
+    with gpu.Context(pr_precision=p, termcheck_threshold=T, pull_band_kib=band_kib,
+                     pull_compact_kib=compact_kib) as ctx:
         ctx.upload_csr(off, col)
         ctx.pr_init()
         st = ctx.pr_run()
@@ -110,14 +112,15 @@ def test_step_and_upload(gpu):
     assert np.abs(r - ro).max() < 1e-15 and np.abs(d - do).max() < 1e-15
 
 
-@pytest.mark.parametrize("band_kib", [0, 16])
+@pytest.mark.parametrize("band_kib,compact_kib", [(0, 0), (16, 0), (0, 1), (16, 1)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_incremental_reseed_matches_scratch(gpu, prec, band_kib):
+def test_incremental_reseed_matches_scratch(gpu, prec, band_kib, compact_kib):
     g = oracle.Graph.rmat(14, 16, seed=3)
     off, col, _ = g.csr()
     T = 1e-10 * g.n
     p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
-    with gpu.Context(pr_precision=p, termcheck_threshold=T, pull_band_kib=band_kib) as ctx:
+    with gpu.Context(pr_precision=p, termcheck_threshold=T, pull_band_kib=band_kib,
+                     pull_compact_kib=compact_kib) as ctx:
         ctx.upload_csr(off, col)
         ctx.pr_init()
         ctx.pr_run()

@@ -97,13 +97,17 @@ def test_sum_pr_query_equals_fixed_point(gpu, kind, prec):
     assert st["reserved"][5] > 0  # sketch rounds ran
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_sum_pr_incremental(gpu, prec):
+# compact=1: the residual phase pulls over compact source positions in 16 KiB
+# source bands (the default only on large graphs) while the sketch rounds pull by
+# vertex id from the same contribution buffer
+@pytest.mark.parametrize("prec,compact", [("fp64", 0), ("fp32", 0), ("fp32", 1)])
+def test_sum_pr_incremental(gpu, prec, compact):
     g = oracle.Graph.rmat(13, 16, seed=3)
     off, col, _ = g.csr()
     T = 1e-10 * g.n
     p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
-    with gpu.Context(pr_precision=p, termcheck_threshold=T) as ctx:
+    kw = {"pull_compact_kib": 1, "pull_band_kib": 16} if compact else {}
+    with gpu.Context(pr_precision=p, termcheck_threshold=T, **kw) as ctx:
         ctx.upload_csr(off, col)
         ctx.set_clusters(blocks(g.n, 4096, 8))
         ctx.sum_build(2, 1024, -1e9)
