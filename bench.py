@@ -525,7 +525,7 @@ def main():
                     help="SSSP workload: query over the supernode sketch (on) or the plain CSR (off)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="length of the CPU baseline sample")
     ap.add_argument("--pull-band-kib", type=int, default=0,
-                    help="PageRank workloads: source band of the banded pull in KiB (0: library default, -1: off)")
+                    help="PageRank workloads: source band of the banded pull in KiB (0: off, the library default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device (counter calibration for rocprofv3 --pmc) and exit")
     args = ap.parse_args()

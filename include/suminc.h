@@ -80,10 +80,11 @@ typedef struct suminc_params {
   int32_t shortcut_max_rounds;/* cap on the in-cluster rounds of a shortcut / precompute (the reference's 100,
                                  iter_compressor.h:422) */
   int32_t pull_band_kib;      /* banded pull: a dense PageRank round gathers the contributions of one source
-                                 band of at most this many KiB at a time (0: 61440; negative: one band) */
+                                 band of at most this many KiB at a time (<= 0: one band, the default) */
   int32_t pull_compact_kib;   /* the plain pull gathers contributions of the vertices with out-edges only,
-                                 renamed to dense positions, once the contributions exceed this many KiB
-                                 (0: 32768; negative: never) */
+                                 renamed to dense positions (by decreasing out-degree; in id order when
+                                 banded), once the contributions exceed this many KiB (0: 32768;
+                                 negative: never) */
   int32_t reserved[5];
 } suminc_params;
 

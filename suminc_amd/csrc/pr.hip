@@ -686,7 +686,29 @@ __global__ void k_col_pos(uint32_t n, const uint32_t* __restrict__ ioff, const u
     icol_c[e] = pos[icol[e]];
 }
 
-static void build_src_positions(suminc_ctx* c, uint64_t m_cap) {
+// Degree order: keys ~outdeg (a stable sort puts larger out-degrees first, ties
+// by id, dangling vertices last), then pos[id] = rank.
+__global__ void k_deg_keys(uint32_t n, const uint32_t* __restrict__ off, uint32_t* __restrict__ key,
+                           uint32_t* __restrict__ id) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) {
+    key[v] = 0xFFFFFFFFu - (off[v + 1] - off[v]);
+    id[v] = v;
+  }
+}
+
+__global__ void k_deg_pos(uint32_t n, const uint32_t* __restrict__ key, const uint32_t* __restrict__ id,
+                          uint32_t* __restrict__ pos) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) pos[id[i]] = key[i] != 0xFFFFFFFFu ? i : INF32;
+}
+
+// by_degree: positions by decreasing out-degree, so the most gathered
+// contributions share cache lines (measured on MI355X at 2^26 vertices: 9.0 ms per
+// round, against 9.8 ms for id order with source bands and 12.2 ms without);
+// otherwise id order, which keeps CSC rows sorted by position (the banded pull
+// needs that).
+static void build_src_positions(suminc_ctx* c, uint64_t m_cap, bool by_degree) {
   const uint32_t n = c->n;
   // measured on MI355X: at 2^22 vertices (16 MiB of FP32 contributions) renaming
   // costs more per batch than it saves; from 2^24 on it pays
@@ -695,18 +717,30 @@ static void build_src_positions(suminc_ctx* c, uint64_t m_cap) {
   c->pull_compact = kib >= 0 && (uint64_t)n * es > (uint64_t)(kib ? kib : 32768) * 1024;
   if (!c->pull_compact) return;
   c->src_pos.ensure((size_t)n * 4 + 4);
-  c->src_pre.ensure((size_t)n * 8 + 8);  // flags, then their exclusive prefix
+  c->src_pre.ensure((size_t)n * 16 + 16);  // id order: flags + prefix; degree order: keys + ids, sorted pairs
   c->in_col_c.ensure_slack(m_cap * 4 + 4);
   uint32_t* flag = c->src_pre.as<uint32_t>();
   uint32_t* pre = flag + n + 1;
-  k_src_flags<<<grid_for((uint64_t)n + 1), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), flag);
   size_t bytes = 0;
-  SUMINC_HIP(rocprim::exclusive_scan(nullptr, bytes, flag, pre, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(),
-                                     c->stream));
-  c->tmp.ensure(bytes);
-  SUMINC_HIP(rocprim::exclusive_scan(c->tmp.p, bytes, flag, pre, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(),
-                                     c->stream));
-  k_src_pos<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), pre, c->src_pos.as<uint32_t>());
+  if (by_degree) {
+    uint32_t* k1 = c->src_pre.as<uint32_t>();
+    uint32_t* i1 = k1 + n;
+    uint32_t* k2 = i1 + n;
+    uint32_t* i2 = k2 + n;
+    k_deg_keys<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), k1, i1);
+    SUMINC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, k1, k2, i1, i2, (size_t)n, 0, 32, c->stream));
+    c->tmp.ensure(bytes);
+    SUMINC_HIP(rocprim::radix_sort_pairs(c->tmp.p, bytes, k1, k2, i1, i2, (size_t)n, 0, 32, c->stream));
+    k_deg_pos<<<grid_for(n), kBlock, 0, c->stream>>>(n, k2, i2, c->src_pos.as<uint32_t>());
+  } else {
+    k_src_flags<<<grid_for((uint64_t)n + 1), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), flag);
+    SUMINC_HIP(rocprim::exclusive_scan(nullptr, bytes, flag, pre, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(),
+                                       c->stream));
+    c->tmp.ensure(bytes);
+    SUMINC_HIP(rocprim::exclusive_scan(c->tmp.p, bytes, flag, pre, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(),
+                                       c->stream));
+    k_src_pos<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), pre, c->src_pos.as<uint32_t>());
+  }
   k_col_pos<<<grid_for(m_cap, kBlock, 65536), kBlock, 0, c->stream>>>(n, c->in_off.as<uint32_t>(),
                                                                       c->in_col.as<uint32_t>(),
                                                                       c->src_pos.as<uint32_t>(),
@@ -715,16 +749,19 @@ static void build_src_positions(suminc_ctx* c, uint64_t m_cap) {
 }
 
 // Band tables for the lists just built (sizes read back from the device).  Bands
-// cut the compact source positions (CSC rows are sorted by them too).
+// cut the compact source positions in id order (CSC rows are sorted by them too).
+static bool banded_pull(const suminc_ctx* c) { return c->prm.pull_band_kib > 0; }
+
 static void build_band_tables(suminc_ctx* c, uint64_t m_cap) {
   const int kib = c->prm.pull_band_kib;
   const uint64_t es = c->pr_prec == SUMINC_PR_FP64 ? 8 : 4;
   c->band_P = 1;
-  if (kib < 0) return;
-  // bands of at most kib KiB of contributions, equal in width (measured on MI355X:
-  // 2 bands of 32 MiB at 2^24 vertices and 4 of 64 MiB at 2^26 are fastest; 4 MiB
-  // bands lose to the unbanded pull at 2^22 vertices, 128 MiB ones at 2^26)
-  const uint64_t band_bytes = std::max<uint64_t>(4096, (uint64_t)(kib ? kib : 61440) * 1024);
+  if (!banded_pull(c)) return;
+  // bands of at most kib KiB of contributions, equal in width (measured on MI355X
+  // with id-order positions: 2 bands of 32 MiB at 2^24 vertices and 3-4 of 34-60
+  // MiB at 2^26 are fastest; 4 MiB bands lose to the unbanded pull at 2^22
+  // vertices, 128 MiB ones at 2^26)
+  const uint64_t band_bytes = std::max<uint64_t>(4096, (uint64_t)kib * 1024);
   if ((uint64_t)c->n * es <= band_bytes) return;  // n_src <= n
   unsigned long long h[3];
   uint32_t n_src = c->n;
@@ -788,7 +825,7 @@ void build_pull_lists(suminc_ctx* c, uint64_t m_cap) {
                                            c->pull_short.as<uint32_t>(), c->pull_wave.as<uint32_t>(),
                                            c->pull_heavy.as<uint2>());
   SUMINC_HIP(hipGetLastError());
-  build_src_positions(c, m_cap);
+  build_src_positions(c, m_cap, !banded_pull(c));
   build_band_tables(c, m_cap);
 }
 

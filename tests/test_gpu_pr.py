@@ -41,9 +41,9 @@ def test_golden_test_base(gpu):
 
 
 # band_kib=16: the banded pull (source bands of 4096 / 2048 vertices at FP32 /
-# FP64 on 2^16 vertices), which the default 60 MiB bands only use above 15.7M
-# vertices (FP32); compact_kib=1: the pull over compact source positions, used by
-# default above 32 MiB of contributions (8.4M vertices in FP32)
+# FP64 on 2^16 vertices, over compact positions in id order when compact_kib=1);
+# compact_kib=1 alone: the pull over compact source positions by out-degree, used
+# by default above 32 MiB of contributions (8.4M vertices in FP32)
 @pytest.mark.parametrize("scale,band_kib,compact_kib", [(12, 0, 0), (16, 0, 0), (16, 16, 0), (16, 0, 1), (16, 16, 1)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_pagerank_vs_fp64_oracle(gpu, scale, band_kib, compact_kib, prec):
