@@ -703,6 +703,32 @@ __global__ void k_deg_pos(uint32_t n, const uint32_t* __restrict__ key, const ui
   if (i < n) pos[id[i]] = key[i] != 0xFFFFFFFFu ? i : INF32;
 }
 
+void build_positions_by_degree(suminc_ctx* c, const uint32_t* deg_off, const uint32_t* rows_off,
+                               const uint32_t* cols, uint64_t m_cap, DevBuf& pos, DevBuf& scratch, DevBuf& cols_c) {
+  const uint32_t n = c->n;
+  pos.ensure((size_t)n * 4 + 4);
+  scratch.ensure((size_t)n * 16 + 16);
+  cols_c.ensure_slack(m_cap * 4 + 4);
+  uint32_t* k1 = scratch.as<uint32_t>();
+  uint32_t* i1 = k1 + n;
+  uint32_t* k2 = i1 + n;
+  uint32_t* i2 = k2 + n;
+  size_t bytes = 0;
+  k_deg_keys<<<grid_for(n), kBlock, 0, c->stream>>>(n, deg_off, k1, i1);
+  SUMINC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, k1, k2, i1, i2, (size_t)n, 0, 32, c->stream));
+  c->tmp.ensure(bytes);
+  SUMINC_HIP(rocprim::radix_sort_pairs(c->tmp.p, bytes, k1, k2, i1, i2, (size_t)n, 0, 32, c->stream));
+  k_deg_pos<<<grid_for(n), kBlock, 0, c->stream>>>(n, k2, i2, pos.as<uint32_t>());
+  k_col_pos<<<grid_for(m_cap, kBlock, 65536), kBlock, 0, c->stream>>>(n, rows_off, cols, pos.as<uint32_t>(),
+                                                                      cols_c.as<uint32_t>());
+  SUMINC_HIP(hipGetLastError());
+}
+
+bool compact_pull(const suminc_ctx* c, uint64_t es) {
+  const int kib = c->prm.pull_compact_kib;
+  return kib >= 0 && (uint64_t)c->n * es > (uint64_t)(kib ? kib : 32768) * 1024;
+}
+
 // by_degree: positions by decreasing out-degree, so the most gathered
 // contributions share cache lines (measured on MI355X at 2^26 vertices: 9.0 ms per
 // round, against 9.8 ms for id order with source bands and 12.2 ms without);
@@ -712,35 +738,26 @@ static void build_src_positions(suminc_ctx* c, uint64_t m_cap, bool by_degree) {
   const uint32_t n = c->n;
   // measured on MI355X: at 2^22 vertices (16 MiB of FP32 contributions) renaming
   // costs more per batch than it saves; from 2^24 on it pays
-  const uint64_t es = c->pr_prec == SUMINC_PR_FP64 ? 8 : 4;
-  const int kib = c->prm.pull_compact_kib;
-  c->pull_compact = kib >= 0 && (uint64_t)n * es > (uint64_t)(kib ? kib : 32768) * 1024;
+  c->pull_compact = compact_pull(c, c->pr_prec == SUMINC_PR_FP64 ? 8 : 4);
   if (!c->pull_compact) return;
+  if (by_degree) {
+    build_positions_by_degree(c, c->out_off.as<uint32_t>(), c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
+                              m_cap, c->src_pos, c->src_pre, c->in_col_c);
+    return;
+  }
   c->src_pos.ensure((size_t)n * 4 + 4);
-  c->src_pre.ensure((size_t)n * 16 + 16);  // id order: flags + prefix; degree order: keys + ids, sorted pairs
+  c->src_pre.ensure((size_t)n * 16 + 16);  // flags, then their exclusive prefix (pre[n] = n_src)
   c->in_col_c.ensure_slack(m_cap * 4 + 4);
   uint32_t* flag = c->src_pre.as<uint32_t>();
   uint32_t* pre = flag + n + 1;
   size_t bytes = 0;
-  if (by_degree) {
-    uint32_t* k1 = c->src_pre.as<uint32_t>();
-    uint32_t* i1 = k1 + n;
-    uint32_t* k2 = i1 + n;
-    uint32_t* i2 = k2 + n;
-    k_deg_keys<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), k1, i1);
-    SUMINC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, k1, k2, i1, i2, (size_t)n, 0, 32, c->stream));
-    c->tmp.ensure(bytes);
-    SUMINC_HIP(rocprim::radix_sort_pairs(c->tmp.p, bytes, k1, k2, i1, i2, (size_t)n, 0, 32, c->stream));
-    k_deg_pos<<<grid_for(n), kBlock, 0, c->stream>>>(n, k2, i2, c->src_pos.as<uint32_t>());
-  } else {
-    k_src_flags<<<grid_for((uint64_t)n + 1), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), flag);
-    SUMINC_HIP(rocprim::exclusive_scan(nullptr, bytes, flag, pre, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(),
-                                       c->stream));
-    c->tmp.ensure(bytes);
-    SUMINC_HIP(rocprim::exclusive_scan(c->tmp.p, bytes, flag, pre, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(),
-                                       c->stream));
-    k_src_pos<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), pre, c->src_pos.as<uint32_t>());
-  }
+  k_src_flags<<<grid_for((uint64_t)n + 1), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), flag);
+  SUMINC_HIP(rocprim::exclusive_scan(nullptr, bytes, flag, pre, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(),
+                                     c->stream));
+  c->tmp.ensure(bytes);
+  SUMINC_HIP(rocprim::exclusive_scan(c->tmp.p, bytes, flag, pre, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(),
+                                     c->stream));
+  k_src_pos<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), pre, c->src_pos.as<uint32_t>());
   k_col_pos<<<grid_for(m_cap, kBlock, 65536), kBlock, 0, c->stream>>>(n, c->in_off.as<uint32_t>(),
                                                                       c->in_col.as<uint32_t>(),
                                                                       c->src_pos.as<uint32_t>(),

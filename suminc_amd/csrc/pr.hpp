@@ -19,6 +19,13 @@ void pr_step(suminc_ctx* c, double* residual);
 void pr_download(suminc_ctx* c, double* rank, double* delta);
 void pr_upload(suminc_ctx* c, const double* rank, const double* delta);
 unsigned push_grid();
+// positions of the vertices by decreasing degree (deg_off's row lengths; degree 0:
+// INF32) and cols (rows rows_off, up to m_cap entries) renamed to them; scratch
+// holds 16 B per vertex
+void build_positions_by_degree(suminc_ctx* c, const uint32_t* deg_off, const uint32_t* rows_off,
+                               const uint32_t* cols, uint64_t m_cap, DevBuf& pos, DevBuf& scratch, DevBuf& cols_c);
+// whether n contributions of es bytes are large enough for compact positions
+bool compact_pull(const suminc_ctx* c, uint64_t es);
 // the plain pull's contribution index (nullptr: vertex ids) and CSC columns
 const uint32_t* pull_pos(suminc_ctx* c);
 const uint32_t* pull_cols(suminc_ctx* c);

@@ -123,7 +123,8 @@ __device__ __forceinline__ void kahan(T* sum, float* comp, uint32_t v, T x) {
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_wpr_step(uint32_t n, T* __restrict__ x, T* __restrict__ y,
                                                      T* __restrict__ sum, float* __restrict__ comp,
-                                                     double* __restrict__ partial, const Counters* cnt) {
+                                                     double* __restrict__ partial, const Counters* cnt,
+                                                     const uint32_t* __restrict__ pos, T* __restrict__ xc) {
   __shared__ double s_p[kBlock / 64];
   if (cnt->done) return;
   const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
@@ -133,6 +134,7 @@ __global__ void __launch_bounds__(kBlock) k_wpr_step(uint32_t n, T* __restrict__
     if (xo != T(0)) kahan(sum, comp, v, xo);
     x[v] = xn;
     y[v] = T(0);
+    if (pos && pos[v] != INF32) xc[pos[v]] = xn;  // the next round gathers the compact copy
     a = fabs((double)xn);
   }
 #pragma unroll
@@ -172,13 +174,15 @@ __global__ void __launch_bounds__(1024) k_wpr_decide(const double* __restrict__ 
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_wpr_seed(uint32_t n, const float* __restrict__ src, T scale,
                                                      T* __restrict__ x, T* __restrict__ y, T* __restrict__ sum,
-                                                     float* __restrict__ comp, double* __restrict__ partial) {
+                                                     float* __restrict__ comp, double* __restrict__ partial,
+                                                     const uint32_t* __restrict__ pos, T* __restrict__ xc) {
   __shared__ double s_p[kBlock / 64];
   const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
   double a = 0;
   if (v < n) {
     const T s = scale * (T)src[v];
     x[v] = s;
+    if (pos && pos[v] != INF32) xc[pos[v]] = s;
     y[v] = T(0);
     sum[v] = T(0);
     comp[v] = 0.f;
@@ -323,9 +327,14 @@ __global__ void k_to_f64(uint32_t n, const T* __restrict__ src, double* __restri
 // ---------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------
+// Row lists of one direction and, on large graphs, compact positions of the
+// columns (by decreasing number of appearances) with the columns renamed to them,
+// so a round gathers from an array of the vertices that can be gathered at all.
 struct WprDir {
   DevBuf shrt, wave, heavy, counts;
   uint64_t n_short = 0, n_wave = 0, n_heavy = 0;
+  bool compact = false;
+  DevBuf pos, scratch, colc;
 };
 
 static void build_lists(suminc_ctx* c, WprDir& L, const DevBuf& off) {
@@ -358,6 +367,7 @@ struct WprState {
   DevBuf s;                          // row sums (f64)
   DevBuf a_out, a_in;                // coefficients d w / s (T) in CSR / CSC order
   DevBuf x, y, sum, comp;            // round state (T)
+  DevBuf xc;                         // x at the compact positions (large graphs)
   DevBuf r64, lam64, yl64;           // FP64 results of the last forward / backward
   DevBuf l1;                         // sum |seed|
 };
@@ -378,6 +388,16 @@ static void wpr_prepare_graph(suminc_ctx* c, WprState& S) {
   const uint64_t m = c->m;
   build_lists(c, S.out, c->out_off);
   build_lists(c, S.in, c->in_off);
+  // the forward gathers over the CSC (columns: sources, counted by out-degree), the
+  // adjoint over the CSR (columns: destinations, counted by in-degree)
+  const bool compact = compact_pull(c, c->prm.pr_precision == SUMINC_PR_FP64 ? 8 : 4);
+  S.in.compact = S.out.compact = compact && m > 0;
+  if (S.in.compact) {
+    build_positions_by_degree(c, c->out_off.as<uint32_t>(), c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(), m,
+                              S.in.pos, S.in.scratch, S.in.colc);
+    build_positions_by_degree(c, c->in_off.as<uint32_t>(), c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
+                              m, S.out.pos, S.out.scratch, S.out.colc);
+  }
   S.eid.ensure(m * 4 + 4);
   if (m) {
     DevBuf ka, kb, ia;
@@ -421,10 +441,12 @@ static void wpr_prepare_coef(suminc_ctx* c, WprState& S) {
   S.coef_prec = prec;
 }
 
-static RowLists lists_of(const WprDir& L, const DevBuf& off, const DevBuf& col) {
+// compact: the renamed columns when the direction has them (then the gathered
+// vector must be at the compact positions)
+static RowLists lists_of(const WprDir& L, const DevBuf& off, const DevBuf& col, bool compact) {
   RowLists R;
   R.off = off.as<uint32_t>();
-  R.col = col.as<uint32_t>();
+  R.col = compact && L.compact ? L.colc.as<uint32_t>() : col.as<uint32_t>();
   R.shrt = L.shrt.as<uint32_t>();
   R.wave = L.wave.as<uint32_t>();
   R.heavy = L.heavy.as<uint2>();
@@ -445,11 +467,16 @@ static void spmv(suminc_ctx* c, const RowLists& R, const T* a, const T* x, T* y,
 
 // Runs sum = sum_t x_t with x_{t+1} = A x_t from x_0 = scale * seed until
 // sum|x| < tol.  Result (FP64) in out64, optionally FP32 in out32.
+// pos != nullptr: R's columns are compact positions, and the rounds gather the
+// compact copy of x.
 template <class T>
-static void wpr_series(suminc_ctx* c, WprState& S, const RowLists& R, const T* a, const float* seed_dev, T scale,
-                       double tol_abs, bool relative, float* out32, double* out64, suminc_stats* st) {
+static void wpr_series(suminc_ctx* c, WprState& S, const RowLists& R, const uint32_t* pos, const T* a,
+                       const float* seed_dev, T scale, double tol_abs, bool relative, float* out32, double* out64,
+                       suminc_stats* st) {
   const uint32_t n = c->n;
   S.x.ensure((size_t)n * sizeof(T) + 8);
+  if (pos) S.xc.ensure((size_t)n * sizeof(T) + 8);
+  T* xc = pos ? S.xc.as<T>() : nullptr;
   S.y.ensure((size_t)n * sizeof(T) + 8);
   S.sum.ensure((size_t)n * sizeof(T) + 8);
   S.comp.ensure((size_t)n * 4 + 4);
@@ -459,7 +486,7 @@ static void wpr_series(suminc_ctx* c, WprState& S, const RowLists& R, const T* a
   Counters* cnt = dev_counters(c);
   SUMINC_HIP(hipMemsetAsync(cnt, 0, sizeof(Counters), c->stream));
   k_wpr_seed<T><<<g, kBlock, 0, c->stream>>>(n, seed_dev, scale, S.x.as<T>(), S.y.as<T>(), S.sum.as<T>(),
-                                             S.comp.as<float>(), c->partial.as<double>());
+                                             S.comp.as<float>(), c->partial.as<double>(), pos, xc);
   k_sum_partials<<<1, 1024, 0, c->stream>>>(c->partial.as<double>(), g, S.l1.as<double>());
   double tol = tol_abs;
   if (relative) {
@@ -473,9 +500,9 @@ static void wpr_series(suminc_ctx* c, WprState& S, const RowLists& R, const T* a
   int group = 8;
   while (true) {
     for (int i = 0; i < group; ++i) {
-      spmv<T>(c, R, a, S.x.as<T>(), S.y.as<T>(), cnt);
+      spmv<T>(c, R, a, pos ? xc : S.x.as<T>(), S.y.as<T>(), cnt);
       k_wpr_step<T><<<g, kBlock, 0, c->stream>>>(n, S.x.as<T>(), S.y.as<T>(), S.sum.as<T>(), S.comp.as<float>(),
-                                                 c->partial.as<double>(), cnt);
+                                                 c->partial.as<double>(), cnt, pos, xc);
       k_wpr_decide<<<1, 1024, 0, c->stream>>>(c->partial.as<double>(), g, cnt, tol, max_rounds, c->m);
     }
     SUMINC_HIP(hipGetLastError());
@@ -534,8 +561,9 @@ static void wpr_forward_t(suminc_ctx* c, const float* p, float* r, suminc_stats*
   SUMINC_HIP(hipMemcpyAsync(pd.p, p, (size_t)n * 4, hipMemcpyDefault, c->stream));
   S.r64.ensure((size_t)n * 8 + 8);
   r32.ensure((size_t)n * 4 + 4);
-  const RowLists R = lists_of(S.in, c->in_off, c->in_col);
-  wpr_series<T>(c, S, R, S.a_in.as<T>(), pd.as<float>(), (T)(1.0 - c->prm.damping), c->prm.termcheck_threshold,
+  const RowLists R = lists_of(S.in, c->in_off, c->in_col, true);
+  const uint32_t* pos = S.in.compact ? S.in.pos.as<uint32_t>() : nullptr;
+  wpr_series<T>(c, S, R, pos, S.a_in.as<T>(), pd.as<float>(), (T)(1.0 - c->prm.damping), c->prm.termcheck_threshold,
                 false, r32.as<float>(), S.r64.as<double>(), st);
   if (r) SUMINC_HIP(hipMemcpyAsync(r, r32.p, (size_t)n * 4, hipMemcpyDefault, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
@@ -556,8 +584,9 @@ static void wpr_backward_t(suminc_ctx* c, const float* cvec, float* grad_p, floa
   SUMINC_HIP(hipMemcpyAsync(cd.p, cvec, (size_t)n * 4, hipMemcpyDefault, c->stream));
   S.lam64.ensure((size_t)n * 8 + 8);
   S.yl64.ensure((size_t)n * 8 + 8);
-  const RowLists R = lists_of(S.out, c->out_off, c->out_col);
-  wpr_series<T>(c, S, R, S.a_out.as<T>(), cd.as<float>(), (T)1, c->prm.termcheck_threshold, true, nullptr,
+  const RowLists R = lists_of(S.out, c->out_off, c->out_col, true);
+  const uint32_t* pos = S.out.compact ? S.out.pos.as<uint32_t>() : nullptr;
+  wpr_series<T>(c, S, R, pos, S.a_out.as<T>(), cd.as<float>(), (T)1, c->prm.termcheck_threshold, true, nullptr,
                 S.lam64.as<double>(), st);
   // y_u = sum_k a_uk lambda_k (one more product, in FP64)
   DevBuf lam_t, y_t;
@@ -565,7 +594,7 @@ static void wpr_backward_t(suminc_ctx* c, const float* cvec, float* grad_p, floa
   y_t.ensure((size_t)n * sizeof(T) + 8);
   SUMINC_HIP(hipMemsetAsync(y_t.p, 0, (size_t)n * sizeof(T), c->stream));
   k_to_t<T><<<grid_for(n), kBlock, 0, c->stream>>>(n, S.lam64.as<double>(), lam_t.as<T>());
-  spmv<T>(c, R, S.a_out.as<T>(), lam_t.as<T>(), y_t.as<T>(), nullptr);
+  spmv<T>(c, lists_of(S.out, c->out_off, c->out_col, false), S.a_out.as<T>(), lam_t.as<T>(), y_t.as<T>(), nullptr);
   DevBuf yd;
   yd.ensure((size_t)n * 8 + 8);
   k_to_f64<T><<<grid_for(n), kBlock, 0, c->stream>>>(n, y_t.as<T>(), yd.as<double>());
