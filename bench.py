@@ -55,6 +55,10 @@ WORKLOADS = {
                  desc="incremental FP32/Kahan delta-PageRank, R-MAT s22 ef16 seed3, d=0.85f, 1% batches"),
     "pr26": dict(kind="pr", scale=26, seed=5, frac=0.01, T=1e-7, max_node=1024,
                  desc="incremental FP32/Kahan delta-PageRank, R-MAT s26 ef16 seed5, d=0.85f, 1% batches"),
+    "wpr22": dict(kind="wpr", scale=22, seed=4, seeds=1024, T=1e-9, fd_edges=32, h=1e-3,
+                  desc="differentiable weighted PageRank, R-MAT s22 ef16 seed4, FP32 edge weights U(0,1] "
+                       "row-normalized, personalization Dirichlet(1) over 1024 random seeds, loss sum c_i r_i "
+                       "with c ~ N(0,1); a step = forward + adjoint (gradients w.r.t. weights and p)"),
     "web26": dict(kind="web", scale=26, seed=5, wmin=1, wmax=255, frac=0.01, T=1e-7, source=0, max_node=1024,
                   adjoint_seeds=1024,
                   desc="web-like incremental run, R-MAT s26 ef16 seed5, successive 1% batches; per batch "
@@ -493,6 +497,104 @@ def run_web(args, wl):
     return gteps, ms_per_step, "f32+u32", roof, cpu, extra
 
 
+def run_wpr(args, wl):
+    """Forward + adjoint of the weighted personalized PageRank, timed per step;
+    checked against an FP64 run of both on the same GPU and against central
+    differences of the FP64 forward on random edges."""
+    import oracle
+    import suminc_amd
+
+    rng = np.random.default_rng(wl["seed"])
+    ctx = suminc_amd.Context(termcheck_threshold=wl["T"], pr_precision=suminc_amd.PR_FP32_KAHAN,
+                             max_rounds=1_000_000)
+    t0 = time.time()
+    ctx.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"], weights="f32")
+    n, m, _ = ctx.graph_info()
+    log(f"graph: n={n} m={m} ({time.time() - t0:.1f}s)")
+    p = np.zeros(n, np.float32)
+    seeds = rng.choice(n, wl["seeds"], replace=False)
+    p[seeds] = rng.dirichlet(np.ones(wl["seeds"])).astype(np.float32)
+    cvec = rng.standard_normal(n, dtype=np.float32)
+    for _ in range(args.warmup):
+        ctx.wpr_forward(p)
+        ctx.wpr_backward(cvec)
+    ctx.sync()
+    edges, kms, rounds = 0.0, 0.0, 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, sf = ctx.wpr_forward(p)
+        gp, gw, sb = ctx.wpr_backward(cvec)
+        edges += sf["edges_traversed"] + sb["edges_traversed"]
+        kms += sf["kernel_ms"] + sb["kernel_ms"]
+        rounds += sf["rounds"] + sb["rounds"]
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    ms_per_step = 1e3 * dt / args.steps
+    gteps = edges / dt / 1e9
+    log(f"forward+adjoint: {ms_per_step:.3f} ms/step ({rounds / args.steps:.0f} rounds), {gteps:.3f} GTEPS")
+    # roofline of the series rounds (row walk, coefficient and gathered value per
+    # edge; previous value, new value and the compensated sum per vertex)
+    es = 4.0
+    alg = rounds * (m * (4.0 + 2 * es) + n * (4 * es + 4.0))
+    achieved = alg / (kms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_spmv_* + k_wpr_step (series rounds)", "avg_launch_us": round(1e3 * kms / rounds, 2),
+            "alg_bytes_per_launch": round(alg / rounds), "kernel_ms_total": round(kms, 3)}
+    r32, _ = ctx.wpr_download()
+    off, col, w = ctx.download_csr()
+    ctx.close()
+    extra = {"rounds_per_step": rounds / args.steps}
+    if not args.no_verify:
+        h = wl["h"]
+        with suminc_amd.Context(termcheck_threshold=1e-15, pr_precision=suminc_amd.PR_FP64,
+                                max_rounds=1_000_000) as c64:
+            c64.upload_csr(off, col, w)
+            c64.wpr_forward(p)
+            r64, _ = c64.wpr_download()
+            gp64, gw64, _ = c64.wpr_backward(cvec)
+            extra["forward_l1_vs_fp64"] = float(np.abs(r32 - r64).sum())
+            extra["grad_w_max_rel_vs_fp64"] = float(np.abs(gw - gw64).max() / np.abs(gw64).max())
+            extra["grad_p_max_rel_vs_fp64"] = float(np.abs(gp - gp64).max() / np.abs(gp64).max())
+            edges_fd = rng.choice(m, wl["fd_edges"], replace=False)
+            fd = []
+            t1 = time.time()
+            for e in edges_fd:
+                w0 = w[e]
+                w[e] = w0 + np.float32(h)
+                hp = np.float64(w[e])
+                c64.wpr_set_weights(w)
+                c64.wpr_forward(p)
+                l1 = cvec.astype(np.float64) @ c64.wpr_download()[0]
+                w[e] = w0 - np.float32(h)
+                hm = np.float64(w[e])
+                c64.wpr_set_weights(w)
+                c64.wpr_forward(p)
+                l2 = cvec.astype(np.float64) @ c64.wpr_download()[0]
+                w[e] = w0
+                fd.append((l1 - l2) / (hp - hm))
+            fd = np.array(fd)
+            ga = gw[edges_fd].astype(np.float64)
+            rel = np.abs(fd - ga) / np.maximum(np.abs(fd), 1e-2 * np.abs(fd).max())
+            extra["fd_check"] = {"edges": int(wl["fd_edges"]), "h": h, "max_rel_err": float(rel.max()),
+                                 "within_1e-4": bool(rel.max() <= 1e-4),
+                                 "reference": "central differences of the FP64 forward on the same GPU"}
+            log(f"verify: {extra} ({time.time() - t1:.1f}s)")
+    cpu = None
+    if not args.no_cpu:
+        g = oracle.Graph.from_csr(off, col, w)
+        cores = min(host_cores(), 16)
+        oracle.set_threads(cores)
+        t1 = time.perf_counter()
+        _, ro = g.wpr_forward(w, p.astype(np.float64), DAMPING, tol=1e-30, max_rounds=10)
+        tc = time.perf_counter() - t1
+        cpu = {"value": round(ro * m / tc / 1e9, 4), "unit": "GTEPS", "cores": cores, "kind": "port",
+               "sample": f"oracle weighted-PageRank series (FP64, OpenMP) on the same s{wl['scale']} graph, "
+                         f"first {ro} rounds: {ro * m:.3e} edges in {tc:.2f}s"}
+        log(f"cpu baseline: {cpu}")
+    return gteps, ms_per_step, "f32", roof, cpu, extra
+
+
 def calibrate_copy(nbytes):
     """One device-to-device copy of nbytes (the runtime's copy kernel), so that the
     FETCH_SIZE / WRITE_SIZE counters can be read against a known byte count."""
@@ -511,6 +613,16 @@ def calibrate_copy(nbytes):
     ctx._check(lib.suminc_device_free(ctx._h, b))
     ctx.close()
     log(f"copied {nbytes} bytes device to device, 3 times")
+
+
+def supernodes_note(args, wl):
+    if wl["kind"] == "wpr":
+        return "none (no batches, no sketch)"
+    if wl["kind"] == "web":
+        return "SSSP over supernodes (leaf clusters <= %d members), PageRank on the plain CSR" % wl["max_node"]
+    if args.supernodes == "on" and wl.get("max_node"):
+        return "on (leaf clusters <= %d members, compress threshold 1)" % wl["max_node"]
+    return "off (plain CSR path)"
 
 
 def main():
@@ -542,6 +654,8 @@ def main():
         value, ms, dtype, roof, cpu, extra = run_sssp(args, wl)
     elif wl["kind"] == "web":
         value, ms, dtype, roof, cpu, extra = run_web(args, wl)
+    elif wl["kind"] == "wpr":
+        value, ms, dtype, roof, cpu, extra = run_wpr(args, wl)
     else:
         value, ms, dtype, roof, cpu, extra = run_pr(args, wl)
     out = {
@@ -556,12 +670,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": dtype,
-        "data": "synthetic (seeded R-MAT + seeded update batches, generated on the device)",
-        "config": {"workload": wl["desc"], "scale": wl["scale"], "edge_factor": 16, "batch_fraction": wl["frac"],
-                   "supernodes": (("on (leaf clusters <= %d members, compress threshold 1)" % wl["max_node"])
-                                  if args.supernodes == "on" and wl.get("max_node") else "off (plain CSR path)")
-                   if wl["kind"] != "web" else "SSSP over supernodes (leaf clusters <= %d members), PageRank on "
-                                               "the plain CSR" % wl["max_node"]},
+        "data": "synthetic (seeded R-MAT" + (", seeded weights and loss vectors" if wl["kind"] == "wpr" else
+                                            " + seeded update batches") + ", generated on the device)",
+        "config": {"workload": wl["desc"], "scale": wl["scale"], "edge_factor": 16,
+                   "batch_fraction": wl.get("frac"), "supernodes": supernodes_note(args, wl)},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
