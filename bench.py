@@ -55,6 +55,9 @@ WORKLOADS = {
                  desc="incremental FP32/Kahan delta-PageRank, R-MAT s22 ef16 seed3, d=0.85f, 1% batches"),
     "pr26": dict(kind="pr", scale=26, seed=5, frac=0.01, T=1e-7, max_node=1024,
                  desc="incremental FP32/Kahan delta-PageRank, R-MAT s26 ef16 seed5, d=0.85f, 1% batches"),
+    "pr16": dict(kind="prfull", scale=16, seed=1, eps=1e-9,
+                 desc="plain delta-PageRank from scratch, R-MAT s16 ef16 seed1, d=0.85f, per-vertex threshold 1e-9 "
+                      "(T = 1e-9 N), FP64, no compression, no updates (the reference's CPU case)"),
     "wpr22": dict(kind="wpr", scale=22, seed=4, seeds=1024, T=1e-9, fd_edges=32, h=1e-3,
                   desc="differentiable weighted PageRank, R-MAT s22 ef16 seed4, FP32 edge weights U(0,1] "
                        "row-normalized, personalization Dirichlet(1) over 1024 random seeds, loss sum c_i r_i "
@@ -497,6 +500,60 @@ def run_web(args, wl):
     return gteps, ms_per_step, "f32+u32", roof, cpu, extra
 
 
+def run_prfull(args, wl):
+    """The reference's CPU-runnable case on the device: plain FP64 delta-PageRank
+    from scratch, each step a full run from the initial state; the oracle's
+    restatement of the reference's CPU loop is timed on the same graph."""
+    import oracle
+    import suminc_amd
+    import suminc_amd._lib as L
+
+    g = oracle.Graph.rmat(wl["scale"], 16, seed=wl["seed"])
+    off, col, _ = g.csr()
+    T = wl["eps"] * g.n
+    ctx = suminc_amd.Context(termcheck_threshold=T, pr_precision=suminc_amd.PR_FP64, max_rounds=1_000_000)
+    ctx.upload_csr(off, col)
+    for _ in range(args.warmup):
+        ctx.pr_init()
+        ctx.pr_run()
+    ctx.sync()
+    ctx.profile_enable(True)
+    edges = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.pr_init()
+        st = ctx.pr_run()
+        edges += st["edges_traversed"]
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    kids = [L.KERNEL_PR_PUSH, L.KERNEL_PR_PULL, L.KERNEL_PR_GATHER]
+    kernel, roof = roofline(ctx, kids, args.workload)
+    extra = {"kernels": kernel_table(ctx, kids), "rounds": st["rounds"], "termcheck_threshold": T}
+    ctx.profile_enable(False)
+    r, _ = ctx.pr_download()
+    ctx.close()
+    ms_per_step = 1e3 * dt / args.steps
+    log(f"full run: {ms_per_step:.3f} ms ({st['rounds']:.0f} rounds)")
+    if not args.no_verify:
+        rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
+        extra["l1_vs_fp64_fixed_point"] = float(np.abs(r - rs).sum())
+        extra["l1_bound"] = st["residual"] / (1 - DAMPING)
+    cpu = None
+    if not args.no_cpu:
+        cores = min(host_cores(), 16)
+        oracle.set_threads(cores)
+        t1 = time.perf_counter()
+        ro, _, sto = g.pr_ingress(DAMPING, T, max_rounds=100000)
+        tc = time.perf_counter() - t1
+        cpu = {"value": round(sto["edges"] / tc / 1e9, 4), "unit": "GTEPS", "cores": cores, "kind": "port",
+               "ms_per_run": round(1e3 * tc, 3),
+               "sample": f"oracle delta-PageRank Ingress (reference CPU loop restated, FP64) to the same stop rule: "
+                         f"{sto['rounds']:.0f} rounds, {sto['edges']:.3e} edges in {tc:.2f}s"}
+        extra["cpu_l1_vs_gpu"] = float(np.abs(ro - r).sum())
+        log(f"cpu baseline: {cpu}")
+    return edges / dt / 1e9, ms_per_step, "f64", roof, cpu, extra
+
+
 def run_wpr(args, wl):
     """Forward + adjoint of the weighted personalized PageRank, timed per step;
     checked against an FP64 run of both on the same GPU and against central
@@ -616,7 +673,7 @@ def calibrate_copy(nbytes):
 
 
 def supernodes_note(args, wl):
-    if wl["kind"] == "wpr":
+    if wl["kind"] in ("wpr", "prfull"):
         return "none (no batches, no sketch)"
     if wl["kind"] == "web":
         return "SSSP over supernodes (leaf clusters <= %d members), PageRank on the plain CSR" % wl["max_node"]
@@ -656,6 +713,8 @@ def main():
         value, ms, dtype, roof, cpu, extra = run_web(args, wl)
     elif wl["kind"] == "wpr":
         value, ms, dtype, roof, cpu, extra = run_wpr(args, wl)
+    elif wl["kind"] == "prfull":
+        value, ms, dtype, roof, cpu, extra = run_prfull(args, wl)
     else:
         value, ms, dtype, roof, cpu, extra = run_pr(args, wl)
     out = {
@@ -670,8 +729,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": dtype,
-        "data": "synthetic (seeded R-MAT" + (", seeded weights and loss vectors" if wl["kind"] == "wpr" else
-                                            " + seeded update batches") + ", generated on the device)",
+        "data": {"wpr": "synthetic (seeded R-MAT with seeded weights, generated on the device; seeded "
+                        "personalization and loss vectors)",
+                 "prfull": "synthetic (seeded R-MAT from the host generator, uploaded)"}.get(
+            wl["kind"], "synthetic (seeded R-MAT + seeded update batches, generated on the device)"),
         "config": {"workload": wl["desc"], "scale": wl["scale"], "edge_factor": 16,
                    "batch_fraction": wl.get("frac"), "supernodes": supernodes_note(args, wl)},
         "roofline": roof,
