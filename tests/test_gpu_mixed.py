@@ -78,3 +78,33 @@ def test_two_batches_before_the_sssp_query(gpu):
         ctx.sssp_run()
         check_sssp(ctx, gcur, 0)
         check_pr(ctx, gcur, st)
+
+
+def test_pagerank_after_sssp_only_batches(gpu):
+    """Batches on an SSSP-only context leave the CSC stale; a PageRank state made
+    afterwards rebuilds it, and its next batches merge it again."""
+    g = oracle.Graph.rmat(13, 16, seed=7, weights="u32", wmin=1, wmax=64)
+    off, col, w = g.csr()
+    with gpu.Context(termcheck_threshold=1e-9) as ctx:
+        ctx.upload_csr(off, col, w)
+        ctx.sssp_init(0)
+        ctx.sssp_run()
+        gcur = g
+        n_ops = len(col) // 100
+        for b in range(2):
+            ops = gcur.gen_batch(seed=7, batch_id=b, n_del=n_ops // 2, n_ins=n_ops - n_ops // 2, wmin=1, wmax=64)
+            gcur, _ = gcur.apply_batch(ops)
+            ctx.apply_batch(ops)
+            ctx.sssp_run()
+            check_sssp(ctx, gcur, 0)
+        ctx.pr_init()
+        st = ctx.pr_run()
+        check_pr(ctx, gcur, st)
+        for b in range(2, 4):
+            ops = gcur.gen_batch(seed=7, batch_id=b, n_del=n_ops // 2, n_ins=n_ops - n_ops // 2, wmin=1, wmax=64)
+            gcur, _ = gcur.apply_batch(ops)
+            ctx.apply_batch(ops)
+            st = ctx.pr_run()
+            ctx.sssp_run()
+            check_sssp(ctx, gcur, 0)
+            check_pr(ctx, gcur, st)
