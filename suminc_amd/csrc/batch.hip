@@ -16,6 +16,10 @@
 // rows get their position from two prefix counts over the row's groups (existing
 // pairs before, adds before), adds are placed symmetrically.  The same kernels
 // run on the transposed groups for the CSC.  Cost O(m + G log).
+//
+// Reseed: each of the two AmendValue passes is a push over the touched sources'
+// rows, or, when their edges exceed m / pull_threshold_div, one pull over the
+// CSC with the touched sources' contributions (chosen on the device).
 #include <cstring>
 #include <rocprim/rocprim.hpp>
 

@@ -18,9 +18,13 @@
 //   k_pr_push    persistent blocks over light tiles (block-cooperative adjacency
 //                walk, coalesced column reads) then heavy chunks; scatter with the
 //                hardware float/double atomic add.
-//   k_pr_pull_*  CSC gather: light tiles of 256 consecutive vertices accumulate
-//                in LDS (ds_add) and add to delta once; vertices with in-degree
-//                > 256 are summed per 2048-edge chunk and added with one atomic.
+//   k_pr_pull_*  CSC gather by in-degree class (lists built once per graph
+//                version): a thread per row up to 32 in-edges, a wave per row up
+//                to 256, above that 2048-edge chunks summed per workgroup and
+//                added with one atomic.  Above 32 MiB of contributions the gather
+//                reads a compact array of the vertices with out-edges, ordered by
+//                decreasing out-degree (CSC columns renamed once per version);
+//                an optional banded variant pulls one source band at a time.
 //   k_pr_end     list counters reset, statistics.
 // Kernels not selected for the round return at entry.
 #include <cstring>
