@@ -13,7 +13,9 @@
 // Frontier: double-buffered vertex lists + bitmaps.  A vertex whose distance
 // drops is appended to the next list once (atomicOr on its bitmap word, then a
 // wave64-aggregated append).  Per round: k_sssp_push (light tiles, heavy
-// vertices deferred to 4096-edge chunks), k_sssp_heavy, k_sssp_end.
+// vertices deferred to 2048-edge chunks), k_sssp_heavy, k_sssp_end.  The lists,
+// bitmaps and the count of pending seeds belong to the SSSP state: batches append
+// seeds, and each query takes them at its start (k_sssp_begin).
 #include <algorithm>
 
 #include "ctx.hpp"
