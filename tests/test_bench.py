@@ -1,0 +1,36 @@
+"""bench.py's command-line contract, checked without a GPU: more than one GPU is
+refused with a message, every workload names its kind and sizes, and the
+per-workload traffic summaries the roofline reads are found."""
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(ROOT))
+
+import bench  # noqa: E402
+
+
+def test_more_than_one_gpu_is_refused():
+    out = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                         timeout=120, cwd=str(ROOT))
+    assert out.returncode == 2
+    assert "one GPU" in out.stderr
+    assert out.stdout == ""
+
+
+def test_workloads_cover_the_configurations():
+    kinds = {w["kind"] for w in bench.WORKLOADS.values()}
+    assert kinds == {"sssp", "pr", "prfull", "wpr", "web"}
+    for name, w in bench.WORKLOADS.items():
+        assert w["scale"] in (16, 22, 24, 26), name
+        assert w["desc"], name
+    # the default workload is the incremental SSSP over supernodes at scale 22
+    assert bench.WORKLOADS["sssp22"]["frac"] == 0.001 and bench.WORKLOADS["sssp22"]["max_node"] == 1024
+
+
+def test_traffic_summaries_are_found():
+    assert bench.measured_traffic("k_copy_old", "sssp22") > 1e9
+    assert bench.measured_traffic("k_pr_pull", "pr22") > 1e9
+    assert bench.measured_traffic("k_pr_pull", "pr26") > 1e10
+    assert bench.measured_traffic("k_pr_pull", "pr16") is None
