@@ -475,11 +475,12 @@ __global__ void __launch_bounds__(kBlock) k_reseed_push(const uint32_t* __restri
   if (cnt->reseed_mode != 0) return;
   const unsigned long long nl = cnt->wl_count[0];
   const unsigned long long nh = cnt->heavy_count;
-  const unsigned long long tiles = (nl + kBlock - 1) / kBlock;
+  const uint32_t vt = frontier_tile<kBlock>(nl);
+  const unsigned long long tiles = (nl + vt - 1) / vt;
   for (unsigned long long t = blockIdx.x; t < tiles; t += gridDim.x) {
-    unsigned long long i = t * kBlock + threadIdx.x;
+    unsigned long long i = t * vt + threadIdx.x;
     uint32_t start = 0, len = 0;
-    if (i < nl) {
+    if (threadIdx.x < vt && i < nl) {
       uint32_t v = wl_v[i];
       start = off[v];
       len = off[v + 1] - start;
@@ -594,11 +595,12 @@ __global__ void __launch_bounds__(kBlock) k_reset_round(const uint32_t* off, con
   }
   if (stop) return;
   const unsigned long long b = S, e = S + C, base = e;
-  const unsigned long long tiles = (e - b + kBlock - 1) / kBlock;
+  const uint32_t vt = frontier_tile<kBlock>(C);
+  const unsigned long long tiles = (C + vt - 1) / vt;
   for (unsigned long long t = blockIdx.x; t < tiles; t += gridDim.x) {
-    unsigned long long i = b + t * kBlock + threadIdx.x;
+    unsigned long long i = b + t * vt + threadIdx.x;
     uint32_t start = 0, len = 0;
-    if (i < e) {
+    if (threadIdx.x < vt && i < e) {
       uint32_t u = rlist[i];
       s_u[threadIdx.x] = u;
       start = off[u];
@@ -659,11 +661,12 @@ __global__ void __launch_bounds__(kBlock) k_reset_pull(const uint32_t* ioff, con
   __shared__ uint32_t s_warp[kBlock / 64 + 1];
   __shared__ uint32_t s_x[kBlock];
   const unsigned long long nr = cnt->aux[4];
-  const unsigned long long tiles = (nr + kBlock - 1) / kBlock;
+  const uint32_t vt = frontier_tile<kBlock>(nr);
+  const unsigned long long tiles = (nr + vt - 1) / vt;
   for (unsigned long long t = blockIdx.x; t < tiles; t += gridDim.x) {
-    unsigned long long i = t * kBlock + threadIdx.x;
+    unsigned long long i = t * vt + threadIdx.x;
     uint32_t start = 0, len = 0;
-    if (i < nr) {
+    if (threadIdx.x < vt && i < nr) {
       uint32_t x = rlist[i];
       s_x[threadIdx.x] = x;
       start = ioff[x];

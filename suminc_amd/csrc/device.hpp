@@ -97,6 +97,19 @@ __device__ __forceinline__ void block_adjacency(uint32_t start, uint32_t len, ui
   __syncthreads();
 }
 
+// Vertices per tile when a grid of workgroups walks the adjacency of a frontier of
+// nl vertices: BLOCK (one per thread) once the frontier fills the grid, fewer
+// below that, down to one vertex per workgroup.  A short frontier then spreads
+// its edges over the whole grid, each lane taking a few (every edge is a chain of
+// dependent column, distance and atomic accesses), instead of queueing a tile's
+// whole adjacency on a few workgroups.  Tile t holds vertices [t * vt, t * vt + vt);
+// thread j < vt takes the j-th.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t frontier_tile(unsigned long long nl) {
+  const unsigned long long per = (nl + gridDim.x - 1) / gridDim.x;
+  return per >= (unsigned long long)BLOCK ? (uint32_t)BLOCK : per ? (uint32_t)per : 1u;
+}
+
 template <class T>
 __device__ __forceinline__ T fabs_t(T x) { return x < T(0) ? -x : x; }
 

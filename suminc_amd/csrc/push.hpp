@@ -26,11 +26,12 @@ __global__ void __launch_bounds__(kBlock) k_pr_push(const uint32_t* __restrict__
   if (cnt->done || cnt->mode != 0) return;
   const unsigned long long nl = cnt->wl_count[0];
   const unsigned long long nh = cnt->heavy_count;
-  const unsigned long long tiles = (nl + kBlock - 1) / kBlock;
+  const uint32_t vt = frontier_tile<kBlock>(nl);
+  const unsigned long long tiles = (nl + vt - 1) / vt;
   for (unsigned long long t = blockIdx.x; t < tiles; t += gridDim.x) {
-    unsigned long long i = t * kBlock + threadIdx.x;
+    unsigned long long i = t * vt + threadIdx.x;
     uint32_t start = 0, len = 0;
-    if (i < nl) {
+    if (threadIdx.x < vt && i < nl) {
       uint32_t v = wl_v[i];
       start = off[v];
       len = off[v + 1] - start;

@@ -512,13 +512,14 @@ __global__ void __launch_bounds__(kBlock) k_ss_push(SsArgs a, uint32_t r, uint32
     }
   }
   if (stop) return;
-  const unsigned long long tiles = (nl + kBlock - 1) / kBlock;
+  const uint32_t vt = frontier_tile<kBlock>(nl);
+  const unsigned long long tiles = (nl + vt - 1) / vt;
   if (threadIdx.x == 0) s_edges = s_sc = 0;
   __syncthreads();
   for (unsigned long long t = blockIdx.x; t < tiles; t += gridDim.x) {
-    const unsigned long long i = t * kBlock + threadIdx.x;
+    const unsigned long long i = t * vt + threadIdx.x;
     uint32_t len = 0;
-    if (i < nl) {
+    if (threadIdx.x < vt && i < nl) {
       const uint32_t u = a.wl[cur][i];
       atomicAnd(&a.bits[cur][u >> 5], ~(1u << (u & 31)));
       const uint32_t du = (uint32_t)(a.dp[u] >> 32);

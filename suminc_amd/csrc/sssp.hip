@@ -65,13 +65,14 @@ __global__ void __launch_bounds__(kBlock) k_sssp_push(SsspArgs a) {
   if (a.cnt->done) return;
   const int cur = (int)(a.cnt->aux[1] & 1), nxt = cur ^ 1;
   const unsigned long long nl = a.cnt->wl_count[cur];
-  const unsigned long long tiles = (nl + kBlock - 1) / kBlock;
+  const uint32_t vt = frontier_tile<kBlock>(nl);
+  const unsigned long long tiles = (nl + vt - 1) / vt;
   if (threadIdx.x == 0) s_edges = 0;
   __syncthreads();
   for (unsigned long long t = blockIdx.x; t < tiles; t += gridDim.x) {
-    unsigned long long i = t * kBlock + threadIdx.x;
+    unsigned long long i = t * vt + threadIdx.x;
     uint32_t start = 0, len = 0;
-    if (i < nl) {
+    if (threadIdx.x < vt && i < nl) {
       uint32_t u = a.wl[cur][i];
       atomicAnd(&a.bits[cur][u >> 5], ~(1u << (u & 31)));
       uint32_t du = (uint32_t)(a.dp[u] >> 32);
