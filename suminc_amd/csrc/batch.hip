@@ -538,21 +538,29 @@ __global__ void k_sssp_roots(const uint32_t* grow, const uint32_t* gcol, const u
 // Rows longer than kHeavyDeg are not walked by the tile that holds them: they are
 // cut into kChunk-edge chunks (heavy list) that the *_heavy kernels spread over
 // all workgroups, so a hub does not serialise one workgroup.
-__device__ __forceinline__ void reset_child(const unsigned long long* dp, uint32_t u, uint32_t x, uint32_t* rbits,
-                                            uint32_t* rlist, unsigned long long base, unsigned long long* ctr) {
+__device__ __forceinline__ void heavy_chunks(uint32_t v, uint32_t len, uint2* heavy, unsigned long long* ctr) {
+  const uint32_t nch = (len + kChunk - 1) / kChunk;
+  const unsigned long long slot = atomicAdd(ctr, (unsigned long long)nch);
+  for (uint32_t k = 0; k < nch; ++k) heavy[slot + k] = make_uint2(v, k);
+}
+
+// x joins the walk when its parent is u.  A joining vertex with a long row is
+// also cut into chunks at once, walked by the next round beside the light
+// rows of the next frontier (not one round later, after the frontier has met it).
+__device__ __forceinline__ void reset_child(const uint32_t* off, const unsigned long long* dp, uint32_t u, uint32_t x,
+                                            uint32_t* rbits, uint32_t* rlist, unsigned long long base,
+                                            unsigned long long* ctr, uint2* hnew, unsigned long long* hcount) {
   bool add = false;
   if (x != u && (uint32_t)dp[x] == u && (uint32_t)(dp[x] >> 32) != INF32) {
     uint32_t bit = 1u << (x & 31);
     add = !(atomicOr(&rbits[x >> 5], bit) & bit);
   }
   unsigned long long slot = wave_append(add, ctr);
-  if (add) rlist[base + slot] = x;
-}
-
-__device__ __forceinline__ void heavy_chunks(uint32_t v, uint32_t len, uint2* heavy, unsigned long long* ctr) {
-  const uint32_t nch = (len + kChunk - 1) / kChunk;
-  const unsigned long long slot = atomicAdd(ctr, (unsigned long long)nch);
-  for (uint32_t k = 0; k < nch; ++k) heavy[slot + k] = make_uint2(v, k);
+  if (add) {
+    rlist[base + slot] = x;
+    const uint32_t len = off[x + 1] - off[x];
+    if (len > kHeavyDeg) heavy_chunks(x, len, hnew, hcount);
+  }
 }
 
 // The walk's rounds are numbered on the host, so no kernel closes a round.  rlist
@@ -561,8 +569,10 @@ __device__ __forceinline__ void heavy_chunks(uint32_t v, uint32_t len, uint2* he
 // and C_r the count of round r - 1's appends, rc[r % 3] (C_0 = aux[4]); round r
 // appends at S_r + C_r, counting in rc[(r + 1) % 3].  Rows longer than kHeavyDeg
 // become kChunk-edge chunks walked by the next round (the walk is a reachability
-// closure, so the order does not matter): round r writes chunks to heavy half
-// r % 2 counted by hc[r % 3] and walks the previous round's (hc[(r - 1) % 3]).  Its
+// closure, so the order does not matter): round 0 cuts its heavy roots, later
+// rounds cut each heavy vertex as it joins (reset_child) and skip it in the
+// frontier; round r writes chunks to heavy half r % 2 counted by hc[r % 3] and
+// walks the previous round's (hc[(r - 1) % 3]).  Its
 // first workgroup records S_{r+1}, clears rc[(r + 2) % 3] and hc[(r + 1) % 3] (last
 // read by round r - 1), and when nothing is left sets done, the total in aux[4]
 // and the depth in aux[9].
@@ -606,12 +616,12 @@ __global__ void __launch_bounds__(kBlock) k_reset_round(const uint32_t* off, con
       start = off[u];
       len = off[u + 1] - start;
       if (len > kHeavyDeg) {
-        heavy_chunks(u, len, hnew, hcount);
+        if (r == 0) heavy_chunks(u, len, hnew, hcount);  // later rounds: cut when it joined
         len = 0;
       }
     }
     block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t, uint32_t k) {
-      reset_child(dp, s_u[slot], colv[k], rbits, rlist, base, out);
+      reset_child(off, dp, s_u[slot], colv[k], rbits, rlist, base, out, hnew, hcount);
     });
   }
   // the previous round's heavy chunks
@@ -623,7 +633,7 @@ __global__ void __launch_bounds__(kBlock) k_reset_round(const uint32_t* off, con
     for (uint32_t k0 = cb; k0 < ce; k0 += kBlock) {  // same trip count for every lane (wave ballots)
       const uint32_t k = k0 + threadIdx.x;
       if (k < ce) {
-        reset_child(dp, u, colv[k], rbits, rlist, base, out);
+        reset_child(off, dp, u, colv[k], rbits, rlist, base, out, hnew, hcount);
       } else {
         unsigned long long unused = wave_append(false, out);
         (void)unused;

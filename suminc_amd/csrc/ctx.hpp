@@ -142,6 +142,8 @@ struct Sketch {
   DevBuf cl_off, cl_mem, cl_nexit, cl_eoff, cl_ent, ent_of, sc_base, ss_base;
   DevBuf slot_ent;           // M: the member in this slot is an entry
   DevBuf touched;            // K: cluster touched by a batch since the last layout
+  DevBuf large;              // clusters of more than 64 members (laid out one workgroup each)
+  uint32_t n_large = 0;
   // internal edges: the list sorted by (base slot of dst, src), kept up to date by
   // every batch (sorted adds merged in, dead entries dropped), and the
   // per-cluster arrays the in-cluster kernels read

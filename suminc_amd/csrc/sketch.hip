@@ -327,11 +327,16 @@ __global__ void k_row_ptr32(const K* rowkey, uint32_t M, uint32_t K_, uint32_t* 
   for (int64_t r = prev + 1; r <= row; ++r) off[r] = i;
 }
 
-// One wave per cluster touched since the last layout (untouched clusters keep
-// their types, order and counts): node types from the edge counts (BothOutIn /
-// OnlyIn / OnlyOut / Inner), the stable exits-first member order with an entry
-// flag per slot, local indices and the per-cluster counts for the scan.  The
-// first 64 members stay in registers between the two passes.
+// Clusters of at most kWaveCluster members are laid out one wave each, larger ones
+// one workgroup each (from the list of large clusters).
+constexpr uint32_t kWaveCluster = 64;
+constexpr int kLargeRegs = kMaxCluster / kBlock;  // member slots one thread of a large cluster holds
+
+// One wave per cluster of at most kWaveCluster members touched since the last
+// layout (untouched clusters keep their types, order and counts): node types from
+// the edge counts (BothOutIn / OnlyIn / OnlyOut / Inner), the stable exits-first
+// member order with an entry flag per slot, local indices and the per-cluster
+// counts for the scan.  The members stay in registers between the two passes.
 __global__ void __launch_bounds__(kBlock) k_member_partition(uint32_t K, const uint32_t* __restrict__ cl_off,
                                                              const uint32_t* __restrict__ base_mem,
                                                              const uint32_t* __restrict__ ext_out,
@@ -345,6 +350,7 @@ __global__ void __launch_bounds__(kBlock) k_member_partition(uint32_t K, const u
   for (uint32_t k = blockIdx.x * kWaves + (threadIdx.x >> 6); k < K; k += nw) {
     if (!touched[k]) continue;  // whole wave
     const uint32_t b = cl_off[k], size = cl_off[k + 1] - b;
+    if (size > kWaveCluster) continue;  // k_member_partition_large
     uint32_t nex = 0, nen = 0;
     uint32_t v0 = 0;
     bool en0 = false, ex0 = false;
@@ -397,6 +403,83 @@ __global__ void __launch_bounds__(kBlock) k_member_partition(uint32_t K, const u
   if (blockIdx.x == 0 && threadIdx.x == 0) cl_cnt[K] = ClBase{0ull, 0ull, 0u, 0u};
 }
 
+// k_member_partition for the clusters of more than kWaveCluster members, one
+// workgroup each: slot r * kBlock + t sits in register r of thread t, every load
+// of a pass is issued before any of its results is used (a 1024-member cluster
+// costs a few memory round trips, not a few per 64 members), and the stable
+// exits-first order comes from workgroup scans.
+__global__ void __launch_bounds__(kBlock) k_member_partition_large(const uint32_t* __restrict__ large, uint32_t L,
+                                                                   const uint32_t* __restrict__ cl_off,
+                                                                   const uint32_t* __restrict__ base_mem,
+                                                                   const uint32_t* __restrict__ ext_out,
+                                                                   const uint32_t* __restrict__ ext_in,
+                                                                   const uint8_t* __restrict__ touched, uint8_t* ntype,
+                                                                   uint32_t* cl_mem, uint8_t* slot_ent, uint32_t* lidx,
+                                                                   uint32_t* cl_nexit, ClBase* cl_cnt) {
+  __shared__ uint32_t s_w0[kBlock / 64 + 1], s_w1[kBlock / 64 + 1];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t li = blockIdx.x; li < L; li += gridDim.x) {
+    const uint32_t k = large[li];
+    if (!touched[k]) continue;  // whole workgroup
+    const uint32_t b = cl_off[k], size = cl_off[k + 1] - b;
+    const uint32_t nr = (size + kBlock - 1) / kBlock;  // <= kLargeRegs: at most kMaxCluster members
+    // threads past the end re-read the last member and are masked out below
+    uint32_t v[kLargeRegs], ein[kLargeRegs], eout[kLargeRegs], old[kLargeRegs];
+#pragma unroll
+    for (int r = 0; r < kLargeRegs; ++r)
+      if ((uint32_t)r < nr) v[r] = base_mem[b + min(r * kBlock + tid, size - 1)];
+#pragma unroll
+    for (int r = 0; r < kLargeRegs; ++r)
+      if ((uint32_t)r < nr) {
+        ein[r] = ext_in[v[r]];
+        eout[r] = ext_out[v[r]];
+        old[r] = ntype[v[r]];
+      }
+    uint32_t enb = 0, exb = 0, myex = 0, myen = 0;  // enb, exb: bit r = slot r * kBlock + tid
+#pragma unroll
+    for (int r = 0; r < kLargeRegs; ++r)
+      if ((uint32_t)r < nr) {
+        const bool in = r * kBlock + tid < size;
+        const bool en = in && ein[r] != 0, ex = in && eout[r] != 0;
+        if (in) {
+          const uint32_t t = en && ex ? 3 : en ? 2 : ex ? 1 : 4;
+          if (old[r] != t) ntype[v[r]] = (uint8_t)t;
+        }
+        enb |= (en ? 1u : 0u) << r;
+        exb |= (ex ? 1u : 0u) << r;
+        myex += ex ? 1u : 0u;
+        myen += en ? 1u : 0u;
+      }
+    uint32_t nex, nen;
+    block_exclusive_scan<kBlock>(myex, s_w0, &nex);
+    block_exclusive_scan<kBlock>(myen, s_w1, &nen);
+    __syncthreads();  // the scans below reuse s_w0 and s_w1
+    uint32_t pe = 0, pi = nex;
+#pragma unroll
+    for (int r = 0; r < kLargeRegs; ++r)
+      if ((uint32_t)r < nr) {
+        const bool in = r * kBlock + tid < size;
+        const bool ex = (exb >> r) & 1u, en = (enb >> r) & 1u;
+        uint32_t te, ti;
+        const uint32_t oe = block_exclusive_scan<kBlock>(ex ? 1u : 0u, s_w0, &te);
+        const uint32_t oi = block_exclusive_scan<kBlock>(in && !ex ? 1u : 0u, s_w1, &ti);
+        if (in) {
+          const uint32_t pos = ex ? pe + oe : pi + oi;
+          cl_mem[b + pos] = v[r];
+          slot_ent[b + pos] = en;
+          lidx[v[r]] = pos;
+        }
+        pe += te;
+        pi += ti;
+        __syncthreads();
+      }
+    if (tid == 0) {
+      cl_nexit[k] = nex;
+      cl_cnt[k] = ClBase{(unsigned long long)nen * size, (unsigned long long)nen * nex, nen, 0u};
+    }
+  }
+}
+
 // Node-type histogram of the members (for the structure report only).
 __global__ void k_type_hist(uint32_t M, const uint32_t* __restrict__ base_mem, const uint8_t* __restrict__ ntype,
                             uint32_t* hist) {
@@ -409,9 +492,10 @@ __global__ void k_type_hist(uint32_t M, const uint32_t* __restrict__ base_mem, c
   if (threadIdx.x < 8 && s_h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], s_h[threadIdx.x]);
 }
 
-// One wave per cluster: entries in member order, their shortcut-row bases and
-// ent_of (INF32 for the other members of touched clusters; untouched clusters
-// have the same non-entries as before).  Clears the touched flags.
+// One wave per cluster of at most kWaveCluster members: entries in member order,
+// their shortcut-row bases and ent_of (INF32 for the other members of touched
+// clusters; untouched clusters have the same non-entries as before).  Clears the
+// touched flags.
 __global__ void __launch_bounds__(kBlock) k_entry_layout(uint32_t K, const uint32_t* __restrict__ cl_off,
                                                          const uint32_t* __restrict__ cl_mem,
                                                          const uint32_t* __restrict__ cl_nexit,
@@ -424,6 +508,7 @@ __global__ void __launch_bounds__(kBlock) k_entry_layout(uint32_t K, const uint3
   const uint32_t nw = gridDim.x * kWaves;
   for (uint32_t k = blockIdx.x * kWaves + (threadIdx.x >> 6); k < K; k += nw) {
     const uint32_t b = cl_off[k], size = cl_off[k + 1] - b, nex = cl_nexit[k];
+    if (size > kWaveCluster) continue;  // k_entry_layout_large
     const bool tk = touched[k] != 0;
     const ClBase cb = base[k];
     uint32_t r = 0;
@@ -454,6 +539,67 @@ __global__ void __launch_bounds__(kBlock) k_entry_layout(uint32_t K, const uint3
     sc_base[t.ent] = t.sc;
     ss_base[t.ent] = t.ss;
   }
+}
+
+// k_entry_layout for the clusters of more than kWaveCluster members, one
+// workgroup each, the slots in registers as in k_member_partition_large.
+__global__ void __launch_bounds__(kBlock) k_entry_layout_large(const uint32_t* __restrict__ large, uint32_t L,
+                                                               const uint32_t* __restrict__ cl_off,
+                                                               const uint32_t* __restrict__ cl_mem,
+                                                               const uint32_t* __restrict__ cl_nexit,
+                                                               const uint8_t* __restrict__ slot_ent, uint8_t* touched,
+                                                               const ClBase* __restrict__ base, uint32_t* cl_eoff,
+                                                               uint32_t* cl_ent, uint32_t* ent_of,
+                                                               unsigned long long* sc_base, unsigned long long* ss_base) {
+  __shared__ uint32_t s_w[kBlock / 64 + 1];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t li = blockIdx.x; li < L; li += gridDim.x) {
+    const uint32_t k = large[li];
+    const uint32_t b = cl_off[k], size = cl_off[k + 1] - b, nex = cl_nexit[k];
+    const bool tk = touched[k] != 0;
+    const ClBase cb = base[k];
+    const uint32_t nr = (size + kBlock - 1) / kBlock;  // <= kLargeRegs
+    uint32_t se[kLargeRegs], mv[kLargeRegs];
+#pragma unroll
+    for (int r = 0; r < kLargeRegs; ++r)
+      if ((uint32_t)r < nr) {
+        const uint32_t j = min(r * kBlock + tid, size - 1);
+        se[r] = slot_ent[b + j];
+        mv[r] = cl_mem[b + j];
+      }
+    uint32_t q0 = 0;
+#pragma unroll
+    for (int r = 0; r < kLargeRegs; ++r)
+      if ((uint32_t)r < nr) {
+        const bool in = r * kBlock + tid < size;
+        const bool en = in && se[r] != 0;
+        uint32_t tot;
+        const uint32_t o = block_exclusive_scan<kBlock>(en ? 1u : 0u, s_w, &tot);
+        if (en) {
+          const uint32_t v = mv[r], q = q0 + o, e = cb.ent + q;
+          cl_ent[e] = v;
+          ent_of[v] = e;
+          sc_base[e] = cb.sc + (unsigned long long)q * size;
+          ss_base[e] = cb.ss + (unsigned long long)q * nex;
+        } else if (tk && in) {
+          ent_of[mv[r]] = INF32;
+        }
+        q0 += tot;
+        __syncthreads();  // the next scan reuses s_w
+      }
+    if (tid == 0) {
+      cl_eoff[k] = cb.ent;
+      if (tk) touched[k] = 0;
+    }
+  }
+}
+
+__global__ void k_large_list(uint32_t K, const uint32_t* __restrict__ cl_off, uint32_t* list,
+                             unsigned long long* count) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool large = k < K && cl_off[k + 1] - cl_off[k] > kWaveCluster;
+  const unsigned long long slot = wave_append(large, count);
+  if (large) list[slot] = k;
 }
 
 // First build: the internal in-edges of every accepted member, one wave per
@@ -574,11 +720,29 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
   if (!rebuild) {  // every cluster is laid out by the first layout
     s.touched.ensure((size_t)K + 1);
     SUMINC_HIP(hipMemsetAsync(s.touched.p, 1, (size_t)K + 1, c->stream));
+    // the clusters laid out one workgroup each (membership is fixed from here on)
+    s.large.ensure((size_t)K * 4 + 4);
+    DevBuf lc;
+    lc.ensure(8);
+    SUMINC_HIP(hipMemsetAsync(lc.p, 0, 8, c->stream));
+    if (K)
+      k_large_list<<<grid_for(K), kBlock, 0, c->stream>>>(K, s.cl_off.as<uint32_t>(), s.large.as<uint32_t>(),
+                                                          lc.as<unsigned long long>());
+    unsigned long long nl = 0;
+    SUMINC_HIP(hipMemcpyAsync(&nl, lc.p, 8, hipMemcpyDeviceToHost, c->stream));
+    SUMINC_HIP(hipStreamSynchronize(c->stream));
+    s.n_large = (uint32_t)nl;
   }
+  const unsigned large_grid = std::max<uint32_t>(1u, std::min<uint32_t>(s.n_large, 4096u));
   k_member_partition<<<layout_grid(K), kBlock, 0, c->stream>>>(
       K, s.cl_off.as<uint32_t>(), s.base_mem.as<uint32_t>(), s.ext_out.as<uint32_t>(), s.ext_in.as<uint32_t>(),
       s.touched.as<uint8_t>(), s.ntype.as<uint8_t>(), s.cl_mem.as<uint32_t>(), s.slot_ent.as<uint8_t>(),
       s.lidx.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), cnt);
+  if (s.n_large)
+    k_member_partition_large<<<large_grid, kBlock, 0, c->stream>>>(
+        s.large.as<uint32_t>(), s.n_large, s.cl_off.as<uint32_t>(), s.base_mem.as<uint32_t>(),
+        s.ext_out.as<uint32_t>(), s.ext_in.as<uint32_t>(), s.touched.as<uint8_t>(), s.ntype.as<uint8_t>(),
+        s.cl_mem.as<uint32_t>(), s.slot_ent.as<uint8_t>(), s.lidx.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), cnt);
   cub(c, [&](void* p, size_t& b) {
     return rocprim::exclusive_scan(p, b, cnt, base, ClBase{0ull, 0ull, 0u, 0u}, (size_t)K + 1, ClBasePlus(),
                                    c->stream);
@@ -587,6 +751,12 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
       K, s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), s.slot_ent.as<uint8_t>(),
       s.touched.as<uint8_t>(), base, s.cl_eoff.as<uint32_t>(), s.cl_ent.as<uint32_t>(), s.ent_of.as<uint32_t>(),
       s.sc_base.as<unsigned long long>(), s.ss_base.as<unsigned long long>());
+  if (s.n_large)
+    k_entry_layout_large<<<large_grid, kBlock, 0, c->stream>>>(
+        s.large.as<uint32_t>(), s.n_large, s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(),
+        s.cl_nexit.as<uint32_t>(), s.slot_ent.as<uint8_t>(), s.touched.as<uint8_t>(), base, s.cl_eoff.as<uint32_t>(),
+        s.cl_ent.as<uint32_t>(), s.ent_of.as<uint32_t>(), s.sc_base.as<unsigned long long>(),
+        s.ss_base.as<unsigned long long>());
   // in-cluster edge arrays
   const uint64_t ie_m = s.iem_n;
   s.ie_m = ie_m;
@@ -1020,9 +1190,12 @@ __device__ uint32_t cluster_iterate(uint32_t size, uint32_t pb, uint32_t pe, con
   return rounds;
 }
 
+// One workgroup per entry (grid-stride over the E entries): a cluster's entries
+// are independent injections, so a cluster with many entries spreads over many
+// workgroups.  Entries of clusters whose rows are valid are skipped.
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_pr_shortcuts(const uint32_t* __restrict__ cl_off,
-                                                         const uint32_t* __restrict__ cl_eoff,
+__global__ void __launch_bounds__(kBlock) k_pr_shortcuts(uint32_t E, const uint32_t* __restrict__ cid,
+                                                         const uint32_t* __restrict__ cl_off,
                                                          const uint32_t* __restrict__ cl_ent,
                                                          const uint32_t* __restrict__ lidx,
                                                          const unsigned long long* __restrict__ sc_base,
@@ -1031,13 +1204,14 @@ __global__ void __launch_bounds__(kBlock) k_pr_shortcuts(const uint32_t* __restr
                                                          const uint8_t* __restrict__ valid, unsigned long long* work) {
   __shared__ T s_r[kMaxCluster], s_d[kMaxCluster], s_n[kMaxCluster];
   __shared__ T s_red[kBlock / 64];
-  const uint32_t k = blockIdx.x;
-  if (valid[k]) return;
-  const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0;
-  const double thr = thr_per_vertex * size;
   unsigned long long edges = 0;
-  for (uint32_t e = cl_eoff[k]; e < cl_eoff[k + 1]; ++e) {
-    const uint32_t src = lidx[cl_ent[e]];
+  for (uint32_t e = blockIdx.x; e < E; e += gridDim.x) {
+    const uint32_t v = cl_ent[e];
+    const uint32_t k = cid[v];
+    if (valid[k]) continue;
+    const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0;
+    const double thr = thr_per_vertex * size;
+    const uint32_t src = lidx[v];
     for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
       s_r[j] = T(0);
       s_d[j] = j == src ? T(1) : T(0);
@@ -1073,9 +1247,9 @@ static void compute_pr_rows(suminc_ctx* c) {
   SUMINC_HIP(hipMemsetAsync(work.p, 0, 8, c->stream));
   const double thr_pv = c->prm.termcheck_threshold / (double)c->n;  // T / N_old * |C| per cluster
   const int maxit = c->prm.shortcut_max_rounds > 0 ? c->prm.shortcut_max_rounds : 100;
-  if (s.K)
-    k_pr_shortcuts<T><<<s.K, kBlock, 0, c->stream>>>(
-        s.cl_off.as<uint32_t>(), s.cl_eoff.as<uint32_t>(), s.cl_ent.as<uint32_t>(), s.lidx.as<uint32_t>(),
+  if (s.E)
+    k_pr_shortcuts<T><<<std::min<uint32_t>(s.E, 16384), kBlock, 0, c->stream>>>(
+        s.E, s.cid.as<uint32_t>(), s.cl_off.as<uint32_t>(), s.cl_ent.as<uint32_t>(), s.lidx.as<uint32_t>(),
         s.sc_base.as<unsigned long long>(), internal_of(s), (T)c->prm.damping, thr_pv, maxit, s.sc_R.as<T>(), s.sc_D.as<T>(), s.pr_valid.as<uint8_t>(),
         work.as<unsigned long long>());
   SUMINC_HIP(hipGetLastError());

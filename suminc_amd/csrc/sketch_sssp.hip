@@ -94,18 +94,49 @@ __device__ uint32_t cluster_bf(uint32_t size, uint32_t pb, uint32_t pe, const Ed
 constexpr unsigned kClusterGrid = 2048;
 constexpr uint32_t kSmallCluster = 64;  // clusters up to this size run one per wave
 
+// Workers (workgroups or waves) per cluster when W workers share nk clusters:
+// one when the clusters fill the workers, else W / nk, each taking every P-th
+// source (entry or exit) of its cluster.  A cluster's Bellman-Ford runs are
+// independent, so its sources spread over P workers instead of queueing on one.
+__device__ __forceinline__ uint32_t workers_per_cluster(uint32_t nk, uint32_t W) {
+  return nk == 0 || nk >= W ? 1u : W / nk;
+}
+
 // Clusters whose flag equals `want`, split by size: clusters of at most
 // kSmallCluster members at list[0, counts[0]), larger ones at
-// list[K - counts[1], K) (filled from the back).
-__global__ void k_cluster_lists(uint32_t K, const uint8_t* __restrict__ flag, uint8_t want,
-                                const uint32_t* __restrict__ cl_off, uint32_t* list, unsigned long long* counts) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+// list[K - counts[1], K) (filled from the back).  One counter atomic per
+// workgroup and list (every cluster is visited, so per-wave atomics on the two
+// counters would queue behind each other).
+__global__ void __launch_bounds__(kBlock) k_cluster_lists(uint32_t K, const uint8_t* __restrict__ flag, uint8_t want,
+                                                          const uint32_t* __restrict__ cl_off, uint32_t* list,
+                                                          unsigned long long* counts) {
+  __shared__ uint32_t s_cnt[2][kBlock / 64];
+  __shared__ unsigned long long s_base[2];
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  const int wv = threadIdx.x >> 6, lane = lane_id();
   const bool take = k < K && flag[k] == want;
   const bool small = take && cl_off[k + 1] - cl_off[k] <= kSmallCluster;
-  const unsigned long long s1 = wave_append(small, &counts[0]);
-  const unsigned long long s2 = wave_append(take && !small, &counts[1]);
-  if (small) list[s1] = k;
-  else if (take) list[K - 1 - s2] = k;
+  const bool large = take && !small;
+  const unsigned long long ms = __ballot(small), ml = __ballot(large);
+  if (lane == 0) {
+    s_cnt[0][wv] = (uint32_t)__popcll(ms);
+    s_cnt[1][wv] = (uint32_t)__popcll(ml);
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    uint32_t tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) tot += s_cnt[threadIdx.x][w];
+    s_base[threadIdx.x] = tot ? atomicAdd(&counts[threadIdx.x], (unsigned long long)tot) : 0ull;
+  }
+  __syncthreads();
+  unsigned long long bs = s_base[0], bl = s_base[1];
+  for (int w = 0; w < wv; ++w) {
+    bs += s_cnt[0][w];
+    bl += s_cnt[1][w];
+  }
+  const unsigned long long below = (1ull << lane) - 1ull;
+  if (small) list[bs + __popcll(ms & below)] = k;
+  else if (large) list[K - 1 - (bl + __popcll(ml & below))] = k;
 }
 
 // Orders this wave's LDS traffic before what follows (wave-local exchange).
@@ -180,20 +211,22 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts(const uint32_t* __restr
   __shared__ uint32_t s_id[kMaxClusterS];
   __shared__ int s_flag[3];
   const uint32_t nk = (uint32_t)counts[1];  // the large clusters, at the back of the list
+  const uint32_t P = workers_per_cluster(nk, gridDim.x);
   unsigned long long edges = 0;
-  for (uint32_t li = blockIdx.x; li < nk; li += gridDim.x) {
+  for (uint32_t wi = blockIdx.x; wi < nk * P; wi += gridDim.x) {  // nk * P <= max(nk, grid)
+    const uint32_t li = wi % nk, part = wi / nk;
     const uint32_t k = list[K - 1 - li];
     const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0, nexit = cl_nexit[k];
     const uint32_t pb = ie.off[k], pe = ie.off[k + 1];
     const uint32_t eb = cl_eoff[k], ee = cl_eoff[k + 1], nent = ee - eb;
-    if (threadIdx.x == 0) valid[k] = 1;
-    if (nexit == 0 || nent == 0) continue;
+    if (threadIdx.x == 0 && part == 0) valid[k] = 1;
+    if (nexit == 0 || nent == 0 || part >= (nent <= nexit ? nent : nexit)) continue;
     for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_id[j] = cl_mem[slot0 + j];
     const bool fwd = nent <= nexit;
     EdgeCache ec;
     ec.load(pb, pe, fwd ? ie.src : ie.dst, fwd ? ie.dst : ie.src, ie.wt);
     if (fwd) {
-      for (uint32_t e = eb; e < ee; ++e) {
+      for (uint32_t e = eb + part; e < ee; e += P) {
         const uint32_t src = lidx[cl_ent[e]];
         const unsigned long long base = ss_base[e];
         if (nexit == 1 && src == 0) {  // the only exit is the entry itself
@@ -207,7 +240,7 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts(const uint32_t* __restr
         __syncthreads();
       }
     } else {
-      for (uint32_t x = 0; x < nexit; ++x) {
+      for (uint32_t x = part; x < nexit; x += P) {
         for (uint32_t j = threadIdx.x; j < size; j += kBlock) s_dp[j] = j == x ? 0xFFFFFFFFull : kInf64;
         const uint32_t rounds = cluster_bf(size, pb, pe, ec, ie.dst, ie.src, ie.wt, s_id, s_dp, s_flag);
         edges += (unsigned long long)rounds * (pe - pb);
@@ -243,13 +276,16 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts_small(const uint32_t* _
   if (threadIdx.x == 0) s_edges = 0;
   __syncthreads();
   const uint32_t nk = (uint32_t)counts[0];
+  const uint32_t nwaves = gridDim.x * (kBlock / 64);
+  const uint32_t P = workers_per_cluster(nk, nwaves);
   unsigned long long edges = 0;
-  for (uint32_t li = blockIdx.x * (kBlock / 64) + wv; li < nk; li += gridDim.x * (kBlock / 64)) {
+  for (uint32_t wi = blockIdx.x * (kBlock / 64) + wv; wi < nk * P; wi += nwaves) {  // nk * P <= max(nk, nwaves)
+    const uint32_t li = wi % nk, part = wi / nk;
     const uint32_t k = list[li];
     const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0, nexit = cl_nexit[k];
     const uint32_t pb = ie.off[k], pe = ie.off[k + 1];
     const unsigned long long rbase = clb[k].ss;
-    if (lane == 0) valid[k] = 1;
+    if (lane == 0 && part == 0) valid[k] = 1;
     bool en = false;
     if (lane < (int)size) {
       s_id[lane] = cl_mem[slot0 + lane];
@@ -257,12 +293,13 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts_small(const uint32_t* _
     }
     const unsigned long long ents = __ballot(en);
     const uint32_t nent = (uint32_t)__popcll(ents);
-    if (nexit == 0 || nent == 0) continue;
+    if (nexit == 0 || nent == 0 || part >= (nent <= nexit ? nent : nexit)) continue;
     if (nent <= nexit) {
       unsigned long long left = ents;
       for (uint32_t q = 0; left; ++q) {
         const uint32_t src = (uint32_t)__ffsll((long long)left) - 1;
         left &= left - 1;
+        if (q % P != part) continue;
         const unsigned long long r0 = rbase + (unsigned long long)q * nexit;
         if (nexit == 1 && src == 0) {  // the only exit is the entry itself
           if (lane == 0) sd[r0] = 0;
@@ -277,7 +314,7 @@ __global__ void __launch_bounds__(kBlock) k_ss_shortcuts_small(const uint32_t* _
       }
     } else {
       const unsigned long long q = __popcll(ents & below);  // this lane's entry rank
-      for (uint32_t x = 0; x < nexit; ++x) {
+      for (uint32_t x = part; x < nexit; x += P) {
         if (lane < (int)size) s_dp[lane] = (uint32_t)lane == x ? 0xFFFFFFFFull : kInf64;
         wave_lds_sync();
         const uint32_t rounds = cluster_bf_wave(size, pb, pe, ie.dst, ie.src, ie.wt, s_id, s_dp);
