@@ -257,8 +257,15 @@ __global__ void __launch_bounds__(kBlock) k_new_off(const uint32_t* __restrict__
 // with a new weight, which k_place_adds writes).  Groups are sorted by (row, col),
 // so their positions are non-decreasing; each workgroup copies a tile of
 // kCopyTile edges, finds its first group once and stages the tile's groups in LDS.
-constexpr uint32_t kCopyTile = 2048;
-constexpr uint32_t kCopyStage = 2048;  // LDS-staged groups per tile
+// Tile t owns the output range [base_t, base_{t+1}) (its surviving edges and the
+// add slots among them), so the tiles partition the new edge array: a tile
+// reads its edges with 16-byte loads, places each survivor in LDS at its offset
+// in the range, and streams the range out with aligned 16-byte stores.  The add
+// slots inside a range receive stale LDS words; k_place_adds, which runs after
+// this kernel, writes them.  The old arrays are not read again and the new ones
+// exceed the caches, so both directions use non-temporal accesses.
+constexpr uint32_t kCopyTile = 1024;   // 4 edges per thread
+constexpr uint32_t kCopyStage = 256;   // LDS-staged groups per tile
 
 // first group of every tile (groups with pos >= t * kCopyTile); entry ntiles = G
 __global__ void k_tile_groups(const unsigned* countp, const uint32_t* __restrict__ pos, uint32_t ntiles,
@@ -276,43 +283,19 @@ __global__ void k_tile_groups(const unsigned* countp, const uint32_t* __restrict
   tstart[t] = lo;
 }
 
-// Position bookkeeping of one old edge e inside a tile: i = the tile's groups with
-// pos < e, k = those with pos <= e (the caller keeps them for the next edge).
-struct TileGroups {
-  const uint32_t* s_pos;
-  const uint32_t* s_ex;   // ex_pre | (pair exists) << 31
-  const uint32_t* s_add;  // add_pre + (group is an add)
-  uint32_t cnt, ex_base, add_base;
-  __device__ __forceinline__ void seek(uint32_t e, uint32_t& i) const {
-    uint32_t lo = i, hi = cnt;  // i only moves forward
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (s_pos[mid] < e) lo = mid + 1; else hi = mid;
-    }
-    i = lo;
-  }
-  // destination of e, or INF32 when e's pair is grouped as existing
-  __device__ __forceinline__ uint32_t dest(uint32_t e, uint32_t i) const {
-    uint32_t k = i;
-    bool dropped = false;
-    while (k < cnt && s_pos[k] == e) {
-      dropped |= (s_ex[k] >> 31) != 0;
-      ++k;
-    }
-    if (dropped) return INF32;
-    const uint32_t drops = i ? (s_ex[i - 1] & 0x7FFFFFFFu) + (s_ex[i - 1] >> 31) : ex_base;
-    const uint32_t adds = k ? s_add[k - 1] : add_base;
-    return e - drops + adds;
-  }
-};
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(kBlock) k_copy_old(uint32_t m, const uint32_t* __restrict__ colv,
                                                      const uint32_t* __restrict__ wv, uint32_t* __restrict__ ncol,
                                                      uint32_t* __restrict__ nw, const uint32_t* __restrict__ tstart,
                                                      Groups gs) {
+  static_assert(kCopyTile == 4 * kBlock, "one 16-byte load per thread and array");
   __shared__ uint32_t s_pos[kCopyStage];
-  __shared__ uint32_t s_ex[kCopyStage];
-  __shared__ uint32_t s_add[kCopyStage];
+  __shared__ uint32_t s_ex[kCopyStage];   // ex_pre | (pair exists) << 31
+  __shared__ uint32_t s_add[kCopyStage];  // add_pre + (group is an add)
+  // the tile's output range holds at most kCopyTile survivors plus kCopyStage adds
+  __shared__ __attribute__((aligned(16))) uint32_t s_c[kCopyTile + kCopyStage + 8];
+  __shared__ __attribute__((aligned(16))) uint32_t s_w[kCopyTile + kCopyStage + 8];
   const uint32_t e0 = blockIdx.x * kCopyTile;
   const uint32_t e1 = (uint32_t)min((uint64_t)m, (uint64_t)e0 + kCopyTile);
   const uint32_t j0 = tstart[blockIdx.x], cnt = tstart[blockIdx.x + 1] - j0;
@@ -340,30 +323,73 @@ __global__ void __launch_bounds__(kBlock) k_copy_old(uint32_t m, const uint32_t*
     s_ex[i] = gs.ex_pre[g] | (gs.ex[g] ? 0x80000000u : 0u);
     s_add[i] = gs.add_pre[g] + (gs.act[g] == 0 ? 1u : 0u);
   }
-  __syncthreads();
-  const TileGroups T{s_pos, s_ex, s_add, cnt, gs.ex_pre[j0], gs.add_pre[j0]};
-  // lane-consecutive edges: every load and (between group positions) every store
-  // of a wave covers 256 contiguous bytes; all loads of the tile are issued first
-  constexpr int kPer = kCopyTile / kBlock;
-  uint32_t cv[kPer], wvv[kPer];
+  // output range [base, end): every survivor lands at or after base (each
+  // existing pair grouped before it dropped a distinct edge of the tile)
+  const uint32_t ex_base = gs.ex_pre[j0], add_base = gs.add_pre[j0];
+  const uint32_t base = e0 - ex_base + add_base;
+  const uint32_t end = e1 - gs.ex_pre[j0 + cnt] + gs.add_pre[j0 + cnt];
+  const uint32_t abase = base & ~3u;
+  const uint32_t eb = e0 + 4 * threadIdx.x;
+  u32x4 cv = {0u, 0u, 0u, 0u}, wq = {0u, 0u, 0u, 0u};
+  if (eb + 4 <= e1) {
+    cv = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(colv + eb));
+    if (nw) wq = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(wv + eb));
+  } else {
 #pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const uint32_t e = e0 + r * kBlock + threadIdx.x;
-    if (e < e1) {
-      cv[r] = colv[e];
-      if (nw) wvv[r] = wv[e];
+    for (int j = 0; j < 4; ++j)
+      if (eb + j < e1) {
+        cv[j] = colv[eb + j];
+        if (nw) wq[j] = wv[eb + j];
+      }
+  }
+  __syncthreads();
+  uint32_t i = 0;  // first staged group with pos >= the current edge
+  {
+    uint32_t hi = cnt;
+    while (i < hi) {
+      const uint32_t mid = (i + hi) >> 1;
+      if (s_pos[mid] < eb) i = mid + 1; else hi = mid;
     }
   }
-  uint32_t i = 0;
 #pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const uint32_t e = e0 + r * kBlock + threadIdx.x;
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t e = eb + j;
     if (e >= e1) break;
-    T.seek(e, i);
-    const uint32_t dst = T.dest(e, i);
-    if (dst == INF32) continue;
-    ncol[dst] = cv[r];
-    if (nw) nw[dst] = wvv[r];
+    while (i < cnt && s_pos[i] < e) ++i;
+    uint32_t k = i;
+    bool dropped = false;
+    while (k < cnt && s_pos[k] == e) {
+      dropped |= (s_ex[k] >> 31) != 0;
+      ++k;
+    }
+    if (dropped) continue;
+    const uint32_t drops = i ? (s_ex[i - 1] & 0x7FFFFFFFu) + (s_ex[i - 1] >> 31) : ex_base;
+    const uint32_t adds = k ? s_add[k - 1] : add_base;
+    const uint32_t l = e - drops + adds - abase;
+    s_c[l] = cv[j];
+    s_w[l] = wq[j];
+  }
+  __syncthreads();
+  // aligned 16-byte chunks [a0, a1) of the range, single words before and after
+  const uint32_t a0 = (base + 3) & ~3u, a1 = end & ~3u;
+  const bool chunks = a0 <= a1;             // else the range lies inside one chunk
+  const uint32_t h1 = chunks ? a0 : end;
+  for (uint32_t d = base + threadIdx.x; d < h1; d += kBlock) {
+    ncol[d] = s_c[d - abase];
+    if (nw) nw[d] = s_w[d - abase];
+  }
+  if (chunks) {
+    for (uint32_t d = a1 + threadIdx.x; d < end; d += kBlock) {
+      ncol[d] = s_c[d - abase];
+      if (nw) nw[d] = s_w[d - abase];
+    }
+    for (uint32_t q = a0 / 4 + threadIdx.x; q < a1 / 4; q += kBlock) {
+      __builtin_nontemporal_store(*reinterpret_cast<const u32x4*>(&s_c[q * 4 - abase]),
+                                  reinterpret_cast<u32x4*>(ncol) + q);
+      if (nw)
+        __builtin_nontemporal_store(*reinterpret_cast<const u32x4*>(&s_w[q * 4 - abase]),
+                                    reinterpret_cast<u32x4*>(nw) + q);
+    }
   }
 }
 
