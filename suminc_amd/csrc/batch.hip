@@ -714,7 +714,9 @@ __global__ void __launch_bounds__(kBlock) k_reset_pull(const uint32_t* ioff, con
     }
     block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t, uint32_t k) {
       const unsigned long long cand = pull_candidate(icol, iw, rbits, dp, k);
-      if (cand != ~0ull) atomicMin(&dp[s_x[slot]], cand);
+      // the reset words only decrease here: a candidate no smaller than a plain
+      // read cannot win and skips the atomic
+      if (cand != ~0ull && cand < dp[s_x[slot]]) atomicMin(&dp[s_x[slot]], cand);
     });
   }
 }

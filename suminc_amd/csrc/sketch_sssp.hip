@@ -495,10 +495,15 @@ __global__ void __launch_bounds__(kBlock) k_ss_relax_small(SsArgs a, Internal ie
 __device__ __forceinline__ void ss_relax(const SsArgs& a, int nxt, unsigned long long* ctr, uint32_t v,
                                          unsigned long long nd, uint32_t parent) {
   bool add = false;
+  // words only decrease during a query, so a candidate no smaller than a plain
+  // read of the word (possibly stale, never below the current value) cannot win:
+  // only the candidates that may improve pay for the atomic
   if (nd < 0xFFFFFFFFull) {
     const unsigned long long cand = (nd << 32) | parent;
-    const unsigned long long old = atomicMin(&a.dp[v], cand);
-    add = (old >> 32) > nd;
+    if (cand < a.dp[v]) {
+      const unsigned long long old = atomicMin(&a.dp[v], cand);
+      add = (old >> 32) > nd;
+    }
   }
   ss_append(a, nxt, ctr, v, add);
 }

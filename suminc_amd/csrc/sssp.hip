@@ -42,8 +42,12 @@ __device__ __forceinline__ void sssp_relax(const SsspArgs& a, int nxt, uint32_t 
   bool add = false;
   if (nd < INF32) {
     unsigned long long cand = (nd << 32) | u;
-    unsigned long long old = atomicMin(&a.dp[v], cand);
-    add = (old >> 32) > nd;
+    // words only decrease while the rounds run: a candidate no smaller than a
+    // plain (possibly stale) read cannot win, so it skips the atomic
+    if (cand < a.dp[v]) {
+      unsigned long long old = atomicMin(&a.dp[v], cand);
+      add = (old >> 32) > nd;
+    }
   }
   if (add) {
     uint32_t bit = 1u << (v & 31);
