@@ -105,6 +105,52 @@ def test_apply_batch_equals_oracle(gpu, weights):
                 assert np.array_equal(iw.view(np.uint32), c_w.view(np.uint32))
 
 
+@pytest.mark.parametrize("weights", [None, "u32"])
+def test_apply_batch_concentrated_rows(gpu, weights):
+    """A batch packed into a few rows: hundreds of deletes and adds inside one
+    out-row and hundreds of adds into one in-row put more groups into a copy tile
+    than the merge stages in LDS (its per-edge path); the rest of the graph takes
+    the staged path in the same merge.  CSR and CSC bit-exact vs the oracle."""
+    g = oracle.Graph.rmat(12, 16, seed=6, weights=weights)
+    off, col, w = g.csr()
+    n = g.n
+    deg = np.diff(off.astype(np.int64))
+    hub = int(np.argmax(deg))
+    row = col[off[hub]:off[hub + 1]]
+    rng = np.random.default_rng(5)
+    present = np.zeros(n, bool)
+    present[row] = True
+    dels = rng.choice(row, size=min(len(row) // 2, 600), replace=False)
+    absent = np.nonzero(~present)[0]
+    adds = rng.choice(absent, size=600, replace=False)
+    sink = int(np.argmin(np.bincount(col, minlength=n)))  # an in-row with few entries
+    srcs = rng.choice(np.setdiff1d(np.arange(n), [sink]), size=700, replace=False)
+    ops = np.zeros(len(dels) + len(adds) + len(srcs), gpu.UPDATE_DTYPE)
+    k = 0
+    for v in dels:
+        ops[k] = (1, hub, v, 0, 0, 0); k += 1
+    for v in adds:
+        ops[k] = (0, hub, v, 7, 0, 0); k += 1
+    for u in srcs:
+        ops[k] = (0, u, sink, 3, 0, 0); k += 1
+    ops = ops[rng.permutation(len(ops))]
+    with gpu.Context() as ctx:
+        ctx.upload_csr(off, col, w)
+        ctx.apply_batch(ops)
+        g2, _ = g.apply_batch(ops)
+        o_off, o_col, o_w = g2.csr()
+        off2, col2, w2 = ctx.download_csr()
+        assert np.array_equal(off2, o_off)
+        assert np.array_equal(col2, o_col)
+        if weights:
+            assert np.array_equal(w2.view(np.uint32), o_w.view(np.uint32))
+        c_off, c_row, c_w = csc_from_csr(o_off, o_col, o_w)
+        ioff, irow, iw = ctx.download_csc()
+        assert np.array_equal(ioff, c_off) and np.array_equal(irow, c_row)
+        if weights:
+            assert np.array_equal(iw.view(np.uint32), c_w.view(np.uint32))
+
+
 def test_apply_batch_rejects_bad_ops(gpu):
     g = oracle.Graph.rmat(8, 8, seed=1)
     off, col, _ = g.csr()
