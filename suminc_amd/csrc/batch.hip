@@ -196,9 +196,12 @@ __global__ void k_touched(const uint32_t* row, const uint32_t* t_raw, const unsi
   tix[row[g]] = t;
 }
 
-__global__ void k_untouch(const uint32_t* t_list, const unsigned* bc, int which, uint32_t N, uint32_t* tix) {
+// Clears both touched-row maps (CSR: count bc[1], CSC: count bc[2]) for the next batch.
+__global__ void k_untouch(const uint32_t* t_list, uint32_t* tix, const uint32_t* tt_list, uint32_t* ttix,
+                          const unsigned* bc, uint32_t N) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < N && t < bc[which]) tix[t_list[t]] = INF32;
+  if (t < N && t < bc[1]) tix[t_list[t]] = INF32;
+  if (t < N && t < bc[2]) ttix[tt_list[t]] = INF32;
 }
 
 __global__ void k_new_m(const unsigned* bc_in, const uint32_t* ex_pre, const uint32_t* add_pre, uint32_t m,
@@ -668,11 +671,16 @@ __global__ void __launch_bounds__(kBlock) k_reset_round(const uint32_t* off, con
   }
 }
 
-__global__ void k_reset_apply(const uint32_t* rlist, const Counters* cnt, unsigned long long* dp) {
-  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= cnt->aux[4]) return;
-  uint32_t x = rlist[i];
-  dp[x] = ((unsigned long long)INF32 << 32) | x;
+// Resets the words of the walk's vertices; also clears the chunk count the re-pull
+// appends to.
+__global__ void k_reset_apply(const uint32_t* rlist, Counters* cnt, unsigned long long* dp) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) cnt->heavy_count = 0;
+  const unsigned long long nr = cnt->aux[4];
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < nr;
+       i += (unsigned long long)gridDim.x * blockDim.x) {
+    uint32_t x = rlist[i];
+    dp[x] = ((unsigned long long)INF32 << 32) | x;
+  }
 }
 
 // reset vertices pull from surviving (non-reset) in-neighbours over the NEW CSC
@@ -754,27 +762,29 @@ __global__ void __launch_bounds__(kBlock) k_reset_pull_heavy(const uint32_t* iof
   }
 }
 
-__global__ void k_heavy_clear(Counters* cnt) {
-  if (threadIdx.x == 0) cnt->heavy_count = 0;
-}
-
 // Seeds SSSP list 0 with every finite reset vertex and clears the reset bitmap
-// (the next repair starts from a clear one).
+// (the next repair starts from a clear one) and the re-pull's chunk count.
 __global__ void k_reset_seed(const uint32_t* rlist, uint32_t* rbits, const unsigned long long* dp, uint32_t* wl0,
-                             uint32_t* bits0, const Counters* cnt, unsigned long long* seeds) {
-  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-  bool add = false;
-  uint32_t x = 0;
-  if (i < cnt->aux[4]) {
-    x = rlist[i];
-    atomicAnd(&rbits[x >> 5], ~(1u << (x & 31)));
-    if ((uint32_t)(dp[x] >> 32) != INF32) {
-      uint32_t bit = 1u << (x & 31);
-      add = !(atomicOr(&bits0[x >> 5], bit) & bit);
+                             uint32_t* bits0, Counters* cnt, unsigned long long* seeds) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) cnt->heavy_count = 0;
+  const unsigned long long nr = cnt->aux[4];
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  // whole waves iterate together (the append's ballot)
+  for (unsigned long long i0 = (unsigned long long)blockIdx.x * blockDim.x; i0 < nr; i0 += stride) {
+    const unsigned long long i = i0 + threadIdx.x;
+    bool add = false;
+    uint32_t x = 0;
+    if (i < nr) {
+      x = rlist[i];
+      atomicAnd(&rbits[x >> 5], ~(1u << (x & 31)));
+      if ((uint32_t)(dp[x] >> 32) != INF32) {
+        uint32_t bit = 1u << (x & 31);
+        add = !(atomicOr(&bits0[x >> 5], bit) & bit);
+      }
     }
+    unsigned long long slot = wave_append(add, seeds);
+    if (add) wl0[slot] = x;
   }
-  unsigned long long slot = wave_append(add, seeds);
-  if (add) wl0[slot] = x;
 }
 
 // relax every add (new pair or new weight) from its current source distance
@@ -1133,7 +1143,6 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
                                                  nullptr, b.tg_pos.as<uint32_t>());
   gt.pos = b.tg_pos.as<uint32_t>();
   merge_direction(c, c->in_off, c->in_col, c->in_w, b.noff_in, b.ncol_in, b.nw_in, gt, N, cap_m, 2);
-  k_untouch<<<grid_for(N), kBlock, 0, c->stream>>>(b.tt_list.as<uint32_t>(), bc, 2, N, b.ttix.as<uint32_t>());
 
   // 6. PageRank reseed +1 on the new out-edges; the pull lists of the new CSC are
   // built here (they serve the next query as well)
@@ -1142,23 +1151,22 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
     if (c->pr_prec == SUMINC_PR_FP64) pr_reseed<double>(c, N, +1.0, pr_pull);
     else pr_reseed<float>(c, N, +1.0, pr_pull);
   }
-  k_untouch<<<grid_for(N), kBlock, 0, c->stream>>>(b.t_list.as<uint32_t>(), bc, 1, N, b.tix.as<uint32_t>());
+  k_untouch<<<grid_for(N), kBlock, 0, c->stream>>>(b.t_list.as<uint32_t>(), b.tix.as<uint32_t>(),
+                                                   b.tt_list.as<uint32_t>(), b.ttix.as<uint32_t>(), bc, N);
 
   // 7. SSSP: reset, re-pull, relax inserts -> frontier list 0
   if (sssp_reinit) sssp_init(c, c->sssp_src);
   if (sssp_repair) {
-    k_reset_apply<<<grid_for(n), kBlock, 0, c->stream>>>(b.rlist.as<uint32_t>(), cnt,
+    k_reset_apply<<<push_grid(), kBlock, 0, c->stream>>>(b.rlist.as<uint32_t>(), cnt,
                                                          c->sssp_dp.as<unsigned long long>());
     const uint32_t* iw = c->wkind == SUMINC_W_U32 ? c->in_w.as<uint32_t>() : nullptr;
-    k_heavy_clear<<<1, 64, 0, c->stream>>>(cnt);
     k_reset_pull<<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(), iw,
                                                         b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(), cnt,
                                                         c->sssp_dp.as<unsigned long long>(), c->heavy.as<uint2>());
     k_reset_pull_heavy<<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(), iw,
                                                               b.rbits.as<uint32_t>(), c->heavy.as<uint2>(), cnt,
                                                               c->sssp_dp.as<unsigned long long>());
-    k_heavy_clear<<<1, 64, 0, c->stream>>>(cnt);
-    k_reset_seed<<<grid_for(n), kBlock, 0, c->stream>>>(b.rlist.as<uint32_t>(), b.rbits.as<uint32_t>(),
+    k_reset_seed<<<push_grid(), kBlock, 0, c->stream>>>(b.rlist.as<uint32_t>(), b.rbits.as<uint32_t>(),
                                                         c->sssp_dp.as<unsigned long long>(),
                                                         c->sssp_wl0.as<uint32_t>(), c->sssp_flag0.as<uint32_t>(), cnt,
                                                         c->sssp_seeds.as<unsigned long long>());
