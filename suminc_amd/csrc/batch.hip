@@ -1005,15 +1005,11 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   BatchWs& b = c->bw;
   k_batch_zero<<<1, 64, 0, c->stream>>>(cnt, b.val.as<unsigned>());
   if (ops_on_device) {
-    // validate before any kernel indexes by the ops' vertex ids
-    unsigned res[2];
+    // validated before any kernel indexes by the ops' vertex ids; the sort and
+    // grouping below (which only read the ops) run while the host waits for it
     k_validate_ops<<<grid_for(N), kBlock, 0, c->stream>>>(ops, N, n, c->wkind, b.val.as<unsigned>());
     SUMINC_HIP(hipGetLastError());
-    SUMINC_HIP(hipMemcpyAsync(res, b.val.p, 8, hipMemcpyDeviceToHost, c->stream));
-    SUMINC_HIP(hipStreamSynchronize(c->stream));
-    SUMINC_REQUIRE(res[0] == 0, SUMINC_E_INVALID,
-                   std::to_string(res[0]) + " invalid ops (first at line " + std::to_string(res[1]) +
-                       "): type must be add/delete, ids below n, integer weights at least 1");
+    check_begin(c, b.val.p, 8);
   }
   // heavy-chunk list sized for the largest graph this batch can produce
   if (c->pr_ready || c->sssp_ready) c->heavy.ensure_slack((cap_m / kChunk + cap_m / kHeavyDeg + 2) * sizeof(uint4));
@@ -1047,6 +1043,12 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   k_split_rec<<<grid_for(N), kBlock, 0, c->stream>>>(b.grec.as<uint4>(), bc, N, b.g_row.as<uint32_t>(),
                                                      b.g_col.as<uint32_t>(), b.g_act.as<uint8_t>(),
                                                      b.g_w.as<uint32_t>());
+  if (ops_on_device) {
+    const unsigned* res = static_cast<const unsigned*>(check_wait(c));
+    SUMINC_REQUIRE(res[0] == 0, SUMINC_E_INVALID,
+                   std::to_string(res[0]) + " invalid ops (first at line " + std::to_string(res[1]) +
+                       "): type must be add/delete, ids below n, integer weights at least 1");
+  }
   k_probe<<<grid_for(N), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
                                                  c->wkind ? c->out_w.as<uint32_t>() : nullptr, b.g_row.as<uint32_t>(),
                                                  b.g_col.as<uint32_t>(), bc, b.g_ex.as<uint8_t>(),
@@ -1068,6 +1070,13 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   }
 
   // 4. SSSP: reset roots + subtree walk on the old graph
+  uint32_t walk_r = 0;
+  auto reset_round = [&](uint32_t r) {
+    k_reset_round<<<push_grid(), kBlock, 0, c->stream>>>(
+        c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(), c->sssp_dp.as<unsigned long long>(),
+        b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(), c->heavy.as<uint2>(), c->heavy.cap / sizeof(uint2) / 2, cnt,
+        r);
+  };
   if (sssp_repair) {
     const size_t words = ((size_t)n + 31) / 32 + 1;
     if (b.rbits.cap < words * 4 || !b.rbits_clear) {  // otherwise the last repair left it clear
@@ -1083,39 +1092,15 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
                                                         c->sssp_dp.as<unsigned long long>(), c->sssp_src,
                                                         b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(), cnt);
     // rounds are enqueued in groups (the walk's depth of the previous batch + 1
-    // first), the host checks the device's stop flag between groups
-    int group = c->hint_reset;
-    int more = 4;  // then a few more rounds at a time, widening
-    uint32_t r = 0;
-    while (true) {
-      for (int i = 0; i < group; ++i, ++r) {
-        k_reset_round<<<push_grid(), kBlock, 0, c->stream>>>(
-            c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(), c->sssp_dp.as<unsigned long long>(),
-            b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(), c->heavy.as<uint2>(), c->heavy.cap / sizeof(uint2) / 2,
-            cnt, r);
-      }
-      SUMINC_HIP(hipGetLastError());
-      read_counters(c);
-      if (c->host_counters->done) break;
-      group = more;
-      more = std::min(more * 2, 16);
-    }
-    c->hint_reset = std::max(2, (int)c->host_counters->aux[9] + 1);
-    c->stats.reserved[0] = (double)c->host_counters->aux[4];  // reset vertex count
+    // first); the transposed grouping below keeps the device busy while the host
+    // reads the stop flag of the first group
+    for (; walk_r < (uint32_t)c->hint_reset; ++walk_r) reset_round(walk_r);
+    SUMINC_HIP(hipGetLastError());
+    counters_begin(c);
   }
 
-  // 5. topology: CSR, then the CSC from the transposed groups
-  Groups gs;
-  gs.countp = bc;
-  gs.row = b.g_row.as<uint32_t>(); gs.col = b.g_col.as<uint32_t>(); gs.act = b.g_act.as<uint8_t>();
-  gs.w = b.g_w.as<uint32_t>(); gs.ex = b.g_ex.as<uint8_t>();
-  gs.ex_pre = b.g_expre.as<uint32_t>(); gs.add_pre = b.g_addpre.as<uint32_t>();
-  gs.tix = b.tix.as<uint32_t>(); gs.t_gbeg = b.t_gbeg.as<uint32_t>();
-  gs.t_list = b.t_list.as<uint32_t>(); gs.tcount = bc + 1;
-  gs.pos = b.g_pos.as<uint32_t>();
-  const uint64_t m_old = c->m;
-  merge_direction(c, c->out_off, c->out_col, c->out_w, b.noff_out, b.ncol_out, b.nw_out, gs, N, cap_m, 0);
-
+  // 5a. the transposed groups (CSC view), which neither the walk nor the CSR merge
+  // touch; positions of the transposed pairs in the old in-CSR
   k_transpose_group_keys<<<grid_for(N), kBlock, 0, c->stream>>>(b.g_row.as<uint32_t>(), b.g_col.as<uint32_t>(), bc,
                                                                 N, b.keys_a.as<uint64_t>(), b.idx_a.as<uint32_t>());
   cub_call(c, c->tmp, [&](void* p, size_t& bytes) {
@@ -1137,11 +1122,36 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   gt.ex_pre = b.tg_expre.as<uint32_t>(); gt.add_pre = b.tg_addpre.as<uint32_t>();
   gt.tix = b.ttix.as<uint32_t>(); gt.t_gbeg = b.tt_gbeg.as<uint32_t>();
   gt.t_list = b.tt_list.as<uint32_t>(); gt.tcount = bc + 2;
-  // positions of the transposed pairs in the old in-CSR
   k_probe<<<grid_for(N), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(), nullptr,
                                                  b.tg_row.as<uint32_t>(), b.tg_col.as<uint32_t>(), bc, nullptr,
                                                  nullptr, b.tg_pos.as<uint32_t>());
   gt.pos = b.tg_pos.as<uint32_t>();
+
+  // 4b. the rest of the walk, a few more rounds at a time, widening
+  if (sssp_repair) {
+    counters_wait(c);
+    int more = 4;
+    while (!c->host_counters->done) {
+      for (int i = 0; i < more; ++i, ++walk_r) reset_round(walk_r);
+      SUMINC_HIP(hipGetLastError());
+      read_counters(c);
+      more = std::min(more * 2, 16);
+    }
+    c->hint_reset = std::max(2, (int)c->host_counters->aux[9] + 1);
+    c->stats.reserved[0] = (double)c->host_counters->aux[4];  // reset vertex count
+  }
+
+  // 5b. topology: the CSR, then the CSC from the transposed groups
+  Groups gs;
+  gs.countp = bc;
+  gs.row = b.g_row.as<uint32_t>(); gs.col = b.g_col.as<uint32_t>(); gs.act = b.g_act.as<uint8_t>();
+  gs.w = b.g_w.as<uint32_t>(); gs.ex = b.g_ex.as<uint8_t>();
+  gs.ex_pre = b.g_expre.as<uint32_t>(); gs.add_pre = b.g_addpre.as<uint32_t>();
+  gs.tix = b.tix.as<uint32_t>(); gs.t_gbeg = b.t_gbeg.as<uint32_t>();
+  gs.t_list = b.t_list.as<uint32_t>(); gs.tcount = bc + 1;
+  gs.pos = b.g_pos.as<uint32_t>();
+  const uint64_t m_old = c->m;
+  merge_direction(c, c->out_off, c->out_col, c->out_w, b.noff_out, b.ncol_out, b.nw_out, gs, N, cap_m, 0);
   merge_direction(c, c->in_off, c->in_col, c->in_w, b.noff_in, b.ncol_in, b.nw_in, gt, N, cap_m, 2);
 
   // 6. PageRank reseed +1 on the new out-edges; the pull lists of the new CSC are

@@ -14,6 +14,24 @@ void read_counters(suminc_ctx* c) {
   SUMINC_HIP(hipStreamSynchronize(c->stream));
 }
 
+// Enqueues a copy of `bytes` (at most 256) device bytes into the pinned scratch and
+// records ev_check after it; check_wait() returns once the copy has landed, while
+// whatever was enqueued after check_begin() keeps the device busy.
+void check_begin(suminc_ctx* c, const void* dev, size_t bytes) {
+  SUMINC_REQUIRE(bytes <= 256, SUMINC_E_STATE, "check copy too large");
+  SUMINC_HIP(hipMemcpyAsync(c->host_scratch, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipEventRecord(c->ev_check, c->stream));
+}
+const void* check_wait(suminc_ctx* c) {
+  SUMINC_HIP(hipEventSynchronize(c->ev_check));
+  return c->host_scratch;
+}
+void counters_begin(suminc_ctx* c) {
+  SUMINC_HIP(hipMemcpyAsync(c->host_counters, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipEventRecord(c->ev_check, c->stream));
+}
+void counters_wait(suminc_ctx* c) { SUMINC_HIP(hipEventSynchronize(c->ev_check)); }
+
 hipEvent_t prof_event(suminc_ctx* c, size_t slot) {
   while (c->prof_events.size() <= slot) {
     hipEvent_t e;
@@ -104,6 +122,11 @@ int suminc_ctx_create(int device, const suminc_params* params, suminc_ctx** out)
     SUMINC_HIP(hipHostMalloc(&hp, sizeof(Counters), hipHostMallocDefault));
     c->host_counters = reinterpret_cast<Counters*>(hp);
     std::memset(c->host_counters, 0, sizeof(Counters));
+    void* hs = nullptr;
+    SUMINC_HIP(hipHostMalloc(&hs, 256, hipHostMallocDefault));
+    c->host_scratch = reinterpret_cast<unsigned long long*>(hs);
+    std::memset(c->host_scratch, 0, 256);
+    SUMINC_HIP(hipEventCreateWithFlags(&c->ev_check, hipEventDisableTiming));
     SUMINC_HIP(hipStreamSynchronize(c->stream));
   });
   if (rc != SUMINC_OK) {
@@ -119,6 +142,8 @@ int suminc_ctx_destroy(suminc_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->host_counters) (void)hipHostFree(c->host_counters);
+  if (c->host_scratch) (void)hipHostFree(c->host_scratch);
+  if (c->ev_check) (void)hipEventDestroy(c->ev_check);
   for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
   if (c->ev_start) (void)hipEventDestroy(c->ev_start);
   if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);

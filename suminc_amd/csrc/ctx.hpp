@@ -226,6 +226,10 @@ struct suminc_ctx {
   suminc::Sketch sk;
   suminc::DevBuf counters;                  // suminc::Counters
   suminc::Counters* host_counters = nullptr;  // pinned mirror
+  // pinned scratch for small device results the host reads while later kernels
+  // run (copied before ev_check, read after hipEventSynchronize(ev_check))
+  unsigned long long* host_scratch = nullptr;
+  hipEvent_t ev_check = nullptr;
 
   suminc_stats stats{};
 
@@ -254,6 +258,13 @@ inline Counters* dev_counters(suminc_ctx* c) { return c->counters.as<Counters>()
 
 // Copies the device counters block to the pinned mirror and waits.
 void read_counters(suminc_ctx* c);
+// A small device result read by the host while later kernels run: check_begin
+// enqueues its copy into pinned memory, check_wait waits for that copy only.
+void check_begin(suminc_ctx* c, const void* dev, size_t bytes);
+const void* check_wait(suminc_ctx* c);
+// The same for the whole counters block, into the pinned mirror.
+void counters_begin(suminc_ctx* c);
+void counters_wait(suminc_ctx* c);
 
 // --- graph helpers (graph.hip) ---
 void build_csc(suminc_ctx* c);  // from out-CSR

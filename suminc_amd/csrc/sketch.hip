@@ -747,6 +747,9 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
     return rocprim::exclusive_scan(p, b, cnt, base, ClBase{0ull, 0ull, 0u, 0u}, (size_t)K + 1, ClBasePlus(),
                                    c->stream);
   });
+  // the totals (entries, row lengths) size the row buffers below; the host reads
+  // them while the entry layout and the in-cluster arrays are built
+  check_begin(c, base + K, sizeof(ClBase));
   k_entry_layout<<<layout_grid(K), kBlock, 0, c->stream>>>(
       K, s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), s.slot_ent.as<uint8_t>(),
       s.touched.as<uint8_t>(), base, s.cl_eoff.as<uint32_t>(), s.cl_ent.as<uint32_t>(), s.ent_of.as<uint32_t>(),
@@ -767,9 +770,8 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
       s.cid.as<uint32_t>(), s.lidx.as<uint32_t>(), c->out_off.as<uint32_t>(), s.ie_off.as<uint32_t>(),
       s.ie_src.as<uint32_t>(), s.ie_dst.as<uint32_t>(), s.ie_deg.as<uint32_t>(), s.ie_wt.as<uint32_t>());
   SUMINC_HIP(hipGetLastError());
-  ClBase tot{};
-  SUMINC_HIP(hipMemcpyAsync(&tot, base + K, sizeof(ClBase), hipMemcpyDeviceToHost, c->stream));
-  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  ClBase tot;
+  std::memcpy(&tot, check_wait(c), sizeof(ClBase));
   s.E = tot.ent;
   s.sc_len = tot.sc;
   s.ss_len = tot.ss;
