@@ -243,19 +243,15 @@ __device__ __forceinline__ T gather_sum(const uint32_t* __restrict__ icol, const
 }
 
 // Pull, short rows (in-degree 1..kPullThread): one thread per vertex sums its
-// in-edges in order; no atomics (each vertex has one owner).
-template <class T, bool kSketch = false>
-__global__ void __launch_bounds__(kBlock) k_pr_pull_short(const uint32_t* __restrict__ list,
-                                                          const unsigned long long* __restrict__ counts,
-                                                          const uint32_t* __restrict__ ioff,
-                                                          const uint32_t* __restrict__ icol,
-                                                          const T* __restrict__ contrib, T* __restrict__ delta,
-                                                          const Counters* cnt, int gate,
-                                                          const uint32_t* __restrict__ cid,
-                                                          const uint8_t* __restrict__ chk) {
-  if (!pull_runs(cnt, gate)) return;
-  const uint32_t count = (uint32_t)counts[0];
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
+// in-edges in order; no atomics (each vertex has one owner).  bid / nb: this
+// workgroup's index among the nb workgroups of the part.
+template <class T, bool kSketch>
+__device__ __forceinline__ void pull_short_part(uint32_t bid, uint32_t nb, const uint32_t* __restrict__ list,
+                                                uint32_t count, const uint32_t* __restrict__ ioff,
+                                                const uint32_t* __restrict__ icol, const T* __restrict__ contrib,
+                                                T* __restrict__ delta, const uint32_t* __restrict__ cid,
+                                                const uint8_t* __restrict__ chk) {
+  for (uint32_t i = bid * kBlock + threadIdx.x; i < count; i += nb * kBlock) {
     const uint32_t v = list[i];
     const uint32_t b = ioff[v], e = ioff[v + 1];
     const uint32_t cv = pull_cluster<kSketch>(cid, chk, v);
@@ -267,19 +263,14 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_short(const uint32_t* __rest
 
 // Pull, medium rows (kPullThread < in-degree <= kHeavyDeg): one wave per vertex,
 // lanes stride the row (coalesced column reads), shuffle reduction.
-template <class T, bool kSketch = false>
-__global__ void __launch_bounds__(kBlock) k_pr_pull_wave(const uint32_t* __restrict__ list,
-                                                         const unsigned long long* __restrict__ counts,
-                                                         const uint32_t* __restrict__ ioff,
-                                                         const uint32_t* __restrict__ icol,
-                                                         const T* __restrict__ contrib, T* __restrict__ delta,
-                                                         const Counters* cnt, int gate,
-                                                         const uint32_t* __restrict__ cid,
-                                                         const uint8_t* __restrict__ chk) {
-  if (!pull_runs(cnt, gate)) return;
-  const uint32_t count = (uint32_t)counts[1];
-  const uint32_t waves = gridDim.x * (kBlock / 64);
-  for (uint32_t i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); i < count; i += waves) {
+template <class T, bool kSketch>
+__device__ __forceinline__ void pull_wave_part(uint32_t bid, uint32_t nb, const uint32_t* __restrict__ list,
+                                               uint32_t count, const uint32_t* __restrict__ ioff,
+                                               const uint32_t* __restrict__ icol, const T* __restrict__ contrib,
+                                               T* __restrict__ delta, const uint32_t* __restrict__ cid,
+                                               const uint8_t* __restrict__ chk) {
+  const uint32_t waves = nb * (kBlock / 64);
+  for (uint32_t i = bid * (kBlock / 64) + (threadIdx.x >> 6); i < count; i += waves) {
     const uint32_t v = list[i];
     const uint32_t b = ioff[v], e = ioff[v + 1];
     const uint32_t cv = pull_cluster<kSketch>(cid, chk, v);
@@ -292,20 +283,15 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_wave(const uint32_t* __restr
   }
 }
 
-// Pull, heavy part: 2048-edge chunks of vertices with in-degree > 256.
-template <class T, bool kSketch = false>
-__global__ void __launch_bounds__(kBlock) k_pr_pull_heavy(const uint32_t* __restrict__ ioff,
-                                                          const uint32_t* __restrict__ icol,
-                                                          const T* __restrict__ contrib, T* __restrict__ delta,
-                                                          const uint2* __restrict__ chunks,
-                                                          const unsigned long long* __restrict__ counts,
-                                                          const Counters* cnt, int gate,
-                                                          const uint32_t* __restrict__ cid,
-                                                          const uint8_t* __restrict__ chk) {
-  __shared__ T s_red[kBlock / 64];
-  if (!pull_runs(cnt, gate)) return;
-  const uint64_t nchunks = counts[2];
-  for (uint64_t h = blockIdx.x; h < nchunks; h += gridDim.x) {
+// Pull, heavy part: 2048-edge chunks of vertices with in-degree > 256, one
+// workgroup per chunk, one atomic per chunk.
+template <class T, bool kSketch>
+__device__ __forceinline__ void pull_heavy_part(uint32_t bid, uint32_t nb, T* s_red, const uint2* __restrict__ chunks,
+                                                uint64_t nchunks, const uint32_t* __restrict__ ioff,
+                                                const uint32_t* __restrict__ icol, const T* __restrict__ contrib,
+                                                T* __restrict__ delta, const uint32_t* __restrict__ cid,
+                                                const uint8_t* __restrict__ chk) {
+  for (uint64_t h = bid; h < nchunks; h += nb) {
     const uint2 ch = chunks[h];
     const uint32_t b = ioff[ch.x] + ch.y * kChunk;
     const uint32_t e = min(ioff[ch.x + 1], b + kChunk);
@@ -322,6 +308,31 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_heavy(const uint32_t* __rest
     }
     __syncthreads();
   }
+}
+
+// One launch for the three row classes: workgroups [0, gh) take the heavy chunks
+// (the longest work first), [gh, gh + gw) the wave rows, the rest the short rows,
+// so the classes' tails overlap instead of following each other.
+template <class T, bool kSketch = false>
+__global__ void __launch_bounds__(kBlock) k_pr_pull(const uint32_t* __restrict__ short_list,
+                                                    const uint32_t* __restrict__ wave_list,
+                                                    const uint2* __restrict__ chunks,
+                                                    const unsigned long long* __restrict__ counts,
+                                                    const uint32_t* __restrict__ ioff,
+                                                    const uint32_t* __restrict__ icol, const T* __restrict__ contrib,
+                                                    T* __restrict__ delta, const Counters* cnt, int gate,
+                                                    const uint32_t* __restrict__ cid,
+                                                    const uint8_t* __restrict__ chk, uint32_t gh, uint32_t gw) {
+  __shared__ T s_red[kBlock / 64];
+  if (!pull_runs(cnt, gate)) return;
+  const uint32_t bid = blockIdx.x;
+  if (bid < gh)
+    pull_heavy_part<T, kSketch>(bid, gh, s_red, chunks, counts[2], ioff, icol, contrib, delta, cid, chk);
+  else if (bid < gh + gw)
+    pull_wave_part<T, kSketch>(bid - gh, gw, wave_list, (uint32_t)counts[1], ioff, icol, contrib, delta, cid, chk);
+  else
+    pull_short_part<T, kSketch>(bid - gh - gw, gridDim.x - gh - gw, short_list, (uint32_t)counts[0], ioff, icol,
+                                contrib, delta, cid, chk);
 }
 
 __global__ void __launch_bounds__(1024) k_pr_end(Counters* cnt, const unsigned long long* __restrict__ act_partial,
@@ -884,20 +895,14 @@ void pull_scatter(suminc_ctx* c, const uint32_t* cid, const uint8_t* chk, int ga
     }
     return;
   }
-  auto ks = sk ? k_pr_pull_short<T, true> : k_pr_pull_short<T, false>;
-  ks<<<grid_for(c->n, kBlock, 8192), kBlock, 0, c->stream>>>(c->pull_short.as<uint32_t>(), counts,
-                                                              c->in_off.as<uint32_t>(), icol,
-                                                              c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, gate,
-                                                              cid, chk);
-  auto kw = sk ? k_pr_pull_wave<T, true> : k_pr_pull_wave<T, false>;
-  kw<<<grid_for(c->n, kBlock / 64, 8192), kBlock, 0, c->stream>>>(c->pull_wave.as<uint32_t>(), counts,
-                                                                   c->in_off.as<uint32_t>(), icol,
-                                                                   c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt,
-                                                                   gate, cid, chk);
-  auto kh = sk ? k_pr_pull_heavy<T, true> : k_pr_pull_heavy<T, false>;
-  kh<<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), icol,
-                                            c->pr_contrib.as<T>(), c->pr_delta.as<T>(), c->pull_heavy.as<uint2>(),
-                                            counts, cnt, gate, cid, chk);
+  // grid: the heavy part as before (push_grid), the wave and short parts sized to
+  // the vertex count (at most 8192 workgroups each)
+  const uint32_t gh = push_grid(), gw = grid_for(c->n, kBlock / 64, 8192), gs = grid_for(c->n, kBlock, 8192);
+  auto kp = sk ? k_pr_pull<T, true> : k_pr_pull<T, false>;
+  kp<<<gh + gw + gs, kBlock, 0, c->stream>>>(c->pull_short.as<uint32_t>(), c->pull_wave.as<uint32_t>(),
+                                             c->pull_heavy.as<uint2>(), counts, c->in_off.as<uint32_t>(), icol,
+                                             c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, gate, cid, chk, gh,
+                                             gw);
 }
 template void pull_scatter<float>(suminc_ctx* c, const uint32_t* cid, const uint8_t* chk, int gate);
 template void pull_scatter<double>(suminc_ctx* c, const uint32_t* cid, const uint8_t* chk, int gate);
