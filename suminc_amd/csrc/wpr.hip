@@ -53,55 +53,75 @@ struct RowLists {
   uint64_t n_heavy;
 };
 
-// y[v] = sum_{e in row v} a[e] x[col[e]]; y must be zero before (heavy rows add).
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_spmv_short(RowLists L, const T* __restrict__ a, const T* __restrict__ x,
-                                                       T* __restrict__ y, const Counters* cnt) {
-  if (cnt && cnt->done) return;
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < L.n_short; i += gridDim.x * kBlock) {
-    const uint32_t v = L.shrt[i];
-    T acc = T(0);
-    for (uint32_t e = L.off[v]; e < L.off[v + 1]; ++e) acc += a[e] * x[L.col[e]];
-    y[v] = acc;
-  }
-}
-
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_spmv_wave(RowLists L, const T* __restrict__ a, const T* __restrict__ x,
-                                                      T* __restrict__ y, const Counters* cnt) {
-  if (cnt && cnt->done) return;
-  const uint32_t waves = gridDim.x * (kBlock / 64);
-  for (uint32_t i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); i < L.n_wave; i += waves) {
-    const uint32_t v = L.wave[i];
-    T acc = T(0);
-    for (uint32_t e = L.off[v] + lane_id(); e < L.off[v + 1]; e += 64) acc += a[e] * x[L.col[e]];
+// Sum of a[k] * x[col[k]] over k = k0 + r * stride (r < U, k < e).  The U column
+// and coefficient loads are issued before the U gathers, so a thread keeps U
+// independent gathers in flight.
+template <class T, int U>
+__device__ __forceinline__ T spmv_sum(const uint32_t* __restrict__ col, const T* __restrict__ a,
+                                      const T* __restrict__ x, uint32_t k0, uint32_t stride, uint32_t e) {
+  uint32_t u[U];
+  T w[U];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if (lane_id() == 0) y[v] = acc;
+  for (int r = 0; r < U; ++r) {
+    const uint32_t k = k0 + r * stride;
+    const bool in = k < e;
+    u[r] = in ? col[k] : 0u;
+    w[r] = in ? a[k] : T(0);
   }
+  T acc = T(0);
+#pragma unroll
+  for (int r = 0; r < U; ++r)
+    if (w[r] != T(0)) acc += w[r] * x[u[r]];
+  return acc;
 }
 
+// y[v] = sum_{e in row v} a[e] x[col[e]]; y must be zero before (heavy rows add).
+// One launch for the three row classes: workgroups [0, gh) take the heavy
+// chunks (one workgroup per chunk, one atomic), [gh, gh + gw) the wave rows (one
+// wave per row), the rest the short rows (one thread per row).
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_spmv_heavy(RowLists L, const T* __restrict__ a, const T* __restrict__ x,
-                                                       T* __restrict__ y, const Counters* cnt) {
+__global__ void __launch_bounds__(kBlock) k_spmv(RowLists L, const T* __restrict__ a, const T* __restrict__ x,
+                                                 T* __restrict__ y, const Counters* cnt, uint32_t gh, uint32_t gw) {
   __shared__ T s_red[kBlock / 64];
   if (cnt && cnt->done) return;
-  for (uint64_t h = blockIdx.x; h < L.n_heavy; h += gridDim.x) {
-    const uint2 ch = L.heavy[h];
-    const uint32_t b = L.off[ch.x] + ch.y * kChunk;
-    const uint32_t e = min(L.off[ch.x + 1], b + kChunk);
-    T acc = T(0);
-    for (uint32_t k = b + threadIdx.x; k < e; k += kBlock) acc += a[k] * x[L.col[k]];
+  const uint32_t bid = blockIdx.x;
+  if (bid < gh) {
+    for (uint64_t h = bid; h < L.n_heavy; h += gh) {
+      const uint2 ch = L.heavy[h];
+      const uint32_t b = L.off[ch.x] + ch.y * kChunk;
+      const uint32_t e = min(L.off[ch.x + 1], b + kChunk);
+      T acc = spmv_sum<T, kChunk / kBlock>(L.col, a, x, b + threadIdx.x, kBlock, e);
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if (lane_id() == 0) s_red[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      T s = T(0);
-      for (int w = 0; w < kBlock / 64; ++w) s += s_red[w];
-      if (s != T(0)) atomicAdd(&y[ch.x], s);
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+      if (lane_id() == 0) s_red[threadIdx.x >> 6] = acc;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        T s = T(0);
+        for (int w = 0; w < kBlock / 64; ++w) s += s_red[w];
+        if (s != T(0)) atomicAdd(&y[ch.x], s);
+      }
+      __syncthreads();
     }
-    __syncthreads();
+  } else if (bid < gh + gw) {
+    const uint32_t waves = gw * (kBlock / 64);
+    for (uint32_t i = (bid - gh) * (kBlock / 64) + (threadIdx.x >> 6); i < L.n_wave; i += waves) {
+      const uint32_t v = L.wave[i];
+      const uint32_t b = L.off[v], e = L.off[v + 1];
+      T acc = T(0);
+      for (uint32_t k = b + lane_id(); k < e; k += 4 * 64) acc += spmv_sum<T, 4>(L.col, a, x, k, 64, e);
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+      if (lane_id() == 0) y[v] = acc;
+    }
+  } else {
+    const uint32_t nb = gridDim.x - gh - gw;
+    for (uint32_t i = (bid - gh - gw) * kBlock + threadIdx.x; i < L.n_short; i += nb * kBlock) {
+      const uint32_t v = L.shrt[i];
+      const uint32_t b = L.off[v], e = L.off[v + 1];
+      T acc = T(0);
+      for (uint32_t k = b; k < e; k += 4) acc += spmv_sum<T, 4>(L.col, a, x, k, 1, e);
+      y[v] = acc;
+    }
   }
 }
 
@@ -458,11 +478,10 @@ static RowLists lists_of(const WprDir& L, const DevBuf& off, const DevBuf& col, 
 
 template <class T>
 static void spmv(suminc_ctx* c, const RowLists& R, const T* a, const T* x, T* y, const Counters* cnt) {
-  if (R.n_short)
-    k_spmv_short<T><<<grid_for(R.n_short, kBlock, 8192), kBlock, 0, c->stream>>>(R, a, x, y, cnt);
-  if (R.n_wave)
-    k_spmv_wave<T><<<grid_for(R.n_wave, kBlock / 64, 8192), kBlock, 0, c->stream>>>(R, a, x, y, cnt);
-  if (R.n_heavy) k_spmv_heavy<T><<<push_grid(), kBlock, 0, c->stream>>>(R, a, x, y, cnt);
+  const uint32_t gh = R.n_heavy ? push_grid() : 0u;
+  const uint32_t gw = R.n_wave ? grid_for(R.n_wave, kBlock / 64, 8192) : 0u;
+  const uint32_t gs = R.n_short ? grid_for(R.n_short, kBlock, 8192) : 0u;
+  if (gh + gw + gs) k_spmv<T><<<gh + gw + gs, kBlock, 0, c->stream>>>(R, a, x, y, cnt, gh, gw);
 }
 
 // Runs sum = sum_t x_t with x_{t+1} = A x_t from x_0 = scale * seed until
