@@ -234,6 +234,15 @@ __global__ void __launch_bounds__(kBlock) k_new_off(const uint32_t* __restrict__
   __shared__ uint32_t s_row[kOffTile];
   __shared__ uint32_t s_delta[kOffTile + 1];
   const uint32_t tile = blockIdx.x;
+  // the tile's old row pointers are loaded first: their latency overlaps the
+  // staging of the touched rows
+  constexpr int kPer = kOffTile / kBlock;
+  uint32_t ov[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const uint32_t v = tile * kOffTile + j * kBlock + threadIdx.x;
+    ov[j] = v <= n ? off[v] : 0u;
+  }
   const uint32_t lo = vts[tile], hi = vts[tile + 1], r = hi - lo;  // touched rows of the tile (<= kOffTile)
   for (uint32_t i = threadIdx.x; i <= r; i += kBlock) {
     const uint32_t g = gs.t_gbeg[lo + i];
@@ -241,7 +250,8 @@ __global__ void __launch_bounds__(kBlock) k_new_off(const uint32_t* __restrict__
     s_delta[i] = gs.add_pre[g] - gs.ex_pre[g];
   }
   __syncthreads();
-  for (uint32_t j = 0; j < 8; ++j) {
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
     const uint32_t v = tile * kOffTile + j * kBlock + threadIdx.x;
     if (v > n) break;
     uint32_t a = 0, b = r;  // first touched row >= v
@@ -249,7 +259,7 @@ __global__ void __launch_bounds__(kBlock) k_new_off(const uint32_t* __restrict__
       const uint32_t mid = (a + b) >> 1;
       if (s_row[mid] < v) a = mid + 1; else b = mid;
     }
-    noff[v] = off[v] + s_delta[a];
+    noff[v] = ov[j] + s_delta[a];
   }
 }
 
