@@ -39,6 +39,8 @@
 
 namespace suminc {
 
+constexpr int kCountPer = 4;  // vertices per thread of k_pr_count
+
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_pr_count(uint32_t n, const uint32_t* __restrict__ off,
                                                      const T* __restrict__ delta, double eps, Counters* cnt,
@@ -46,12 +48,23 @@ __global__ void __launch_bounds__(kBlock) k_pr_count(uint32_t n, const uint32_t*
   __shared__ double s_sum[kBlock / 64];
   __shared__ unsigned long long s_edges[kBlock / 64];
   if (cnt->done) return;
-  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  // kCountPer vertices a thread (coalesced: vertex base + j * kBlock + t), their
+  // residual loads issued before any is used
+  const uint32_t base = blockIdx.x * (kBlock * kCountPer);
+  T dl[kCountPer];
+#pragma unroll
+  for (int j = 0; j < kCountPer; ++j) {
+    const uint32_t v = base + j * kBlock + threadIdx.x;
+    dl[j] = v < n ? delta[v] : T(0);
+  }
   double absd = 0.0;
   unsigned long long deg = 0;
-  if (v < n) {
-    absd = (double)fabs_t(delta[v]);
-    if (absd > eps) deg = off[v + 1] - off[v];
+#pragma unroll
+  for (int j = 0; j < kCountPer; ++j) {
+    const uint32_t v = base + j * kBlock + threadIdx.x;
+    const double a = (double)fabs_t(dl[j]);
+    absd += a;
+    if (v < n && a > eps) deg += off[v + 1] - off[v];
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -913,10 +926,11 @@ template <class T>
 static void enqueue_round(suminc_ctx* c, double eps, double T_stop, int max_rounds, int div, size_t slot) {
   Counters* cnt = dev_counters(c);
   const unsigned g = grid_for(c->n);
+  const unsigned gc = grid_for(c->n, kBlock * kCountPer);  // the count's partials (gc <= g)
   if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 4 * slot), c->stream));
-  k_pr_count<T><<<g, kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), c->pr_delta.as<T>(), eps, cnt,
-                                             c->partial.as<double>());
-  k_pr_decide<<<1, 1024, 0, c->stream>>>(c->partial.as<double>(), g, cnt, T_stop, max_rounds, c->m, div,
+  k_pr_count<T><<<gc, kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), c->pr_delta.as<T>(), eps, cnt,
+                                              c->partial.as<double>());
+  k_pr_decide<<<1, 1024, 0, c->stream>>>(c->partial.as<double>(), gc, cnt, T_stop, max_rounds, c->m, div,
                                          c->tmp4.as<uint8_t>());
   if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 4 * slot + 1), c->stream));
   k_pr_gather<T><<<g, kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), c->pr_delta.as<T>(),
