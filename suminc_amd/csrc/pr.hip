@@ -39,7 +39,7 @@
 
 namespace suminc {
 
-constexpr int kCountPer = 4;  // vertices per thread of k_pr_count
+constexpr int kCountPer = 4;  // vertices per thread of k_pr_count and k_pr_gather
 
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_pr_count(uint32_t n, const uint32_t* __restrict__ off,
@@ -139,64 +139,77 @@ __global__ void __launch_bounds__(kBlock) k_pr_gather(uint32_t n, const uint32_t
   __shared__ unsigned long long s_act[kBlock / 64];
   if (cnt->done) return;
   const bool pull = cnt->mode != 0;
-  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-  bool light = false, active = false;
-  T c = T(0);
-  if (v < n) {
-    T dv = delta[v];
-    if ((double)fabs_t(dv) > eps) {
-      active = true;
-      delta[v] = T(0);
-      if constexpr (sizeof(T) == 4) {
-        float r = rank[v], cc = comp[v];
-        float y = (float)dv - cc;
-        float t = r + y;
-        comp[v] = (t - r) - y;
-        rank[v] = t;
-      } else {
-        rank[v] += dv;
-      }
-      uint32_t b = off[v], e = off[v + 1];
-      uint32_t deg = e - b;
-      if (deg > 0) {
-        c = d * dv / (T)deg;
-        if (!pull) {
-          if (deg <= kHeavyDeg) {
-            light = true;
-          } else {
-            uint32_t nch = (deg + kChunk - 1) / kChunk;
-            unsigned long long slot = atomicAdd(&cnt->heavy_count, (unsigned long long)nch);
-            for (uint32_t k = 0; k < nch; ++k) {
-              uint4 h;
-              h.x = v;
-              h.y = k;
-              if constexpr (sizeof(T) == 4) {
-                h.z = __float_as_uint((float)c);
-                h.w = 0;
-              } else {
-                unsigned long long bits = __double_as_longlong((double)c);
-                h.z = (uint32_t)bits;
-                h.w = (uint32_t)(bits >> 32);
+  // kCountPer vertices a thread (vertex base + j * kBlock + t, coalesced), the
+  // residual loads issued first; every lane runs every j (the wave appends)
+  const uint32_t base = blockIdx.x * (kBlock * kCountPer);
+  T dl[kCountPer];
+#pragma unroll
+  for (int j = 0; j < kCountPer; ++j) {
+    const uint32_t v = base + j * kBlock + threadIdx.x;
+    dl[j] = v < n ? delta[v] : T(0);
+  }
+  unsigned long long nact = 0;
+#pragma unroll
+  for (int j = 0; j < kCountPer; ++j) {
+    const uint32_t v = base + j * kBlock + threadIdx.x;
+    bool light = false;
+    T c = T(0);
+    if (v < n) {
+      const T dv = dl[j];
+      if ((double)fabs_t(dv) > eps) {
+        ++nact;
+        delta[v] = T(0);
+        if constexpr (sizeof(T) == 4) {
+          float r = rank[v], cc = comp[v];
+          float y = (float)dv - cc;
+          float t = r + y;
+          comp[v] = (t - r) - y;
+          rank[v] = t;
+        } else {
+          rank[v] += dv;
+        }
+        uint32_t b = off[v], e = off[v + 1];
+        uint32_t deg = e - b;
+        if (deg > 0) {
+          c = d * dv / (T)deg;
+          if (!pull) {
+            if (deg <= kHeavyDeg) {
+              light = true;
+            } else {
+              uint32_t nch = (deg + kChunk - 1) / kChunk;
+              unsigned long long slot = atomicAdd(&cnt->heavy_count, (unsigned long long)nch);
+              for (uint32_t k = 0; k < nch; ++k) {
+                uint4 h;
+                h.x = v;
+                h.y = k;
+                if constexpr (sizeof(T) == 4) {
+                  h.z = __float_as_uint((float)c);
+                  h.w = 0;
+                } else {
+                  unsigned long long bits = __double_as_longlong((double)c);
+                  h.z = (uint32_t)bits;
+                  h.w = (uint32_t)(bits >> 32);
+                }
+                heavy[slot + k] = h;
               }
-              heavy[slot + k] = h;
             }
           }
         }
       }
+      if (pull) {  // the contribution of every vertex (pos: of every vertex with out-edges, at its compact position)
+        const uint32_t p = pos ? pos[v] : v;
+        if (p != INF32) contrib[p] = c;
+      }
     }
-    if (pull) {  // the contribution of every vertex (pos: of every vertex with out-edges, at its compact position)
-      const uint32_t p = pos ? pos[v] : v;
-      if (p != INF32) contrib[p] = c;
+    if (!pull) {
+      unsigned long long slot = wave_append(light, &cnt->wl_count[0]);
+      if (light) {
+        wl_v[slot] = v;
+        wl_c[slot] = c;
+      }
     }
   }
-  if (!pull) {
-    unsigned long long slot = wave_append(light, &cnt->wl_count[0]);
-    if (light) {
-      wl_v[slot] = v;
-      wl_c[slot] = c;
-    }
-  }
-  unsigned long long a = active ? 1ull : 0ull;
+  unsigned long long a = nact;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
   if (lane_id() == 0) s_act[threadIdx.x >> 6] = a;
@@ -926,14 +939,15 @@ template <class T>
 static void enqueue_round(suminc_ctx* c, double eps, double T_stop, int max_rounds, int div, size_t slot) {
   Counters* cnt = dev_counters(c);
   const unsigned g = grid_for(c->n);
-  const unsigned gc = grid_for(c->n, kBlock * kCountPer);  // the count's partials (gc <= g)
+  // count and gather take kCountPer vertices a thread: gc (<= g) workgroups and partials
+  const unsigned gc = grid_for(c->n, kBlock * kCountPer);
   if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 4 * slot), c->stream));
   k_pr_count<T><<<gc, kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), c->pr_delta.as<T>(), eps, cnt,
                                               c->partial.as<double>());
   k_pr_decide<<<1, 1024, 0, c->stream>>>(c->partial.as<double>(), gc, cnt, T_stop, max_rounds, c->m, div,
                                          c->tmp4.as<uint8_t>());
   if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 4 * slot + 1), c->stream));
-  k_pr_gather<T><<<g, kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), c->pr_delta.as<T>(),
+  k_pr_gather<T><<<gc, kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), c->pr_delta.as<T>(),
                                               c->pr_rank.as<T>(), c->pr_comp.as<float>(), eps, (T)c->prm.damping,
                                               c->wl0.as<uint32_t>(), c->wl1.as<T>(), c->heavy.as<uint4>(),
                                               c->pr_contrib.as<T>(), pull_pos(c), cnt,
@@ -944,7 +958,8 @@ static void enqueue_round(suminc_ctx* c, double eps, double T_stop, int max_roun
                                                       c->heavy.as<uint4>(), cnt);
   if (div > 0) pull_scatter<T>(c, nullptr, nullptr, 0);
   if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 4 * slot + 3), c->stream));
-  k_pr_end<<<1, 1024, 0, c->stream>>>(cnt, reinterpret_cast<unsigned long long*>(c->partial.as<double>()) + 2 * g, g);
+  k_pr_end<<<1, 1024, 0, c->stream>>>(cnt, reinterpret_cast<unsigned long long*>(c->partial.as<double>()) + 2 * g,
+                                      gc);
 }
 
 // Adds the profile of the first `rounds` enqueued rounds (the ones that did work).
