@@ -139,7 +139,10 @@ __device__ __forceinline__ void kahan(T* sum, float* comp, uint32_t v, T x) {
 }
 
 // One vertex pass of a round: sum += x (compensated), x <- y, y <- 0, and the
-// FP64 block partial of sum |x_new|.
+// FP64 block partial of sum |x_new|.  kStepPer vertices a thread (vertex base +
+// j * kBlock + t, coalesced), their x and y loads issued first.
+constexpr int kStepPer = 4;
+
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_wpr_step(uint32_t n, T* __restrict__ x, T* __restrict__ y,
                                                      T* __restrict__ sum, float* __restrict__ comp,
@@ -147,15 +150,25 @@ __global__ void __launch_bounds__(kBlock) k_wpr_step(uint32_t n, T* __restrict__
                                                      const uint32_t* __restrict__ pos, T* __restrict__ xc) {
   __shared__ double s_p[kBlock / 64];
   if (cnt->done) return;
-  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t base = blockIdx.x * (kBlock * kStepPer);
+  T xo[kStepPer], xn[kStepPer];
+#pragma unroll
+  for (int j = 0; j < kStepPer; ++j) {
+    const uint32_t v = base + j * kBlock + threadIdx.x;
+    xo[j] = v < n ? x[v] : T(0);
+    xn[j] = v < n ? y[v] : T(0);
+  }
   double a = 0;
-  if (v < n) {
-    const T xo = x[v], xn = y[v];
-    if (xo != T(0)) kahan(sum, comp, v, xo);
-    x[v] = xn;
-    y[v] = T(0);
-    if (pos && pos[v] != INF32) xc[pos[v]] = xn;  // the next round gathers the compact copy
-    a = fabs((double)xn);
+#pragma unroll
+  for (int j = 0; j < kStepPer; ++j) {
+    const uint32_t v = base + j * kBlock + threadIdx.x;
+    if (v < n) {
+      if (xo[j] != T(0)) kahan(sum, comp, v, xo[j]);
+      x[v] = xn[j];
+      y[v] = T(0);
+      if (pos && pos[v] != INF32) xc[pos[v]] = xn[j];  // the next round gathers the compact copy
+      a += fabs((double)xn[j]);
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
@@ -501,6 +514,7 @@ static void wpr_series(suminc_ctx* c, WprState& S, const RowLists& R, const uint
   S.comp.ensure((size_t)n * 4 + 4);
   S.l1.ensure(8);
   const unsigned g = grid_for(n);
+  const unsigned gs = grid_for(n, kBlock * kStepPer);  // the round's vertex pass and its partials
   c->partial.ensure((size_t)(g + 1) * 8 * 3);
   Counters* cnt = dev_counters(c);
   SUMINC_HIP(hipMemsetAsync(cnt, 0, sizeof(Counters), c->stream));
@@ -520,9 +534,9 @@ static void wpr_series(suminc_ctx* c, WprState& S, const RowLists& R, const uint
   while (true) {
     for (int i = 0; i < group; ++i) {
       spmv<T>(c, R, a, pos ? xc : S.x.as<T>(), S.y.as<T>(), cnt);
-      k_wpr_step<T><<<g, kBlock, 0, c->stream>>>(n, S.x.as<T>(), S.y.as<T>(), S.sum.as<T>(), S.comp.as<float>(),
-                                                 c->partial.as<double>(), cnt, pos, xc);
-      k_wpr_decide<<<1, 1024, 0, c->stream>>>(c->partial.as<double>(), g, cnt, tol, max_rounds, c->m);
+      k_wpr_step<T><<<gs, kBlock, 0, c->stream>>>(n, S.x.as<T>(), S.y.as<T>(), S.sum.as<T>(),
+                                                  S.comp.as<float>(), c->partial.as<double>(), cnt, pos, xc);
+      k_wpr_decide<<<1, 1024, 0, c->stream>>>(c->partial.as<double>(), gs, cnt, tol, max_rounds, c->m);
     }
     SUMINC_HIP(hipGetLastError());
     read_counters(c);
