@@ -296,7 +296,7 @@ def run_pr(args, wl):
     for b in range(args.warmup, args.warmup + args.steps):
         sa = ctx.apply_batch(batches[b])
         st = run()
-        edges += st["edges_traversed"] + sa["reserved"][3]
+        edges += st["edges_traversed"] + sa["reseed_edges"]
     ctx.sync()
     dt = time.perf_counter() - t0
     kids = [L.KERNEL_PR_PUSH, L.KERNEL_PR_PULL, L.KERNEL_PR_GATHER, L.KERNEL_BATCH_COPY]
@@ -403,7 +403,7 @@ def run_web(args, wl):
         ss = ctx.sum_sssp_run()
         ctx.sync()
         t3 = time.perf_counter()
-        e = sp["edges_traversed"] + sa["reserved"][3] + ss["edges_traversed"]
+        e = sp["edges_traversed"] + sa["reseed_edges"] + ss["edges_traversed"]
         edges += e
         per.append({"batch": b, "ms": round(1e3 * (t3 - t0), 3), "apply_ms": round(1e3 * (t1 - t0), 3),
                     "pr_ms": round(1e3 * (t2 - t1), 3), "sssp_ms": round(1e3 * (t3 - t2), 3),

@@ -95,7 +95,24 @@ typedef struct suminc_stats {
   double residual;            /* FP64 sum|delta| at stop (PageRank) */
   double kernel_ms;           /* device time of the last run, from HIP events */
   double push_rounds, pull_rounds;
-  double reserved[10];
+  /* PageRank runs: the FP64 mass balance of the final state, computed on the device.
+   * Every push (r += delta, d * delta spread over the out-edges) and every reseed
+   * keeps  sum r + sum delta - d * sum_{outdeg(v) > 0} r_v  equal to its initial
+   * value (1 - d) (pagerank_ingress.h:27-70; the reference prints the sums in
+   * check_result, sum_sync_iter_worker.h:2851-2871).  mass_defect is that sum minus
+   * (1 - d): rounding only, for a state that suminc_pr_init started. */
+  double mass_defect;
+  double mass_total;          /* FP64 sum r + sum delta */
+  double reset_vertices;      /* apply_batch: SSSP vertices reset with their dependency subtree */
+  double batch_groups;        /* apply_batch: distinct (u, v) pairs named by the ops */
+  double touched_rows;        /* apply_batch: out-CSR rows the batch changed */
+  double reseed_edges;        /* apply_batch: edges traversed by the PageRank reseed */
+  double shortcut_edges;      /* internal edges read by the last shortcut-row build */
+  double sketch_rounds;       /* SumInc queries: rounds of the sketch phase */
+  double sketch_ms;           /* SumInc PageRank query: device time of the sketch phase */
+  double residual_rounds;     /* SumInc PageRank query: rounds of the plain residual phase */
+  double precompute_edges;    /* SumInc queries: internal edges read by the cluster precompute */
+  double reserved[7];
 } suminc_stats;
 
 typedef struct suminc_ctx suminc_ctx;
@@ -263,7 +280,10 @@ int suminc_sum_sssp_run(suminc_ctx* ctx, suminc_stats* stats);
  * gives lambda; dL/dp = (1-d) lambda, dL/dw_uv = (d r_u / s_u)(lambda_v - sum_k W[u,k]
  * lambda_k).  State in FP32 with Kahan sums or FP64 (pr_precision).  Vectors are n
  * floats, edge weights and their gradient m floats in out-CSR order; every pointer
- * may be host or device memory of the context's GPU.  All four synchronise. */
+ * may be host or device memory of the context's GPU.  All four synchronise.
+ * suminc_wpr_backward differentiates the context's last forward (its rank and
+ * weights); a caller that runs several forwards before their backwards runs the
+ * forward again first (SUMINC_E_STATE when the weights or the graph changed since). */
 int suminc_wpr_set_weights(suminc_ctx* ctx, const float* weights);
 int suminc_wpr_forward(suminc_ctx* ctx, const float* personalization, float* rank, suminc_stats* stats);
 int suminc_wpr_backward(suminc_ctx* ctx, const float* grad_rank, float* grad_personalization, float* grad_weights,

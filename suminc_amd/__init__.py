@@ -6,6 +6,12 @@ after edge-update batches.  The device code and the C ABI live in csrc/ and are
 built into libsuminc.so (include/suminc.h); this package is the Python handle.
 """
 from ._lib import (  # noqa: F401
+    E_INVALID,
+    E_NOMEM,
+    E_NOT_CONVERGED,
+    E_NUMERIC,
+    E_STATE,
+    E_TOO_LARGE,
     INF_U32,
     OP_ADD,
     OP_DEL,
@@ -16,4 +22,5 @@ from ._lib import (  # noqa: F401
 )
 from .context import Context  # noqa: F401
 
-__all__ = ["Context", "SumincError", "UPDATE_DTYPE", "INF_U32", "OP_ADD", "OP_DEL", "PR_FP64", "PR_FP32_KAHAN"]
+__all__ = ["Context", "SumincError", "UPDATE_DTYPE", "INF_U32", "OP_ADD", "OP_DEL", "PR_FP64", "PR_FP32_KAHAN",
+           "E_INVALID", "E_NOMEM", "E_NOT_CONVERGED", "E_NUMERIC", "E_STATE", "E_TOO_LARGE"]

@@ -86,13 +86,22 @@ class Stats(C.Structure):
         ("kernel_ms", C.c_double),
         ("push_rounds", C.c_double),
         ("pull_rounds", C.c_double),
-        ("reserved", C.c_double * 10),
+        ("mass_defect", C.c_double),
+        ("mass_total", C.c_double),
+        ("reset_vertices", C.c_double),
+        ("batch_groups", C.c_double),
+        ("touched_rows", C.c_double),
+        ("reseed_edges", C.c_double),
+        ("shortcut_edges", C.c_double),
+        ("sketch_rounds", C.c_double),
+        ("sketch_ms", C.c_double),
+        ("residual_rounds", C.c_double),
+        ("precompute_edges", C.c_double),
+        ("reserved", C.c_double * 7),
     ]
 
     def as_dict(self) -> dict:
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
-        d["reserved"] = list(self.reserved)
-        return d
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class KernelProfile(C.Structure):

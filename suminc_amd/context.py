@@ -167,6 +167,10 @@ class Context:
         self._check(self._lib.suminc_copy_to_device(self._h, p, L.ptr(ops), ops.nbytes))
         return DeviceBatch(self, p, len(ops))
 
+    def device_array(self, arr) -> "DeviceArray":
+        """Copies a numpy array into device memory of this context's GPU."""
+        return DeviceArray(self, arr)
+
     def apply_batch_device(self, batch: "DeviceBatch"):
         self._check(self._lib.suminc_apply_batch_device(self._h, batch.ptr, batch.n_ops))
         return self.stats()
@@ -259,13 +263,20 @@ class Context:
 
     # ---- weighted personalized PageRank + adjoint ------------------------------
     # Arrays may be numpy arrays (host) or torch tensors on this context's GPU.
+    # The backward reads the rank and weights of the last forward on the context;
+    # wpr_token counts forwards and weight changes, so a caller holding an older
+    # forward (the autograd wrapper) can tell that it must run its forward again.
+    wpr_token = 0
+
     def wpr_set_weights(self, w):
+        self.wpr_token += 1
         self._check(self._lib.suminc_wpr_set_weights(self._h, _buf(w, np.float32, self.graph_info()[1])))
 
     def wpr_forward(self, p, out=None) -> tuple:
         n = self.graph_info()[0]
         r = np.zeros(n, np.float32) if out is None else out
         s = L.Stats()
+        self.wpr_token += 1
         self._check(self._lib.suminc_wpr_forward(self._h, _buf(p, np.float32, n), _buf(r, np.float32, n),
                                                  C.byref(s)))
         return r, s.as_dict()
@@ -302,6 +313,44 @@ class Context:
         parent = np.zeros(n, np.uint32)
         self._check(self._lib.suminc_sssp_download(self._h, L.ptr(dist), L.ptr(parent)))
         return dist, parent
+
+
+class DeviceArray:
+    """A numpy array copied into device memory of a context's GPU (a caller-owned
+    suminc_device_alloc buffer).  It exposes the tensor protocol `_buf` reads, so it
+    can be passed wherever the ABI accepts device memory; `set` rewrites single
+    elements from the host."""
+
+    def __init__(self, ctx: Context, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        self._ctx = ctx
+        self.dtype = arr.dtype
+        self.size = arr.size
+        self.ptr = C.c_void_p()
+        ctx._check(ctx._lib.suminc_device_alloc(ctx._h, max(arr.nbytes, 1), C.byref(self.ptr)))
+        ctx._check(ctx._lib.suminc_copy_to_device(ctx._h, self.ptr, L.ptr(arr), arr.nbytes))
+
+    def data_ptr(self) -> int:
+        return self.ptr.value
+
+    def numel(self) -> int:
+        return self.size
+
+    def element_size(self) -> int:
+        return self.dtype.itemsize
+
+    def is_contiguous(self) -> bool:
+        return True
+
+    def set(self, index: int, value):
+        v = np.array([value], dtype=self.dtype)
+        dst = C.c_void_p(self.ptr.value + int(index) * self.dtype.itemsize)
+        self._ctx._check(self._ctx._lib.suminc_copy_to_device(self._ctx._h, dst, L.ptr(v), v.nbytes))
+
+    def free(self):
+        if self.ptr is not None and self._ctx._h:
+            self._ctx._check(self._ctx._lib.suminc_device_free(self._ctx._h, self.ptr))
+        self.ptr = None
 
 
 class DeviceBatch:

@@ -1148,7 +1148,7 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
       more = std::min(more * 2, 16);
     }
     c->hint_reset = std::max(2, (int)c->host_counters->aux[9] + 1);
-    c->stats.reserved[0] = (double)c->host_counters->aux[4];  // reset vertex count
+    c->stats.reset_vertices = (double)c->host_counters->aux[4];  // reset vertex count
   }
 
   // 5b. topology: the CSR, then the CSC from the transposed groups
@@ -1218,9 +1218,9 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   if (pr_pull) c->pull_heavy_version = c->graph_version;  // built in step 6
   // per-round scratch sized by m may need to grow
   if (c->pr_ready || c->sssp_ready) c->heavy.ensure_slack((c->m / kChunk + c->m / kHeavyDeg + 2) * sizeof(uint4));
-  c->stats.reserved[1] = (double)hbc[0];
-  c->stats.reserved[2] = (double)hbc[1];
-  c->stats.reserved[3] = (double)c->host_counters->aux[5];  // reseed edges
+  c->stats.batch_groups = (double)hbc[0];
+  c->stats.touched_rows = (double)hbc[1];
+  c->stats.reseed_edges = (double)c->host_counters->aux[5];  // reseed edges
 }
 
 }  // namespace suminc

@@ -185,6 +185,7 @@ struct suminc_ctx {
   bool pr_ready = false;
   int pr_prec = SUMINC_PR_FP32_KAHAN;
   suminc::DevBuf pr_delta, pr_rank, pr_comp, pr_contrib;
+  suminc::DevBuf mass_part;  // block sums of the FP64 mass balance
   // pull mode: static chunk list of high in-degree vertices, rebuilt per graph version
   suminc::DevBuf pull_heavy, pull_short, pull_wave;
   suminc::DevBuf pull_cnt;    // u64 x3: lengths of the short, wave and chunk lists

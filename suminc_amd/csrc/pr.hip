@@ -499,6 +499,62 @@ __global__ void k_pr_from_f64(uint32_t n, T* delta, T* rank, float* comp, const 
   }
 }
 
+// FP64 mass balance of the state: per block of a fixed grid, the sums of
+// r + delta and of d * r over the vertices with out-edges (r = rank - comp in
+// FP32/Kahan mode); k_pr_mass_end adds the block sums in a fixed order.
+constexpr unsigned kMassGrid = 1024;
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pr_mass(uint32_t n, const uint32_t* __restrict__ off,
+                                                    const T* __restrict__ delta, const T* __restrict__ rank,
+                                                    const float* __restrict__ comp, double d,
+                                                    double* __restrict__ partial) {
+  __shared__ double s_tot[kBlock / 64], s_out[kBlock / 64];
+  double tot = 0.0, out = 0.0;
+  for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += kMassGrid * kBlock) {
+    double r = (double)rank[v];
+    if (comp) r -= (double)comp[v];
+    tot += r + (double)delta[v];
+    if (off[v + 1] > off[v]) out += d * r;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    tot += __shfl_xor(tot, o);
+    out += __shfl_xor(out, o);
+  }
+  if (lane_id() == 0) {
+    s_tot[threadIdx.x >> 6] = tot;
+    s_out[threadIdx.x >> 6] = out;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      a += s_tot[w];
+      b += s_out[w];
+    }
+    partial[blockIdx.x] = a;
+    partial[kMassGrid + blockIdx.x] = b;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_pr_mass_end(const double* __restrict__ partial, double* __restrict__ out) {
+  __shared__ double s[1024], so[1024];
+  s[threadIdx.x] = partial[threadIdx.x];
+  so[threadIdx.x] = partial[kMassGrid + threadIdx.x];
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s[threadIdx.x] += s[threadIdx.x + o];
+      so[threadIdx.x] += so[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s[0];
+    out[1] = so[0];
+  }
+}
+
 __global__ void k_zero_counters(Counters* cnt) {
   if (threadIdx.x == 0) {
     Counters z{};
@@ -1007,6 +1063,26 @@ static void pr_profile_rounds(suminc_ctx* c, size_t rounds) {
                  (double)n_pull * c->n * es + (double)h->push_entries * (4.0 + es);
 }
 
+// FP64 mass balance of the current state into st->mass_total / mass_defect.
+void pr_mass(suminc_ctx* c, suminc_stats* st) {
+  if (!c->pr_ready || !st) return;
+  c->mass_part.ensure((2 * kMassGrid + 2) * sizeof(double));
+  double* part = c->mass_part.as<double>();
+  const double d = c->prm.damping;
+  if (c->pr_prec == SUMINC_PR_FP64)
+    k_pr_mass<double><<<kMassGrid, kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), c->pr_delta.as<double>(),
+                                                          c->pr_rank.as<double>(), nullptr, d, part);
+  else
+    k_pr_mass<float><<<kMassGrid, kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), c->pr_delta.as<float>(),
+                                                         c->pr_rank.as<float>(), c->pr_comp.as<float>(), d, part);
+  k_pr_mass_end<<<1, 1024, 0, c->stream>>>(part, part + 2 * kMassGrid);
+  SUMINC_HIP(hipGetLastError());
+  check_begin(c, part + 2 * kMassGrid, 2 * sizeof(double));
+  const double* h = reinterpret_cast<const double*>(check_wait(c));
+  st->mass_total = h[0];
+  st->mass_defect = h[0] - h[1] - (1.0 - d);
+}
+
 // Runs rounds until the device sets the stop flag; host polls every `group` rounds.
 void pr_run(suminc_ctx* c, suminc_stats* st) {
   SUMINC_REQUIRE(c->pr_ready, SUMINC_E_STATE, "PageRank state not initialised");
@@ -1042,9 +1118,12 @@ void pr_run(suminc_ctx* c, suminc_stats* st) {
   c->stats.kernel_ms = ms;
   c->stats.pull_rounds = (double)h->pull_rounds;
   c->stats.push_rounds = (double)h->rounds - (double)h->pull_rounds;
+  const bool numeric_fail = h->aux[0] != 0;
+  const double residual = h->residual;
+  pr_mass(c, &c->stats);
   if (st) *st = c->stats;
-  SUMINC_REQUIRE(!h->aux[0], SUMINC_E_NUMERIC, "NaN/Inf in PageRank residual");
-  SUMINC_REQUIRE(!(h->residual >= T), SUMINC_E_NOT_CONVERGED, "round cap reached before sum|delta| < T");
+  SUMINC_REQUIRE(!numeric_fail, SUMINC_E_NUMERIC, "NaN/Inf in PageRank residual");
+  SUMINC_REQUIRE(!(residual >= T), SUMINC_E_NOT_CONVERGED, "round cap reached before sum|delta| < T");
 }
 
 void pr_step(suminc_ctx* c, double* residual) {

@@ -18,6 +18,8 @@ void pr_run(suminc_ctx* c, suminc_stats* st);
 void pr_step(suminc_ctx* c, double* residual);
 void pr_download(suminc_ctx* c, double* rank, double* delta);
 void pr_upload(suminc_ctx* c, const double* rank, const double* delta);
+// FP64 mass balance of the current PageRank state into st->mass_total / mass_defect
+void pr_mass(suminc_ctx* c, suminc_stats* st);
 unsigned push_grid();
 // positions of the vertices by decreasing degree (deg_off's row lengths; degree 0:
 // INF32) and cols (rows rows_off, up to m_cap entries) renamed to them; scratch

@@ -1259,7 +1259,7 @@ static void compute_pr_rows(suminc_ctx* c) {
   unsigned long long w = 0;
   SUMINC_HIP(hipMemcpyAsync(&w, work.p, 8, hipMemcpyDeviceToHost, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
-  c->stats.reserved[4] = (double)w;  // internal edges read by the shortcut build
+  c->stats.shortcut_edges = (double)w;  // internal edges read by the shortcut build
   s.pr_rows_for = c->pr_prec;
   s.pr_need = false;
 }
@@ -1731,7 +1731,7 @@ static void sum_pr_run_t(suminc_ctx* c, suminc_stats* st) {
     if (group < 32) group *= 2;
   }
   const Counters* h = c->host_counters;
-  out.reserved[5] = (double)h->rounds;  // sketch rounds
+  out.sketch_rounds = (double)h->rounds;  // sketch rounds
   SUMINC_REQUIRE(!h->aux[0], SUMINC_E_NUMERIC, "NaN/Inf in the sketch residual");
   // 3. flush the exits' pending inflow, 4. correction
   k_lists_clear<<<1, 64, 0, c->stream>>>(cnt);
@@ -1757,15 +1757,17 @@ static void sum_pr_run_t(suminc_ctx* c, suminc_stats* st) {
   // 5. residual phase: plain Ingress rounds on the full graph
   suminc_stats rs{};
   pr_run(c, &rs);
-  out.rounds = rs.rounds + out.reserved[5];
+  out.rounds = rs.rounds + out.sketch_rounds;
   out.edges_traversed = rs.edges_traversed + sketch_edges + (double)wk;
   out.residual = rs.residual;
   out.kernel_ms = rs.kernel_ms + ms_sketch;
   out.push_rounds = rs.push_rounds;
   out.pull_rounds = rs.pull_rounds;
-  out.reserved[6] = ms_sketch;
-  out.reserved[7] = rs.rounds;  // residual-phase rounds
-  out.reserved[8] = (double)wk;  // internal edges read by precompute
+  out.mass_defect = rs.mass_defect;
+  out.mass_total = rs.mass_total;
+  out.sketch_ms = ms_sketch;
+  out.residual_rounds = rs.rounds;  // residual-phase rounds
+  out.precompute_edges = (double)wk;  // internal edges read by precompute
   c->stats = out;
   if (st) *st = out;
 }

@@ -815,7 +815,7 @@ void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
   out.edges_traversed = edges + (double)wk;
   out.kernel_ms = ms;
   out.push_rounds = rounds;
-  out.reserved[8] = (double)wk;  // internal edges read by the precompute
+  out.precompute_edges = (double)wk;  // internal edges read by the precompute
   c->stats = out;
   if (st) *st = out;
 }

@@ -34,15 +34,25 @@ class WeightedPageRank(torch.autograd.Function):
         sctx.wpr_forward(p, out=r)  # synchronous
         fctx.sctx = sctx
         fctx.m = w.numel()
+        # the library's backward uses the rank of the context's last forward: keep
+        # this forward's inputs and its token to run it again if another forward
+        # (or a weight change) came in between
+        fctx.save_for_backward(w, p)
+        fctx.token = sctx.wpr_token
         return r
 
     @staticmethod
     def backward(fctx, grad_r: torch.Tensor):
         g = grad_r.detach().to(torch.float32).contiguous()
         torch.cuda.current_stream().synchronize()
+        sctx = fctx.sctx
+        if sctx.wpr_token != fctx.token:
+            w, p = fctx.saved_tensors
+            sctx.wpr_set_weights(w)
+            sctx.wpr_forward(p, out=torch.empty_like(p))
         gp = torch.empty_like(g)
         gw = torch.empty(fctx.m, dtype=torch.float32, device=g.device)
-        fctx.sctx.wpr_backward(g, grad_p=gp, grad_w=gw)  # synchronous
+        sctx.wpr_backward(g, grad_p=gp, grad_w=gw)  # synchronous
         return gw, gp, None
 
 

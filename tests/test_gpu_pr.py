@@ -134,3 +134,70 @@ def test_incremental_reseed_matches_scratch(gpu, prec, band_kib, compact_kib):
             rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
             err = np.abs(r - rs).sum()
             assert err <= st["residual"] / (1 - DAMPING) + (1e-11 if prec == "fp64" else 2e-6), (b, err, st)
+
+
+def signed_mass_defect(off, r, delta, d):
+    nd = np.diff(off) > 0
+    return r.sum() + delta.sum() - d * r[nd].sum() - (1 - d)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_device_mass_balance_matches_host(gpu, prec):
+    """suminc_stats.mass_defect (FP64 on the device) equals the same sum over the
+    downloaded state, after a run and after an incremental batch."""
+    g = oracle.Graph.rmat(14, 16, seed=5)
+    off, col, _ = g.csr()
+    p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
+    with gpu.Context(pr_precision=p, termcheck_threshold=1e-9 * g.n) as ctx:
+        ctx.upload_csr(off, col)
+        ctx.pr_init()
+        for b in range(2):
+            st = ctx.pr_run()
+            r, d = ctx.pr_download()
+            host = signed_mass_defect(off, r, d, DAMPING)
+            assert abs(st["mass_defect"] - host) < 1e-13, (st["mass_defect"], host)
+            assert abs(st["mass_total"] - (r.sum() + d.sum())) < 1e-13
+            assert abs(st["mass_defect"]) < (1e-13 if prec == "fp64" else 1e-6)
+            ops = g.gen_batch(seed=5, batch_id=b, n_del=200, n_ins=200)
+            ctx.apply_batch(ops)
+            g, _ = g.apply_batch(ops)
+            off, col, _ = g.csr()
+
+
+def test_nan_in_state_reports_numeric(gpu):
+    """A NaN planted in the delta vector comes back as SUMINC_E_NUMERIC, not a hang
+    or a silent result."""
+    g = oracle.Graph.rmat(10, 16, seed=1)
+    off, col, _ = g.csr()
+    r0 = np.zeros(g.n)
+    d0 = np.full(g.n, (1 - DAMPING) / g.n)
+    d0[17] = np.nan
+    for prec in (gpu.PR_FP64, gpu.PR_FP32_KAHAN):
+        with gpu.Context(pr_precision=prec, termcheck_threshold=1e-9) as ctx:
+            ctx.upload_csr(off, col)
+            ctx.pr_upload(r0, d0)
+            with pytest.raises(gpu.SumincError) as e:
+                ctx.pr_run()
+            assert e.value.code == gpu.E_NUMERIC
+            d0[17] = np.inf
+            ctx.pr_upload(r0, d0)
+            with pytest.raises(gpu.SumincError) as e:
+                ctx.pr_run()
+            assert e.value.code == gpu.E_NUMERIC
+            d0[17] = np.nan
+
+
+def test_oversized_allocation_reports_nomem(gpu):
+    """A device allocation beyond the GPU's memory comes back as SUMINC_E_NOMEM and
+    leaves the context usable."""
+    import ctypes as C
+
+    g = oracle.Graph.rmat(10, 16, seed=1)
+    off, col, _ = g.csr()
+    with gpu.Context(pr_precision=gpu.PR_FP64, termcheck_threshold=1e-9) as ctx:
+        ptr = C.c_void_p()
+        rc = ctx._lib.suminc_device_alloc(ctx._h, 1 << 50, C.byref(ptr))
+        assert rc == gpu.E_NOMEM and not ptr.value
+        ctx.upload_csr(off, col)
+        ctx.pr_init()
+        assert ctx.pr_run()["residual"] < 1e-9
