@@ -224,6 +224,10 @@ int suminc_device_alloc(suminc_ctx* c, uint64_t bytes, void** dev_ptr) {
     *dev_ptr = nullptr;
     if (bytes) {
       hipError_t e = hipMalloc(dev_ptr, bytes);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();  // not left as the thread's last error for later calls
+        *dev_ptr = nullptr;
+      }
       SUMINC_REQUIRE(e == hipSuccess, SUMINC_E_NOMEM, "device allocation failed");
     }
   });

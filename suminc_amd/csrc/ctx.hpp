@@ -20,12 +20,17 @@ struct Error : std::runtime_error {
   Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
 
+// A failed runtime call also leaves its error as the thread's last error, which a
+// later hipGetLastError (ours or a library's) would report again: it is cleared
+// before the failure is turned into a code.
 #define SUMINC_HIP(expr)                                                              \
   do {                                                                                \
     hipError_t e_ = (expr);                                                           \
-    if (e_ != hipSuccess)                                                             \
+    if (e_ != hipSuccess) {                                                           \
+      (void)hipGetLastError();                                                        \
       throw ::suminc::Error(e_ == hipErrorOutOfMemory ? SUMINC_E_NOMEM : SUMINC_E_HIP, \
                             std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    }                                                                                 \
   } while (0)
 
 // Message of the last failed call without a context on this thread (io.hip).
