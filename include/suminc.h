@@ -181,6 +181,8 @@ int suminc_device_alloc(suminc_ctx* ctx, uint64_t bytes, void** dev_ptr);
 int suminc_device_free(suminc_ctx* ctx, void* dev_ptr);
 /* src may be host memory or device memory of the context's GPU */
 int suminc_copy_to_device(suminc_ctx* ctx, void* dev_dst, const void* host_src, uint64_t bytes);
+/* device memory of the context's GPU to host memory (synchronises) */
+int suminc_copy_from_device(suminc_ctx* ctx, void* host_dst, const void* dev_src, uint64_t bytes);
 
 /* ---- PageRank (plain Ingress path) --------------------------------------------
  * init: delta = (1-d)/N, rank = 0 (PageRankIngress::init_c/init_v,
@@ -288,6 +290,22 @@ int suminc_wpr_set_weights(suminc_ctx* ctx, const float* weights);
 int suminc_wpr_forward(suminc_ctx* ctx, const float* personalization, float* rank, suminc_stats* stats);
 int suminc_wpr_backward(suminc_ctx* ctx, const float* grad_rank, float* grad_personalization, float* grad_weights,
                         suminc_stats* stats);
+/* Allocation-free variant of suminc_wpr_forward.  No counterpart in the reference;
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that answer one personalization after another,
+ * harnesses that forbid hipMalloc).  personalization and rank must be device
+ * memory of the context's GPU; the call allocates nothing.  The round state lives
+ * in one fixed 64 MiB device array that the library holds on every GPU it runs on
+ * until the process ends: about 16 B per vertex in FP32 and 28 B per vertex in
+ * FP64, so graphs up to about 4.1M (FP32) and 2.3M (FP64) vertices; larger ones
+ * get SUMINC_E_TOO_LARGE.  Needs suminc_wpr_set_weights and one
+ * suminc_wpr_forward in the same precision on the current weights first (they
+ * prepare the row lists and per-edge coefficients), else SUMINC_E_STATE.  Every
+ * call on one GPU uses that GPU's one array, which keeps the last call's contents:
+ * calls that overlap on one GPU, from any context or thread, race.  The rank it
+ * computes is not kept for suminc_wpr_backward (run suminc_wpr_forward before a
+ * backward). */
+int suminc_wpr_forward_noalloc(suminc_ctx* ctx, const float* personalization, float* rank, suminc_stats* stats);
 /* FP64 copies (host memory) of the last forward's r and the last backward's lambda. */
 int suminc_wpr_download(suminc_ctx* ctx, double* rank, double* adjoint);
 

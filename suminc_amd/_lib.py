@@ -148,6 +148,7 @@ SIGNATURES = {
     "suminc_device_alloc": (i32, [vp, u64, C.POINTER(vp)]),
     "suminc_device_free": (i32, [vp, vp]),
     "suminc_copy_to_device": (i32, [vp, vp, vp, u64]),
+    "suminc_copy_from_device": (i32, [vp, vp, vp, u64]),
     "suminc_pr_init": (i32, [vp]),
     "suminc_pr_run": (i32, [vp, C.POINTER(Stats)]),
     "suminc_pr_step": (i32, [vp, C.POINTER(f64)]),
@@ -168,6 +169,7 @@ SIGNATURES = {
     "suminc_sum_sssp_run": (i32, [vp, C.POINTER(Stats)]),
     "suminc_wpr_set_weights": (i32, [vp, vp]),
     "suminc_wpr_forward": (i32, [vp, vp, vp, C.POINTER(Stats)]),
+    "suminc_wpr_forward_noalloc": (i32, [vp, vp, vp, C.POINTER(Stats)]),
     "suminc_wpr_backward": (i32, [vp, vp, vp, vp, C.POINTER(Stats)]),
     "suminc_wpr_download": (i32, [vp, vp, vp]),
     "suminc_profile_enable": (i32, [vp, i32]),

@@ -342,6 +342,11 @@ class DeviceArray:
     def is_contiguous(self) -> bool:
         return True
 
+    def download(self) -> np.ndarray:
+        out = np.zeros(self.size, self.dtype)
+        self._ctx._check(self._ctx._lib.suminc_copy_from_device(self._ctx._h, L.ptr(out), self.ptr, out.nbytes))
+        return out
+
     def set(self, index: int, value):
         v = np.array([value], dtype=self.dtype)
         dst = C.c_void_p(self.ptr.value + int(index) * self.dtype.itemsize)

@@ -127,6 +127,7 @@ int suminc_ctx_create(int device, const suminc_params* params, suminc_ctx** out)
     c->host_scratch = reinterpret_cast<unsigned long long*>(hs);
     std::memset(c->host_scratch, 0, 256);
     SUMINC_HIP(hipEventCreateWithFlags(&c->ev_check, hipEventDisableTiming));
+    wpr_scratch_setup(device);  // device address of the allocation-free forward's scratch array
     SUMINC_HIP(hipStreamSynchronize(c->stream));
   });
   if (rc != SUMINC_OK) {
@@ -250,6 +251,16 @@ int suminc_copy_to_device(suminc_ctx* c, void* dev_dst, const void* host_src, ui
   });
 }
 
+int suminc_copy_from_device(suminc_ctx* c, void* host_dst, const void* dev_src, uint64_t bytes) {
+  return guarded(c, [&] {
+    SUMINC_REQUIRE((host_dst && dev_src) || bytes == 0, SUMINC_E_INVALID, "null buffer");
+    if (bytes) {
+      SUMINC_HIP(hipMemcpyAsync(host_dst, dev_src, bytes, hipMemcpyDeviceToHost, c->stream));
+      SUMINC_HIP(hipStreamSynchronize(c->stream));
+    }
+  });
+}
+
 int suminc_pr_init(suminc_ctx* c) {
   return guarded(c, [&] { pr_init(c); });
 }
@@ -336,6 +347,10 @@ int suminc_wpr_forward(suminc_ctx* c, const float* personalization, float* rank,
 int suminc_wpr_backward(suminc_ctx* c, const float* grad_rank, float* grad_personalization, float* grad_weights,
                         suminc_stats* stats) {
   return guarded(c, [&] { wpr_backward(c, grad_rank, grad_personalization, grad_weights, stats); });
+}
+
+int suminc_wpr_forward_noalloc(suminc_ctx* c, const float* personalization, float* rank, suminc_stats* stats) {
+  return guarded(c, [&] { wpr_forward_noalloc(c, personalization, rank, stats); });
 }
 
 int suminc_wpr_download(suminc_ctx* c, double* rank, double* adjoint) {

@@ -57,6 +57,10 @@ void wpr_set_weights(suminc_ctx* c, const float* w);
 void wpr_forward(suminc_ctx* c, const float* p, float* r, suminc_stats* st);
 void wpr_backward(suminc_ctx* c, const float* cvec, float* grad_p, float* grad_w, suminc_stats* st);
 void wpr_download(suminc_ctx* c, double* r, double* lam);
+// the forward over the library's fixed per-device scratch array (no allocation);
+// wpr_scratch_setup takes that array's device address, once per device
+void wpr_forward_noalloc(suminc_ctx* c, const float* p, float* r, suminc_stats* st);
+void wpr_scratch_setup(int device);
 void wpr_release(suminc_ctx* c);
 
 }  // namespace suminc

@@ -17,6 +17,7 @@
 // round has no per-edge atomics.  State: FP32 deltas with Kahan-compensated FP32
 // sums, or FP64; the stop rule and every residual are FP64 block partials.
 #include <cstring>
+#include <mutex>
 #include <rocprim/rocprim.hpp>
 
 #include "ctx.hpp"
@@ -232,6 +233,14 @@ __global__ void __launch_bounds__(kBlock) k_wpr_seed(uint32_t n, const float* __
   }
 }
 
+// zeroes the counters block (a kernel, not a memset: callers may capture the series)
+__global__ void k_wpr_clear(Counters* cnt) {
+  if (threadIdx.x == 0) {
+    Counters z{};
+    *cnt = z;
+  }
+}
+
 __global__ void __launch_bounds__(1024) k_sum_partials(const double* __restrict__ partial, uint32_t nparts,
                                                        double* out) {
   __shared__ double s[1024];
@@ -257,7 +266,7 @@ __global__ void k_wpr_finish(uint32_t n, const T* __restrict__ x, T* __restrict_
   double val = (double)sum[v];
   if constexpr (sizeof(T) == 4) val -= (double)comp[v];
   if (out32) out32[v] = (float)val;
-  out64[v] = val;
+  if (out64) out64[v] = val;
 }
 
 // Wave-per-row kernels stride over the rows with at most this many workgroups (a
@@ -497,53 +506,101 @@ static void spmv(suminc_ctx* c, const RowLists& R, const T* a, const T* x, T* y,
   if (gh + gw + gs) k_spmv<T><<<gh + gw + gs, kBlock, 0, c->stream>>>(R, a, x, y, cnt, gh, gw);
 }
 
-// Runs sum = sum_t x_t with x_{t+1} = A x_t from x_0 = scale * seed until
-// sum|x| < tol.  Result (FP64) in out64, optionally FP32 in out32.
-// pos != nullptr: R's columns are compact positions, and the rounds gather the
-// compact copy of x.
+// Device buffers of one series run: the round state (x, y, the compensated sum),
+// the compact copy of x (large graphs), the FP64 block partials and sum |seed|.
 template <class T>
-static void wpr_series(suminc_ctx* c, WprState& S, const RowLists& R, const uint32_t* pos, const T* a,
-                       const float* seed_dev, T scale, double tol_abs, bool relative, float* out32, double* out64,
-                       suminc_stats* st) {
+struct SeriesBufs {
+  T *x, *y, *sum, *xc;
+  float* comp;
+  double *partial, *l1;
+};
+
+// Bytes of the series buffers for n vertices (partials: one per vertex block, three
+// words each, as the PageRank rounds size them).
+template <class T>
+static size_t series_bytes(uint32_t n, bool compact) {
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t g = grid_for(n);
+  return 3 * al((size_t)n * sizeof(T) + 8) + (compact ? al((size_t)n * sizeof(T) + 8) : 0) + al((size_t)n * 4 + 4) +
+         al((g + 1) * 8 * 3) + al(8);
+}
+
+// Carves the series buffers out of `base` (series_bytes<T>(n, compact) bytes).
+template <class T>
+static SeriesBufs<T> series_carve(unsigned char* base, uint32_t n, bool compact) {
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t g = grid_for(n);
+  SeriesBufs<T> B;
+  size_t o = 0;
+  auto take = [&](size_t bytes) { unsigned char* p = base + o; o += al(bytes); return p; };
+  B.x = reinterpret_cast<T*>(take((size_t)n * sizeof(T) + 8));
+  B.y = reinterpret_cast<T*>(take((size_t)n * sizeof(T) + 8));
+  B.sum = reinterpret_cast<T*>(take((size_t)n * sizeof(T) + 8));
+  B.xc = compact ? reinterpret_cast<T*>(take((size_t)n * sizeof(T) + 8)) : nullptr;
+  B.comp = reinterpret_cast<float*>(take((size_t)n * 4 + 4));
+  B.partial = reinterpret_cast<double*>(take((g + 1) * 8 * 3));
+  B.l1 = reinterpret_cast<double*>(take(8));
+  return B;
+}
+
+// The series buffers in the context's weighted-PageRank state (grown on demand).
+template <class T>
+static SeriesBufs<T> series_state(suminc_ctx* c, WprState& S, bool compact) {
   const uint32_t n = c->n;
   S.x.ensure((size_t)n * sizeof(T) + 8);
-  if (pos) S.xc.ensure((size_t)n * sizeof(T) + 8);
-  T* xc = pos ? S.xc.as<T>() : nullptr;
+  if (compact) S.xc.ensure((size_t)n * sizeof(T) + 8);
   S.y.ensure((size_t)n * sizeof(T) + 8);
   S.sum.ensure((size_t)n * sizeof(T) + 8);
   S.comp.ensure((size_t)n * 4 + 4);
   S.l1.ensure(8);
+  c->partial.ensure((size_t)(grid_for(n) + 1) * 8 * 3);
+  SeriesBufs<T> B;
+  B.x = S.x.as<T>();
+  B.y = S.y.as<T>();
+  B.sum = S.sum.as<T>();
+  B.xc = compact ? S.xc.as<T>() : nullptr;
+  B.comp = S.comp.as<float>();
+  B.partial = c->partial.as<double>();
+  B.l1 = S.l1.as<double>();
+  return B;
+}
+
+// Runs sum = sum_t x_t with x_{t+1} = A x_t from x_0 = scale * seed until
+// sum|x| < tol.  Result (FP64) in out64, optionally FP32 in out32 (either may be
+// null).  pos != nullptr: R's columns are compact positions, and the rounds
+// gather the compact copy of x (B.xc).  Allocates nothing itself.
+template <class T>
+static void wpr_series(suminc_ctx* c, const SeriesBufs<T>& B, const RowLists& R, const uint32_t* pos, const T* a,
+                       const float* seed_dev, T scale, double tol_abs, bool relative, float* out32, double* out64,
+                       suminc_stats* st) {
+  const uint32_t n = c->n;
+  T* xc = pos ? B.xc : nullptr;
   const unsigned g = grid_for(n);
   const unsigned gs = grid_for(n, kBlock * kStepPer);  // the round's vertex pass and its partials
-  c->partial.ensure((size_t)(g + 1) * 8 * 3);
   Counters* cnt = dev_counters(c);
-  SUMINC_HIP(hipMemsetAsync(cnt, 0, sizeof(Counters), c->stream));
-  k_wpr_seed<T><<<g, kBlock, 0, c->stream>>>(n, seed_dev, scale, S.x.as<T>(), S.y.as<T>(), S.sum.as<T>(),
-                                             S.comp.as<float>(), c->partial.as<double>(), pos, xc);
-  k_sum_partials<<<1, 1024, 0, c->stream>>>(c->partial.as<double>(), g, S.l1.as<double>());
+  k_wpr_clear<<<1, 64, 0, c->stream>>>(cnt);
+  k_wpr_seed<T><<<g, kBlock, 0, c->stream>>>(n, seed_dev, scale, B.x, B.y, B.sum, B.comp, B.partial, pos, xc);
+  k_sum_partials<<<1, 1024, 0, c->stream>>>(B.partial, g, B.l1);
   double tol = tol_abs;
   if (relative) {
-    double l1 = 0;
-    SUMINC_HIP(hipMemcpyAsync(&l1, S.l1.p, 8, hipMemcpyDeviceToHost, c->stream));
-    SUMINC_HIP(hipStreamSynchronize(c->stream));
-    tol = tol_abs * l1;
+    check_begin(c, B.l1, 8);
+    tol = tol_abs * *reinterpret_cast<const double*>(check_wait(c));
   }
   const int max_rounds = c->prm.max_rounds > 0 ? c->prm.max_rounds : 100000;
   SUMINC_HIP(hipEventRecord(c->ev_start, c->stream));
   int group = 8;
   while (true) {
     for (int i = 0; i < group; ++i) {
-      spmv<T>(c, R, a, pos ? xc : S.x.as<T>(), S.y.as<T>(), cnt);
-      k_wpr_step<T><<<gs, kBlock, 0, c->stream>>>(n, S.x.as<T>(), S.y.as<T>(), S.sum.as<T>(),
-                                                  S.comp.as<float>(), c->partial.as<double>(), cnt, pos, xc);
-      k_wpr_decide<<<1, 1024, 0, c->stream>>>(c->partial.as<double>(), gs, cnt, tol, max_rounds, c->m);
+      spmv<T>(c, R, a, pos ? xc : B.x, B.y, cnt);
+      k_wpr_step<T><<<gs, kBlock, 0, c->stream>>>(n, B.x, B.y, B.sum, B.comp, B.partial, cnt, pos, xc);
+      k_wpr_decide<<<1, 1024, 0, c->stream>>>(B.partial, gs, cnt, tol, max_rounds, c->m);
     }
     SUMINC_HIP(hipGetLastError());
     read_counters(c);
     if (c->host_counters->done) break;
     if (group < 64) group *= 2;
   }
-  k_wpr_finish<T><<<g, kBlock, 0, c->stream>>>(n, S.x.as<T>(), S.sum.as<T>(), S.comp.as<float>(), out32, out64);
+  k_wpr_finish<T><<<g, kBlock, 0, c->stream>>>(n, B.x, B.sum, B.comp, out32, out64);
   SUMINC_HIP(hipGetLastError());
   SUMINC_HIP(hipEventRecord(c->ev_stop, c->stream));
   SUMINC_HIP(hipEventSynchronize(c->ev_stop));
@@ -596,7 +653,8 @@ static void wpr_forward_t(suminc_ctx* c, const float* p, float* r, suminc_stats*
   r32.ensure((size_t)n * 4 + 4);
   const RowLists R = lists_of(S.in, c->in_off, c->in_col, true);
   const uint32_t* pos = S.in.compact ? S.in.pos.as<uint32_t>() : nullptr;
-  wpr_series<T>(c, S, R, pos, S.a_in.as<T>(), pd.as<float>(), (T)(1.0 - c->prm.damping), c->prm.termcheck_threshold,
+  wpr_series<T>(c, series_state<T>(c, S, pos != nullptr), R, pos, S.a_in.as<T>(), pd.as<float>(),
+                (T)(1.0 - c->prm.damping), c->prm.termcheck_threshold,
                 false, r32.as<float>(), S.r64.as<double>(), st);
   if (r) SUMINC_HIP(hipMemcpyAsync(r, r32.p, (size_t)n * 4, hipMemcpyDefault, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
@@ -619,7 +677,8 @@ static void wpr_backward_t(suminc_ctx* c, const float* cvec, float* grad_p, floa
   S.yl64.ensure((size_t)n * 8 + 8);
   const RowLists R = lists_of(S.out, c->out_off, c->out_col, true);
   const uint32_t* pos = S.out.compact ? S.out.pos.as<uint32_t>() : nullptr;
-  wpr_series<T>(c, S, R, pos, S.a_out.as<T>(), cd.as<float>(), (T)1, c->prm.termcheck_threshold, true, nullptr,
+  wpr_series<T>(c, series_state<T>(c, S, pos != nullptr), R, pos, S.a_out.as<T>(), cd.as<float>(), (T)1,
+                c->prm.termcheck_threshold, true, nullptr,
                 S.lam64.as<double>(), st);
   // y_u = sum_k a_uk lambda_k (one more product, in FP64)
   DevBuf lam_t, y_t;
@@ -657,6 +716,77 @@ void wpr_backward(suminc_ctx* c, const float* cvec, float* grad_p, float* grad_w
   SUMINC_REQUIRE(cvec, SUMINC_E_INVALID, "null loss weights");
   if (c->prm.pr_precision == SUMINC_PR_FP64) wpr_backward_t<double>(c, cvec, grad_p, grad_w, st);
   else wpr_backward_t<float>(c, cvec, grad_p, grad_w, st);
+}
+
+// ---- the allocation-free forward -------------------------------------------------
+// Its round state lives in `scratch`, one fixed array per GPU that the library
+// holds until the process ends.  Host code never passes the array's name to a
+// kernel: wpr_scratch_setup() takes the array's device address once per device
+// (with that device current) and every launch uses that address.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_dev[kMaxDevices] = {};
+
+void wpr_scratch_setup(int device) {
+  SUMINC_REQUIRE(device >= 0 && device < kMaxDevices, SUMINC_E_INVALID, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device]) return;
+  void* p = nullptr;
+  SUMINC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  SUMINC_REQUIRE(p, SUMINC_E_HIP, "no device address for the scratch array");
+  g_scratch_dev[device] = reinterpret_cast<unsigned char*>(p);
+}
+
+static unsigned char* scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return device >= 0 && device < kMaxDevices ? g_scratch_dev[device] : nullptr;
+}
+
+static bool on_device(const void* p) {
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+template <class T>
+static void wpr_forward_noalloc_t(suminc_ctx* c, const float* p, float* r, suminc_stats* st) {
+  SUMINC_REQUIRE(c->wpr, SUMINC_E_STATE, "no weighted PageRank state: call suminc_wpr_set_weights and one "
+                                        "suminc_wpr_forward first");
+  WprState& S = *reinterpret_cast<WprState*>(c->wpr);
+  const int prec = sizeof(T) == 8 ? SUMINC_PR_FP64 : SUMINC_PR_FP32_KAHAN;
+  SUMINC_REQUIRE(S.graph_version == c->graph_version && S.coef_version == S.weights_version && S.coef_prec == prec,
+                 SUMINC_E_STATE, "the coefficients of the current graph, weights and precision are not prepared: "
+                                 "run one suminc_wpr_forward first");
+  const uint32_t n = c->n;
+  const bool compact = S.in.compact;
+  const size_t need = series_bytes<T>(n, compact);
+  if (need > kScratchBytes)
+    throw Error(SUMINC_E_TOO_LARGE, "suminc_wpr_forward_noalloc: " + std::to_string(n) + " vertices need " +
+                                        std::to_string(need) + " bytes of scratch, the fixed array holds " +
+                                        std::to_string(kScratchBytes));
+  unsigned char* base = scratch_of(c->device);
+  SUMINC_REQUIRE(base, SUMINC_E_STATE, "scratch array not set up on this device");
+  const SeriesBufs<T> B = series_carve<T>(base, n, compact);
+  const RowLists R = lists_of(S.in, c->in_off, c->in_col, true);
+  const uint32_t* pos = compact ? S.in.pos.as<uint32_t>() : nullptr;
+  S.fwd_ready = false;  // the rank of this forward is not kept for a backward
+  wpr_series<T>(c, B, R, pos, S.a_in.as<T>(), p, (T)(1.0 - c->prm.damping), c->prm.termcheck_threshold, false, r,
+                nullptr, st);
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+}
+
+void wpr_forward_noalloc(suminc_ctx* c, const float* p, float* r, suminc_stats* st) {
+  check_weighted(c);
+  SUMINC_REQUIRE(p && r, SUMINC_E_INVALID, "null personalization or rank");
+  SUMINC_REQUIRE(on_device(p) && on_device(r), SUMINC_E_INVALID,
+                 "suminc_wpr_forward_noalloc needs device memory for the personalization and the rank");
+  if (c->prm.pr_precision == SUMINC_PR_FP64) wpr_forward_noalloc_t<double>(c, p, r, st);
+  else wpr_forward_noalloc_t<float>(c, p, r, st);
 }
 
 void wpr_download(suminc_ctx* c, double* r, double* lam) {
