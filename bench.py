@@ -4,16 +4,19 @@ MI355X (GTEPS, fraction of the HBM roofline, speedup vs full recompute on the
 same GPU, and the CPU oracle timed on the host as a baseline).
 
 A step = apply one seeded update batch (device CSR/CSC delta-merge + reseed or
-SSSP repair) and run the incremental query to its fixed point.  The graph is
-generated on the device and the SSSP batches are uploaded to device memory before
-the timed region, so the timed steps read their inputs from HBM (the rate with the
-ops passed as host buffers is reported beside it as host_ops_ms_per_step).
+SSSP repair) and run the incremental queries to their fixed points.  The graph is
+generated on the device and the batches are uploaded to device memory before the
+timed region, so the timed steps read their inputs from HBM.
 
-Default workload (--workload sssp22): directed R-MAT scale 22 (a=.57 b=.19
-c=.19, edge factor 16, seed 2, self-loops/duplicates removed), u32 weights
-uniform in [1,64], source 0, supernodes from leaf clustering (at most 1024
-members, compress threshold 1), 0.1% batches (half deletes of existing edges,
-half inserts with uniform endpoints and weights in [1,64]).
+Default workload (--workload web26, the metric's large configuration): directed
+R-MAT scale 26 (67.1M vertices, a=.57 b=.19 c=.19, edge factor 16, seed 5,
+self-loops/duplicates removed), u32 weights uniform in [1,255], supernodes from
+leaf clustering (at most 1024 members, compress threshold 1), successive seeded 1%
+batches (half deletes of existing edges, half inserts with uniform endpoints); a
+step brings FP32/Kahan delta-PageRank (the SumInc query over the sketch, FP64
+residual, T = 1e-7) and SSSP over the sketch (source 0) back to their fixed points.
+The other workloads are the metric's smaller configurations (--workload pr16 |
+sssp22 | pr22 | pr24 | pr26 | wpr22).
 """
 from __future__ import annotations
 
@@ -64,9 +67,11 @@ WORKLOADS = {
                        "with c ~ N(0,1); a step = forward + adjoint (gradients w.r.t. weights and p)"),
     "web26": dict(kind="web", scale=26, seed=5, wmin=1, wmax=255, frac=0.01, T=1e-7, source=0, max_node=1024,
                   adjoint_seeds=1024,
-                  desc="web-like incremental run, R-MAT s26 ef16 seed5, successive 1% batches; per batch "
-                       "FP32/Kahan delta-PageRank (plain CSR) + SSSP over supernodes (u32 w~U[1,255], source 0, "
-                       "clusters <= 1024 members); weighted-PageRank forward + adjoint on the final graph"),
+                  desc="web-like incremental run, R-MAT s26 ef16 seed5 (a=.57 b=.19 c=.19), supernode compression "
+                       "(leaf clusters <= 1024 members, compress threshold 1), successive seeded 1% batches (half "
+                       "deletes of existing edges, half inserts); per batch FP32/Kahan delta-PageRank with an FP64 "
+                       "residual (the SumInc query over the sketch, T = 1e-7) + SSSP over the sketch (u32 "
+                       "w~U[1,255], source 0); weighted-PageRank forward + adjoint on the same graph"),
 }
 
 
@@ -95,8 +100,9 @@ def roofline(ctx, kernel_ids, workload):
     achieved = p["alg_bytes"] / (p["ms"] * 1e-3) / 1e9
     name = KERNEL_NAMES[k]
     alg = p["alg_bytes"] / p["launches"]
-    # the s26 PageRank pull was measured on the PageRank-only run of the same graph
-    traffic = measured_traffic(name, "pr26" if workload == "web26" else workload)
+    # web26 measures the traffic live (run_web); the other workloads read the
+    # committed counter summary of the same command
+    traffic = None if workload == "web26" else measured_traffic(name, workload)
     return name, {
         "bound": "hbm",
         "achieved": round(achieved, 2),
@@ -104,6 +110,8 @@ def roofline(ctx, kernel_ids, workload):
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None if traffic is None else round(traffic),
+        "traffic_note": None if traffic is None else "committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this "
+                                                     "workload (profiles/), not measured in this run",
         "kernel": name,
         "avg_launch_us": round(1e3 * p["ms"] / p["launches"], 2),
         "alg_bytes_per_launch": round(alg),
@@ -345,18 +353,95 @@ def run_pr(args, wl):
     return gteps, ms_per_step, "f32", roof, cpu, extra
 
 
+def _pmc_traffic(args, kernel_sub):
+    """HBM traffic per launch of the kernel whose rocprof name contains kernel_sub,
+    measured now: two rocprofv3 --pmc runs of a short child of this script
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
+    counter reads for a plain device-to-device copy of known size (FETCH x2, WRITE
+    x1 on MI355X, profiles/k_copy_old_sssp22_traffic.txt).  Returns (bytes, note)
+    or (None, reason)."""
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if os.path.exists("/opt/rocm/bin/rocprofv3")
+                                         else None)
+    if prof is None:
+        return None, "rocprofv3 not found"
+    scale = {"FETCH_SIZE": 2.0, "WRITE_SIZE": 1.0}
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix=f"pmc_{counter.lower()}_", dir=os.environ.get("TMPDIR", "/tmp"))
+        cmd = ["timeout", "-s", "KILL", str(args.traffic_timeout), prof, "--pmc", counter, "--output-format", "csv",
+               "-d", d, "-o", "run", "--", sys.executable, str(Path(__file__).resolve()), "--traffic-child",
+               "--workload", args.workload, "--pull-band-kib", str(args.pull_band_kib)]
+        log(f"traffic: {' '.join(cmd[4:])}")
+        r = subprocess.run(cmd, stdout=sys.stderr, stderr=sys.stderr, cwd=str(ROOT))  # progress passes through
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            shutil.rmtree(d, ignore_errors=True)
+            return None, f"rocprofv3 --pmc {counter} wrote no counter file (rc {r.returncode})"
+        vals = []
+        import csv
+
+        for f in files:
+            with open(f, newline="") as fh:
+                for row in csv.DictReader(fh):
+                    name = row.get("Kernel_Name") or row.get("Kernel-Name") or ""
+                    if kernel_sub in name and row.get("Counter_Name", counter) == counter:
+                        vals.append(float(row["Counter_Value"]))
+        shutil.rmtree(d, ignore_errors=True)
+        if not vals:
+            return None, f"no {kernel_sub} dispatch in the {counter} run"
+        top = max(vals)
+        work = [v for v in vals if v >= 0.1 * top]  # dispatches that did work (idle ones return at entry)
+        out[counter] = (float(np.mean(work)), len(work))
+    b = sum(out[k][0] * 1024.0 * scale[k] for k in out)
+    note = (f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over {out['FETCH_SIZE'][1]} dispatches of {kernel_sub} "
+            f"in a {args.workload} child run: FETCH {out['FETCH_SIZE'][0]:.0f} KiB x2, WRITE "
+            f"{out['WRITE_SIZE'][0]:.0f} KiB x1 per dispatch")
+    return b, note
+
+
+def traffic_child(args, wl):
+    """The short run the traffic measurement profiles: the same graph, sketch and
+    precision as the workload, one SumInc PageRank query from the initial state cut
+    at a few rounds per phase (every round from the initial state is a dense pull)."""
+    import suminc_amd
+
+    ctx = suminc_amd.Context(termcheck_threshold=wl["T"], pr_precision=suminc_amd.PR_FP32_KAHAN, max_rounds=4,
+                             pull_band_kib=args.pull_band_kib)
+    ctx.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"])
+    ctx.cluster_leaves(wl["max_node"], 2)
+    ctx.sum_build(2, wl["max_node"], 1.0)
+    ctx.pr_init()
+    try:
+        ctx.sum_pr_run()
+    except suminc_amd.SumincError as e:
+        if e.code != suminc_amd.E_NOT_CONVERGED:
+            raise
+    ctx.pr_init()
+    ctx.pr_run(allow_not_converged=True)
+    ctx.sync()
+    ctx.close()
+
+
 def run_web(args, wl):
-    """Successive batches on one large graph: each step applies one batch (CSR/CSC
-    merge, PageRank reseed, SSSP repair, sketch bookkeeping) and brings both the
-    PageRank and the SSSP state back to their fixed points.  Checks on the final
-    graph: SSSP equals a full plain recompute on the same GPU bit for bit, PageRank
-    is compared with an FP64 run to 1e-12 on the same GPU (L1, Linf), and the
-    weighted-PageRank forward + adjoint run once."""
+    """The web-like incremental run (s26, successive 1% batches).  A step applies one
+    batch from HBM (CSR/CSC merge, PageRank reseed, SSSP repair, sketch bookkeeping)
+    and brings both states back to their fixed points: the SumInc PageRank query
+    over the supernode sketch (precompute, sketch rounds, correction, residual
+    rounds) and the SumInc SSSP query.  Beside it: the same batches with PageRank on
+    the plain CSR, full recomputes on the same GPU, SSSP against a plain recompute
+    bit for bit, PageRank against an FP64 run (L1, Linf), the weighted-PageRank
+    forward + adjoint, and the reference's incremental step restated on the CPU."""
     import oracle
     import suminc_amd
     import suminc_amd._lib as L
 
-    ctx = suminc_amd.Context(termcheck_threshold=wl["T"], pr_precision=suminc_amd.PR_FP32_KAHAN,
+    T = wl["T"]
+    ctx = suminc_amd.Context(termcheck_threshold=T, pr_precision=suminc_amd.PR_FP32_KAHAN,
                              max_rounds=1_000_000, pull_band_kib=args.pull_band_kib)
     t0 = time.time()
     ctx.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"], weights="u32", wmin=wl["wmin"],
@@ -367,17 +452,23 @@ def run_web(args, wl):
     ctx.cluster_leaves(wl["max_node"], 2)
     info = ctx.sum_build(2, wl["max_node"], 1.0)
     log(f"sketch: {info} ({time.time() - t0:.1f}s)")
-    extra = {"sketch": {k: info[k] for k in ("clusters", "clustered_vertices", "entries", "sketch_edges",
-                                             "internal_edges", "shortcut_slots", "max_cluster")}}
+    sk = {k: info[k] for k in ("clusters", "clustered_vertices", "entries", "sketch_edges", "internal_edges",
+                               "shortcut_slots", "max_cluster")}
+    sk["node_type_count"] = info["node_type_count"]
+    # edges the sketch rounds carry: Singles' and exits' external out-edges plus one
+    # shortcut slot per (entry, exit) pair, against the plain graph's m
+    sk["compression_ratio"] = round((info["sketch_edges"] + info["shortcut_slots"]) / m, 4)
+    extra = {"sketch": sk}
     ctx.pr_init()
     t0 = time.time()
-    st = ctx.pr_run()
-    log(f"initial PR: rounds={st['rounds']:.0f} edges={st['edges_traversed']:.3e} ({time.time() - t0:.1f}s)")
+    st = ctx.sum_pr_run()
+    log(f"initial PR (SumInc query): rounds={st['rounds']:.0f} ({time.time() - t0:.1f}s)")
     ctx.sssp_init(wl["source"])
     st = ctx.sum_sssp_run()
     log(f"initial SSSP: rounds={st['rounds']:.0f} edges={st['edges_traversed']:.3e}")
     n_ops = int(wl["frac"] * m)
-    nb = args.warmup + args.steps
+    n_plain = 3
+    nb = args.warmup + args.steps + n_plain
     dev = []
     for b in range(nb):  # inputs resident in HBM before the timed region
         ops = ctx.generate_batch(wl["seed"], b, n_ops // 2, n_ops - n_ops // 2, wl["wmin"], wl["wmax"])
@@ -385,19 +476,19 @@ def run_web(args, wl):
     del ops
     for b in range(args.warmup):
         ctx.apply_batch_device(dev[b])
-        ctx.pr_run()
+        ctx.sum_pr_run()
         ctx.sum_sssp_run()
     ctx.sync()
     ctx.profile_enable(True)
     per = []
     edges = 0.0
     t_all = time.perf_counter()
-    for b in range(args.warmup, nb):
+    for b in range(args.warmup, args.warmup + args.steps):
         t0 = time.perf_counter()
         sa = ctx.apply_batch_device(dev[b])
         ctx.sync()
         t1 = time.perf_counter()
-        sp = ctx.pr_run()
+        sp = ctx.sum_pr_run()
         ctx.sync()
         t2 = time.perf_counter()
         ss = ctx.sum_sssp_run()
@@ -407,29 +498,25 @@ def run_web(args, wl):
         edges += e
         per.append({"batch": b, "ms": round(1e3 * (t3 - t0), 3), "apply_ms": round(1e3 * (t1 - t0), 3),
                     "pr_ms": round(1e3 * (t2 - t1), 3), "sssp_ms": round(1e3 * (t3 - t2), 3),
-                    "pr_rounds": sp["rounds"], "sssp_rounds": ss["rounds"], "edges": e,
+                    "pr_sketch_rounds": sp["sketch_rounds"], "pr_residual_rounds": sp["residual_rounds"],
+                    "sssp_rounds": ss["rounds"], "mass_defect": sp["mass_defect"], "edges": e,
                     "GTEPS": round(e / (t3 - t0) / 1e9, 3)})
         log(f"batch {b}: {per[-1]}")
+    ctx.sync()
     dt = time.perf_counter() - t_all
     kids = [L.KERNEL_PR_PUSH, L.KERNEL_PR_PULL, L.KERNEL_PR_GATHER, L.KERNEL_SSSP_SKETCH, L.KERNEL_BATCH_COPY]
     kernel, roof = roofline(ctx, kids, args.workload)
     extra["kernels"] = kernel_table(ctx, kids)
     ctx.profile_enable(False)
-    for d in dev:
-        d.free()
     ms_per_step = 1e3 * dt / args.steps
     gteps = edges / dt / 1e9
     log(f"incremental: {ms_per_step:.3f} ms/step, {edges / args.steps:.3e} edges/step, {gteps:.3f} GTEPS")
     extra["batches"] = per
 
-    # the incremental states, then full recomputes on the same GPU and final graph
-    rank, _ = ctx.pr_download()
+    # checks on the graph of the last step: SSSP against a plain full recompute bit
+    # for bit, PageRank against an FP64 run (L1, Linf), and the full recomputes' time
+    rank, delta = ctx.pr_download()
     dist, par = ctx.sssp_download()
-    t0 = time.perf_counter()
-    ctx.pr_init()
-    ctx.pr_run()
-    ctx.sync()
-    full_pr = 1e3 * (time.perf_counter() - t0)
     t0 = time.perf_counter()
     ctx.sssp_init(wl["source"])
     ctx.sssp_run()
@@ -438,17 +525,13 @@ def run_web(args, wl):
     d2, p2 = ctx.sssp_download()
     extra["sssp_equals_full_recompute"] = bool(np.array_equal(dist, d2) and np.array_equal(par, p2))
     del d2, p2
-    mean = lambda k: float(np.mean([x[k] for x in per]))  # noqa: E731
-    extra.update({"full_recompute_ms": {"pagerank": round(full_pr, 3), "sssp": round(full_ss, 3)},
-                  "incremental_ms": {"pagerank": round(mean("pr_ms") + mean("apply_ms"), 3),
-                                     "sssp": round(mean("sssp_ms"), 3)},
-                  "speedup_vs_full_recompute": {
-                      "pagerank": round(full_pr / (mean("pr_ms") + mean("apply_ms")), 2),
-                      "sssp": round(full_ss / mean("sssp_ms"), 2),
-                      "step": round((full_pr + full_ss) / ms_per_step, 2)},
-                  "edges_per_step": edges / args.steps, "termcheck_threshold": wl["T"]})
-    log(f"full recompute: pagerank {full_pr:.1f} ms, sssp {full_ss:.1f} ms; sssp bit-exact vs full: "
-        f"{extra['sssp_equals_full_recompute']}")
+    t0 = time.perf_counter()
+    ctx.pr_init()
+    stf = ctx.pr_run()
+    ctx.sync()
+    full_pr = 1e3 * (time.perf_counter() - t0)
+    mean = lambda xs, k: float(np.mean([x[k] for x in xs]))  # noqa: E731
+    extra["mass_defect_max"] = float(max(abs(x["mass_defect"]) for x in per))
     if not args.no_verify:
         ctx.set_params(pr_precision=suminc_amd.PR_FP64, termcheck_threshold=1e-12)
         ctx.pr_init()
@@ -456,48 +539,167 @@ def run_web(args, wl):
         r64, _ = ctx.pr_download()
         extra["l1_vs_fp64"] = float(np.abs(rank - r64).sum())
         extra["linf_vs_fp64"] = float(np.abs(rank - r64).max())
-        extra["fp64_reference"] = f"plain FP64 delta-PageRank on the same GPU to sum|delta| < 1e-12 " \
-                                  f"({st['rounds']:.0f} rounds)"
+        extra["fp64_reference"] = f"plain FP64 delta-PageRank on the same GPU and graph to sum|delta| < 1e-12 " \
+                                  f"({st['rounds']:.0f} rounds, mass defect {st['mass_defect']:.2e})"
         log(f"L1 vs FP64 = {extra['l1_vs_fp64']:.3e}, Linf {extra['linf_vs_fp64']:.3e}")
         del r64
-        ctx.set_params(pr_precision=suminc_amd.PR_FP32_KAHAN, termcheck_threshold=wl["T"])
+        ctx.set_params(pr_precision=suminc_amd.PR_FP32_KAHAN, termcheck_threshold=T)
+    ctx.pr_upload(rank, delta)  # back to the incremental state
+
+    # the same batch size with PageRank on the plain CSR (SSSP as in the steps)
+    plain = []
+    for b in range(args.warmup + args.steps, nb):
+        t0 = time.perf_counter()
+        ctx.apply_batch_device(dev[b])
+        ctx.sync()
+        t1 = time.perf_counter()
+        sp = ctx.pr_run()
+        ctx.sync()
+        t2 = time.perf_counter()
+        ctx.sum_sssp_run()
+        plain.append({"apply_ms": 1e3 * (t1 - t0), "pr_ms": 1e3 * (t2 - t1), "rounds": sp["rounds"]})
+    for d in dev:
+        d.free()
+    extra["plain_csr_pagerank"] = {"batches": n_plain, "pr_ms": round(mean(plain, "pr_ms"), 3),
+                                   "apply_ms": round(mean(plain, "apply_ms"), 3),
+                                   "rounds": round(mean(plain, "rounds"), 1)}
+    log(f"plain CSR PageRank: {extra['plain_csr_pagerank']}")
+    inc_pr = mean(per, "pr_ms") + mean(per, "apply_ms")
+    inc_plain = mean(plain, "pr_ms") + mean(plain, "apply_ms")
+    extra.update({"full_recompute_ms": {"pagerank": round(full_pr, 3), "pagerank_rounds": stf["rounds"],
+                                        "sssp": round(full_ss, 3)},
+                  "incremental_ms": {"pagerank_sketch": round(inc_pr, 3), "pagerank_plain": round(inc_plain, 3),
+                                     "sssp": round(mean(per, "sssp_ms"), 3)},
+                  "speedup_vs_full_recompute": {
+                      "pagerank_sketch": round(full_pr / inc_pr, 2),
+                      "pagerank_plain": round(full_pr / inc_plain, 2),
+                      "sssp": round(full_ss / mean(per, "sssp_ms"), 2),
+                      "step": round((full_pr + full_ss) / ms_per_step, 2)},
+                  "edges_per_step": edges / args.steps, "termcheck_threshold": T})
+    log(f"full recompute: pagerank {full_pr:.1f} ms, sssp {full_ss:.1f} ms; sssp bit-exact vs full: "
+        f"{extra['sssp_equals_full_recompute']}")
+    rank, delta = ctx.pr_download()
+    dist, par = ctx.sssp_download()
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline: downloading the final graph")
-        off, col, _ = ctx.download_csr()
-        g = oracle.Graph.from_csr(off, col)
-        del off, col
-        cores = min(host_cores(), 16)
-        oracle.set_threads(cores)
-        t1 = time.perf_counter()
-        _, _, sto = g.pr_ingress(DAMPING, wl["T"], max_rounds=3)
-        tc = time.perf_counter() - t1
-        cpu = {"value": round(sto["edges"] / tc / 1e9, 4), "unit": "GTEPS", "cores": cores, "kind": "port",
-               "sample": f"oracle delta-PageRank Ingress (reference CPU loop restated, FP64) from scratch on "
-                         f"the final s{wl['scale']} graph, first {sto['rounds']:.0f} rounds: "
-                         f"{sto['edges']:.3e} edges in {tc:.2f}s"}
-        log(f"cpu baseline: {cpu}")
+        cpu = cpu_incremental_step(args, wl, ctx, rank, delta, dist, par, mean(plain, "rounds"))
     ctx.close()
-    # weighted personalized PageRank + adjoint on the same R-MAT graph with FP32
-    # weights from the generator (a context of its own: the SSSP graph's weights
-    # are integers), Dirichlet(1) personalization over random seeds, c ~ N(0,1)
-    with suminc_amd.Context(termcheck_threshold=wl["T"], pr_precision=suminc_amd.PR_FP32_KAHAN,
-                            max_rounds=1_000_000, pull_band_kib=args.pull_band_kib) as cw:
-        cw.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"], weights="f32")
-        rng = np.random.default_rng(wl["seed"])
-        pvec = np.zeros(n, np.float32)
-        seeds = rng.choice(n, wl["adjoint_seeds"], replace=False)
-        pvec[seeds] = rng.dirichlet(np.ones(wl["adjoint_seeds"])).astype(np.float32)
-        cvec = rng.standard_normal(n, dtype=np.float32)
-        _, sf = cw.wpr_forward(pvec)
-        _, gw, sb = cw.wpr_backward(cvec)
-        mw = cw.graph_info()[1]
-        extra["adjoint"] = {"edges": mw, "forward_ms": round(sf["kernel_ms"], 3), "forward_rounds": sf["rounds"],
-                            "backward_ms": round(sb["kernel_ms"], 3), "backward_rounds": sb["rounds"],
-                            "grad_w_finite": bool(np.isfinite(gw).all())}
-        log(f"adjoint: {extra['adjoint']}")
-        del gw
+    del rank, delta, dist, par
+    if not args.no_adjoint:
+        # weighted personalized PageRank + adjoint on the same R-MAT graph with FP32
+        # weights (a context of its own: the SSSP graph's weights are integers),
+        # Dirichlet(1) personalization over random seeds, c ~ N(0,1)
+        with suminc_amd.Context(termcheck_threshold=wl["T"], pr_precision=suminc_amd.PR_FP32_KAHAN,
+                                max_rounds=1_000_000, pull_band_kib=args.pull_band_kib) as cw:
+            cw.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"], weights="f32")
+            rng = np.random.default_rng(wl["seed"])
+            pvec = np.zeros(n, np.float32)
+            seeds = rng.choice(n, wl["adjoint_seeds"], replace=False)
+            pvec[seeds] = rng.dirichlet(np.ones(wl["adjoint_seeds"])).astype(np.float32)
+            cvec = rng.standard_normal(n, dtype=np.float32)
+            _, sf = cw.wpr_forward(pvec)
+            _, gw, sb = cw.wpr_backward(cvec)
+            mw = cw.graph_info()[1]
+            extra["adjoint"] = {"edges": mw, "forward_ms": round(sf["kernel_ms"], 3),
+                                "forward_rounds": sf["rounds"], "backward_ms": round(sb["kernel_ms"], 3),
+                                "backward_rounds": sb["rounds"], "grad_w_finite": bool(np.isfinite(gw).all())}
+            log(f"adjoint: {extra['adjoint']}")
+            del gw
+    if roof is not None and not args.no_traffic:
+        sub = {"k_pr_pull": "k_pr_pull<float>"}.get(kernel)
+        if sub is None:
+            roof["traffic_note"] = f"no live counter run for {kernel}"
+        else:
+            tb, note = _pmc_traffic(args, sub)
+            roof["traffic"] = None if tb is None else round(tb)
+            roof["traffic_note"] = note
+            if tb is not None:
+                roof["traffic_over_algorithmic"] = round(tb / roof["alg_bytes_per_launch"], 3)
     return gteps, ms_per_step, "f32+u32", roof, cpu, extra
+
+
+def cpu_incremental_step(args, wl, ctx, rank, delta, dist, par, plain_rounds):
+    """The reference's incremental step restated on the host (oracle, FP64): the
+    graph update (IncFragmentBuilder's rebuild), AmendValue(-1) over every vertex on
+    the old graph and AmendValue(+1) on the new one (sum_sync_iter_worker.h:139-345,
+    the full sweeps the reference runs), Ingress rounds (:857-882), and the
+    incremental SSSP step (deltaCompute reset + resend + rounds,
+    sum_sync_traversal_worker.h:113-352).  All cores, then one thread, on a bounded
+    sample: the PageRank rounds stop after about cpu_seconds."""
+    import oracle
+
+    log("cpu baseline: downloading the final graph")
+    off, col, w = ctx.download_csr()
+    n = len(off) - 1
+    m = len(col)
+    g = oracle.Graph.from_csr(off, col, w)
+    del off, col, w
+    # one more batch of the same size; the updated graph comes from the device's
+    # merge (the reference rebuilds its fragment on the host, which is not timed here)
+    n_ops = int(wl["frac"] * m)
+    ops = ctx.generate_batch(wl["seed"], 10_000, n_ops // 2, n_ops - n_ops // 2, wl["wmin"], wl["wmax"])
+    ctx.apply_batch(ops)
+    log("cpu baseline: downloading the updated graph")
+    off, col, w = ctx.download_csr()
+    g2 = oracle.Graph.from_csr(off, col, w)
+    del off, col, w
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or host_cores()
+    nproc = os.cpu_count()
+    res = {}
+    for label, th in (("all", threads), ("single", 1)):
+        oracle.set_threads(th)
+        r = rank.copy()
+        dl = delta.copy()
+        sample = 1.0 if label == "all" else 0.125  # one thread amends a seeded eighth of the vertices
+        touched = None
+        if sample < 1.0:
+            touched = (np.random.default_rng(wl["seed"]).random(n) < sample).astype(np.uint8)
+        t0 = time.perf_counter()
+        g.pr_amend(DAMPING, -1, r, dl, touched)
+        g2.pr_amend(DAMPING, +1, r, dl, touched)
+        t_amend = time.perf_counter() - t0
+        amend_edges = sample * (g.m + g2.m)
+        log(f"cpu baseline ({label}, {th} threads): amend {t_amend:.1f}s")
+        t0 = time.perf_counter()
+        rounds, redges = 0, 0.0
+        budget = args.cpu_seconds if label == "all" else args.cpu_seconds / 2
+        while time.perf_counter() - t0 < budget and rounds < plain_rounds:
+            r, dl, sto = g2.pr_ingress(DAMPING, wl["T"], max_rounds=1, r=r, delta=dl)
+            rounds += 1
+            redges += sto["edges"]
+            log(f"cpu baseline ({label}): round {rounds} {time.perf_counter() - t0:.1f}s")
+        t_rounds = time.perf_counter() - t0
+        t_ss, ss_edges = 0.0, 0.0
+        if label == "all":
+            t0 = time.perf_counter()
+            _, _, sts = g2.sssp_inc(g, wl["source"], dist, par)
+            t_ss = time.perf_counter() - t0
+            ss_edges = sts["edges"]
+            log(f"cpu baseline ({label}): sssp step {t_ss:.1f}s")
+        t_tot = t_amend + t_rounds + t_ss
+        e_tot = amend_edges + redges + ss_edges
+        per_round = t_rounds / max(rounds, 1)
+        res[label] = {"threads": th, "GTEPS": round(e_tot / t_tot / 1e9, 4),
+                      "amend_s": round(t_amend, 3), "amend_sample": sample, "pr_rounds_run": rounds,
+                      "pr_round_s": round(per_round, 3), "sssp_inc_s": round(t_ss, 3),
+                      "edges": e_tot}
+        if label == "all":
+            res[label]["ms_per_step_estimate"] = round(1e3 * (t_amend + per_round * plain_rounds + t_ss), 1)
+        log(f"cpu baseline ({label}): {res[label]}")
+    del g, g2
+    oracle.set_threads(threads)
+    a = res["all"]
+    return {"value": a["GTEPS"], "unit": "GTEPS", "cores": threads, "kind": "port",
+            "nproc": nproc, "affinity_cpus": host_cores(),
+            "single_thread": res["single"], "all_cores": a,
+            "ms_per_step_estimate": a["ms_per_step_estimate"],
+            "sample": f"oracle restatement (FP64, OpenMP) of the reference's incremental step on the final "
+                      f"s{wl['scale']} graph after one more {100 * wl['frac']:g}% batch: AmendValue(-1/+1) "
+                      f"sweeps over every vertex, {a['pr_rounds_run']} Ingress rounds (the GPU's plain path "
+                      f"needs {plain_rounds:.0f}; ms_per_step_estimate scales the measured round time to that), "
+                      f"the incremental SSSP step; GTEPS = edges amended/pushed/relaxed per second; one thread: "
+                      f"amend over a seeded eighth of the vertices plus {res['single']['pr_rounds_run']} rounds; "
+                      f"the host fragment rebuild is not in the sample (the updated graph comes from the device)"}
 
 
 def run_prfull(args, wl):
@@ -676,7 +878,8 @@ def supernodes_note(args, wl):
     if wl["kind"] in ("wpr", "prfull"):
         return "none (no batches, no sketch)"
     if wl["kind"] == "web":
-        return "SSSP over supernodes (leaf clusters <= %d members), PageRank on the plain CSR" % wl["max_node"]
+        return ("PageRank and SSSP over supernodes (leaf clusters <= %d members); PageRank on the plain CSR "
+                "reported beside it" % wl["max_node"])
     if args.supernodes == "on" and wl.get("max_node"):
         return "on (leaf clusters <= %d members, compress threshold 1)" % wl["max_node"]
     return "off (plain CSR path)"
@@ -687,7 +890,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="sssp22", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="web26", choices=sorted(WORKLOADS))
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--supernodes", default="on", choices=["on", "off"],
@@ -697,8 +900,16 @@ def main():
                     help="PageRank workloads: source band of the banded pull in KiB (0: off, the library default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device (counter calibration for rocprofv3 --pmc) and exit")
+    ap.add_argument("--no-adjoint", action="store_true", help="web26: skip the weighted-PageRank forward + adjoint")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc child runs that measure the dominant kernel's HBM traffic")
+    ap.add_argument("--traffic-timeout", type=int, default=240, help="seconds per rocprofv3 counter run")
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     KERNEL_NAMES.update(kernel_names())
+    if args.traffic_child:
+        traffic_child(args, WORKLOADS[args.workload])
+        return
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return

@@ -218,7 +218,7 @@ int suminc_sssp_download(suminc_ctx* ctx, uint32_t* dist, uint32_t* parent);
  * 4636-4899; iter_compressor.h:65-196, 318-491, 534-785; sum_sync_iter_worker.h:
  * 348-438, 949-1127, 1385-1491). */
 typedef struct suminc_sketch_info {
-  uint64_t clusters;            /* accepted clusters (supernode groups) */
+  uint64_t clusters;            /* accepted clusters (supernode groups) not dissolved */
   uint64_t raw_clusters;        /* clusters in the input assignment */
   uint64_t entries;             /* entry vertices = supernodes (Fc >= 0) */
   uint64_t clustered_vertices;
@@ -227,7 +227,12 @@ typedef struct suminc_sketch_info {
   uint64_t internal_edges;      /* edges inside accepted clusters */
   uint64_t shortcut_slots;      /* sum over entries of their cluster size (dense rows) */
   uint64_t max_cluster;
-  uint64_t reserved[8];
+  /* accepted clusters dissolved after batches: the acceptance rule of the build
+   * (internal edges - entries * exits > compress_threshold) is checked again for
+   * every cluster a batch touches; one that fails it has its members laid out as
+   * Singles from then on (their cluster_of reads SUMINC_INF_U32) */
+  uint64_t dissolved_clusters;
+  uint64_t reserved[7];
 } suminc_sketch_info;
 /* Built-in clustering: every vertex whose only neighbour (in or out) is h joins
  * h's cluster (a pair anchors on its smaller id); clusters of fewer than
@@ -320,7 +325,7 @@ int suminc_get_stats(suminc_ctx* ctx, suminc_stats* stats);
 #define SUMINC_KERNEL_PR_PUSH 0   /* PageRank scatter: 12 B/edge + 16 B/vertex */
 #define SUMINC_KERNEL_PR_GATHER 1 /* PageRank vertex pass: 4 B/vertex + 16 B/active vertex */
 #define SUMINC_KERNEL_SSSP_PUSH 2 /* SSSP relax: 16 B/edge + 16 B/vertex */
-#define SUMINC_KERNEL_PR_PULL 3   /* PageRank CSC gather: 8 B/edge + 16 B/vertex (FP32) */
+#define SUMINC_KERNEL_PR_PULL 3   /* PageRank CSC gather, plain and sketch rounds: 8 B/edge + 16 B/vertex (FP32) */
 #define SUMINC_KERNEL_SSSP_SKETCH 4 /* SSSP sketch rounds: 16 B/sketch edge + 20 B/shortcut slot + 16 B/vertex */
 #define SUMINC_KERNEL_BATCH_COPY 5  /* batch merge, old-edge copy: 8 B/edge per array (read + write), CSR + CSC */
 #define SUMINC_KERNEL_COUNT 8

@@ -69,7 +69,8 @@ class SketchInfo(C.Structure):
         ("internal_edges", C.c_uint64),
         ("shortcut_slots", C.c_uint64),
         ("max_cluster", C.c_uint64),
-        ("reserved", C.c_uint64 * 8),
+        ("dissolved_clusters", C.c_uint64),
+        ("reserved", C.c_uint64 * 7),
     ]
 
     def as_dict(self) -> dict:

@@ -921,7 +921,7 @@ static void pr_reseed(suminc_ctx* c, uint32_t N, double type, bool lists) {
   k_reseed_push<T><<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
                                                           c->pr_delta.as<T>(), c->wl0.as<uint32_t>(), c->wl1.as<T>(),
                                                           c->heavy.as<uint4>(), cnt);
-  if (div > 0) pull_scatter<T>(c, nullptr, nullptr, 1);
+  if (div > 0) pull_scatter<T>(c, 1);
   k_reset_lists<<<1, 64, 0, c->stream>>>(cnt);
   SUMINC_HIP(hipGetLastError());
 }

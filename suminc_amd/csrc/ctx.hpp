@@ -157,7 +157,10 @@ struct Sketch {
   DevBuf sc_R, sc_D;         // PageRank rows (reach mass / residual) per entry
   DevBuf ss_dist;            // SSSP rows (in-cluster distance to every exit) per entry
   DevBuf bound, spnode;      // PageRank sketch accumulators
-  DevBuf pull_chk;           // n: an internal in-edge from an exit of the vertex's cluster (sketch pull)
+  DevBuf xi_list, xi_count;  // internal edges leaving exits: (contribution slot of the exit, destination)
+  DevBuf exits, exit_count;  // the exits of the current layout (sketch rounds' exit pass)
+  DevBuf dead;               // K: cluster dissolved (its benefit fell to the threshold after batches)
+  double threshold = 1.0;    // compress_threshold of the build (the dissolution test)
   DevBuf dirty;              // K: cluster touched since its last precompute
   DevBuf ss_dirty;           // K: SSSP precompute flags (clusters of frontier members)
   bool ss_dirty_clear = false;  // a finished query left every flag clear

@@ -227,31 +227,12 @@ __device__ __forceinline__ bool pull_runs(const Counters* cnt, int gate) {
   return gate ? cnt->reseed_mode == 1 : (!cnt->done && cnt->mode == 1);
 }
 
-// kSketch: the sketch rounds' pull, where an in-edge from a member of v's own
-// cluster is internal (carried by the shortcuts) and skipped.
-template <bool kSketch>
-__device__ __forceinline__ bool pull_skip(const uint32_t* cid, uint32_t u, uint32_t cv) {
-  if constexpr (kSketch) return cv != INF32 && cid[u] == cv;
-  return false;
-}
-
-// The cluster whose in-edges v must check.  In the sketch pull only Singles and
-// exits contribute, so only an internal in-edge from an exit of v's cluster must
-// be skipped: chk[v] != 0 marks the vertices that have one.  For the others an
-// internal in-edge adds zero and needs no check.
-template <bool kSketch>
-__device__ __forceinline__ uint32_t pull_cluster(const uint32_t* cid, const uint8_t* chk, uint32_t v) {
-  if constexpr (kSketch) return chk[v] ? cid[v] : INF32;
-  return INF32;
-}
-
 // Sum of contrib over the in-edges k = k0 + r * stride (r < U, k < e).  All U
 // column loads are issued before the U gathers, so a thread keeps U independent
 // gathers in flight instead of one dependent (column, gather) pair per edge: the
 // gathers are cache-miss latency bound, not bandwidth bound.
-template <class T, bool kSketch, int U>
-__device__ __forceinline__ T gather_sum(const uint32_t* __restrict__ icol, const T* __restrict__ contrib,
-                                        const uint32_t* __restrict__ cid, uint32_t cv, uint32_t k0,
+template <class T, int U>
+__device__ __forceinline__ T gather_sum(const uint32_t* __restrict__ icol, const T* __restrict__ contrib, uint32_t k0,
                                         uint32_t stride, uint32_t e) {
   uint32_t u[U];
 #pragma unroll
@@ -261,7 +242,7 @@ __device__ __forceinline__ T gather_sum(const uint32_t* __restrict__ icol, const
   }
   T x[U];
 #pragma unroll
-  for (int r = 0; r < U; ++r) x[r] = u[r] != INF32 && !pull_skip<kSketch>(cid, u[r], cv) ? contrib[u[r]] : T(0);
+  for (int r = 0; r < U; ++r) x[r] = u[r] != INF32 ? contrib[u[r]] : T(0);
   T acc = T(0);
 #pragma unroll
   for (int r = 0; r < U; ++r) acc += x[r];
@@ -271,38 +252,34 @@ __device__ __forceinline__ T gather_sum(const uint32_t* __restrict__ icol, const
 // Pull, short rows (in-degree 1..kPullThread): one thread per vertex sums its
 // in-edges in order; no atomics (each vertex has one owner).  bid / nb: this
 // workgroup's index among the nb workgroups of the part.
-template <class T, bool kSketch>
+template <class T>
 __device__ __forceinline__ void pull_short_part(uint32_t bid, uint32_t nb, const uint32_t* __restrict__ list,
                                                 uint32_t count, const uint32_t* __restrict__ ioff,
                                                 const uint32_t* __restrict__ icol, const T* __restrict__ contrib,
-                                                T* __restrict__ delta, const uint32_t* __restrict__ cid,
-                                                const uint8_t* __restrict__ chk) {
+                                                T* __restrict__ delta) {
   for (uint32_t i = bid * kBlock + threadIdx.x; i < count; i += nb * kBlock) {
     const uint32_t v = list[i];
     const uint32_t b = ioff[v], e = ioff[v + 1];
-    const uint32_t cv = pull_cluster<kSketch>(cid, chk, v);
     T acc = T(0);
-    for (uint32_t k = b; k < e; k += 4) acc += gather_sum<T, kSketch, 4>(icol, contrib, cid, cv, k, 1, e);
+    for (uint32_t k = b; k < e; k += 4) acc += gather_sum<T, 4>(icol, contrib, k, 1, e);
     if (acc != T(0)) delta[v] += acc;
   }
 }
 
 // Pull, medium rows (kPullThread < in-degree <= kHeavyDeg): one wave per vertex,
 // lanes stride the row (coalesced column reads), shuffle reduction.
-template <class T, bool kSketch>
+template <class T>
 __device__ __forceinline__ void pull_wave_part(uint32_t bid, uint32_t nb, const uint32_t* __restrict__ list,
                                                uint32_t count, const uint32_t* __restrict__ ioff,
                                                const uint32_t* __restrict__ icol, const T* __restrict__ contrib,
-                                               T* __restrict__ delta, const uint32_t* __restrict__ cid,
-                                               const uint8_t* __restrict__ chk) {
+                                               T* __restrict__ delta) {
   const uint32_t waves = nb * (kBlock / 64);
   for (uint32_t i = bid * (kBlock / 64) + (threadIdx.x >> 6); i < count; i += waves) {
     const uint32_t v = list[i];
     const uint32_t b = ioff[v], e = ioff[v + 1];
-    const uint32_t cv = pull_cluster<kSketch>(cid, chk, v);
     T acc = T(0);
     for (uint32_t k = b + lane_id(); k < e; k += 4 * 64)  // one pass for rows up to 256
-      acc += gather_sum<T, kSketch, 4>(icol, contrib, cid, cv, k, 64, e);
+      acc += gather_sum<T, 4>(icol, contrib, k, 64, e);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
     if (lane_id() == 0 && acc != T(0)) delta[v] += acc;
@@ -311,18 +288,16 @@ __device__ __forceinline__ void pull_wave_part(uint32_t bid, uint32_t nb, const 
 
 // Pull, heavy part: 2048-edge chunks of vertices with in-degree > 256, one
 // workgroup per chunk, one atomic per chunk.
-template <class T, bool kSketch>
+template <class T>
 __device__ __forceinline__ void pull_heavy_part(uint32_t bid, uint32_t nb, T* s_red, const uint2* __restrict__ chunks,
                                                 uint64_t nchunks, const uint32_t* __restrict__ ioff,
                                                 const uint32_t* __restrict__ icol, const T* __restrict__ contrib,
-                                                T* __restrict__ delta, const uint32_t* __restrict__ cid,
-                                                const uint8_t* __restrict__ chk) {
+                                                T* __restrict__ delta) {
   for (uint64_t h = bid; h < nchunks; h += nb) {
     const uint2 ch = chunks[h];
     const uint32_t b = ioff[ch.x] + ch.y * kChunk;
     const uint32_t e = min(ioff[ch.x + 1], b + kChunk);
-    const uint32_t cv = pull_cluster<kSketch>(cid, chk, ch.x);
-    T acc = gather_sum<T, kSketch, kChunk / kBlock>(icol, contrib, cid, cv, b + threadIdx.x, kBlock, e);
+    T acc = gather_sum<T, kChunk / kBlock>(icol, contrib, b + threadIdx.x, kBlock, e);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
     if (lane_id() == 0) s_red[threadIdx.x >> 6] = acc;
@@ -339,26 +314,25 @@ __device__ __forceinline__ void pull_heavy_part(uint32_t bid, uint32_t nb, T* s_
 // One launch for the three row classes: workgroups [0, gh) take the heavy chunks
 // (the longest work first), [gh, gh + gw) the wave rows, the rest the short rows,
 // so the classes' tails overlap instead of following each other.
-template <class T, bool kSketch = false>
+template <class T>
 __global__ void __launch_bounds__(kBlock) k_pr_pull(const uint32_t* __restrict__ short_list,
                                                     const uint32_t* __restrict__ wave_list,
                                                     const uint2* __restrict__ chunks,
                                                     const unsigned long long* __restrict__ counts,
                                                     const uint32_t* __restrict__ ioff,
                                                     const uint32_t* __restrict__ icol, const T* __restrict__ contrib,
-                                                    T* __restrict__ delta, const Counters* cnt, int gate,
-                                                    const uint32_t* __restrict__ cid,
-                                                    const uint8_t* __restrict__ chk, uint32_t gh, uint32_t gw) {
+                                                    T* __restrict__ delta, const Counters* cnt, int gate, uint32_t gh,
+                                                    uint32_t gw) {
   __shared__ T s_red[kBlock / 64];
   if (!pull_runs(cnt, gate)) return;
   const uint32_t bid = blockIdx.x;
   if (bid < gh)
-    pull_heavy_part<T, kSketch>(bid, gh, s_red, chunks, counts[2], ioff, icol, contrib, delta, cid, chk);
+    pull_heavy_part<T>(bid, gh, s_red, chunks, counts[2], ioff, icol, contrib, delta);
   else if (bid < gh + gw)
-    pull_wave_part<T, kSketch>(bid - gh, gw, wave_list, (uint32_t)counts[1], ioff, icol, contrib, delta, cid, chk);
+    pull_wave_part<T>(bid - gh, gw, wave_list, (uint32_t)counts[1], ioff, icol, contrib, delta);
   else
-    pull_short_part<T, kSketch>(bid - gh - gw, gridDim.x - gh - gw, short_list, (uint32_t)counts[0], ioff, icol,
-                                contrib, delta, cid, chk);
+    pull_short_part<T>(bid - gh - gw, gridDim.x - gh - gw, short_list, (uint32_t)counts[0], ioff, icol, contrib,
+                       delta);
 }
 
 __global__ void __launch_bounds__(1024) k_pr_end(Counters* cnt, const unsigned long long* __restrict__ act_partial,
@@ -748,7 +722,7 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_band_heavy(const uint4* __re
   const uint32_t h1 = off[p + 1];
   for (uint32_t h = off[p] + blockIdx.x; h < h1; h += gridDim.x) {
     const uint4 ch = chunks[h];
-    T acc = gather_sum<T, false, kChunk / kBlock>(icol, contrib, nullptr, INF32, ch.y + threadIdx.x, kBlock, ch.z);
+    T acc = gather_sum<T, kChunk / kBlock>(icol, contrib, ch.y + threadIdx.x, kBlock, ch.z);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
     if (lane_id() == 0) s_red[threadIdx.x >> 6] = acc;
@@ -950,17 +924,14 @@ void ensure_pull_heavy(suminc_ctx* c) {
 }
 
 // The pull launches of a round (kernels return unless the gate opens, see
-// pull_runs).  The list lengths stay on the device.  cid != nullptr: the sketch
-// rounds' pull (internal in-edges skipped; chk as in pull_cluster).
+// pull_runs).  The list lengths stay on the device.  The contributions are at the
+// compact positions when those are in use (the columns renamed to them).
 template <class T>
-void pull_scatter(suminc_ctx* c, const uint32_t* cid, const uint8_t* chk, int gate) {
+void pull_scatter(suminc_ctx* c, int gate) {
   Counters* cnt = dev_counters(c);
-  const bool sk = cid != nullptr;
   const unsigned long long* counts = c->pull_cnt.as<unsigned long long>();
-  // the plain pull gathers compact contributions through the renamed columns; the
-  // sketch pull keeps vertex ids (its skip test reads the source's cluster)
-  const uint32_t* icol = sk ? c->in_col.as<uint32_t>() : pull_cols(c);
-  if (!sk && c->band_P > 1) {  // every row band by band
+  const uint32_t* icol = pull_cols(c);
+  if (c->band_P > 1) {  // every row band by band
     const uint32_t W = (uint32_t)c->band_W, S = (uint32_t)c->band_S;
     for (uint32_t p = 0; p < c->band_P; ++p) {
       if (S)
@@ -980,14 +951,13 @@ void pull_scatter(suminc_ctx* c, const uint32_t* cid, const uint8_t* chk, int ga
   // grid: the heavy part as before (push_grid), the wave and short parts sized to
   // the vertex count (at most 8192 workgroups each)
   const uint32_t gh = push_grid(), gw = grid_for(c->n, kBlock / 64, 8192), gs = grid_for(c->n, kBlock, 8192);
-  auto kp = sk ? k_pr_pull<T, true> : k_pr_pull<T, false>;
-  kp<<<gh + gw + gs, kBlock, 0, c->stream>>>(c->pull_short.as<uint32_t>(), c->pull_wave.as<uint32_t>(),
-                                             c->pull_heavy.as<uint2>(), counts, c->in_off.as<uint32_t>(), icol,
-                                             c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, gate, cid, chk, gh,
-                                             gw);
+  k_pr_pull<T><<<gh + gw + gs, kBlock, 0, c->stream>>>(c->pull_short.as<uint32_t>(), c->pull_wave.as<uint32_t>(),
+                                                       c->pull_heavy.as<uint2>(), counts, c->in_off.as<uint32_t>(),
+                                                       icol, c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, gate, gh,
+                                                       gw);
 }
-template void pull_scatter<float>(suminc_ctx* c, const uint32_t* cid, const uint8_t* chk, int gate);
-template void pull_scatter<double>(suminc_ctx* c, const uint32_t* cid, const uint8_t* chk, int gate);
+template void pull_scatter<float>(suminc_ctx* c, int gate);
+template void pull_scatter<double>(suminc_ctx* c, int gate);
 
 // Enqueues one round; with profiling on, events 4*slot .. 4*slot+3 bracket the
 // count+decide, gather and scatter (push or pull) launches.
@@ -1012,7 +982,7 @@ static void enqueue_round(suminc_ctx* c, double eps, double T_stop, int max_roun
   k_pr_push<T><<<push_grid(), kBlock, 0, c->stream>>>(c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(),
                                                       c->pr_delta.as<T>(), c->wl0.as<uint32_t>(), c->wl1.as<T>(),
                                                       c->heavy.as<uint4>(), cnt);
-  if (div > 0) pull_scatter<T>(c, nullptr, nullptr, 0);
+  if (div > 0) pull_scatter<T>(c, 0);
   if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 4 * slot + 3), c->stream));
   k_pr_end<<<1, 1024, 0, c->stream>>>(cnt, reinterpret_cast<unsigned long long*>(c->partial.as<double>()) + 2 * g,
                                       gc);

@@ -36,11 +36,10 @@ const uint32_t* pull_cols(suminc_ctx* c);
 void build_pull_lists(suminc_ctx* c, uint64_t m_cap);
 // the same, once per graph version
 void ensure_pull_heavy(suminc_ctx* c);
-// the pull launches of a PageRank round (gate 0) or reseed (gate 1); cid != nullptr:
-// the sketch rounds' pull, chk[v] != 0 for the vertices with an internal in-edge
-// from an exit
+// the pull launches of a PageRank round (gate 0) or reseed (gate 1), over the
+// contributions at the compact positions when those are in use
 template <class T>
-void pull_scatter(suminc_ctx* c, const uint32_t* cid, const uint8_t* chk, int gate);
+void pull_scatter(suminc_ctx* c, int gate);
 
 // sssp.hip
 void sssp_init(suminc_ctx* c, uint32_t source);

@@ -337,13 +337,45 @@ constexpr int kLargeRegs = kMaxCluster / kBlock;  // member slots one thread of 
 // the edge counts (BothOutIn / OnlyIn / OnlyOut / Inner), the stable exits-first
 // member order with an entry flag per slot, local indices and the per-cluster
 // counts for the scan.  The members stay in registers between the two passes.
+// A touched cluster whose benefit on the current graph, the acceptance rule of the
+// first build (internal edges - entries * exits > compress_threshold, compared as
+// float), no longer holds is dissolved: from this layout on its members are
+// Singles (no entries, no exits, no shortcut rows), and it stays dissolved.  The
+// internal edges of cluster k are the keys [cl_off[k] << 32, cl_off[k + 1] << 32)
+// of the sorted internal edge list.
+struct Dissolve {
+  const unsigned long long* key;  // internal edge list, sorted
+  uint64_t n;
+  float threshold;
+  uint8_t* dead;                  // per cluster: dissolved
+};
+
+__device__ __forceinline__ uint64_t key_lower_bound(const unsigned long long* __restrict__ a, uint64_t n,
+                                                    unsigned long long x) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ bool dissolves(const Dissolve& dz, uint32_t k, uint32_t b, uint32_t size, uint32_t nen,
+                                          uint32_t nex) {
+  if (dz.dead[k]) return true;
+  const uint64_t inner = key_lower_bound(dz.key, dz.n, (unsigned long long)(b + size) << 32) -
+                         key_lower_bound(dz.key, dz.n, (unsigned long long)b << 32);
+  const float benefit = (float)((double)inner - (double)nen * (double)nex);
+  return !(benefit > dz.threshold);
+}
+
 __global__ void __launch_bounds__(kBlock) k_member_partition(uint32_t K, const uint32_t* __restrict__ cl_off,
                                                              const uint32_t* __restrict__ base_mem,
                                                              const uint32_t* __restrict__ ext_out,
                                                              const uint32_t* __restrict__ ext_in,
                                                              const uint8_t* __restrict__ touched, uint8_t* ntype,
                                                              uint32_t* cl_mem, uint8_t* slot_ent, uint32_t* lidx,
-                                                             uint32_t* cl_nexit, ClBase* cl_cnt) {
+                                                             uint32_t* cl_nexit, ClBase* cl_cnt, Dissolve dz) {
   const int lane = lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
   const uint32_t nw = gridDim.x * kWaves;
@@ -351,6 +383,20 @@ __global__ void __launch_bounds__(kBlock) k_member_partition(uint32_t K, const u
     if (!touched[k]) continue;  // whole wave
     const uint32_t b = cl_off[k], size = cl_off[k + 1] - b;
     if (size > kWaveCluster) continue;  // k_member_partition_large
+    if (dz.dead[k]) {  // whole wave: members stay Singles in base order
+      for (uint32_t j = lane; j < size; j += 64) {
+        const uint32_t v = base_mem[b + j];
+        ntype[v] = 0;
+        cl_mem[b + j] = v;
+        slot_ent[b + j] = 0;
+        lidx[v] = j;
+      }
+      if (lane == 0) {
+        cl_nexit[k] = 0;
+        cl_cnt[k] = ClBase{0ull, 0ull, 0u, 0u};
+      }
+      continue;
+    }
     uint32_t nex = 0, nen = 0;
     uint32_t v0 = 0;
     bool en0 = false, ex0 = false;
@@ -373,6 +419,21 @@ __global__ void __launch_bounds__(kBlock) k_member_partition(uint32_t K, const u
       const unsigned long long mex = __ballot(ex), men = __ballot(en);
       nex += __popcll(mex);
       nen += __popcll(men);
+    }
+    if (dissolves(dz, k, b, size, nen, nex)) {  // wave-uniform: every lane has the same counts
+      for (uint32_t j = lane; j < size; j += 64) {
+        const uint32_t v = base_mem[b + j];
+        ntype[v] = 0;
+        cl_mem[b + j] = v;
+        slot_ent[b + j] = 0;
+        lidx[v] = j;
+      }
+      if (lane == 0) {
+        dz.dead[k] = 1;
+        cl_nexit[k] = 0;
+        cl_cnt[k] = ClBase{0ull, 0ull, 0u, 0u};
+      }
+      continue;
     }
     uint32_t pe = 0, pi = nex;
     for (uint32_t j0 = 0; j0 < size; j0 += 64) {
@@ -415,7 +476,7 @@ __global__ void __launch_bounds__(kBlock) k_member_partition_large(const uint32_
                                                                    const uint32_t* __restrict__ ext_in,
                                                                    const uint8_t* __restrict__ touched, uint8_t* ntype,
                                                                    uint32_t* cl_mem, uint8_t* slot_ent, uint32_t* lidx,
-                                                                   uint32_t* cl_nexit, ClBase* cl_cnt) {
+                                                                   uint32_t* cl_nexit, ClBase* cl_cnt, Dissolve dz) {
   __shared__ uint32_t s_w0[kBlock / 64 + 1], s_w1[kBlock / 64 + 1];
   const uint32_t tid = threadIdx.x;
   for (uint32_t li = blockIdx.x; li < L; li += gridDim.x) {
@@ -454,6 +515,22 @@ __global__ void __launch_bounds__(kBlock) k_member_partition_large(const uint32_
     block_exclusive_scan<kBlock>(myex, s_w0, &nex);
     block_exclusive_scan<kBlock>(myen, s_w1, &nen);
     __syncthreads();  // the scans below reuse s_w0 and s_w1
+    if (dissolves(dz, k, b, size, nen, nex)) {  // workgroup-uniform: every thread has the same counts
+      for (uint32_t j = tid; j < size; j += kBlock) {
+        const uint32_t u = base_mem[b + j];
+        ntype[u] = 0;
+        cl_mem[b + j] = u;
+        slot_ent[b + j] = 0;
+        lidx[u] = j;
+      }
+      __syncthreads();  // every thread has read dead[k] before it is set
+      if (tid == 0) {
+        dz.dead[k] = 1;
+        cl_nexit[k] = 0;
+        cl_cnt[k] = ClBase{0ull, 0ull, 0u, 0u};
+      }
+      continue;
+    }
     uint32_t pe = 0, pi = nex;
 #pragma unroll
     for (int r = 0; r < kLargeRegs; ++r)
@@ -734,15 +811,20 @@ static void sketch_layout(suminc_ctx* c, bool rebuild) {
     s.n_large = (uint32_t)nl;
   }
   const unsigned large_grid = std::max<uint32_t>(1u, std::min<uint32_t>(s.n_large, 4096u));
+  // the sorted internal edge list (merged after every batch) gives each cluster's
+  // internal edge count for the dissolution test
+  s.dead.ensure((size_t)K + 1);
+  if (!rebuild) SUMINC_HIP(hipMemsetAsync(s.dead.p, 0, (size_t)K + 1, c->stream));
+  const Dissolve dz{s.iem_key.as<unsigned long long>(), s.iem_n, (float)s.threshold, s.dead.as<uint8_t>()};
   k_member_partition<<<layout_grid(K), kBlock, 0, c->stream>>>(
       K, s.cl_off.as<uint32_t>(), s.base_mem.as<uint32_t>(), s.ext_out.as<uint32_t>(), s.ext_in.as<uint32_t>(),
       s.touched.as<uint8_t>(), s.ntype.as<uint8_t>(), s.cl_mem.as<uint32_t>(), s.slot_ent.as<uint8_t>(),
-      s.lidx.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), cnt);
+      s.lidx.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), cnt, dz);
   if (s.n_large)
     k_member_partition_large<<<large_grid, kBlock, 0, c->stream>>>(
         s.large.as<uint32_t>(), s.n_large, s.cl_off.as<uint32_t>(), s.base_mem.as<uint32_t>(),
         s.ext_out.as<uint32_t>(), s.ext_in.as<uint32_t>(), s.touched.as<uint8_t>(), s.ntype.as<uint8_t>(),
-        s.cl_mem.as<uint32_t>(), s.slot_ent.as<uint8_t>(), s.lidx.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), cnt);
+        s.cl_mem.as<uint32_t>(), s.slot_ent.as<uint8_t>(), s.lidx.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), cnt, dz);
   cub(c, [&](void* p, size_t& b) {
     return rocprim::exclusive_scan(p, b, cnt, base, ClBase{0ull, 0ull, 0u, 0u}, (size_t)K + 1, ClBasePlus(),
                                    c->stream);
@@ -869,6 +951,7 @@ void sketch_build(suminc_ctx* c, uint32_t min_node, uint32_t max_node, double th
   SUMINC_HIP(hipMemsetAsync(icount.p, 0, 8, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
   s.K = K;
+  s.threshold = threshold;
   {
     // the internal edges of the accepted clusters size the list
     std::vector<uint32_t> hin(Kr + 1), hsize(Kr + 1);
@@ -1312,6 +1395,7 @@ __device__ __forceinline__ void push_item(bool want, uint32_t v, T cval, uint32_
 
 struct SkArgs {
   uint32_t n;
+  const uint32_t* pos;     // compact source position of each vertex for the pull's contributions (null: ids)
   const uint32_t* off;     // full out-CSR row pointers (degrees)
   const uint8_t* ntype;
   const uint32_t* ent_of;
@@ -1324,28 +1408,37 @@ struct SkArgs {
 };
 
 // residual of the sketch phase: |delta| of Single/entry vertices + |bound| of exits,
-// and the out-edges the round's senders will scatter (the push / pull choice)
+// and the out-edges the round's senders will scatter (the push / pull choice).
+// kSkPer vertices a thread (vertex base + j * kBlock + t, coalesced).
+constexpr int kSkPer = 4;
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_sk_count(SkArgs a, const T* __restrict__ delta,
                                                      const T* __restrict__ bound, double eps) {
   __shared__ double s_sum[kBlock / 64];
   __shared__ unsigned long long s_edges[kBlock / 64];
   if (a.cnt->done) return;
-  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
   double x = 0;
   unsigned long long ed = 0;
-  if (v < a.n) {
+  const uint32_t base = blockIdx.x * (kBlock * kSkPer);
+#pragma unroll
+  for (int j = 0; j < kSkPer; ++j) {
+    const uint32_t v = base + j * kBlock + threadIdx.x;
+    if (v >= a.n) break;
     const uint8_t t = a.ntype[v];
+    if (t == 4) continue;
     const uint32_t deg = a.off[v + 1] - a.off[v];
-    if (t != 4 && t != 1) {
-      const double dv = (double)fabs_t(delta[v]);
+    double dv = 0;
+    if (t != 1) {
+      dv = (double)fabs_t(delta[v]);
       x += dv;
       if (t == 0 && dv > eps) ed += deg;
     }
     if (t == 1 || t == 3) {
       const double b = (double)fabs_t(bound[v]);
       x += b;
-      if (b > eps) ed += deg;
+      // an exit sends in the round its cluster's entries inject: its own entry's
+      // mass (a BothOutIn hub's) leaves it this round
+      if (b > eps || (t == 3 && dv > eps)) ed += deg;
     }
   }
 #pragma unroll
@@ -1373,7 +1466,8 @@ __global__ void __launch_bounds__(kBlock) k_sk_count(SkArgs a, const T* __restri
 // Stop rule of the sketch phase and the round's scatter: pull over the CSC when
 // the senders' out-edges exceed m / div (as the plain rounds decide), else push.
 __global__ void __launch_bounds__(1024) k_sk_decide(const double* __restrict__ partial, uint32_t nparts, Counters* cnt,
-                                                    double T, int max_rounds, uint64_t m, int div) {
+                                                    double T, int max_rounds, uint64_t m, int div,
+                                                    uint8_t* __restrict__ mode_hist) {
   __shared__ double s[1024];
   __shared__ unsigned long long se[1024];
   if (cnt->done) return;
@@ -1403,60 +1497,29 @@ __global__ void __launch_bounds__(1024) k_sk_decide(const double* __restrict__ p
       return;
     }
     cnt->mode = div > 0 && se[0] * (unsigned long long)div > m ? 1 : 0;
+    if (cnt->rounds < 65536) mode_hist[cnt->rounds] = (uint8_t)cnt->mode;
     cnt->rounds += 1;
   }
 }
 
-// Node-type dispatch of one sketch round (sum_sync_iter_worker.h:949-1111).
-// flush != 0: only exits, every non-zero inflow (end of the sketch phase).
+// Node-type dispatch of one sketch round (sum_sync_iter_worker.h:949-1111), in
+// two passes around the shortcut launch: k_sk_send lets Singles push and entries
+// inject into their shortcut rows; k_sk_exits, after the rows moved that mass to
+// the exits, lets every exit turn its inflow into rank and send d/outdeg of it out
+// of the cluster, so mass crosses a cluster within the round it entered (an order
+// the reference's parallel loop may take as well).  The flush at the end of the
+// sketch phase is the exit pass over every non-zero inflow.
+// Block sums of the round's edges: k_sk_send in partial[g, 2g) (u64 view, g its
+// grid), k_sk_exits in partial[2g, 2g + its grid) (the count's slots, read by then).
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_sk_gather(SkArgs a, T* __restrict__ delta, T* __restrict__ bound,
-                                                      T* __restrict__ spnode, T* __restrict__ rank,
-                                                      float* __restrict__ comp, T* __restrict__ wl_c,
-                                                      T* __restrict__ sc_x, T* __restrict__ contrib, double eps, T d,
-                                                      int flush) {
-  if (a.cnt->done) return;
-  const bool pull = a.cnt->mode != 0;  // contributions for the CSC pull instead of push items
-  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-  bool want_push = false, want_sc = false;
-  T cval = T(0), sval = T(0);
-  uint32_t rowlen = 0;
-  if (v < a.n) {
-    const uint8_t t = a.ntype[v];
-    const uint32_t deg = a.off[v + 1] - a.off[v];
-    rowlen = deg;  // the push skips an exit's edges inside its cluster
-    if (t == 0 && !flush) {  // SingleNode: plain push over its out-list
-      T dv = delta[v];
-      if ((double)fabs_t(dv) > eps) {
-        delta[v] = T(0);
-        kahan_add(rank, comp, v, dv);
-        if (deg) { want_push = true; cval = d * dv / (T)deg; }
-      }
-    }
-    if ((t == 2 || t == 3) && !flush) {  // entry: inject into the shortcut
-      T dv = delta[v];
-      if ((double)fabs_t(dv) > eps) {
-        delta[v] = T(0);
-        spnode[v] += dv;
-        want_sc = true;
-        sval = dv;
-      }
-      sc_x[a.ent_of[v]] = sval;  // every entry's slot is written each round (0: idle)
-    }
-    if (t == 1 || t == 3) {  // exit: inflow becomes rank, d/outdeg of it leaves the cluster
-      T b = bound[v];
-      if (flush ? b != T(0) : (double)fabs_t(b) > eps) {
-        bound[v] = T(0);
-        kahan_add(rank, comp, v, b);
-        if (deg && rowlen) { want_push = true; cval = d * b / (T)deg; }
-      }
-    }
-  }
-  if (pull && v < a.n) contrib[v] = want_push ? cval : T(0);
-  push_item<T>(want_push && !pull, v, cval, rowlen, a.wl_v, wl_c, a.heavy, a.cnt);
-  // edges of this round: sketch out-edges pushed + shortcut slots used
+__device__ __forceinline__ void sk_contrib(const SkArgs& a, T* contrib, uint32_t v, T x) {
+  const uint32_t slot = a.pos ? a.pos[v] : v;  // vertices without out-edges have no position and never send
+  if (slot != INF32) contrib[slot] = x;
+}
+
+template <class T>
+__device__ __forceinline__ void sk_block_edges(unsigned long long ed, unsigned long long* out) {
   __shared__ unsigned long long s_e[kBlock / 64];
-  unsigned long long ed = (want_push ? rowlen : 0u) + (want_sc ? a.cl_nexit[a.cid[v]] : 0u);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) ed += __shfl_xor(ed, o);
   if (lane_id() == 0) s_e[threadIdx.x >> 6] = ed;
@@ -1464,8 +1527,95 @@ __global__ void __launch_bounds__(kBlock) k_sk_gather(SkArgs a, T* __restrict__ 
   if (threadIdx.x == 0) {
     unsigned long long t = 0;
     for (int w = 0; w < kBlock / 64; ++w) t += s_e[w];
-    reinterpret_cast<unsigned long long*>(a.partial)[gridDim.x + blockIdx.x] = t;
+    *out = t;
   }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_sk_send(SkArgs a, T* __restrict__ delta, T* __restrict__ spnode,
+                                                    T* __restrict__ rank, float* __restrict__ comp,
+                                                    T* __restrict__ wl_c, T* __restrict__ sc_x,
+                                                    T* __restrict__ contrib, double eps, T d) {
+  if (a.cnt->done) return;
+  const bool pull = a.cnt->mode != 0;  // contributions for the CSC pull instead of push items
+  const uint32_t base = blockIdx.x * (kBlock * kSkPer);
+  unsigned long long ed = 0;
+#pragma unroll
+  for (int j = 0; j < kSkPer; ++j) {  // uniform trip count: push_item appends per wave
+    const uint32_t v = base + j * kBlock + threadIdx.x;
+    bool want_push = false;
+    T cval = T(0);
+    uint32_t deg = 0;
+    if (v < a.n) {
+      const uint8_t t = a.ntype[v];
+      deg = a.off[v + 1] - a.off[v];
+      if (t == 0) {  // SingleNode: plain push over its out-list
+        const T dv = delta[v];
+        if ((double)fabs_t(dv) > eps) {
+          delta[v] = T(0);
+          kahan_add(rank, comp, v, dv);
+          if (deg) { want_push = true; cval = d * dv / (T)deg; }
+        }
+      } else if (t == 2 || t == 3) {  // entry: inject into the shortcut
+        const T dv = delta[v];
+        T sval = T(0);
+        if ((double)fabs_t(dv) > eps) {
+          delta[v] = T(0);
+          spnode[v] += dv;
+          sval = dv;
+          ed += a.cl_nexit[a.cid[v]];
+        }
+        sc_x[a.ent_of[v]] = sval;  // every entry's slot is written each round (0: idle)
+      }
+      if (pull) sk_contrib(a, contrib, v, want_push ? cval : T(0));  // every slot: the exits' come after
+    }
+    push_item<T>(want_push && !pull, v, cval, deg, a.wl_v, wl_c, a.heavy, a.cnt);
+    if (want_push) ed += deg;
+  }
+  sk_block_edges<T>(ed, reinterpret_cast<unsigned long long*>(a.partial) + gridDim.x + blockIdx.x);
+}
+
+// The exits of the current layout, one per thread (flush: every non-zero inflow).
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_sk_exits(SkArgs a, const uint32_t* __restrict__ exits,
+                                                     const unsigned long long* __restrict__ n_exits,
+                                                     T* __restrict__ bound, T* __restrict__ rank,
+                                                     float* __restrict__ comp, T* __restrict__ wl_c,
+                                                     T* __restrict__ contrib, double eps, T d, int flush,
+                                                     uint32_t part_base) {
+  if (a.cnt->done) return;
+  const bool pull = a.cnt->mode != 0;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  bool want_push = false;
+  T cval = T(0);
+  uint32_t v = 0, deg = 0;
+  if (i < *n_exits) {
+    v = exits[i];
+    deg = a.off[v + 1] - a.off[v];
+    const T b = bound[v];
+    if (flush ? b != T(0) : (double)fabs_t(b) > eps) {
+      bound[v] = T(0);
+      kahan_add(rank, comp, v, b);
+      // the push skips an exit's edges inside its cluster; the pull takes them back
+      if (deg) { want_push = true; cval = d * b / (T)deg; }
+    }
+    if (pull && want_push) sk_contrib(a, contrib, v, cval);
+  }
+  push_item<T>(want_push && !pull, v, cval, deg, a.wl_v, wl_c, a.heavy, a.cnt);
+  sk_block_edges<T>(want_push ? deg : 0u, reinterpret_cast<unsigned long long*>(a.partial) + part_base + blockIdx.x);
+}
+
+// The exits of the current layout: the first cl_nexit[k] member slots of every
+// cluster k (a thread per cluster: most clusters have one exit or a few).
+__global__ void __launch_bounds__(kBlock) k_exit_list(uint32_t K, const uint32_t* __restrict__ cl_off,
+                                                      const uint32_t* __restrict__ cl_mem,
+                                                      const uint32_t* __restrict__ cl_nexit, uint32_t* __restrict__ list,
+                                                      unsigned long long* __restrict__ count) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t nx = k < K ? cl_nexit[k] : 0u;
+  const unsigned long long at = nx ? atomicAdd(count, (unsigned long long)nx) : 0ull;
+  const uint32_t base = nx ? cl_off[k] : 0u;
+  for (uint32_t j = 0; j < nx; ++j) list[at + j] = cl_mem[base + j];
 }
 
 // Shortcut items: bound[x] += R[e][x] * delta for the exits x of the entry's
@@ -1540,20 +1690,27 @@ __global__ void k_lists_clear(Counters* cnt) {
 // Precompute of one dirty cluster per workgroup: the members' pending delta is
 // iterated inside the cluster (run_to_convergence_for_precpt), the mass that
 // arrived becomes rank, the residual stays in delta, and every exit sends d/outdeg
-// of its arrived mass over its external edges (first_step, :389-434).
+// of its arrived mass over its external edges (first_step, :389-434): the exit's
+// arrived mass is left in bound, so the first sketch round's exit pass turns it
+// into rank and sends it with that round's scatter.  Dissolved clusters are
+// Singles and are not iterated.
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_sk_precompute(SkArgs a, const uint32_t* __restrict__ cl_off,
                                                           const uint32_t* __restrict__ cl_mem,
                                                           const uint32_t* __restrict__ cl_nexit, Internal ie,
-                                                          uint8_t* dirty,
+                                                          uint8_t* dirty, const uint8_t* __restrict__ dead,
                                                           T* __restrict__ delta, T* __restrict__ rank,
-                                                          float* __restrict__ comp, T* __restrict__ wl_c, T d,
+                                                          float* __restrict__ comp, T* __restrict__ bound, T d,
                                                           double thr_per_vertex, int maxit,
                                                           unsigned long long* work) {
   __shared__ T s_r[kMaxCluster], s_d[kMaxCluster], s_n[kMaxCluster];
   __shared__ T s_red[kBlock / 64];
   const uint32_t k = blockIdx.x;
   if (!dirty[k]) return;
+  if (dead[k]) {
+    if (threadIdx.x == 0) dirty[k] = 0;
+    return;
+  }
   const uint32_t slot0 = cl_off[k], size = cl_off[k + 1] - slot0, nexit = cl_nexit[k];
   for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
     uint32_t v = cl_mem[slot0 + j];
@@ -1563,22 +1720,13 @@ __global__ void __launch_bounds__(kBlock) k_sk_precompute(SkArgs a, const uint32
   __syncthreads();
   const uint32_t pb = ie.off[k], pe = ie.off[k + 1];
   uint32_t rounds = cluster_iterate<T>(size, pb, pe, ie, d, thr_per_vertex * size, maxit, s_r, s_d, s_n, s_red);
-  for (uint32_t j0 = 0; j0 < size; j0 += kBlock) {  // uniform trip count: push_item uses wave ballots
-    const uint32_t j = j0 + threadIdx.x;
-    bool want = false;
-    uint32_t v = 0, rowlen = 0;
-    T cval = T(0);
-    if (j < size) {
-      v = cl_mem[slot0 + j];
-      delta[v] = s_d[j];
-      if (s_r[j] != T(0)) kahan_add(rank, comp, v, s_r[j]);
-      if (j < nexit && s_r[j] != T(0)) {
-        const uint32_t deg = a.off[v + 1] - a.off[v];
-        rowlen = deg;
-        if (deg) { want = true; cval = d * s_r[j] / (T)deg; }
-      }
+  for (uint32_t j = threadIdx.x; j < size; j += kBlock) {
+    const uint32_t v = cl_mem[slot0 + j];
+    delta[v] = s_d[j];
+    if (s_r[j] != T(0)) {
+      if (j < nexit) bound[v] += s_r[j];  // sent by the next exit pass
+      else kahan_add(rank, comp, v, s_r[j]);
     }
-    push_item<T>(want, v, cval, rowlen, a.wl_v, wl_c, a.heavy, a.cnt);
   }
   if (threadIdx.x == 0) {
     dirty[k] = 0;
@@ -1632,17 +1780,50 @@ __global__ void __launch_bounds__(kBlock) k_sk_correct(const uint32_t* __restric
 // (local index < nexit) to v.  Only such in-edges carry a contribution the
 // sketch pull must skip; every other vertex sums all its in-edges unchecked.
 // chk is all zero on entry (the caller clears it).
-__global__ void __launch_bounds__(kBlock) k_pull_marks(uint32_t K, const uint32_t* __restrict__ ie_off,
-                                                       const uint32_t* __restrict__ ie_src,
-                                                       const uint32_t* __restrict__ ie_dst,
-                                                       const uint32_t* __restrict__ cl_off,
-                                                       const uint32_t* __restrict__ cl_mem,
-                                                       const uint32_t* __restrict__ cl_nexit, uint8_t* __restrict__ chk) {
-  const uint32_t k = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);  // one wave per cluster
+// The internal edges leaving an exit of the current layout, as (contribution slot
+// of the exit, destination): a thread per cluster.
+__global__ void __launch_bounds__(kBlock) k_pull_internal(uint32_t K, const uint32_t* __restrict__ ie_off,
+                                                          const uint32_t* __restrict__ ie_src,
+                                                          const uint32_t* __restrict__ ie_dst,
+                                                          const uint32_t* __restrict__ cl_off,
+                                                          const uint32_t* __restrict__ cl_mem,
+                                                          const uint32_t* __restrict__ cl_nexit,
+                                                          const uint32_t* __restrict__ pos, uint2* __restrict__ list,
+                                                          unsigned long long* __restrict__ count) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= K) return;
-  const uint32_t nx = cl_nexit[k], base = cl_off[k];
-  for (uint32_t i = ie_off[k] + lane_id(); i < ie_off[k + 1]; i += 64)
-    if (ie_src[i] < nx) chk[cl_mem[base + ie_dst[i]]] = 1;
+  const uint32_t nx = cl_nexit[k];
+  if (!nx) return;
+  const uint32_t base = cl_off[k], b = ie_off[k], e = ie_off[k + 1];
+  uint32_t take = 0;
+  for (uint32_t i = b; i < e; ++i) take += ie_src[i] < nx;
+  if (!take) return;
+  unsigned long long at = atomicAdd(count, (unsigned long long)take);
+  for (uint32_t i = b; i < e; ++i) {
+    if (ie_src[i] >= nx) continue;
+    uint32_t su = cl_mem[base + ie_src[i]];
+    if (pos) su = pos[su];
+    list[at++] = make_uint2(su, cl_mem[base + ie_dst[i]]);
+  }
+}
+
+// After a sketch round's pull: the pull gathered every in-edge, so the mass an
+// exit sent along its internal edges (carried by the shortcut rows instead) is
+// taken back.  list holds (contribution slot of the exit, destination) per
+// internal edge leaving an exit.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pull_unskip(const uint2* __restrict__ list,
+                                                        const unsigned long long* __restrict__ count,
+                                                        const T* __restrict__ contrib, T* __restrict__ delta,
+                                                        const Counters* cnt) {
+  if (cnt->done || cnt->mode == 0) return;
+  const unsigned long long L = *count;
+  for (unsigned long long i = (unsigned long long)blockIdx.x * kBlock + threadIdx.x; i < L;
+       i += (unsigned long long)gridDim.x * kBlock) {
+    const uint2 ed = list[i];
+    const T x = contrib[ed.x];
+    if (x != T(0)) unsafeAtomicAdd(&delta[ed.y], -x);
+  }
 }
 
 template <class T>
@@ -1672,16 +1853,22 @@ static void sum_pr_run_t(suminc_ctx* c, suminc_stats* st) {
   SUMINC_HIP(hipMemsetAsync(s.bound.p, 0, (size_t)n * es, c->stream));
   SUMINC_HIP(hipMemsetAsync(s.spnode.p, 0, (size_t)n * es, c->stream));
   c->tmp3.ensure((size_t)n * es + 8);
-  if (div > 0) {
-    s.pull_chk.ensure((size_t)n);
-    SUMINC_HIP(hipMemsetAsync(s.pull_chk.p, 0, (size_t)n, c->stream));
-    if (s.K)
-      k_pull_marks<<<grid_for(s.K, kBlock / 64), kBlock, 0, c->stream>>>(
-          s.K, s.ie_off.as<uint32_t>(), s.ie_src.as<uint32_t>(), s.ie_dst.as<uint32_t>(), s.cl_off.as<uint32_t>(),
-          s.cl_mem.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), s.pull_chk.as<uint8_t>());
-  }
+  // the sketch rounds pull like the plain rounds (contributions at the plain pull's
+  // compact positions when it uses them), then take back what exits sent along
+  // their internal edges: the list of those edges for this layout
   SkArgs a;
   a.n = n;
+  a.pos = div > 0 && c->pull_compact ? c->src_pos.as<uint32_t>() : nullptr;
+  if (div > 0) {
+    s.xi_list.ensure((s.ie_m + 1) * sizeof(uint2));
+    s.xi_count.ensure(8);
+    SUMINC_HIP(hipMemsetAsync(s.xi_count.p, 0, 8, c->stream));
+    if (s.K)
+      k_pull_internal<<<grid_for(s.K), kBlock, 0, c->stream>>>(
+          s.K, s.ie_off.as<uint32_t>(), s.ie_src.as<uint32_t>(), s.ie_dst.as<uint32_t>(), s.cl_off.as<uint32_t>(),
+          s.cl_mem.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), a.pos, s.xi_list.as<uint2>(),
+          s.xi_count.as<unsigned long long>());
+  }
   a.off = c->out_off.as<uint32_t>();
   a.ntype = s.ntype.as<uint8_t>();
   a.ent_of = s.ent_of.as<uint32_t>();
@@ -1701,29 +1888,52 @@ static void sum_pr_run_t(suminc_ctx* c, suminc_stats* st) {
   if (s.K)
     k_sk_precompute<T><<<s.K, kBlock, 0, c->stream>>>(a, s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(),
                                                       s.cl_nexit.as<uint32_t>(), internal_of(s),
-                                                      s.dirty.as<uint8_t>(), c->pr_delta.as<T>(), c->pr_rank.as<T>(),
-                                                      c->pr_comp.as<float>(), c->wl1.as<T>(), d, eps, maxit,
+                                                      s.dirty.as<uint8_t>(), s.dead.as<uint8_t>(),
+                                                      c->pr_delta.as<T>(), c->pr_rank.as<T>(),
+                                                      c->pr_comp.as<float>(), s.bound.as<T>(), d, eps, maxit,
                                                       work.as<unsigned long long>());
-  sk_push_pass<T>(c);
   k_lists_clear<<<1, 64, 0, c->stream>>>(cnt);
   SUMINC_HIP(hipGetLastError());
+  // the exits of this layout, for the rounds' exit pass
+  s.exits.ensure((size_t)s.M * 4 + 4);
+  s.exit_count.ensure(8);
+  SUMINC_HIP(hipMemsetAsync(s.exit_count.p, 0, 8, c->stream));
+  if (s.K)
+    k_exit_list<<<grid_for(s.K), kBlock, 0, c->stream>>>(
+        s.K, s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.cl_nexit.as<uint32_t>(), s.exits.as<uint32_t>(),
+        s.exit_count.as<unsigned long long>());
+  check_begin(c, s.exit_count.p, 8);
+  const uint64_t n_exits = *reinterpret_cast<const unsigned long long*>(check_wait(c));
   // 2. sketch rounds
-  const unsigned g = grid_for(n);
+  const unsigned g4 = grid_for(n, kBlock * kSkPer);  // the vertex passes: kSkPer vertices a thread
+  const unsigned ge = grid_for(n_exits);             // the exit pass
+  c->tmp4.ensure(65536);  // per-round scatter mode (profile attribution)
   int group = 4;
+  size_t slot = 0;
   while (true) {
-    for (int i = 0; i < group; ++i) {
-      k_sk_count<T><<<g, kBlock, 0, c->stream>>>(a, c->pr_delta.as<T>(), s.bound.as<T>(), eps);
-      k_sk_decide<<<1, 1024, 0, c->stream>>>(a.partial, g, cnt, T_stop, max_rounds, c->m, div);
-      k_sk_gather<T><<<g, kBlock, 0, c->stream>>>(a, c->pr_delta.as<T>(), s.bound.as<T>(), s.spnode.as<T>(),
-                                                  c->pr_rank.as<T>(), c->pr_comp.as<float>(), c->wl1.as<T>(),
-                                                  c->tmp3.as<T>(), c->pr_contrib.as<T>(), eps, d, 0);
-      sk_push_pass<T>(c);
-      if (div > 0) pull_scatter<T>(c, s.cid.as<uint32_t>(), s.pull_chk.as<uint8_t>(), 0);
+    for (int i = 0; i < group; ++i, ++slot) {
+      k_sk_count<T><<<g4, kBlock, 0, c->stream>>>(a, c->pr_delta.as<T>(), s.bound.as<T>(), eps);
+      k_sk_decide<<<1, 1024, 0, c->stream>>>(a.partial, g4, cnt, T_stop, max_rounds, c->m, div,
+                                             c->tmp4.as<uint8_t>());
+      k_sk_send<T><<<g4, kBlock, 0, c->stream>>>(a, c->pr_delta.as<T>(), s.spnode.as<T>(), c->pr_rank.as<T>(),
+                                                 c->pr_comp.as<float>(), c->wl1.as<T>(), c->tmp3.as<T>(),
+                                                 c->pr_contrib.as<T>(), eps, d);
       k_sk_shortcut<T><<<grid_for(s.E, kBlock, push_grid()), kBlock, 0, c->stream>>>(
           s.E, c->tmp3.as<T>(), cnt, s.cl_ent.as<uint32_t>(), s.cid.as<uint32_t>(),
           s.cl_off.as<uint32_t>(), s.cl_mem.as<uint32_t>(), s.cl_nexit.as<uint32_t>(),
           s.sc_base.as<unsigned long long>(), s.sc_R.as<T>(), s.bound.as<T>());
-      k_sk_end<<<1, 1024, 0, c->stream>>>(cnt, reinterpret_cast<unsigned long long*>(a.partial) + g, g);
+      k_sk_exits<T><<<ge, kBlock, 0, c->stream>>>(a, s.exits.as<uint32_t>(), s.exit_count.as<unsigned long long>(),
+                                                  s.bound.as<T>(), c->pr_rank.as<T>(), c->pr_comp.as<float>(),
+                                                  c->wl1.as<T>(), c->pr_contrib.as<T>(), eps, d, 0, 2 * g4);
+      sk_push_pass<T>(c);
+      if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 2 * slot), c->stream));
+      if (div > 0) pull_scatter<T>(c, 0);
+      if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, 2 * slot + 1), c->stream));
+      if (div > 0)
+        k_pull_unskip<T><<<grid_for(s.ie_m + 1, kBlock, 4096), kBlock, 0, c->stream>>>(
+            s.xi_list.as<uint2>(), s.xi_count.as<unsigned long long>(), c->pr_contrib.as<T>(), c->pr_delta.as<T>(),
+            cnt);
+      k_sk_end<<<1, 1024, 0, c->stream>>>(cnt, reinterpret_cast<unsigned long long*>(a.partial) + g4, g4 + ge);
     }
     SUMINC_HIP(hipGetLastError());
     read_counters(c);
@@ -1732,12 +1942,30 @@ static void sum_pr_run_t(suminc_ctx* c, suminc_stats* st) {
   }
   const Counters* h = c->host_counters;
   out.sketch_rounds = (double)h->rounds;  // sketch rounds
+  if (c->prof_on && h->rounds) {
+    // the sketch rounds' pull launches (the plain rounds' kernel): device time of
+    // the rounds that pulled, and their algorithmic bytes (row index + contribution
+    // per in-edge; row pointers and the delta read-modify-write per vertex)
+    std::vector<uint8_t> mode(std::min<size_t>((size_t)h->rounds, 65536));
+    SUMINC_HIP(hipMemcpy(mode.data(), c->tmp4.p, mode.size(), hipMemcpyDeviceToHost));
+    suminc_kernel_profile& q = c->prof[SUMINC_KERNEL_PR_PULL];
+    for (size_t i = 0; i < mode.size(); ++i) {
+      if (!mode[i]) continue;
+      float t = 0;
+      SUMINC_HIP(hipEventElapsedTime(&t, c->prof_events[2 * i], c->prof_events[2 * i + 1]));
+      q.launches += 1;
+      q.ms += t;
+      q.edges += (double)c->m;
+      q.vertices += (double)n;
+      q.alg_bytes += (double)c->m * (4.0 + es) + (double)n * (8.0 + 2.0 * es);
+    }
+  }
   SUMINC_REQUIRE(!h->aux[0], SUMINC_E_NUMERIC, "NaN/Inf in the sketch residual");
   // 3. flush the exits' pending inflow, 4. correction
   k_lists_clear<<<1, 64, 0, c->stream>>>(cnt);
-  k_sk_gather<T><<<g, kBlock, 0, c->stream>>>(a, c->pr_delta.as<T>(), s.bound.as<T>(), s.spnode.as<T>(),
-                                              c->pr_rank.as<T>(), c->pr_comp.as<float>(), c->wl1.as<T>(),
-                                              c->tmp3.as<T>(), c->pr_contrib.as<T>(), eps, d, 1);
+  k_sk_exits<T><<<ge, kBlock, 0, c->stream>>>(a, s.exits.as<uint32_t>(), s.exit_count.as<unsigned long long>(),
+                                              s.bound.as<T>(), c->pr_rank.as<T>(), c->pr_comp.as<float>(),
+                                              c->wl1.as<T>(), c->pr_contrib.as<T>(), eps, d, 1, 2 * g4);
   sk_push_pass<T>(c);
   k_lists_clear<<<1, 64, 0, c->stream>>>(cnt);
   if (s.K)
@@ -1797,23 +2025,54 @@ void sketch_info(suminc_ctx* c, suminc_sketch_info* out) {
     s.counts[t] = hist[t];
     typed += hist[t];
   }
-  SUMINC_REQUIRE(typed == s.M, SUMINC_E_STATE, "supernode member count changed");
-  s.counts[0] = c->n - s.M;
-  out->clusters = s.K;
+  // members of dissolved clusters are Singles (hist[0] over the members)
+  SUMINC_REQUIRE(typed + hist[0] == s.M, SUMINC_E_STATE, "supernode member count changed");
+  s.counts[0] = c->n - typed;
+  // dissolved clusters and the internal edges of the live ones
+  std::vector<uint8_t> dead(s.K);
+  std::vector<uint32_t> ieo((size_t)s.K + 1);
+  if (s.K) {
+    SUMINC_HIP(hipMemcpy(dead.data(), s.dead.p, s.K, hipMemcpyDeviceToHost));
+    SUMINC_HIP(hipMemcpy(ieo.data(), s.ie_off.p, ((size_t)s.K + 1) * 4, hipMemcpyDeviceToHost));
+  }
+  uint64_t n_dead = 0, live_ie = 0;
+  for (uint32_t k = 0; k < s.K; ++k) {
+    if (dead[k]) ++n_dead;
+    else live_ie += ieo[k + 1] - ieo[k];
+  }
+  out->clusters = s.K - n_dead;
+  out->dissolved_clusters = n_dead;
   out->raw_clusters = s.Kraw;
   out->entries = s.E;
-  out->clustered_vertices = s.M;
+  out->clustered_vertices = typed;
   for (int t = 0; t < 5; ++t) out->node_type_count[t] = s.counts[t];
-  out->sketch_edges = c->m - s.ie_m;  // Singles' CSR rows + the exits' external edges
-  out->internal_edges = s.ie_m;
+  out->sketch_edges = c->m - live_ie;  // Singles' CSR rows + the exits' external edges
+  out->internal_edges = live_ie;
   out->shortcut_slots = s.sc_len;
   out->max_cluster = s.max_cluster;
+}
+
+// cluster of every vertex, SUMINC_INF_U32 for Singles and members of dissolved clusters
+__global__ void k_live_cid(uint32_t n, const uint32_t* __restrict__ cid, const uint8_t* __restrict__ dead,
+                           uint32_t* __restrict__ out) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const uint32_t k = cid[v];
+  out[v] = k != INF32 && !dead[k] ? k : INF32;
 }
 
 void sketch_download(suminc_ctx* c, uint32_t* cluster_of, uint8_t* node_type) {
   sketch_refresh(c);
   Sketch& s = c->sk;
-  if (cluster_of) SUMINC_HIP(hipMemcpy(cluster_of, s.cid.p, (size_t)c->n * 4, hipMemcpyDeviceToHost));
+  if (cluster_of) {
+    DevBuf live;
+    live.ensure((size_t)c->n * 4 + 4);
+    k_live_cid<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, s.cid.as<uint32_t>(), s.dead.as<uint8_t>(),
+                                                        live.as<uint32_t>());
+    SUMINC_HIP(hipGetLastError());
+    SUMINC_HIP(hipMemcpyAsync(cluster_of, live.p, (size_t)c->n * 4, hipMemcpyDeviceToHost, c->stream));
+    SUMINC_HIP(hipStreamSynchronize(c->stream));
+  }
   if (node_type) SUMINC_HIP(hipMemcpy(node_type, s.ntype.p, c->n, hipMemcpyDeviceToHost));
 }
 

@@ -25,8 +25,13 @@ def test_workloads_cover_the_configurations():
     for name, w in bench.WORKLOADS.items():
         assert w["scale"] in (16, 22, 24, 26), name
         assert w["desc"], name
-    # the default workload is the incremental SSSP over supernodes at scale 22
     assert bench.WORKLOADS["sssp22"]["frac"] == 0.001 and bench.WORKLOADS["sssp22"]["max_node"] == 1024
+    # the default workload is the metric's large configuration: s26, seed 5, 1% batches,
+    # supernodes, SSSP weights U[1,255]
+    w = bench.WORKLOADS["web26"]
+    assert (w["scale"], w["seed"], w["frac"], w["max_node"], w["wmax"]) == (26, 5, 0.01, 1024, 255)
+    src = (ROOT / "bench.py").read_text()
+    assert 'ap.add_argument("--workload", default="web26"' in src
 
 
 def test_traffic_summaries_are_found():

@@ -73,15 +73,16 @@ def test_shortcut_rows_equal_dense_solve(gpu, prec):
             assert np.all(np.isin(typ[mem[:nexit]], [1, 3])) and not np.any(np.isin(typ[mem[nexit:]], [1, 3]))
 
 
+@pytest.mark.parametrize("compact", [0, 1])
 @pytest.mark.parametrize("kind", ["leaf", "blocks"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_sum_pr_query_equals_fixed_point(gpu, kind, prec):
+def test_sum_pr_query_equals_fixed_point(gpu, kind, prec, compact):
     g = oracle.Graph.rmat(14, 16, seed=1)
     off, col, _ = g.csr()
     rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
     T = 1e-10 * g.n
     p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
-    with gpu.Context(pr_precision=p, termcheck_threshold=T) as ctx:
+    with gpu.Context(pr_precision=p, termcheck_threshold=T, pull_compact_kib=compact) as ctx:
         ctx.upload_csr(off, col)
         if kind == "leaf":
             ctx.cluster_leaves(1024, 2)
@@ -97,16 +98,17 @@ def test_sum_pr_query_equals_fixed_point(gpu, kind, prec):
     assert st["sketch_rounds"] > 0  # sketch rounds ran
 
 
-# compact=1: the residual phase pulls over compact source positions in 16 KiB
-# source bands (the default only on large graphs) while the sketch rounds pull by
-# vertex id from the same contribution buffer
-@pytest.mark.parametrize("prec,compact", [("fp64", 0), ("fp32", 0), ("fp32", 1)])
+# compact=1: both pulls gather from compact source positions (the default only on
+# large graphs), the residual phase in 16 KiB source bands over positions in id
+# order; compact=2: positions by decreasing out-degree, unbanded (the large-graph
+# default), so the sketch rounds' skip test reads the cluster by position
+@pytest.mark.parametrize("prec,compact", [("fp64", 0), ("fp32", 0), ("fp32", 1), ("fp32", 2), ("fp64", 2)])
 def test_sum_pr_incremental(gpu, prec, compact):
     g = oracle.Graph.rmat(13, 16, seed=3)
     off, col, _ = g.csr()
     T = 1e-10 * g.n
     p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
-    kw = {"pull_compact_kib": 1, "pull_band_kib": 16} if compact else {}
+    kw = {1: {"pull_compact_kib": 1, "pull_band_kib": 16}, 2: {"pull_compact_kib": 1}}.get(compact, {})
     with gpu.Context(pr_precision=p, termcheck_threshold=T, **kw) as ctx:
         ctx.upload_csr(off, col)
         ctx.set_clusters(blocks(g.n, 4096, 8))
@@ -299,3 +301,60 @@ def test_rows_follow_small_batches(gpu):
                 omem, x = shortcut_dense(g, cid, int(s), DAMPING)
                 order = np.argsort(mem)
                 assert np.abs(reach[order] - x).max() < 1e-6, (b, int(s))
+
+
+def _canonical(cid):
+    """Cluster labels replaced by the smallest member id (INF stays): partitions
+    compare equal whatever their numbering."""
+    out = np.full(len(cid), INF, np.uint64)
+    m = cid != INF
+    if m.any():
+        ids = np.nonzero(m)[0]
+        first = {}
+        for v, k in zip(ids, cid[m]):
+            first.setdefault(int(k), int(v))
+        out[m] = [first[int(k)] for k in cid[m]]
+    return out
+
+
+def test_degraded_clusters_dissolve(gpu):
+    """Batches that give a cluster so many entries and exits that its benefit falls
+    to the build's threshold dissolve it (its members become Singles and stay so).
+    The expected structure after every batch: the oracle's acceptance on the
+    current graph over the clusters that survived every earlier batch.  The SumInc
+    PageRank and SSSP queries stay equal to the fixed point and to Dijkstra."""
+    g = oracle.Graph.rmat(13, 16, seed=6, weights="u32", wmin=1, wmax=64)
+    off, col, w = g.csr()
+    raw = g.leaf_clusters(1024, 2)
+    with gpu.Context(pr_precision=gpu.PR_FP64, termcheck_threshold=1e-10 * g.n) as ctx:
+        ctx.upload_csr(off, col, w)
+        ctx.set_clusters(raw)
+        info0 = ctx.sum_build(2, 1024, 1.0)
+        assert info0["dissolved_clusters"] == 0
+        ctx.pr_init()
+        ctx.sum_pr_run()
+        ctx.sssp_init(0)
+        ctx.sum_sssp_run()
+        ocid0, _ = g.sum_structure(raw, 2, 1024, 1.0)
+        alive = np.where(ocid0 != INF, raw, INF).astype(np.uint32)  # accepted by the build
+        for b in range(4):
+            n_ops = int(0.05 * g.m)
+            ops = g.gen_batch(seed=6, batch_id=b, n_del=n_ops // 2, n_ins=n_ops - n_ops // 2, wmin=1, wmax=64)
+            ctx.apply_batch(ops)
+            g, _ = g.apply_batch(ops)
+            st = ctx.sum_pr_run()
+            ctx.sum_sssp_run()
+            cid, typ = ctx.sum_download()
+            info = ctx.sum_info()
+            ocid, otyp = g.sum_structure(alive, 2, 1024, 1.0)
+            assert np.array_equal(_canonical(cid), _canonical(ocid)), b
+            assert np.array_equal(typ, otyp), b
+            alive = np.where(ocid != INF, raw, INF).astype(np.uint32)  # the survivors, by raw id
+            r, _ = ctx.pr_download()
+            rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
+            assert np.abs(r - rs).sum() <= st["residual"] / (1 - DAMPING) + 1e-11
+            dist, par = ctx.sssp_download()
+            od, op = g.dijkstra(0)
+            assert np.array_equal(dist, od) and np.array_equal(par, op)
+        assert info["dissolved_clusters"] > 0
+        assert info["clusters"] + info["dissolved_clusters"] == info0["clusters"]
