@@ -210,6 +210,13 @@ int suminc_sssp_init(suminc_ctx* ctx, uint32_t source);
  * path uses unit weights, my_ssspworker.cu:390). */
 int suminc_sssp_run(suminc_ctx* ctx, suminc_stats* stats);
 int suminc_sssp_download(suminc_ctx* ctx, uint32_t* dist, uint32_t* parent);
+/* One min-plus round from the pending frontier (the source after
+ * suminc_sssp_init, the seeds of a batch, or the previous step's output), for a
+ * worker that keeps its own loop: replaces one tjnsssp::g_function call with
+ * setCurSize/getAllSize (sum_sync_traversal_worker.h:1090, 1136-1142).
+ * *frontier_out gets the size of the frontier the round produced, which becomes
+ * the pending one (0: fixed point). */
+int suminc_sssp_step(suminc_ctx* ctx, uint32_t* frontier_out);
 
 /* ---- supernode sketch ------------------------------------------------------------
  * Clustering input, acceptance, node types, shortcut precompute and the SumInc
@@ -275,6 +282,131 @@ int suminc_sum_sssp_shortcut(suminc_ctx* ctx, uint32_t entry, uint32_t cap, uint
  * SumSyncTraversalWorker::Query (grape/worker/sum_sync_traversal_worker.h:1160-1233)
  * and its correction (:1322-1406). */
 int suminc_sum_sssp_run(suminc_ctx* ctx, suminc_stats* stats);
+
+/* ---- the reference compressor's sketch, driven phase by phase -------------------
+ * For a worker that keeps the reference's compressor (host) and its phase machine
+ * (batch -> compressed rounds -> correction -> Ingress rounds -> update,
+ * sum_sync_iter_worker.h:1372-1694): the compressor's products cross the ABI as
+ * they are (compressor_base.h:4636-4899, 5974-5987; iter_compressor.h:433-491),
+ * mirrors included.  Ids n .. all_node_num-1 are mirrors.  Row pointers have
+ * rows + 1 entries; every pointer is host memory, copied during the call.  After
+ * a batch changes the graph the sketch must be uploaded again (the reference
+ * re-runs sketch2csr_mirror after inc_run). */
+typedef struct suminc_sketch_csr {
+  uint32_t n;                    /* vertices: must equal the uploaded graph's */
+  uint32_t all_node_num;         /* n + mirrors */
+  const uint8_t* node_type;      /* n: NodeType 0..6 (get_nodetype_mirror, compressor_base.h:4144-4182) */
+  const uint32_t* mirror_master; /* all_node_num - n: the vertex each mirror copies (mirrorid2vid) */
+  const uint64_t* is_off;        /* all_node_num rows: is_e_, a supernode source's bound_delta */
+  const uint32_t* is_col;        /*   (exits and out-mirrors of its cluster) */
+  const float* is_w;
+  const uint64_t* ib_off;        /* n rows: ib_e_, a Single's out-edges, an exit's external ones */
+  const uint32_t* ib_col;
+  const uint64_t* sync_off;      /* all_node_num rows: sync_e_, master -> its in-mirrors, */
+  const uint32_t* sync_col;      /*   out-mirror -> its master */
+  const uint64_t* iv_off;        /* all_node_num rows: inner_value of the supernode a vertex or */
+  const uint32_t* iv_col;        /*   in-mirror is the source of (SuperNodeForIter, super_node.h:8-36) */
+  const float* iv_w;
+  const uint64_t* id_off;        /* all_node_num rows: its inner_delta */
+  const uint32_t* id_col;
+  const float* id_w;
+  const uint64_t* sub_off;       /* all_node_num rows, optional (NULL: no precompute): the cluster */
+  const uint32_t* sub_col;       /*   subgraph (build_subgraph_mirror, compressor_base.h:3786-3856): a
+                                  *   member's out-edges inside its cluster plus the out-mirror ids of the
+                                  *   edges its cluster's out-mirrors carry; an in-mirror's master's edges
+                                  *   into the cluster; empty for Singles and out-mirrors */
+} suminc_sketch_csr;
+/* Replaces the worker's flattening of is_e_/ib_e_/sync_e_ and node types and their
+ * upload with tjn::init (sum_sync_iter_worker.h:573-784).  The sketch accumulators
+ * (bound_node_values, spnode_datas) start at zero and persist across
+ * suminc_pr_step_phase calls until suminc_pr_correct, a new PageRank state
+ * (suminc_pr_init / suminc_pr_upload) or another upload. */
+int suminc_upload_sketch(suminc_ctx* ctx, const suminc_sketch_csr* sketch);
+#define SUMINC_PHASE_PLAIN 0  /* an Ingress round over the full CSR (suminc_pr_step) */
+#define SUMINC_PHASE_SKETCH 1 /* a round over the uploaded sketch by node type, then the out-mirror sync */
+/* One round of the given phase; residual_out = the FP64 residual before the round
+ * (plain: sum |delta|; sketch: sum |delta| + sum |bound_node_values|).  The sketch
+ * round replaces tjn::g_function_compr + tjn::OutMirrorSyncToMaster + deltaSum
+ * (sum_sync_iter_worker.h:932-933, 1129-1142; cudaCode/my_worker.cu:235-293,
+ * 331-437). */
+int suminc_pr_step_phase(suminc_ctx* ctx, int phase, double* residual_out);
+/* The precompute that opens the compressed phase (sum_sync_iter_worker.h:387-435;
+ * iter_compressor.h:534-560, 714-765): every cluster member's pending delta runs
+ * to convergence inside its cluster over the subgraph rows (Jacobi rounds of all
+ * clusters at once, until the mass moved in a round is at most termcheck_threshold
+ * / n per member or shortcut_max_rounds; what is left stays in delta), taken into
+ * rank; the out-mirrors pass what they collected to their masters' delta; then
+ * every exit sends the mass that passed through it along its ib_e_ row and, as a
+ * master, into its in-mirrors' rows (spnode_datas, bound_node_values).  Needs the
+ * subgraph rows (suminc_compress_mirror gives them; suminc_upload_sketch takes them
+ * in sub_off / sub_col). */
+int suminc_pr_precompute(suminc_ctx* ctx);
+/* The correction at the end of the compressed phase (sum_sync_iter_worker.h:
+ * 1385-1491) on the device: the exits' pending inflow is flushed first, in exit
+ * passes until sum |bound| <= termcheck_threshold / 1000 (at most 256; the rest is
+ * dropped, where the reference leaves all of it in bound_node_values), then every
+ * non-zero spnode_datas entry is spread through its supernode's inner_value (rank)
+ * and inner_delta (delta). */
+int suminc_pr_correct(suminc_ctx* ctx);
+/* The whole query over the uploaded sketch: the precompute (when the sketch has
+ * subgraph rows), sketch rounds until sum |delta| + sum |bound| <
+ * termcheck_threshold, correction, plain Ingress rounds. */
+int suminc_sketch_pr_run(suminc_ctx* ctx, suminc_stats* stats);
+
+/* The compressor with mirror vertices, from a cluster assignment (cluster_of[v]:
+ * cluster id < n or SUMINC_INF_U32; clusters of min_node_num .. max_node_num members
+ * taken in id order), on the host as in the reference, then the shortcut rows on
+ * the device (one workgroup per supernode source, no cluster-size cap), uploaded as
+ * the context's sketch (suminc_upload_sketch's layout).  Restates
+ * get_init_supernode_by_clusters (mirror selection by mirror_k: an outside vertex
+ * with more than mirror_k edges into (out of) a cluster becomes an in-mirror
+ * (out-mirror) of it; the best of the four benefits above compress_threshold
+ * accepts; compressor_base.h:3253-3611), final_build_supernode (:3616-3783),
+ * build_subgraph_mirror (:3786-3856), judge_out_bound_node (:3924-4007),
+ * get_nodetype_mirror (:4144-4182), sketch2csr_mirror (:4636-4899) and
+ * build_iter_index_mirror (iter_compressor.h:318-491).  mirror_k = 1e8 turns mirrors
+ * off (the reference's switch, compressor_base.h:3302; its default is 4,
+ * examples/analytical_apps/flags.cc:62).  Uses a PageRank state: call after
+ * suminc_pr_init. */
+typedef struct suminc_compress_info {
+  uint64_t clusters;            /* accepted clusters */
+  uint64_t all_node_num;        /* vertices + mirrors */
+  uint64_t supernodes;          /* supernode sources: entries and in-mirrors */
+  uint64_t in_mirrors, out_mirrors;
+  uint64_t absorbed;            /* unclustered mirror candidates taken into the cluster instead */
+  uint64_t node_type_count[7];  /* NodeType 0..6 over the vertices */
+  uint64_t is_edges, ib_edges, sync_edges, subgraph_edges;
+  uint64_t inner_value_entries, inner_delta_entries;
+  uint64_t shortcut_work;       /* internal edges read by the shortcut build */
+  uint64_t reserved[8];
+} suminc_compress_info;
+int suminc_compress_mirror(suminc_ctx* ctx, const uint32_t* cluster_of, uint32_t min_node_num, uint32_t max_node_num,
+                           double compress_threshold, double mirror_k, suminc_compress_info* info);
+/* The last suminc_compress_mirror's products in suminc_sketch_csr's layout, into
+ * caller buffers sized from its suminc_compress_info (all_node_num, is_edges, ...;
+ * row pointers rows + 1 entries), plus cluster_of (all_node_num: accepted cluster
+ * of every vertex and mirror, SUMINC_INF_U32 for Singles).  Any pointer may be NULL. */
+typedef struct suminc_sketch_buffers {
+  uint8_t* node_type;
+  uint32_t* mirror_master;
+  uint32_t* cluster_of;
+  uint64_t* is_off;
+  uint32_t* is_col;
+  float* is_w;
+  uint64_t* ib_off;
+  uint32_t* ib_col;
+  uint64_t* sync_off;
+  uint32_t* sync_col;
+  uint64_t* iv_off;
+  uint32_t* iv_col;
+  float* iv_w;
+  uint64_t* id_off;
+  uint32_t* id_col;
+  float* id_w;
+  uint64_t* sub_off;            /* all_node_num + 1 */
+  uint32_t* sub_col;            /* subgraph_edges */
+} suminc_sketch_buffers;
+int suminc_sketch_download(suminc_ctx* ctx, suminc_sketch_buffers* buffers);
 
 /* ---- weighted personalized PageRank and its adjoint ------------------------------
  * No counterpart in the reference (its nearest app is the weighted personalized

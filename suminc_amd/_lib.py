@@ -105,6 +105,57 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class SketchCsr(C.Structure):
+    """suminc_sketch_csr: the reference compressor's products (pointers into numpy
+    arrays the caller keeps alive for the call)."""
+    _fields_ = [
+        ("n", C.c_uint32),
+        ("all_node_num", C.c_uint32),
+        ("node_type", C.c_void_p),
+        ("mirror_master", C.c_void_p),
+        ("is_off", C.c_void_p), ("is_col", C.c_void_p), ("is_w", C.c_void_p),
+        ("ib_off", C.c_void_p), ("ib_col", C.c_void_p),
+        ("sync_off", C.c_void_p), ("sync_col", C.c_void_p),
+        ("iv_off", C.c_void_p), ("iv_col", C.c_void_p), ("iv_w", C.c_void_p),
+        ("id_off", C.c_void_p), ("id_col", C.c_void_p), ("id_w", C.c_void_p),
+        ("sub_off", C.c_void_p), ("sub_col", C.c_void_p),
+    ]
+
+
+PHASE_PLAIN, PHASE_SKETCH = 0, 1
+
+
+class CompressInfo(C.Structure):
+    _fields_ = [
+        ("clusters", C.c_uint64),
+        ("all_node_num", C.c_uint64),
+        ("supernodes", C.c_uint64),
+        ("in_mirrors", C.c_uint64),
+        ("out_mirrors", C.c_uint64),
+        ("absorbed", C.c_uint64),
+        ("node_type_count", C.c_uint64 * 7),
+        ("is_edges", C.c_uint64),
+        ("ib_edges", C.c_uint64),
+        ("sync_edges", C.c_uint64),
+        ("subgraph_edges", C.c_uint64),
+        ("inner_value_entries", C.c_uint64),
+        ("inner_delta_entries", C.c_uint64),
+        ("shortcut_work", C.c_uint64),
+        ("reserved", C.c_uint64 * 8),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("reserved", "node_type_count")}
+        d["node_type_count"] = list(self.node_type_count)
+        return d
+
+
+class SketchBuffers(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name in (
+        "node_type", "mirror_master", "cluster_of", "is_off", "is_col", "is_w", "ib_off", "ib_col", "sync_off",
+        "sync_col", "iv_off", "iv_col", "iv_w", "id_off", "id_col", "id_w", "sub_off", "sub_col")]
+
+
 class KernelProfile(C.Structure):
     _fields_ = [
         ("launches", C.c_double),
@@ -158,6 +209,14 @@ SIGNATURES = {
     "suminc_sssp_init": (i32, [vp, u32]),
     "suminc_sssp_run": (i32, [vp, C.POINTER(Stats)]),
     "suminc_sssp_download": (i32, [vp, vp, vp]),
+    "suminc_sssp_step": (i32, [vp, C.POINTER(u32)]),
+    "suminc_upload_sketch": (i32, [vp, C.POINTER(SketchCsr)]),
+    "suminc_pr_step_phase": (i32, [vp, i32, C.POINTER(C.c_double)]),
+    "suminc_pr_correct": (i32, [vp]),
+    "suminc_pr_precompute": (i32, [vp]),
+    "suminc_sketch_pr_run": (i32, [vp, C.POINTER(Stats)]),
+    "suminc_compress_mirror": (i32, [vp, vp, u32, u32, C.c_double, C.c_double, C.POINTER(CompressInfo)]),
+    "suminc_sketch_download": (i32, [vp, C.POINTER(SketchBuffers)]),
     "suminc_get_stats": (i32, [vp, C.POINTER(Stats)]),
     "suminc_cluster_leaves": (i32, [vp, u32, u32, vp]),
     "suminc_set_clusters": (i32, [vp, vp]),

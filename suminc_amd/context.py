@@ -261,6 +261,75 @@ class Context:
         self._check(self._lib.suminc_sum_sssp_run(self._h, C.byref(s)))
         return s.as_dict()
 
+    # ---- the reference compressor's sketch, phase by phase --------------------
+    def compress_mirror(self, cluster_of, min_node_num=2, max_node_num=1024, compress_threshold=1.0,
+                        mirror_k=4.0) -> dict:
+        """The compressor with mirrors; its products become the context's sketch."""
+        c = np.ascontiguousarray(cluster_of, np.uint32)
+        info = L.CompressInfo()
+        self._check(self._lib.suminc_compress_mirror(self._h, L.ptr(c), min_node_num, max_node_num,
+                                                     float(compress_threshold), float(mirror_k), C.byref(info)))
+        self._compress_info = info.as_dict()
+        return self._compress_info
+
+    def sketch_download(self) -> dict:
+        """The last compress_mirror's products as numpy arrays (suminc_sketch_csr layout)."""
+        i = self._compress_info
+        n = self.graph_info()[0]
+        a = i["all_node_num"]
+        out = {"n": n, "all_node_num": a,
+               "node_type": np.zeros(n, np.uint8), "mirror_master": np.zeros(a - n, np.uint32),
+               "cluster_of": np.zeros(a, np.uint32),
+               "is_off": np.zeros(a + 1, np.uint64), "is_col": np.zeros(i["is_edges"], np.uint32),
+               "is_w": np.zeros(i["is_edges"], np.float32),
+               "ib_off": np.zeros(n + 1, np.uint64), "ib_col": np.zeros(i["ib_edges"], np.uint32),
+               "sync_off": np.zeros(a + 1, np.uint64), "sync_col": np.zeros(i["sync_edges"], np.uint32),
+               "iv_off": np.zeros(a + 1, np.uint64), "iv_col": np.zeros(i["inner_value_entries"], np.uint32),
+               "iv_w": np.zeros(i["inner_value_entries"], np.float32),
+               "id_off": np.zeros(a + 1, np.uint64), "id_col": np.zeros(i["inner_delta_entries"], np.uint32),
+               "id_w": np.zeros(i["inner_delta_entries"], np.float32),
+               "sub_off": np.zeros(a + 1, np.uint64), "sub_col": np.zeros(i["subgraph_edges"], np.uint32)}
+        b = L.SketchBuffers()
+        for name, _ in b._fields_:
+            setattr(b, name, out[name].ctypes.data if out[name].size else None)
+        self._check(self._lib.suminc_sketch_download(self._h, C.byref(b)))
+        return out
+
+    def upload_sketch(self, sk: dict):
+        """Uploads a sketch in the reference compressor's layout (dict of arrays as
+        sketch_download returns)."""
+        arrs = {}
+        dt = {"node_type": np.uint8, "mirror_master": np.uint32, "is_off": np.uint64, "is_col": np.uint32,
+              "is_w": np.float32, "ib_off": np.uint64, "ib_col": np.uint32, "sync_off": np.uint64,
+              "sync_col": np.uint32, "iv_off": np.uint64, "iv_col": np.uint32, "iv_w": np.float32,
+              "id_off": np.uint64, "id_col": np.uint32, "id_w": np.float32, "sub_off": np.uint64,
+              "sub_col": np.uint32}
+        s = L.SketchCsr()
+        s.n = int(sk["n"])
+        s.all_node_num = int(sk["all_node_num"])
+        for name, t in dt.items():
+            if sk.get(name) is None:  # the subgraph rows are optional
+                continue
+            arrs[name] = np.ascontiguousarray(sk[name], t)
+            setattr(s, name, arrs[name].ctypes.data if arrs[name].size else None)
+        self._check(self._lib.suminc_upload_sketch(self._h, C.byref(s)))
+
+    def pr_step_phase(self, phase) -> float:
+        r = C.c_double()
+        self._check(self._lib.suminc_pr_step_phase(self._h, int(phase), C.byref(r)))
+        return r.value
+
+    def pr_correct(self):
+        self._check(self._lib.suminc_pr_correct(self._h))
+
+    def pr_precompute(self):
+        self._check(self._lib.suminc_pr_precompute(self._h))
+
+    def sketch_pr_run(self) -> dict:
+        s = L.Stats()
+        self._check(self._lib.suminc_sketch_pr_run(self._h, C.byref(s)))
+        return s.as_dict()
+
     # ---- weighted personalized PageRank + adjoint ------------------------------
     # Arrays may be numpy arrays (host) or torch tensors on this context's GPU.
     # The backward reads the rank and weights of the last forward on the context;
@@ -306,6 +375,12 @@ class Context:
         s = L.Stats()
         self._check(self._lib.suminc_sssp_run(self._h, C.byref(s)))
         return s.as_dict()
+
+    def sssp_step(self) -> int:
+        """One min-plus round; returns the size of the frontier it produced."""
+        f = C.c_uint32()
+        self._check(self._lib.suminc_sssp_step(self._h, C.byref(f)))
+        return f.value
 
     def sssp_download(self):
         n = self.graph_info()[0]

@@ -150,6 +150,8 @@ int suminc_ctx_destroy(suminc_ctx* c) {
   if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
   hipStream_t s = c->stream;
   wpr_release(c);
+  rsk_release(c);
+  mirror_release(c);
   delete c;  // DevBuf destructors free device memory
   if (s) (void)hipStreamDestroy(s);
   return SUMINC_OK;
@@ -287,6 +289,45 @@ int suminc_sssp_init(suminc_ctx* c, uint32_t source) {
 
 int suminc_sssp_run(suminc_ctx* c, suminc_stats* stats) {
   return guarded(c, [&] { sssp_run(c, stats); });
+}
+
+int suminc_sssp_step(suminc_ctx* c, uint32_t* frontier_out) {
+  return guarded(c, [&] { sssp_step(c, frontier_out); });
+}
+
+int suminc_upload_sketch(suminc_ctx* c, const suminc_sketch_csr* sketch) {
+  return guarded(c, [&] { rsk_upload(c, sketch); });
+}
+
+int suminc_pr_step_phase(suminc_ctx* c, int phase, double* residual_out) {
+  return guarded(c, [&] {
+    SUMINC_REQUIRE(phase == SUMINC_PHASE_PLAIN || phase == SUMINC_PHASE_SKETCH, SUMINC_E_INVALID, "unknown phase");
+    if (phase == SUMINC_PHASE_PLAIN) pr_step(c, residual_out);
+    else rsk_step(c, residual_out);
+  });
+}
+
+int suminc_pr_correct(suminc_ctx* c) {
+  return guarded(c, [&] { rsk_correct(c); });
+}
+
+int suminc_pr_precompute(suminc_ctx* c) {
+  return guarded(c, [&] { rsk_precompute(c); });
+}
+
+int suminc_sketch_pr_run(suminc_ctx* c, suminc_stats* stats) {
+  return guarded(c, [&] { rsk_run(c, stats); });
+}
+
+int suminc_compress_mirror(suminc_ctx* c, const uint32_t* cluster_of, uint32_t min_node_num, uint32_t max_node_num,
+                           double compress_threshold, double mirror_k, suminc_compress_info* info) {
+  return guarded(c, [&] {
+    compress_mirror(c, cluster_of, min_node_num, max_node_num, compress_threshold, mirror_k, info);
+  });
+}
+
+int suminc_sketch_download(suminc_ctx* c, suminc_sketch_buffers* buffers) {
+  return guarded(c, [&] { sketch_csr_download(c, buffers); });
 }
 
 int suminc_sssp_download(suminc_ctx* c, uint32_t* dist, uint32_t* parent) {

@@ -191,6 +191,7 @@ struct suminc_ctx {
 
   // PageRank state
   bool pr_ready = false;
+  uint64_t pr_state_id = 0;  // counts suminc_pr_init / suminc_pr_upload: a new PageRank state
   int pr_prec = SUMINC_PR_FP32_KAHAN;
   suminc::DevBuf pr_delta, pr_rank, pr_comp, pr_contrib;
   suminc::DevBuf mass_part;  // block sums of the FP64 mass balance
@@ -228,6 +229,10 @@ struct suminc_ctx {
 
   // weighted PageRank / adjoint state (wpr.hip), created on first use
   void* wpr = nullptr;
+  // the uploaded sketch in the reference compressor's layout (rsketch.hip), and
+  // the host products of the last compression with mirrors (compress.hip)
+  void* rsk = nullptr;
+  void* msk = nullptr;
 
   // work lists and scratch
   suminc::DevBuf wl0, wl1, heavy, partial, tmp, tmp2, tmp3, tmp4, tmp5;

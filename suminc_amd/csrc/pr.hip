@@ -577,6 +577,7 @@ void pr_init(suminc_ctx* c) {
                                                               c->pr_comp.as<float>(), (float)init);
   SUMINC_HIP(hipGetLastError());
   c->pr_ready = true;
+  ++c->pr_state_id;
 }
 
 // ---------------------------------------------------------------------------
@@ -1151,6 +1152,7 @@ void pr_upload(suminc_ctx* c, const double* rank, const double* delta) {
   SUMINC_HIP(hipGetLastError());
   SUMINC_HIP(hipStreamSynchronize(c->stream));
   c->pr_ready = true;
+  ++c->pr_state_id;
 }
 
 }  // namespace suminc

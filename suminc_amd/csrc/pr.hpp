@@ -60,6 +60,24 @@ void wpr_download(suminc_ctx* c, double* r, double* lam);
 // wpr_scratch_setup takes that array's device address, once per device
 void wpr_forward_noalloc(suminc_ctx* c, const float* p, float* r, suminc_stats* st);
 void wpr_scratch_setup(int device);
+
+// the reference-layout sketch (rsketch.hip)
+void rsk_upload(suminc_ctx* c, const suminc_sketch_csr* s, const double* is_w64 = nullptr,
+                const double* iv_w64 = nullptr, const double* id_w64 = nullptr);
+void rsk_step(suminc_ctx* c, double* residual);
+void rsk_correct(suminc_ctx* c);
+void rsk_precompute(suminc_ctx* c);
+void rsk_run(suminc_ctx* c, suminc_stats* st);
+void rsk_release(suminc_ctx* c);
+
+// compress.hip: the compressor with mirrors, its products uploaded as the sketch
+void compress_mirror(suminc_ctx* c, const uint32_t* cluster_of, uint32_t min_node, uint32_t max_node,
+                     double threshold, double mirror_k, suminc_compress_info* info);
+void sketch_csr_download(suminc_ctx* c, suminc_sketch_buffers* b);
+void mirror_release(suminc_ctx* c);
+
+// sssp.hip: one round from the pending frontier, the next frontier's size out
+void sssp_step(suminc_ctx* c, uint32_t* frontier_out);
 void wpr_release(suminc_ctx* c);
 
 }  // namespace suminc

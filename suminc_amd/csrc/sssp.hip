@@ -271,6 +271,37 @@ void sssp_run(suminc_ctx* c, suminc_stats* st) {
   }
 }
 
+// the next frontier (list 1 after a round) becomes the pending seeds of list 0
+__global__ void k_sssp_carry(const Counters* cnt, unsigned long long* seeds) {
+  if (threadIdx.x == 0) *seeds = cnt->wl_count[1];
+}
+
+// One min-plus round from the pending frontier (the seeds of init or a batch, or
+// the previous step's output): tjnsssp::g_function + setCurSize/getAllSize
+// (sum_sync_traversal_worker.h:1090, 1136-1142).  The frontier the round produced
+// becomes the pending one; *frontier_out gets its size (0: fixed point).
+void sssp_step(suminc_ctx* c, uint32_t* frontier_out) {
+  SUMINC_REQUIRE(c->sssp_ready, SUMINC_E_STATE, "SSSP state not initialised");
+  SsspArgs a = sssp_args(c);
+  c->sssp_pending = false;
+  k_sssp_begin<<<1, 64, 0, c->stream>>>(a.cnt, c->sssp_seeds.as<unsigned long long>());
+  k_sssp_push<<<push_grid(), kBlock, 0, c->stream>>>(a);
+  k_sssp_heavy<<<push_grid(), kBlock, 0, c->stream>>>(a);
+  k_sssp_end<<<1, 64, 0, c->stream>>>(a.cnt, 1 << 30);
+  k_sssp_carry<<<1, 64, 0, c->stream>>>(a.cnt, c->sssp_seeds.as<unsigned long long>());
+  SUMINC_HIP(hipGetLastError());
+  read_counters(c);
+  const Counters* h = c->host_counters;
+  c->stats = suminc_stats{};
+  c->stats.rounds = 1;
+  c->stats.edges_traversed = (double)h->edges;
+  c->stats.push_rounds = 1;
+  // list 1 holds the next frontier with its bitmap: make it list 0
+  c->sssp_wl0.swap(c->sssp_wl1);
+  c->sssp_flag0.swap(c->sssp_flag1);
+  if (frontier_out) *frontier_out = (uint32_t)h->wl_count[1];
+}
+
 void sssp_download(suminc_ctx* c, uint32_t* dist, uint32_t* parent) {
   SUMINC_REQUIRE(c->sssp_ready, SUMINC_E_STATE, "SSSP state not initialised");
   DevBuf dd, pp;

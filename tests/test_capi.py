@@ -63,7 +63,8 @@ def test_struct_layouts_match_the_header(tmp_path):
     import numpy as np
 
     mirrors = {"suminc_params": L.Params, "suminc_stats": L.Stats, "suminc_sketch_info": L.SketchInfo,
-               "suminc_kernel_profile": L.KernelProfile}
+               "suminc_kernel_profile": L.KernelProfile, "suminc_sketch_csr": L.SketchCsr,
+               "suminc_compress_info": L.CompressInfo, "suminc_sketch_buffers": L.SketchBuffers}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "suminc.h"', "int main(void) {"]
     for cname, py in mirrors.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
