@@ -217,6 +217,34 @@ int suminc_sssp_download(suminc_ctx* ctx, uint32_t* dist, uint32_t* parent);
  * *frontier_out gets the size of the frontier the round produced, which becomes
  * the pending one (0: fixed point). */
 int suminc_sssp_step(suminc_ctx* ctx, uint32_t* frontier_out);
+/* The other traversal Ingress apps on the same kernels, state, batch repair and
+ * single step (suminc_sssp_run / _step / _download and suminc_apply_batch then act
+ * on the chosen app; the supernode SSSP query takes SUMINC_APP_SSSP only):
+ *   SUMINC_APP_SSSP  min-plus over the edge weights (sssp_ingress.h:131-175);
+ *   SUMINC_APP_BFS   hops from the source, every edge counting 1 whatever its
+ *                    weight (bfs/bfs_ingress.h:60-95);
+ *   SUMINC_APP_SSWP  widest path: the largest over paths of the smallest edge
+ *                    weight (sswp/sswp_ingress.h:60-140); the source holds
+ *                    0xFFFFFFFF, an unreached vertex 0 (its identity element).
+ * The value's word is dist (SSSP, BFS) or ~width (SSWP), so one 64-bit atomicMin
+ * serves all three.  SSSP and BFS keep the smallest-id predecessor among the
+ * optimal ones as parent; SSWP changes a vertex only on a strictly wider path (as
+ * SetIfGreaterThan does), so its parent is a predecessor whose strict improvement
+ * gave the final width (the smallest id among those of one round) and the parents
+ * never form a cycle. */
+#define SUMINC_APP_SSSP 0
+#define SUMINC_APP_BFS 1
+#define SUMINC_APP_SSWP 2
+int suminc_traversal_init(suminc_ctx* ctx, int app, uint32_t source);
+
+/* PHP, penalized hitting probability (php/php_ingress.h:15-120), on the weighted
+ * series rounds: delta_0 = 1 at the source, a vertex u sends d * w_uv / s_u of its
+ * delta along each out-edge to v != source, s_u = the sum of u's out-edge weights to
+ * vertices other than the source (weight_sum, :30-45; unit weights on an unweighted
+ * graph); values[v] (n FP64, may be NULL) = the sum of the deltas v took, until
+ * the round's sum |delta| < termcheck_threshold.  The reference's d is 0.80f
+ * (php_ingress.h:31). */
+int suminc_php_run(suminc_ctx* ctx, uint32_t source, double damping, double* values, suminc_stats* stats);
 
 /* ---- supernode sketch ------------------------------------------------------------
  * Clustering input, acceptance, node types, shortcut precompute and the SumInc

@@ -123,6 +123,7 @@ class SketchCsr(C.Structure):
 
 
 PHASE_PLAIN, PHASE_SKETCH = 0, 1
+APP_SSSP, APP_BFS, APP_SSWP = 0, 1, 2
 
 
 class CompressInfo(C.Structure):
@@ -214,6 +215,8 @@ SIGNATURES = {
     "suminc_pr_step_phase": (i32, [vp, i32, C.POINTER(C.c_double)]),
     "suminc_pr_correct": (i32, [vp]),
     "suminc_pr_precompute": (i32, [vp]),
+    "suminc_traversal_init": (i32, [vp, C.c_int, u32]),
+    "suminc_php_run": (i32, [vp, u32, C.c_double, vp, vp]),
     "suminc_sketch_pr_run": (i32, [vp, C.POINTER(Stats)]),
     "suminc_compress_mirror": (i32, [vp, vp, u32, u32, C.c_double, C.c_double, C.POINTER(CompressInfo)]),
     "suminc_sketch_download": (i32, [vp, C.POINTER(SketchBuffers)]),

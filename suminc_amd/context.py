@@ -371,6 +371,19 @@ class Context:
     def sssp_init(self, source=0):
         self._check(self._lib.suminc_sssp_init(self._h, source))
 
+    def traversal_init(self, app, source=0):
+        """SSSP, BFS or SSWP state (APP_SSSP / APP_BFS / APP_SSWP) for sssp_run / sssp_step /
+        sssp_download and the batch repair."""
+        self._check(self._lib.suminc_traversal_init(self._h, int(app), source))
+
+    def php_run(self, source=0, damping=float(np.float32(0.8))):
+        """PHP values (FP64) from `source`; the reference's d is 0.80f."""
+        n = self.graph_info()[0]
+        out = np.zeros(n)
+        s = L.Stats()
+        self._check(self._lib.suminc_php_run(self._h, source, float(damping), L.ptr(out), C.byref(s)))
+        return out, s.as_dict()
+
     def sssp_run(self) -> dict:
         s = L.Stats()
         self._check(self._lib.suminc_sssp_run(self._h, C.byref(s)))

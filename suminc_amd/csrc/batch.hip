@@ -696,12 +696,12 @@ __global__ void k_reset_apply(const uint32_t* rlist, Counters* cnt, unsigned lon
 // reset vertices pull from surviving (non-reset) in-neighbours over the NEW CSC
 __device__ __forceinline__ unsigned long long pull_candidate(const uint32_t* icol, const uint32_t* iw,
                                                              const uint32_t* rbits, const unsigned long long* dp,
-                                                             uint32_t k) {
+                                                             uint32_t k, int app) {
   const uint32_t u = icol[k];
   if ((rbits[u >> 5] >> (u & 31)) & 1u) return ~0ull;
   const uint32_t du = (uint32_t)(dp[u] >> 32);
   if (du == INF32) return ~0ull;
-  const unsigned long long nd = (unsigned long long)du + (iw ? iw[k] : 1u);
+  const unsigned long long nd = trav_extend(app, du, iw ? iw[k] : 1u);
   if (nd >= INF32) return ~0ull;
   return (nd << 32) | u;
 }
@@ -709,7 +709,7 @@ __device__ __forceinline__ unsigned long long pull_candidate(const uint32_t* ico
 __global__ void __launch_bounds__(kBlock) k_reset_pull(const uint32_t* ioff, const uint32_t* icol,
                                                        const uint32_t* iw, const uint32_t* rbits,
                                                        const uint32_t* rlist, Counters* cnt,
-                                                       unsigned long long* dp, uint2* heavy) {
+                                                       unsigned long long* dp, uint2* heavy, int app) {
   __shared__ uint32_t s_scan[kBlock + 1];
   __shared__ uint32_t s_start[kBlock];
   __shared__ uint32_t s_warp[kBlock / 64 + 1];
@@ -731,7 +731,7 @@ __global__ void __launch_bounds__(kBlock) k_reset_pull(const uint32_t* ioff, con
       }
     }
     block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t, uint32_t k) {
-      const unsigned long long cand = pull_candidate(icol, iw, rbits, dp, k);
+      const unsigned long long cand = pull_candidate(icol, iw, rbits, dp, k, app);
       // the reset words only decrease here: a candidate no smaller than a plain
       // read cannot win and skips the atomic
       if (cand != ~0ull && cand < dp[s_x[slot]]) atomicMin(&dp[s_x[slot]], cand);
@@ -743,7 +743,7 @@ __global__ void __launch_bounds__(kBlock) k_reset_pull(const uint32_t* ioff, con
 __global__ void __launch_bounds__(kBlock) k_reset_pull_heavy(const uint32_t* ioff, const uint32_t* icol,
                                                              const uint32_t* iw, const uint32_t* rbits,
                                                              const uint2* heavy, Counters* cnt,
-                                                             unsigned long long* dp) {
+                                                             unsigned long long* dp, int app) {
   __shared__ unsigned long long s_min[kBlock / 64];
   const unsigned long long nh = cnt->heavy_count;
   for (unsigned long long h = blockIdx.x; h < nh; h += gridDim.x) {
@@ -753,7 +753,7 @@ __global__ void __launch_bounds__(kBlock) k_reset_pull_heavy(const uint32_t* iof
     const uint32_t e = min(ioff[x + 1], b + kChunk);
     unsigned long long best = ~0ull;
     for (uint32_t k = b + threadIdx.x; k < e; k += kBlock) {
-      const unsigned long long cand = pull_candidate(icol, iw, rbits, dp, k);
+      const unsigned long long cand = pull_candidate(icol, iw, rbits, dp, k, app);
       best = cand < best ? cand : best;
     }
 #pragma unroll
@@ -799,7 +799,7 @@ __global__ void k_reset_seed(const uint32_t* rlist, uint32_t* rbits, const unsig
 
 // relax every add (new pair or new weight) from its current source distance
 __global__ void k_insert_relax(const uint32_t* grow, const uint32_t* gcol, const uint8_t* gact, const uint32_t* gw,
-                               const unsigned* bc, uint32_t N, int unit, unsigned long long* dp, uint32_t* wl0,
+                               const unsigned* bc, uint32_t N, int unit, int app, unsigned long long* dp, uint32_t* wl0,
                                uint32_t* bits0, unsigned long long* seeds) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   bool add = false;
@@ -808,8 +808,9 @@ __global__ void k_insert_relax(const uint32_t* grow, const uint32_t* gcol, const
     uint32_t u = grow[g];
     v = gcol[g];
     uint32_t du = (uint32_t)(dp[u] >> 32);
-    unsigned long long nd = (unsigned long long)du + (unit ? 1u : gw[g]);
-    if (du != INF32 && nd < INF32) {
+    unsigned long long nd = trav_extend(app, du, unit ? 1u : gw[g]);
+    // SSWP: strictly better words only (no parent change on a tie)
+    if (du != INF32 && nd < INF32 && (app != SUMINC_APP_SSWP || nd < (dp[v] >> 32))) {
       unsigned long long cand = (nd << 32) | u;
       unsigned long long old = atomicMin(&dp[v], cand);
       if ((old >> 32) > nd) {
@@ -1179,21 +1180,22 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   if (sssp_repair) {
     k_reset_apply<<<push_grid(), kBlock, 0, c->stream>>>(b.rlist.as<uint32_t>(), cnt,
                                                          c->sssp_dp.as<unsigned long long>());
-    const uint32_t* iw = c->wkind == SUMINC_W_U32 ? c->in_w.as<uint32_t>() : nullptr;
+    const uint32_t* iw = c->wkind == SUMINC_W_U32 && c->sssp_app != SUMINC_APP_BFS ? c->in_w.as<uint32_t>() : nullptr;
     k_reset_pull<<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(), iw,
                                                         b.rbits.as<uint32_t>(), b.rlist.as<uint32_t>(), cnt,
-                                                        c->sssp_dp.as<unsigned long long>(), c->heavy.as<uint2>());
+                                                        c->sssp_dp.as<unsigned long long>(), c->heavy.as<uint2>(),
+                                                        c->sssp_app);
     k_reset_pull_heavy<<<push_grid(), kBlock, 0, c->stream>>>(c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(), iw,
                                                               b.rbits.as<uint32_t>(), c->heavy.as<uint2>(), cnt,
-                                                              c->sssp_dp.as<unsigned long long>());
+                                                              c->sssp_dp.as<unsigned long long>(), c->sssp_app);
     k_reset_seed<<<push_grid(), kBlock, 0, c->stream>>>(b.rlist.as<uint32_t>(), b.rbits.as<uint32_t>(),
                                                         c->sssp_dp.as<unsigned long long>(),
                                                         c->sssp_wl0.as<uint32_t>(), c->sssp_flag0.as<uint32_t>(), cnt,
                                                         c->sssp_seeds.as<unsigned long long>());
     k_insert_relax<<<grid_for(N), kBlock, 0, c->stream>>>(b.g_row.as<uint32_t>(), b.g_col.as<uint32_t>(),
                                                           b.g_act.as<uint8_t>(), b.g_w.as<uint32_t>(), bc, N,
-                                                          c->wkind != SUMINC_W_U32,
-                                                          c->sssp_dp.as<unsigned long long>(),
+                                                          c->wkind != SUMINC_W_U32 || c->sssp_app == SUMINC_APP_BFS,
+                                                          c->sssp_app, c->sssp_dp.as<unsigned long long>(),
                                                           c->sssp_wl0.as<uint32_t>(), c->sssp_flag0.as<uint32_t>(),
                                                           c->sssp_seeds.as<unsigned long long>());
   }

@@ -284,7 +284,15 @@ int suminc_pr_upload(suminc_ctx* c, const double* rank, const double* delta) {
 }
 
 int suminc_sssp_init(suminc_ctx* c, uint32_t source) {
-  return guarded(c, [&] { sssp_init(c, source); });
+  return guarded(c, [&] { traversal_init(c, SUMINC_APP_SSSP, source); });
+}
+
+int suminc_traversal_init(suminc_ctx* c, int app, uint32_t source) {
+  return guarded(c, [&] { traversal_init(c, app, source); });
+}
+
+int suminc_php_run(suminc_ctx* c, uint32_t source, double damping, double* values, suminc_stats* stats) {
+  return guarded(c, [&] { php_run(c, source, damping, values, stats); });
 }
 
 int suminc_sssp_run(suminc_ctx* c, suminc_stats* stats) {

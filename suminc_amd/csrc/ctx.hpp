@@ -218,6 +218,7 @@ struct suminc_ctx {
   bool sssp_ready = false;
   bool sssp_pending = false;  // initialised, not yet run
   uint32_t sssp_src = 0;
+  int sssp_app = SUMINC_APP_SSSP;  // the traversal app of the SSSP state
   suminc::DevBuf sssp_dp, sssp_flag0, sssp_flag1;
   // frontier lists 0/1 and the length of list 0's seeds (set by init, appended by
   // batches, taken by the next query): owned by SSSP, so that PageRank and the

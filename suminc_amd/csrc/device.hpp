@@ -5,6 +5,8 @@
 
 #include <cstdint>
 
+#include "../../include/suminc.h"
+
 namespace suminc {
 
 // Philox4x32-10 (Salmon et al., SC'11), same stream layout as the synthetic-input
@@ -112,5 +114,17 @@ __device__ __forceinline__ uint32_t frontier_tile(unsigned long long nl) {
 
 template <class T>
 __device__ __forceinline__ T fabs_t(T x) { return x < T(0) ? -x : x; }
+
+// The traversal apps' path extension on the high word of the packed
+// {word, parent} state: SSSP dist + w, BFS hops + 1, SSWP ~min(width, w) with the
+// word holding ~width (so that the smaller word is the better value for all
+// three).  Returns a value >= 0xFFFFFFFF when the extension does not reach.
+__device__ __forceinline__ unsigned long long trav_extend(int app, uint32_t word_u, uint32_t w) {
+  if (app == SUMINC_APP_SSWP) {
+    const uint32_t width = ~word_u;
+    return (unsigned long long)(uint32_t)~(w < width ? w : width);
+  }
+  return (unsigned long long)word_u + (app == SUMINC_APP_BFS ? 1u : w);
+}
 
 }  // namespace suminc

@@ -733,6 +733,8 @@ __global__ void k_ss_begin(Counters* cnt, unsigned long long* lists, unsigned lo
 
 void sum_sssp_run(suminc_ctx* c, suminc_stats* st) {
   SUMINC_REQUIRE(c->sssp_ready, SUMINC_E_STATE, "SSSP state not initialised");
+  SUMINC_REQUIRE(c->sssp_app == SUMINC_APP_SSSP, SUMINC_E_STATE,
+                 "the supernode query runs SSSP only (the state was set up for BFS or SSWP)");
   sketch_refresh(c);
   Sketch& s = c->sk;
   s.ss_work.ensure(64);

@@ -33,18 +33,20 @@ struct SsspArgs {
   uint32_t* bits[2];
   uint2* heavy;
   Counters* cnt;
+  int app;  // SUMINC_APP_*
 };
 
 __device__ __forceinline__ void sssp_relax(const SsspArgs& a, int nxt, uint32_t u, uint32_t du, uint32_t e) {
   uint32_t v = a.col[e];
   uint32_t w = a.w ? a.w[e] : 1u;
-  unsigned long long nd = (unsigned long long)du + w;  // < 2^32 - 1 unless the path saturates
+  unsigned long long nd = trav_extend(a.app, du, w);  // < 2^32 - 1 unless the path saturates
   bool add = false;
   if (nd < INF32) {
     unsigned long long cand = (nd << 32) | u;
     // words only decrease while the rounds run: a candidate no smaller than a
-    // plain (possibly stale) read cannot win, so it skips the atomic
-    if (cand < a.dp[v]) {
+    // plain (possibly stale) read cannot win, so it skips the atomic.  SSWP takes
+    // strictly better words only (no parent change on a tie: no parent cycles)
+    if (a.app == SUMINC_APP_SSWP ? nd < (a.dp[v] >> 32) : cand < a.dp[v]) {
       unsigned long long old = atomicMin(&a.dp[v], cand);
       add = (old >> 32) > nd;
     }
@@ -158,11 +160,11 @@ __global__ void k_sssp_begin(Counters* cnt, unsigned long long* seeds) {
   cnt->edges = 0;
 }
 
-__global__ void k_sssp_unpack(uint32_t n, const unsigned long long* dp, uint32_t* dist, uint32_t* parent) {
+__global__ void k_sssp_unpack(uint32_t n, const unsigned long long* dp, int app, uint32_t* dist, uint32_t* parent) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   unsigned long long x = dp[v];
-  dist[v] = (uint32_t)(x >> 32);
+  dist[v] = app == SUMINC_APP_SSWP ? ~(uint32_t)(x >> 32) : (uint32_t)(x >> 32);
   parent[v] = (uint32_t)x;
 }
 
@@ -170,8 +172,9 @@ SsspArgs sssp_args(suminc_ctx* c) {
   SsspArgs a;
   a.off = c->out_off.as<uint32_t>();
   a.col = c->out_col.as<uint32_t>();
-  a.w = c->wkind == SUMINC_W_U32 ? c->out_w.as<uint32_t>() : nullptr;
+  a.w = c->wkind == SUMINC_W_U32 && c->sssp_app != SUMINC_APP_BFS ? c->out_w.as<uint32_t>() : nullptr;
   a.dp = c->sssp_dp.as<unsigned long long>();
+  a.app = c->sssp_app;
   a.wl[0] = c->sssp_wl0.as<uint32_t>();
   a.wl[1] = c->sssp_wl1.as<uint32_t>();
   a.bits[0] = c->sssp_flag0.as<uint32_t>();
@@ -197,6 +200,13 @@ void sssp_clear_flags(suminc_ctx* c) {
   const size_t words = ((size_t)c->n + 31) / 32 + 1;
   SUMINC_HIP(hipMemsetAsync(c->sssp_flag0.p, 0, words * 4, c->stream));
   SUMINC_HIP(hipMemsetAsync(c->sssp_flag1.p, 0, words * 4, c->stream));
+}
+
+void traversal_init(suminc_ctx* c, int app, uint32_t source) {
+  SUMINC_REQUIRE(app == SUMINC_APP_SSSP || app == SUMINC_APP_BFS || app == SUMINC_APP_SSWP, SUMINC_E_INVALID,
+                 "unknown traversal app");
+  c->sssp_app = app;
+  sssp_init(c, source);
 }
 
 void sssp_init(suminc_ctx* c, uint32_t source) {
@@ -307,7 +317,7 @@ void sssp_download(suminc_ctx* c, uint32_t* dist, uint32_t* parent) {
   DevBuf dd, pp;
   dd.ensure((size_t)c->n * 4);
   pp.ensure((size_t)c->n * 4);
-  k_sssp_unpack<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->sssp_dp.as<unsigned long long>(),
+  k_sssp_unpack<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->sssp_dp.as<unsigned long long>(), c->sssp_app,
                                                         dd.as<uint32_t>(), pp.as<uint32_t>());
   SUMINC_HIP(hipGetLastError());
   if (dist) SUMINC_HIP(hipMemcpyAsync(dist, dd.p, (size_t)c->n * 4, hipMemcpyDeviceToHost, c->stream));

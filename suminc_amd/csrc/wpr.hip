@@ -412,6 +412,7 @@ struct WprState {
   DevBuf xc;                         // x at the compact positions (large graphs)
   DevBuf r64, lam64, yl64;           // FP64 results of the last forward / backward
   DevBuf l1;                         // sum |seed|
+  DevBuf php_a_out, php_a_in, php64;  // PHP: coefficients (T) in CSR / CSC order, FP64 result
 };
 
 static WprState& wstate(suminc_ctx* c) {
@@ -716,6 +717,76 @@ void wpr_backward(suminc_ctx* c, const float* cvec, float* grad_p, float* grad_w
   SUMINC_REQUIRE(cvec, SUMINC_E_INVALID, "null loss weights");
   if (c->prm.pr_precision == SUMINC_PR_FP64) wpr_backward_t<double>(c, cvec, grad_p, grad_w, st);
   else wpr_backward_t<float>(c, cvec, grad_p, grad_w, st);
+}
+
+// ---- PHP: penalized hitting probability on the series kernels --------------------
+// PHPIngress (php/php_ingress.h:15-120): value = sum of the deltas a vertex takes,
+// delta_0 = 1 at the source and 0 elsewhere, and a vertex u sends
+// d * w_uv / s_u of its delta along every out-edge u -> v with v != source, where
+// s_u = the sum of u's out-edge weights to vertices other than the source
+// (frag.weight_sum, :30-45); nothing enters the source again.  That is the weighted
+// series x_{t+1} = A x_t with a_uv = d w_uv / s_u (0 into the source), x_0 = e_src,
+// value = sum_t x_t: the weighted-PageRank rounds over the same CSC row classes.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_php_coef_csr(uint32_t n, const uint32_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ col,
+                                                         const uint32_t* __restrict__ wu,
+                                                         const float* __restrict__ wf, uint32_t src, double d,
+                                                         T* __restrict__ a) {
+  const uint32_t waves = gridDim.x * (kBlock / 64);
+  for (uint32_t v = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); v < n; v += waves) {  // whole waves
+    double acc = 0;
+    for (uint32_t e = off[v] + lane_id(); e < off[v + 1]; e += 64)
+      if (col[e] != src) acc += wu ? (double)wu[e] : wf ? (double)wf[e] : 1.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    const double inv = acc > 0 ? d / acc : 0.0;
+    for (uint32_t e = off[v] + lane_id(); e < off[v + 1]; e += 64) {
+      const double w = wu ? (double)wu[e] : wf ? (double)wf[e] : 1.0;
+      a[e] = col[e] != src ? (T)(w * inv) : T(0);
+    }
+  }
+}
+
+__global__ void k_php_seed(uint32_t n, uint32_t src, float* __restrict__ p) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) p[v] = v == src ? 1.f : 0.f;
+}
+
+template <class T>
+static void php_run_t(suminc_ctx* c, uint32_t src, double d, double* values, suminc_stats* st) {
+  WprState& S = wstate(c);
+  wpr_prepare_graph(c, S);
+  const uint32_t n = c->n;
+  const uint64_t m = c->m;
+  S.php_a_out.ensure(m * sizeof(T) + 8);
+  S.php_a_in.ensure(m * sizeof(T) + 8);
+  S.php64.ensure((size_t)n * 8 + 8);
+  const uint32_t* wu = c->wkind == SUMINC_W_U32 ? c->out_w.as<uint32_t>() : nullptr;
+  const float* wf = c->wkind == SUMINC_W_F32 ? c->out_w.as<float>() : nullptr;
+  k_php_coef_csr<T><<<grid_for(n, kBlock / 64, kWaveGrid), kBlock, 0, c->stream>>>(
+      n, c->out_off.as<uint32_t>(), c->out_col.as<uint32_t>(), wu, wf, src, d, S.php_a_out.as<T>());
+  if (m)
+    k_wpr_gather<T><<<grid_for(m), kBlock, 0, c->stream>>>(m, S.eid.as<uint32_t>(), S.php_a_out.as<T>(),
+                                                         S.php_a_in.as<T>());
+  DevBuf seed;
+  seed.ensure((size_t)n * 4 + 4);
+  k_php_seed<<<grid_for(n), kBlock, 0, c->stream>>>(n, src, seed.as<float>());
+  SUMINC_HIP(hipGetLastError());
+  const RowLists R = lists_of(S.in, c->in_off, c->in_col, true);
+  const uint32_t* pos = S.in.compact ? S.in.pos.as<uint32_t>() : nullptr;
+  wpr_series<T>(c, series_state<T>(c, S, pos != nullptr), R, pos, S.php_a_in.as<T>(), seed.as<float>(), (T)1,
+                c->prm.termcheck_threshold, false, nullptr, S.php64.as<double>(), st);
+  if (values) SUMINC_HIP(hipMemcpyAsync(values, S.php64.p, (size_t)n * 8, hipMemcpyDefault, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+}
+
+void php_run(suminc_ctx* c, uint32_t source, double damping, double* values, suminc_stats* st) {
+  SUMINC_REQUIRE(c->has_graph, SUMINC_E_STATE, "no graph");
+  SUMINC_REQUIRE(source < c->n, SUMINC_E_INVALID, "source out of range");
+  SUMINC_REQUIRE(damping > 0.0 && damping < 1.0, SUMINC_E_INVALID, "damping must be in (0, 1)");
+  if (c->prm.pr_precision == SUMINC_PR_FP64) php_run_t<double>(c, source, damping, values, st);
+  else php_run_t<float>(c, source, damping, values, st);
 }
 
 // ---- the allocation-free forward -------------------------------------------------
