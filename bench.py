@@ -273,7 +273,7 @@ def run_pr(args, wl):
     import suminc_amd._lib as L
 
     ctx = suminc_amd.Context(termcheck_threshold=wl["T"], pr_precision=suminc_amd.PR_FP32_KAHAN,
-                             max_rounds=1_000_000, pull_band_kib=args.pull_band_kib)
+                             max_rounds=1_000_000, pull_band_kib=args.pull_band_kib, pull_tile_kib=args.pull_tile_kib)
     t0 = time.time()
     ctx.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"])
     n, m, _ = ctx.graph_info()
@@ -375,7 +375,8 @@ def _pmc_traffic(args, kernel_sub):
         d = tempfile.mkdtemp(prefix=f"pmc_{counter.lower()}_", dir=os.environ.get("TMPDIR", "/tmp"))
         cmd = ["timeout", "-s", "KILL", str(args.traffic_timeout), prof, "--pmc", counter, "--output-format", "csv",
                "-d", d, "-o", "run", "--", sys.executable, str(Path(__file__).resolve()), "--traffic-child",
-               "--workload", args.workload, "--pull-band-kib", str(args.pull_band_kib)]
+               "--workload", args.workload, "--pull-band-kib", str(args.pull_band_kib),
+               "--pull-tile-kib", str(args.pull_tile_kib)]
         log(f"traffic: {' '.join(cmd[4:])}")
         r = subprocess.run(cmd, stdout=sys.stderr, stderr=sys.stderr, cwd=str(ROOT))  # progress passes through
         files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
@@ -411,7 +412,7 @@ def traffic_child(args, wl):
     import suminc_amd
 
     ctx = suminc_amd.Context(termcheck_threshold=wl["T"], pr_precision=suminc_amd.PR_FP32_KAHAN, max_rounds=4,
-                             pull_band_kib=args.pull_band_kib)
+                             pull_band_kib=args.pull_band_kib, pull_tile_kib=args.pull_tile_kib)
     ctx.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"])
     ctx.cluster_leaves(wl["max_node"], 2)
     ctx.sum_build(2, wl["max_node"], 1.0)
@@ -442,7 +443,7 @@ def run_web(args, wl):
 
     T = wl["T"]
     ctx = suminc_amd.Context(termcheck_threshold=T, pr_precision=suminc_amd.PR_FP32_KAHAN,
-                             max_rounds=1_000_000, pull_band_kib=args.pull_band_kib)
+                             max_rounds=1_000_000, pull_band_kib=args.pull_band_kib, pull_tile_kib=args.pull_tile_kib)
     t0 = time.time()
     ctx.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"], weights="u32", wmin=wl["wmin"],
                       wmax=wl["wmax"])
@@ -590,7 +591,7 @@ def run_web(args, wl):
         # weights (a context of its own: the SSSP graph's weights are integers),
         # Dirichlet(1) personalization over random seeds, c ~ N(0,1)
         with suminc_amd.Context(termcheck_threshold=wl["T"], pr_precision=suminc_amd.PR_FP32_KAHAN,
-                                max_rounds=1_000_000, pull_band_kib=args.pull_band_kib) as cw:
+                                max_rounds=1_000_000, pull_band_kib=args.pull_band_kib, pull_tile_kib=args.pull_tile_kib) as cw:
             cw.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"], weights="f32")
             rng = np.random.default_rng(wl["seed"])
             pvec = np.zeros(n, np.float32)
@@ -898,6 +899,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="length of the CPU baseline sample")
     ap.add_argument("--pull-band-kib", type=int, default=0,
                     help="PageRank workloads: source band of the banded pull in KiB (0: off, the library default)")
+    ap.add_argument("--pull-tile-kib", type=int, default=0,
+                    help="tiled pull source window in KiB (0: the library default, negative: the row-class pull)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device (counter calibration for rocprofv3 --pmc) and exit")
     ap.add_argument("--no-adjoint", action="store_true", help="web26: skip the weighted-PageRank forward + adjoint")

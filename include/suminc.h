@@ -85,7 +85,12 @@ typedef struct suminc_params {
                                  renamed to dense positions (by decreasing out-degree; in id order when
                                  banded), once the contributions exceed this many KiB (0: 32768;
                                  negative: never) */
-  int32_t reserved[5];
+  int32_t pull_tile_kib;      /* tiled pull: when the banded pull is off and the contributions exceed 64 MiB,
+                                 a dense round's gather runs one workgroup per tile of destinations (their
+                                 sums in LDS) over source windows of this many KiB of contributions, taken
+                                 in the same order by every workgroup (0: 1024; negative: never; the
+                                 threshold does not apply when set > 0) */
+  int32_t reserved[4];
 } suminc_params;
 
 typedef struct suminc_stats {

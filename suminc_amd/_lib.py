@@ -54,7 +54,8 @@ class Params(C.Structure):
         ("shortcut_max_rounds", C.c_int32),
         ("pull_band_kib", C.c_int32),
         ("pull_compact_kib", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("pull_tile_kib", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 

@@ -212,6 +212,13 @@ struct suminc_ctx {
   // of the n_src non-dangling contributions only
   suminc::DevBuf src_pos, src_pre, in_col_c;
   bool pull_compact = false;  // the positions are in use (contributions above 32 MiB)
+  // tiled pull (per graph version, with the pull lists): tile table (uint4: first
+  // destination, destination count | split flag << 31, CSC edge range), the CSC
+  // edges of each tile grouped by source window and packed into 32 bits, and each
+  // tile's window offsets (tile_P + 1 per tile)
+  suminc::DevBuf tile_tab, tile_edges, tile_seg, tile_bounds, tile_edges2, tile_flat;
+  uint32_t tile_n = 0, tile_P = 0, tile_win_log2 = 0, tile_bsize = 0;
+  int pull_prec = -1;  // the PageRank precision the pull lists, positions and tiles were built for
   uint64_t graph_version = 0;
 
   // SSSP state: packed {dist:hi32, parent:lo32}

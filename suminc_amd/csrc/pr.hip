@@ -901,6 +901,246 @@ static void build_band_tables(suminc_ctx* c, uint64_t m_cap) {
   c->band_S = S;
 }
 
+// ---------------------------------------------------------------------------------
+// Tiled pull.  A dense round reads contrib at random once per in-edge; once the
+// contributions outgrow L2 every such read fills a whole line from the Infinity
+// Cache or HBM (the row-class pull above moves about 7x its algorithmic bytes at
+// 2^26 vertices).  The tiled layout reorders the CSC once per graph version:
+// destinations in blocks of kTileDst, each block's in-edges (cut into tiles of at
+// most kTileEdges when longer) grouped by source window (2^win_log2 contribution
+// slots, sized to a fraction of L2), each edge packed into 32 bits as (destination
+// - block start) << kTileSrcBits | slot within the window.  A round launches one
+// workgroup per tile: it sums its destinations in LDS while it walks the windows
+// in order (every workgroup walks them in the same order, so a window's lines are
+// shared in L2 by the workgroups on an XCD), and adds each destination's sum to
+// delta once at the end (atomically for the tiles of a block cut in several).
+// The edge stream is 4 B per edge instead of the CSC's column and the gather.
+// ---------------------------------------------------------------------------------
+constexpr uint32_t kTileDstBits = 13;
+constexpr uint32_t kTileSrcBits = 32 - kTileDstBits;
+constexpr uint32_t kTileEdges = 16u << kTileDstBits;  // in-edges per tile
+constexpr uint32_t kTileMaxWindows = 1024;
+// destinations per block (LDS sums): 8192 in FP32, 4096 in FP64 (32 KiB either way)
+template <class T>
+constexpr uint32_t kTileAcc = (32u << 10) / sizeof(T);
+inline uint32_t tile_dst(int prec) { return prec == SUMINC_PR_FP64 ? kTileAcc<double> : kTileAcc<float>; }
+
+// in_off at every block start (blocks + 1 values)
+__global__ void k_tile_bounds(uint32_t n, uint32_t blocks, uint32_t bsize, const uint32_t* __restrict__ ioff,
+                              uint32_t* __restrict__ out) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k <= blocks) out[k] = ioff[min((uint64_t)k * bsize, (uint64_t)n)];
+}
+
+// One workgroup per tile: window histogram of the tile's edges, their offsets, then
+// every edge placed in its window's segment (order within a segment: arrival).
+__global__ void __launch_bounds__(kBlock) k_tile_build(const uint4* __restrict__ tiles,
+                                                       const uint32_t* __restrict__ ioff,
+                                                       const uint32_t* __restrict__ icol, uint32_t win_log2,
+                                                       uint32_t P, uint32_t* __restrict__ packed,
+                                                       uint32_t* __restrict__ seg) {
+  __shared__ uint32_t s_pre[kTileMaxWindows + 1];
+  __shared__ uint32_t s_fill[kTileMaxWindows];
+  const uint4 t = tiles[blockIdx.x];
+  const uint32_t d0 = t.x, nd = t.y & 0x7FFFFFFFu, e0 = t.z, e1 = t.w;
+  for (uint32_t i = threadIdx.x; i <= P; i += kBlock) {
+    s_pre[i] = 0;
+    if (i < P) s_fill[i] = 0;
+  }
+  __syncthreads();
+  for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) atomicAdd(&s_pre[icol[e] >> win_log2], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {  // exclusive scan (P <= kTileMaxWindows)
+    uint32_t run = 0;
+    for (uint32_t b = 0; b < P; ++b) {
+      const uint32_t x = s_pre[b];
+      s_pre[b] = run;
+      run += x;
+    }
+    s_pre[P] = run;
+  }
+  __syncthreads();
+  uint32_t* sg = seg + (size_t)blockIdx.x * (P + 1);
+  for (uint32_t b = threadIdx.x; b <= P; b += kBlock) sg[b] = e0 + s_pre[b];
+  const uint32_t mask = (1u << win_log2) - 1u;
+  for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
+    // the row of e: the last v in [d0, d0 + nd) with ioff[v] <= e
+    uint32_t lo = d0, hi = d0 + nd;  // ioff[lo] <= e < ioff[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (ioff[mid] <= e) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t p = icol[e];
+    const uint32_t b = p >> win_log2;
+    const uint32_t slot = atomicAdd(&s_fill[b], 1u);
+    packed[e0 + s_pre[b] + slot] = ((lo - d0) << kTileSrcBits) | (p & mask);
+  }
+}
+
+// segment offsets of every (tile, window) in one array: flat[t * P + b] = the
+// segment start, flat[tiles * P] = the last tile's end (tiles are consecutive)
+__global__ void k_tile_flat(const uint32_t* __restrict__ seg, uint32_t tiles, uint32_t P, uint32_t* __restrict__ flat) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t total = (uint64_t)tiles * P;
+  if (i < total) flat[i] = seg[(i / P) * (P + 1) + i % P];
+  else if (i == total) flat[i] = seg[(uint64_t)(tiles - 1) * (P + 1) + P];
+}
+
+// One workgroup per tile: it walks the windows in order (their offsets held in
+// LDS), four edges a thread at a time: the packed loads, the gathers from the
+// window, the LDS sums.  Each window's run is sorted by slot, so neighbouring lanes
+// read neighbouring contributions.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pr_pull_tiled(const uint4* __restrict__ tiles,
+                                                          const uint32_t* __restrict__ packed,
+                                                          const uint32_t* __restrict__ seg, uint32_t P,
+                                                          uint32_t win_log2, const T* __restrict__ contrib,
+                                                          T* __restrict__ delta, const Counters* cnt, int gate) {
+  __shared__ T s_acc[kTileAcc<T>];
+  __shared__ uint32_t s_seg[kTileMaxWindows + 1];
+  if (!pull_runs(cnt, gate)) return;
+  const uint4 t = tiles[blockIdx.x];
+  const uint32_t d0 = t.x, nd = t.y & 0x7FFFFFFFu;
+  const bool split = (t.y >> 31) != 0;
+  const uint32_t* sg = seg + (size_t)blockIdx.x * (P + 1);
+  for (uint32_t i = threadIdx.x; i <= P; i += kBlock) s_seg[i] = sg[i];
+  for (uint32_t i = threadIdx.x; i < nd; i += kBlock) s_acc[i] = T(0);
+  __syncthreads();
+  const uint32_t mask = (1u << win_log2) - 1u;
+  for (uint32_t b = 0; b < P; ++b) {
+    const uint32_t s = s_seg[b], e = s_seg[b + 1];
+    const T* w = contrib + ((size_t)b << win_log2);
+    for (uint32_t k = s + threadIdx.x; k < e; k += 4 * kBlock) {
+      uint32_t q[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t kk = k + r * kBlock;
+        q[r] = kk < e ? __builtin_nontemporal_load(packed + kk) : 0u;  // streamed: keep the windows in L2
+      }
+      T x[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) x[r] = k + r * kBlock < e ? w[q[r] & mask] : T(0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (x[r] != T(0)) atomicAdd(&s_acc[q[r] >> kTileSrcBits], x[r]);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nd; i += kBlock) {
+    const T x = s_acc[i];
+    if (x != T(0)) {
+      if (split) unsafeAtomicAdd(&delta[d0 + i], x);
+      else delta[d0 + i] += x;
+    }
+  }
+}
+
+// Vertices with out-edges (the compact positions' count), added into *out.
+__global__ void k_count_sources(uint32_t n, const uint32_t* __restrict__ off, unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long s[kBlock / 64];
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  unsigned long long c = v < n && off[v + 1] > off[v] ? 1ull : 0ull;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if (lane_id() == 0) s[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0;
+    for (int w = 0; w < kBlock / 64; ++w) a += s[w];
+    if (a) atomicAdd(out, a);
+  }
+}
+
+// The tiles of the current graph (after the positions and the band tables).
+static void build_tiles(suminc_ctx* c, uint64_t m_cap) {
+  c->tile_n = 0;
+  const int kib = c->prm.pull_tile_kib;
+  const uint64_t es = c->pr_prec == SUMINC_PR_FP64 ? 8 : 4;
+  const uint32_t n = c->n;
+  if (kib < 0 || c->band_P > 1 || n == 0) return;
+  // measured on MI355X (FP32, 1 MiB windows): 7.1-7.3 against 8.6 ms per dense
+  // round at 2^26 vertices, equal at 2^24, 0.67 against 0.38 ms at 2^22
+  if (kib == 0 && (uint64_t)n * es <= (64ull << 20)) return;
+  const uint64_t win_bytes = (uint64_t)(kib ? kib : 1024) * 1024;
+  const uint32_t bsize = tile_dst(c->pr_prec);  // the kernel's LDS sums hold as many destinations
+  const uint32_t blocks = (n + bsize - 1) / bsize;
+  // the sources a column can name: the compact positions' count, or every vertex
+  const size_t src_at = (((size_t)blocks + 1) * 4 + 7) / 8 * 8;  // the count after the bounds, 8-aligned
+  c->tile_bounds.ensure(src_at + 8);
+  unsigned long long* d_src = reinterpret_cast<unsigned long long*>(c->tile_bounds.as<uint8_t>() + src_at);
+  SUMINC_HIP(hipMemsetAsync(d_src, 0, 8, c->stream));
+  if (c->pull_compact)
+    k_count_sources<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), d_src);
+  k_tile_bounds<<<grid_for((uint64_t)blocks + 1), kBlock, 0, c->stream>>>(n, blocks, bsize, c->in_off.as<uint32_t>(),
+                                                                         c->tile_bounds.as<uint32_t>());
+  std::vector<uint32_t> hb(blocks + 1);
+  unsigned long long n_src = n;
+  SUMINC_HIP(hipMemcpyAsync(hb.data(), c->tile_bounds.p, hb.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  if (c->pull_compact) SUMINC_HIP(hipMemcpyAsync(&n_src, d_src, 8, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  n_src = std::max<unsigned long long>(n_src, 1);
+  // the CSC's edge count from the device (during a batch the context's m is the
+  // old graph's until the batch ends)
+  const uint32_t m_now = hb[blocks];
+  if (m_now == 0) return;
+  uint32_t win_log2 = 8;
+  while (win_log2 < kTileSrcBits && (2ull << win_log2) * es <= win_bytes) ++win_log2;
+  while (win_log2 < kTileSrcBits && ((n_src + (1ull << win_log2) - 1) >> win_log2) > kTileMaxWindows) ++win_log2;
+  const uint32_t P = (uint32_t)((n_src + (1ull << win_log2) - 1) >> win_log2);
+  if (P > kTileMaxWindows) return;  // beyond 2^30 sources: the row-class pull
+  std::vector<uint4> tiles;
+  for (uint32_t k = 0; k < blocks; ++k) {
+    const uint32_t e0 = hb[k], e1 = hb[k + 1];
+    if (e1 == e0) continue;
+    const uint32_t d0 = k * bsize, nd = std::min<uint32_t>(bsize, n - d0);
+    const uint32_t parts = (e1 - e0 + kTileEdges - 1) / kTileEdges;
+    for (uint32_t j = 0; j < parts; ++j) {
+      uint4 t;
+      t.x = d0;
+      t.y = nd | (parts > 1 ? 0x80000000u : 0u);
+      t.z = e0 + j * kTileEdges;
+      t.w = std::min<uint32_t>(e1, t.z + kTileEdges);
+      tiles.push_back(t);
+    }
+  }
+  if (tiles.empty()) return;
+  c->tile_tab.ensure(tiles.size() * sizeof(uint4));
+  SUMINC_HIP(hipMemcpyAsync(c->tile_tab.p, tiles.data(), tiles.size() * sizeof(uint4), hipMemcpyHostToDevice,
+                            c->stream));
+  c->tile_edges.ensure_slack(m_cap * 4 + 4);
+  c->tile_seg.ensure_slack(tiles.size() * ((size_t)P + 1) * 4);
+  k_tile_build<<<(unsigned)tiles.size(), kBlock, 0, c->stream>>>(c->tile_tab.as<uint4>(), c->in_off.as<uint32_t>(),
+                                                                 pull_cols(c), win_log2, P,
+                                                                 c->tile_edges.as<uint32_t>(),
+                                                                 c->tile_seg.as<uint32_t>());
+  SUMINC_HIP(hipGetLastError());
+  // each (tile, window) segment sorted by slot, so neighbouring lanes of a round
+  // read neighbouring contributions (one line serves the edges of the hot sources)
+  {
+    const uint64_t T = tiles.size(), nseg = T * P;
+    c->tile_flat.ensure((nseg + 1) * 4);
+    c->tile_edges2.ensure_slack(m_cap * 4 + 4);
+    k_tile_flat<<<grid_for(nseg + 1), kBlock, 0, c->stream>>>(c->tile_seg.as<uint32_t>(), (uint32_t)T, P,
+                                                             c->tile_flat.as<uint32_t>());
+    const uint32_t* flat = c->tile_flat.as<uint32_t>();
+    size_t bytes = 0;
+    SUMINC_HIP(rocprim::segmented_radix_sort_keys(nullptr, bytes, c->tile_edges.as<uint32_t>(),
+                                                  c->tile_edges2.as<uint32_t>(), m_now, (unsigned)nseg,
+                                                  flat, flat + 1, 0u, win_log2, c->stream));
+    c->tmp.ensure(bytes);
+    SUMINC_HIP(rocprim::segmented_radix_sort_keys(c->tmp.p, bytes, c->tile_edges.as<uint32_t>(),
+                                                  c->tile_edges2.as<uint32_t>(), m_now, (unsigned)nseg,
+                                                  flat, flat + 1, 0u, win_log2, c->stream));
+    c->tile_edges.swap(c->tile_edges2);
+  }
+  SUMINC_HIP(hipStreamSynchronize(c->stream));  // `tiles` goes out of scope
+  c->tile_n = (uint32_t)tiles.size();
+  c->tile_P = P;
+  c->tile_win_log2 = win_log2;
+  c->tile_bsize = bsize;
+}
+
 void build_pull_lists(suminc_ctx* c, uint64_t m_cap) {
   const unsigned g = grid_for(c->n);
   c->pull_heavy.ensure_slack((m_cap / kChunk + m_cap / kHeavyDeg + 2) * sizeof(uint2));
@@ -916,10 +1156,12 @@ void build_pull_lists(suminc_ctx* c, uint64_t m_cap) {
   SUMINC_HIP(hipGetLastError());
   build_src_positions(c, m_cap, !banded_pull(c));
   build_band_tables(c, m_cap);
+  build_tiles(c, m_cap);
+  c->pull_prec = c->pr_prec;  // positions, bands and tiles depend on the element size
 }
 
 void ensure_pull_heavy(suminc_ctx* c) {
-  if (c->pull_heavy_version == c->graph_version) return;
+  if (c->pull_heavy_version == c->graph_version && c->pull_prec == c->pr_prec) return;
   build_pull_lists(c, c->m);
   c->pull_heavy_version = c->graph_version;
 }
@@ -932,6 +1174,13 @@ void pull_scatter(suminc_ctx* c, int gate) {
   Counters* cnt = dev_counters(c);
   const unsigned long long* counts = c->pull_cnt.as<unsigned long long>();
   const uint32_t* icol = pull_cols(c);
+  if (c->tile_n) {
+    SUMINC_REQUIRE(c->tile_bsize <= kTileAcc<T>, SUMINC_E_STATE, "tiled pull built for another precision");
+    k_pr_pull_tiled<T><<<c->tile_n, kBlock, 0, c->stream>>>(c->tile_tab.as<uint4>(), c->tile_edges.as<uint32_t>(),
+                                                           c->tile_seg.as<uint32_t>(), c->tile_P, c->tile_win_log2,
+                                                           c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, gate);
+    return;
+  }
   if (c->band_P > 1) {  // every row band by band
     const uint32_t W = (uint32_t)c->band_W, S = (uint32_t)c->band_S;
     for (uint32_t p = 0; p < c->band_P; ++p) {

@@ -43,17 +43,22 @@ def test_golden_test_base(gpu):
 # band_kib=16: the banded pull (source bands of 4096 / 2048 vertices at FP32 /
 # FP64 on 2^16 vertices, over compact positions in id order when compact_kib=1);
 # compact_kib=1 alone: the pull over compact source positions by out-degree, used
-# by default above 32 MiB of contributions (8.4M vertices in FP32)
-@pytest.mark.parametrize("scale,band_kib,compact_kib", [(12, 0, 0), (16, 0, 0), (16, 16, 0), (16, 0, 1), (16, 16, 1)])
+# by default above 32 MiB of contributions (8.4M vertices in FP32); tile_kib: the
+# tiled pull (destination tiles summed in LDS over source windows of 1 or 4 KiB,
+# blocks of the hub rows cut into several tiles that add atomically), the default
+# above 64 MiB of contributions
+@pytest.mark.parametrize("scale,band_kib,compact_kib,tile_kib", [(12, 0, 0, -1), (16, 0, 0, -1), (16, 16, 0, -1),
+                                                                 (16, 0, 1, -1), (16, 16, 1, -1), (16, 0, 0, 1),
+                                                                 (16, 0, 1, 1), (16, 0, 1, 4), (12, 0, 0, 1)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_pagerank_vs_fp64_oracle(gpu, scale, band_kib, compact_kib, prec):
+def test_pagerank_vs_fp64_oracle(gpu, scale, band_kib, compact_kib, tile_kib, prec):
     g = oracle.Graph.rmat(scale, 16, seed=1)
     off, col, _ = g.csr()
     rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
     T = 1e-9 * g.n  # per-vertex threshold 1e-9 (the C1 setting)
     p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
     with gpu.Context(pr_precision=p, termcheck_threshold=T, pull_band_kib=band_kib,
-                     pull_compact_kib=compact_kib) as ctx:
+                     pull_compact_kib=compact_kib, pull_tile_kib=tile_kib) as ctx:
         ctx.upload_csr(off, col)
         ctx.pr_init()
         st = ctx.pr_run()
@@ -112,15 +117,16 @@ def test_step_and_upload(gpu):
     assert np.abs(r - ro).max() < 1e-15 and np.abs(d - do).max() < 1e-15
 
 
-@pytest.mark.parametrize("band_kib,compact_kib", [(0, 0), (16, 0), (0, 1), (16, 1)])
+@pytest.mark.parametrize("band_kib,compact_kib,tile_kib", [(0, 0, -1), (16, 0, -1), (0, 1, -1), (16, 1, -1),
+                                                          (0, 0, 1), (0, 1, 1)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_incremental_reseed_matches_scratch(gpu, prec, band_kib, compact_kib):
+def test_incremental_reseed_matches_scratch(gpu, prec, band_kib, compact_kib, tile_kib):
     g = oracle.Graph.rmat(14, 16, seed=3)
     off, col, _ = g.csr()
     T = 1e-10 * g.n
     p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
     with gpu.Context(pr_precision=p, termcheck_threshold=T, pull_band_kib=band_kib,
-                     pull_compact_kib=compact_kib) as ctx:
+                     pull_compact_kib=compact_kib, pull_tile_kib=tile_kib) as ctx:
         ctx.upload_csr(off, col)
         ctx.pr_init()
         ctx.pr_run()
@@ -201,3 +207,31 @@ def test_oversized_allocation_reports_nomem(gpu):
         ctx.upload_csr(off, col)
         ctx.pr_init()
         assert ctx.pr_run()["residual"] < 1e-9
+
+
+def test_tiled_pull_follows_precision_and_growth(gpu):
+    """The tiled layout is rebuilt when the precision changes (its LDS tile holds
+    8192 FP32 or 4096 FP64 sums) and after batches that grow the graph (the tiles
+    cover the new CSC, not the old edge count)."""
+    g = oracle.Graph.rmat(14, 16, seed=2)
+    off, col, _ = g.csr()
+    T = 1e-10 * g.n
+    with gpu.Context(pr_precision=gpu.PR_FP32_KAHAN, termcheck_threshold=T, pull_tile_kib=1) as ctx:
+        ctx.upload_csr(off, col)
+        for prec in (gpu.PR_FP32_KAHAN, gpu.PR_FP64, gpu.PR_FP32_KAHAN):
+            ctx.set_params(pr_precision=prec)
+            ctx.pr_init()
+            st = ctx.pr_run()
+            r, _ = ctx.pr_download()
+            rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
+            tol = 1e-11 if prec == gpu.PR_FP64 else 2e-6
+            assert np.abs(r - rs).sum() <= st["residual"] / (1 - DAMPING) + tol, prec
+        for b in range(3):  # three times as many inserts as deletes: m grows
+            ops = g.gen_batch(seed=11, batch_id=b, n_del=int(0.005 * g.m), n_ins=int(0.015 * g.m))
+            ctx.apply_batch(ops)
+            g, _ = g.apply_batch(ops)
+            st = ctx.pr_run()
+            r, _ = ctx.pr_download()
+            rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
+            assert np.abs(r - rs).sum() <= st["residual"] / (1 - DAMPING) + 2e-6, b
+        assert ctx.graph_info()[1] > len(col)
