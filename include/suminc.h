@@ -552,6 +552,29 @@ int suminc_clusters_read(const char* path, uint32_t n, uint32_t min_node_num, ui
  * non-empty cluster in id order, members ascending. */
 int suminc_clusters_write(const char* path, uint32_t n, const uint32_t* cluster_of);
 
+/* ---- binary caches of parsed inputs (host only, no device work) ------------------
+ * The reference serializes a loaded fragment once under --serialization_prefix and
+ * reads it back on later runs instead of parsing the text (loader.h:30-61,
+ * SetSerialize).  These calls cache the edge list and the update records in a binary
+ * format of this library (little-endian; an FNV-1a 64 checksum ends each file):
+ *   edges    "SUMINCE1", u32 weight_kind, u32 n, u64 m, u64 skipped lines,
+ *            u64 offsets[n + 1], u32 cols[m], u32 weight bits[m] (weighted graphs only)
+ *   updates  "SUMINCU1", u64 count, count suminc_update records (24 bytes each)
+ * A load checks the magic, the sizes against the file size, the checksum and, for an
+ * edge list, the offsets (0 .. m, never decreasing) and that every column is < n. */
+/* The reference's cache path "<prefix>/<efile + vfile + workers, '/' -> '_'>/frag_<fid>.s"
+ * (loader.h:33-42, config.h:49) into out (cap bytes, NUL-terminated); *length gets its
+ * length; out may be NULL to ask for the length. */
+int suminc_serialization_path(const char* prefix, const char* efile, const char* vfile, uint32_t workers,
+                              uint32_t fid, char* out, uint64_t cap, uint64_t* length);
+int suminc_edges_save(const suminc_edges* edges, const char* path);
+/* Free the result with suminc_edges_free. */
+int suminc_edges_load(const char* path, suminc_edges** out);
+int suminc_updates_save(const char* path, const suminc_update* ops, uint64_t count);
+/* *count gets the record count; the records are written to ops when ops != NULL (cap
+ * must be at least *count). */
+int suminc_updates_load(const char* path, suminc_update* ops, uint64_t cap, uint64_t* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -217,6 +217,11 @@ SIGNATURES = {
     "suminc_pr_correct": (i32, [vp]),
     "suminc_pr_precompute": (i32, [vp]),
     "suminc_traversal_init": (i32, [vp, C.c_int, u32]),
+    "suminc_serialization_path": (i32, [C.c_char_p, C.c_char_p, C.c_char_p, u32, u32, C.c_char_p, u64, vp]),
+    "suminc_edges_save": (i32, [vp, C.c_char_p]),
+    "suminc_edges_load": (i32, [C.c_char_p, vp]),
+    "suminc_updates_save": (i32, [C.c_char_p, vp, u64]),
+    "suminc_updates_load": (i32, [C.c_char_p, vp, u64, vp]),
     "suminc_php_run": (i32, [vp, u32, C.c_double, vp, vp]),
     "suminc_sketch_pr_run": (i32, [vp, C.POINTER(Stats)]),
     "suminc_compress_mirror": (i32, [vp, vp, u32, u32, C.c_double, C.c_double, C.POINTER(CompressInfo)]),

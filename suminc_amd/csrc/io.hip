@@ -30,15 +30,7 @@
 #include <vector>
 
 #include "ctx.hpp"
-
-struct suminc_edges {
-  uint32_t n = 0;
-  int wkind = SUMINC_W_NONE;
-  uint64_t skipped = 0;     // lines that did not parse (skipped, as the reference's loaders do)
-  std::vector<uint64_t> off;
-  std::vector<uint32_t> col;
-  std::vector<uint32_t> w;  // weight bits (u32 or f32), empty without weights
-};
+#include "io.hpp"
 
 namespace suminc {
 
@@ -467,6 +459,8 @@ int host_guarded(F&& f) {
 }  // namespace
 
 const char* io_last_error() { return t_err.c_str(); }
+
+int host_call(const std::function<void()>& f) { return host_guarded(f); }
 
 void upload_edges(suminc_ctx* c, const suminc_edges* e) {
   SUMINC_REQUIRE(e, SUMINC_E_INVALID, "null edge list");

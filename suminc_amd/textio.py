@@ -4,6 +4,8 @@
 * update files   "a u v [w]" / "d u v" per line (grape/fragment/inc_fragment_builder.h:29-72)
 * cluster files  "<size> <id> ... <id>" per cluster, named "<efile>.c_<max_node_num>"
                  (grape/fragment/compressor_base.h:3153-3250)
+* binary caches  of a parsed edge list and of update records, under the reference's
+                 serialization path rule (grape/fragment/loader.h:30-61)
 
 The parsing runs in libsuminc (multi-threaded C++); these wrappers only move the
 results into numpy arrays.  None of them needs a GPU.
@@ -38,12 +40,16 @@ def _path(p) -> bytes:
 
 class EdgeFile:
     """An edge file parsed into a CSR with sorted rows (first line of a repeated pair
-    kept).  n = largest vertex id + 1."""
+    kept).  n = largest vertex id + 1.  cache=True reads the binary cache written by
+    save() instead of a text file."""
 
-    def __init__(self, path, weights=None, directed=True):
+    def __init__(self, path, weights=None, directed=True, cache=False):
         lib = L.lib()
         h = C.c_void_p()
-        _check(lib.suminc_edges_read_tsv(_path(path), _kind(weights), 1 if directed else 0, C.byref(h)))
+        if cache:
+            _check(lib.suminc_edges_load(_path(path), C.byref(h)))
+        else:
+            _check(lib.suminc_edges_read_tsv(_path(path), _kind(weights), 1 if directed else 0, C.byref(h)))
         self._h = h
         n, m, wk, skipped = C.c_uint32(), C.c_uint64(), C.c_int(), C.c_uint64()
         _check(lib.suminc_edges_info(h, C.byref(n), C.byref(m), C.byref(wk), C.byref(skipped)))
@@ -60,6 +66,10 @@ class EdgeFile:
             w = np.zeros(self.m, np.float32)
         _check(L.lib().suminc_edges_copy(self._h, L.ptr(off), L.ptr(col), L.ptr(w)))
         return off, col, w
+
+    def save(self, path) -> None:
+        """The binary cache of this edge list (suminc_edges_save)."""
+        _check(L.lib().suminc_edges_save(self._h, _path(path)))
 
     def upload(self, ctx) -> None:
         """suminc_edges_upload: the parsed CSR becomes the context's graph."""
@@ -92,6 +102,33 @@ def read_updates(path, weights=None, directed=True) -> np.ndarray:
     ops = np.zeros(cnt.value, L.UPDATE_DTYPE)
     _check(lib.suminc_updates_read_tsv(_path(path), kind, 1 if directed else 0, L.ptr(ops), cnt.value, C.byref(cnt)))
     return ops
+
+
+def save_updates(path, ops) -> None:
+    """The binary cache of update records (suminc_updates_save)."""
+    a = np.ascontiguousarray(ops, L.UPDATE_DTYPE)
+    _check(L.lib().suminc_updates_save(_path(path), L.ptr(a), len(a)))
+
+
+def load_updates(path) -> np.ndarray:
+    """Update records from a binary cache (suminc_updates_load)."""
+    lib = L.lib()
+    cnt = C.c_uint64()
+    _check(lib.suminc_updates_load(_path(path), None, 0, C.byref(cnt)))
+    ops = np.zeros(cnt.value, L.UPDATE_DTYPE)
+    _check(lib.suminc_updates_load(_path(path), L.ptr(ops), cnt.value, C.byref(cnt)))
+    return ops
+
+
+def serialization_path(prefix, efile, vfile="", workers=1, fid=0) -> str:
+    """The reference's cache path "<prefix>/<efile + vfile + workers, '/' -> '_'>/frag_<fid>.s"."""
+    n = C.c_uint64()
+    lib = L.lib()
+    _check(lib.suminc_serialization_path(_path(prefix), _path(efile), _path(vfile), workers, fid, None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    _check(lib.suminc_serialization_path(_path(prefix), _path(efile), _path(vfile), workers, fid, buf, n.value + 1,
+                                         C.byref(n)))
+    return os.fsdecode(buf.value)
 
 
 def read_clusters(path, n: int, min_node_num: int = 2):
