@@ -440,6 +440,21 @@ typedef struct suminc_sketch_buffers {
   uint32_t* sub_col;            /* subgraph_edges */
 } suminc_sketch_buffers;
 int suminc_sketch_download(suminc_ctx* ctx, suminc_sketch_buffers* buffers);
+/* The cluster cache (compress_by_cluster writes it on a first run and reads it on
+ * later ones, compressor_base.h:3160-3170, 3243-3249): the accepted clusters of the
+ * last suminc_compress_mirror, before the final build, in the reference's layout
+ * (serialize_cluster, :3081-3150; little-endian): u32 init_mirror_num, u32 cluster
+ * count; per cluster five lists of u32 count + u32 ids (supernode_ids, cluster_ids,
+ * supernode_source, supernode_in_mirror, supernode_out_mirror); i32 Fc[n] (-(cluster
+ * + 1) for members, INT32_MAX otherwise); u32 id2spids[n] (cluster or 0xFFFFFFFF);
+ * i32 -1.  The mirror sets are written in ascending order (the reference's hash
+ * order is arbitrary). */
+int suminc_compress_mirror_save(suminc_ctx* ctx, const char* path);
+/* Reads such a file (de_serialize_cluster, :2987-3079), checks it against itself
+ * (lists, Fc, id2spids, the mirror count, the end marker) and runs the final build,
+ * the shortcut rows and the upload from it: the products of suminc_compress_mirror
+ * without its acceptance pass. */
+int suminc_compress_mirror_load(suminc_ctx* ctx, const char* path, suminc_compress_info* info);
 
 /* ---- weighted personalized PageRank and its adjoint ------------------------------
  * No counterpart in the reference (its nearest app is the weighted personalized
@@ -551,6 +566,20 @@ int suminc_clusters_read(const char* path, uint32_t n, uint32_t min_node_num, ui
 /* Writes cluster_of (ids < n or SUMINC_INF_U32) in the same format: one line per
  * non-empty cluster in id order, members ascending. */
 int suminc_clusters_write(const char* path, uint32_t n, const uint32_t* cluster_of);
+/* Community clustering for the cluster file, host only: a deterministic stand-in for
+ * the reference's offline Louvain pipeline (sh/louvain_all.sh:24-49), finding
+ * communities rather than the leaves of hubs.  Labels start as vertex ids; each of
+ * `rounds` synchronous rounds gives every vertex with at most max_degree neighbours
+ * (out- and in-edges, an edge counting once for each endpoint) the label most
+ * frequent among them (a tie keeps the current label when it is among the most
+ * frequent, else takes the smallest); then every label class, in vertex id order, is
+ * cut into pieces of at most max_node_num members, pieces below min_node_num are
+ * dropped (SUMINC_INF_U32), and the clusters are numbered by their smallest member.
+ * The result does not depend on the thread count; it goes to suminc_set_clusters,
+ * suminc_compress_mirror or suminc_clusters_write. */
+int suminc_cluster_lpa(uint32_t n, const uint64_t* offsets, const uint32_t* cols, uint32_t max_node_num,
+                       uint32_t min_node_num, uint32_t rounds, uint32_t max_degree, uint32_t* cluster_of,
+                       uint32_t* n_clusters);
 
 /* ---- binary caches of parsed inputs (host only, no device work) ------------------
  * The reference serializes a loaded fragment once under --serialization_prefix and

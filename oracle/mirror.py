@@ -118,6 +118,9 @@ def compress(off, col, cluster_of, min_node=2, max_node=1024, threshold=1.0, mir
             fc[u] = -1
             id2sp[u] = cid
     K = len(members)
+    # the accepted clusters before the final build (the cluster cache's content)
+    acc = {"members": [list(P) for P in members], "sources": [list(S) for S in sources],
+           "in": [sorted(x) for x in in_m], "out": [sorted(x) for x in out_m]}
     # final_build_supernode: absorption, mirror ids, supernode sources
     all_n = n
     master = []
@@ -215,7 +218,30 @@ def compress(off, col, cluster_of, min_node=2, max_node=1024, threshold=1.0, mir
     return {"n": n, "all_node_num": all_n, "K": K, "members": members, "ids": ids, "in_m": in_m, "out_m": out_m,
             "in_ids": in_ids, "out_ids": out_ids, "master": master, "cluster_of": cl_of, "sub": sub,
             "out_bound": out_bound, "node_type": typ, "sources": src_list, "source_cluster": src_cluster,
-            "out": out}
+            "out": out, "acceptance": acc}
+
+
+def cluster_cache_bytes(st):
+    """serialize_cluster (compressor_base.h:3081-3150) of the accepted clusters,
+    little-endian, lists ascending: u32 init_mirror_num, u32 K, per cluster five
+    lists (supernode_ids, cluster_ids, supernode_source, in-mirror masters, out-mirror
+    masters) as u32 count + ids, i32 Fc[n] (-(cluster + 1) for members, INT32_MAX
+    otherwise), u32 id2spids[n] (cluster or 0xFFFFFFFF), i32 -1."""
+    a = st["acceptance"]
+    n = st["n"]
+    K = len(a["members"])
+    words = [sum(len(x) for x in a["in"]) + sum(len(x) for x in a["out"]), K]
+    fc = np.full(n, 2**31 - 1, np.int64)
+    sp = np.full(n, INF, np.int64)
+    for i in range(K):
+        for lst in (a["members"][i], a["members"][i], a["sources"][i], a["in"][i], a["out"][i]):
+            words.append(len(lst))
+            words.extend(lst)
+        for u in a["members"][i]:
+            fc[u] = -(i + 1)
+            sp[u] = i
+    out = np.array(words, "<u4").tobytes() + fc.astype("<i4").tobytes() + sp.astype("<u4").tobytes()
+    return out + np.array([-1], "<i4").tobytes()
 
 
 def shortcut_rows(st, d):

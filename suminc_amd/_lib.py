@@ -217,6 +217,9 @@ SIGNATURES = {
     "suminc_pr_correct": (i32, [vp]),
     "suminc_pr_precompute": (i32, [vp]),
     "suminc_traversal_init": (i32, [vp, C.c_int, u32]),
+    "suminc_cluster_lpa": (i32, [u32, vp, vp, u32, u32, u32, u32, vp, vp]),
+    "suminc_compress_mirror_save": (i32, [vp, C.c_char_p]),
+    "suminc_compress_mirror_load": (i32, [vp, C.c_char_p, vp]),
     "suminc_serialization_path": (i32, [C.c_char_p, C.c_char_p, C.c_char_p, u32, u32, C.c_char_p, u64, vp]),
     "suminc_edges_save": (i32, [vp, C.c_char_p]),
     "suminc_edges_load": (i32, [C.c_char_p, vp]),

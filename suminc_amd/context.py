@@ -6,6 +6,7 @@ This is plumbing for tests, the benchmark and the autograd wrapper
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -269,6 +270,17 @@ class Context:
         info = L.CompressInfo()
         self._check(self._lib.suminc_compress_mirror(self._h, L.ptr(c), min_node_num, max_node_num,
                                                      float(compress_threshold), float(mirror_k), C.byref(info)))
+        self._compress_info = info.as_dict()
+        return self._compress_info
+
+    def compress_mirror_save(self, path):
+        """The accepted clusters of the last compress_mirror, in the reference's cluster cache layout."""
+        self._check(self._lib.suminc_compress_mirror_save(self._h, os.fsencode(os.fspath(path))))
+
+    def compress_mirror_load(self, path) -> dict:
+        """compress_mirror's products from a cluster cache (no acceptance pass)."""
+        info = L.CompressInfo()
+        self._check(self._lib.suminc_compress_mirror_load(self._h, os.fsencode(os.fspath(path)), C.byref(info)))
         self._compress_info = info.as_dict()
         return self._compress_info
 

@@ -334,6 +334,14 @@ int suminc_compress_mirror(suminc_ctx* c, const uint32_t* cluster_of, uint32_t m
   });
 }
 
+int suminc_compress_mirror_save(suminc_ctx* c, const char* path) {
+  return guarded(c, [&] { compress_mirror_save(c, path); });
+}
+
+int suminc_compress_mirror_load(suminc_ctx* c, const char* path, suminc_compress_info* info) {
+  return guarded(c, [&] { compress_mirror_load(c, path, info); });
+}
+
 int suminc_sketch_download(suminc_ctx* c, suminc_sketch_buffers* buffers) {
   return guarded(c, [&] { sketch_csr_download(c, buffers); });
 }

@@ -20,7 +20,10 @@
 // numbered cluster by cluster, in-mirrors before out-mirrors, by master id.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <unordered_map>
 #include <vector>
 
@@ -67,8 +70,16 @@ struct MirrorSketch {
   uint64_t in_mirrors = 0, out_mirrors = 0, absorbed = 0;
 };
 
-static void compress_host(const HostGraph& g, const uint32_t* cluster_of, uint32_t min_node, uint32_t max_node,
-                          double threshold, double mirror_k, MirrorSketch& S) {
+// The accepted clusters before the final build: what the reference writes to its
+// cluster cache (serialize_cluster, compressor_base.h:3081-3150).
+struct Acceptance {
+  std::vector<std::vector<uint32_t>> members;       // supernode_ids (= cluster_ids here), ascending
+  std::vector<std::vector<uint32_t>> sources;       // supernode_source
+  std::vector<std::vector<uint32_t>> in_m, out_m;   // supernode_in_mirror / _out_mirror (masters, ascending)
+};
+
+static void accept_clusters(const HostGraph& g, const uint32_t* cluster_of, uint32_t min_node, uint32_t max_node,
+                            double threshold, double mirror_k, Acceptance& A) {
   const uint32_t n = g.n;
   const bool use_mirror = mirror_k != 1e8;  // the reference's switch (compressor_base.h:3302)
   const long long k = (long long)mirror_k;
@@ -89,8 +100,6 @@ static void compress_host(const HostGraph& g, const uint32_t* cluster_of, uint32
     }
   // acceptance with mirrors, cluster by cluster (compressor_base.h:3316-3600)
   std::vector<uint32_t> id2sp(n, kNone);        // accepted cluster so far
-  std::vector<int8_t> fc(n, 0);                 // 0: FC_default, -1: member, 1: entry (Fc >= 0)
-  std::vector<std::vector<uint32_t>> src_sets;  // S per accepted cluster
   for (uint32_t j = 0; j < clusters.size(); ++j) {
     const std::vector<uint32_t>& P = clusters[j];
     std::unordered_map<uint32_t, long long> in_freq, out_freq;
@@ -103,7 +112,7 @@ static void compress_host(const HostGraph& g, const uint32_t* cluster_of, uint32
         if (id2cl[w] != j) {
           if (use_mirror) {
             const uint32_t sp = id2sp[w];
-            if (sp == kNone || !has(S.out_m[sp], u)) in_freq[w] += 1;
+            if (sp == kNone || !has(A.out_m[sp], u)) in_freq[w] += 1;
           }
           ext_in = true;
         } else {
@@ -115,7 +124,7 @@ static void compress_host(const HostGraph& g, const uint32_t* cluster_of, uint32
         if (id2cl[w] != j) {
           if (use_mirror) {
             const uint32_t sp = id2sp[w];
-            if (sp == kNone || !has(S.in_m[sp], u)) out_freq[w] += 1;
+            if (sp == kNone || !has(A.in_m[sp], u)) out_freq[w] += 1;
           }
           ext_out = true;
         }
@@ -167,16 +176,31 @@ static void compress_host(const HostGraph& g, const uint32_t* cluster_of, uint32
       Sn = old_entry;
       in_mirror.clear();
     }
-    const uint32_t id = (uint32_t)S.members.size();
-    S.members.push_back(P);
-    src_sets.push_back(Sn);
-    S.in_m.push_back(in_mirror);
-    S.out_m.push_back(out_mirror);
-    for (uint32_t u : P) {
-      fc[u] = -1;
-      id2sp[u] = id;
-    }
+    const uint32_t id = (uint32_t)A.members.size();
+    A.members.push_back(P);
+    A.sources.push_back(Sn);
+    A.in_m.push_back(in_mirror);
+    A.out_m.push_back(out_mirror);
+    for (uint32_t u : P) id2sp[u] = id;
   }
+}
+
+// The final build from the accepted clusters (final_build_supernode and what
+// follows it, compressor_base.h:3616-4182).
+static void final_build(const HostGraph& g, const Acceptance& A, MirrorSketch& S) {
+  const uint32_t n = g.n;
+  S = MirrorSketch();
+  S.members = A.members;
+  S.in_m = A.in_m;
+  S.out_m = A.out_m;
+  std::vector<std::vector<uint32_t>> src_sets = A.sources;
+  std::vector<uint32_t> id2sp(n, kNone);
+  std::vector<int8_t> fc(n, 0);  // 0: FC_default, -1: member, 1: supernode source (Fc >= 0)
+  for (uint32_t i = 0; i < A.members.size(); ++i)
+    for (uint32_t u : A.members[i]) {
+      fc[u] = -1;
+      id2sp[u] = i;
+    }
   const uint32_t K = (uint32_t)S.members.size();
   S.K = K;
   S.n = n;
@@ -570,6 +594,7 @@ static void sketch_rows(const HostGraph& g, const MirrorSketch& S, const std::ve
 
 // The compressor's state kept on the context for suminc_sketch_download.
 struct MirrorState {
+  Acceptance acc;
   MirrorSketch S;
   SketchRows rows;
 };
@@ -578,6 +603,8 @@ void mirror_release(suminc_ctx* c) {
   delete reinterpret_cast<MirrorState*>(c->msk);
   c->msk = nullptr;
 }
+
+static void compress_finish(suminc_ctx* c, const HostGraph& g, MirrorState* M, suminc_compress_info* info);
 
 void compress_mirror(suminc_ctx* c, const uint32_t* cluster_of, uint32_t min_node, uint32_t max_node,
                      double threshold, double mirror_k, suminc_compress_info* info) {
@@ -592,8 +619,141 @@ void compress_mirror(suminc_ctx* c, const uint32_t* cluster_of, uint32_t min_nod
   MirrorState* M = new MirrorState();
   mirror_release(c);
   c->msk = M;
+  accept_clusters(g, cluster_of, min_node, max_node, threshold, mirror_k, M->acc);
+  final_build(g, M->acc, M->S);
+  compress_finish(c, g, M, info);
+}
+
+// ---- the cluster cache in the reference's layout ---------------------------------
+// serialize_cluster / de_serialize_cluster (compressor_base.h:2987-3150), all
+// little-endian: u32 init_mirror_num (in- plus out-mirrors), u32 cluster count; per
+// cluster five lists, each u32 count + u32 ids: supernode_ids, cluster_ids (equal at
+// this point), supernode_source, supernode_in_mirror, supernode_out_mirror (the
+// masters); then i32 Fc[n] (-(cluster + 1) for members, INT32_MAX otherwise), u32
+// id2spids[n] (cluster, or 0xFFFFFFFF), i32 -1.  The reference writes the mirror sets
+// in hash order; this writer writes them ascending.
+void compress_mirror_save(suminc_ctx* c, const char* path) {
+  SUMINC_REQUIRE(c->msk, SUMINC_E_STATE, "no compression on this context (suminc_compress_mirror)");
+  SUMINC_REQUIRE(path, SUMINC_E_INVALID, "null path");
+  const Acceptance& A = reinterpret_cast<MirrorState*>(c->msk)->acc;
+  const uint32_t n = c->n, K = (uint32_t)A.members.size();
+  std::vector<uint32_t> out;
+  uint64_t mirrors = 0;
+  for (uint32_t i = 0; i < K; ++i) mirrors += A.in_m[i].size() + A.out_m[i].size();
+  out.push_back((uint32_t)mirrors);
+  out.push_back(K);
+  auto list = [&](const std::vector<uint32_t>& l) {
+    out.push_back((uint32_t)l.size());
+    out.insert(out.end(), l.begin(), l.end());
+  };
+  std::vector<int32_t> fc(n, INT32_MAX);
+  std::vector<uint32_t> sp(n, kNone);
+  for (uint32_t i = 0; i < K; ++i) {
+    list(A.members[i]);
+    list(A.members[i]);
+    list(A.sources[i]);
+    list(A.in_m[i]);
+    list(A.out_m[i]);
+    for (uint32_t u : A.members[i]) {
+      fc[u] = -(int32_t)(i + 1);
+      sp[u] = i;
+    }
+  }
+  FILE* f = std::fopen(path, "wb");
+  SUMINC_REQUIRE(f, SUMINC_E_INVALID, std::string("cannot open ") + path);
+  const int32_t eof = -1;
+  bool ok = std::fwrite(out.data(), 4, out.size(), f) == out.size();
+  ok = ok && std::fwrite(fc.data(), 4, n, f) == n;
+  ok = ok && std::fwrite(sp.data(), 4, n, f) == n;
+  ok = ok && std::fwrite(&eof, 4, 1, f) == 1;
+  ok = (std::fclose(f) == 0) && ok;
+  SUMINC_REQUIRE(ok, SUMINC_E_INVALID, std::string("write failed: ") + path);
+}
+
+void compress_mirror_load(suminc_ctx* c, const char* path, suminc_compress_info* info) {
+  SUMINC_REQUIRE(c->has_graph, SUMINC_E_STATE, "no graph");
+  SUMINC_REQUIRE(path, SUMINC_E_INVALID, "null path");
+  const uint32_t n = c->n;
+  FILE* f = std::fopen(path, "rb");
+  SUMINC_REQUIRE(f, SUMINC_E_INVALID, std::string("cannot open ") + path);
+  std::vector<uint32_t> words;
+  {
+    uint32_t buf[4096];
+    size_t got;
+    while ((got = std::fread(buf, 4, 4096, f)) > 0) words.insert(words.end(), buf, buf + got);
+    std::fclose(f);
+  }
+  size_t at = 0;
+  auto next = [&]() -> uint32_t {
+    SUMINC_REQUIRE(at < words.size(), SUMINC_E_INVALID, std::string("truncated cluster cache: ") + path);
+    return words[at++];
+  };
+  Acceptance A;
+  const uint32_t mirrors = next(), K = next();
+  SUMINC_REQUIRE(K <= n, SUMINC_E_INVALID, "cluster cache: more clusters than vertices");
+  std::vector<uint32_t> owner(n, kNone);
+  uint64_t mirror_count = 0;
+  auto list = [&](std::vector<uint32_t>& l) {
+    const uint32_t cnt = next();
+    SUMINC_REQUIRE(cnt <= n, SUMINC_E_INVALID, "cluster cache: list longer than the vertex count");
+    l.resize(cnt);
+    for (uint32_t k = 0; k < cnt; ++k) {
+      l[k] = next();
+      SUMINC_REQUIRE(l[k] < n, SUMINC_E_INVALID, "cluster cache: vertex id out of range");
+    }
+    std::sort(l.begin(), l.end());
+    SUMINC_REQUIRE(std::adjacent_find(l.begin(), l.end()) == l.end(), SUMINC_E_INVALID,
+                   "cluster cache: repeated id in a list");
+  };
+  A.members.resize(K);
+  A.sources.resize(K);
+  A.in_m.resize(K);
+  A.out_m.resize(K);
+  for (uint32_t i = 0; i < K; ++i) {
+    std::vector<uint32_t> cl;
+    list(A.members[i]);
+    list(cl);
+    list(A.sources[i]);
+    list(A.in_m[i]);
+    list(A.out_m[i]);
+    SUMINC_REQUIRE(cl == A.members[i], SUMINC_E_INVALID, "cluster cache: cluster_ids differ from supernode_ids");
+    for (uint32_t u : A.members[i]) {
+      SUMINC_REQUIRE(owner[u] == kNone, SUMINC_E_INVALID, "cluster cache: a vertex in two clusters");
+      owner[u] = i;
+    }
+    mirror_count += A.in_m[i].size() + A.out_m[i].size();
+  }
+  for (uint32_t i = 0; i < K; ++i) {
+    for (uint32_t s : A.sources[i])
+      SUMINC_REQUIRE(owner[s] == i, SUMINC_E_INVALID, "cluster cache: a source outside its cluster");
+    for (uint32_t v : A.in_m[i])
+      SUMINC_REQUIRE(owner[v] != i, SUMINC_E_INVALID, "cluster cache: a mirror of its own cluster");
+    for (uint32_t v : A.out_m[i])
+      SUMINC_REQUIRE(owner[v] != i, SUMINC_E_INVALID, "cluster cache: a mirror of its own cluster");
+  }
+  SUMINC_REQUIRE(mirror_count == mirrors, SUMINC_E_INVALID, "cluster cache: init_mirror_num differs from the lists");
+  for (uint32_t v = 0; v < n; ++v) {
+    const int32_t fc = (int32_t)next();
+    SUMINC_REQUIRE(owner[v] == kNone ? fc == INT32_MAX : fc == -(int32_t)(owner[v] + 1), SUMINC_E_INVALID,
+                   "cluster cache: Fc differs from the lists");
+  }
+  for (uint32_t v = 0; v < n; ++v)
+    SUMINC_REQUIRE(next() == owner[v], SUMINC_E_INVALID, "cluster cache: id2spids differs from the lists");
+  SUMINC_REQUIRE(next() == 0xFFFFFFFFu && at == words.size(), SUMINC_E_INVALID,
+                 "cluster cache: missing end marker or trailing data");
+  HostGraph g;
+  download_graph(c, g);
+  MirrorState* M = new MirrorState();
+  mirror_release(c);
+  c->msk = M;
+  M->acc = std::move(A);
+  final_build(g, M->acc, M->S);
+  compress_finish(c, g, M, info);
+}
+
+// the device shortcut rows, the CSR products and their upload; the info
+static void compress_finish(suminc_ctx* c, const HostGraph& g, MirrorState* M, suminc_compress_info* info) {
   MirrorSketch& S = M->S;
-  compress_host(g, cluster_of, min_node, max_node, threshold, mirror_k, S);
   // full out-degrees (a mirror: its master's) and the device rows
   std::vector<uint32_t> outdeg(S.all);
   for (uint32_t v = 0; v < S.n; ++v) outdeg[v] = g.outdeg(v);

@@ -44,6 +44,8 @@ void pull_scatter(suminc_ctx* c, int gate);
 // sssp.hip
 void sssp_init(suminc_ctx* c, uint32_t source);
 void traversal_init(suminc_ctx* c, int app, uint32_t source);
+void compress_mirror_save(suminc_ctx* c, const char* path);
+void compress_mirror_load(suminc_ctx* c, const char* path, suminc_compress_info* info);
 void php_run(suminc_ctx* c, uint32_t source, double damping, double* values, suminc_stats* st);
 void sssp_run(suminc_ctx* c, suminc_stats* st);
 void sssp_download(suminc_ctx* c, uint32_t* dist, uint32_t* parent);

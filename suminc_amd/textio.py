@@ -144,6 +144,18 @@ def write_clusters(path, cluster_of: np.ndarray) -> None:
     _check(L.lib().suminc_clusters_write(_path(path), a.size, L.ptr(a)))
 
 
+def cluster_lpa(offsets, cols, max_node_num=1024, min_node_num=2, rounds=10, max_degree=4096):
+    """Community clustering of a CSR (suminc_cluster_lpa): (cluster_of u32[n], count)."""
+    off = np.ascontiguousarray(offsets, np.uint64)
+    col = np.ascontiguousarray(cols, np.uint32)
+    n = len(off) - 1
+    out = np.zeros(n, np.uint32)
+    k = C.c_uint32()
+    _check(L.lib().suminc_cluster_lpa(n, L.ptr(off), L.ptr(col), max_node_num, min_node_num, rounds, max_degree,
+                                      L.ptr(out), C.byref(k)))
+    return out, k.value
+
+
 def cluster_file_path(efile: str, max_node_num: int) -> str:
     """The cluster file the reference's compressor reads for an edge file:
     efile + ".c_" + max_node_num, with "_w.", "_ud.", "_w1." and ".random." folded

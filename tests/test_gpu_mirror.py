@@ -286,3 +286,62 @@ def test_sssp_step_equals_dijkstra(gpu, weights):
     exp, _ = g.dijkstra(0)
     assert np.array_equal(dist, exp)
     assert steps > 3
+
+
+@pytest.mark.parametrize("name,k", [("hand", 1.0), ("planted1", 4.0), ("planted2", 1.0)])
+def test_cluster_cache_equals_oracle_and_reloads(gpu, tmp_path, name, k):
+    """suminc_compress_mirror_save writes the oracle's bytes (the reference's layout);
+    suminc_compress_mirror_load on a fresh context gives the same products."""
+    g, cl = GRAPHS[name]()
+    off, col, _ = g.csr()
+    st = M.compress(off, col, cl, 2, 1024, 1.0, k)
+    path = tmp_path / "clusters.bin"
+    with gpu.Context(termcheck_threshold=1e-13, shortcut_max_rounds=20000) as ctx:
+        ctx.upload_csr(off, col)
+        ctx.pr_init()
+        info = ctx.compress_mirror(cl, 2, 1024, 1.0, k)
+        sk = ctx.sketch_download()
+        ctx.compress_mirror_save(path)
+    assert path.read_bytes() == M.cluster_cache_bytes(st)
+    with gpu.Context(termcheck_threshold=1e-13, shortcut_max_rounds=20000) as ctx:
+        ctx.upload_csr(off, col)
+        ctx.pr_init()
+        info2 = ctx.compress_mirror_load(path)
+        sk2 = ctx.sketch_download()
+        # a damaged cache is refused
+        raw = path.read_bytes()
+        bad = tmp_path / "bad.bin"
+        for blob in (raw[:-4], raw[:8] + (int.from_bytes(raw[8:12], "little") + 1).to_bytes(4, "little") + raw[12:],
+                     raw[:-8 - 4 * g.n] + (5).to_bytes(4, "little") + raw[-4 - 4 * g.n:]):
+            bad.write_bytes(blob)
+            with pytest.raises(gpu.SumincError) as e:
+                ctx.compress_mirror_load(bad)
+            assert e.value.code == gpu.E_INVALID
+    assert {k2: v for k2, v in info2.items() if k2 != "shortcut_work"} == \
+        {k2: v for k2, v in info.items() if k2 != "shortcut_work"}
+    for key, v in sk.items():
+        if key.endswith("_w"):
+            assert np.allclose(sk2[key], v, rtol=1e-6, atol=1e-12), key
+        else:
+            assert np.array_equal(np.asarray(sk2[key]), np.asarray(v)), key
+
+
+def test_community_clusters_through_the_query(gpu):
+    """Label-propagation clusters (suminc_cluster_lpa) into the compressor with
+    mirrors and the whole query: the plain fixed point."""
+    from suminc_amd import textio
+
+    g, _ = planted(1)
+    off, col, _ = g.csr()
+    cl, k = textio.cluster_lpa(off, col, 1024, 2, 10)
+    assert k >= 8
+    T = 1e-10 * g.n
+    rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
+    with gpu.Context(pr_precision=gpu.PR_FP64, termcheck_threshold=T) as ctx:
+        ctx.upload_csr(off, col)
+        ctx.pr_init()
+        info = ctx.compress_mirror(cl, 2, 1024, 1.0, 4.0)
+        st = ctx.sketch_pr_run()
+        r, _ = ctx.pr_download()
+    assert info["clusters"] > 0
+    assert np.abs(r - rs).sum() <= (st["residual"] + 1e-3 * T) / (1 - DAMPING) + 1e-11

@@ -259,3 +259,22 @@ def test_sketch_iteration_catches_a_wrong_row():
     r, _, _ = g.pr_solve(DAMPING, tol=1e-15, r=rank, delta=delta)
     rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
     assert np.abs(r - rs).sum() > 1e-9
+
+
+def test_hand_example_cluster_cache_bytes():
+    """The cluster cache of the hand example, word by word: 5 mirrors (A: in 4, 6, 7;
+    out 5, 8), A = {0..3} with no sources, B = {4, 5, 6} with sources 4, 5; Fc and
+    id2spids for the ten vertices; the end marker."""
+    import struct
+    g, cl = hand_graph()
+    off, col, _ = g.csr()
+    st = M.compress(off, col, cl, 2, 1024, 1.0, 1.0)
+    words = [5, 2,
+             4, 0, 1, 2, 3, 4, 0, 1, 2, 3, 0, 3, 4, 6, 7, 2, 5, 8,
+             3, 4, 5, 6, 3, 4, 5, 6, 2, 4, 5, 0, 0]
+    m = 2**31 - 1
+    fc = [-1, -1, -1, -1, -2, -2, -2, m, m, m]
+    sp = [0, 0, 0, 0, 1, 1, 1, INF, INF, INF]
+    want = struct.pack(f"<{len(words)}I", *words) + struct.pack("<10i", *fc) + struct.pack("<10I", *sp)
+    want += struct.pack("<i", -1)
+    assert M.cluster_cache_bytes(st) == want
