@@ -75,6 +75,23 @@ WORKLOADS = {
 }
 
 
+def tiled_pull(args, n) -> bool:
+    """Whether the library's dense PageRank rounds run the tiled pull (its rule: a
+    window is given, or the FP32 contributions exceed 64 MiB and it is not off)."""
+    return args.pull_band_kib <= 0 and (args.pull_tile_kib > 0 or (args.pull_tile_kib == 0 and n * 4 > (64 << 20)))
+
+
+def name_pull(args, n, kernel, roof, table):
+    """The profiled pull launches are the tiled kernel's when the tiled pull runs."""
+    if not tiled_pull(args, n):
+        return kernel
+    if roof is not None and roof.get("kernel") == "k_pr_pull":
+        roof["kernel"] = "k_pr_pull_tiled"
+    if table and "k_pr_pull" in table:
+        table["k_pr_pull_tiled"] = table.pop("k_pr_pull")
+    return "k_pr_pull_tiled" if kernel == "k_pr_pull" else kernel
+
+
 def kernel_names():
     import suminc_amd._lib as L
 
@@ -190,6 +207,7 @@ def run_sssp(args, wl):
     kids = [L.KERNEL_SSSP_SKETCH if sketch else L.KERNEL_SSSP_PUSH, L.KERNEL_BATCH_COPY]
     kernel, roof = roofline(ctx, kids, args.workload)
     extra["kernels"] = kernel_table(ctx, kids)
+    kernel = name_pull(args, ctx.graph_info()[0], kernel, roof, extra["kernels"])
     ctx.profile_enable(False)
     for d in dev:
         d.free()
@@ -310,6 +328,7 @@ def run_pr(args, wl):
     kids = [L.KERNEL_PR_PUSH, L.KERNEL_PR_PULL, L.KERNEL_PR_GATHER, L.KERNEL_BATCH_COPY]
     kernel, roof = roofline(ctx, kids, args.workload)
     ktab = kernel_table(ctx, kids)
+    kernel = name_pull(args, ctx.graph_info()[0], kernel, roof, ktab)
     ctx.profile_enable(False)
     ms_per_step = 1e3 * dt / args.steps
     gteps = edges / dt / 1e9
@@ -508,6 +527,7 @@ def run_web(args, wl):
     kids = [L.KERNEL_PR_PUSH, L.KERNEL_PR_PULL, L.KERNEL_PR_GATHER, L.KERNEL_SSSP_SKETCH, L.KERNEL_BATCH_COPY]
     kernel, roof = roofline(ctx, kids, args.workload)
     extra["kernels"] = kernel_table(ctx, kids)
+    kernel = name_pull(args, ctx.graph_info()[0], kernel, roof, extra["kernels"])
     ctx.profile_enable(False)
     ms_per_step = 1e3 * dt / args.steps
     gteps = edges / dt / 1e9
@@ -607,7 +627,7 @@ def run_web(args, wl):
             log(f"adjoint: {extra['adjoint']}")
             del gw
     if roof is not None and not args.no_traffic:
-        sub = {"k_pr_pull": "k_pr_pull<float>"}.get(kernel)
+        sub = {"k_pr_pull": "k_pr_pull<float>", "k_pr_pull_tiled": "k_pr_pull_tiled<float>"}.get(kernel)
         if sub is None:
             roof["traffic_note"] = f"no live counter run for {kernel}"
         else:
@@ -732,6 +752,7 @@ def run_prfull(args, wl):
     kids = [L.KERNEL_PR_PUSH, L.KERNEL_PR_PULL, L.KERNEL_PR_GATHER]
     kernel, roof = roofline(ctx, kids, args.workload)
     extra = {"kernels": kernel_table(ctx, kids), "rounds": st["rounds"], "termcheck_threshold": T}
+    kernel = name_pull(args, ctx.graph_info()[0], kernel, roof, extra["kernels"])
     ctx.profile_enable(False)
     r, _ = ctx.pr_download()
     ctx.close()

@@ -1036,11 +1036,13 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_tiled(const uint4* __restric
   }
 }
 
-// Vertices with out-edges (the compact positions' count), added into *out.
+// Vertices with out-edges (the compact positions' count), added into *out (a
+// grid of at most 1024 workgroups striding over the vertices: one atomic each).
 __global__ void k_count_sources(uint32_t n, const uint32_t* __restrict__ off, unsigned long long* __restrict__ out) {
   __shared__ unsigned long long s[kBlock / 64];
-  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-  unsigned long long c = v < n && off[v + 1] > off[v] ? 1ull : 0ull;
+  unsigned long long c = 0;
+  for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += gridDim.x * kBlock)
+    c += off[v + 1] > off[v] ? 1ull : 0ull;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
   if (lane_id() == 0) s[threadIdx.x >> 6] = c;
@@ -1071,7 +1073,7 @@ static void build_tiles(suminc_ctx* c, uint64_t m_cap) {
   unsigned long long* d_src = reinterpret_cast<unsigned long long*>(c->tile_bounds.as<uint8_t>() + src_at);
   SUMINC_HIP(hipMemsetAsync(d_src, 0, 8, c->stream));
   if (c->pull_compact)
-    k_count_sources<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), d_src);
+    k_count_sources<<<grid_for(n, kBlock, 1024), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), d_src);
   k_tile_bounds<<<grid_for((uint64_t)blocks + 1), kBlock, 0, c->stream>>>(n, blocks, bsize, c->in_off.as<uint32_t>(),
                                                                          c->tile_bounds.as<uint32_t>());
   std::vector<uint32_t> hb(blocks + 1);
