@@ -963,18 +963,27 @@ __global__ void __launch_bounds__(kBlock) k_tile_build(const uint4* __restrict__
   uint32_t* sg = seg + (size_t)blockIdx.x * (P + 1);
   for (uint32_t b = threadIdx.x; b <= P; b += kBlock) sg[b] = e0 + s_pre[b];
   const uint32_t mask = (1u << win_log2) - 1u;
-  for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
-    // the row of e: the last v in [d0, d0 + nd) with ioff[v] <= e
-    uint32_t lo = d0, hi = d0 + nd;  // ioff[lo] <= e < ioff[hi]
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (ioff[mid] <= e) lo = mid;
-      else hi = mid;
+  // the rows of the block in chunks of kBlock, each row clipped to the tile's edges,
+  // their edges walked by the whole workgroup (the row of an edge from the LDS scan)
+  __shared__ uint32_t s_scan[kBlock + 1];
+  __shared__ uint32_t s_start[kBlock];
+  __shared__ uint32_t s_warp[kBlock / 64 + 1];
+  for (uint32_t r0 = d0; r0 < d0 + nd; r0 += kBlock) {
+    const uint32_t v = r0 + threadIdx.x;
+    uint32_t start = 0, len = 0;
+    if (v < d0 + nd) {
+      const uint32_t b = max(ioff[v], e0), e = min(ioff[v + 1], e1);
+      if (e > b) {
+        start = b;
+        len = e - b;
+      }
     }
-    const uint32_t p = icol[e];
-    const uint32_t b = p >> win_log2;
-    const uint32_t slot = atomicAdd(&s_fill[b], 1u);
-    packed[e0 + s_pre[b] + slot] = ((lo - d0) << kTileSrcBits) | (p & mask);
+    block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t, uint32_t e) {
+      const uint32_t p = icol[e];
+      const uint32_t b = p >> win_log2;
+      const uint32_t at = atomicAdd(&s_fill[b], 1u);
+      packed[e0 + s_pre[b] + at] = ((r0 + (uint32_t)slot - d0) << kTileSrcBits) | (p & mask);
+    });
   }
 }
 
