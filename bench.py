@@ -61,7 +61,7 @@ WORKLOADS = {
     "pr16": dict(kind="prfull", scale=16, seed=1, eps=1e-9,
                  desc="plain delta-PageRank from scratch, R-MAT s16 ef16 seed1, d=0.85f, per-vertex threshold 1e-9 "
                       "(T = 1e-9 N), FP64, no compression, no updates (the reference's CPU case)"),
-    "wpr22": dict(kind="wpr", scale=22, seed=4, seeds=1024, T=1e-9, fd_edges=32, h=1e-3,
+    "wpr22": dict(kind="wpr", scale=22, seed=4, seeds=1024, T=1e-9, fd_edges=256, h=1e-3,
                   desc="differentiable weighted PageRank, R-MAT s22 ef16 seed4, FP32 edge weights U(0,1] "
                        "row-normalized, personalization Dirichlet(1) over 1024 random seeds, loss sum c_i r_i "
                        "with c ~ N(0,1); a step = forward + adjoint (gradients w.r.t. weights and p)"),
@@ -503,6 +503,7 @@ def run_web(args, wl):
     per = []
     edges = 0.0
     t_all = time.perf_counter()
+    pull_prev = ctx.profile(L.KERNEL_PR_PULL)
     for b in range(args.warmup, args.warmup + args.steps):
         t0 = time.perf_counter()
         sa = ctx.apply_batch_device(dev[b])
@@ -511,6 +512,10 @@ def run_web(args, wl):
         sp = ctx.sum_pr_run()
         ctx.sync()
         t2 = time.perf_counter()
+        pull_cur = ctx.profile(L.KERNEL_PR_PULL)
+        n_pull = pull_cur["launches"] - pull_prev["launches"]
+        pull_ms = (pull_cur["ms"] - pull_prev["ms"]) / n_pull if n_pull else 0.0
+        pull_prev = pull_cur
         ss = ctx.sum_sssp_run()
         ctx.sync()
         t3 = time.perf_counter()
@@ -520,6 +525,7 @@ def run_web(args, wl):
                     "pr_ms": round(1e3 * (t2 - t1), 3), "sssp_ms": round(1e3 * (t3 - t2), 3),
                     "pr_sketch_rounds": sp["sketch_rounds"], "pr_residual_rounds": sp["residual_rounds"],
                     "sssp_rounds": ss["rounds"], "mass_defect": sp["mass_defect"], "edges": e,
+                    "pull_ms_per_round": round(pull_ms, 3),
                     "GTEPS": round(e / (t3 - t0) / 1e9, 3)})
         log(f"batch {b}: {per[-1]}")
     ctx.sync()

@@ -235,3 +235,32 @@ def test_tiled_pull_follows_precision_and_growth(gpu):
             rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
             assert np.abs(r - rs).sum() <= st["residual"] / (1 - DAMPING) + 2e-6, b
         assert ctx.graph_info()[1] > len(col)
+
+
+@pytest.mark.parametrize("path", ["plain", "sketch", "tiled"])
+def test_incremental_fp32_meets_l1_target(gpu, path):
+    """The north-star bar after every incremental FP32/Kahan batch: L1 <= 1e-6 against
+    the FP64 fixed point of the updated graph, at the benchmark's stop rule T = 1e-7,
+    on the plain path, the SumInc query over the sketch and the tiled pull."""
+    g = oracle.Graph.rmat(14, 16, seed=6)
+    off, col, _ = g.csr()
+    kw = {"pull_tile_kib": 1} if path == "tiled" else {}
+    with gpu.Context(pr_precision=gpu.PR_FP32_KAHAN, termcheck_threshold=1e-7, **kw) as ctx:
+        ctx.upload_csr(off, col)
+        if path == "sketch":
+            ctx.cluster_leaves(1024, 2)
+            ctx.sum_build(2, 1024, 1.0)
+        run = ctx.sum_pr_run if path == "sketch" else ctx.pr_run
+        ctx.pr_init()
+        run()
+        for b in range(3):
+            n_ops = int(0.01 * g.m)
+            ops = g.gen_batch(seed=6, batch_id=b, n_del=n_ops // 2, n_ins=n_ops - n_ops // 2)
+            ctx.apply_batch(ops)
+            g, _ = g.apply_batch(ops)
+            st = run()
+            r, _ = ctx.pr_download()
+            rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
+            err = float(np.abs(r - rs).sum())
+            assert err <= 1e-6, (path, b, err, st["residual"])
+            assert abs(st["mass_defect"]) < 1e-6
