@@ -75,21 +75,39 @@ WORKLOADS = {
 }
 
 
-def tiled_pull(args, n) -> bool:
-    """Whether the library's dense PageRank rounds run the tiled pull (its rule: a
-    window is given, or the FP32 contributions exceed 64 MiB and it is not off)."""
-    return args.pull_band_kib <= 0 and (args.pull_tile_kib > 0 or (args.pull_tile_kib == 0 and n * 4 > (64 << 20)))
+def pull_kw(args) -> dict:
+    """The pull layout parameters of every context this script opens."""
+    return dict(pull_band_kib=args.pull_band_kib, pull_tile_kib=args.pull_tile_kib, pull_bin_kib=args.pull_bin_kib,
+                pull_bin_direct=args.pull_bin_direct)
+
+
+def pull_mode(args, n) -> str:
+    """Which pull the library's dense PageRank rounds run (its rules: the binned pull
+    when a window is given, or the FP32 contributions exceed 64 MiB and it is not
+    off; else the tiled pull by the same rule; else the row classes)."""
+    if args.pull_band_kib > 0:
+        return "rows"
+    big = n * 4 > (64 << 20)
+    if args.pull_bin_kib > 0 or (args.pull_bin_kib == 0 and big):
+        return "binned"
+    if args.pull_tile_kib > 0 or (args.pull_tile_kib == 0 and big):
+        return "tiled"
+    return "rows"
+
+
+PULL_KERNELS = {"rows": "k_pr_pull", "tiled": "k_pr_pull_tiled", "binned": "k_pr_bin_gather+k_pr_bin_sum"}
 
 
 def name_pull(args, n, kernel, roof, table):
-    """The profiled pull launches are the tiled kernel's when the tiled pull runs."""
-    if not tiled_pull(args, n):
+    """The profiled pull launches are the tiled or binned kernels' when those run."""
+    name = PULL_KERNELS[pull_mode(args, n)]
+    if name == "k_pr_pull":
         return kernel
     if roof is not None and roof.get("kernel") == "k_pr_pull":
-        roof["kernel"] = "k_pr_pull_tiled"
+        roof["kernel"] = name
     if table and "k_pr_pull" in table:
-        table["k_pr_pull_tiled"] = table.pop("k_pr_pull")
-    return "k_pr_pull_tiled" if kernel == "k_pr_pull" else kernel
+        table[name] = table.pop("k_pr_pull")
+    return name if kernel == "k_pr_pull" else kernel
 
 
 def kernel_names():
@@ -291,7 +309,7 @@ def run_pr(args, wl):
     import suminc_amd._lib as L
 
     ctx = suminc_amd.Context(termcheck_threshold=wl["T"], pr_precision=suminc_amd.PR_FP32_KAHAN,
-                             max_rounds=1_000_000, pull_band_kib=args.pull_band_kib, pull_tile_kib=args.pull_tile_kib)
+                             max_rounds=1_000_000, **pull_kw(args))
     t0 = time.time()
     ctx.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"])
     n, m, _ = ctx.graph_info()
@@ -372,13 +390,14 @@ def run_pr(args, wl):
     return gteps, ms_per_step, "f32", roof, cpu, extra
 
 
-def _pmc_traffic(args, kernel_sub):
-    """HBM traffic per launch of the kernel whose rocprof name contains kernel_sub,
-    measured now: two rocprofv3 --pmc runs of a short child of this script
-    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
-    counter reads for a plain device-to-device copy of known size (FETCH x2, WRITE
-    x1 on MI355X, profiles/k_copy_old_sssp22_traffic.txt).  Returns (bytes, note)
-    or (None, reason)."""
+def _pmc_traffic(args, kernel_subs):
+    """HBM traffic per round of the kernels whose rocprof names contain the strings
+    of kernel_subs (one launch of each per round), measured now: two rocprofv3 --pmc
+    runs of a short child of this script (FETCH_SIZE, then WRITE_SIZE, KiB per
+    dispatch), each scaled by what the same counter reads for a plain
+    device-to-device copy of known size (FETCH x2, WRITE x1 on MI355X,
+    profiles/k_copy_old_sssp22_traffic.txt), the per-dispatch means of the kernels
+    added.  Returns (bytes, note) or (None, reason)."""
     import glob
     import shutil
     import subprocess
@@ -395,32 +414,38 @@ def _pmc_traffic(args, kernel_sub):
         cmd = ["timeout", "-s", "KILL", str(args.traffic_timeout), prof, "--pmc", counter, "--output-format", "csv",
                "-d", d, "-o", "run", "--", sys.executable, str(Path(__file__).resolve()), "--traffic-child",
                "--workload", args.workload, "--pull-band-kib", str(args.pull_band_kib),
-               "--pull-tile-kib", str(args.pull_tile_kib)]
+               "--pull-tile-kib", str(args.pull_tile_kib), "--pull-bin-kib", str(args.pull_bin_kib),
+               "--pull-bin-direct", str(args.pull_bin_direct)]
         log(f"traffic: {' '.join(cmd[4:])}")
         r = subprocess.run(cmd, stdout=sys.stderr, stderr=sys.stderr, cwd=str(ROOT))  # progress passes through
         files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
         if not files:
             shutil.rmtree(d, ignore_errors=True)
             return None, f"rocprofv3 --pmc {counter} wrote no counter file (rc {r.returncode})"
-        vals = []
+        vals = {k: [] for k in kernel_subs}
         import csv
 
         for f in files:
             with open(f, newline="") as fh:
                 for row in csv.DictReader(fh):
                     name = row.get("Kernel_Name") or row.get("Kernel-Name") or ""
-                    if kernel_sub in name and row.get("Counter_Name", counter) == counter:
-                        vals.append(float(row["Counter_Value"]))
+                    for k in kernel_subs:
+                        if k in name and row.get("Counter_Name", counter) == counter:
+                            vals[k].append(float(row["Counter_Value"]))
         shutil.rmtree(d, ignore_errors=True)
-        if not vals:
-            return None, f"no {kernel_sub} dispatch in the {counter} run"
-        top = max(vals)
-        work = [v for v in vals if v >= 0.1 * top]  # dispatches that did work (idle ones return at entry)
-        out[counter] = (float(np.mean(work)), len(work))
+        mean_kib, count = 0.0, 0
+        for k, v in vals.items():
+            if not v:
+                return None, f"no {k} dispatch in the {counter} run"
+            top = max(v)
+            work = [x for x in v if x >= 0.1 * top]  # dispatches that did work (idle ones return at entry)
+            mean_kib += float(np.mean(work))
+            count = len(work)
+        out[counter] = (mean_kib, count)
     b = sum(out[k][0] * 1024.0 * scale[k] for k in out)
-    note = (f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over {out['FETCH_SIZE'][1]} dispatches of {kernel_sub} "
-            f"in a {args.workload} child run: FETCH {out['FETCH_SIZE'][0]:.0f} KiB x2, WRITE "
-            f"{out['WRITE_SIZE'][0]:.0f} KiB x1 per dispatch")
+    note = (f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over {out['FETCH_SIZE'][1]} dispatches of "
+            f"{' + '.join(kernel_subs)} in a {args.workload} child run: FETCH {out['FETCH_SIZE'][0]:.0f} KiB x2, "
+            f"WRITE {out['WRITE_SIZE'][0]:.0f} KiB x1 per round")
     return b, note
 
 
@@ -431,7 +456,7 @@ def traffic_child(args, wl):
     import suminc_amd
 
     ctx = suminc_amd.Context(termcheck_threshold=wl["T"], pr_precision=suminc_amd.PR_FP32_KAHAN, max_rounds=4,
-                             pull_band_kib=args.pull_band_kib, pull_tile_kib=args.pull_tile_kib)
+                             **pull_kw(args))
     ctx.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"])
     ctx.cluster_leaves(wl["max_node"], 2)
     ctx.sum_build(2, wl["max_node"], 1.0)
@@ -462,7 +487,7 @@ def run_web(args, wl):
 
     T = wl["T"]
     ctx = suminc_amd.Context(termcheck_threshold=T, pr_precision=suminc_amd.PR_FP32_KAHAN,
-                             max_rounds=1_000_000, pull_band_kib=args.pull_band_kib, pull_tile_kib=args.pull_tile_kib)
+                             max_rounds=1_000_000, **pull_kw(args))
     t0 = time.time()
     ctx.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"], weights="u32", wmin=wl["wmin"],
                       wmax=wl["wmax"])
@@ -617,7 +642,7 @@ def run_web(args, wl):
         # weights (a context of its own: the SSSP graph's weights are integers),
         # Dirichlet(1) personalization over random seeds, c ~ N(0,1)
         with suminc_amd.Context(termcheck_threshold=wl["T"], pr_precision=suminc_amd.PR_FP32_KAHAN,
-                                max_rounds=1_000_000, pull_band_kib=args.pull_band_kib, pull_tile_kib=args.pull_tile_kib) as cw:
+                                max_rounds=1_000_000, **pull_kw(args)) as cw:
             cw.generate_rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=wl["seed"], weights="f32")
             rng = np.random.default_rng(wl["seed"])
             pvec = np.zeros(n, np.float32)
@@ -633,7 +658,8 @@ def run_web(args, wl):
             log(f"adjoint: {extra['adjoint']}")
             del gw
     if roof is not None and not args.no_traffic:
-        sub = {"k_pr_pull": "k_pr_pull<float>", "k_pr_pull_tiled": "k_pr_pull_tiled<float>"}.get(kernel)
+        sub = {"k_pr_pull": ["k_pr_pull<float>"], "k_pr_pull_tiled": ["k_pr_pull_tiled<float>"],
+               PULL_KERNELS["binned"]: ["k_pr_bin_gather<float>", "k_pr_bin_sum<float>"]}.get(kernel)
         if sub is None:
             roof["traffic_note"] = f"no live counter run for {kernel}"
         else:
@@ -928,6 +954,10 @@ def main():
                     help="PageRank workloads: source band of the banded pull in KiB (0: off, the library default)")
     ap.add_argument("--pull-tile-kib", type=int, default=0,
                     help="tiled pull source window in KiB (0: the library default, negative: the row-class pull)")
+    ap.add_argument("--pull-bin-kib", type=int, default=0,
+                    help="binned pull source window in KiB (0: the library default, negative: off)")
+    ap.add_argument("--pull-bin-direct", type=int, default=-1,
+                    help="binned pull: windows the summing pass gathers directly (negative: the library default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device (counter calibration for rocprofv3 --pmc) and exit")
     ap.add_argument("--no-adjoint", action="store_true", help="web26: skip the weighted-PageRank forward + adjoint")

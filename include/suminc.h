@@ -90,7 +90,16 @@ typedef struct suminc_params {
                                  sums in LDS) over source windows of this many KiB of contributions, taken
                                  in the same order by every workgroup (0: 1024; negative: never; the
                                  threshold does not apply when set > 0) */
-  int32_t reserved[4];
+  int32_t pull_bin_kib;       /* binned pull (propagation blocking): when the banded pull is off, a dense round
+                                 first copies the contributions named by the in-edges into an edge-ordered
+                                 value stream, one source window of this many KiB at a time (every window in
+                                 the caches while it is read), then sums the stream by destination block in
+                                 LDS; takes precedence over the tiled pull (0: on above 64 MiB of
+                                 contributions, 2048 KiB windows; negative: never; > 0: always) */
+  int32_t pull_bin_direct;    /* binned pull: the first windows (the hottest sources) that the summing pass
+                                 gathers directly instead of through the value stream (negative: the
+                                 default, 1) */
+  int32_t reserved[2];
 } suminc_params;
 
 typedef struct suminc_stats {

@@ -55,7 +55,9 @@ class Params(C.Structure):
         ("pull_band_kib", C.c_int32),
         ("pull_compact_kib", C.c_int32),
         ("pull_tile_kib", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("pull_bin_kib", C.c_int32),
+        ("pull_bin_direct", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 

@@ -98,6 +98,7 @@ void suminc_default_params(suminc_params* p) {
   p->pr_precision = SUMINC_PR_FP32_KAHAN;
   p->sssp_source = 0;
   p->pull_threshold_div = 3;
+  p->pull_bin_direct = -1;
 }
 
 int suminc_ctx_create(int device, const suminc_params* params, suminc_ctx** out) {

@@ -218,6 +218,14 @@ struct suminc_ctx {
   // tile's window offsets (tile_P + 1 per tile)
   suminc::DevBuf tile_tab, tile_edges, tile_seg, tile_bounds, tile_edges2, tile_flat;
   uint32_t tile_n = 0, tile_P = 0, tile_win_log2 = 0, tile_bsize = 0;
+  // binned pull (per graph version, with the pull lists; exclusive with the tiles):
+  // the in-edges in (source window, destination block) order, packed as the tiles'
+  // are, in tile_edges; segment offsets bin_F[w * bin_B + b] (bin_P * bin_B + 1);
+  // the summing pass's parts (uint4: block | split << 31, first | last window << 16,
+  // first and end edge); the value stream (one element per in-edge)
+  suminc::DevBuf bin_F, bin_hist, bin_parts, bin_vals;
+  uint32_t bin_n = 0, bin_P = 0, bin_B = 0, bin_wl = 0, bin_bsize = 0, bin_direct = 0;
+  uint64_t bin_first = 0, bin_m = 0;  // the value stream's edges: [bin_first, bin_m)
   int pull_prec = -1;  // the PageRank precision the pull lists, positions and tiles were built for
   uint64_t graph_version = 0;
 

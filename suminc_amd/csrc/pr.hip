@@ -662,7 +662,7 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_band_rows(const uint32_t* __
   __shared__ uint32_t s_scan[kBlock + 1];
   __shared__ uint32_t s_start[kBlock];
   __shared__ uint32_t s_warp[kBlock / 64 + 1];
-  __shared__ T s_acc[kBlock];
+  __shared__ double s_acc[kBlock];  // FP64: see kTileDst
   if (!pull_runs(cnt, gate)) return;
   const uint32_t tiles = (W + kBlock - 1) / kBlock;
   for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
@@ -673,7 +673,7 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_band_rows(const uint32_t* __
       lo = split[(size_t)p * W + i];
       len = split[(size_t)(p + 1) * W + i] - lo;
     }
-    s_acc[threadIdx.x] = T(0);
+    s_acc[threadIdx.x] = 0.0;
     uint32_t total;
     s_scan[threadIdx.x] = block_exclusive_scan<kBlock>(len, s_warp, &total);  // syncs: s_acc is clear
     s_start[threadIdx.x] = lo;
@@ -703,10 +703,10 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_band_rows(const uint32_t* __
       for (int r = 0; r < 4; ++r) x[r] = slot[r] >= 0 ? contrib[u[r]] : T(0);
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        if (slot[r] >= 0) atomicAdd(&s_acc[slot[r]], x[r]);
+        if (slot[r] >= 0) atomicAdd(&s_acc[slot[r]], (double)x[r]);
     }
     __syncthreads();
-    if (i < W && s_acc[threadIdx.x] != T(0)) delta[v] += s_acc[threadIdx.x];
+    if (i < W && s_acc[threadIdx.x] != 0.0) delta[v] += (T)s_acc[threadIdx.x];
     __syncthreads();  // s_scan / s_start / s_acc are rewritten by the next tile
   }
 }
@@ -920,10 +920,12 @@ constexpr uint32_t kTileDstBits = 13;
 constexpr uint32_t kTileSrcBits = 32 - kTileDstBits;
 constexpr uint32_t kTileEdges = 16u << kTileDstBits;  // in-edges per tile
 constexpr uint32_t kTileMaxWindows = 1024;
-// destinations per block (LDS sums): 8192 in FP32, 4096 in FP64 (32 KiB either way)
-template <class T>
-constexpr uint32_t kTileAcc = (32u << 10) / sizeof(T);
-inline uint32_t tile_dst(int prec) { return prec == SUMINC_PR_FP64 ? kTileAcc<double> : kTileAcc<float>; }
+// destinations per block: 4096, their sums in LDS in FP64 (32 KiB) in both
+// precisions.  The FP64 LDS add is the faster one on MI355X (a probe of random
+// adds into 32 KiB: 1.5e12 FP64 against 2.0e11 FP32 lane additions a second, the
+// FP32 add the bottleneck of a dense round), and it rounds each sum once.
+constexpr uint32_t kTileDst = (32u << 10) / sizeof(double);
+inline uint32_t tile_dst(int) { return kTileDst; }
 
 // in_off at every block start (blocks + 1 values)
 __global__ void k_tile_bounds(uint32_t n, uint32_t blocks, uint32_t bsize, const uint32_t* __restrict__ ioff,
@@ -1006,7 +1008,7 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_tiled(const uint4* __restric
                                                           const uint32_t* __restrict__ seg, uint32_t P,
                                                           uint32_t win_log2, const T* __restrict__ contrib,
                                                           T* __restrict__ delta, const Counters* cnt, int gate) {
-  __shared__ T s_acc[kTileAcc<T>];
+  __shared__ double s_acc[kTileDst];
   __shared__ uint32_t s_seg[kTileMaxWindows + 1];
   if (!pull_runs(cnt, gate)) return;
   const uint4 t = tiles[blockIdx.x];
@@ -1014,7 +1016,7 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_tiled(const uint4* __restric
   const bool split = (t.y >> 31) != 0;
   const uint32_t* sg = seg + (size_t)blockIdx.x * (P + 1);
   for (uint32_t i = threadIdx.x; i <= P; i += kBlock) s_seg[i] = sg[i];
-  for (uint32_t i = threadIdx.x; i < nd; i += kBlock) s_acc[i] = T(0);
+  for (uint32_t i = threadIdx.x; i < nd; i += kBlock) s_acc[i] = 0.0;
   __syncthreads();
   const uint32_t mask = (1u << win_log2) - 1u;
   for (uint32_t b = 0; b < P; ++b) {
@@ -1032,15 +1034,15 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_tiled(const uint4* __restric
       for (int r = 0; r < 4; ++r) x[r] = k + r * kBlock < e ? w[q[r] & mask] : T(0);
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        if (x[r] != T(0)) atomicAdd(&s_acc[q[r] >> kTileSrcBits], x[r]);
+        if (x[r] != T(0)) atomicAdd(&s_acc[q[r] >> kTileSrcBits], (double)x[r]);
     }
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < nd; i += kBlock) {
-    const T x = s_acc[i];
-    if (x != T(0)) {
-      if (split) unsafeAtomicAdd(&delta[d0 + i], x);
-      else delta[d0 + i] += x;
+    const double x = s_acc[i];
+    if (x != 0.0) {
+      if (split) unsafeAtomicAdd(&delta[d0 + i], (T)x);
+      else delta[d0 + i] += (T)x;
     }
   }
 }
@@ -1152,6 +1154,408 @@ static void build_tiles(suminc_ctx* c, uint64_t m_cap) {
   c->tile_bsize = bsize;
 }
 
+// ---------------------------------------------------------------------------------
+// Binned pull (propagation blocking: Beamer, Asanovic and Patterson, "Reducing
+// PageRank communication via propagation blocking", IPDPS 2017).  The tiled pull
+// still reads each contribution where an in-edge names it, and at 2^26 vertices
+// most of those reads miss L2: the tiles walk the windows at their own pace, so an
+// XCD's L2 holds few of the windows its workgroups read.  Here a dense round makes
+// two passes over the in-edges, held once per graph version in (source window,
+// destination block) order, each packed into 32 bits as the tiles' are
+// ((destination - block start) << kTileSrcBits | slot within the window):
+//   k_pr_bin_gather  the value stream vals[i] = contrib[window(i) start + slot(i)]:
+//                    workgroups over consecutive edges, so the workgroups in flight
+//                    read one or two windows (2 MiB each), which stay in every
+//                    XCD's L2; the packed reads and the value writes stream.
+//   k_pr_bin_sum     one workgroup per part of a destination block: the block's
+//                    segments of every window, taken as one run (their bounds in
+//                    LDS), summed into LDS, then delta += the sums (atomically for
+//                    a block cut into several parts).  The first `direct` windows
+//                    (the hottest sources) are gathered here from contrib directly
+//                    and left out of the value stream.
+// Per streamed in-edge 16 bytes move (the packed edge twice, the value written and
+// read back) and no line is filled for a single 4-byte gather.  Within the segment
+// of one window and one build piece (at most kTileEdges in-edges of a block) the
+// edges are sorted by slot.
+// ---------------------------------------------------------------------------------
+constexpr uint32_t kBinGatherEdges = 8192;  // in-edges per workgroup of the gather pass
+
+// Vertices with out-edges when the compact positions are in use (the positions'
+// count), else n.  Waits for the stream.
+static unsigned long long pull_source_count(suminc_ctx* c) {
+  if (!c->pull_compact) return c->n;
+  c->bin_hist.ensure(8);
+  unsigned long long* d = c->bin_hist.as<unsigned long long>();
+  SUMINC_HIP(hipMemsetAsync(d, 0, 8, c->stream));
+  k_count_sources<<<grid_for(c->n, kBlock, 1024), kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), d);
+  unsigned long long h = 0;
+  SUMINC_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  return h;
+}
+
+// The build works on pieces of at most kTileEdges in-edges of one block (the
+// tiles' table: first destination, destination count, CSC edge range), one
+// workgroup each.  Counting: the piece's edges by source window into
+// hist[w * J + j] (J pieces).
+__global__ void __launch_bounds__(kBlock) k_bin_hist(const uint4* __restrict__ pieces, uint32_t J,
+                                                     const uint32_t* __restrict__ icol, uint32_t wl, uint32_t P,
+                                                     uint32_t* __restrict__ hist) {
+  __shared__ uint32_t s_h[kTileMaxWindows];
+  const uint4 t = pieces[blockIdx.x];
+  for (uint32_t w = threadIdx.x; w < P; w += kBlock) s_h[w] = 0;
+  __syncthreads();
+  for (uint32_t e = t.z + threadIdx.x; e < t.w; e += kBlock) atomicAdd(&s_h[icol[e] >> wl], 1u);
+  __syncthreads();
+  for (uint32_t w = threadIdx.x; w < P; w += kBlock) hist[(size_t)w * J + blockIdx.x] = s_h[w];
+}
+
+// Placing: the piece's edges packed into their (window, piece) segments, whose
+// offsets F2[w * J + j] are the exclusive scan of hist (so the pieces of a block
+// make up its (window, block) segments).
+__global__ void __launch_bounds__(kBlock) k_bin_scatter(const uint4* __restrict__ pieces, uint32_t J,
+                                                        const uint32_t* __restrict__ ioff,
+                                                        const uint32_t* __restrict__ icol, uint32_t wl, uint32_t P,
+                                                        const uint32_t* __restrict__ F2,
+                                                        uint32_t* __restrict__ packed) {
+  __shared__ uint32_t s_at[kTileMaxWindows];
+  __shared__ uint32_t s_scan[kBlock + 1];
+  __shared__ uint32_t s_start[kBlock];
+  __shared__ uint32_t s_warp[kBlock / 64 + 1];
+  const uint4 t = pieces[blockIdx.x];
+  const uint32_t d0 = t.x, nd = t.y & 0x7FFFFFFFu, e0 = t.z, e1 = t.w;
+  for (uint32_t w = threadIdx.x; w < P; w += kBlock) s_at[w] = F2[(size_t)w * J + blockIdx.x];
+  __syncthreads();
+  const uint32_t mask = (1u << wl) - 1u;
+  // the block's rows in chunks of kBlock, each clipped to the piece's edges
+  for (uint32_t r0 = d0; r0 < d0 + nd; r0 += kBlock) {
+    const uint32_t v = r0 + threadIdx.x;
+    uint32_t start = 0, len = 0;
+    if (v < d0 + nd) {
+      const uint32_t b = max(ioff[v], e0), e = min(ioff[v + 1], e1);
+      if (e > b) {
+        start = b;
+        len = e - b;
+      }
+    }
+    block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t, uint32_t e) {
+      const uint32_t p = icol[e];
+      const uint32_t at = atomicAdd(&s_at[p >> wl], 1u);
+      packed[at] = ((r0 + (uint32_t)slot - d0) << kTileSrcBits) | (p & mask);
+    });
+  }
+}
+
+// The (window, block) segment offsets from the pieces': F[w * B + b] =
+// F2[w * J + first[b]] (first[b]: the first piece of block b or after it), and
+// F[P * B] = F2[P * J] (the edge count).
+__global__ void k_bin_fold(const uint32_t* __restrict__ F2, uint32_t J, const uint32_t* __restrict__ first,
+                           uint32_t B, uint32_t P, uint32_t* __restrict__ F) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t total = (uint64_t)P * B;
+  if (i < total) F[i] = F2[(i / B) * J + first[i % B]];
+  else if (i == total) F[i] = F2[(uint64_t)P * J];
+}
+
+// Four consecutive elements, loaded or stored as one access (the edge arrays and
+// the value stream are read in 4-aligned groups).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+template <class T>
+using vec4_t = T __attribute__((ext_vector_type(4)));
+
+// The gather pass over the streamed edges [first, end) (first a multiple of 4; the
+// edges before the stream's first that it also covers belong to direct windows,
+// whose values nobody reads): kBinGatherEdges per workgroup, eight 4-edge groups a
+// thread, all loads issued before the gathers.  A full workgroup inside one window
+// (the window of its first edge by binary search over the window starts F[w * B])
+// takes the vector path; the others (a window start or the end inside) go edge by
+// edge, each thread's window only moving forward.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pr_bin_gather(const uint32_t* __restrict__ packed,
+                                                          const uint32_t* __restrict__ F, uint32_t B, uint32_t P,
+                                                          uint32_t first, uint32_t end, uint32_t wl,
+                                                          const T* __restrict__ contrib, T* __restrict__ vals,
+                                                          const Counters* cnt, int gate) {
+  constexpr int G = kBinGatherEdges / (4 * kBlock);
+  __shared__ uint32_t s_w;
+  if (!pull_runs(cnt, gate)) return;
+  const uint32_t i0 = (uint32_t)((uint64_t)first + (uint64_t)blockIdx.x * kBinGatherEdges);  // < end
+  const uint32_t i1 = (uint32_t)min((uint64_t)i0 + kBinGatherEdges, (uint64_t)end);
+  if (threadIdx.x == 0) {
+    uint32_t lo = 0, hi = P - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (F[(size_t)mid * B] <= i0) lo = mid;
+      else hi = mid - 1;
+    }
+    s_w = lo;
+  }
+  __syncthreads();
+  const uint32_t w0 = s_w;
+  const uint32_t wend = w0 + 1 < P ? F[(size_t)(w0 + 1) * B] : end;
+  const uint32_t mask = (1u << wl) - 1u;
+  if (i1 - i0 == kBinGatherEdges && i1 <= wend) {
+    const T* win = contrib + ((size_t)w0 << wl);
+    const u32x4* p4 = reinterpret_cast<const u32x4*>(packed + i0);
+    vec4_t<T>* v4 = reinterpret_cast<vec4_t<T>*>(vals + i0);
+    u32x4 q[G];
+#pragma unroll
+    for (int r = 0; r < G; ++r) q[r] = __builtin_nontemporal_load(p4 + threadIdx.x + r * kBlock);
+    vec4_t<T> x[G];
+#pragma unroll
+    for (int r = 0; r < G; ++r) {
+      x[r].x = win[q[r].x & mask];
+      x[r].y = win[q[r].y & mask];
+      x[r].z = win[q[r].z & mask];
+      x[r].w = win[q[r].w & mask];
+    }
+#pragma unroll
+    for (int r = 0; r < G; ++r) __builtin_nontemporal_store(x[r], v4 + threadIdx.x + r * kBlock);
+  } else {
+    uint32_t w = w0;  // a thread's edges ascend, so its window only moves forward
+    for (uint32_t k = i0 + threadIdx.x; k < i1; k += kBlock) {
+      while (w + 1 < P && F[(size_t)(w + 1) * B] <= k) ++w;
+      vals[k] = contrib[((size_t)w << wl) + (packed[k] & mask)];
+    }
+  }
+}
+
+// The summing pass: one workgroup per part of a destination block.  The block's
+// segment of window w is [s_w, e_w); it is read in 4-aligned groups of edges,
+// those from s_w & ~3 to past e_w - 1, the edges of a group outside [s_w, e_w)
+// masked off.  The groups of all windows in order form the block's run, a part is
+// the run's range [z, zend); the segment bounds and the run position of every
+// window go to LDS first (a workgroup scan), then every thread walks its groups in
+// ascending order, its window pointer only moving forward, four groups (sixteen
+// edges) an iteration with all their loads issued first.
+constexpr uint32_t kBinMaxWindows = 2 * kBlock;
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pr_bin_sum(const uint4* __restrict__ parts,
+                                                       const uint32_t* __restrict__ F, uint32_t B, uint32_t P,
+                                                       uint32_t bsize, uint32_t n, const uint32_t* __restrict__ packed,
+                                                       const T* __restrict__ vals, const T* __restrict__ contrib,
+                                                       uint32_t wl, uint32_t direct, T* __restrict__ delta,
+                                                       const Counters* cnt, int gate) {
+  constexpr int U = 4;
+  __shared__ double s_acc[kTileDst];
+  __shared__ uint32_t s_beg[kBinMaxWindows];
+  __shared__ uint32_t s_end[kBinMaxWindows];
+  __shared__ uint32_t s_pre[kBinMaxWindows + 1];
+  __shared__ uint32_t s_warp[kBlock / 64 + 1];
+  if (!pull_runs(cnt, gate)) return;
+  const uint4 t = parts[blockIdx.x];
+  const uint32_t b = t.x & 0x7FFFFFFFu, z = t.z, zend = t.w;
+  const bool split = (t.x >> 31) != 0;
+  const uint32_t d0 = b * bsize, nd = min(bsize, n - d0);
+  for (uint32_t i = threadIdx.x; i < nd; i += kBlock) s_acc[i] = 0.0;
+  // every window's segment and group count, two windows a thread (P <= 2 * kBlock)
+  uint32_t ng[2], sum = 0;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const uint32_t w = threadIdx.x * 2 + r;
+    ng[r] = 0;
+    if (w < P) {
+      const uint32_t s = F[(size_t)w * B + b], e = F[(size_t)w * B + b + 1];
+      s_beg[w] = s;
+      s_end[w] = e;
+      ng[r] = e > s ? ((e + 3) >> 2) - (s >> 2) : 0u;
+    }
+    sum += ng[r];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan<kBlock>(sum, s_warp, &total);
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const uint32_t w = threadIdx.x * 2 + r;
+    if (w < P) s_pre[w] = run;
+    run += ng[r];
+  }
+  if (threadIdx.x == 0) s_pre[P] = total;
+  __syncthreads();
+  const uint32_t mask = (1u << wl) - 1u;
+  uint32_t j[U];
+#pragma unroll
+  for (int r = 0; r < U; ++r) {
+    const uint32_t g = z + threadIdx.x + r * kBlock;
+    uint32_t lo = 0, hi = P - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (s_pre[mid] <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    j[r] = lo;
+  }
+  for (uint32_t g = z + threadIdx.x; g < zend; g += U * kBlock) {
+    u32x4 q[U];
+    vec4_t<T> x[U];
+    uint32_t k0[U], lo[U], hi[U];
+#pragma unroll
+    for (int r = 0; r < U; ++r) {
+      const uint32_t gr = g + r * kBlock;
+      q[r] = u32x4(0u);
+      x[r] = vec4_t<T>(T(0));
+      k0[r] = 0;
+      lo[r] = 1;
+      hi[r] = 0;
+      if (gr < zend) {
+        while (s_pre[j[r] + 1] <= gr) ++j[r];
+        const uint32_t jr = j[r];
+        k0[r] = (s_beg[jr] & ~3u) + 4 * (gr - s_pre[jr]);
+        lo[r] = s_beg[jr];
+        hi[r] = s_end[jr];
+        q[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(packed + k0[r]));
+        if (jr >= direct) x[r] = __builtin_nontemporal_load(reinterpret_cast<const vec4_t<T>*>(vals + k0[r]));
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < U; ++r) {
+      const bool in0 = k0[r] >= lo[r] && k0[r] < hi[r], in1 = k0[r] + 1 >= lo[r] && k0[r] + 1 < hi[r];
+      const bool in2 = k0[r] + 2 >= lo[r] && k0[r] + 2 < hi[r], in3 = k0[r] + 3 >= lo[r] && k0[r] + 3 < hi[r];
+      if (j[r] < direct && g + r * kBlock < zend) {
+        const T* win = contrib + ((size_t)j[r] << wl);
+        x[r].x = in0 ? win[q[r].x & mask] : T(0);
+        x[r].y = in1 ? win[q[r].y & mask] : T(0);
+        x[r].z = in2 ? win[q[r].z & mask] : T(0);
+        x[r].w = in3 ? win[q[r].w & mask] : T(0);
+      } else {
+        if (!in0) x[r].x = T(0);
+        if (!in1) x[r].y = T(0);
+        if (!in2) x[r].z = T(0);
+        if (!in3) x[r].w = T(0);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < U; ++r) {
+      if (x[r].x != T(0)) atomicAdd(&s_acc[q[r].x >> kTileSrcBits], (double)x[r].x);
+      if (x[r].y != T(0)) atomicAdd(&s_acc[q[r].y >> kTileSrcBits], (double)x[r].y);
+      if (x[r].z != T(0)) atomicAdd(&s_acc[q[r].z >> kTileSrcBits], (double)x[r].z);
+      if (x[r].w != T(0)) atomicAdd(&s_acc[q[r].w >> kTileSrcBits], (double)x[r].w);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nd; i += kBlock) {
+    const double x = s_acc[i];
+    if (x != 0.0) {
+      if (split) unsafeAtomicAdd(&delta[d0 + i], (T)x);
+      else delta[d0 + i] += (T)x;
+    }
+  }
+}
+
+// The binned layout of the current graph (after the positions).
+static void build_bins(suminc_ctx* c, uint64_t m_cap) {
+  c->bin_n = 0;
+  const int kib = c->prm.pull_bin_kib;
+  const uint64_t es = c->pr_prec == SUMINC_PR_FP64 ? 8 : 4;
+  const uint32_t n = c->n;
+  if (kib < 0 || c->band_P > 1 || n == 0) return;
+  if (kib == 0 && (uint64_t)n * es <= (64ull << 20)) return;
+  const uint64_t win_bytes = (uint64_t)(kib ? kib : 2048) * 1024;
+  const uint32_t bsize = tile_dst(c->pr_prec);  // the summing pass's LDS holds as many destinations
+  const uint32_t B = (n + bsize - 1) / bsize;
+  const unsigned long long n_src = std::max<unsigned long long>(pull_source_count(c), 1);
+  uint32_t wl = 8;
+  while (wl < kTileSrcBits && (2ull << wl) * es <= win_bytes) ++wl;
+  while (wl < kTileSrcBits && ((n_src + (1ull << wl) - 1) >> wl) > kTileMaxWindows) ++wl;
+  const uint32_t P = (uint32_t)((n_src + (1ull << wl) - 1) >> wl);
+  if (P > kBinMaxWindows) return;  // beyond 2^28 sources: the tiles or the row classes
+  const uint32_t* ioff = c->in_off.as<uint32_t>();
+  const uint32_t* cols = pull_cols(c);
+  // the blocks' CSC bounds, cut into pieces of at most kTileEdges edges
+  c->tile_bounds.ensure(((size_t)B + 1) * 4);
+  k_tile_bounds<<<grid_for((uint64_t)B + 1), kBlock, 0, c->stream>>>(n, B, bsize, ioff, c->tile_bounds.as<uint32_t>());
+  std::vector<uint32_t> hb(B + 1);
+  SUMINC_HIP(hipMemcpyAsync(hb.data(), c->tile_bounds.p, hb.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  // the CSC's edge count (during a batch the context's m is the old graph's)
+  const uint32_t m_now = hb[B];
+  if (m_now == 0) return;
+  std::vector<uint4> pieces;
+  std::vector<uint32_t> first(B);
+  for (uint32_t k = 0; k < B; ++k) {
+    first[k] = (uint32_t)pieces.size();
+    const uint32_t e0 = hb[k], e1 = hb[k + 1];
+    const uint32_t d0 = k * bsize, nd = std::min<uint32_t>(bsize, n - d0);
+    for (uint64_t e = e0; e < e1; e += kTileEdges)
+      pieces.push_back(make_uint4(d0, nd, (uint32_t)e, (uint32_t)std::min<uint64_t>(e1, e + kTileEdges)));
+  }
+  const uint32_t J = (uint32_t)pieces.size();
+  const uint64_t nseg2 = (uint64_t)P * J, nseg = (uint64_t)P * B;
+  c->tile_tab.ensure(((size_t)J + B) * sizeof(uint4));
+  uint4* d_pieces = c->tile_tab.as<uint4>();
+  uint32_t* d_first = reinterpret_cast<uint32_t*>(d_pieces + J);
+  SUMINC_HIP(hipMemcpyAsync(d_pieces, pieces.data(), (size_t)J * sizeof(uint4), hipMemcpyHostToDevice, c->stream));
+  SUMINC_HIP(hipMemcpyAsync(d_first, first.data(), (size_t)B * 4, hipMemcpyHostToDevice, c->stream));
+  c->bin_hist.ensure(nseg2 * 4);
+  c->tile_seg.ensure_slack((nseg2 + 1) * 4);  // F2: the pieces' segment offsets
+  c->bin_F.ensure((nseg + 1) * 4);
+  uint32_t* hist = c->bin_hist.as<uint32_t>();
+  uint32_t* F2 = c->tile_seg.as<uint32_t>();
+  uint32_t* F = c->bin_F.as<uint32_t>();
+  k_bin_hist<<<J, kBlock, 0, c->stream>>>(d_pieces, J, cols, wl, P, hist);
+  SUMINC_HIP(hipGetLastError());
+  SUMINC_HIP(hipMemsetAsync(F2, 0, 4, c->stream));
+  size_t bytes = 0;
+  SUMINC_HIP(rocprim::inclusive_scan(nullptr, bytes, hist, F2 + 1, (size_t)nseg2, rocprim::plus<uint32_t>(),
+                                     c->stream));
+  c->tmp.ensure(bytes);
+  SUMINC_HIP(rocprim::inclusive_scan(c->tmp.p, bytes, hist, F2 + 1, (size_t)nseg2, rocprim::plus<uint32_t>(),
+                                     c->stream));
+  c->tile_edges.ensure_slack(m_cap * 4 + 16);  // the summing pass reads 4-aligned groups
+  k_bin_scatter<<<J, kBlock, 0, c->stream>>>(d_pieces, J, ioff, cols, wl, P, F2, c->tile_edges.as<uint32_t>());
+  SUMINC_HIP(hipGetLastError());
+  // each (window, piece) segment sorted by slot: neighbouring lanes read
+  // neighbouring contributions (one line serves the edges of the hot sources), and
+  // the edges of one destination (adjacent in arrival order) are spread apart, so
+  // a wave's LDS additions seldom meet on one address
+  c->tile_edges2.ensure_slack(m_cap * 4 + 16);
+  bytes = 0;
+  SUMINC_HIP(rocprim::segmented_radix_sort_keys(nullptr, bytes, c->tile_edges.as<uint32_t>(),
+                                                c->tile_edges2.as<uint32_t>(), m_now, (unsigned)nseg2, F2, F2 + 1,
+                                                0u, wl, c->stream));
+  c->tmp.ensure(bytes);
+  SUMINC_HIP(rocprim::segmented_radix_sort_keys(c->tmp.p, bytes, c->tile_edges.as<uint32_t>(),
+                                                c->tile_edges2.as<uint32_t>(), m_now, (unsigned)nseg2, F2, F2 + 1,
+                                                0u, wl, c->stream));
+  c->tile_edges.swap(c->tile_edges2);
+  k_bin_fold<<<grid_for(nseg + 1), kBlock, 0, c->stream>>>(F2, J, d_first, B, P, F);
+  SUMINC_HIP(hipGetLastError());
+  std::vector<uint32_t> hF(nseg + 1);
+  SUMINC_HIP(hipMemcpyAsync(hF.data(), F, hF.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipStreamSynchronize(c->stream));
+  SUMINC_REQUIRE(hF[nseg] == m_now, SUMINC_E_STATE, "binned pull layout does not cover the CSC");
+  // the summing pass's parts: each block's run of 4-edge groups cut into parts of
+  // at most `ep` edges, enough parts to fill the chip
+  const uint64_t ep = std::min<uint64_t>(1u << 16, std::max<uint64_t>(1u << 12, m_now / 8192));
+  std::vector<uint4> parts;
+  for (uint32_t b = 0; b < B; ++b) {
+    uint64_t total = 0;
+    for (uint32_t w = 0; w < P; ++w) {
+      const uint32_t s = hF[(size_t)w * B + b], e = hF[(size_t)w * B + b + 1];
+      total += e > s ? ((e + 3) >> 2) - (s >> 2) : 0;
+    }
+    if (!total) continue;
+    const uint64_t np = (total + ep - 1) / ep, per = (total + np - 1) / np;
+    const uint32_t flag = np > 1 ? 0x80000000u : 0u;
+    for (uint64_t z = 0; z < total; z += per)
+      parts.push_back(make_uint4(b | flag, 0u, (uint32_t)z, (uint32_t)std::min(total, z + per)));
+  }
+  c->bin_parts.ensure(parts.size() * sizeof(uint4));
+  SUMINC_HIP(hipMemcpyAsync(c->bin_parts.p, parts.data(), parts.size() * sizeof(uint4), hipMemcpyHostToDevice,
+                            c->stream));
+  const uint32_t direct = c->prm.pull_bin_direct < 0 ? 1u : std::min<uint32_t>(P, (uint32_t)c->prm.pull_bin_direct);
+  c->bin_vals.ensure_slack((m_cap + 4) * es);
+  SUMINC_HIP(hipStreamSynchronize(c->stream));  // `parts`, `pieces` and `first` go out of scope
+  c->bin_n = (uint32_t)parts.size();
+  c->bin_P = P;
+  c->bin_B = B;
+  c->bin_wl = wl;
+  c->bin_bsize = bsize;
+  c->bin_direct = direct;
+  c->bin_first = hF[(size_t)direct * B];
+  c->bin_m = m_now;
+}
+
 void build_pull_lists(suminc_ctx* c, uint64_t m_cap) {
   const unsigned g = grid_for(c->n);
   c->pull_heavy.ensure_slack((m_cap / kChunk + m_cap / kHeavyDeg + 2) * sizeof(uint2));
@@ -1167,7 +1571,9 @@ void build_pull_lists(suminc_ctx* c, uint64_t m_cap) {
   SUMINC_HIP(hipGetLastError());
   build_src_positions(c, m_cap, !banded_pull(c));
   build_band_tables(c, m_cap);
-  build_tiles(c, m_cap);
+  build_bins(c, m_cap);
+  c->tile_n = 0;
+  if (!c->bin_n) build_tiles(c, m_cap);
   c->pull_prec = c->pr_prec;  // positions, bands and tiles depend on the element size
 }
 
@@ -1185,8 +1591,24 @@ void pull_scatter(suminc_ctx* c, int gate) {
   Counters* cnt = dev_counters(c);
   const unsigned long long* counts = c->pull_cnt.as<unsigned long long>();
   const uint32_t* icol = pull_cols(c);
+  if (c->bin_n) {
+    SUMINC_REQUIRE(c->bin_bsize <= kTileDst, SUMINC_E_STATE, "binned pull blocks exceed the LDS sums");
+    const uint32_t* F = c->bin_F.as<uint32_t>();
+    const uint64_t first = c->bin_first & ~3ull;  // 4-aligned, for the vector path
+    if (c->bin_m > c->bin_first)
+      k_pr_bin_gather<T><<<(unsigned)((c->bin_m - first + kBinGatherEdges - 1) / kBinGatherEdges), kBlock, 0,
+                           c->stream>>>(c->tile_edges.as<uint32_t>(), F, c->bin_B, c->bin_P, (uint32_t)first,
+                                        (uint32_t)c->bin_m, c->bin_wl,
+                                           c->pr_contrib.as<T>(), c->bin_vals.as<T>(), cnt, gate);
+    k_pr_bin_sum<T><<<c->bin_n, kBlock, 0, c->stream>>>(c->bin_parts.as<uint4>(), F, c->bin_B, c->bin_P,
+                                                       c->bin_bsize, c->n,
+                                                       c->tile_edges.as<uint32_t>(), c->bin_vals.as<T>(),
+                                                       c->pr_contrib.as<T>(), c->bin_wl, c->bin_direct,
+                                                       c->pr_delta.as<T>(), cnt, gate);
+    return;
+  }
   if (c->tile_n) {
-    SUMINC_REQUIRE(c->tile_bsize <= kTileAcc<T>, SUMINC_E_STATE, "tiled pull built for another precision");
+    SUMINC_REQUIRE(c->tile_bsize <= kTileDst, SUMINC_E_STATE, "tiled pull blocks exceed the LDS sums");
     k_pr_pull_tiled<T><<<c->tile_n, kBlock, 0, c->stream>>>(c->tile_tab.as<uint4>(), c->tile_edges.as<uint32_t>(),
                                                            c->tile_seg.as<uint32_t>(), c->tile_P, c->tile_win_log2,
                                                            c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, gate);

@@ -74,6 +74,33 @@ def test_pagerank_vs_fp64_oracle(gpu, scale, band_kib, compact_kib, tile_kib, pr
     assert np.abs(ro - rs).sum() <= sto["residual"] / (1 - DAMPING) + 1e-12
 
 
+# the binned pull: 1 or 4 KiB source windows (the value stream, then the summing
+# pass by destination block; blocks cut into parts that add atomically when long),
+# the first `direct` windows gathered by the summing pass itself (-1: the default,
+# one; 0: every window through the stream; 1000: none through it), the default
+# above 64 MiB of contributions
+@pytest.mark.parametrize("scale,compact_kib,bin_kib,direct", [(16, 1, 1, -1), (16, 0, 4, -1), (16, 1, 1, 0),
+                                                             (16, 1, 1, 1000), (12, 0, 1, 3), (16, 1, 4, 2)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_binned_pull_vs_fp64_oracle(gpu, scale, compact_kib, bin_kib, direct, prec):
+    g = oracle.Graph.rmat(scale, 16, seed=1)
+    off, col, _ = g.csr()
+    rs, _, _ = g.pr_solve(DAMPING, tol=1e-15)
+    T = 1e-9 * g.n
+    p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
+    with gpu.Context(pr_precision=p, termcheck_threshold=T, pull_compact_kib=compact_kib, pull_bin_kib=bin_kib,
+                     pull_bin_direct=direct) as ctx:
+        ctx.upload_csr(off, col)
+        ctx.pr_init()
+        st = ctx.pr_run()
+        r, d = ctx.pr_download()
+    assert st["residual"] < T
+    err = np.abs(r - rs).sum()
+    extra = 1e-12 if prec == "fp64" else 1e-6
+    assert err <= st["residual"] / (1 - DAMPING) + extra, (err, st)
+    assert mass_defect(off, r, d, DAMPING) < (1e-12 if prec == "fp64" else 1e-6)
+
+
 def test_l1_target_fp32_kahan(gpu):
     """FP32 deltas + Kahan ranks reach L1 <= 1e-6 vs the FP64 fixed point."""
     g = oracle.Graph.rmat(16, 16, seed=3)
@@ -117,16 +144,17 @@ def test_step_and_upload(gpu):
     assert np.abs(r - ro).max() < 1e-15 and np.abs(d - do).max() < 1e-15
 
 
-@pytest.mark.parametrize("band_kib,compact_kib,tile_kib", [(0, 0, -1), (16, 0, -1), (0, 1, -1), (16, 1, -1),
-                                                          (0, 0, 1), (0, 1, 1)])
+@pytest.mark.parametrize("band_kib,compact_kib,tile_kib,bin_kib", [(0, 0, -1, 0), (16, 0, -1, 0), (0, 1, -1, 0),
+                                                                  (16, 1, -1, 0), (0, 0, 1, 0), (0, 1, 1, 0),
+                                                                  (0, 1, -1, 1), (0, 0, -1, 2)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_incremental_reseed_matches_scratch(gpu, prec, band_kib, compact_kib, tile_kib):
+def test_incremental_reseed_matches_scratch(gpu, prec, band_kib, compact_kib, tile_kib, bin_kib):
     g = oracle.Graph.rmat(14, 16, seed=3)
     off, col, _ = g.csr()
     T = 1e-10 * g.n
     p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
     with gpu.Context(pr_precision=p, termcheck_threshold=T, pull_band_kib=band_kib,
-                     pull_compact_kib=compact_kib, pull_tile_kib=tile_kib) as ctx:
+                     pull_compact_kib=compact_kib, pull_tile_kib=tile_kib, pull_bin_kib=bin_kib) as ctx:
         ctx.upload_csr(off, col)
         ctx.pr_init()
         ctx.pr_run()
@@ -209,14 +237,16 @@ def test_oversized_allocation_reports_nomem(gpu):
         assert ctx.pr_run()["residual"] < 1e-9
 
 
-def test_tiled_pull_follows_precision_and_growth(gpu):
-    """The tiled layout is rebuilt when the precision changes (its LDS tile holds
-    8192 FP32 or 4096 FP64 sums) and after batches that grow the graph (the tiles
-    cover the new CSC, not the old edge count)."""
+@pytest.mark.parametrize("layout", ["tiled", "binned"])
+def test_tiled_pull_follows_precision_and_growth(gpu, layout):
+    """The tiled and binned layouts are rebuilt when the precision changes (their
+    source windows are sized in bytes of contributions) and after batches that grow
+    the graph (the layout covers the new CSC, not the old edge count)."""
     g = oracle.Graph.rmat(14, 16, seed=2)
     off, col, _ = g.csr()
     T = 1e-10 * g.n
-    with gpu.Context(pr_precision=gpu.PR_FP32_KAHAN, termcheck_threshold=T, pull_tile_kib=1) as ctx:
+    kw = {"pull_tile_kib": 1} if layout == "tiled" else {"pull_bin_kib": 1}
+    with gpu.Context(pr_precision=gpu.PR_FP32_KAHAN, termcheck_threshold=T, **kw) as ctx:
         ctx.upload_csr(off, col)
         for prec in (gpu.PR_FP32_KAHAN, gpu.PR_FP64, gpu.PR_FP32_KAHAN):
             ctx.set_params(pr_precision=prec)
@@ -237,20 +267,21 @@ def test_tiled_pull_follows_precision_and_growth(gpu):
         assert ctx.graph_info()[1] > len(col)
 
 
-@pytest.mark.parametrize("path", ["plain", "sketch", "tiled"])
+@pytest.mark.parametrize("path", ["plain", "sketch", "tiled", "binned", "sketch-binned"])
 def test_incremental_fp32_meets_l1_target(gpu, path):
     """The north-star bar after every incremental FP32/Kahan batch: L1 <= 1e-6 against
     the FP64 fixed point of the updated graph, at the benchmark's stop rule T = 1e-7,
-    on the plain path, the SumInc query over the sketch and the tiled pull."""
+    on the plain path, the SumInc query over the sketch, the tiled and the binned
+    pull."""
     g = oracle.Graph.rmat(14, 16, seed=6)
     off, col, _ = g.csr()
-    kw = {"pull_tile_kib": 1} if path == "tiled" else {}
+    kw = {"pull_tile_kib": 1} if path == "tiled" else {"pull_bin_kib": 1} if path.endswith("binned") else {}
     with gpu.Context(pr_precision=gpu.PR_FP32_KAHAN, termcheck_threshold=1e-7, **kw) as ctx:
         ctx.upload_csr(off, col)
-        if path == "sketch":
+        if path.startswith("sketch"):
             ctx.cluster_leaves(1024, 2)
             ctx.sum_build(2, 1024, 1.0)
-        run = ctx.sum_pr_run if path == "sketch" else ctx.pr_run
+        run = ctx.sum_pr_run if path.startswith("sketch") else ctx.pr_run
         ctx.pr_init()
         run()
         for b in range(3):

@@ -102,16 +102,17 @@ def test_sum_pr_query_equals_fixed_point(gpu, kind, prec, compact):
 # large graphs), the residual phase in 16 KiB source bands over positions in id
 # order; compact=2: positions by decreasing out-degree, unbanded (the large-graph
 # default), so the sketch rounds' skip test reads the cluster by position;
-# compact=3: the same positions under the tiled pull (1 KiB source windows)
+# compact=3: the same positions under the tiled pull (1 KiB source windows);
+# compact=4: under the binned pull (1 KiB windows)
 @pytest.mark.parametrize("prec,compact", [("fp64", 0), ("fp32", 0), ("fp32", 1), ("fp32", 2), ("fp64", 2), ("fp32", 3),
-                                          ("fp64", 3)])
+                                          ("fp64", 3), ("fp32", 4), ("fp64", 4)])
 def test_sum_pr_incremental(gpu, prec, compact):
     g = oracle.Graph.rmat(13, 16, seed=3)
     off, col, _ = g.csr()
     T = 1e-10 * g.n
     p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
     kw = {1: {"pull_compact_kib": 1, "pull_band_kib": 16}, 2: {"pull_compact_kib": 1},
-          3: {"pull_compact_kib": 1, "pull_tile_kib": 1}}.get(compact, {})
+          3: {"pull_compact_kib": 1, "pull_tile_kib": 1}, 4: {"pull_compact_kib": 1, "pull_bin_kib": 1}}.get(compact, {})
     with gpu.Context(pr_precision=p, termcheck_threshold=T, **kw) as ctx:
         ctx.upload_csr(off, col)
         ctx.set_clusters(blocks(g.n, 4096, 8))
