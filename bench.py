@@ -83,14 +83,14 @@ def pull_kw(args) -> dict:
 
 def pull_mode(args, n) -> str:
     """Which pull the library's dense PageRank rounds run (its rules: the binned pull
-    when a window is given, or the FP32 contributions exceed 64 MiB and it is not
-    off; else the tiled pull by the same rule; else the row classes)."""
+    when a window is given, or the FP32 contributions exceed 8 MiB and it is not
+    off; else the tiled pull when a window is given or they exceed 64 MiB; else the
+    row classes)."""
     if args.pull_band_kib > 0:
         return "rows"
-    big = n * 4 > (64 << 20)
-    if args.pull_bin_kib > 0 or (args.pull_bin_kib == 0 and big):
+    if args.pull_bin_kib > 0 or (args.pull_bin_kib == 0 and n * 4 > (8 << 20)):
         return "binned"
-    if args.pull_tile_kib > 0 or (args.pull_tile_kib == 0 and big):
+    if args.pull_tile_kib > 0 or (args.pull_tile_kib == 0 and n * 4 > (64 << 20)):
         return "tiled"
     return "rows"
 

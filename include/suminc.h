@@ -85,7 +85,7 @@ typedef struct suminc_params {
                                  renamed to dense positions (by decreasing out-degree; in id order when
                                  banded), once the contributions exceed this many KiB (0: 32768;
                                  negative: never) */
-  int32_t pull_tile_kib;      /* tiled pull: when the banded pull is off and the contributions exceed 64 MiB,
+  int32_t pull_tile_kib;      /* tiled pull: when the banded and binned pulls are off and the contributions exceed 64 MiB,
                                  a dense round's gather runs one workgroup per tile of destinations (their
                                  sums in LDS) over source windows of this many KiB of contributions, taken
                                  in the same order by every workgroup (0: 1024; negative: never; the
@@ -94,7 +94,7 @@ typedef struct suminc_params {
                                  first copies the contributions named by the in-edges into an edge-ordered
                                  value stream, one source window of this many KiB at a time (every window in
                                  the caches while it is read), then sums the stream by destination block in
-                                 LDS; takes precedence over the tiled pull (0: on above 64 MiB of
+                                 LDS; takes precedence over the tiled pull (0: on above 8 MiB of
                                  contributions, 2048 KiB windows; negative: never; > 0: always) */
   int32_t pull_bin_direct;    /* binned pull: the first windows (the hottest sources) that the summing pass
                                  gathers directly instead of through the value stream (negative: the

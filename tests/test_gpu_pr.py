@@ -46,7 +46,7 @@ def test_golden_test_base(gpu):
 # by default above 32 MiB of contributions (8.4M vertices in FP32); tile_kib: the
 # tiled pull (destination tiles summed in LDS over source windows of 1 or 4 KiB,
 # blocks of the hub rows cut into several tiles that add atomically), the default
-# above 64 MiB of contributions
+# above 64 MiB of contributions when the binned pull is off
 @pytest.mark.parametrize("scale,band_kib,compact_kib,tile_kib", [(12, 0, 0, -1), (16, 0, 0, -1), (16, 16, 0, -1),
                                                                  (16, 0, 1, -1), (16, 16, 1, -1), (16, 0, 0, 1),
                                                                  (16, 0, 1, 1), (16, 0, 1, 4), (12, 0, 0, 1)])
@@ -78,7 +78,7 @@ def test_pagerank_vs_fp64_oracle(gpu, scale, band_kib, compact_kib, tile_kib, pr
 # pass by destination block; blocks cut into parts that add atomically when long),
 # the first `direct` windows gathered by the summing pass itself (-1: the default,
 # one; 0: every window through the stream; 1000: none through it), the default
-# above 64 MiB of contributions
+# above 8 MiB of contributions
 @pytest.mark.parametrize("scale,compact_kib,bin_kib,direct", [(16, 1, 1, -1), (16, 0, 4, -1), (16, 1, 1, 0),
                                                              (16, 1, 1, 1000), (12, 0, 1, 3), (16, 1, 4, 2)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
