@@ -758,13 +758,25 @@ __global__ void k_col_pos(uint32_t n, const uint32_t* __restrict__ ioff, const u
     icol_c[e] = pos[icol[e]];
 }
 
-// Degree order: keys ~outdeg (a stable sort puts larger out-degrees first, ties
-// by id, dangling vertices last), then pos[id] = rank.
+// Degree order by classes: four classes per octave of the degree (the leading bit
+// and the two below it), 8-bit keys, larger degrees first, dangling vertices
+// (degree 0) last.  The stable sort keeps id order inside a class, so a vertex
+// pass writing pos[v] for consecutive v writes each class's positions in order
+// (a few sequential streams instead of scattered 4-byte writes), while the hot
+// sources still fill the first positions.  Then pos[id] = rank.
+constexpr uint32_t kDegKeyDangling = 255;
+__device__ __forceinline__ uint32_t degree_class_key(uint32_t deg) {
+  if (deg == 0) return kDegKeyDangling;
+  const uint32_t lead = 31u - (uint32_t)__clz(deg);  // floor(log2 deg)
+  const uint32_t sub = lead >= 2 ? (deg >> (lead - 2)) & 3u : (deg << (2 - lead)) & 3u;
+  return kDegKeyDangling - 1u - (lead * 4u + sub);  // in [126, 254]
+}
+
 __global__ void k_deg_keys(uint32_t n, const uint32_t* __restrict__ off, uint32_t* __restrict__ key,
                            uint32_t* __restrict__ id) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v < n) {
-    key[v] = 0xFFFFFFFFu - (off[v + 1] - off[v]);
+    key[v] = degree_class_key(off[v + 1] - off[v]);
     id[v] = v;
   }
 }
@@ -772,7 +784,7 @@ __global__ void k_deg_keys(uint32_t n, const uint32_t* __restrict__ off, uint32_
 __global__ void k_deg_pos(uint32_t n, const uint32_t* __restrict__ key, const uint32_t* __restrict__ id,
                           uint32_t* __restrict__ pos) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) pos[id[i]] = key[i] != 0xFFFFFFFFu ? i : INF32;
+  if (i < n) pos[id[i]] = key[i] != kDegKeyDangling ? i : INF32;
 }
 
 void build_positions_by_degree(suminc_ctx* c, const uint32_t* deg_off, const uint32_t* rows_off,
@@ -787,9 +799,9 @@ void build_positions_by_degree(suminc_ctx* c, const uint32_t* deg_off, const uin
   uint32_t* i2 = k2 + n;
   size_t bytes = 0;
   k_deg_keys<<<grid_for(n), kBlock, 0, c->stream>>>(n, deg_off, k1, i1);
-  SUMINC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, k1, k2, i1, i2, (size_t)n, 0, 32, c->stream));
+  SUMINC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, k1, k2, i1, i2, (size_t)n, 0, 8, c->stream));
   c->tmp.ensure(bytes);
-  SUMINC_HIP(rocprim::radix_sort_pairs(c->tmp.p, bytes, k1, k2, i1, i2, (size_t)n, 0, 32, c->stream));
+  SUMINC_HIP(rocprim::radix_sort_pairs(c->tmp.p, bytes, k1, k2, i1, i2, (size_t)n, 0, 8, c->stream));
   k_deg_pos<<<grid_for(n), kBlock, 0, c->stream>>>(n, k2, i2, pos.as<uint32_t>());
   k_col_pos<<<grid_for(m_cap, kBlock, 65536), kBlock, 0, c->stream>>>(n, rows_off, cols, pos.as<uint32_t>(),
                                                                       cols_c.as<uint32_t>());

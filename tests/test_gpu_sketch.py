@@ -100,7 +100,7 @@ def test_sum_pr_query_equals_fixed_point(gpu, kind, prec, compact):
 
 # compact=1: both pulls gather from compact source positions (the default only on
 # large graphs), the residual phase in 16 KiB source bands over positions in id
-# order; compact=2: positions by decreasing out-degree, unbanded (the large-graph
+# order; compact=2: positions by decreasing out-degree class, unbanded (the large-graph
 # default), so the sketch rounds' skip test reads the cluster by position;
 # compact=3: the same positions under the tiled pull (1 KiB source windows);
 # compact=4: under the binned pull (1 KiB windows)
