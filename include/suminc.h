@@ -82,10 +82,9 @@ typedef struct suminc_params {
   int32_t pull_band_kib;      /* banded pull: a dense PageRank round gathers the contributions of one source
                                  band of at most this many KiB at a time (<= 0: one band, the default) */
   int32_t pull_compact_kib;   /* the plain pull gathers contributions of the vertices with out-edges only,
-                                 renamed to dense positions (by decreasing out-degree class, four classes an octave, id order
-                                 inside a class; in id order when
-                                 banded), once the contributions exceed this many KiB (0: 32768;
-                                 negative: never) */
+                                 renamed to dense positions (by decreasing out-degree class, four classes
+                                 an octave, id order inside a class; in id order when banded), once the
+                                 contributions exceed this many KiB (0: 32768; negative: never) */
   int32_t pull_tile_kib;      /* tiled pull: when the banded and binned pulls are off and the contributions exceed 64 MiB,
                                  a dense round's gather runs one workgroup per tile of destinations (their
                                  sums in LDS) over source windows of this many KiB of contributions, taken
