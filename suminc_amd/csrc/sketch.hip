@@ -1420,16 +1420,29 @@ __global__ void __launch_bounds__(kBlock) k_sk_count(SkArgs a, const T* __restri
   double x = 0;
   unsigned long long ed = 0;
   const uint32_t base = blockIdx.x * (kBlock * kSkPer);
+  // the kSkPer vertices' type, delta and row bounds loaded first (delta of a type
+  // that holds none is read and ignored), then the sums
+  uint8_t ty[kSkPer];
+  T dl[kSkPer];
+  uint32_t o0[kSkPer], o1[kSkPer];
 #pragma unroll
   for (int j = 0; j < kSkPer; ++j) {
     const uint32_t v = base + j * kBlock + threadIdx.x;
-    if (v >= a.n) break;
-    const uint8_t t = a.ntype[v];
+    const bool in = v < a.n;
+    ty[j] = in ? a.ntype[v] : (uint8_t)4;
+    dl[j] = in ? delta[v] : T(0);
+    o0[j] = in ? a.off[v] : 0u;
+    o1[j] = in ? a.off[v + 1] : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < kSkPer; ++j) {
+    const uint32_t v = base + j * kBlock + threadIdx.x;
+    const uint8_t t = ty[j];
     if (t == 4) continue;
-    const uint32_t deg = a.off[v + 1] - a.off[v];
+    const uint32_t deg = o1[j] - o0[j];
     double dv = 0;
     if (t != 1) {
-      dv = (double)fabs_t(delta[v]);
+      dv = (double)fabs_t(dl[j]);
       x += dv;
       if (t == 0 && dv > eps) ed += deg;
     }
@@ -1540,6 +1553,21 @@ __global__ void __launch_bounds__(kBlock) k_sk_send(SkArgs a, T* __restrict__ de
   const bool pull = a.cnt->mode != 0;  // contributions for the CSC pull instead of push items
   const uint32_t base = blockIdx.x * (kBlock * kSkPer);
   unsigned long long ed = 0;
+  // the kSkPer vertices' type, delta and row bounds loaded first
+  uint8_t ty[kSkPer];
+  T dl[kSkPer];
+  uint32_t o0[kSkPer], o1[kSkPer], ps[kSkPer];
+#pragma unroll
+  for (int j = 0; j < kSkPer; ++j) {
+    const uint32_t v = base + j * kBlock + threadIdx.x;
+    const bool in = v < a.n;
+    ty[j] = in ? a.ntype[v] : (uint8_t)4;
+    dl[j] = in ? delta[v] : T(0);
+    o0[j] = in ? a.off[v] : 0u;
+    o1[j] = in ? a.off[v + 1] : 0u;
+    // the contribution's slot (vertices without out-edges have no position and never send)
+    ps[j] = in && pull && a.pos ? a.pos[v] : v;
+  }
 #pragma unroll
   for (int j = 0; j < kSkPer; ++j) {  // uniform trip count: push_item appends per wave
     const uint32_t v = base + j * kBlock + threadIdx.x;
@@ -1547,17 +1575,17 @@ __global__ void __launch_bounds__(kBlock) k_sk_send(SkArgs a, T* __restrict__ de
     T cval = T(0);
     uint32_t deg = 0;
     if (v < a.n) {
-      const uint8_t t = a.ntype[v];
-      deg = a.off[v + 1] - a.off[v];
+      const uint8_t t = ty[j];
+      deg = o1[j] - o0[j];
       if (t == 0) {  // SingleNode: plain push over its out-list
-        const T dv = delta[v];
+        const T dv = dl[j];
         if ((double)fabs_t(dv) > eps) {
           delta[v] = T(0);
           kahan_add(rank, comp, v, dv);
           if (deg) { want_push = true; cval = d * dv / (T)deg; }
         }
       } else if (t == 2 || t == 3) {  // entry: inject into the shortcut
-        const T dv = delta[v];
+        const T dv = dl[j];
         T sval = T(0);
         if ((double)fabs_t(dv) > eps) {
           delta[v] = T(0);
@@ -1567,7 +1595,7 @@ __global__ void __launch_bounds__(kBlock) k_sk_send(SkArgs a, T* __restrict__ de
         }
         sc_x[a.ent_of[v]] = sval;  // every entry's slot is written each round (0: idle)
       }
-      if (pull) sk_contrib(a, contrib, v, want_push ? cval : T(0));  // every slot: the exits' come after
+      if (pull && ps[j] != INF32) contrib[ps[j]] = want_push ? cval : T(0);  // every slot: the exits' come after
     }
     push_item<T>(want_push && !pull, v, cval, deg, a.wl_v, wl_c, a.heavy, a.cnt);
     if (want_push) ed += deg;
