@@ -1563,13 +1563,17 @@ static void build_bins(suminc_ctx* c, uint64_t m_cap) {
   // the summing pass's parts: each block's run of 4-edge groups cut into parts of
   // at most `ep` edges, enough parts to fill the chip
   const uint64_t ep = std::min<uint64_t>(1u << 16, std::max<uint64_t>(1u << 12, m_now / 8192));
+  std::vector<uint64_t> totals(B, 0);  // groups per block, window by window (sequential in hF)
+  for (uint32_t w = 0; w < P; ++w) {
+    const uint32_t* fw = hF.data() + (size_t)w * B;
+    for (uint32_t b = 0; b < B; ++b) {
+      const uint32_t s = fw[b], e = fw[b + 1];
+      totals[b] += e > s ? ((e + 3) >> 2) - (s >> 2) : 0;
+    }
+  }
   std::vector<uint4> parts;
   for (uint32_t b = 0; b < B; ++b) {
-    uint64_t total = 0;
-    for (uint32_t w = 0; w < P; ++w) {
-      const uint32_t s = hF[(size_t)w * B + b], e = hF[(size_t)w * B + b + 1];
-      total += e > s ? ((e + 3) >> 2) - (s >> 2) : 0;
-    }
+    const uint64_t total = totals[b];
     if (!total) continue;
     const uint64_t np = (total + ep - 1) / ep, per = (total + np - 1) / np;
     const uint32_t flag = np > 1 ? 0x80000000u : 0u;
