@@ -865,7 +865,9 @@ def run_wpr(args, wl):
     achieved = alg / (kms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_spmv_* + k_wpr_step (series rounds)", "avg_launch_us": round(1e3 * kms / rounds, 2),
+            "kernel": ("k_pr_bin_gather+k_pr_bin_sum (with coefficients)" if pull_mode(args, n) == "binned"
+                       else "k_spmv") + " + k_wpr_step (series rounds)",
+            "avg_launch_us": round(1e3 * kms / rounds, 2),
             "alg_bytes_per_launch": round(alg / rounds), "kernel_ms_total": round(kms, 3)}
     r32, _ = ctx.wpr_download()
     off, col, w = ctx.download_csr()

@@ -77,6 +77,19 @@ struct DevBuf {
   }
 };
 
+// A binned pull layout (pr.hip): the in-edges of a CSR-shaped adjacency (rows =
+// destinations, columns = source slots) in (source window, destination block)
+// order, each packed into 32 bits ((destination - block start) << 19 | slot within
+// the window); F[w * B + b] the offset of segment (w, b) (P * B + 1 values); the
+// summing pass's parts (uint4: block | split << 31, 0, first and end 4-edge group of
+// the block's run); the value stream (one element per edge); optionally the row
+// order's index of every edge (eid) and a coefficient per edge in layout order.
+struct BinLayout {
+  DevBuf packed, packed2, eid, eid2, F2, F, hist, pieces, parts, vals, bounds, coef;
+  uint32_t n_parts = 0, P = 0, B = 0, wl = 0, bsize = 0, direct = 0;
+  uint64_t first = 0, m = 0;  // the value stream's edges: [first, m)
+};
+
 // Device-side counters block (one allocation, read back in one copy).
 struct Counters {
   unsigned long long wl_count[2];   // worklist lengths (double-buffered frontiers)
@@ -218,14 +231,8 @@ struct suminc_ctx {
   // tile's window offsets (tile_P + 1 per tile)
   suminc::DevBuf tile_tab, tile_edges, tile_seg, tile_bounds, tile_edges2, tile_flat;
   uint32_t tile_n = 0, tile_P = 0, tile_win_log2 = 0, tile_bsize = 0;
-  // binned pull (per graph version, with the pull lists; exclusive with the tiles):
-  // the in-edges in (source window, destination block) order, packed as the tiles'
-  // are, in tile_edges; segment offsets bin_F[w * bin_B + b] (bin_P * bin_B + 1);
-  // the summing pass's parts (uint4: block | split << 31, first | last window << 16,
-  // first and end edge); the value stream (one element per in-edge)
-  suminc::DevBuf bin_F, bin_hist, bin_parts, bin_vals;
-  uint32_t bin_n = 0, bin_P = 0, bin_B = 0, bin_wl = 0, bin_bsize = 0, bin_direct = 0;
-  uint64_t bin_first = 0, bin_m = 0;  // the value stream's edges: [bin_first, bin_m)
+  // binned pull (per graph version, with the pull lists; exclusive with the tiles)
+  suminc::BinLayout bins;
   int pull_prec = -1;  // the PageRank precision the pull lists, positions and tiles were built for
   uint64_t graph_version = 0;
 

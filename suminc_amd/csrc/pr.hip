@@ -222,8 +222,10 @@ __global__ void __launch_bounds__(kBlock) k_pr_gather(uint32_t n, const uint32_t
 }
 
 // Whether the pull launches of this point in the stream run: gate 0, a round
-// that chose pull (not stopped); gate 1, a reseed that chose pull.
+// that chose pull (not stopped); gate 1, a reseed that chose pull; gate 2, a
+// weighted series round (not stopped).
 __device__ __forceinline__ bool pull_runs(const Counters* cnt, int gate) {
+  if (gate == 2) return cnt == nullptr || !cnt->done;  // a series round (weighted PageRank)
   return gate ? cnt->reseed_mode == 1 : (!cnt->done && cnt->mode == 1);
 }
 
@@ -1192,14 +1194,13 @@ static void build_tiles(suminc_ctx* c, uint64_t m_cap) {
 // ---------------------------------------------------------------------------------
 constexpr uint32_t kBinGatherEdges = 8192;  // in-edges per workgroup of the gather pass
 
-// Vertices with out-edges when the compact positions are in use (the positions'
-// count), else n.  Waits for the stream.
-static unsigned long long pull_source_count(suminc_ctx* c) {
-  if (!c->pull_compact) return c->n;
-  c->bin_hist.ensure(8);
-  unsigned long long* d = c->bin_hist.as<unsigned long long>();
+// Rows with at least one entry in off (n + 1 row pointers), counted on the
+// device into scratch (8 bytes).  Waits for the stream.
+unsigned long long nonempty_rows(suminc_ctx* c, const uint32_t* off, DevBuf& scratch) {
+  scratch.ensure(8);
+  unsigned long long* d = scratch.as<unsigned long long>();
   SUMINC_HIP(hipMemsetAsync(d, 0, 8, c->stream));
-  k_count_sources<<<grid_for(c->n, kBlock, 1024), kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), d);
+  k_count_sources<<<grid_for(c->n, kBlock, 1024), kBlock, 0, c->stream>>>(c->n, off, d);
   unsigned long long h = 0;
   SUMINC_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
@@ -1229,7 +1230,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_scatter(const uint4* __restrict_
                                                         const uint32_t* __restrict__ ioff,
                                                         const uint32_t* __restrict__ icol, uint32_t wl, uint32_t P,
                                                         const uint32_t* __restrict__ F2,
-                                                        uint32_t* __restrict__ packed) {
+                                                        uint32_t* __restrict__ packed, uint32_t* __restrict__ eid) {
   __shared__ uint32_t s_at[kTileMaxWindows];
   __shared__ uint32_t s_scan[kBlock + 1];
   __shared__ uint32_t s_start[kBlock];
@@ -1254,6 +1255,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_scatter(const uint4* __restrict_
       const uint32_t p = icol[e];
       const uint32_t at = atomicAdd(&s_at[p >> wl], 1u);
       packed[at] = ((r0 + (uint32_t)slot - d0) << kTileSrcBits) | (p & mask);
+      if (eid) eid[at] = e;  // the edge's index in row order, for per-edge coefficients
     });
   }
 }
@@ -1281,13 +1283,14 @@ using vec4_t = T __attribute__((ext_vector_type(4)));
 // thread, all loads issued before the gathers.  A full workgroup inside one window
 // (the window of its first edge by binary search over the window starts F[w * B])
 // takes the vector path; the others (a window start or the end inside) go edge by
-// edge, each thread's window only moving forward.
-template <class T>
+// edge, each thread's window only moving forward.  COEF: each value is multiplied
+// by its edge's coefficient (coef, layout order).
+template <class T, bool COEF>
 __global__ void __launch_bounds__(kBlock) k_pr_bin_gather(const uint32_t* __restrict__ packed,
                                                           const uint32_t* __restrict__ F, uint32_t B, uint32_t P,
                                                           uint32_t first, uint32_t end, uint32_t wl,
-                                                          const T* __restrict__ contrib, T* __restrict__ vals,
-                                                          const Counters* cnt, int gate) {
+                                                          const T* __restrict__ contrib, const T* __restrict__ coef,
+                                                          T* __restrict__ vals, const Counters* cnt, int gate) {
   constexpr int G = kBinGatherEdges / (4 * kBlock);
   __shared__ uint32_t s_w;
   if (!pull_runs(cnt, gate)) return;
@@ -1313,6 +1316,12 @@ __global__ void __launch_bounds__(kBlock) k_pr_bin_gather(const uint32_t* __rest
     u32x4 q[G];
 #pragma unroll
     for (int r = 0; r < G; ++r) q[r] = __builtin_nontemporal_load(p4 + threadIdx.x + r * kBlock);
+    vec4_t<T> a[G];
+    if constexpr (COEF) {
+      const vec4_t<T>* c4 = reinterpret_cast<const vec4_t<T>*>(coef + i0);
+#pragma unroll
+      for (int r = 0; r < G; ++r) a[r] = __builtin_nontemporal_load(c4 + threadIdx.x + r * kBlock);
+    }
     vec4_t<T> x[G];
 #pragma unroll
     for (int r = 0; r < G; ++r) {
@@ -1320,6 +1329,7 @@ __global__ void __launch_bounds__(kBlock) k_pr_bin_gather(const uint32_t* __rest
       x[r].y = win[q[r].y & mask];
       x[r].z = win[q[r].z & mask];
       x[r].w = win[q[r].w & mask];
+      if constexpr (COEF) x[r] *= a[r];
     }
 #pragma unroll
     for (int r = 0; r < G; ++r) __builtin_nontemporal_store(x[r], v4 + threadIdx.x + r * kBlock);
@@ -1327,7 +1337,8 @@ __global__ void __launch_bounds__(kBlock) k_pr_bin_gather(const uint32_t* __rest
     uint32_t w = w0;  // a thread's edges ascend, so its window only moves forward
     for (uint32_t k = i0 + threadIdx.x; k < i1; k += kBlock) {
       while (w + 1 < P && F[(size_t)(w + 1) * B] <= k) ++w;
-      vals[k] = contrib[((size_t)w << wl) + (packed[k] & mask)];
+      const T v = contrib[((size_t)w << wl) + (packed[k] & mask)];
+      vals[k] = COEF ? v * coef[k] : v;
     }
   }
 }
@@ -1342,13 +1353,13 @@ __global__ void __launch_bounds__(kBlock) k_pr_bin_gather(const uint32_t* __rest
 // edges) an iteration with all their loads issued first.
 constexpr uint32_t kBinBlock = 512;  // threads of a summing workgroup (four fit a CU's LDS)
 constexpr uint32_t kBinMaxWindows = kBinBlock;
-template <class T>
+template <class T, bool COEF>
 __global__ void __launch_bounds__(kBinBlock) k_pr_bin_sum(const uint4* __restrict__ parts,
                                                        const uint32_t* __restrict__ F, uint32_t B, uint32_t P,
                                                        uint32_t bsize, uint32_t n, const uint32_t* __restrict__ packed,
                                                        const T* __restrict__ vals, const T* __restrict__ contrib,
-                                                       uint32_t wl, uint32_t direct, T* __restrict__ delta,
-                                                       const Counters* cnt, int gate) {
+                                                       const T* __restrict__ coef, uint32_t wl, uint32_t direct,
+                                                       T* __restrict__ delta, const Counters* cnt, int gate) {
   constexpr int U = 4;
   __shared__ double s_acc[kTileDst];
   __shared__ uint32_t s_beg[kBinMaxWindows];
@@ -1411,9 +1422,12 @@ __global__ void __launch_bounds__(kBinBlock) k_pr_bin_sum(const uint4* __restric
     vec4_t<T> x[U];
 #pragma unroll
     for (int r = 0; r < U; ++r) q[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(packed + k0[r]));
+    // the streamed groups' values; a direct group reads its coefficients (COEF) or
+    // the stream's first group (masked off below)
 #pragma unroll
     for (int r = 0; r < U; ++r)
-      x[r] = __builtin_nontemporal_load(reinterpret_cast<const vec4_t<T>*>(vals + (dir[r] ? 0u : k0[r])));
+      x[r] = __builtin_nontemporal_load(
+          reinterpret_cast<const vec4_t<T>*>(dir[r] ? (COEF ? coef + k0[r] : vals) : vals + k0[r]));
     bool in[U][4];
 #pragma unroll
     for (int r = 0; r < U; ++r)
@@ -1434,10 +1448,10 @@ __global__ void __launch_bounds__(kBinBlock) k_pr_bin_sum(const uint4* __restric
       }
 #pragma unroll
       for (int r = 0; r < U; ++r) {
-        x[r].x = in[r][0] ? a[r][0] : T(0);
-        x[r].y = in[r][1] ? a[r][1] : T(0);
-        x[r].z = in[r][2] ? a[r][2] : T(0);
-        x[r].w = in[r][3] ? a[r][3] : T(0);
+        x[r].x = in[r][0] ? (COEF ? a[r][0] * x[r].x : a[r][0]) : T(0);
+        x[r].y = in[r][1] ? (COEF ? a[r][1] * x[r].y : a[r][1]) : T(0);
+        x[r].z = in[r][2] ? (COEF ? a[r][2] * x[r].z : a[r][2]) : T(0);
+        x[r].w = in[r][3] ? (COEF ? a[r][3] * x[r].w : a[r][3]) : T(0);
       }
     } else {
 #pragma unroll
@@ -1446,10 +1460,10 @@ __global__ void __launch_bounds__(kBinBlock) k_pr_bin_sum(const uint4* __restric
           const T* win = contrib + ((size_t)j[r] << wl);
           const T a0 = win[in[r][0] ? q[r].x & mask : 0u], a1 = win[in[r][1] ? q[r].y & mask : 0u];
           const T a2 = win[in[r][2] ? q[r].z & mask : 0u], a3 = win[in[r][3] ? q[r].w & mask : 0u];
-          x[r].x = a0;
-          x[r].y = a1;
-          x[r].z = a2;
-          x[r].w = a3;
+          x[r].x = COEF ? a0 * x[r].x : a0;
+          x[r].y = COEF ? a1 * x[r].y : a1;
+          x[r].z = COEF ? a2 * x[r].z : a2;
+          x[r].w = COEF ? a3 * x[r].w : a3;
         }
         if (!in[r][0]) x[r].x = T(0);
         if (!in[r][1]) x[r].y = T(0);
@@ -1475,35 +1489,31 @@ __global__ void __launch_bounds__(kBinBlock) k_pr_bin_sum(const uint4* __restric
   }
 }
 
-// The binned layout of the current graph (after the positions).
-static void build_bins(suminc_ctx* c, uint64_t m_cap) {
-  c->bin_n = 0;
-  const int kib = c->prm.pull_bin_kib;
-  const uint64_t es = c->pr_prec == SUMINC_PR_FP64 ? 8 : 4;
-  const uint32_t n = c->n;
-  if (kib < 0 || c->band_P > 1 || n == 0) return;
-  // measured on MI355X (FP32, 2 MiB windows, the row classes beside it): 0.25
-  // against 0.38 ms per dense round at 2^22 vertices, 0.86 against 1.70 at 2^24,
-  // 4.65 against 8.6 at 2^26
-  if (kib == 0 && (uint64_t)n * es <= (8ull << 20)) return;
-  const uint64_t win_bytes = (uint64_t)(kib ? kib : 2048) * 1024;
-  const uint32_t bsize = tile_dst(c->pr_prec);  // the summing pass's LDS holds as many destinations
+// Builds L for the adjacency (rows_off, cols: n rows, columns < n_src) with source
+// windows of win_bytes / es slots; leaves L.n_parts = 0 when the layout does not
+// apply (no edges, more than kBinMaxWindows windows).  with_eid: L.eid holds the row
+// order index of every edge, and L.coef has room for a coefficient per edge.
+// m_cap: the edge capacity the buffers are sized for.  Waits for the stream.
+void bin_layout_build(suminc_ctx* c, BinLayout& L, uint32_t n, const uint32_t* rows_off, const uint32_t* cols,
+                      unsigned long long n_src, uint64_t es, uint64_t win_bytes, int direct_param, uint64_t m_cap,
+                      bool with_eid) {
+  L.n_parts = 0;
+  if (n == 0) return;
+  n_src = std::max<unsigned long long>(n_src, 1);
+  const uint32_t bsize = kTileDst;  // the summing pass's LDS holds as many destinations
   const uint32_t B = (n + bsize - 1) / bsize;
-  const unsigned long long n_src = std::max<unsigned long long>(pull_source_count(c), 1);
   uint32_t wl = 8;
   while (wl < kTileSrcBits && (2ull << wl) * es <= win_bytes) ++wl;
-  while (wl < kTileSrcBits && ((n_src + (1ull << wl) - 1) >> wl) > kTileMaxWindows) ++wl;
+  while (wl < kTileSrcBits && ((n_src + (1ull << wl) - 1) >> wl) > kBinMaxWindows) ++wl;
   const uint32_t P = (uint32_t)((n_src + (1ull << wl) - 1) >> wl);
   if (P > kBinMaxWindows) return;  // beyond 2^28 sources: the tiles or the row classes
-  const uint32_t* ioff = c->in_off.as<uint32_t>();
-  const uint32_t* cols = pull_cols(c);
-  // the blocks' CSC bounds, cut into pieces of at most kTileEdges edges
-  c->tile_bounds.ensure(((size_t)B + 1) * 4);
-  k_tile_bounds<<<grid_for((uint64_t)B + 1), kBlock, 0, c->stream>>>(n, B, bsize, ioff, c->tile_bounds.as<uint32_t>());
+  // the blocks' row bounds, cut into pieces of at most kTileEdges edges
+  L.bounds.ensure(((size_t)B + 1) * 4);
+  k_tile_bounds<<<grid_for((uint64_t)B + 1), kBlock, 0, c->stream>>>(n, B, bsize, rows_off, L.bounds.as<uint32_t>());
   std::vector<uint32_t> hb(B + 1);
-  SUMINC_HIP(hipMemcpyAsync(hb.data(), c->tile_bounds.p, hb.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  SUMINC_HIP(hipMemcpyAsync(hb.data(), L.bounds.p, hb.size() * 4, hipMemcpyDeviceToHost, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
-  // the CSC's edge count (during a batch the context's m is the old graph's)
+  // the edge count from the device (during a batch the context's m is the old graph's)
   const uint32_t m_now = hb[B];
   if (m_now == 0) return;
   std::vector<uint4> pieces;
@@ -1517,17 +1527,17 @@ static void build_bins(suminc_ctx* c, uint64_t m_cap) {
   }
   const uint32_t J = (uint32_t)pieces.size();
   const uint64_t nseg2 = (uint64_t)P * J, nseg = (uint64_t)P * B;
-  c->tile_tab.ensure(((size_t)J + B) * sizeof(uint4));
-  uint4* d_pieces = c->tile_tab.as<uint4>();
+  L.pieces.ensure(((size_t)J + B) * sizeof(uint4));
+  uint4* d_pieces = L.pieces.as<uint4>();
   uint32_t* d_first = reinterpret_cast<uint32_t*>(d_pieces + J);
   SUMINC_HIP(hipMemcpyAsync(d_pieces, pieces.data(), (size_t)J * sizeof(uint4), hipMemcpyHostToDevice, c->stream));
   SUMINC_HIP(hipMemcpyAsync(d_first, first.data(), (size_t)B * 4, hipMemcpyHostToDevice, c->stream));
-  c->bin_hist.ensure(nseg2 * 4);
-  c->tile_seg.ensure_slack((nseg2 + 1) * 4);  // F2: the pieces' segment offsets
-  c->bin_F.ensure((nseg + 1) * 4);
-  uint32_t* hist = c->bin_hist.as<uint32_t>();
-  uint32_t* F2 = c->tile_seg.as<uint32_t>();
-  uint32_t* F = c->bin_F.as<uint32_t>();
+  L.hist.ensure(nseg2 * 4);
+  L.F2.ensure_slack((nseg2 + 1) * 4);  // the pieces' segment offsets
+  L.F.ensure((nseg + 1) * 4);
+  uint32_t* hist = L.hist.as<uint32_t>();
+  uint32_t* F2 = L.F2.as<uint32_t>();
+  uint32_t* F = L.F.as<uint32_t>();
   k_bin_hist<<<J, kBlock, 0, c->stream>>>(d_pieces, J, cols, wl, P, hist);
   SUMINC_HIP(hipGetLastError());
   SUMINC_HIP(hipMemsetAsync(F2, 0, 4, c->stream));
@@ -1537,29 +1547,42 @@ static void build_bins(suminc_ctx* c, uint64_t m_cap) {
   c->tmp.ensure(bytes);
   SUMINC_HIP(rocprim::inclusive_scan(c->tmp.p, bytes, hist, F2 + 1, (size_t)nseg2, rocprim::plus<uint32_t>(),
                                      c->stream));
-  c->tile_edges.ensure_slack(m_cap * 4 + 16);  // the summing pass reads 4-aligned groups
-  k_bin_scatter<<<J, kBlock, 0, c->stream>>>(d_pieces, J, ioff, cols, wl, P, F2, c->tile_edges.as<uint32_t>());
+  L.packed.ensure_slack(m_cap * 4 + 16);  // the summing pass reads 4-aligned groups
+  if (with_eid) L.eid.ensure_slack(m_cap * 4 + 16);
+  k_bin_scatter<<<J, kBlock, 0, c->stream>>>(d_pieces, J, rows_off, cols, wl, P, F2, L.packed.as<uint32_t>(),
+                                             with_eid ? L.eid.as<uint32_t>() : nullptr);
   SUMINC_HIP(hipGetLastError());
   // each (window, piece) segment sorted by slot: neighbouring lanes read
   // neighbouring contributions (one line serves the edges of the hot sources), and
   // the edges of one destination (adjacent in arrival order) are spread apart, so
   // a wave's LDS additions seldom meet on one address
-  c->tile_edges2.ensure_slack(m_cap * 4 + 16);
+  L.packed2.ensure_slack(m_cap * 4 + 16);
   bytes = 0;
-  SUMINC_HIP(rocprim::segmented_radix_sort_keys(nullptr, bytes, c->tile_edges.as<uint32_t>(),
-                                                c->tile_edges2.as<uint32_t>(), m_now, (unsigned)nseg2, F2, F2 + 1,
-                                                0u, wl, c->stream));
-  c->tmp.ensure(bytes);
-  SUMINC_HIP(rocprim::segmented_radix_sort_keys(c->tmp.p, bytes, c->tile_edges.as<uint32_t>(),
-                                                c->tile_edges2.as<uint32_t>(), m_now, (unsigned)nseg2, F2, F2 + 1,
-                                                0u, wl, c->stream));
-  c->tile_edges.swap(c->tile_edges2);
+  if (with_eid) {
+    L.eid2.ensure_slack(m_cap * 4 + 16);
+    SUMINC_HIP(rocprim::segmented_radix_sort_pairs(nullptr, bytes, L.packed.as<uint32_t>(), L.packed2.as<uint32_t>(),
+                                                   L.eid.as<uint32_t>(), L.eid2.as<uint32_t>(), m_now,
+                                                   (unsigned)nseg2, F2, F2 + 1, 0u, wl, c->stream));
+    c->tmp.ensure(bytes);
+    SUMINC_HIP(rocprim::segmented_radix_sort_pairs(c->tmp.p, bytes, L.packed.as<uint32_t>(), L.packed2.as<uint32_t>(),
+                                                   L.eid.as<uint32_t>(), L.eid2.as<uint32_t>(), m_now,
+                                                   (unsigned)nseg2, F2, F2 + 1, 0u, wl, c->stream));
+    L.eid.swap(L.eid2);
+    L.coef.ensure_slack((m_cap + 4) * 8);
+  } else {
+    SUMINC_HIP(rocprim::segmented_radix_sort_keys(nullptr, bytes, L.packed.as<uint32_t>(), L.packed2.as<uint32_t>(),
+                                                  m_now, (unsigned)nseg2, F2, F2 + 1, 0u, wl, c->stream));
+    c->tmp.ensure(bytes);
+    SUMINC_HIP(rocprim::segmented_radix_sort_keys(c->tmp.p, bytes, L.packed.as<uint32_t>(), L.packed2.as<uint32_t>(),
+                                                  m_now, (unsigned)nseg2, F2, F2 + 1, 0u, wl, c->stream));
+  }
+  L.packed.swap(L.packed2);
   k_bin_fold<<<grid_for(nseg + 1), kBlock, 0, c->stream>>>(F2, J, d_first, B, P, F);
   SUMINC_HIP(hipGetLastError());
   std::vector<uint32_t> hF(nseg + 1);
   SUMINC_HIP(hipMemcpyAsync(hF.data(), F, hF.size() * 4, hipMemcpyDeviceToHost, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
-  SUMINC_REQUIRE(hF[nseg] == m_now, SUMINC_E_STATE, "binned pull layout does not cover the CSC");
+  SUMINC_REQUIRE(hF[nseg] == m_now, SUMINC_E_STATE, "binned pull layout does not cover the adjacency");
   // the summing pass's parts: each block's run of 4-edge groups cut into parts of
   // at most `ep` edges, enough parts to fill the chip
   const uint64_t ep = std::min<uint64_t>(1u << 16, std::max<uint64_t>(1u << 12, m_now / 8192));
@@ -1580,20 +1603,59 @@ static void build_bins(suminc_ctx* c, uint64_t m_cap) {
     for (uint64_t z = 0; z < total; z += per)
       parts.push_back(make_uint4(b | flag, 0u, (uint32_t)z, (uint32_t)std::min(total, z + per)));
   }
-  c->bin_parts.ensure(parts.size() * sizeof(uint4));
-  SUMINC_HIP(hipMemcpyAsync(c->bin_parts.p, parts.data(), parts.size() * sizeof(uint4), hipMemcpyHostToDevice,
+  L.parts.ensure(parts.size() * sizeof(uint4));
+  SUMINC_HIP(hipMemcpyAsync(L.parts.p, parts.data(), parts.size() * sizeof(uint4), hipMemcpyHostToDevice,
                             c->stream));
-  const uint32_t direct = c->prm.pull_bin_direct < 0 ? 1u : std::min<uint32_t>(P, (uint32_t)c->prm.pull_bin_direct);
-  c->bin_vals.ensure_slack((m_cap + 4) * es);
+  const uint32_t direct = direct_param < 0 ? 1u : std::min<uint32_t>(P, (uint32_t)direct_param);
+  L.vals.ensure_slack((m_cap + 4) * es);
   SUMINC_HIP(hipStreamSynchronize(c->stream));  // `parts`, `pieces` and `first` go out of scope
-  c->bin_n = (uint32_t)parts.size();
-  c->bin_P = P;
-  c->bin_B = B;
-  c->bin_wl = wl;
-  c->bin_bsize = bsize;
-  c->bin_direct = direct;
-  c->bin_first = hF[(size_t)direct * B];
-  c->bin_m = m_now;
+  L.n_parts = (uint32_t)parts.size();
+  L.P = P;
+  L.B = B;
+  L.wl = wl;
+  L.bsize = bsize;
+  L.direct = direct;
+  L.first = hF[(size_t)direct * B];
+  L.m = m_now;
+}
+
+// y[v] += sum over the edges (u, v) of L of x[u] (times the edge's coefficient,
+// L.coef, when COEF): the gather pass, then the summing pass.  gate as for the
+// pull launches (2: unless cnt is done).
+template <class T, bool COEF>
+void bin_layout_spmv(suminc_ctx* c, const BinLayout& L, const T* x, T* y, const Counters* cnt, int gate) {
+  SUMINC_REQUIRE(L.bsize <= kTileDst, SUMINC_E_STATE, "binned pull blocks exceed the LDS sums");
+  SUMINC_REQUIRE(L.vals.cap >= (L.m + 4) * sizeof(T) && (!COEF || L.coef.cap >= (L.m + 4) * sizeof(T)),
+                 SUMINC_E_STATE, "binned pull layout built for a smaller element");
+  const uint32_t* F = L.F.as<uint32_t>();
+  const T* coef = COEF ? L.coef.as<T>() : nullptr;
+  const uint64_t first = L.first & ~3ull;  // 4-aligned, for the vector path
+  if (L.m > L.first)
+    k_pr_bin_gather<T, COEF><<<(unsigned)((L.m - first + kBinGatherEdges - 1) / kBinGatherEdges), kBlock, 0,
+                               c->stream>>>(L.packed.as<uint32_t>(), F, L.B, L.P, (uint32_t)first, (uint32_t)L.m,
+                                            L.wl, x, coef, L.vals.as<T>(), cnt, gate);
+  k_pr_bin_sum<T, COEF><<<L.n_parts, kBinBlock, 0, c->stream>>>(L.parts.as<uint4>(), F, L.B, L.P, L.bsize, c->n,
+                                                               L.packed.as<uint32_t>(), L.vals.as<T>(), x, coef,
+                                                               L.wl, L.direct, y, cnt, gate);
+}
+template void bin_layout_spmv<float, true>(suminc_ctx*, const BinLayout&, const float*, float*, const Counters*, int);
+template void bin_layout_spmv<double, true>(suminc_ctx*, const BinLayout&, const double*, double*, const Counters*,
+                                            int);
+
+// The binned layout of the current graph (after the positions).
+static void build_bins(suminc_ctx* c, uint64_t m_cap) {
+  c->bins.n_parts = 0;
+  const int kib = c->prm.pull_bin_kib;
+  const uint64_t es = c->pr_prec == SUMINC_PR_FP64 ? 8 : 4;
+  const uint32_t n = c->n;
+  if (kib < 0 || c->band_P > 1 || n == 0) return;
+  // measured on MI355X (FP32, 2 MiB windows, the row classes beside it): 0.25
+  // against 0.38 ms per dense round at 2^22 vertices, 0.86 against 1.70 at 2^24,
+  // 4.65 against 8.6 at 2^26
+  if (kib == 0 && (uint64_t)n * es <= (8ull << 20)) return;
+  const unsigned long long n_src = c->pull_compact ? nonempty_rows(c, c->out_off.as<uint32_t>(), c->bins.hist) : n;
+  bin_layout_build(c, c->bins, n, c->in_off.as<uint32_t>(), pull_cols(c), n_src, es,
+                   (uint64_t)(kib ? kib : 2048) * 1024, c->prm.pull_bin_direct, m_cap, false);
 }
 
 void build_pull_lists(suminc_ctx* c, uint64_t m_cap) {
@@ -1613,7 +1675,7 @@ void build_pull_lists(suminc_ctx* c, uint64_t m_cap) {
   build_band_tables(c, m_cap);
   build_bins(c, m_cap);
   c->tile_n = 0;
-  if (!c->bin_n) build_tiles(c, m_cap);
+  if (!c->bins.n_parts) build_tiles(c, m_cap);
   c->pull_prec = c->pr_prec;  // positions, bands and tiles depend on the element size
 }
 
@@ -1631,20 +1693,8 @@ void pull_scatter(suminc_ctx* c, int gate) {
   Counters* cnt = dev_counters(c);
   const unsigned long long* counts = c->pull_cnt.as<unsigned long long>();
   const uint32_t* icol = pull_cols(c);
-  if (c->bin_n) {
-    SUMINC_REQUIRE(c->bin_bsize <= kTileDst, SUMINC_E_STATE, "binned pull blocks exceed the LDS sums");
-    const uint32_t* F = c->bin_F.as<uint32_t>();
-    const uint64_t first = c->bin_first & ~3ull;  // 4-aligned, for the vector path
-    if (c->bin_m > c->bin_first)
-      k_pr_bin_gather<T><<<(unsigned)((c->bin_m - first + kBinGatherEdges - 1) / kBinGatherEdges), kBlock, 0,
-                           c->stream>>>(c->tile_edges.as<uint32_t>(), F, c->bin_B, c->bin_P, (uint32_t)first,
-                                        (uint32_t)c->bin_m, c->bin_wl,
-                                           c->pr_contrib.as<T>(), c->bin_vals.as<T>(), cnt, gate);
-    k_pr_bin_sum<T><<<c->bin_n, kBinBlock, 0, c->stream>>>(c->bin_parts.as<uint4>(), F, c->bin_B, c->bin_P,
-                                                       c->bin_bsize, c->n,
-                                                       c->tile_edges.as<uint32_t>(), c->bin_vals.as<T>(),
-                                                       c->pr_contrib.as<T>(), c->bin_wl, c->bin_direct,
-                                                       c->pr_delta.as<T>(), cnt, gate);
+  if (c->bins.n_parts) {
+    bin_layout_spmv<T, false>(c, c->bins, c->pr_contrib.as<T>(), c->pr_delta.as<T>(), cnt, gate);
     return;
   }
   if (c->tile_n) {

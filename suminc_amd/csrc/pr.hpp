@@ -31,6 +31,18 @@ bool compact_pull(const suminc_ctx* c, uint64_t es);
 // the plain pull's contribution index (nullptr: vertex ids) and CSC columns
 const uint32_t* pull_pos(suminc_ctx* c);
 const uint32_t* pull_cols(suminc_ctx* c);
+// rows with at least one entry in off (n + 1 row pointers); scratch: 8 bytes
+unsigned long long nonempty_rows(suminc_ctx* c, const uint32_t* off, DevBuf& scratch);
+// a binned pull layout of the adjacency (rows_off, cols) with columns < n_src, source
+// windows of win_bytes / es slots (L.n_parts = 0 when it does not apply); with_eid
+// keeps the row-order index of every edge in L.eid (for coefficients in L.coef)
+void bin_layout_build(suminc_ctx* c, BinLayout& L, uint32_t n, const uint32_t* rows_off, const uint32_t* cols,
+                      unsigned long long n_src, uint64_t es, uint64_t win_bytes, int direct_param, uint64_t m_cap,
+                      bool with_eid);
+// y[v] += sum over the edges (u, v) of L of x[u] (times L.coef when COEF); gate as
+// the pull launches' (2: a series round, skipped once cnt is done)
+template <class T, bool COEF>
+void bin_layout_spmv(suminc_ctx* c, const BinLayout& L, const T* x, T* y, const Counters* cnt, int gate);
 // in-degree classes of the CSC for the pull rounds: built on the stream (lengths
 // stay on the device) for the current CSC, sized for up to m_cap edges
 void build_pull_lists(suminc_ctx* c, uint64_t m_cap);

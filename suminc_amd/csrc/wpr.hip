@@ -413,6 +413,11 @@ struct WprState {
   DevBuf r64, lam64, yl64;           // FP64 results of the last forward / backward
   DevBuf l1;                         // sum |seed|
   DevBuf php_a_out, php_a_in, php64;  // PHP: coefficients (T) in CSR / CSC order, FP64 result
+  // binned series rounds (large graphs, the PageRank pull's rule): the forward over
+  // the CSC, the adjoint over the CSR, their coefficients in layout order; built for
+  // elements of bin_es bytes
+  BinLayout bin_in, bin_out;
+  uint64_t bin_es = 0;
 };
 
 static WprState& wstate(suminc_ctx* c) {
@@ -426,7 +431,8 @@ void wpr_release(suminc_ctx* c) {
 }
 
 static void wpr_prepare_graph(suminc_ctx* c, WprState& S) {
-  if (S.graph_version == c->graph_version) return;
+  const uint64_t es = c->prm.pr_precision == SUMINC_PR_FP64 ? 8 : 4;
+  if (S.graph_version == c->graph_version && S.bin_es == es) return;
   const uint32_t n = c->n;
   const uint64_t m = c->m;
   build_lists(c, S.out, c->out_off);
@@ -460,6 +466,24 @@ static void wpr_prepare_graph(suminc_ctx* c, WprState& S) {
                                          S.eid.as<uint32_t>(), (size_t)m, 0, end_bit, c->stream));
     SUMINC_HIP(hipStreamSynchronize(c->stream));
   }
+  // binned series rounds, by the PageRank pull's rule (on above 8 MiB of elements
+  // or when a window is given): the gather pass multiplies by the coefficient
+  S.bin_in.n_parts = S.bin_out.n_parts = 0;
+  const int kib = c->prm.pull_bin_kib;
+  if (m && kib >= 0 && (kib > 0 || (uint64_t)n * es > (8ull << 20))) {
+    const uint64_t win = (uint64_t)(kib ? kib : 2048) * 1024;
+    const unsigned long long src_in =
+        S.in.compact ? nonempty_rows(c, c->out_off.as<uint32_t>(), S.bin_in.hist) : n;
+    bin_layout_build(c, S.bin_in, n, c->in_off.as<uint32_t>(),
+                     S.in.compact ? S.in.colc.as<uint32_t>() : c->in_col.as<uint32_t>(), src_in, es, win,
+                     c->prm.pull_bin_direct, m, true);
+    const unsigned long long src_out =
+        S.out.compact ? nonempty_rows(c, c->in_off.as<uint32_t>(), S.bin_out.hist) : n;
+    bin_layout_build(c, S.bin_out, n, c->out_off.as<uint32_t>(),
+                     S.out.compact ? S.out.colc.as<uint32_t>() : c->out_col.as<uint32_t>(), src_out, es, win,
+                     c->prm.pull_bin_direct, m, true);
+  }
+  S.bin_es = es;
   S.graph_version = c->graph_version;
   S.coef_version = ~0ull;
   S.fwd_ready = false;
@@ -571,10 +595,15 @@ static SeriesBufs<T> series_state(suminc_ctx* c, WprState& S, bool compact) {
 // null).  pos != nullptr: R's columns are compact positions, and the rounds
 // gather the compact copy of x (B.xc).  Allocates nothing itself.
 template <class T>
-static void wpr_series(suminc_ctx* c, const SeriesBufs<T>& B, const RowLists& R, const uint32_t* pos, const T* a,
-                       const float* seed_dev, T scale, double tol_abs, bool relative, float* out32, double* out64,
-                       suminc_stats* st) {
+static void wpr_series(suminc_ctx* c, const SeriesBufs<T>& B, const RowLists& R, const BinLayout* L,
+                       const uint32_t* pos, const T* a, const float* seed_dev, T scale, double tol_abs, bool relative,
+                       float* out32, double* out64, suminc_stats* st) {
   const uint32_t n = c->n;
+  // the binned layout of R's direction when it exists: its coefficients taken from a
+  // in layout order, the rounds through the gather and summing passes
+  const bool binned = L && L->n_parts;
+  if (binned && L->m)
+    k_wpr_gather<T><<<grid_for(L->m), kBlock, 0, c->stream>>>(L->m, L->eid.as<uint32_t>(), a, L->coef.as<T>());
   T* xc = pos ? B.xc : nullptr;
   const unsigned g = grid_for(n);
   const unsigned gs = grid_for(n, kBlock * kStepPer);  // the round's vertex pass and its partials
@@ -592,7 +621,8 @@ static void wpr_series(suminc_ctx* c, const SeriesBufs<T>& B, const RowLists& R,
   int group = 8;
   while (true) {
     for (int i = 0; i < group; ++i) {
-      spmv<T>(c, R, a, pos ? xc : B.x, B.y, cnt);
+      if (binned) bin_layout_spmv<T, true>(c, *L, pos ? xc : B.x, B.y, cnt, 2);
+      else spmv<T>(c, R, a, pos ? xc : B.x, B.y, cnt);
       k_wpr_step<T><<<gs, kBlock, 0, c->stream>>>(n, B.x, B.y, B.sum, B.comp, B.partial, cnt, pos, xc);
       k_wpr_decide<<<1, 1024, 0, c->stream>>>(B.partial, gs, cnt, tol, max_rounds, c->m);
     }
@@ -654,7 +684,7 @@ static void wpr_forward_t(suminc_ctx* c, const float* p, float* r, suminc_stats*
   r32.ensure((size_t)n * 4 + 4);
   const RowLists R = lists_of(S.in, c->in_off, c->in_col, true);
   const uint32_t* pos = S.in.compact ? S.in.pos.as<uint32_t>() : nullptr;
-  wpr_series<T>(c, series_state<T>(c, S, pos != nullptr), R, pos, S.a_in.as<T>(), pd.as<float>(),
+  wpr_series<T>(c, series_state<T>(c, S, pos != nullptr), R, &S.bin_in, pos, S.a_in.as<T>(), pd.as<float>(),
                 (T)(1.0 - c->prm.damping), c->prm.termcheck_threshold,
                 false, r32.as<float>(), S.r64.as<double>(), st);
   if (r) SUMINC_HIP(hipMemcpyAsync(r, r32.p, (size_t)n * 4, hipMemcpyDefault, c->stream));
@@ -678,8 +708,8 @@ static void wpr_backward_t(suminc_ctx* c, const float* cvec, float* grad_p, floa
   S.yl64.ensure((size_t)n * 8 + 8);
   const RowLists R = lists_of(S.out, c->out_off, c->out_col, true);
   const uint32_t* pos = S.out.compact ? S.out.pos.as<uint32_t>() : nullptr;
-  wpr_series<T>(c, series_state<T>(c, S, pos != nullptr), R, pos, S.a_out.as<T>(), cd.as<float>(), (T)1,
-                c->prm.termcheck_threshold, true, nullptr,
+  wpr_series<T>(c, series_state<T>(c, S, pos != nullptr), R, &S.bin_out, pos, S.a_out.as<T>(), cd.as<float>(),
+                (T)1, c->prm.termcheck_threshold, true, nullptr,
                 S.lam64.as<double>(), st);
   // y_u = sum_k a_uk lambda_k (one more product, in FP64)
   DevBuf lam_t, y_t;
@@ -775,8 +805,8 @@ static void php_run_t(suminc_ctx* c, uint32_t src, double d, double* values, sum
   SUMINC_HIP(hipGetLastError());
   const RowLists R = lists_of(S.in, c->in_off, c->in_col, true);
   const uint32_t* pos = S.in.compact ? S.in.pos.as<uint32_t>() : nullptr;
-  wpr_series<T>(c, series_state<T>(c, S, pos != nullptr), R, pos, S.php_a_in.as<T>(), seed.as<float>(), (T)1,
-                c->prm.termcheck_threshold, false, nullptr, S.php64.as<double>(), st);
+  wpr_series<T>(c, series_state<T>(c, S, pos != nullptr), R, &S.bin_in, pos, S.php_a_in.as<T>(),
+                seed.as<float>(), (T)1, c->prm.termcheck_threshold, false, nullptr, S.php64.as<double>(), st);
   if (values) SUMINC_HIP(hipMemcpyAsync(values, S.php64.p, (size_t)n * 8, hipMemcpyDefault, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
 }
@@ -846,7 +876,8 @@ static void wpr_forward_noalloc_t(suminc_ctx* c, const float* p, float* r, sumin
   const RowLists R = lists_of(S.in, c->in_off, c->in_col, true);
   const uint32_t* pos = compact ? S.in.pos.as<uint32_t>() : nullptr;
   S.fwd_ready = false;  // the rank of this forward is not kept for a backward
-  wpr_series<T>(c, B, R, pos, S.a_in.as<T>(), p, (T)(1.0 - c->prm.damping), c->prm.termcheck_threshold, false, r,
+  wpr_series<T>(c, B, R, &S.bin_in, pos, S.a_in.as<T>(), p, (T)(1.0 - c->prm.damping), c->prm.termcheck_threshold,
+                false, r,
                 nullptr, st);
   SUMINC_HIP(hipStreamSynchronize(c->stream));
 }

@@ -120,16 +120,18 @@ def test_bfs_ignores_weights_and_sketch_refuses_other_apps(gpu):
         assert e.value.code == gpu.E_INVALID
 
 
+# binned: the series rounds through the binned layout (1 KiB source windows)
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("weights", [None, "u32", "f32"])
-def test_php_equals_closed_form(gpu, prec, weights):
+@pytest.mark.parametrize("weights,binned", [(None, False), ("u32", False), ("f32", False), ("f32", True)])
+def test_php_equals_closed_form(gpu, prec, weights, binned):
     g = oracle.Graph.rmat(13, 16, seed=7, weights=weights, wmin=1, wmax=64)
     off, col, w = g.csr()
     src = 3
     exp = apps.php(off, col, w, src, D_PHP)
     T = 1e-12 if prec == "fp64" else 1e-7
     p = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
-    with gpu.Context(pr_precision=p, termcheck_threshold=T) as ctx:
+    kw = {"pull_bin_kib": 1} if binned else {}
+    with gpu.Context(pr_precision=p, termcheck_threshold=T, **kw) as ctx:
         ctx.upload_csr(off, col, w)
         val, st = ctx.php_run(src)
     assert val[src] == pytest.approx(1.0, abs=1e-6)

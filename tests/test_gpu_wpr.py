@@ -22,15 +22,21 @@ def problem(scale, seed=4, seeds=64):
 
 
 # compact=1: both series gather at compact positions by degree (the default only
-# above 32 MiB of FP32 state, 8.4M vertices)
-@pytest.mark.parametrize("prec,compact", [("fp64", 0), ("fp32", 0), ("fp64", 1), ("fp32", 1)])
+# above 32 MiB of FP32 state, 8.4M vertices); compact=2 / 3: the binned series
+# rounds (1 KiB source windows, the coefficients in layout order) over compact
+# positions / vertex ids (the default above 8 MiB of state)
+@pytest.mark.parametrize("prec,compact", [("fp64", 0), ("fp32", 0), ("fp64", 1), ("fp32", 1), ("fp64", 2),
+                                          ("fp32", 2), ("fp64", 3), ("fp32", 3)])
 def test_forward_and_gradients_equal_oracle(gpu, prec, compact):
     g, off, col, w, p, c, _ = problem(14)
     r0, _ = g.wpr_forward(w, p.astype(np.float64), DAMPING)
     lam0, gp0, gw0 = g.wpr_adjoint(w, r0, c.astype(np.float64), DAMPING)
     pp = gpu.PR_FP64 if prec == "fp64" else gpu.PR_FP32_KAHAN
     T = 1e-15 if prec == "fp64" else 1e-9
-    with gpu.Context(pr_precision=pp, termcheck_threshold=T, pull_compact_kib=1 if compact else 0) as ctx:
+    kw = {"pull_compact_kib": 1 if compact in (1, 2) else 0}
+    if compact >= 2:
+        kw["pull_bin_kib"] = 1
+    with gpu.Context(pr_precision=pp, termcheck_threshold=T, **kw) as ctx:
         ctx.upload_csr(off, col, w)
         r, st = ctx.wpr_forward(p)
         r64, _ = ctx.wpr_download()
