@@ -406,8 +406,9 @@ __global__ void __launch_bounds__(kBlock) k_copy_old(uint32_t m, const uint32_t*
   }
 }
 
+// ncolc (carried compact positions): the renamed column, pos[column], as well
 __global__ void k_place_adds(const uint32_t* off, const uint32_t* colv, const uint32_t* noff, uint32_t* ncol,
-                             uint32_t* nw, Groups gs) {
+                             uint32_t* nw, Groups gs, uint32_t* ncolc, const uint32_t* pos) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= *gs.countp || gs.act[g] != 0) return;
   uint32_t u = gs.row[g], v = gs.col[g];
@@ -419,6 +420,7 @@ __global__ void k_place_adds(const uint32_t* off, const uint32_t* colv, const ui
   uint32_t dst = noff[u] + kept_before + adds_before;
   ncol[dst] = v;
   if (nw) nw[dst] = gs.w[g];
+  if (ncolc) ncolc[dst] = pos[v];
 }
 
 __global__ void k_transpose_group_keys(const uint32_t* row, const uint32_t* col, const unsigned* bc, uint32_t N,
@@ -870,13 +872,17 @@ static void cub_call(suminc_ctx* c, DevBuf& tmp, F&& f) {
 
 // Rebuilds one adjacency direction into the workspace buffers, then swaps them
 // with the live arrays (the old arrays become next batch's workspace).
+// colc / ncolc (the CSC with carried compact positions): the renamed columns are
+// merged the same way, an added edge's from pos.
 static void merge_direction(suminc_ctx* c, DevBuf& off, DevBuf& col, DevBuf& w, DevBuf& noff, DevBuf& ncol,
-                            DevBuf& nw, const Groups& gs, uint32_t N, uint64_t cap_m, size_t prof_slot) {
+                            DevBuf& nw, const Groups& gs, uint32_t N, uint64_t cap_m, size_t prof_slot,
+                            DevBuf* colc = nullptr, DevBuf* ncolc = nullptr, const uint32_t* pos = nullptr) {
   const uint32_t n = c->n;
   BatchWs& b = c->bw;
   noff.ensure(((size_t)n + 1) * 4);
   ncol.ensure_slack(cap_m * 4 + 4);
   if (c->wkind) nw.ensure_slack(cap_m * 4 + 4);
+  if (colc) ncolc->ensure_slack(cap_m * 4 + 4);
   const uint32_t vtiles = (n + 1 + kOffTile - 1) / kOffTile;
   b.vts.ensure(((size_t)vtiles + 1) * 4);
   k_off_tiles<<<grid_for((uint64_t)N + 1), kBlock, 0, c->stream>>>(gs.t_list, gs.tcount, vtiles, b.vts.as<uint32_t>());
@@ -892,14 +898,18 @@ static void merge_direction(suminc_ctx* c, DevBuf& off, DevBuf& col, DevBuf& w, 
                                                  ncol.as<uint32_t>(), c->wkind ? nw.as<uint32_t>() : nullptr,
                                                  b.tstart.as<uint32_t>(), gs);
     if (c->prof_on) SUMINC_HIP(hipEventRecord(prof_event(c, prof_slot + 1), c->stream));
+    if (colc)
+      k_copy_old<<<ntiles, kBlock, 0, c->stream>>>(m, colc->as<uint32_t>(), nullptr, ncolc->as<uint32_t>(), nullptr,
+                                                   b.tstart.as<uint32_t>(), gs);
   }
   k_place_adds<<<grid_for(N), kBlock, 0, c->stream>>>(off.as<uint32_t>(), col.as<uint32_t>(), noff.as<uint32_t>(),
                                                       ncol.as<uint32_t>(), c->wkind ? nw.as<uint32_t>() : nullptr,
-                                                      gs);
+                                                      gs, colc ? ncolc->as<uint32_t>() : nullptr, pos);
   SUMINC_HIP(hipGetLastError());
   off.swap(noff);
   col.swap(ncol);
   if (c->wkind) w.swap(nw);
+  if (colc) colc->swap(*ncolc);
 }
 
 // AmendValue over the touched sources' out-edges of the current graph: push for a
@@ -1162,8 +1172,14 @@ void apply_batch(suminc_ctx* c, const suminc_update* ops, uint64_t n_ops, bool o
   gs.t_list = b.t_list.as<uint32_t>(); gs.tcount = bc + 1;
   gs.pos = b.g_pos.as<uint32_t>();
   const uint64_t m_old = c->m;
+  // the compact positions carried over (most vertices keep theirs; the vertices
+  // that gained out-edges get new ones after the CSR merge) instead of renumbered
+  const bool carry = pr_pull && pr_positions_carry(c);
   merge_direction(c, c->out_off, c->out_col, c->out_w, b.noff_out, b.ncol_out, b.nw_out, gs, N, cap_m, 0);
-  merge_direction(c, c->in_off, c->in_col, c->in_w, b.noff_in, b.ncol_in, b.nw_in, gt, N, cap_m, 2);
+  if (carry) pr_positions_append(c);
+  merge_direction(c, c->in_off, c->in_col, c->in_w, b.noff_in, b.ncol_in, b.nw_in, gt, N, cap_m, 2,
+                  carry ? &c->in_col_c : nullptr, carry ? &b.ncolc_in : nullptr,
+                  carry ? c->src_pos.as<uint32_t>() : nullptr);
 
   // 6. PageRank reseed +1 on the new out-edges; the pull lists of the new CSC are
   // built here (they serve the next query as well)

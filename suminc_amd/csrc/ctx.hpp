@@ -123,6 +123,7 @@ struct BatchWs {
   DevBuf tg_row, tg_col, tg_act, tg_w, tg_ex, tg_exu, tg_addu, tg_expre, tg_addpre, thead, tt_raw, tt_list, tt_gbeg;
   DevBuf tix, ttix;  // row -> touched index, all INF32 between batches
   DevBuf vts, noff_out, ncol_out, nw_out, noff_in, ncol_in, nw_in;
+  DevBuf ncolc_in;  // the renamed CSC columns of the new graph (carried positions)
   DevBuf rbits, rlist;
   DevBuf val;  // validation of device-resident ops: count, first bad index
   DevBuf bc;  // device counts: [0]=G groups, [1]=touched rows (CSR), [2]=touched rows (CSC), [3]=new m
@@ -225,6 +226,13 @@ struct suminc_ctx {
   // of the n_src non-dangling contributions only
   suminc::DevBuf src_pos, src_pre, in_col_c;
   bool pull_compact = false;  // the positions are in use (contributions above 32 MiB)
+  // the compact positions and the renamed CSC columns agree with the current CSC
+  // (pos_live); positions in use are below pos_count; a batch may carry them over
+  // (pos_carried: the merge renamed its columns and appended pos_appended
+  // positions for new sources) instead of renumbering every vertex
+  bool pos_live = false, pos_carried = false;
+  uint32_t pos_count = 0, pos_appended = 0;
+  suminc::DevBuf pos_ctr;
   // tiled pull (per graph version, with the pull lists): tile table (uint4: first
   // destination, destination count | split flag << 31, CSC edge range), the CSC
   // edges of each tile grouped by source window and packed into 32 bits, and each

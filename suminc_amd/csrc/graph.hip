@@ -212,6 +212,7 @@ void generate_rmat(suminc_ctx* c, int scale, int edge_factor, double a, double b
   SUMINC_HIP(hipStreamSynchronize(c->stream));
   build_csc(c);
   c->has_graph = true;
+  c->pos_live = false;  // a new graph: positions are renumbered
   c->graph_version++;
 }
 
@@ -248,6 +249,7 @@ void upload_csr(suminc_ctx* c, uint32_t n, uint64_t m, const uint64_t* off, cons
       SUMINC_REQUIRE(col[e - 1] < col[e], SUMINC_E_INVALID, "adjacency must be sorted and free of duplicates");
   build_csc(c);
   c->has_graph = true;
+  c->pos_live = false;  // a new graph: positions are renumbered
   c->graph_version++;
 }
 

@@ -824,13 +824,33 @@ static void build_src_positions(suminc_ctx* c, uint64_t m_cap, bool by_degree) {
   const uint32_t n = c->n;
   // measured on MI355X: at 2^22 vertices (16 MiB of FP32 contributions) renaming
   // costs more per batch than it saves; from 2^24 on it pays
+  const bool was = c->pull_compact;
   c->pull_compact = compact_pull(c, c->pr_prec == SUMINC_PR_FP64 ? 8 : 4);
-  if (!c->pull_compact) return;
-  if (by_degree) {
-    build_positions_by_degree(c, c->out_off.as<uint32_t>(), c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
-                              m_cap, c->src_pos, c->src_pre, c->in_col_c);
+  if (!c->pull_compact) {
+    c->pos_live = c->pos_carried = false;
     return;
   }
+  if (by_degree) {
+    if (c->pos_carried && was && c->pos_live) {
+      // the batch renamed the new CSC's columns during its merge and appended the
+      // positions of the new sources: only the count comes back
+      unsigned long long cnt = 0;
+      SUMINC_HIP(hipMemcpyAsync(&cnt, c->pos_ctr.p, 8, hipMemcpyDeviceToHost, c->stream));
+      SUMINC_HIP(hipStreamSynchronize(c->stream));
+      c->pos_appended += (uint32_t)(cnt - c->pos_count);
+      c->pos_count = (uint32_t)cnt;
+      c->pos_carried = false;
+      return;
+    }
+    build_positions_by_degree(c, c->out_off.as<uint32_t>(), c->in_off.as<uint32_t>(), c->in_col.as<uint32_t>(),
+                              m_cap, c->src_pos, c->src_pre, c->in_col_c);
+    c->pos_count = (uint32_t)nonempty_rows(c, c->out_off.as<uint32_t>(), c->pos_ctr);  // positions [0, count)
+    c->pos_appended = 0;
+    c->pos_live = true;
+    c->pos_carried = false;
+    return;
+  }
+  c->pos_live = c->pos_carried = false;  // id order (the banded pull) is renumbered every time
   c->src_pos.ensure((size_t)n * 4 + 4);
   c->src_pre.ensure((size_t)n * 16 + 16);  // flags, then their exclusive prefix (pre[n] = n_src)
   c->in_col_c.ensure_slack(m_cap * 4 + 4);
@@ -849,6 +869,35 @@ static void build_src_positions(suminc_ctx* c, uint64_t m_cap, bool by_degree) {
                                                                       c->src_pos.as<uint32_t>(),
                                                                       c->in_col_c.as<uint32_t>());
   SUMINC_HIP(hipGetLastError());
+  c->pos_count = (uint32_t)nonempty_rows(c, c->out_off.as<uint32_t>(), c->pos_ctr);
+}
+
+// New positions for the vertices that gained out-edges (count: ctr, starting at
+// the positions in use; one atomic per wave).
+__global__ void k_pos_append(uint32_t n, const uint32_t* __restrict__ off, uint32_t* __restrict__ pos,
+                             unsigned long long* __restrict__ ctr) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool want = v < n && off[v + 1] > off[v] && pos[v] == INF32;
+  const unsigned long long at = wave_append(want, ctr);
+  if (want) pos[v] = (uint32_t)at;
+}
+
+bool pr_positions_carry(const suminc_ctx* c) {
+  // a sixteenth of the positions appended since the last renumbering: renumber by
+  // degree class again
+  return c->pull_compact && c->pos_live && c->pull_prec == c->pr_prec && c->prm.pull_band_kib <= 0 &&
+         (uint64_t)c->pos_appended * 16 < c->pos_count;
+}
+
+__global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+void pr_positions_append(suminc_ctx* c) {
+  c->pos_ctr.ensure(8);
+  k_set_u64<<<1, 1, 0, c->stream>>>(c->pos_ctr.as<unsigned long long>(), c->pos_count);
+  k_pos_append<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), c->src_pos.as<uint32_t>(),
+                                                         c->pos_ctr.as<unsigned long long>());
+  SUMINC_HIP(hipGetLastError());
+  c->pos_carried = true;
 }
 
 // Band tables for the lists just built (sizes read back from the device).  Bands
@@ -1092,21 +1141,14 @@ static void build_tiles(suminc_ctx* c, uint64_t m_cap) {
   const uint64_t win_bytes = (uint64_t)(kib ? kib : 1024) * 1024;
   const uint32_t bsize = tile_dst(c->pr_prec);  // the kernel's LDS sums hold as many destinations
   const uint32_t blocks = (n + bsize - 1) / bsize;
-  // the sources a column can name: the compact positions' count, or every vertex
-  const size_t src_at = (((size_t)blocks + 1) * 4 + 7) / 8 * 8;  // the count after the bounds, 8-aligned
-  c->tile_bounds.ensure(src_at + 8);
-  unsigned long long* d_src = reinterpret_cast<unsigned long long*>(c->tile_bounds.as<uint8_t>() + src_at);
-  SUMINC_HIP(hipMemsetAsync(d_src, 0, 8, c->stream));
-  if (c->pull_compact)
-    k_count_sources<<<grid_for(n, kBlock, 1024), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(), d_src);
+  // the sources a column can name: the compact positions in use, or every vertex
+  c->tile_bounds.ensure(((size_t)blocks + 1) * 4);
   k_tile_bounds<<<grid_for((uint64_t)blocks + 1), kBlock, 0, c->stream>>>(n, blocks, bsize, c->in_off.as<uint32_t>(),
                                                                          c->tile_bounds.as<uint32_t>());
   std::vector<uint32_t> hb(blocks + 1);
-  unsigned long long n_src = n;
   SUMINC_HIP(hipMemcpyAsync(hb.data(), c->tile_bounds.p, hb.size() * 4, hipMemcpyDeviceToHost, c->stream));
-  if (c->pull_compact) SUMINC_HIP(hipMemcpyAsync(&n_src, d_src, 8, hipMemcpyDeviceToHost, c->stream));
   SUMINC_HIP(hipStreamSynchronize(c->stream));
-  n_src = std::max<unsigned long long>(n_src, 1);
+  const unsigned long long n_src = std::max<unsigned long long>(c->pull_compact ? c->pos_count : n, 1);
   // the CSC's edge count from the device (during a batch the context's m is the
   // old graph's until the batch ends)
   const uint32_t m_now = hb[blocks];
@@ -1653,7 +1695,7 @@ static void build_bins(suminc_ctx* c, uint64_t m_cap) {
   // against 0.38 ms per dense round at 2^22 vertices, 0.86 against 1.70 at 2^24,
   // 4.65 against 8.6 at 2^26
   if (kib == 0 && (uint64_t)n * es <= (8ull << 20)) return;
-  const unsigned long long n_src = c->pull_compact ? nonempty_rows(c, c->out_off.as<uint32_t>(), c->bins.hist) : n;
+  const unsigned long long n_src = c->pull_compact ? c->pos_count : n;  // positions in use
   bin_layout_build(c, c->bins, n, c->in_off.as<uint32_t>(), pull_cols(c), n_src, es,
                    (uint64_t)(kib ? kib : 2048) * 1024, c->prm.pull_bin_direct, m_cap, false);
 }

@@ -31,6 +31,12 @@ bool compact_pull(const suminc_ctx* c, uint64_t es);
 // the plain pull's contribution index (nullptr: vertex ids) and CSC columns
 const uint32_t* pull_pos(suminc_ctx* c);
 const uint32_t* pull_cols(suminc_ctx* c);
+// whether the compact positions can be carried through the coming batch (else the
+// pull lists renumber them); the batch then appends the positions of the vertices
+// that gained out-edges (after its CSR merge) and renames its CSC columns while
+// merging them
+bool pr_positions_carry(const suminc_ctx* c);
+void pr_positions_append(suminc_ctx* c);
 // rows with at least one entry in off (n + 1 row pointers); scratch: 8 bytes
 unsigned long long nonempty_rows(suminc_ctx* c, const uint32_t* off, DevBuf& scratch);
 // a binned pull layout of the adjacency (rows_off, cols) with columns < n_src, source
