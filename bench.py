@@ -460,7 +460,7 @@ def live_traffic(args, kernel, roof):
     """roofline.traffic of the FP32 PageRank pull kernels, measured now by two counter
     runs of the traffic child (_pmc_traffic)."""
     sub = {"k_pr_pull": ["k_pr_pull<float>"], "k_pr_pull_tiled": ["k_pr_pull_tiled<float>"],
-           PULL_KERNELS["binned"]: ["k_pr_bin_gather<float>", "k_pr_bin_sum<float>"]}.get(kernel)
+           PULL_KERNELS["binned"]: ["k_pr_bin_gather<float, false>", "k_pr_bin_sum<float, false>"]}.get(kernel)
     if sub is None:
         roof["traffic_note"] = f"no live counter run for {kernel}"
         return
