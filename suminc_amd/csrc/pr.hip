@@ -872,14 +872,21 @@ static void build_src_positions(suminc_ctx* c, uint64_t m_cap, bool by_degree) {
   c->pos_count = (uint32_t)nonempty_rows(c, c->out_off.as<uint32_t>(), c->pos_ctr);
 }
 
-// New positions for the vertices that gained out-edges (count: ctr, starting at
-// the positions in use; one atomic per wave).
-__global__ void k_pos_append(uint32_t n, const uint32_t* __restrict__ off, uint32_t* __restrict__ pos,
-                             unsigned long long* __restrict__ ctr) {
+// New positions for the vertices that gained out-edges: flag them, an exclusive
+// scan of the flags gives each its rank among them, and the count of positions in
+// use (ctr) grows by their number.
+__global__ void k_pos_new_flags(uint32_t n, const uint32_t* __restrict__ off, const uint32_t* __restrict__ pos,
+                                uint32_t* __restrict__ flag) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool want = v < n && off[v + 1] > off[v] && pos[v] == INF32;
-  const unsigned long long at = wave_append(want, ctr);
-  if (want) pos[v] = (uint32_t)at;
+  if (v < n) flag[v] = off[v + 1] > off[v] && pos[v] == INF32 ? 1u : 0u;
+  else if (v == n) flag[n] = 0u;
+}
+
+__global__ void k_pos_new_assign(uint32_t n, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pre,
+                                 uint32_t base, uint32_t* __restrict__ pos, unsigned long long* __restrict__ ctr) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n && flag[v]) pos[v] = base + pre[v];
+  else if (v == n) *ctr = (unsigned long long)base + pre[n];
 }
 
 bool pr_positions_carry(const suminc_ctx* c) {
@@ -889,13 +896,23 @@ bool pr_positions_carry(const suminc_ctx* c) {
          (uint64_t)c->pos_appended * 16 < c->pos_count;
 }
 
-__global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
-
 void pr_positions_append(suminc_ctx* c) {
+  const uint32_t n = c->n;
   c->pos_ctr.ensure(8);
-  k_set_u64<<<1, 1, 0, c->stream>>>(c->pos_ctr.as<unsigned long long>(), c->pos_count);
-  k_pos_append<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->n, c->out_off.as<uint32_t>(), c->src_pos.as<uint32_t>(),
-                                                         c->pos_ctr.as<unsigned long long>());
+  c->src_pre.ensure((size_t)n * 16 + 16);  // flags (n + 1), then their exclusive prefix
+  uint32_t* flag = c->src_pre.as<uint32_t>();
+  uint32_t* pre = flag + n + 1;
+  k_pos_new_flags<<<grid_for((uint64_t)n + 1), kBlock, 0, c->stream>>>(n, c->out_off.as<uint32_t>(),
+                                                                      c->src_pos.as<uint32_t>(), flag);
+  size_t bytes = 0;
+  SUMINC_HIP(rocprim::exclusive_scan(nullptr, bytes, flag, pre, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(),
+                                     c->stream));
+  c->tmp.ensure(bytes);
+  SUMINC_HIP(rocprim::exclusive_scan(c->tmp.p, bytes, flag, pre, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(),
+                                     c->stream));
+  k_pos_new_assign<<<grid_for((uint64_t)n + 1), kBlock, 0, c->stream>>>(n, flag, pre, c->pos_count,
+                                                                       c->src_pos.as<uint32_t>(),
+                                                                       c->pos_ctr.as<unsigned long long>());
   SUMINC_HIP(hipGetLastError());
   c->pos_carried = true;
 }
