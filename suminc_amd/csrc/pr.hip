@@ -890,10 +890,12 @@ __global__ void k_pos_new_assign(uint32_t n, const uint32_t* __restrict__ flag, 
 }
 
 bool pr_positions_carry(const suminc_ctx* c) {
-  // a sixteenth of the positions appended since the last renumbering: renumber by
-  // degree class again
+  // the appended vertices are the coldest sources (they had no out-edge before the
+  // batch), so appending keeps the degree-class order's hot windows; renumber once
+  // the appended positions reach half of those renumbered (a 1% batch of uniform
+  // inserts at 2^26 vertices appends about 2.4M to 37M)
   return c->pull_compact && c->pos_live && c->pull_prec == c->pr_prec && c->prm.pull_band_kib <= 0 &&
-         (uint64_t)c->pos_appended * 16 < c->pos_count;
+         (uint64_t)c->pos_appended * 2 < (uint64_t)c->pos_count - c->pos_appended;
 }
 
 void pr_positions_append(suminc_ctx* c) {
