@@ -1340,8 +1340,10 @@ using vec4_t = T __attribute__((ext_vector_type(4)));
 
 // The gather pass over the streamed edges [first, end) (first a multiple of 4; the
 // edges before the stream's first that it also covers belong to direct windows,
-// whose values nobody reads): kBinGatherEdges per workgroup, eight 4-edge groups a
-// thread, all loads issued before the gathers.  A full workgroup inside one window
+// whose values nobody reads): kBinGatherEdges per workgroup, 32 edges a thread
+// kBlock apart, so that each gather instruction of a wave reads 64 consecutive
+// edges (ascending slots: fewer distinct lines an instruction), all loads issued
+// before the gathers.  A full workgroup inside one window
 // (the window of its first edge by binary search over the window starts F[w * B])
 // takes the vector path; the others (a window start or the end inside) go edge by
 // edge, each thread's window only moving forward.  COEF: each value is multiplied
@@ -1352,7 +1354,7 @@ __global__ void __launch_bounds__(kBlock) k_pr_bin_gather(const uint32_t* __rest
                                                           uint32_t first, uint32_t end, uint32_t wl,
                                                           const T* __restrict__ contrib, const T* __restrict__ coef,
                                                           T* __restrict__ vals, const Counters* cnt, int gate) {
-  constexpr int G = kBinGatherEdges / (4 * kBlock);
+  constexpr int G = kBinGatherEdges / kBlock;
   __shared__ uint32_t s_w;
   if (!pull_runs(cnt, gate)) return;
   const uint32_t i0 = (uint32_t)((uint64_t)first + (uint64_t)blockIdx.x * kBinGatherEdges);  // < end
@@ -1372,28 +1374,24 @@ __global__ void __launch_bounds__(kBlock) k_pr_bin_gather(const uint32_t* __rest
   const uint32_t mask = (1u << wl) - 1u;
   if (i1 - i0 == kBinGatherEdges && i1 <= wend) {
     const T* win = contrib + ((size_t)w0 << wl);
-    const u32x4* p4 = reinterpret_cast<const u32x4*>(packed + i0);
-    vec4_t<T>* v4 = reinterpret_cast<vec4_t<T>*>(vals + i0);
-    u32x4 q[G];
+    const uint32_t* p = packed + i0 + threadIdx.x;
+    T* v = vals + i0 + threadIdx.x;
+    uint32_t q[G];
 #pragma unroll
-    for (int r = 0; r < G; ++r) q[r] = __builtin_nontemporal_load(p4 + threadIdx.x + r * kBlock);
-    vec4_t<T> a[G];
+    for (int r = 0; r < G; ++r) q[r] = __builtin_nontemporal_load(p + r * kBlock);
+    T a[G];
     if constexpr (COEF) {
-      const vec4_t<T>* c4 = reinterpret_cast<const vec4_t<T>*>(coef + i0);
 #pragma unroll
-      for (int r = 0; r < G; ++r) a[r] = __builtin_nontemporal_load(c4 + threadIdx.x + r * kBlock);
+      for (int r = 0; r < G; ++r) a[r] = __builtin_nontemporal_load(coef + i0 + threadIdx.x + r * kBlock);
     }
-    vec4_t<T> x[G];
+    T x[G];
 #pragma unroll
     for (int r = 0; r < G; ++r) {
-      x[r].x = win[q[r].x & mask];
-      x[r].y = win[q[r].y & mask];
-      x[r].z = win[q[r].z & mask];
-      x[r].w = win[q[r].w & mask];
+      x[r] = win[q[r] & mask];
       if constexpr (COEF) x[r] *= a[r];
     }
 #pragma unroll
-    for (int r = 0; r < G; ++r) __builtin_nontemporal_store(x[r], v4 + threadIdx.x + r * kBlock);
+    for (int r = 0; r < G; ++r) __builtin_nontemporal_store(x[r], v + r * kBlock);
   } else {
     uint32_t w = w0;  // a thread's edges ascend, so its window only moves forward
     for (uint32_t k = i0 + threadIdx.x; k < i1; k += kBlock) {
@@ -1411,7 +1409,12 @@ __global__ void __launch_bounds__(kBlock) k_pr_bin_gather(const uint32_t* __rest
 // the run's range [z, zend); the segment bounds and the run position of every
 // window go to LDS first (a workgroup scan), then every thread walks its groups in
 // ascending order, its window pointer only moving forward, four groups (sixteen
-// edges) an iteration with all their loads issued first.
+// edges) an iteration with all their loads issued first.  A wave's 64 consecutive
+// groups that lie in one window segment are taken across the wave instead: lane l
+// takes the span's edges 64 c + l (c = 0..3), so that a gather instruction reads 64
+// consecutive edges of the slot-sorted segment, and the vector L1 looks up fewer
+// distinct lines an instruction (a lane's own group would spread an instruction
+// over 256 edges).
 constexpr uint32_t kBinBlock = 512;  // threads of a summing workgroup (four fit a CU's LDS)
 constexpr uint32_t kBinMaxWindows = kBinBlock;
 template <class T, bool COEF>
@@ -1479,21 +1482,61 @@ __global__ void __launch_bounds__(kBinBlock) k_pr_bin_sum(const uint4* __restric
       hi[r] = valid ? s_end[jr] : 0u;
       dir[r] = jr < direct;
     }
+    // a span across the wave: all 64 lanes active with valid groups of one window
+    // (then k0 = lane 0's k0 + 4 lane, and lo, hi and dir agree over the wave), for
+    // all four groups of the iteration
+    bool span = true;
+#pragma unroll
+    for (int r = 0; r < U; ++r) {
+      const uint32_t j0 = __builtin_amdgcn_readfirstlane(j[r]);
+      span = span && __ballot(g + r * kBinBlock < zend && j[r] == j0) == ~0ull;
+    }
     u32x4 q[U];
     vec4_t<T> x[U];
-#pragma unroll
-    for (int r = 0; r < U; ++r) q[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(packed + k0[r]));
-    // the streamed groups' values; a direct group reads its coefficients (COEF) or
-    // the stream's first group (masked off below)
-#pragma unroll
-    for (int r = 0; r < U; ++r)
-      x[r] = __builtin_nontemporal_load(
-          reinterpret_cast<const vec4_t<T>*>(dir[r] ? (COEF ? coef + k0[r] : vals) : vals + k0[r]));
     bool in[U][4];
+    if (span) {
+      const uint32_t l3 = 3u * (threadIdx.x & 63u);
 #pragma unroll
-    for (int r = 0; r < U; ++r)
+      for (int r = 0; r < U; ++r) {
+        const uint32_t* p = packed + (k0[r] - l3);  // lane 0's k0 + lane
+        q[r].x = __builtin_nontemporal_load(p);
+        q[r].y = __builtin_nontemporal_load(p + 64);
+        q[r].z = __builtin_nontemporal_load(p + 128);
+        q[r].w = __builtin_nontemporal_load(p + 192);
+      }
 #pragma unroll
-      for (int c = 0; c < 4; ++c) in[r][c] = k0[r] + c >= lo[r] && k0[r] + c < hi[r];
+      for (int r = 0; r < U; ++r) {
+        if (!dir[r] || COEF) {
+          const T* p = (dir[r] ? coef : vals) + (k0[r] - l3);
+          x[r].x = __builtin_nontemporal_load(p);
+          x[r].y = __builtin_nontemporal_load(p + 64);
+          x[r].z = __builtin_nontemporal_load(p + 128);
+          x[r].w = __builtin_nontemporal_load(p + 192);
+        } else {
+          x[r] = vec4_t<T>(T(0));
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < U; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const uint32_t k = k0[r] - l3 + 64u * c;
+          in[r][c] = k >= lo[r] && k < hi[r];
+        }
+    } else {
+#pragma unroll
+      for (int r = 0; r < U; ++r) q[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(packed + k0[r]));
+      // the streamed groups' values; a direct group reads its coefficients (COEF) or
+      // the stream's first group (masked off below)
+#pragma unroll
+      for (int r = 0; r < U; ++r)
+        x[r] = __builtin_nontemporal_load(
+            reinterpret_cast<const vec4_t<T>*>(dir[r] ? (COEF ? coef + k0[r] : vals) : vals + k0[r]));
+#pragma unroll
+      for (int r = 0; r < U; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) in[r][c] = k0[r] + c >= lo[r] && k0[r] + c < hi[r];
+    }
     if (dir[0] && dir[1] && dir[2] && dir[3]) {
       // all four groups in direct windows (the common case inside the hottest
       // window's long segments): the sixteen gathers issued together, unconditional
