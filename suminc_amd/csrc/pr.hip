@@ -1271,7 +1271,9 @@ unsigned long long nonempty_rows(suminc_ctx* c, const uint32_t* off, DevBuf& scr
 // The build works on pieces of at most kTileEdges in-edges of one block (the
 // tiles' table: first destination, destination count, CSC edge range), one
 // workgroup each.  Counting: the piece's edges by source window into
-// hist[w * J + j] (J pieces).
+// hist[w * J + j] (J pieces).  Window 0 (the hottest sources, most of the edges)
+// is counted with one LDS atomic a wave; one counter taking most lanes' atomics
+// would serialise them.
 __global__ void __launch_bounds__(kBlock) k_bin_hist(const uint4* __restrict__ pieces, uint32_t J,
                                                      const uint32_t* __restrict__ icol, uint32_t wl, uint32_t P,
                                                      uint32_t* __restrict__ hist) {
@@ -1279,7 +1281,13 @@ __global__ void __launch_bounds__(kBlock) k_bin_hist(const uint4* __restrict__ p
   const uint4 t = pieces[blockIdx.x];
   for (uint32_t w = threadIdx.x; w < P; w += kBlock) s_h[w] = 0;
   __syncthreads();
-  for (uint32_t e = t.z + threadIdx.x; e < t.w; e += kBlock) atomicAdd(&s_h[icol[e] >> wl], 1u);
+  for (uint32_t e0 = t.z; e0 < t.w; e0 += kBlock) {  // the same trip count for every lane (ballots)
+    const uint32_t e = e0 + threadIdx.x;
+    const uint32_t w = e < t.w ? icol[e] >> wl : INF32;
+    const unsigned long long hot = __ballot(w == 0);
+    if (lane_id() == 0 && hot) atomicAdd(&s_h[0], (uint32_t)__popcll(hot));
+    if (w != 0 && w != INF32) atomicAdd(&s_h[w], 1u);
+  }
   __syncthreads();
   for (uint32_t w = threadIdx.x; w < P; w += kBlock) hist[(size_t)w * J + blockIdx.x] = s_h[w];
 }
@@ -1314,7 +1322,18 @@ __global__ void __launch_bounds__(kBlock) k_bin_scatter(const uint4* __restrict_
     }
     block_adjacency<kBlock>(start, len, s_scan, s_start, s_warp, [&](int slot, uint32_t, uint32_t e) {
       const uint32_t p = icol[e];
-      const uint32_t at = atomicAdd(&s_at[p >> wl], 1u);
+      const uint32_t w = p >> wl;
+      // window 0's edges of the wave's active lanes take one LDS atomic together
+      const unsigned long long hot = __ballot(w == 0);
+      uint32_t at;
+      if (w == 0) {
+        const uint32_t below = (uint32_t)__popcll(hot & ((1ull << lane_id()) - 1ull));
+        uint32_t base = 0;
+        if (below == 0) base = atomicAdd(&s_at[0], (uint32_t)__popcll(hot));
+        at = __builtin_amdgcn_readfirstlane(base) + below;  // the first active lane here took the atomic
+      } else {
+        at = atomicAdd(&s_at[w], 1u);
+      }
       packed[at] = ((r0 + (uint32_t)slot - d0) << kTileSrcBits) | (p & mask);
       if (eid) eid[at] = e;  // the edge's index in row order, for per-edge coefficients
     });
