@@ -1359,10 +1359,8 @@ using vec4_t = T __attribute__((ext_vector_type(4)));
 
 // The gather pass over the streamed edges [first, end) (first a multiple of 4; the
 // edges before the stream's first that it also covers belong to direct windows,
-// whose values nobody reads): kBinGatherEdges per workgroup, 32 edges a thread
-// kBlock apart, so that each gather instruction of a wave reads 64 consecutive
-// edges (ascending slots: fewer distinct lines an instruction), all loads issued
-// before the gathers.  A full workgroup inside one window
+// whose values nobody reads): kBinGatherEdges per workgroup, eight 4-edge groups a
+// thread, all loads issued before the gathers.  A full workgroup inside one window
 // (the window of its first edge by binary search over the window starts F[w * B])
 // takes the vector path; the others (a window start or the end inside) go edge by
 // edge, each thread's window only moving forward.  COEF: each value is multiplied
@@ -1373,7 +1371,7 @@ __global__ void __launch_bounds__(kBlock) k_pr_bin_gather(const uint32_t* __rest
                                                           uint32_t first, uint32_t end, uint32_t wl,
                                                           const T* __restrict__ contrib, const T* __restrict__ coef,
                                                           T* __restrict__ vals, const Counters* cnt, int gate) {
-  constexpr int G = kBinGatherEdges / kBlock;
+  constexpr int G = kBinGatherEdges / (4 * kBlock);
   __shared__ uint32_t s_w;
   if (!pull_runs(cnt, gate)) return;
   const uint32_t i0 = (uint32_t)((uint64_t)first + (uint64_t)blockIdx.x * kBinGatherEdges);  // < end
@@ -1393,24 +1391,28 @@ __global__ void __launch_bounds__(kBlock) k_pr_bin_gather(const uint32_t* __rest
   const uint32_t mask = (1u << wl) - 1u;
   if (i1 - i0 == kBinGatherEdges && i1 <= wend) {
     const T* win = contrib + ((size_t)w0 << wl);
-    const uint32_t* p = packed + i0 + threadIdx.x;
-    T* v = vals + i0 + threadIdx.x;
-    uint32_t q[G];
+    const u32x4* p4 = reinterpret_cast<const u32x4*>(packed + i0);
+    vec4_t<T>* v4 = reinterpret_cast<vec4_t<T>*>(vals + i0);
+    u32x4 q[G];
 #pragma unroll
-    for (int r = 0; r < G; ++r) q[r] = __builtin_nontemporal_load(p + r * kBlock);
-    T a[G];
+    for (int r = 0; r < G; ++r) q[r] = __builtin_nontemporal_load(p4 + threadIdx.x + r * kBlock);
+    vec4_t<T> a[G];
     if constexpr (COEF) {
+      const vec4_t<T>* c4 = reinterpret_cast<const vec4_t<T>*>(coef + i0);
 #pragma unroll
-      for (int r = 0; r < G; ++r) a[r] = __builtin_nontemporal_load(coef + i0 + threadIdx.x + r * kBlock);
+      for (int r = 0; r < G; ++r) a[r] = __builtin_nontemporal_load(c4 + threadIdx.x + r * kBlock);
     }
-    T x[G];
+    vec4_t<T> x[G];
 #pragma unroll
     for (int r = 0; r < G; ++r) {
-      x[r] = win[q[r] & mask];
+      x[r].x = win[q[r].x & mask];
+      x[r].y = win[q[r].y & mask];
+      x[r].z = win[q[r].z & mask];
+      x[r].w = win[q[r].w & mask];
       if constexpr (COEF) x[r] *= a[r];
     }
 #pragma unroll
-    for (int r = 0; r < G; ++r) __builtin_nontemporal_store(x[r], v + r * kBlock);
+    for (int r = 0; r < G; ++r) __builtin_nontemporal_store(x[r], v4 + threadIdx.x + r * kBlock);
   } else {
     uint32_t w = w0;  // a thread's edges ascend, so its window only moves forward
     for (uint32_t k = i0 + threadIdx.x; k < i1; k += kBlock) {
