@@ -10,7 +10,7 @@ below run at their own scale, seed, batch fraction and precision:
       (distances and parents).
   C3  s24 seed 3, one 1% batch, FP32 deltas + Kahan ranks + FP64 residual, over
       the plain CSR and over the supernode sketch: L1 <= 1e-6 against the FP64 run
-      on the same graph, and the FP64 mass balance.
+      on the same graph (the metric's tolerance), and the FP64 mass balance.
   C4  s22 seed 4, FP32 weights U(0,1], Dirichlet(1) personalization over 1024
       seeds, loss sum c_i r_i with c ~ N(0,1): the FP32 adjoint gradient against
       central differences (h = 1e-3) of the FP64 forward on 256 random edges

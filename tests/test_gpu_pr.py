@@ -1,6 +1,6 @@
 """Delta-PageRank on the device vs the oracle's closed form / FP64 runs.
 
-Tolerances (north star: L1 <= 1e-6 vs the FP64 oracle):
+Tolerances (the metric: L1 <= 1e-6 vs the FP64 oracle):
   * FP64 path: ||r - r*||_1 <= residual / (1 - d) + 1e-12 (the stop-rule bound).
   * FP32 + Kahan path: ||r - r*||_1 <= residual / (1 - d) + 1e-6, and the
     FP64 mass invariant holds to 1e-6.
@@ -269,7 +269,7 @@ def test_tiled_pull_follows_precision_and_growth(gpu, layout):
 
 @pytest.mark.parametrize("path", ["plain", "sketch", "tiled", "binned", "sketch-binned"])
 def test_incremental_fp32_meets_l1_target(gpu, path):
-    """The north-star bar after every incremental FP32/Kahan batch: L1 <= 1e-6 against
+    """The accuracy bar after every incremental FP32/Kahan batch: L1 <= 1e-6 against
     the FP64 fixed point of the updated graph, at the benchmark's stop rule T = 1e-7,
     on the plain path, the SumInc query over the sketch, the tiled and the binned
     pull."""
