@@ -1,8 +1,8 @@
 // compress.hip -- the supernode compressor with mirror vertices, and the device
 // shortcut build over its clusters.
 //
-// The compressor stays on the host, as in the reference (the north star keeps the
-// compressor host-side); it restates, from a cluster assignment:
+// The compressor stays on the host, as in the reference (the device path takes its
+// products through the sketch upload); it restates, from a cluster assignment:
 //   acceptance with mirror selection   compressor_base.h:3253-3611 (get_init_supernode_by_clusters)
 //   mirrors, absorption, supernodes    :3616-3783 (final_build_supernode)
 //   cluster subgraphs with mirrors     :3786-3856 (build_subgraph_mirror)
