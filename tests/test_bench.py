@@ -39,3 +39,25 @@ def test_traffic_summaries_are_found():
     assert bench.measured_traffic("k_pr_pull", "pr22") > 1e9
     assert bench.measured_traffic("k_pr_pull", "pr26") > 1e10
     assert bench.measured_traffic("k_pr_pull", "pr16") is None
+
+
+def test_pull_mode_follows_the_library_rules():
+    """The kernel the roofline names is the one the library's dense rounds run:
+    binned above 8 MiB of FP32 contributions (pr.hip build_bins), else tiled above
+    64 MiB (build_tiles), else the row classes; a window given forces its pull,
+    a negative one turns it off, and the banded pull keeps the row classes."""
+    from types import SimpleNamespace
+
+    def mode(n, band=0, tile=0, binned=0):
+        return bench.pull_mode(SimpleNamespace(pull_band_kib=band, pull_tile_kib=tile, pull_bin_kib=binned), n)
+
+    assert mode(1 << 16) == "rows"
+    assert mode(1 << 21) == "rows"  # exactly 8 MiB: not above the threshold
+    assert mode((1 << 21) + 1) == "binned"
+    assert mode(1 << 22) == "binned" and mode(1 << 26) == "binned"
+    assert mode(1 << 26, binned=-1) == "tiled"
+    assert mode(1 << 24, binned=-1) == "rows"  # 64 MiB: not above the tiled threshold
+    assert mode(1 << 16, binned=4) == "binned" and mode(1 << 16, binned=-1, tile=4) == "tiled"
+    assert mode(1 << 26, band=32768) == "rows"
+    assert mode(1 << 26, binned=-1, tile=-1) == "rows"
+    assert bench.PULL_KERNELS["binned"] == "k_pr_bin_gather+k_pr_bin_sum"
